@@ -1,0 +1,238 @@
+"""Pins the CPU oracle to the reference's own known answers.
+
+Each test restates one assertion of the reference's test-suite on the oracle:
+tests/mapping.rs, tests/integration.rs, src/index/hashtable.rs:326-437 and the
+homopolymer battery of examples/tune_homopolymer.rs.
+"""
+import numpy as np
+import pytest
+
+from oracle import pbw_oracle as O
+from refcases import ECOLI_READ, ECOLI_WINDOW, Xorshift64, homopolymer_battery, homopolymer_scenario, q40, revcomp
+
+
+def build(seq, name="chr1"):
+    g = O.Genome().add_contig(name, seq)
+    ht = O.HashTable(g)
+    return g, ht, O.Mapper(g, ht)
+
+
+def se(m, reads, quals=None, ids=None):
+    quals = quals or [q40(r) for r in reads]
+    return m.map_se(O.ReadBatch(reads, quals, ids))
+
+
+def cigar_totals(recs, i):
+    t = {"M": 0, "D": 0, "I": 0}
+    for op, n in O.cigar_ops(recs.cigar_words(i)):
+        if op in t:
+            t[op] += n
+    return t["M"], t["D"], t["I"]
+
+
+# tests/mapping.rs:67-103
+def test_forward_and_reverse_strand_mapping():
+    seq = Xorshift64(42).gen_seq(400)
+    g, ht, m = build(seq)
+    r = se(m, [seq[40:80], revcomp(seq[200:240])])
+    assert (r.rec[0]["pos"], r.rec[0]["strand"], r.cigar_str(0), r.rec[0]["mapq"]) == (41, 0, "40M", 80)
+    assert (r.rec[1]["pos"], r.rec[1]["strand"], r.cigar_str(1), r.rec[1]["mapq"]) == (201, 1, "40M", 80)
+    assert r.rec[0]["contig"] == 0 and r.rec[1]["contig"] == 0
+
+
+# tests/mapping.rs:105-160
+def test_mismatch_deletion_and_unmapped():
+    seq = Xorshift64(7).gen_seq(400)
+    g, ht, m = build(seq)
+    base = seq[40:80]
+    mm = list(base)
+    mm[10] = "C" if mm[10] == "A" else "A"
+    r = se(m, ["".join(mm), base[:20] + base[22:], "T" * 40])
+    assert (r.rec[0]["pos"], r.cigar_str(0), r.rec[0]["mapq"]) == (41, "40M", 80)
+    assert r.rec[1]["pos"] == 41
+    assert cigar_totals(r, 1) == (38, 2, 0)
+    assert r.rec[1]["mapq"] > 0
+    assert r.rec[2]["pos"] == 0 and r.rec[2]["mapq"] == 0 and r.cigar_str(2) == "40S"
+
+
+# tests/mapping.rs:171-193
+def test_large_deletion_is_found():
+    seq = Xorshift64(555).gen_seq(500)
+    g, ht, m = build(seq)
+    w = seq[100:160]
+    r = se(m, [w[:25] + w[50:]])
+    assert r.rec[0]["pos"] == 101
+    assert sum(n for op, n in O.cigar_ops(r.cigar_words(0)) if op == "D") == 25
+
+
+# tests/mapping.rs:195-233
+def test_paired_end_proper_and_improper():
+    seq = Xorshift64(99).gen_seq(400)
+    g, ht, m = build(seq)
+    m1, m2 = seq[40:80], revcomp(seq[200:240])
+    r = m.map_pe(O.ReadBatch([m1], [q40(m1)], ["pair1"]), O.ReadBatch([m2], [q40(m2)], ["pair1"]))
+    a, b = r.rec[0], r.rec[1]
+    assert a["flag"] & 0x40 and b["flag"] & 0x80
+    assert a["flag"] & 0x2 and b["flag"] & 0x2
+    assert a["strand"] == 0 and b["strand"] == 1
+    assert a["tlen"] == 200 and b["tlen"] == -200
+    m1b, m2b = seq[40:80], seq[200:240]
+    r = m.map_pe(O.ReadBatch([m1b], [q40(m1b)]), O.ReadBatch([m2b], [q40(m2b)]))
+    assert not (r.rec[0]["flag"] & 0x2) and not (r.rec[1]["flag"] & 0x2)
+
+
+# tests/mapping.rs:240-258
+def test_large_contig_positions():
+    seq = Xorshift64(2024).gen_seq(80000)
+    g, ht, m = build(seq)
+    r = se(m, [seq[70000:70050]])
+    assert r.rec[0]["contig"] == 0 and r.rec[0]["pos"] == 70001 and r.rec[0]["mapq"] == 99
+
+
+# tests/mapping.rs:267-302
+def test_mate_rescue_recovers_unseedable_mate():
+    seq = Xorshift64(2468).gen_seq(400)
+    g, ht, m = build(seq)
+    m1 = seq[40:80]
+    m2 = list(revcomp(seq[200:240]))
+    for p in range(0, len(m2), 6):
+        m2[p] = next(b for b in "ACGT" if b != m2[p])
+    m2 = "".join(m2)
+    assert se(m, [m2]).rec[0]["pos"] == 0
+    r = m.map_pe(O.ReadBatch([m1], [q40(m1)]), O.ReadBatch([m2], [q40(m2)]))
+    assert r.rec[0]["pos"] == 41
+    assert r.rec[1]["pos"] != 0 and r.rec[1]["contig"] == 0 and r.rec[1]["strand"] == 1
+    assert abs(int(r.rec[1]["pos"]) - 201) <= 5
+
+
+# tests/mapping.rs:304-334
+def test_n_run_round_trip():
+    rng = Xorshift64(7)
+    left, right = rng.gen_seq(100), rng.gen_seq(100)
+    seq = left + "N" * 20 + right
+    g, ht, m = build(seq, "chrN")
+    assert g.slice(0, 95, 125) == (seq[95:125]).encode()
+    rb = O.ReadBatch([seq[105:135], seq[150:190]], [q40(seq[105:135]), q40(seq[150:190])])
+    r = m.map_se(rb)
+    assert r.rec[0]["pos"] == 106
+    assert rb.seq_of(0)[:15] == b"N" * 15 and r.rec[0]["seq_rc"] == 0
+    assert r.rec[1]["pos"] == 151 and r.rec[1]["mapq"] == 80
+
+
+# tests/mapping.rs:342-394
+def test_synthetic_scale_mapping_accuracy():
+    rng = Xorshift64(12345)
+    glen, rl = 20000, 60
+    seq = rng.gen_seq(glen)
+    g, ht, m = build(seq)
+    reads, truth = [], []
+    for _ in range(300):
+        start = rng.next_u64() % (glen - rl)
+        b = list(seq[start:start + rl])
+        if rng.next_u64() % 10 < 3:
+            p = rng.next_u64() % rl
+            b[p] = next(x for x in "ACGT" if x != b[p])
+        s = "".join(b)
+        rev = rng.next_u64() % 2 == 0
+        if rev:
+            s = revcomp(s)
+        reads.append(s)
+        truth.append((start + 1, 1 if rev else 0))
+    r = se(m, reads)
+    ok = sum(1 for i, (p, s) in enumerate(truth)
+             if r.rec[i]["contig"] == 0 and r.rec[i]["strand"] == s and abs(int(r.rec[i]["pos"]) - p) <= 2)
+    assert ok / 300 >= 0.95
+
+
+# tests/mapping.rs:421-507
+def test_paired_disambiguates_repeat_copy_via_anchor():
+    rng = Xorshift64(2026)
+    unique_a, spacer, unit, filler = rng.gen_seq(100), rng.gen_seq(110), rng.gen_seq(100), rng.gen_seq(10000)
+    tc = list(unit)
+    tc[20] = {"A": "C", "C": "A", "G": "T"}.get(tc[20], "G")
+    genome = unique_a + spacer + "".join(tc) + filler + unit
+    true_pos, decoy_pos = 210, len(genome) - 100
+    g, ht, m = build(genome)
+    r1s = unique_a[:40]
+    r2s = revcomp(unit[:40])
+    q2 = [35] * 40
+    q2[39 - 20] = 10
+    alone = m.map_se(O.ReadBatch([r2s], [bytes(q2)]))
+    assert alone.rec[0]["pos"] == decoy_pos + 1
+    r = m.map_pe(O.ReadBatch([r1s], [bytes([35] * 40)]), O.ReadBatch([r2s], [bytes(q2)]))
+    assert r.rec[0]["pos"] == 1
+    assert r.rec[1]["pos"] == true_pos + 1 and r.rec[1]["strand"] == 1
+    assert (r.rec[0]["flag"] & 2) and (r.rec[1]["flag"] & 2)
+    assert r.rec[1]["mapq"] > 0
+
+
+# tests/integration.rs:130-153
+def test_divergent_region_does_not_invent_spurious_indels():
+    s, c, off, _ = O.SmithWatermanAligner(len(ECOLI_WINDOW), 40, 3).align(ECOLI_READ, bytes([30] * 100), ECOLI_WINDOW, 513)
+    assert off == 513
+    assert all(op == "M" for op, _ in O.cigar_ops(c))
+
+
+# tests/integration.rs:14-21, 30-64
+def test_hash_pack_reverse_complement():
+    assert O.hash_kmer("ACGTACGTACGTACG") == O.hash_kmer("ACGTACGTACGTACG")
+    assert O.hash_kmer("ACGTACGTACGTACG") != O.hash_kmer("TTTTTTTTTTTTTTT")
+    for c, p, s in ((5, 1234, 0), (5, 1234, 1), (42, 200_000_000, 0)):
+        assert O.unpack_position(O.pack_position(c, p, s)) == (c, p, s)
+    assert revcomp("ACGT") == "ACGT" and revcomp("AACCGGTT") == "AACCGGTT"
+
+
+# src/index/hashtable.rs:335-391
+def test_n_windows_are_not_indexed():
+    seq = "A" * 20 + "N" * 20 + "C" * 20
+    g = O.Genome().add_contig("c1", seq)
+    ht = O.HashTable(g, k=15, stride=1)
+    for packed in ht.lookup("A" * 15):
+        c, p, s = O.unpack_position(packed)
+        if s == 0:
+            assert p + 15 <= 20 or p >= 40
+    for h, bucket in ht.buckets():
+        for packed in bucket:
+            c, p, s = O.unpack_position(int(packed))
+            if s == 0 and 6 <= p < 40:
+                assert p + 15 <= 20 or p >= 40
+    assert any(O.unpack_position(p)[1:] == (0, 0) for p in ht.lookup("A" * 15))
+
+
+# src/index/hashtable.rs:397-402
+def test_is_acgt_only():
+    assert O.is_acgt_only("ACGTacgtACGTACG")
+    assert not O.is_acgt_only("ACGTNCGTACGTACG")
+    assert not O.is_acgt_only("ACGTRCGTACGTACG")
+
+
+# src/index/hashtable.rs:410-437
+def test_max_count_caps_a_hyper_repetitive_kmer():
+    unit = "ACGTACGTACGTACG"
+    g = O.Genome().add_contig("c1", unit * 50)
+    ht = O.HashTable(g, k=15, stride=1, max_count=5)
+    assert len(ht.lookup(unit)) == 5
+
+
+# examples/tune_homopolymer.rs:87-146.  At the current gap-open of -90 the
+# reference states the battery passes "at 100%" (smith_waterman.rs:47-52); the
+# 284/285 figure of smith_waterman.rs:266-271 predates that recalibration.
+def test_homopolymer_battery_recall():
+    correct = total = 0
+    for run_len, del_len, n_err, seed, vs in homopolymer_battery():
+        genome, read, true_pos = homopolymer_scenario(vs, run_len, del_len, n_err)
+        g, ht, m = build(genome)
+        r = m.map_se(O.ReadBatch([read], [bytes([10] * len(read))]))
+        dels = sum(n for op, n in O.cigar_ops(r.cigar_words(0)) if op == "D")
+        total += 1
+        if abs(int(r.rec[0]["pos"]) - true_pos) <= 2 and dels == del_len:
+            correct += 1
+    assert total == 285
+    assert correct == 285
+
+
+# hashtable.rs:104-124: mix() is a bijection on u32, so distinct 15-mers never
+# collide and a table keyed by the raw 30-bit code is lookup-equivalent.
+@pytest.mark.slow
+def test_mix_is_bijective():
+    assert O.mix_is_bijective()
