@@ -1,0 +1,177 @@
+/*
+ * probebwa_amd — MI355X (gfx950) implementation of probebwa's `map` hot path.
+ *
+ * C ABI: plain pointers and sizes, no exceptions across the boundary.  Every
+ * call returns 0 on success and a negative status on error; the message is
+ * available from pbw_last_error() (per context) or pbw_global_error().
+ *
+ * Each entry point names the reference interface it replaces
+ * (GUIBA-EX/probebwa, Rust; paths relative to that repository).
+ */
+#ifndef PROBEBWA_AMD_H
+#define PROBEBWA_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PBW_OK 0
+#define PBW_ERR_ARG (-1)
+#define PBW_ERR_HIP (-2)
+#define PBW_ERR_LIMIT (-3)
+#define PBW_ERR_IO (-4)
+#define PBW_ERR_MISMATCH (-5)
+
+typedef struct pbw_genome pbw_genome;
+typedef struct pbw_ctx pbw_ctx;
+
+/* One output record: the fields of AlignmentRecord (src/types.rs:180-203)
+ * that SAM carries, plus log_lk/posterior.  SEQ/QUAL are produced from the
+ * input read: reverse-complemented/reversed when seq_rc is set
+ * (src/mapper/single.rs:101-103,144-147). */
+typedef struct {
+  uint64_t pos;         /* 1-based; 0 = unmapped */
+  uint64_t mate_pos;    /* PNEXT; 0 = none */
+  int64_t tlen;         /* TLEN; 0 = none */
+  double log_lk;        /* AlignmentRecord::alignment_score */
+  double posterior;     /* AlignmentRecord::posterior_prob */
+  int32_t contig;       /* RNAME index; -1 = '*' */
+  int32_t mate_contig;  /* RNEXT index; -1 = '*' */
+  uint32_t cigar_off;   /* first CIGAR word in pbw_record_batch.cigar */
+  uint32_t cigar_n;     /* CIGAR ops; word = (len << 4) | op, op M=0 I=1 D=2 S=4 H=5 */
+  uint16_t flag;        /* SAM FLAG (src/io/sam.rs:126-143) */
+  uint8_t mapq;
+  uint8_t strand;       /* 0 forward, 1 reverse */
+  uint8_t seq_rc;       /* SEQ/QUAL emitted reverse-complemented/reversed */
+  uint8_t pad[3];
+} pbw_record;
+
+/* MapperOptions (src/types.rs:207-249), defaults from pbw_default_options. */
+typedef struct {
+  double substitution_rate;  /* -s (0.001) */
+  double insert_size_mean;   /* 250 */
+  double insert_size_sd;     /* 60 */
+  double sv_prior_phred;     /* --svprior (55) */
+  uint32_t max_indel_len;    /* band half-width (30) */
+  uint32_t gap_open_phred;   /* --gapopen (40) */
+  uint32_t gap_extend_phred; /* --gapextend (3) */
+  uint32_t batch_reads;      /* reads per device batch (0 = default 131072) */
+} pbw_options;
+
+/* Reads: bases upper-cased (DnaSeq::from_ascii, src/types.rs:12-15), quals
+ * offset-subtracted (QualityScores::from_phred_ascii, src/types.rs:57-60). */
+typedef struct {
+  uint64_t n;
+  const uint8_t* seq;
+  const uint64_t* seq_off;  /* n+1 */
+  const uint8_t* qual;
+  const uint64_t* qual_off; /* n+1 */
+} pbw_read_batch;
+
+/* Library-owned output arena; free with pbw_records_release. */
+typedef struct {
+  uint64_t n;
+  pbw_record* rec;
+  uint32_t* cigar;
+  uint64_t n_cigar;
+} pbw_record_batch;
+
+/* InsertSizeDistribution (src/types.rs:266-270). */
+typedef struct {
+  double mean;
+  double sd;
+  uint64_t n_observed;
+} pbw_insert_model;
+
+/* Kernel-level job: SmithWatermanAligner::new(band, ..).align(read, qual,
+ * ref_seg, ref_offset_hint) (src/align/smith_waterman.rs:113,163).  Host
+ * pointers. */
+typedef struct {
+  const uint8_t* read;
+  const uint8_t* qual;
+  const uint8_t* ref;
+  uint32_t m;     /* read length */
+  uint32_t qlen;  /* quality length (missing quals read as 30) */
+  uint32_t n;     /* ref_seg length */
+  uint32_t band;
+  int64_t hint;
+} pbw_align_job;
+
+typedef struct {
+  int32_t score;
+  uint32_t ref_offset; /* ref_start_offset */
+  uint32_t cigar_off;  /* into the caller's cigar array */
+  uint32_t cigar_n;
+} pbw_align_result;
+
+/* Per-context counters and stage timings of the last map call. */
+typedef struct {
+  uint64_t reads;
+  uint64_t lookups;        /* exact + neighbour probes */
+  uint64_t positions;      /* packed positions gathered */
+  uint64_t filtered_hits;  /* hits that passed the similarity filter */
+  uint64_t dp_jobs;
+  uint64_t dp_cells;       /* counted as the reference loop does */
+  uint64_t wide_jobs;
+  uint64_t wide_cells;
+  uint64_t seed_overflow_reads;
+  uint64_t tier2_cluster_reads;
+  uint64_t generic_dp_jobs;
+  double ms_seed, ms_cluster, ms_dp, ms_mapq, ms_pair, ms_total;
+} pbw_stats;
+
+/* ---- genome: GenomeIndexBuilder (src/index/genome.rs:101-147) ---- */
+int pbw_genome_create(pbw_genome** out);
+/* Contig::from_ascii (src/index/genome.rs:17-45) */
+int pbw_genome_add_contig(pbw_genome* g, const char* name, const uint8_t* seq, uint64_t len);
+/* GenomeIndexBuilder::add_fasta (src/index/genome.rs:119-134) */
+int pbw_genome_add_fasta(pbw_genome* g, const char* path);
+int pbw_genome_n_contigs(const pbw_genome* g);
+const char* pbw_genome_contig_name(const pbw_genome* g, int cid);
+uint64_t pbw_genome_contig_length(const pbw_genome* g, int cid);
+void pbw_genome_destroy(pbw_genome* g);
+
+/* ---- context: index build + upload, SingleEndMapper::new (src/mapper/single.rs:52-70),
+ *      HashTableBuilder::build (src/index/hashtable.rs:248-322, k=15 stride=5 cap=300) ---- */
+void pbw_default_options(pbw_options* o);
+int pbw_ctx_create(const pbw_genome* g, const pbw_options* o, int device, pbw_ctx** out);
+void pbw_ctx_destroy(pbw_ctx* ctx);
+const char* pbw_last_error(const pbw_ctx* ctx);
+const char* pbw_global_error(void);
+
+/* ---- map: replaces the single-end rayon loop of map_reads_with_format
+ *      (src/lib.rs:84-104, SingleEndMapper::map src/mapper/single.rs:164-207);
+ *      records in input order ---- */
+int pbw_map_se(pbw_ctx* ctx, const pbw_read_batch* reads, pbw_record_batch* out);
+
+/* ---- paired-end: map_paired (src/lib.rs:145-185) incl. the 500-pair serial
+ *      warm-up and 2000-pair frozen-snapshot batches; records interleaved
+ *      r1,r2; the insert model is read and updated in place ---- */
+int pbw_map_pe(pbw_ctx* ctx, const pbw_read_batch* m1, const pbw_read_batch* m2, pbw_insert_model* inout,
+               pbw_record_batch* out);
+
+/* ---- kernel level: batched SmithWatermanAligner::align ---- */
+int pbw_align_batch(pbw_ctx* ctx, uint32_t gap_open_phred, uint32_t gap_extend_phred, const pbw_align_job* jobs,
+                    uint64_t n, pbw_align_result* results, uint32_t* cigar, uint64_t cigar_cap);
+
+/* ---- index access: HashTable::lookup (src/index/hashtable.rs:90-102) by 30-bit code;
+ *      copies up to cap packed positions, returns the bucket size (or <0) ---- */
+int64_t pbw_index_lookup(pbw_ctx* ctx, uint32_t code, uint64_t* out, uint64_t cap);
+uint64_t pbw_index_total_entries(const pbw_ctx* ctx);
+
+void pbw_records_release(pbw_record_batch* b);
+int pbw_get_stats(const pbw_ctx* ctx, pbw_stats* out);
+
+/* ---- SAM text: format_sam_records (src/io/sam.rs:37-107); ids is n NUL-terminated
+ *      strings back to back; free with pbw_free_text ---- */
+char* pbw_format_sam(const pbw_genome* g, const pbw_read_batch* reads, const char* ids, const pbw_record_batch* recs,
+                     const char* command_line, const char* read_group, uint64_t* len);
+void pbw_free_text(char* t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,365 @@
+"""probebwa_amd — MI355X (gfx950) implementation of probebwa's `map` hot path.
+
+Python binding (ctypes) of libprobebwa_amd.so, whose C ABI is declared in
+include/probebwa_amd.h.  The names mirror the reference's Rust API
+(GUIBA-EX/probebwa): `GenomeIndexBuilder`/`Genome`, `MapperOptions`,
+`map_reads`, `SingleEndMapper.map`, `PairedEndMapper.map_pair`,
+`SmithWatermanAligner.align`.  Every compute call runs the HIP kernels; there
+is no CPU fallback — importing works without a GPU, but creating a context
+fails loudly when no HIP device or no built library is present.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libprobebwa_amd.so")
+
+PBW_OK = 0
+ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "limit exceeded", -4: "I/O error", -5: "count mismatch"}
+
+RECORD_DTYPE = np.dtype(
+    [
+        ("pos", "<u8"),
+        ("mate_pos", "<u8"),
+        ("tlen", "<i8"),
+        ("log_lk", "<f8"),
+        ("posterior", "<f8"),
+        ("contig", "<i4"),
+        ("mate_contig", "<i4"),
+        ("cigar_off", "<u4"),
+        ("cigar_n", "<u4"),
+        ("flag", "<u2"),
+        ("mapq", "u1"),
+        ("strand", "u1"),
+        ("seq_rc", "u1"),
+        ("pad", "u1", (3,)),
+    ]
+)
+assert RECORD_DTYPE.itemsize == 64
+CIGAR_CHARS = "MIDNSHP=X"
+
+# Every symbol include/probebwa_amd.h declares (checked by the CPU test-suite).
+EXPORTED_SYMBOLS = (
+    "pbw_genome_create", "pbw_genome_add_contig", "pbw_genome_add_fasta", "pbw_genome_n_contigs",
+    "pbw_genome_contig_name", "pbw_genome_contig_length", "pbw_genome_destroy", "pbw_default_options",
+    "pbw_ctx_create", "pbw_ctx_destroy", "pbw_last_error", "pbw_global_error", "pbw_map_se", "pbw_map_pe",
+    "pbw_align_batch", "pbw_index_lookup", "pbw_index_total_entries", "pbw_records_release", "pbw_get_stats",
+    "pbw_format_sam", "pbw_free_text",
+)
+
+
+class MapperOptions(C.Structure):
+    """`MapperOptions` (src/types.rs:207-249) plus the device batch size."""
+
+    _fields_ = [
+        ("substitution_rate", C.c_double),
+        ("insert_size_mean", C.c_double),
+        ("insert_size_sd", C.c_double),
+        ("sv_prior_phred", C.c_double),
+        ("max_indel_len", C.c_uint32),
+        ("gap_open_phred", C.c_uint32),
+        ("gap_extend_phred", C.c_uint32),
+        ("batch_reads", C.c_uint32),
+    ]
+
+    def __init__(self, **kw):
+        super().__init__()
+        lib().pbw_default_options(C.byref(self))
+        for k, v in kw.items():
+            setattr(self, k, v)
+
+
+class _ReadBatch(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("seq", C.c_void_p), ("seq_off", C.c_void_p), ("qual", C.c_void_p),
+                ("qual_off", C.c_void_p)]
+
+
+class _RecordBatch(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("rec", C.c_void_p), ("cigar", C.c_void_p), ("n_cigar", C.c_uint64)]
+
+
+class InsertModel(C.Structure):
+    """`InsertSizeDistribution` (src/types.rs:266-270)."""
+
+    _fields_ = [("mean", C.c_double), ("sd", C.c_double), ("n_observed", C.c_uint64)]
+
+
+class _AlignJob(C.Structure):
+    _fields_ = [("read", C.c_void_p), ("qual", C.c_void_p), ("ref", C.c_void_p), ("m", C.c_uint32),
+                ("qlen", C.c_uint32), ("n", C.c_uint32), ("band", C.c_uint32), ("hint", C.c_int64)]
+
+
+class _AlignResult(C.Structure):
+    _fields_ = [("score", C.c_int32), ("ref_offset", C.c_uint32), ("cigar_off", C.c_uint32), ("cigar_n", C.c_uint32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("reads", "lookups", "positions", "filtered_hits", "dp_jobs", "dp_cells",
+                                           "wide_jobs", "wide_cells", "seed_overflow_reads", "tier2_cluster_reads",
+                                           "generic_dp_jobs")] + \
+               [(n, C.c_double) for n in ("ms_seed", "ms_cluster", "ms_dp", "ms_mapq", "ms_pair", "ms_total")]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class PbwError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Loads libprobebwa_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PbwError("libprobebwa_amd.so is not built: run `make -C probebwa_amd/csrc` "
+                           "(or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        vp, u64, i64, u32, i32 = C.c_void_p, C.c_uint64, C.c_int64, C.c_uint32, C.c_int
+        sig = {
+            "pbw_genome_create": (i32, [C.POINTER(vp)]),
+            "pbw_genome_add_contig": (i32, [vp, C.c_char_p, vp, u64]),
+            "pbw_genome_add_fasta": (i32, [vp, C.c_char_p]),
+            "pbw_genome_n_contigs": (i32, [vp]),
+            "pbw_genome_contig_name": (C.c_char_p, [vp, i32]),
+            "pbw_genome_contig_length": (u64, [vp, i32]),
+            "pbw_genome_destroy": (None, [vp]),
+            "pbw_default_options": (None, [C.POINTER(MapperOptions)]),
+            "pbw_ctx_create": (i32, [vp, C.POINTER(MapperOptions), i32, C.POINTER(vp)]),
+            "pbw_ctx_destroy": (None, [vp]),
+            "pbw_last_error": (C.c_char_p, [vp]),
+            "pbw_global_error": (C.c_char_p, []),
+            "pbw_map_se": (i32, [vp, C.POINTER(_ReadBatch), C.POINTER(_RecordBatch)]),
+            "pbw_map_pe": (i32, [vp, C.POINTER(_ReadBatch), C.POINTER(_ReadBatch), C.POINTER(InsertModel),
+                                 C.POINTER(_RecordBatch)]),
+            "pbw_align_batch": (i32, [vp, u32, u32, C.POINTER(_AlignJob), u64, C.POINTER(_AlignResult), vp, u64]),
+            "pbw_index_lookup": (i64, [vp, u32, vp, u64]),
+            "pbw_index_total_entries": (u64, [vp]),
+            "pbw_records_release": (None, [C.POINTER(_RecordBatch)]),
+            "pbw_get_stats": (i32, [vp, C.POINTER(Stats)]),
+            "pbw_format_sam": (vp, [vp, C.POINTER(_ReadBatch), C.c_char_p, C.POINTER(_RecordBatch), C.c_char_p,
+                                    C.c_char_p, C.POINTER(u64)]),
+            "pbw_free_text": (None, [vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _b(x):
+    return x.encode() if isinstance(x, str) else bytes(x)
+
+
+def _check(rc, ctx=None):
+    if rc != PBW_OK:
+        msg = (lib().pbw_last_error(ctx) if ctx else lib().pbw_global_error()) or b""
+        raise PbwError("%s: %s" % (ERRORS.get(rc, rc), msg.decode(errors="replace")))
+
+
+class Genome:
+    """`GenomeIndexBuilder` + `GenomeIndex` (src/index/genome.rs:71-147), kept in host memory."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _check(lib().pbw_genome_create(C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().pbw_genome_destroy(self.h)
+            self.h = None
+
+    def add_contig(self, name, seq):
+        seq = _b(seq)
+        buf = C.create_string_buffer(seq, len(seq))
+        _check(lib().pbw_genome_add_contig(self.h, _b(name), buf, len(seq)))
+        return self
+
+    def add_fasta(self, path):
+        _check(lib().pbw_genome_add_fasta(self.h, _b(path)))
+        return self
+
+    @property
+    def names(self):
+        return [lib().pbw_genome_contig_name(self.h, i).decode() for i in range(lib().pbw_genome_n_contigs(self.h))]
+
+    @property
+    def lengths(self):
+        return [lib().pbw_genome_contig_length(self.h, i) for i in range(lib().pbw_genome_n_contigs(self.h))]
+
+
+class ReadBatch:
+    """Flat reads: bases upper-cased, quals offset-subtracted (src/types.rs:12-15, :57-60)."""
+
+    def __init__(self, seqs, quals, ids=None):
+        seqs = [_b(s).upper() for s in seqs]
+        quals = [_b(q) for q in quals]
+        self.n = len(seqs)
+        self.seq = np.frombuffer(b"".join(seqs) + b"\0", dtype=np.uint8).copy()
+        self.qual = np.frombuffer(b"".join(quals) + b"\0", dtype=np.uint8).copy()
+        self.seq_off = np.zeros(self.n + 1, np.uint64)
+        self.qual_off = np.zeros(self.n + 1, np.uint64)
+        if self.n:
+            self.seq_off[1:] = np.cumsum([len(s) for s in seqs])
+            self.qual_off[1:] = np.cumsum([len(q) for q in quals])
+        self.ids = list(ids) if ids is not None else ["r%d" % i for i in range(self.n)]
+
+    @classmethod
+    def from_arrays(cls, seq, seq_off, qual, qual_off, ids=None):
+        rb = cls.__new__(cls)
+        rb.n = len(seq_off) - 1
+        rb.seq = np.ascontiguousarray(seq, dtype=np.uint8)
+        rb.qual = np.ascontiguousarray(qual, dtype=np.uint8)
+        rb.seq_off = np.ascontiguousarray(seq_off, dtype=np.uint64)
+        rb.qual_off = np.ascontiguousarray(qual_off, dtype=np.uint64)
+        rb.ids = ids if ids is not None else ["r%d" % i for i in range(rb.n)]
+        return rb
+
+    def cstruct(self):
+        return _ReadBatch(self.n, self.seq.ctypes.data, self.seq_off.ctypes.data, self.qual.ctypes.data,
+                          self.qual_off.ctypes.data)
+
+    def seq_of(self, i):
+        return self.seq[self.seq_off[i]:self.seq_off[i + 1]].tobytes()
+
+    def qual_of(self, i):
+        return self.qual[self.qual_off[i]:self.qual_off[i + 1]].tobytes()
+
+
+class Records:
+    """Records of one map call (numpy structured array + CIGAR words)."""
+
+    def __init__(self, rec, cigar):
+        self.rec = rec
+        self.cigar = cigar
+
+    def __len__(self):
+        return len(self.rec)
+
+    def cigar_words(self, i):
+        r = self.rec[i]
+        return [int(x) for x in self.cigar[r["cigar_off"]:r["cigar_off"] + r["cigar_n"]]]
+
+    def cigar_str(self, i):
+        return "".join("%d%s" % (w >> 4, CIGAR_CHARS[w & 15]) for w in self.cigar_words(i))
+
+
+def _take(rb):
+    n = rb.n
+    rec = np.frombuffer((C.c_char * (n * 64)).from_address(rb.rec), dtype=RECORD_DTYPE).copy() if n else \
+        np.zeros(0, RECORD_DTYPE)
+    cig = np.frombuffer((C.c_char * (rb.n_cigar * 4)).from_address(rb.cigar), dtype=np.uint32).copy() \
+        if rb.n_cigar else np.zeros(0, np.uint32)
+    lib().pbw_records_release(C.byref(rb))
+    return Records(rec, cig)
+
+
+class Context:
+    """Device context: uploads the genome, builds the 15-mer index on it
+    (HashTableBuilder, src/index/hashtable.rs:248-322) and owns the HIP stream.
+    """
+
+    def __init__(self, genome, options=None, device=0, **kw):
+        self.genome = genome
+        self.options = options if options is not None else MapperOptions(**kw)
+        h = C.c_void_p()
+        _check(lib().pbw_ctx_create(genome.h, C.byref(self.options), device, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().pbw_ctx_destroy(self.h)
+            self.h = None
+
+    def map_se(self, reads):
+        """`map_reads` single-end loop (src/lib.rs:84-104); records in input order."""
+        out = _RecordBatch()
+        rs = reads.cstruct()
+        _check(lib().pbw_map_se(self.h, C.byref(rs), C.byref(out)), self.h)
+        return _take(out)
+
+    def map_pe(self, r1, r2, mean=None, sd=None, n_observed=0):
+        """`map_paired` (src/lib.rs:145-185); records interleaved r1, r2."""
+        model = InsertModel(self.options.insert_size_mean if mean is None else mean,
+                            self.options.insert_size_sd if sd is None else sd, n_observed)
+        out = _RecordBatch()
+        a, b = r1.cstruct(), r2.cstruct()
+        _check(lib().pbw_map_pe(self.h, C.byref(a), C.byref(b), C.byref(model), C.byref(out)), self.h)
+        self.insert_model = (model.mean, model.sd, model.n_observed)
+        return _take(out)
+
+    def align_batch(self, jobs, gap_open_phred=40, gap_extend_phred=3):
+        """Batched `SmithWatermanAligner::new(band,..).align(read, qual, ref, hint)`.
+
+        jobs: list of (read, qual, ref_seg, band, hint).  Returns a list of
+        (score, cigar words, ref_start_offset)."""
+        n = len(jobs)
+        keep = []
+        arr = (_AlignJob * max(n, 1))()
+        total = 0
+        for i, (read, qual, ref, band, hint) in enumerate(jobs):
+            bufs = [C.create_string_buffer(_b(x), max(1, len(x))) for x in (read, qual, ref)]
+            keep.append(bufs)
+            arr[i] = _AlignJob(C.addressof(bufs[0]), C.addressof(bufs[1]), C.addressof(bufs[2]), len(read), len(qual),
+                               len(ref), band, hint)
+            total += len(read) + len(ref) + 2
+        res = (_AlignResult * max(n, 1))()
+        cig = np.zeros(max(total, 1), np.uint32)
+        _check(lib().pbw_align_batch(self.h, gap_open_phred, gap_extend_phred, arr, n, res, cig.ctypes.data, len(cig)),
+               self.h)
+        return [(res[i].score, [int(x) for x in cig[res[i].cigar_off:res[i].cigar_off + res[i].cigar_n]],
+                 res[i].ref_offset) for i in range(n)]
+
+    def lookup_code(self, code):
+        """`HashTable::lookup` by 30-bit code: the bucket's packed positions."""
+        n = lib().pbw_index_lookup(self.h, code, None, 0)
+        if n < 0:
+            _check(int(n), self.h)
+        out = np.zeros(max(n, 1), np.uint64)
+        lib().pbw_index_lookup(self.h, code, out.ctypes.data, n)
+        return out[:n]
+
+    @property
+    def total_entries(self):
+        return lib().pbw_index_total_entries(self.h)
+
+    def stats(self):
+        s = Stats()
+        _check(lib().pbw_get_stats(self.h, C.byref(s)), self.h)
+        return s.as_dict()
+
+
+def format_sam(genome, reads, records, command_line="probebwa", read_group=None):
+    """SAM text (src/io/sam.rs:37-107) for records aligned 1:1 with `reads`."""
+    ids = b"".join(_b(i) + b"\0" for i in reads.ids)
+    rs = reads.cstruct()
+    rec = np.ascontiguousarray(records.rec)
+    cig = np.ascontiguousarray(records.cigar) if len(records.cigar) else np.zeros(1, np.uint32)
+    rb = _RecordBatch(len(rec), rec.ctypes.data, cig.ctypes.data, len(records.cigar))
+    ln = C.c_uint64()
+    t = lib().pbw_format_sam(genome.h, C.byref(rs), ids, C.byref(rb), _b(command_line),
+                             _b(read_group) if read_group else None, C.byref(ln))
+    s = C.string_at(t, ln.value)
+    lib().pbw_free_text(t)
+    return s
+
+
+def code_of(kmer):
+    """30-bit code of a 15-mer (the pre-mix() packing of hash_kmer, hashtable.rs:107-113)."""
+    c = 0
+    for ch in _b(kmer)[:15]:
+        c = (c << 2) | {65: 0, 67: 1, 71: 2, 84: 3, 97: 0, 99: 1, 103: 2, 116: 3}.get(ch, 0)
+    return c
+
+
+def map_reads(genome, reads, options=None, device=0):
+    """Single-end `map_reads` (src/lib.rs:49-56) over an in-memory genome and read batch."""
+    return Context(genome, options, device).map_se(reads)

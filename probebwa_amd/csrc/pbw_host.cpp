@@ -1,0 +1,1296 @@
+// Host side of probebwa_amd: genome packing, index build, device context,
+// batch driver and the C ABI declared in include/probebwa_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/probebwa_amd.h"
+#include "pbw_internal.h"
+
+using namespace pbw;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_error;
+
+struct PbwError {
+  int code;
+  std::string msg;
+};
+
+#define HIP_CHECK(x)                                                                                 \
+  do {                                                                                               \
+    hipError_t e_ = (x);                                                                             \
+    if (e_ != hipSuccess)                                                                            \
+      throw PbwError{PBW_ERR_HIP, std::string(#x " failed: ") + hipGetErrorString(e_)};              \
+  } while (0)
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void ensure(size_t count) {
+    if (count <= n && p) return;
+    release();
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    HIP_CHECK(hipMalloc((void**)&p, bytes));
+    n = std::max<size_t>(count, 1);
+  }
+  void upload(const T* h, size_t count, hipStream_t s) {
+    ensure(count);
+    if (count) HIP_CHECK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+};
+
+// ---------------------------------------------------------------------------
+// genome (genome.rs:17-45 packing, 2-bit + N mask; contigs aligned to 64 bases)
+// ---------------------------------------------------------------------------
+struct pbw_genome {
+  std::vector<std::string> names;
+  std::vector<uint64_t> lengths;
+  std::vector<uint64_t> coff;
+  std::vector<uint64_t> g2b;    // 32 bases per word
+  std::vector<uint64_t> nmask;  // 64 bases per word
+  uint64_t total_length = 0;
+  uint64_t next_base = 0;       // first free (64-aligned) base slot
+};
+
+static void genome_add(pbw_genome* g, const std::string& name, const uint8_t* seq, uint64_t len) {
+  uint64_t base = g->next_base;
+  uint64_t end = base + len;
+  uint64_t words2 = (end + 31) / 32 + 2, wordsn = (end + 63) / 64 + 2;
+  if (g->g2b.size() < words2) g->g2b.resize(words2, 0);
+  if (g->nmask.size() < wordsn) g->nmask.resize(wordsn, 0);
+  for (uint64_t p = 0; p < len; p++) {
+    uint64_t gb = base + p;
+    uint64_t code = 0;
+    bool acgt = true;
+    switch (seq[p]) {
+      case 'A': case 'a': code = 0; break;
+      case 'C': case 'c': code = 1; break;
+      case 'G': case 'g': code = 2; break;
+      case 'T': case 't': code = 3; break;
+      default: acgt = false; break;
+    }
+    g->g2b[gb >> 5] |= code << ((gb & 31) * 2);
+    if (!acgt) g->nmask[gb >> 6] |= 1ull << (gb & 63);
+  }
+  g->names.push_back(name);
+  g->lengths.push_back(len);
+  g->coff.push_back(base);
+  g->total_length += len;  // genome.rs:119
+  g->next_base = (end + 63) / 64 * 64;
+}
+
+static inline uint32_t gbase2(const pbw_genome* g, uint64_t gb) { return (uint32_t)(g->g2b[gb >> 5] >> ((gb & 31) * 2)) & 3u; }
+static inline bool gisn(const pbw_genome* g, uint64_t gb) { return (g->nmask[gb >> 6] >> (gb & 63)) & 1ull; }
+
+static inline bool rust_ws(uint8_t c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == 0x0B || c == 0x0C; }
+static std::string trim_str(const std::string& s) {
+  size_t a = 0, b = s.size();
+  while (a < b && rust_ws((uint8_t)s[a])) a++;
+  while (b > a && rust_ws((uint8_t)s[b - 1])) b--;
+  return s.substr(a, b - a);
+}
+static std::string first_tok(const std::string& s) {
+  size_t a = 0;
+  while (a < s.size() && rust_ws((uint8_t)s[a])) a++;
+  size_t b = a;
+  while (b < s.size() && !rust_ws((uint8_t)s[b])) b++;
+  return s.substr(a, b - a);
+}
+static bool read_line(FILE* f, std::string& line) {
+  line.clear();
+  int c;
+  bool any = false;
+  while ((c = fgetc(f)) != EOF) {
+    any = true;
+    line.push_back((char)c);
+    if (c == '\n') break;
+  }
+  return any;
+}
+
+// ---------------------------------------------------------------------------
+// index build: hashtable.rs:248-322 (k=15, stride 5, cap 300 in scan order).
+// Buckets are keyed by the 30-bit code: mix() (hashtable.rs:117-124) is a
+// bijection, so the code -> bucket map equals the hash -> bucket map.
+// ---------------------------------------------------------------------------
+struct HostIndex {
+  std::vector<uint32_t> keys;    // distinct codes, ascending
+  std::vector<uint32_t> kstart;  // bucket start per key (+ total at the end)
+  std::vector<uint64_t> pos;     // packed positions (hashtable.rs:157-164)
+};
+
+static void build_index(const pbw_genome* g, HostIndex& out) {
+  std::vector<uint32_t> code;
+  std::vector<uint64_t> packed;
+  uint64_t est = 0;
+  for (uint64_t L : g->lengths) est += 2 * (L / STRIDE + 1);
+  code.reserve(est);
+  packed.reserve(est);
+  const uint32_t MASK30 = (1u << 30) - 1;
+  for (size_t cid = 0; cid < g->lengths.size(); cid++) {
+    const uint64_t L = g->lengths[cid], base = g->coff[cid];
+    if (L < (uint64_t)K) continue;
+    const uint64_t nw = L - K + 1;
+    std::vector<uint32_t> fcode(nw), rcode(nw);
+    std::vector<uint8_t> clean(nw);
+    uint32_t f = 0, r = 0;
+    int nrun = 0;  // N count in the current window
+    for (uint64_t p = 0; p < L; p++) {
+      uint32_t b = gbase2(g, base + p);
+      bool isn = gisn(g, base + p);
+      uint32_t fb = isn ? 0 : b;  // base2bits('N') == 0
+      f = ((f << 2) | fb) & MASK30;
+      r = (r >> 2) | ((isn ? 0u : (3u - b)) << 28);  // complement of 'N' is 'N' -> 0
+      nrun += isn ? 1 : 0;
+      if (p >= (uint64_t)K) nrun -= gisn(g, base + p - K) ? 1 : 0;
+      if (p + 1 >= (uint64_t)K) {
+        uint64_t w = p + 1 - K;
+        fcode[w] = f;
+        rcode[w] = r;
+        clean[w] = nrun == 0;
+      }
+    }
+    // forward strand: windows i = 0, 5, 10, ...
+    for (uint64_t i = 0; i < nw; i += STRIDE) {
+      if (!clean[i]) continue;
+      code.push_back(fcode[i]);
+      packed.push_back(((uint64_t)cid << 40) | i);
+    }
+    // reverse-complement strand: rc windows i = 0, 5, ... at position L-i-15
+    for (uint64_t i = 0; i < nw; i += STRIDE) {
+      uint64_t w = L - i - K;
+      if (!clean[w]) continue;
+      code.push_back(rcode[w]);
+      packed.push_back((1ull << 63) | ((uint64_t)cid << 40) | w);
+    }
+  }
+  // stable LSD radix sort by code (2 x 15 bits)
+  size_t n = code.size();
+  std::vector<uint32_t> c2(n);
+  std::vector<uint64_t> p2(n);
+  for (int pass = 0; pass < 2; pass++) {
+    int sh = pass * 15;
+    std::vector<uint64_t> cnt((1u << 15) + 1, 0);
+    for (size_t i = 0; i < n; i++) cnt[((code[i] >> sh) & 0x7FFF) + 1]++;
+    for (size_t i = 0; i < (1u << 15); i++) cnt[i + 1] += cnt[i];
+    for (size_t i = 0; i < n; i++) {
+      uint64_t d = cnt[(code[i] >> sh) & 0x7FFF]++;
+      c2[d] = code[i];
+      p2[d] = packed[i];
+    }
+    code.swap(c2);
+    packed.swap(p2);
+  }
+  out.keys.clear();
+  out.kstart.clear();
+  out.pos.clear();
+  out.pos.reserve(n);
+  size_t i = 0;
+  while (i < n) {
+    uint32_t c = code[i];
+    size_t s = i;
+    while (i < n && code[i] == c) i++;
+    size_t take = std::min<size_t>(i - s, MAX_COUNT);  // first 300 in scan order
+    out.keys.push_back(c);
+    out.kstart.push_back((uint32_t)out.pos.size());
+    out.pos.insert(out.pos.end(), packed.begin() + s, packed.begin() + s + take);
+  }
+  out.kstart.push_back((uint32_t)out.pos.size());
+}
+
+// ---------------------------------------------------------------------------
+// host copies of the reference's scalar math (for LUTs)
+// ---------------------------------------------------------------------------
+static const double H_LN_10 = 2.302585092994045684017991454684364208;
+static const double H_DB = -0.1 * H_LN_10;
+static double h_logadd(double a, double b) {
+  if (a > b) std::swap(a, b);
+  if (b >= a + 10.0) return a;
+  if (b >= a + 4.0) return std::max(a - 1.0, 0.0);
+  if (b >= a + 2.0) return std::max(a - 2.0, 0.0);
+  return std::max(a - 3.0, 0.0);
+}
+static double h_py_round(double x) { return std::trunc(x + 0.5); }
+static int32_t h_f64_i32(double x) {
+  if (std::isnan(x)) return 0;
+  if (x >= 2147483647.0) return INT32_MAX;
+  if (x <= -2147483648.0) return INT32_MIN;
+  return (int32_t)x;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct pbw_ctx {
+  const pbw_genome* genome;
+  pbw_options opt;
+  int device;
+  hipStream_t stream = nullptr;
+  std::string error;
+  pbw_stats stats{};
+  // genome + index
+  DevBuf<uint64_t> d_g2b, d_nmask, d_coff, d_clen;
+  DevBuf<uint32_t> d_koff;
+  DevBuf<uint64_t> d_kpos;
+  uint64_t total_entries = 0;
+  // LUTs
+  DevBuf<double> d_lut;  // packed double tables
+  DevBuf<int32_t> d_mis;
+  DevLuts luts{};
+  int32_t gap_open = -90, gap_extend = -1;
+  // batch workspace
+  DevBuf<uint8_t> d_seq, d_qual, d_rcode;
+  DevBuf<uint64_t> d_seq_off, d_qual_off, d_hoff, d_hits, d_hits_ov, d_scratch;
+  DevBuf<uint32_t> d_nhits, d_list, d_hcap, d_tier2, d_ntier2, d_best, d_cigar, d_cigar_out, d_cigar_off;
+  DevBuf<Cand> d_cands;
+  DevBuf<uint8_t> d_ncand, d_njobs, d_generic, d_rec, d_refbytes;
+  DevBuf<DpJob> d_jobs;
+  DevBuf<DpOut> d_out;
+  DevBuf<Counters> d_ctr;
+  DevBuf<int32_t> d_rows;
+  DevBuf<uint64_t> d_tb;
+  // paired-end workspace
+  DevBuf<DpJob> d_wjobs;
+  DevBuf<DpOut> d_wout;
+  DevBuf<uint8_t> d_wvalid, d_slcand, d_has_obs;
+  DevBuf<uint32_t> d_plan, d_wcigar, d_name_class;
+  DevBuf<uint64_t> d_cig_src;
+  DevBuf<double> d_obs;
+
+  DevGenome dev_genome() const {
+    DevGenome G;
+    G.g2b = d_g2b.p;
+    G.nmask = d_nmask.p;
+    G.coff = d_coff.p;
+    G.clen = d_clen.p;
+    G.ncontigs = (int32_t)genome->lengths.size();
+    return G;
+  }
+};
+
+static void ctx_build(pbw_ctx* c) {
+  const pbw_genome* g = c->genome;
+  HIP_CHECK(hipSetDevice(c->device));
+  HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  hipStream_t s = c->stream;
+  if (g->lengths.size() >= (1u << 22)) throw PbwError{PBW_ERR_LIMIT, "more than 2^22 contigs"};
+  for (uint64_t L : g->lengths)
+    if (L >= (1ull << 40)) throw PbwError{PBW_ERR_LIMIT, "contig longer than 2^40 bases"};
+  std::vector<uint64_t> g2b = g->g2b, nm = g->nmask;
+  g2b.resize(std::max<size_t>(g2b.size(), 2) + 2, 0);
+  nm.resize(std::max<size_t>(nm.size(), 2) + 2, 0);
+  c->d_g2b.upload(g2b.data(), g2b.size(), s);
+  c->d_nmask.upload(nm.data(), nm.size(), s);
+  c->d_coff.upload(g->coff.data(), g->coff.size(), s);
+  c->d_clen.upload(g->lengths.data(), g->lengths.size(), s);
+  {  // contig -> first contig with the same name (paired.rs:188 compares names)
+    std::vector<uint32_t> nc(g->names.size());
+    for (size_t i = 0; i < g->names.size(); i++) {
+      nc[i] = (uint32_t)i;
+      for (size_t j = 0; j < i; j++)
+        if (g->names[j] == g->names[i]) {
+          nc[i] = (uint32_t)j;
+          break;
+        }
+    }
+    c->d_name_class.upload(nc.data(), nc.size(), s);
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
+  // index
+  HostIndex hi;
+  build_index(g, hi);
+  c->total_entries = hi.pos.size();
+  if (hi.pos.size() >= (1ull << 32)) throw PbwError{PBW_ERR_LIMIT, "index larger than 2^32 entries"};
+  c->d_kpos.upload(hi.pos.data(), hi.pos.size(), s);
+  DevBuf<uint32_t> dkeys, dstart;
+  dkeys.upload(hi.keys.data(), hi.keys.size(), s);
+  dstart.upload(hi.kstart.data(), hi.kstart.size(), s);
+  c->d_koff.ensure(CODE_SPACE + 1);
+  launch_fill_koff(c->d_koff.p, dkeys.p, dstart.p, hi.keys.size(), (uint32_t)hi.pos.size(), s);
+  HIP_CHECK(hipGetLastError());
+  // LUTs (bayesian.rs:103-110, :142-179, :405-439; seeds.rs:133-134; smith_waterman.rs:272-275)
+  const pbw_options& o = c->opt;
+  double sub_phred = h_py_round(std::log(std::max(o.substitution_rate, 1e-30)) / H_DB);
+  std::vector<double> lut;
+  auto region = [&](size_t n) {
+    size_t off = lut.size();
+    lut.resize(off + n, 0.0);
+    return off;
+  };
+  size_t o_match = region(256), o_mis = region(256), o_omq = region(256), o_qr = region(256), o_nrp = region(256);
+  size_t o_ln = region(LN_INT_MAX + 1);
+  size_t ngc = (size_t)(GC_LUT_MAX + 1) * (GC_LUT_MAX + 2) / 2;
+  size_t o_lnb = region(ngc), o_lnbm1 = region(ngc);
+  std::vector<int32_t> mis(256);
+  for (int q = 0; q < 256; q++) {
+    double eff = h_logadd((double)q, sub_phred);
+    double e = std::pow(10.0, -eff / 10.0);
+    double pm = 1.0 - e;
+    lut[o_match + q] = std::log(std::max(pm, 1e-300));
+    lut[o_mis + q] = std::log(std::max(e / 3.0, 1e-300));
+    double qv = std::min(std::pow(10.0, -eff / 10.0), 1.0 - 1e-12);
+    lut[o_omq + q] = 1.0 - qv;
+    lut[o_qr + q] = qv / (1.0 - qv);
+    lut[o_nrp + q] = std::pow(10.0, -(double)q / 10.0);
+    double scale = std::min(std::max((double)q / 30.0, 0.2), 1.0);
+    mis[q] = h_f64_i32(std::round(-30.0 * scale));
+  }
+  for (int k = 0; k <= LN_INT_MAX; k++) lut[o_ln + k] = std::log((double)k);
+  for (uint32_t t = 0; t <= (uint32_t)GC_LUT_MAX; t++) {
+    for (uint32_t gc = 0; gc <= t; gc++) {
+      double f = t == 0 ? 0.5 : (double)gc / (double)t;
+      double ff = std::min(std::max(f, 1e-6), 1.0 - 1e-6);
+      double b = std::exp(-ff * std::log(ff / 2.0) - (1.0 - ff) * std::log((1.0 - ff) / 2.0));
+      size_t idx = (size_t)t * (t + 1) / 2 + gc;
+      lut[o_lnb + idx] = std::log(b);
+      lut[o_lnbm1 + idx] = std::log(std::max(b - 1.0, 1e-6));
+    }
+  }
+  c->d_lut.upload(lut.data(), lut.size(), s);
+  c->d_mis.upload(mis.data(), mis.size(), s);
+  DevLuts& T = c->luts;
+  T.lk_match = c->d_lut.p + o_match;
+  T.lk_mis = c->d_lut.p + o_mis;
+  T.omq = c->d_lut.p + o_omq;
+  T.qratio = c->d_lut.p + o_qr;
+  T.nrp = c->d_lut.p + o_nrp;
+  T.mis_pen = c->d_mis.p;
+  T.ln_int = c->d_lut.p + o_ln;
+  T.gc_lnb = c->d_lut.p + o_lnb;
+  T.gc_lnbm1 = c->d_lut.p + o_lnbm1;
+  T.ln_quarter = std::log(0.25);
+  T.genome_size = (double)std::max<uint64_t>(g->total_length, 1);
+  T.ln_2g = std::log(2.0 * T.genome_size);
+  T.ln_30 = std::log(30.0);
+  T.sub_phred = sub_phred;
+  T.posterior_floor = std::exp(99.0 * H_DB);
+  // gap costs: smith_waterman.rs:113-130
+  c->gap_open = std::min(h_f64_i32(std::round(-90.0 * (double)o.gap_open_phred / 40.0)), -1);
+  c->gap_extend = std::min(h_f64_i32(std::round(-1.0 * (double)o.gap_extend_phred / 3.0)), -1);
+  c->d_ctr.ensure(1);
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------------------
+// single-end batch: seed -> cluster -> jobs -> DP -> MAPQ
+// ---------------------------------------------------------------------------
+struct HostRecords {
+  std::vector<pbw_record> rec;
+  std::vector<uint32_t> cigar;
+};
+
+static double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+static DpArgs dp_args_base(pbw_ctx* c, const DevReads& R) {
+  DpArgs a{};
+  a.G = c->dev_genome();
+  a.R = R;
+  a.T = c->luts;
+  a.gap_open = c->gap_open;
+  a.gap_extend = c->gap_extend;
+  a.ctr = c->d_ctr.p;
+  return a;
+}
+
+// Runs the tiled DP kernel over a job list (wide windows, odd shapes).
+static void run_dp_tiled(pbw_ctx* c, DpArgs a, const std::vector<uint32_t>& list) {
+  if (list.empty()) return;
+  hipStream_t s = c->stream;
+  c->d_list.upload(list.data(), list.size(), s);
+  a.jlist = c->d_list.p;
+  a.njobs = (uint32_t)list.size();
+  a.job_valid = nullptr;
+  uint32_t nwaves = std::min<uint32_t>((uint32_t)list.size(), 2048);
+  a.tiles = (a.max_n + 1 + 63) / 64;
+  c->d_rows.ensure((size_t)nwaves * 3 * (a.max_n + 1));
+  c->d_tb.ensure((size_t)nwaves * (a.max_m + 1) * a.tiles * 4);
+  a.rows = c->d_rows.p;
+  a.tb = c->d_tb.p;
+  launch_dp_tiled(a, nwaves, s);
+  HIP_CHECK(hipGetLastError());
+}
+
+// Device state of one batch between the stages.
+struct Batch {
+  uint32_t B = 0;
+  uint32_t max_m = 0, max_n = 0, cap = 0;
+  std::vector<uint64_t> so, qo;  // rebased offsets
+  DevReads R{};
+};
+
+// Uploads reads given as per-read (seq, len, qual, qlen) views.
+static void batch_upload(pbw_ctx* c, Batch& b, const std::vector<const uint8_t*>& sp, const std::vector<uint64_t>& sl,
+                         const std::vector<const uint8_t*>& qp, const std::vector<uint64_t>& ql) {
+  hipStream_t s = c->stream;
+  const uint32_t B = (uint32_t)sp.size();
+  b.B = B;
+  b.so.assign(B + 1, 0);
+  b.qo.assign(B + 1, 0);
+  b.max_m = 0;
+  for (uint32_t i = 0; i < B; i++) {
+    if (sl[i] > (uint64_t)MAX_READ_LEN || ql[i] > (uint64_t)MAX_READ_LEN)
+      throw PbwError{PBW_ERR_LIMIT, "read longer than " + std::to_string(MAX_READ_LEN) + " bases"};
+    b.so[i + 1] = b.so[i] + sl[i];
+    b.qo[i + 1] = b.qo[i] + ql[i];
+    b.max_m = std::max<uint32_t>(b.max_m, (uint32_t)std::max(sl[i], ql[i]));
+  }
+  std::vector<uint8_t> seq(b.so[B] + 1), qual(b.qo[B] + 1);
+  for (uint32_t i = 0; i < B; i++) {
+    if (sl[i]) memcpy(seq.data() + b.so[i], sp[i], sl[i]);
+    if (ql[i]) memcpy(qual.data() + b.qo[i], qp[i], ql[i]);
+  }
+  c->d_seq.upload(seq.data(), seq.size(), s);
+  c->d_qual.upload(qual.data(), qual.size(), s);
+  c->d_seq_off.upload(b.so.data(), B + 1, s);
+  c->d_qual_off.upload(b.qo.data(), B + 1, s);
+  b.R = DevReads{c->d_seq.p, c->d_seq_off.p, c->d_qual.p, c->d_qual_off.p, B};
+  c->d_rcode.ensure(b.so[B] + 1);
+  HIP_CHECK(hipStreamSynchronize(s));  // host staging vectors die here
+}
+
+// Uploads a contiguous slice [b0, b1) of a read batch.
+static void batch_upload_slice(pbw_ctx* c, Batch& b, const pbw_read_batch* in, uint64_t b0, uint64_t b1) {
+  hipStream_t s = c->stream;
+  const uint32_t B = (uint32_t)(b1 - b0);
+  b.B = B;
+  b.so.resize(B + 1);
+  b.qo.resize(B + 1);
+  b.max_m = 0;
+  for (uint32_t i = 0; i <= B; i++) {
+    b.so[i] = in->seq_off[b0 + i] - in->seq_off[b0];
+    b.qo[i] = in->qual_off[b0 + i] - in->qual_off[b0];
+  }
+  for (uint32_t i = 0; i < B; i++) {
+    uint64_t L = b.so[i + 1] - b.so[i], ql = b.qo[i + 1] - b.qo[i];
+    if (L > (uint64_t)MAX_READ_LEN || ql > (uint64_t)MAX_READ_LEN)
+      throw PbwError{PBW_ERR_LIMIT, "read longer than " + std::to_string(MAX_READ_LEN) + " bases"};
+    b.max_m = std::max<uint32_t>(b.max_m, (uint32_t)std::max(L, ql));
+  }
+  c->d_seq.upload(in->seq + in->seq_off[b0], b.so[B], s);
+  c->d_qual.upload(in->qual + in->qual_off[b0], b.qo[B], s);
+  c->d_seq_off.upload(b.so.data(), B + 1, s);
+  c->d_qual_off.upload(b.qo.data(), B + 1, s);
+  b.R = DevReads{c->d_seq.p, c->d_seq_off.p, c->d_qual.p, c->d_qual_off.p, B};
+  c->d_rcode.ensure(b.so[B] + 1);
+}
+
+// map_candidates (single.rs:86-162) for every read of the batch: seed,
+// cluster, DP jobs, banded DP.  Results stay on the device.
+static void batch_candidates(pbw_ctx* c, Batch& b) {
+  hipStream_t s = c->stream;
+  const uint32_t B = b.B;
+  const DevReads R = b.R;
+  auto t0 = std::chrono::steady_clock::now();
+  // ---- seed ----
+  std::vector<uint64_t> hoff(B);
+  for (uint32_t i = 0; i < B; i++) hoff[i] = (uint64_t)i * HIT_CAP;
+  c->d_hoff.upload(hoff.data(), B, s);
+  c->d_hits.ensure((size_t)B * HIT_CAP);
+  c->d_nhits.ensure(B);
+  SeedArgs sa{};
+  sa.G = c->dev_genome();
+  sa.X = DevIndex{c->d_koff.p, c->d_kpos.p};
+  sa.R = R;
+  sa.rlist = nullptr;
+  sa.nlist = B;
+  sa.hoff = c->d_hoff.p;
+  sa.hcap = nullptr;
+  sa.hits = c->d_hits.p;
+  sa.nhits = c->d_nhits.p;
+  sa.rcode = c->d_rcode.p;
+  sa.ctr = c->d_ctr.p;
+  launch_seed(sa, s);
+  HIP_CHECK(hipGetLastError());
+  std::vector<uint32_t> nh(B);
+  HIP_CHECK(hipMemcpyAsync(nh.data(), c->d_nhits.p, (size_t)B * 4, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  // overflow: rerun the affected reads with exact capacity into a second arena
+  std::vector<uint32_t> ov;
+  uint64_t ov_total = 0;
+  uint32_t max_hits = 0;
+  for (uint32_t i = 0; i < B; i++) {
+    max_hits = std::max(max_hits, nh[i]);
+    if (nh[i] > (uint32_t)HIT_CAP) {
+      ov.push_back(i);
+      ov_total += nh[i];
+    }
+  }
+  const uint64_t* hits_base = c->d_hits.p;
+  if (!ov.empty()) {
+    c->d_hits_ov.ensure((size_t)B * HIT_CAP + ov_total);
+    HIP_CHECK(hipMemcpyAsync(c->d_hits_ov.p, c->d_hits.p, (size_t)B * HIT_CAP * 8, hipMemcpyDeviceToDevice, s));
+    std::vector<uint64_t> ovoff(ov.size());
+    std::vector<uint32_t> ovcap(ov.size());
+    uint64_t cur = (uint64_t)B * HIT_CAP;
+    for (size_t k = 0; k < ov.size(); k++) {
+      ovoff[k] = cur;
+      ovcap[k] = nh[ov[k]];
+      hoff[ov[k]] = cur;
+      cur += nh[ov[k]];
+    }
+    DevBuf<uint64_t> dovoff;
+    dovoff.upload(ovoff.data(), ov.size(), s);
+    c->d_list.upload(ov.data(), ov.size(), s);
+    c->d_hcap.upload(ovcap.data(), ov.size(), s);
+    SeedArgs sb = sa;
+    sb.rlist = c->d_list.p;
+    sb.nlist = (uint32_t)ov.size();
+    sb.hoff = dovoff.p;
+    sb.hcap = c->d_hcap.p;
+    sb.hits = c->d_hits_ov.p;
+    sb.ctr = nullptr;  // counted once, in the first pass
+    launch_seed(sb, s);
+    HIP_CHECK(hipGetLastError());
+    c->d_hoff.upload(hoff.data(), B, s);
+    HIP_CHECK(hipStreamSynchronize(s));
+    hits_base = c->d_hits_ov.p;
+    c->stats.seed_overflow_reads += ov.size();
+  }
+  c->stats.ms_seed += ms_since(t0);
+  t0 = std::chrono::steady_clock::now();
+  // ---- cluster ----
+  c->d_cands.ensure((size_t)B * TOP_CANDIDATES);
+  c->d_ncand.ensure(B);
+  c->d_tier2.ensure(B);
+  c->d_ntier2.ensure(1);
+  HIP_CHECK(hipMemsetAsync(c->d_ntier2.p, 0, 4, s));
+  ClusterArgs ca{};
+  ca.R = R;
+  ca.hits = hits_base;
+  ca.hoff = c->d_hoff.p;
+  ca.nhits = c->d_nhits.p;
+  ca.cands = c->d_cands.p;
+  ca.ncand = c->d_ncand.p;
+  ca.tier2 = c->d_tier2.p;
+  ca.ntier2 = c->d_ntier2.p;
+  launch_cluster(ca, s);
+  HIP_CHECK(hipGetLastError());
+  uint32_t nt2 = 0;
+  HIP_CHECK(hipMemcpyAsync(&nt2, c->d_ntier2.p, 4, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  if (nt2) {
+    uint64_t p2 = 1;
+    while (p2 < max_hits) p2 <<= 1;
+    uint32_t blocks = std::min<uint32_t>(nt2, 64);
+    ca.scratch_per_block = 6 * p2 + 16;
+    c->d_scratch.ensure((size_t)blocks * ca.scratch_per_block);
+    ca.scratch = c->d_scratch.p;
+    launch_cluster_tier2(ca, c->d_tier2.p, nt2, s);
+    HIP_CHECK(hipGetLastError());
+    c->stats.tier2_cluster_reads += nt2;
+  }
+  HIP_CHECK(hipStreamSynchronize(s));
+  c->stats.ms_cluster += ms_since(t0);
+  t0 = std::chrono::steady_clock::now();
+  // ---- DP jobs ----
+  const int32_t band = (int32_t)c->opt.max_indel_len;
+  const int32_t right_pad = (int32_t)std::max<uint32_t>(c->opt.max_indel_len, 10);  // single.rs:110
+  c->d_jobs.ensure((size_t)B * TOP_CANDIDATES);
+  c->d_njobs.ensure(B);
+  JobArgs ja{};
+  ja.G = c->dev_genome();
+  ja.R = R;
+  ja.cands = c->d_cands.p;
+  ja.ncand = c->d_ncand.p;
+  ja.right_pad = right_pad;
+  ja.band = band;
+  ja.jobs = c->d_jobs.p;
+  ja.njobs = c->d_njobs.p;
+  launch_build_jobs(ja, s);
+  HIP_CHECK(hipGetLastError());
+  b.max_n = b.max_m + (uint32_t)right_pad + LEFT_PAD;
+  b.cap = b.max_m + b.max_n + 2;
+  const size_t nslots = (size_t)B * TOP_CANDIDATES;
+  c->d_out.ensure(nslots);
+  c->d_cigar.ensure(nslots * b.cap);
+  c->d_generic.ensure(nslots);
+  HIP_CHECK(hipMemsetAsync(c->d_generic.p, 0, nslots, s));
+  DpArgs da = dp_args_base(c, R);
+  da.jobs = c->d_jobs.p;
+  da.jlist = nullptr;
+  da.njobs = (uint32_t)nslots;
+  da.job_valid = c->d_njobs.p;
+  da.cigar_cap = b.cap;
+  da.cigar = c->d_cigar.p;
+  da.out = c->d_out.p;
+  da.needs_generic = c->d_generic.p;
+  da.max_m = b.max_m;
+  da.max_n = b.max_n;
+  if (2 * band + 1 <= 63) {
+    launch_dp_band(da, s);
+    HIP_CHECK(hipGetLastError());
+    std::vector<uint8_t> gen(nslots);
+    HIP_CHECK(hipMemcpyAsync(gen.data(), c->d_generic.p, nslots, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<uint32_t> list;
+    for (size_t k = 0; k < nslots; k++)
+      if (gen[k]) list.push_back((uint32_t)k);
+    run_dp_tiled(c, da, list);
+    c->stats.generic_dp_jobs += list.size();
+  } else {
+    std::vector<uint8_t> nj(B);
+    HIP_CHECK(hipMemcpyAsync(nj.data(), c->d_njobs.p, B, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<uint32_t> list;
+    for (uint32_t r = 0; r < B; r++)
+      for (uint32_t k = 0; k < nj[r]; k++) list.push_back(r * TOP_CANDIDATES + k);
+    run_dp_tiled(c, da, list);
+    c->stats.generic_dp_jobs += list.size();
+  }
+  HIP_CHECK(hipStreamSynchronize(s));
+  c->stats.ms_dp += ms_since(t0);
+}
+
+// Downloads records (already in d_rec) and gathers their CIGARs from one or
+// two pools (src bit 63 selects the second); unmapped records get {L}S.
+static void download_records(pbw_ctx* c, const Batch& b, uint32_t nrec, const std::vector<uint64_t>& src,
+                             const uint32_t* pool1, HostRecords& hr) {
+  hipStream_t s = c->stream;
+  size_t r0 = hr.rec.size();
+  hr.rec.resize(r0 + nrec);
+  HIP_CHECK(hipMemcpyAsync(hr.rec.data() + r0, c->d_rec.p, (size_t)nrec * sizeof(pbw_record), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  std::vector<uint64_t> gsrc;
+  std::vector<uint32_t> cnt, offs;
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < nrec; i++) {
+    if (src[i] == ~0ull) continue;
+    gsrc.push_back(src[i]);
+    cnt.push_back(hr.rec[r0 + i].cigar_n);
+    offs.push_back((uint32_t)total);
+    total += hr.rec[r0 + i].cigar_n;
+  }
+  std::vector<uint32_t> words(total);
+  if (total) {
+    DevBuf<uint64_t> dsrc;
+    DevBuf<uint32_t> dcnt;
+    dsrc.upload(gsrc.data(), gsrc.size(), s);
+    dcnt.upload(cnt.data(), cnt.size(), s);
+    c->d_cigar_off.upload(offs.data(), offs.size(), s);
+    c->d_cigar_out.ensure(total);
+    gather_cigar2(c->d_cigar.p, pool1 ? pool1 : c->d_cigar.p, dsrc.p, c->d_cigar_off.p, dcnt.p, (uint32_t)gsrc.size(),
+                  c->d_cigar_out.p, s);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(words.data(), c->d_cigar_out.p, total * 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
+  uint64_t w = 0;
+  for (uint32_t i = 0; i < nrec; i++) {
+    pbw_record& rr = hr.rec[r0 + i];
+    rr.cigar_off = (uint32_t)hr.cigar.size();
+    if (src[i] == ~0ull) {
+      uint64_t L = b.so[i + 1] - b.so[i];
+      hr.cigar.push_back((uint32_t)(L << 4) | 4u);  // SoftClip(L), single.rs:368
+      rr.cigar_n = 1;
+    } else {
+      hr.cigar.insert(hr.cigar.end(), words.begin() + w, words.begin() + w + rr.cigar_n);
+      w += rr.cigar_n;
+    }
+  }
+}
+
+static void collect_counters(pbw_ctx* c, uint32_t reads) {
+  Counters ctr{};
+  HIP_CHECK(hipMemcpy(&ctr, c->d_ctr.p, sizeof(Counters), hipMemcpyDeviceToHost));
+  c->stats.lookups += ctr.lookups;
+  c->stats.positions += ctr.positions;
+  c->stats.filtered_hits += ctr.filtered;
+  c->stats.dp_cells += ctr.cells;
+  c->stats.reads += reads;
+  HIP_CHECK(hipMemsetAsync(c->d_ctr.p, 0, sizeof(Counters), c->stream));
+}
+
+// SingleEndMapper::map (single.rs:164-207) for every read of a batch.
+static void map_se_batch(pbw_ctx* c, const pbw_read_batch* in, uint64_t b0, uint64_t b1, HostRecords& hr) {
+  hipStream_t s = c->stream;
+  Batch b;
+  batch_upload_slice(c, b, in, b0, b1);
+  HIP_CHECK(hipMemsetAsync(c->d_ctr.p, 0, sizeof(Counters), s));
+  batch_candidates(c, b);
+  auto t0 = std::chrono::steady_clock::now();
+  const uint32_t B = b.B;
+  c->d_rec.ensure((size_t)B * sizeof(pbw_record));
+  c->d_best.ensure(B);
+  MapqArgs ma{};
+  ma.G = c->dev_genome();
+  ma.R = b.R;
+  ma.T = c->luts;
+  ma.jobs = c->d_jobs.p;
+  ma.out = c->d_out.p;
+  ma.cigar = c->d_cigar.p;
+  ma.cigar_cap = b.cap;
+  ma.njobs = c->d_njobs.p;
+  ma.cands = c->d_cands.p;
+  ma.rcode = c->d_rcode.p;
+  ma.rec = c->d_rec.p;
+  ma.best = c->d_best.p;
+  launch_mapq(ma, s);
+  HIP_CHECK(hipGetLastError());
+  std::vector<uint32_t> best(B);
+  HIP_CHECK(hipMemcpyAsync(best.data(), c->d_best.p, (size_t)B * 4, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  std::vector<uint64_t> src(B);
+  for (uint32_t i = 0; i < B; i++) src[i] = best[i] == 0xFFFFFFFFu ? ~0ull : (uint64_t)best[i] * b.cap;
+  download_records(c, b, B, src, nullptr, hr);
+  c->stats.ms_mapq += ms_since(t0);
+  collect_counters(c, B);
+}
+
+// ---------------------------------------------------------------------------
+// paired-end: paired.rs:101-234 per batch with a frozen insert-size snapshot
+// ---------------------------------------------------------------------------
+struct HostInsert {  // insert_size.rs:23-44
+  double mean, sd;
+  uint64_t n;
+  void update(double obs) {
+    n += 1;
+    double nn = (double)n;
+    double delta = obs - mean;
+    mean += delta / nn;
+    double delta2 = obs - mean;
+    double var = nn > 1.0 ? ((sd * sd) * (nn - 1.0) + delta * delta2) / nn : sd * sd;
+    sd = std::max(std::sqrt(var), 1.0);
+  }
+};
+
+static uint64_t rust_f64_to_usize(double x) {  // `as usize`: saturating, NaN -> 0
+  if (!(x > 0.0)) return 0;
+  if (x >= 18446744073709551615.0) return UINT64_MAX;
+  return (uint64_t)x;
+}
+
+static void map_pe_batch(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_batch* m2, uint64_t p0, uint64_t p1,
+                         const HostInsert& snap, HostRecords& hr, std::vector<double>& obs_out,
+                         std::vector<uint8_t>& has_out) {
+  hipStream_t s = c->stream;
+  const uint32_t P = (uint32_t)(p1 - p0);
+  std::vector<const uint8_t*> sp(2 * P), qp(2 * P);
+  std::vector<uint64_t> sl(2 * P), ql(2 * P);
+  for (uint32_t i = 0; i < P; i++) {
+    const pbw_read_batch* mm[2] = {m1, m2};
+    for (int k = 0; k < 2; k++) {
+      const pbw_read_batch* x = mm[k];
+      uint64_t r = p0 + i;
+      sp[2 * i + k] = x->seq + x->seq_off[r];
+      sl[2 * i + k] = x->seq_off[r + 1] - x->seq_off[r];
+      qp[2 * i + k] = x->qual + x->qual_off[r];
+      ql[2 * i + k] = x->qual_off[r + 1] - x->qual_off[r];
+    }
+  }
+  Batch b;
+  batch_upload(c, b, sp, sl, qp, ql);
+  batch_candidates(c, b);
+  auto t0 = std::chrono::steady_clock::now();
+  // cross-alignment plan
+  PairArgs pa{};
+  pa.G = c->dev_genome();
+  pa.R = b.R;
+  pa.T = c->luts;
+  pa.jobs = c->d_jobs.p;
+  pa.out = c->d_out.p;
+  pa.cigar = c->d_cigar.p;
+  pa.cigar_cap = b.cap;
+  pa.njobs = c->d_njobs.p;
+  pa.cands = c->d_cands.p;
+  pa.rcode = c->d_rcode.p;
+  const size_t nws = (size_t)P * 40;
+  c->d_wjobs.ensure(nws);
+  c->d_wvalid.ensure(nws);
+  c->d_slcand.ensure(nws);
+  c->d_plan.ensure(P);
+  pa.wjobs = c->d_wjobs.p;
+  pa.wvalid = c->d_wvalid.p;
+  pa.slcand = c->d_slcand.p;
+  pa.plan = c->d_plan.p;
+  pa.mean = snap.mean;
+  pa.sd = snap.sd;
+  pa.ln_norm = std::log(2.0 * M_PI * snap.sd * snap.sd);  // insert_size.rs:27-28
+  pa.sv_prior_log = -c->opt.sv_prior_phred / 10.0 * H_LN_10;  // paired.rs:97
+  pa.mean_usize = rust_f64_to_usize(std::max(snap.mean, 0.0));
+  pa.radius_base = rust_f64_to_usize(std::ceil(5.0 * snap.sd));
+  pa.name_class = c->d_name_class.p;
+  pa.P = P;
+  launch_pair_plan(pa, s);
+  HIP_CHECK(hipGetLastError());
+  std::vector<uint8_t> wv(nws);
+  HIP_CHECK(hipMemcpyAsync(wv.data(), c->d_wvalid.p, nws, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  std::vector<uint32_t> wlist;
+  for (size_t k = 0; k < nws; k++)
+    if (wv[k]) wlist.push_back((uint32_t)k);
+  // wide DP (align_at_window, single.rs:220-275): window <= 2 radius + L
+  const uint64_t wmax = 2 * (pa.radius_base + b.max_m) + b.max_m;
+  if (!wlist.empty() && wmax > 8000)
+    throw PbwError{PBW_ERR_LIMIT, "insert-size search window wider than 8000 bases"};
+  const uint32_t wcap = b.max_m + (uint32_t)wmax + 2;
+  c->d_wout.ensure(nws);
+  c->d_wcigar.ensure(nws * wcap);
+  DpArgs da = dp_args_base(c, b.R);
+  da.jobs = c->d_wjobs.p;
+  da.cigar_cap = wcap;
+  da.cigar = c->d_wcigar.p;
+  da.out = c->d_wout.p;
+  c->d_generic.ensure(nws);
+  da.needs_generic = c->d_generic.p;
+  da.max_m = b.max_m;
+  da.max_n = (uint32_t)wmax;
+  c->stats.wide_jobs += wlist.size();
+  Counters before{};
+  HIP_CHECK(hipMemcpyAsync(&before, c->d_ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  run_dp_tiled(c, da, wlist);
+  Counters after{};
+  HIP_CHECK(hipMemcpyAsync(&after, c->d_ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  c->stats.wide_cells += after.cells - before.cells;
+  HIP_CHECK(hipMemcpyAsync(c->d_ctr.p, &before, sizeof(Counters), hipMemcpyHostToDevice, s));
+  // joint posterior + records
+  c->d_rec.ensure((size_t)2 * P * sizeof(pbw_record));
+  c->d_cig_src.ensure(2 * (size_t)P);
+  c->d_obs.ensure(P);
+  c->d_has_obs.ensure(P);
+  pa.wout = c->d_wout.p;
+  pa.wcigar = c->d_wcigar.p;
+  pa.wcap = wcap;
+  pa.rec = c->d_rec.p;
+  pa.cig_src = c->d_cig_src.p;
+  pa.obs = c->d_obs.p;
+  pa.has_obs = c->d_has_obs.p;
+  launch_pair_finalize(pa, s);
+  HIP_CHECK(hipGetLastError());
+  std::vector<uint64_t> src(2 * (size_t)P);
+  obs_out.resize(P);
+  has_out.resize(P);
+  HIP_CHECK(hipMemcpyAsync(src.data(), c->d_cig_src.p, 2 * (size_t)P * 8, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(obs_out.data(), c->d_obs.p, (size_t)P * 8, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(has_out.data(), c->d_has_obs.p, P, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  for (auto& x : src)
+    if (x != ~0ull && (x >> 63)) x = (1ull << 63) | (x & ~(1ull << 63));
+  download_records(c, b, 2 * P, src, c->d_wcigar.p, hr);
+  c->stats.ms_pair += ms_since(t0);
+  collect_counters(c, 2 * P);
+}
+
+// lib.rs:145-185: 500 serial warm-up pairs, then 2000-pair batches
+static void map_pe_all(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_batch* m2, pbw_insert_model* model,
+                       HostRecords& hr) {
+  HostInsert im{model->mean, model->sd, model->n_observed};
+  const uint64_t n = m1->n;
+  std::vector<double> obs;
+  std::vector<uint8_t> has;
+  const uint64_t warm = std::min<uint64_t>(n, 500);
+  for (uint64_t i = 0; i < warm; i++) {
+    HostInsert snap = im;
+    map_pe_batch(c, m1, m2, i, i + 1, snap, hr, obs, has);
+    if (has[0]) im.update(obs[0]);
+  }
+  for (uint64_t b0 = warm; b0 < n; b0 += 2000) {
+    uint64_t b1 = std::min<uint64_t>(b0 + 2000, n);
+    HostInsert snap = im;
+    map_pe_batch(c, m1, m2, b0, b1, snap, hr, obs, has);
+    for (uint64_t k = 0; k < b1 - b0; k++)
+      if (has[k]) im.update(obs[k]);
+  }
+  model->mean = im.mean;
+  model->sd = im.sd;
+  model->n_observed = im.n;
+}
+
+static void emit_batch(HostRecords& hr, pbw_record_batch* out) {
+  out->n = hr.rec.size();
+  out->rec = (pbw_record*)malloc(std::max<size_t>(hr.rec.size(), 1) * sizeof(pbw_record));
+  out->cigar = (uint32_t*)malloc(std::max<size_t>(hr.cigar.size(), 1) * 4);
+  out->n_cigar = hr.cigar.size();
+  if (!hr.rec.empty()) memcpy(out->rec, hr.rec.data(), hr.rec.size() * sizeof(pbw_record));
+  if (!hr.cigar.empty()) memcpy(out->cigar, hr.cigar.data(), hr.cigar.size() * 4);
+}
+
+template <typename F>
+static int guarded(pbw_ctx* c, F&& f) {
+  try {
+    f();
+    return PBW_OK;
+  } catch (const PbwError& e) {
+    if (c) c->error = e.msg;
+    g_error = e.msg;
+    return e.code;
+  } catch (const std::exception& e) {
+    if (c) c->error = e.what();
+    g_error = e.what();
+    return PBW_ERR_ARG;
+  }
+}
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* pbw_global_error(void) { return g_error.c_str(); }
+const char* pbw_last_error(const pbw_ctx* c) { return c ? c->error.c_str() : g_error.c_str(); }
+
+int pbw_genome_create(pbw_genome** out) {
+  if (!out) return PBW_ERR_ARG;
+  *out = new pbw_genome();
+  return PBW_OK;
+}
+int pbw_genome_add_contig(pbw_genome* g, const char* name, const uint8_t* seq, uint64_t len) {
+  if (!g || !name || (!seq && len)) return PBW_ERR_ARG;
+  genome_add(g, name, seq, len);
+  return PBW_OK;
+}
+// genome.rs:119-134 + FastaReader (:150-213)
+int pbw_genome_add_fasta(pbw_genome* g, const char* path) {
+  if (!g || !path) return PBW_ERR_ARG;
+  FILE* f = fopen(path, "rb");
+  if (!f) {
+    g_error = std::string("cannot open ") + path;
+    return PBW_ERR_IO;
+  }
+  std::string line, pending;
+  bool has_pending = false;
+  for (;;) {
+    std::string header;
+    if (has_pending) {
+      header = pending;
+      has_pending = false;
+    } else {
+      bool found = false;
+      while (read_line(f, line))
+        if (!line.empty() && line[0] == '>') {
+          found = true;
+          break;
+        }
+      if (!found) break;
+      header = line;
+    }
+    size_t a = 0;
+    while (a < header.size() && header[a] == '>') a++;
+    std::string name = first_tok(header.substr(a));
+    std::vector<uint8_t> seq;
+    while (read_line(f, line)) {
+      std::string t = trim_str(line);
+      if (!t.empty() && t[0] == '>') {
+        pending = line;
+        has_pending = true;
+        break;
+      }
+      seq.insert(seq.end(), t.begin(), t.end());
+    }
+    genome_add(g, name, seq.data(), seq.size());
+  }
+  fclose(f);
+  return PBW_OK;
+}
+int pbw_genome_n_contigs(const pbw_genome* g) { return g ? (int)g->lengths.size() : 0; }
+const char* pbw_genome_contig_name(const pbw_genome* g, int cid) { return g->names[cid].c_str(); }
+uint64_t pbw_genome_contig_length(const pbw_genome* g, int cid) { return g->lengths[cid]; }
+void pbw_genome_destroy(pbw_genome* g) { delete g; }
+
+void pbw_default_options(pbw_options* o) {
+  o->substitution_rate = 0.001;
+  o->insert_size_mean = 250.0;
+  o->insert_size_sd = 60.0;
+  o->sv_prior_phred = 55.0;
+  o->max_indel_len = 30;
+  o->gap_open_phred = 40;
+  o->gap_extend_phred = 3;
+  o->batch_reads = 0;
+}
+
+int pbw_ctx_create(const pbw_genome* g, const pbw_options* o, int device, pbw_ctx** out) {
+  if (!g || !out) return PBW_ERR_ARG;
+  pbw_ctx* c = new pbw_ctx();
+  c->genome = g;
+  if (o) c->opt = *o;
+  else pbw_default_options(&c->opt);
+  c->device = device;
+  int rc = guarded(c, [&] { ctx_build(c); });
+  if (rc != PBW_OK) {
+    g_error = c->error;
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    *out = nullptr;
+    return rc;
+  }
+  *out = c;
+  return PBW_OK;
+}
+
+void pbw_ctx_destroy(pbw_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int pbw_map_se(pbw_ctx* c, const pbw_read_batch* reads, pbw_record_batch* out) {
+  if (!c || !reads || !out) return PBW_ERR_ARG;
+  return guarded(c, [&] {
+    HIP_CHECK(hipSetDevice(c->device));
+    c->stats = pbw_stats{};
+    auto t0 = std::chrono::steady_clock::now();
+    HostRecords hr;
+    hr.rec.reserve(reads->n);
+    uint64_t bs = c->opt.batch_reads ? c->opt.batch_reads : 131072;
+    for (uint64_t b = 0; b < reads->n; b += bs) map_se_batch(c, reads, b, std::min(reads->n, b + bs), hr);
+    emit_batch(hr, out);
+    c->stats.ms_total = ms_since(t0);
+  });
+}
+
+int pbw_map_pe(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_batch* m2, pbw_insert_model* model,
+               pbw_record_batch* out) {
+  if (!c || !m1 || !m2 || !model || !out) return PBW_ERR_ARG;
+  if (m1->n != m2->n) {
+    c->error = "paired-end input files have different read counts (" + std::to_string(m1->n) + " vs " +
+               std::to_string(m2->n) + ")";
+    return PBW_ERR_MISMATCH;
+  }
+  return guarded(c, [&] {
+    HIP_CHECK(hipSetDevice(c->device));
+    c->stats = pbw_stats{};
+    auto t0 = std::chrono::steady_clock::now();
+    HostRecords hr;
+    hr.rec.reserve(2 * m1->n);
+    map_pe_all(c, m1, m2, model, hr);
+    emit_batch(hr, out);
+    c->stats.ms_total = ms_since(t0);
+  });
+}
+
+int pbw_align_batch(pbw_ctx* c, uint32_t gop, uint32_t gep, const pbw_align_job* jobs, uint64_t n,
+                    pbw_align_result* res, uint32_t* cigar, uint64_t cigar_cap) {
+  if (!c || (!jobs && n) || (!res && n)) return PBW_ERR_ARG;
+  return guarded(c, [&] {
+    HIP_CHECK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    if (n == 0) return;
+    if (n > (1u << 30)) throw PbwError{PBW_ERR_LIMIT, "too many jobs"};
+    std::vector<uint8_t> seq, qual, ref;
+    std::vector<uint64_t> so(n + 1), qo(n + 1);
+    std::vector<DpJob> dj(n);
+    uint32_t max_m = 0, max_n = 0;
+    for (uint64_t i = 0; i < n; i++) {
+      const pbw_align_job& j = jobs[i];
+      if (j.m > (uint32_t)MAX_READ_LEN || j.qlen > (uint32_t)MAX_READ_LEN || j.n > 8000)
+        throw PbwError{PBW_ERR_LIMIT, "job too large"};
+      so[i] = seq.size();
+      qo[i] = qual.size();
+      seq.insert(seq.end(), j.read, j.read + j.m);
+      qual.insert(qual.end(), j.qual, j.qual + j.qlen);
+      DpJob d{};
+      d.ref_gb = ref.size();
+      d.read = (uint32_t)i;
+      d.n = j.n;
+      d.hint = (int32_t)std::max<int64_t>(std::min<int64_t>(j.hint, INT32_MAX / 2), INT32_MIN / 2);
+      d.band = (int32_t)std::min<uint32_t>(j.band, (uint32_t)(INT32_MAX / 4));
+      d.rc = 0;
+      d.explicit_ref = 1;
+      d.cand = (uint32_t)i;
+      dj[i] = d;
+      ref.insert(ref.end(), j.ref, j.ref + j.n);
+      max_m = std::max(max_m, std::max(j.m, j.qlen));
+      max_n = std::max(max_n, j.n);
+    }
+    so[n] = seq.size();
+    qo[n] = qual.size();
+    c->d_seq.upload(seq.data(), seq.size(), s);
+    c->d_qual.upload(qual.data(), qual.size(), s);
+    c->d_seq_off.upload(so.data(), n + 1, s);
+    c->d_qual_off.upload(qo.data(), n + 1, s);
+    c->d_refbytes.upload(ref.data(), ref.size(), s);
+    c->d_jobs.upload(dj.data(), n, s);
+    DevReads R{c->d_seq.p, c->d_seq_off.p, c->d_qual.p, c->d_qual_off.p, (uint32_t)n};
+    const uint32_t cap = max_m + max_n + 2;
+    c->d_out.ensure(n);
+    c->d_cigar.ensure(n * cap);
+    c->d_generic.ensure(n);
+    HIP_CHECK(hipMemsetAsync(c->d_generic.p, 0, n, s));
+    HIP_CHECK(hipMemsetAsync(c->d_ctr.p, 0, sizeof(Counters), s));
+    DpArgs da = dp_args_base(c, R);
+    da.gap_open = std::min(h_f64_i32(std::round(-90.0 * (double)gop / 40.0)), -1);
+    da.gap_extend = std::min(h_f64_i32(std::round(-1.0 * (double)gep / 3.0)), -1);
+    da.jobs = c->d_jobs.p;
+    da.ref_bytes = c->d_refbytes.p;
+    da.cigar_cap = cap;
+    da.cigar = c->d_cigar.p;
+    da.out = c->d_out.p;
+    da.needs_generic = c->d_generic.p;
+    // banded kernel for narrow bands and moderate windows, tiled for the rest
+    std::vector<uint32_t> band_list, tiled_list;
+    uint32_t bm = 0, bn = 0;
+    for (uint64_t i = 0; i < n; i++) {
+      if (2ull * jobs[i].band + 1 <= 63 && jobs[i].n <= 2048) {
+        band_list.push_back((uint32_t)i);
+        bm = std::max(bm, std::max(jobs[i].m, jobs[i].qlen));
+        bn = std::max(bn, jobs[i].n);
+      } else {
+        tiled_list.push_back((uint32_t)i);
+      }
+    }
+    if (!band_list.empty()) {
+      DevBuf<uint32_t> dl;
+      dl.upload(band_list.data(), band_list.size(), s);
+      DpArgs db = da;
+      db.jlist = dl.p;
+      db.njobs = (uint32_t)band_list.size();
+      db.max_m = bm;
+      db.max_n = bn;
+      launch_dp_band(db, s);
+      HIP_CHECK(hipGetLastError());
+      std::vector<uint8_t> gen(n);
+      HIP_CHECK(hipMemcpyAsync(gen.data(), c->d_generic.p, n, hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      for (uint32_t i : band_list)
+        if (gen[i]) tiled_list.push_back(i);
+    }
+    da.max_m = max_m;
+    da.max_n = max_n;
+    run_dp_tiled(c, da, tiled_list);
+    std::vector<DpOut> out(n);
+    HIP_CHECK(hipMemcpyAsync(out.data(), c->d_out.p, n * sizeof(DpOut), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    uint64_t w = 0;
+    for (uint64_t i = 0; i < n; i++) {
+      res[i].score = out[i].score;
+      res[i].ref_offset = out[i].ref_off;
+      res[i].cigar_off = (uint32_t)w;
+      res[i].cigar_n = out[i].cigar_n;
+      if (cigar && w + out[i].cigar_n <= cigar_cap && out[i].cigar_n)
+        HIP_CHECK(hipMemcpyAsync(cigar + w, c->d_cigar.p + (uint64_t)i * cap, out[i].cigar_n * 4,
+                                 hipMemcpyDeviceToHost, s));
+      w += out[i].cigar_n;
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (cigar && w > cigar_cap) throw PbwError{PBW_ERR_LIMIT, "cigar buffer too small"};
+  });
+}
+
+int64_t pbw_index_lookup(pbw_ctx* c, uint32_t code, uint64_t* out, uint64_t cap) {
+  if (!c || code >= CODE_SPACE) return PBW_ERR_ARG;
+  int64_t result = 0;
+  int rc = guarded(c, [&] {
+    HIP_CHECK(hipSetDevice(c->device));
+    uint32_t off[2];
+    HIP_CHECK(hipMemcpy(off, c->d_koff.p + code, 8, hipMemcpyDeviceToHost));
+    uint64_t n = off[1] - off[0];
+    if (out && n) HIP_CHECK(hipMemcpy(out, c->d_kpos.p + off[0], std::min(n, cap) * 8, hipMemcpyDeviceToHost));
+    result = (int64_t)n;
+  });
+  return rc == PBW_OK ? result : rc;
+}
+uint64_t pbw_index_total_entries(const pbw_ctx* c) { return c ? c->total_entries : 0; }
+
+void pbw_records_release(pbw_record_batch* b) {
+  if (!b) return;
+  free(b->rec);
+  free(b->cigar);
+  b->rec = nullptr;
+  b->cigar = nullptr;
+  b->n = b->n_cigar = 0;
+}
+int pbw_get_stats(const pbw_ctx* c, pbw_stats* out) {
+  if (!c || !out) return PBW_ERR_ARG;
+  *out = c->stats;
+  return PBW_OK;
+}
+
+// sam.rs:37-58 format_sam, :83-107 header (QUAL: raw Phred bytes, sam.rs:52)
+char* pbw_format_sam(const pbw_genome* g, const pbw_read_batch* rd, const char* ids, const pbw_record_batch* recs,
+                     const char* cmd, const char* rg, uint64_t* len) {
+  std::string out = "@HD\tVN:1.6\tSO:unsorted\n";
+  for (size_t i = 0; i < g->names.size(); i++)
+    out += "@SQ\tSN:" + g->names[i] + "\tLN:" + std::to_string(g->lengths[i]) + "\n";
+  out += std::string("@PG\tID:probebwa\tPN:probebwa\tVN:0.1.0\tCL:") + (cmd ? cmd : "") + "\n";
+  std::string rg_id;
+  if (rg) {  // sam.rs:11-32
+    std::string spec(rg), hdr = "@RG";
+    size_t st = 0;
+    for (;;) {
+      size_t e = spec.find(',', st);
+      std::string pair = spec.substr(st, e == std::string::npos ? std::string::npos : e - st);
+      size_t colon = pair.find(':');
+      std::string tag = pair.substr(0, colon), val = colon == std::string::npos ? "" : pair.substr(colon + 1);
+      if (tag == "ID") rg_id = val;
+      hdr += "\t" + tag + ":" + val;
+      if (e == std::string::npos) break;
+      st = e + 1;
+    }
+    out += hdr + "\n";
+  }
+  static const char OPS[] = "MIDNSHP=X";
+  const char* idp = ids;
+  for (uint64_t i = 0; i < recs->n; i++) {
+    const pbw_record& r = recs->rec[i];
+    std::string id = idp ? std::string(idp) : ("r" + std::to_string(i));
+    if (idp) idp += strlen(idp) + 1;
+    out += id;
+    out += "\t" + std::to_string(r.flag);
+    out += "\t" + (r.contig < 0 ? std::string("*") : g->names[r.contig]);
+    out += "\t" + std::to_string(r.pos);
+    out += "\t" + std::to_string(r.mapq);
+    out += "\t";
+    if (r.cigar_n == 0) out += "*";
+    for (uint32_t k = 0; k < r.cigar_n; k++) {
+      uint32_t w = recs->cigar[r.cigar_off + k];
+      out += std::to_string(w >> 4);
+      out += OPS[w & 15];
+    }
+    out += "\t" + (r.mate_contig < 0 ? std::string("*") : g->names[r.mate_contig]);
+    out += "\t" + std::to_string(r.mate_pos);
+    out += "\t" + std::to_string(r.tlen);
+    out += "\t";
+    const uint8_t* sq = rd->seq + rd->seq_off[i];
+    uint64_t sl = rd->seq_off[i + 1] - rd->seq_off[i];
+    const uint8_t* q = rd->qual + rd->qual_off[i];
+    uint64_t ql = rd->qual_off[i + 1] - rd->qual_off[i];
+    if (r.seq_rc) {
+      for (uint64_t t = 0; t < sl; t++) {
+        uint8_t b = sq[sl - 1 - t];
+        out.push_back(b == 'A' ? 'T' : b == 'T' ? 'A' : b == 'C' ? 'G' : b == 'G' ? 'C' : 'N');
+      }
+      out += "\t";
+      for (uint64_t t = 0; t < ql; t++) out.push_back((char)q[ql - 1 - t]);
+    } else {
+      out.append((const char*)sq, sl);
+      out += "\t";
+      out.append((const char*)q, ql);
+    }
+    if (rg) out += "\tRG:Z:" + rg_id;
+    out += "\n";
+  }
+  char* t = (char*)malloc(out.size() + 1);
+  memcpy(t, out.data(), out.size());
+  t[out.size()] = 0;
+  if (len) *len = out.size();
+  return t;
+}
+void pbw_free_text(char* t) { free(t); }
+
+}  // extern "C"

@@ -1,0 +1,225 @@
+// Shared host/device definitions of the probebwa_amd HIP path.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pbw {
+
+// ---- constants of the reference algorithm ----
+constexpr int K = 15;                    // hashtable.rs:191 (k = 15)
+constexpr int STRIDE = 5;                // hashtable.rs:187 DEFAULT_STRIDE
+constexpr int MAX_COUNT = 300;           // hashtable.rs:51 DEFAULT_MAX_COUNT
+constexpr int REPEAT_SKIP = 64;          // seeds.rs:45
+constexpr int MERGE_DISTANCE = 50;       // candidates.rs:27
+constexpr int TOP_CANDIDATES = 20;       // single.rs:114
+constexpr int LEFT_PAD = 10;             // single.rs:109
+constexpr int MATCH_SCORE = 20;          // smith_waterman.rs:81
+constexpr int32_t NEG_INF = INT32_MIN / 2;  // smith_waterman.rs:173
+constexpr uint64_t CODE_SPACE = 1ull << 30;  // 4^15 15-mer codes
+constexpr int DIAG_BITS = 41;            // hit key: (group << 41) | (diag + 2^40)
+constexpr int64_t DIAG_BIAS = 1ll << 40;
+
+// ---- limits of this implementation ----
+constexpr int MAX_READ_LEN = 1024;       // longer reads are rejected with PBW_ERR_LIMIT
+constexpr int HIT_CAP = 1024;            // hits per read in the first seed pass (overflow -> exact rerun)
+constexpr int CLUSTER_LDS_CAP = 1024;    // hits per read sorted in LDS by one wave (else tier 2)
+constexpr int LN_INT_MAX = 4096;         // ln(k) table for phred_n_choose_m
+constexpr int GC_LUT_MAX = 1024;         // (gc, total) table for the GC-adjusted alphabet size
+
+struct DevGenome {
+  const uint64_t* g2b;    // 2-bit bases, 32 per word, LSB first (A0 C1 G2 T3), contigs back to back
+  const uint64_t* nmask;  // 1 bit per base: base was not A/C/G/T (decodes as 'N')
+  const uint64_t* coff;   // base offset of each contig in the arrays above (multiple of 64)
+  const uint64_t* clen;   // contig lengths
+  int32_t ncontigs;
+};
+
+struct DevIndex {
+  const uint32_t* koff;  // [2^30 + 1] bucket start by 15-mer code; bucket = [koff[c], koff[c+1])
+  const uint64_t* kpos;  // packed positions: bit 63 strand, 62..40 contig, 39..0 position
+};
+
+struct DevReads {
+  const uint8_t* seq;
+  const uint64_t* seq_off;
+  const uint8_t* qual;
+  const uint64_t* qual_off;
+  uint32_t n;
+};
+
+// Tables computed on the host with the same libm calls the reference makes,
+// so every quality-indexed FP64 term is the identical double on the device.
+struct DevLuts {
+  const double* lk_match;  // [256] ln(max(1-e_q, 1e-300)), e_q = 10^(-logadd(q,sub)/10)
+  const double* lk_mis;    // [256] ln(max(e_q/3, 1e-300))
+  const double* omq;       // [256] 1 - min(e_q, 1-1e-12)
+  const double* qratio;    // [256] q'/(1-q'), q' = min(e_q, 1-1e-12)
+  const double* nrp;       // [256] 10^(-q/10) (neighbour repeat probability)
+  const int32_t* mis_pen;  // [256] mismatch_penalty(q)
+  const double* ln_int;    // [LN_INT_MAX+1] ln(k)
+  const double* gc_lnb;    // [(GC_LUT_MAX+1)(GC_LUT_MAX+2)/2] ln(b(gc/total))
+  const double* gc_lnbm1;  // same layout: ln(max(b-1, 1e-6))
+  double ln_quarter;       // ln(0.25)
+  double ln_2g;            // ln(2 * genome_size)
+  double ln_30;            // ln(30)
+  double sub_phred;
+  double posterior_floor;  // exp(99 * DB)
+  double genome_size;
+};
+
+// One banded DP job (SmithWatermanAligner::align on one window).
+struct DpJob {
+  uint64_t ref_gb;    // global base index of ref_seg[0] (genome source), or byte offset (explicit source)
+  uint32_t read;      // read index in the batch
+  uint32_t n;         // ref_seg length
+  int32_t hint;       // ref_offset_hint
+  int32_t band;       // band half-width
+  uint8_t rc;         // query = reverse_complement(read), quals reversed
+  uint8_t explicit_ref;
+  uint16_t pad;
+  uint32_t cand;      // candidate slot (read * 20 + k) or job id
+};
+
+struct DpOut {
+  double log_lk;
+  uint32_t ref_off;   // ref_start_offset
+  int32_t score;
+  uint32_t cigar_n;   // number of runs (may exceed the cap: overflow)
+  uint32_t span;      // cigar_reference_span
+  uint64_t cells;
+};
+
+struct Cand {
+  uint64_t pos;       // candidate start (clamped >= 0)
+  uint32_t contig;
+  uint32_t strand_score;  // bit 31 strand, low bits = hit count
+};
+
+struct Counters {
+  unsigned long long lookups, positions, filtered, cells, overflow;
+};
+
+}  // namespace pbw
+
+// ---- kernel launchers (pbw_kernels.hip) ----
+namespace pbw {
+struct SeedArgs {
+  DevGenome G;
+  DevIndex X;
+  DevReads R;
+  const uint32_t* rlist;   // reads to process (nullptr = 0..n-1)
+  uint32_t nlist;
+  const uint64_t* hoff;    // per list entry: hit buffer offset
+  const uint32_t* hcap;    // per list entry: capacity (nullptr = HIT_CAP)
+  uint64_t* hits;
+  uint32_t* nhits;         // per read (indexed by read id)
+  uint8_t* rcode;          // per window repeat code, indexed by seq_off[read] + p
+  Counters* ctr;
+};
+void launch_seed(const SeedArgs& a, hipStream_t s);
+
+struct ClusterArgs {
+  DevReads R;
+  const uint64_t* hits;
+  const uint64_t* hoff;    // per read hit offset
+  const uint32_t* nhits;
+  Cand* cands;             // read * 20
+  uint8_t* ncand;
+  uint32_t* tier2;         // list of reads needing the block-level path
+  uint32_t* ntier2;
+  uint64_t* scratch;       // tier 2: per-block scratch (keys)
+  uint64_t scratch_per_block;
+};
+void launch_cluster(const ClusterArgs& a, hipStream_t s);
+void launch_cluster_tier2(const ClusterArgs& a, const uint32_t* list, uint32_t n, hipStream_t s);
+
+struct JobArgs {
+  DevGenome G;
+  DevReads R;
+  const Cand* cands;
+  const uint8_t* ncand;
+  int32_t right_pad;
+  int32_t band;
+  DpJob* jobs;             // read * 20 + k (valid for k < njobs[read])
+  uint8_t* njobs;
+};
+void launch_build_jobs(const JobArgs& a, hipStream_t s);
+
+struct DpArgs {
+  DevGenome G;
+  DevReads R;
+  DevLuts T;
+  const DpJob* jobs;
+  const uint32_t* jlist;   // job indices to run (nullptr = 0..njobs-1)
+  uint32_t njobs;
+  const uint8_t* job_valid;  // optional: per read count of valid slots (slot = read*20+k)
+  const uint8_t* ref_bytes;  // explicit-ref source
+  int32_t gap_open, gap_extend;
+  uint32_t cigar_cap;
+  uint32_t* cigar;         // job * cigar_cap
+  DpOut* out;
+  uint8_t* needs_generic;  // per job: 1 = shape not handled by the banded kernel
+  Counters* ctr;
+  uint32_t max_m, max_n;   // LDS sizing
+  // tiled kernel scratch
+  int32_t* rows;           // per wave: 3 * (max_n + 1)
+  uint64_t* tb;            // per wave: (max_m + 1) * tiles * 4
+  uint32_t tiles;          // ceil((max_n + 1) / 64)
+  uint32_t wpb;            // waves per block (banded kernel; set by the launcher)
+};
+void launch_dp_band(const DpArgs& a, hipStream_t s);
+void launch_dp_tiled(const DpArgs& a, uint32_t nwaves, hipStream_t s);
+
+struct MapqArgs {
+  DevGenome G;
+  DevReads R;
+  DevLuts T;
+  const DpJob* jobs;
+  const DpOut* out;
+  const uint32_t* cigar;
+  uint32_t cigar_cap;
+  const uint8_t* njobs;    // per read
+  const Cand* cands;
+  const uint8_t* rcode;
+  uint8_t* rec;            // pbw_record array (64 B each)
+  uint32_t* best;          // per read best job slot or UINT32_MAX
+};
+void launch_mapq(const MapqArgs& a, hipStream_t s);
+
+struct PairArgs {
+  DevGenome G;
+  DevReads R;              // 2P reads: read 2p = mate 1, read 2p+1 = mate 2 of pair p
+  DevLuts T;
+  const DpJob* jobs;       // single-end candidate stage (slot = read * 20 + k)
+  const DpOut* out;
+  const uint32_t* cigar;
+  uint32_t cigar_cap;
+  const uint8_t* njobs;
+  const Cand* cands;
+  const uint8_t* rcode;
+  DpJob* wjobs;            // cross-alignment jobs, slot = pair * 40 + s
+  uint8_t* wvalid;
+  uint8_t* slcand;         // shortlist candidate of each slot
+  const DpOut* wout;
+  const uint32_t* wcigar;
+  uint32_t wcap;
+  uint32_t* plan;
+  double mean, sd, ln_norm, sv_prior_log;  // frozen insert-size snapshot
+  uint64_t mean_usize, radius_base;        // mean.max(0) as usize, ceil(5 sd) as usize
+  const uint32_t* name_class;              // contig -> first contig with the same name
+  uint8_t* rec;            // 2P records
+  uint64_t* cig_src;       // per record CIGAR location (bit 63: cross-alignment pool)
+  double* obs;
+  uint8_t* has_obs;
+  uint32_t P;
+};
+void launch_pair_plan(const PairArgs& a, hipStream_t s);
+void launch_pair_finalize(const PairArgs& a, hipStream_t s);
+void gather_cigar2(const uint32_t* cig0, const uint32_t* cig1, const uint64_t* src, const uint32_t* off,
+                   const uint32_t* cnt, uint32_t n, uint32_t* out, hipStream_t s);
+
+void gather_cigar(const uint32_t* cig, const uint64_t* src, const uint32_t* off, const uint32_t* cnt, uint32_t n,
+                  uint32_t* out, hipStream_t s);
+void launch_fill_koff(uint32_t* koff, const uint32_t* keys, const uint32_t* kstart, uint64_t nkeys, uint32_t total,
+                      hipStream_t s);
+}  // namespace pbw

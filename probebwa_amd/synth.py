@@ -1,0 +1,230 @@
+"""Deterministic synthetic genomes and reads for the benchmark workloads.
+
+Generator: numpy PCG64 seeded per workload (recorded in each function), so a
+workload is reproducible byte for byte on any machine.
+
+Workloads (the reference publishes no dataset for its mapping path):
+  probe_se_2mbp      2,000,000 bp uniform chromosome + 50 copies of a 2 kb unit
+                     at 1% divergence; 20,000 x 120 bp reads, 1% substitutions,
+                     0.05% 1-3 bp indel events, qualities U{20..40}.
+  uneven_se_100mbp   10 contigs x 10 Mbp: uniform background, 5% segmental
+                     duplications (1-10 kb copies at 0.5-3% divergence),
+                     homopolymer runs of 5-15 bp every ~2 kb; 2M reads with
+                     L ~ U{80..150}, Q(pos) = 38 - 16 pos/(L-1) + N(0,3)
+                     clamped to [2,41], substitutions at the Phred rate.
+  pairs_2x150        pairs on the probe_se_2mbp genome: fragment ~ N(350,50)
+                     clipped to [200,600], FR orientation, 10% of pairs with
+                     one mate inside a repeat copy.
+"""
+import numpy as np
+
+BASES = np.frombuffer(b"ACGT", dtype=np.uint8)
+COMP = np.zeros(256, np.uint8)
+COMP[:] = ord("N")
+for a, b in zip(b"ACGT", b"TGCA"):
+    COMP[a] = b
+
+
+def revcomp(a):
+    return COMP[a[::-1]]
+
+
+def random_bases(rng, n):
+    return BASES[rng.integers(0, 4, n)]
+
+
+def mutate(rng, seq, rate):
+    """Substitute each base with probability `rate` by a different base."""
+    s = seq.copy()
+    hit = np.nonzero(rng.random(len(s)) < rate)[0]
+    if len(hit):
+        codes = np.searchsorted(BASES, s[hit])
+        s[hit] = BASES[(codes + rng.integers(1, 4, len(hit))) % 4]
+    return s
+
+
+def genome_probe_se_2mbp(seed=1):
+    rng = np.random.default_rng(np.random.PCG64(seed))
+    bg = random_bases(rng, 2_000_000)
+    unit = random_bases(rng, 2000)
+    pieces, last = [], 0
+    for p in np.sort(rng.choice(2_000_000, 50, replace=False)):
+        pieces.append(bg[last:p])
+        pieces.append(mutate(rng, unit, 0.01))
+        last = p
+    pieces.append(bg[last:])
+    return [("chr1", np.concatenate(pieces))]
+
+
+def genome_uneven_100mbp(seed=2, n_contigs=10, contig_len=10_000_000):
+    rng = np.random.default_rng(np.random.PCG64(seed))
+    contigs = []
+    for c in range(n_contigs):
+        contigs.append(random_bases(rng, contig_len))
+    total = n_contigs * contig_len
+    # segmental duplications: copy 1-10 kb segments (0.5-3% divergence) until 5% of the genome
+    dup = 0
+    while dup < total * 0.05:
+        ln = int(rng.integers(1000, 10001))
+        sc, dc = int(rng.integers(0, n_contigs)), int(rng.integers(0, n_contigs))
+        sp, dp = int(rng.integers(0, contig_len - ln)), int(rng.integers(0, contig_len - ln))
+        contigs[dc][dp:dp + ln] = mutate(rng, contigs[sc][sp:sp + ln], float(rng.uniform(0.005, 0.03)))
+        dup += ln
+    # homopolymer runs of 5-15 bp every ~2 kb
+    for c in range(n_contigs):
+        pos = 0
+        while True:
+            pos += int(rng.integers(1000, 3001))
+            ln = int(rng.integers(5, 16))
+            if pos + ln >= contig_len:
+                break
+            contigs[c][pos:pos + ln] = BASES[rng.integers(0, 4)]
+    return [("chr%d" % (c + 1), contigs[c]) for c in range(n_contigs)]
+
+
+def _flat(seqs, quals):
+    so = np.zeros(len(seqs) + 1, np.uint64)
+    qo = np.zeros(len(quals) + 1, np.uint64)
+    so[1:] = np.cumsum([len(s) for s in seqs])
+    qo[1:] = np.cumsum([len(q) for q in quals])
+    seq = np.concatenate(seqs + [np.zeros(1, np.uint8)]).astype(np.uint8)
+    qual = np.concatenate(quals + [np.zeros(1, np.uint8)]).astype(np.uint8)
+    return seq, so, qual, qo
+
+
+def _indels(rng, s, rate):
+    """0.05%-style per-base indel events of length U{1,2,3}, ins/del 50/50."""
+    out, i = [], 0
+    ev = np.nonzero(rng.random(len(s)) < rate)[0]
+    for p in ev:
+        out.append(s[i:p])
+        ln = int(rng.integers(1, 4))
+        if rng.random() < 0.5:
+            out.append(random_bases(rng, ln))
+            i = p
+        else:
+            i = min(len(s), p + ln)
+    out.append(s[i:])
+    return np.concatenate(out) if out else s
+
+
+def reads_probe_se(contigs, n=20_000, L=120, seed=11):
+    """20,000 x 120 bp reads: uniform start, strand 50/50, 1% subs, 0.05% indels, Q ~ U{20..40}."""
+    rng = np.random.default_rng(np.random.PCG64(seed))
+    g = contigs[0][1]
+    seqs, quals, truth = [], [], []
+    for _ in range(n):
+        start = int(rng.integers(0, len(g) - L - 4))
+        s = _indels(rng, g[start:start + L + 3], 0.0005)[:L]
+        s = mutate(rng, s, 0.01)
+        rev = bool(rng.random() < 0.5)
+        if rev:
+            s = revcomp(s)
+        seqs.append(s)
+        quals.append(rng.integers(20, 41, len(s)).astype(np.uint8))
+        truth.append((0, start, rev))
+    return _flat(seqs, quals), truth
+
+
+def reads_uneven_se(contigs, n=2_000_000, seed=12, lmin=80, lmax=150):
+    """Uneven single-end reads: L ~ U{80..150}, decaying Phred qualities, Phred-rate errors."""
+    rng = np.random.default_rng(np.random.PCG64(seed))
+    lens = np.array([len(c[1]) for c in contigs], np.int64)
+    cum = np.cumsum(lens)
+    L = rng.integers(lmin, lmax + 1, n)
+    gpos = (rng.random(n) * (cum[-1] - lmax - 1)).astype(np.int64)
+    cid = np.searchsorted(cum, gpos, side="right")
+    start = gpos - np.concatenate([[0], cum[:-1]])[cid]
+    rev = rng.random(n) < 0.5
+    seqs, quals = [], []
+    for i in range(n):
+        c = contigs[cid[i]][1]
+        st = int(min(start[i], len(c) - L[i]))
+        s = c[st:st + L[i]]
+        pos = np.arange(L[i])
+        q = np.rint(38.0 - 16.0 * pos / (L[i] - 1) + rng.normal(0.0, 3.0, L[i]))
+        q = np.clip(q, 2, 41).astype(np.uint8)
+        err = rng.random(L[i]) < np.power(10.0, -q / 10.0)
+        s = s.copy()
+        if err.any():
+            idx = np.nonzero(err)[0]
+            codes = np.searchsorted(BASES, s[idx])
+            s[idx] = BASES[(codes + rng.integers(1, 4, len(idx))) % 4]
+        if rev[i]:
+            s = revcomp(s)
+            q = q[::-1].copy()
+        seqs.append(s)
+        quals.append(q)
+    return _flat(seqs, quals)
+
+
+def reads_pairs(contigs, n=10_000, L=150, seed=13, repeat_spans=()):
+    """FR pairs: fragment ~ N(350, 50) clipped to [200, 600]; 10% anchored in a repeat span."""
+    rng = np.random.default_rng(np.random.PCG64(seed))
+    g = contigs[0][1]
+    s1, q1, s2, q2 = [], [], [], []
+    for i in range(n):
+        frag = int(np.clip(np.rint(rng.normal(350, 50)), 200, 600))
+        if repeat_spans and rng.random() < 0.1:
+            a, b = repeat_spans[int(rng.integers(0, len(repeat_spans)))]
+            start = int(np.clip(a + rng.integers(0, max(1, b - a - L)) - frag + L, 0, len(g) - frag - 1))
+        else:
+            start = int(rng.integers(0, len(g) - frag - 1))
+        m1 = mutate(rng, g[start:start + L], 0.01)
+        m2 = revcomp(mutate(rng, g[start + frag - L:start + frag], 0.01))
+        if rng.random() < 0.5:
+            m1, m2 = m2, m1
+        s1.append(m1)
+        s2.append(m2)
+        q1.append(rng.integers(20, 41, L).astype(np.uint8))
+        q2.append(rng.integers(20, 41, L).astype(np.uint8))
+    return _flat(s1, q1), _flat(s2, q2)
+
+
+def small_genome(seed, n_contigs=2, length=60_000, repeat_copies=6, unit_len=700, n_run=True):
+    """Small multi-contig genome with repeats, homopolymers and an N run (parity tests)."""
+    rng = np.random.default_rng(np.random.PCG64(seed))
+    unit = random_bases(rng, unit_len)
+    out = []
+    for c in range(n_contigs):
+        s = random_bases(rng, length)
+        for _ in range(repeat_copies):
+            p = int(rng.integers(0, length - unit_len))
+            s[p:p + unit_len] = mutate(rng, unit, 0.01)
+        for _ in range(length // 3000):
+            p = int(rng.integers(0, length - 20))
+            s[p:p + int(rng.integers(5, 16))] = BASES[rng.integers(0, 4)]
+        if n_run:
+            p = int(rng.integers(100, length - 200))
+            s[p:p + int(rng.integers(5, 60))] = ord("N")
+        out.append(("ctg%d" % c, s))
+    return out
+
+
+def sample_reads(contigs, n, seed, lmin=30, lmax=200, sub=0.02, indel=0.002, n_rate=0.001, qmin=2, qmax=41):
+    """Mixed reads for parity: uneven lengths, both strands, subs/indels, Ns, random-sequence reads."""
+    rng = np.random.default_rng(np.random.PCG64(seed))
+    seqs, quals = [], []
+    for i in range(n):
+        L = int(rng.integers(lmin, lmax + 1))
+        kind = rng.random()
+        if kind < 0.05:
+            s = random_bases(rng, L)
+        else:
+            c = contigs[int(rng.integers(0, len(contigs)))][1]
+            if kind < 0.1:  # near contig ends
+                st = int(rng.integers(0, 40)) if rng.random() < 0.5 else max(0, len(c) - L - int(rng.integers(-30, 20)))
+            else:
+                st = int(rng.integers(0, max(1, len(c) - L)))
+            s = _indels(rng, c[st:st + L + 4].copy(), indel)[:L]
+            s = mutate(rng, s, sub)
+            nn = np.nonzero(rng.random(len(s)) < n_rate)[0]
+            s[nn] = ord("N")
+            if rng.random() < 0.5:
+                s = revcomp(s)
+        q = rng.integers(qmin, qmax + 1, len(s)).astype(np.uint8)
+        if rng.random() < 0.02 and len(q) > 3:
+            q = q[: len(q) - int(rng.integers(1, 3))]  # quality track shorter than the sequence
+        seqs.append(s)
+        quals.append(q)
+    return _flat(seqs, quals)

@@ -1,0 +1,309 @@
+"""HIP path vs the CPU oracle: bit-exact SAM fields, posteriors within 1e-12.
+
+Every test here runs the HIP kernels through the C ABI (libprobebwa_amd.so)
+and checks the result against the oracle on the same seeded inputs.
+"""
+import numpy as np
+import pytest
+
+import probebwa_amd as P
+from oracle import pbw_oracle as O
+from probebwa_amd import synth
+from refcases import ECOLI_READ, ECOLI_WINDOW, Xorshift64, homopolymer_battery, homopolymer_scenario, q40, revcomp
+
+pytestmark = pytest.mark.gpu
+
+POSTERIOR_RTOL = 1e-12  # north-star tolerance for raw posteriors (FP64)
+
+
+def pair_of_genomes(contigs):
+    go, gp = O.Genome(), P.Genome()
+    for name, seq in contigs:
+        s = bytes(seq)
+        go.add_contig(name, s)
+        gp.add_contig(name, s)
+    return go, gp
+
+
+def compare_records(ro, rp, reads=None, go=None, gp=None, label=""):
+    """Field-by-field comparison; returns number of mismatching records (asserted 0)."""
+    assert len(ro) == len(rp)
+    bad = []
+    for i in range(len(ro)):
+        a, b = ro.rec[i], rp.rec[i]
+        same = (a["pos"] == b["pos"] and a["contig"] == b["contig"] and a["flag"] == b["flag"]
+                and a["mapq"] == b["mapq"] and a["strand"] == b["strand"] and a["seq_rc"] == b["seq_rc"]
+                and a["mate_pos"] == b["mate_pos"] and a["mate_contig"] == b["mate_contig"] and a["tlen"] == b["tlen"]
+                and ro.cigar_words(i) == rp.cigar_words(i) and a["log_lk"] == b["log_lk"])
+        pa, pb = float(a["posterior"]), float(b["posterior"])
+        if pa != pb and abs(pa - pb) > POSTERIOR_RTOL * max(abs(pa), abs(pb)):
+            same = False
+        if not same:
+            bad.append(i)
+    if bad:
+        i = bad[0]
+        msg = "%s: %d/%d records differ; first #%d oracle=%s %s gpu=%s %s" % (
+            label, len(bad), len(ro), i, ro.rec[i], ro.cigar_str(i), rp.rec[i], rp.cigar_str(i))
+        raise AssertionError(msg)
+    if reads is not None:
+        so = O.format_sam(go, O.ReadBatch.from_arrays(reads.seq, reads.seq_off, reads.qual, reads.qual_off, reads.ids), ro)
+        sp = P.format_sam(gp, reads, rp)
+        assert so == sp, label + ": SAM text differs"
+
+
+def both_reads(seq, so, qual, qo):
+    return (O.ReadBatch.from_arrays(seq, so, qual, qo), P.ReadBatch.from_arrays(seq, so, qual, qo))
+
+
+# ---------------------------------------------------------------------------
+# reference known answers through the HIP path (tests/mapping.rs)
+# ---------------------------------------------------------------------------
+def se_hip(seq, reads, quals=None, name="chr1"):
+    g = P.Genome().add_contig(name, seq)
+    ctx = P.Context(g)
+    quals = quals or [q40(r) for r in reads]
+    return ctx.map_se(P.ReadBatch(reads, quals))
+
+
+def test_hip_forward_and_reverse_strand_mapping():  # tests/mapping.rs:67-103
+    seq = Xorshift64(42).gen_seq(400)
+    r = se_hip(seq, [seq[40:80], revcomp(seq[200:240])])
+    assert (r.rec[0]["pos"], r.rec[0]["strand"], r.cigar_str(0), r.rec[0]["mapq"]) == (41, 0, "40M", 80)
+    assert (r.rec[1]["pos"], r.rec[1]["strand"], r.cigar_str(1), r.rec[1]["mapq"]) == (201, 1, "40M", 80)
+
+
+def test_hip_mismatch_deletion_and_unmapped():  # tests/mapping.rs:105-160
+    seq = Xorshift64(7).gen_seq(400)
+    base = seq[40:80]
+    mm = list(base)
+    mm[10] = "C" if mm[10] == "A" else "A"
+    r = se_hip(seq, ["".join(mm), base[:20] + base[22:], "T" * 40])
+    assert (r.rec[0]["pos"], r.cigar_str(0), r.rec[0]["mapq"]) == (41, "40M", 80)
+    ops = [(w & 15, w >> 4) for w in r.cigar_words(1)]
+    assert r.rec[1]["pos"] == 41 and sum(n for o, n in ops if o == 0) == 38 and sum(n for o, n in ops if o == 2) == 2
+    assert r.rec[1]["mapq"] > 0
+    assert r.rec[2]["pos"] == 0 and r.rec[2]["mapq"] == 0 and r.cigar_str(2) == "40S" and r.rec[2]["flag"] == 4
+
+
+def test_hip_large_deletion_and_large_positions():  # tests/mapping.rs:171-193, :240-258
+    seq = Xorshift64(555).gen_seq(500)
+    w = seq[100:160]
+    r = se_hip(seq, [w[:25] + w[50:]])
+    assert r.rec[0]["pos"] == 101 and sum(w >> 4 for w in r.cigar_words(0) if w & 15 == 2) == 25
+    seq = Xorshift64(2024).gen_seq(80000)
+    r = se_hip(seq, [seq[70000:70050]])
+    assert r.rec[0]["pos"] == 70001 and r.rec[0]["mapq"] == 99
+
+
+def test_hip_n_run_round_trip():  # tests/mapping.rs:304-334
+    rng = Xorshift64(7)
+    left, right = rng.gen_seq(100), rng.gen_seq(100)
+    seq = left + "N" * 20 + right
+    r = se_hip(seq, [seq[105:135], seq[150:190]], name="chrN")
+    assert r.rec[0]["pos"] == 106
+    assert r.rec[1]["pos"] == 151 and r.rec[1]["mapq"] == 80
+
+
+def test_hip_divergent_region_does_not_invent_spurious_indels():  # tests/integration.rs:130-153
+    g = P.Genome().add_contig("x", "ACGTACGTACGTACGTACGT")
+    ctx = P.Context(g)
+    (score, cig, off), = ctx.align_batch([(ECOLI_READ, bytes([30] * 100), ECOLI_WINDOW, len(ECOLI_WINDOW), 513)])
+    assert off == 513 and all(w & 15 == 0 for w in cig)
+    s, c, o, _ = O.SmithWatermanAligner(len(ECOLI_WINDOW)).align(ECOLI_READ, bytes([30] * 100), ECOLI_WINDOW, 513)
+    assert (score, cig, off) == (s, c, o)
+
+
+def test_hip_homopolymer_battery_matches_oracle():  # examples/tune_homopolymer.rs:87-146
+    correct = 0
+    for run_len, del_len, n_err, seed, vs in homopolymer_battery():
+        genome, read, true_pos = homopolymer_scenario(vs, run_len, del_len, n_err)
+        g = P.Genome().add_contig("chr1", genome)
+        r = P.Context(g).map_se(P.ReadBatch([read], [bytes([10] * len(read))]))
+        dels = sum(w >> 4 for w in r.cigar_words(0) if w & 15 == 2)
+        if abs(int(r.rec[0]["pos"]) - true_pos) <= 2 and dels == del_len:
+            correct += 1
+    assert correct == 285
+
+
+# ---------------------------------------------------------------------------
+# kernel-level parity
+# ---------------------------------------------------------------------------
+def test_index_buckets_match_oracle():
+    contigs = synth.small_genome(5, n_contigs=3, length=30000)
+    go, gp = pair_of_genomes(contigs)
+    ht = O.HashTable(go)
+    ctx = P.Context(gp)
+    assert ctx.total_entries == ht.total_entries
+    rng = np.random.default_rng(3)
+    seqs = [bytes(c[1]) for c in contigs]
+    checked = 0
+    for _ in range(3000):
+        s = seqs[rng.integers(0, len(seqs))]
+        p = int(rng.integers(0, len(s) - 15))
+        kmer = s[p:p + 15]
+        if rng.random() < 0.3:
+            kmer = bytes(synth.random_bases(rng, 15))
+        if b"N" in kmer:
+            continue
+        assert list(ctx.lookup_code(P.code_of(kmer))) == ht.lookup(kmer)
+        checked += 1
+    assert checked > 2000
+
+
+def random_align_jobs(rng, n, wide=False):
+    jobs = []
+    for _ in range(n):
+        m = int(rng.integers(1, 60 if not wide else 200))
+        nref = int(rng.integers(1, 90 if not wide else 600))
+        read = bytes(synth.random_bases(rng, m))
+        ref = bytearray(synth.random_bases(rng, nref))
+        if rng.random() < 0.6 and nref > 5:  # plant the read (with edits) in the ref
+            p = int(rng.integers(0, nref))
+            src = synth.mutate(rng, np.frombuffer(read, np.uint8), 0.05).tobytes()
+            ref[p:p + m] = src[: max(0, min(m, nref - p))]
+            ref = ref[:nref]
+        if rng.random() < 0.1:
+            ref[int(rng.integers(0, len(ref)))] = ord("N")
+        if rng.random() < 0.1:
+            read = read[: max(0, m - 1)] + b"N"
+        if rng.random() < 0.2:  # homopolymers
+            p = int(rng.integers(0, len(ref)))
+            ref[p:p + 6] = b"AAAAAA"
+        qual = bytes(rng.integers(0, 45, len(read) - (1 if rng.random() < 0.1 and len(read) > 1 else 0)).astype(np.uint8))
+        band = int(rng.integers(0, 32)) if not wide else int(rng.choice([len(ref), 40, 100]))
+        hint = int(rng.integers(-40, 60)) if rng.random() < 0.9 else int(rng.integers(-200, 200))
+        jobs.append((read, qual, bytes(ref), band, hint))
+    return jobs
+
+
+@pytest.mark.parametrize("wide", [False, True])
+def test_align_batch_matches_oracle(wide):
+    rng = np.random.default_rng(17 + wide)
+    jobs = random_align_jobs(rng, 3000 if not wide else 600, wide)
+    g = P.Genome().add_contig("x", "ACGT" * 10)
+    ctx = P.Context(g)
+    for gop, gep in ((40, 3), (10, 20)):
+        got = ctx.align_batch(jobs, gop, gep)
+        for i, (read, qual, ref, band, hint) in enumerate(jobs):
+            s, c, o, _ = O.SmithWatermanAligner(band, gop, gep).align(read, qual, ref, hint)
+            assert got[i] == (s, c, o), "job %d (m=%d n=%d band=%d hint=%d)" % (i, len(read), len(ref), band, hint)
+
+
+# ---------------------------------------------------------------------------
+# whole single-end pipeline parity
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("seed,n_contigs", [(21, 1), (22, 3), (23, 7)])
+def test_map_se_matches_oracle(seed, n_contigs):
+    contigs = synth.small_genome(seed, n_contigs=n_contigs, length=40000)
+    go, gp = pair_of_genomes(contigs)
+    seq, so, qual, qo = synth.sample_reads(contigs, 1500, seed + 100)
+    ro_b, rp_b = both_reads(seq, so, qual, qo)
+    ro = O.Mapper(go, O.HashTable(go)).map_se(ro_b)
+    rp = P.Context(gp).map_se(rp_b)
+    compare_records(ro, rp, rp_b, go, gp, "seed %d" % seed)
+    assert (rp.rec["pos"] > 0).mean() > 0.8
+
+
+def test_map_se_small_batches_and_options():
+    """Output must not depend on the device batch size; non-default penalties."""
+    contigs = synth.small_genome(31, n_contigs=2, length=30000)
+    go, gp = pair_of_genomes(contigs)
+    seq, so, qual, qo = synth.sample_reads(contigs, 700, 131)
+    ro_b, rp_b = both_reads(seq, so, qual, qo)
+    opts = dict(substitution_rate=0.01, gap_open_phred=30, gap_extend_phred=6)
+    ro = O.Mapper(go, O.HashTable(go), **opts).map_se(ro_b)
+    rp = P.Context(gp, batch_reads=97, **opts).map_se(rp_b)
+    compare_records(ro, rp, label="batch 97")
+
+
+def test_map_se_repeat_heavy_reads():
+    """Hyper-repetitive genome: buckets at the 300 cap, tier-2 clustering, hit overflow."""
+    rng = np.random.default_rng(41)
+    unit = synth.random_bases(rng, 37)
+    seqs = []
+    s = np.concatenate([synth.mutate(rng, unit, 0.03) for _ in range(1500)])
+    seqs.append(("rep", s))
+    seqs.append(("uniq", synth.random_bases(rng, 20000)))
+    go, gp = pair_of_genomes(seqs)
+    seq, so, qual, qo = synth.sample_reads(seqs, 200, 141, lmin=60, lmax=180)
+    ro_b, rp_b = both_reads(seq, so, qual, qo)
+    ro = O.Mapper(go, O.HashTable(go)).map_se(ro_b)
+    ctx = P.Context(gp)
+    rp = ctx.map_se(rp_b)
+    compare_records(ro, rp, label="repeats")
+    st = ctx.stats()
+    assert st["tier2_cluster_reads"] > 0 or st["seed_overflow_reads"] > 0
+
+
+def test_map_se_edge_reads():
+    """Empty reads, reads shorter than k, all-N reads, quality tracks of other lengths."""
+    contigs = synth.small_genome(51, n_contigs=1, length=20000)
+    go, gp = pair_of_genomes(contigs)
+    g = bytes(contigs[0][1])
+    reads = [b"", b"ACGT", g[100:114], g[100:115], b"N" * 50, g[5000:5100], g[:60], g[-60:], g[300:420]]
+    quals = [b"", b"\x05\x05", bytes([30] * 14), bytes([30] * 15), bytes([2] * 50), bytes([40] * 90), bytes([20] * 60),
+             bytes([35] * 70), bytes([11] * 120)]
+    ro = O.Mapper(go, O.HashTable(go)).map_se(O.ReadBatch(reads, quals))
+    rp = P.Context(gp).map_se(P.ReadBatch(reads, quals))
+    compare_records(ro, rp, label="edge")
+
+
+# ---------------------------------------------------------------------------
+# paired-end (paired.rs, lib.rs:145-185)
+# ---------------------------------------------------------------------------
+def pe_hip(seq, m1, m2, q1=None, q2=None):
+    g = P.Genome().add_contig("chr1", seq)
+    ctx = P.Context(g)
+    q1 = q1 or [q40(r) for r in m1]
+    q2 = q2 or [q40(r) for r in m2]
+    return ctx.map_pe(P.ReadBatch(m1, q1), P.ReadBatch(m2, q2))
+
+
+def test_hip_paired_end_proper_and_improper():  # tests/mapping.rs:195-233
+    seq = Xorshift64(99).gen_seq(400)
+    r = pe_hip(seq, [seq[40:80]], [revcomp(seq[200:240])])
+    a, b = r.rec[0], r.rec[1]
+    assert a["flag"] & 0x40 and b["flag"] & 0x80 and a["flag"] & 2 and b["flag"] & 2
+    assert (a["strand"], b["strand"], a["tlen"], b["tlen"]) == (0, 1, 200, -200)
+    r = pe_hip(seq, [seq[40:80]], [seq[200:240]])
+    assert not (r.rec[0]["flag"] & 2) and not (r.rec[1]["flag"] & 2)
+
+
+def test_hip_mate_rescue_recovers_unseedable_mate():  # tests/mapping.rs:267-302
+    seq = Xorshift64(2468).gen_seq(400)
+    m2 = list(revcomp(seq[200:240]))
+    for p in range(0, len(m2), 6):
+        m2[p] = next(b for b in "ACGT" if b != m2[p])
+    m2 = "".join(m2)
+    r = pe_hip(seq, [seq[40:80]], [m2])
+    assert r.rec[0]["pos"] == 41 and r.rec[1]["strand"] == 1 and abs(int(r.rec[1]["pos"]) - 201) <= 5
+
+
+def test_hip_paired_disambiguates_repeat_copy_via_anchor():  # tests/mapping.rs:421-507
+    rng = Xorshift64(2026)
+    unique_a, spacer, unit, filler = rng.gen_seq(100), rng.gen_seq(110), rng.gen_seq(100), rng.gen_seq(10000)
+    tc = list(unit)
+    tc[20] = {"A": "C", "C": "A", "G": "T"}.get(tc[20], "G")
+    genome = unique_a + spacer + "".join(tc) + filler + unit
+    q2 = [35] * 40
+    q2[19] = 10
+    r = pe_hip(genome, [unique_a[:40]], [revcomp(unit[:40])], [bytes([35] * 40)], [bytes(q2)])
+    assert r.rec[0]["pos"] == 1 and r.rec[1]["pos"] == 211 and r.rec[1]["strand"] == 1
+    assert (r.rec[0]["flag"] & 2) and (r.rec[1]["flag"] & 2) and r.rec[1]["mapq"] > 0
+
+
+@pytest.mark.parametrize("seed,n_pairs", [(61, 120), (62, 700)])
+def test_map_pe_matches_oracle(seed, n_pairs):
+    contigs = synth.small_genome(seed, n_contigs=2, length=40000, repeat_copies=8)
+    go, gp = pair_of_genomes(contigs)
+    (a, b) = synth.reads_pairs(contigs, n=n_pairs, L=100, seed=seed + 1,
+                               repeat_spans=())
+    o1, p1 = both_reads(*a)
+    o2, p2 = both_reads(*b)
+    m = O.Mapper(go, O.HashTable(go))
+    ro = m.map_pe(o1, o2)
+    ctx = P.Context(gp)
+    rp = ctx.map_pe(p1, p2)
+    compare_records(ro, rp, label="pairs seed %d" % seed)
+    assert m.insert_model[2] == ctx.insert_model[2]
+    assert m.insert_model[0] == ctx.insert_model[0] and m.insert_model[1] == ctx.insert_model[1]
