@@ -120,7 +120,9 @@ typedef struct {
   uint64_t seed_overflow_reads;
   uint64_t tier2_cluster_reads;
   uint64_t generic_dp_jobs;
-  double ms_seed, ms_cluster, ms_dp, ms_mapq, ms_pair, ms_total;
+  double ms_seed, ms_cluster, ms_dp, ms_mapq, ms_pair, ms_total;  /* host wall time per stage */
+  double ms_k_seed, ms_k_dp;  /* HIP-event time of the seed and banded-DP kernels */
+  uint64_t n_k_seed, n_k_dp;  /* launches timed */
 } pbw_stats;
 
 /* ---- genome: GenomeIndexBuilder (src/index/genome.rs:101-147) ---- */
@@ -146,6 +148,11 @@ const char* pbw_global_error(void);
  *      (src/lib.rs:84-104, SingleEndMapper::map src/mapper/single.rs:164-207);
  *      records in input order ---- */
 int pbw_map_se(pbw_ctx* ctx, const pbw_read_batch* reads, pbw_record_batch* out);
+
+/* ---- same as pbw_map_se for reads already resident in device memory (HBM):
+ *      seq/qual are device pointers, seq_off/qual_off are host arrays.  No
+ *      counterpart in the reference; for callers that keep reads on the GPU. ---- */
+int pbw_map_se_resident(pbw_ctx* ctx, const pbw_read_batch* reads, pbw_record_batch* out);
 
 /* ---- paired-end: map_paired (src/lib.rs:145-185) incl. the 500-pair serial
  *      warm-up and 2000-pair frozen-snapshot batches; records interleaved

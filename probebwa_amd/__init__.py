@@ -44,7 +44,8 @@ CIGAR_CHARS = "MIDNSHP=X"
 EXPORTED_SYMBOLS = (
     "pbw_genome_create", "pbw_genome_add_contig", "pbw_genome_add_fasta", "pbw_genome_n_contigs",
     "pbw_genome_contig_name", "pbw_genome_contig_length", "pbw_genome_destroy", "pbw_default_options",
-    "pbw_ctx_create", "pbw_ctx_destroy", "pbw_last_error", "pbw_global_error", "pbw_map_se", "pbw_map_pe",
+    "pbw_ctx_create", "pbw_ctx_destroy", "pbw_last_error", "pbw_global_error", "pbw_map_se", "pbw_map_se_resident",
+    "pbw_map_pe",
     "pbw_align_batch", "pbw_index_lookup", "pbw_index_total_entries", "pbw_records_release", "pbw_get_stats",
     "pbw_format_sam", "pbw_free_text",
 )
@@ -99,7 +100,9 @@ class Stats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("reads", "lookups", "positions", "filtered_hits", "dp_jobs", "dp_cells",
                                            "wide_jobs", "wide_cells", "seed_overflow_reads", "tier2_cluster_reads",
                                            "generic_dp_jobs")] + \
-               [(n, C.c_double) for n in ("ms_seed", "ms_cluster", "ms_dp", "ms_mapq", "ms_pair", "ms_total")]
+               [(n, C.c_double) for n in ("ms_seed", "ms_cluster", "ms_dp", "ms_mapq", "ms_pair", "ms_total",
+                                           "ms_k_seed", "ms_k_dp")] + \
+               [(n, C.c_uint64) for n in ("n_k_seed", "n_k_dp")]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -135,6 +138,7 @@ def lib():
             "pbw_last_error": (C.c_char_p, [vp]),
             "pbw_global_error": (C.c_char_p, []),
             "pbw_map_se": (i32, [vp, C.POINTER(_ReadBatch), C.POINTER(_RecordBatch)]),
+            "pbw_map_se_resident": (i32, [vp, C.POINTER(_ReadBatch), C.POINTER(_RecordBatch)]),
             "pbw_map_pe": (i32, [vp, C.POINTER(_ReadBatch), C.POINTER(_ReadBatch), C.POINTER(InsertModel),
                                  C.POINTER(_RecordBatch)]),
             "pbw_align_batch": (i32, [vp, u32, u32, C.POINTER(_AlignJob), u64, C.POINTER(_AlignResult), vp, u64]),
@@ -284,6 +288,16 @@ class Context:
         out = _RecordBatch()
         rs = reads.cstruct()
         _check(lib().pbw_map_se(self.h, C.byref(rs), C.byref(out)), self.h)
+        return _take(out)
+
+    def map_se_resident(self, seq_ptr, seq_off, qual_ptr, qual_off):
+        """Single-end map of reads resident in device memory (device pointers
+        seq_ptr/qual_ptr, host offset arrays)."""
+        so = np.ascontiguousarray(seq_off, dtype=np.uint64)
+        qo = np.ascontiguousarray(qual_off, dtype=np.uint64)
+        rs = _ReadBatch(len(so) - 1, seq_ptr, so.ctypes.data, qual_ptr, qo.ctypes.data)
+        out = _RecordBatch()
+        _check(lib().pbw_map_se_resident(self.h, C.byref(rs), C.byref(out)), self.h)
         return _take(out)
 
     def map_pe(self, r1, r2, mean=None, sd=None, n_observed=0):

@@ -247,6 +247,7 @@ struct pbw_ctx {
   pbw_options opt;
   int device;
   hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   std::string error;
   pbw_stats stats{};
   // genome + index
@@ -293,6 +294,7 @@ static void ctx_build(pbw_ctx* c) {
   const pbw_genome* g = c->genome;
   HIP_CHECK(hipSetDevice(c->device));
   HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (auto& e : c->ev) HIP_CHECK(hipEventCreate(&e));
   hipStream_t s = c->stream;
   if (g->lengths.size() >= (1u << 22)) throw PbwError{PBW_ERR_LIMIT, "more than 2^22 contigs"};
   for (uint64_t L : g->lengths)
@@ -472,7 +474,8 @@ static void batch_upload(pbw_ctx* c, Batch& b, const std::vector<const uint8_t*>
 }
 
 // Uploads a contiguous slice [b0, b1) of a read batch.
-static void batch_upload_slice(pbw_ctx* c, Batch& b, const pbw_read_batch* in, uint64_t b0, uint64_t b1) {
+static void batch_upload_slice(pbw_ctx* c, Batch& b, const pbw_read_batch* in, uint64_t b0, uint64_t b1,
+                               bool resident) {
   hipStream_t s = c->stream;
   const uint32_t B = (uint32_t)(b1 - b0);
   b.B = B;
@@ -489,12 +492,17 @@ static void batch_upload_slice(pbw_ctx* c, Batch& b, const pbw_read_batch* in, u
       throw PbwError{PBW_ERR_LIMIT, "read longer than " + std::to_string(MAX_READ_LEN) + " bases"};
     b.max_m = std::max<uint32_t>(b.max_m, (uint32_t)std::max(L, ql));
   }
-  c->d_seq.upload(in->seq + in->seq_off[b0], b.so[B], s);
-  c->d_qual.upload(in->qual + in->qual_off[b0], b.qo[B], s);
   c->d_seq_off.upload(b.so.data(), B + 1, s);
   c->d_qual_off.upload(b.qo.data(), B + 1, s);
-  b.R = DevReads{c->d_seq.p, c->d_seq_off.p, c->d_qual.p, c->d_qual_off.p, B};
+  if (resident) {  // bases and quals already in device memory
+    b.R = DevReads{in->seq + in->seq_off[b0], c->d_seq_off.p, in->qual + in->qual_off[b0], c->d_qual_off.p, B};
+  } else {
+    c->d_seq.upload(in->seq + in->seq_off[b0], b.so[B], s);
+    c->d_qual.upload(in->qual + in->qual_off[b0], b.qo[B], s);
+    b.R = DevReads{c->d_seq.p, c->d_seq_off.p, c->d_qual.p, c->d_qual_off.p, B};
+  }
   c->d_rcode.ensure(b.so[B] + 1);
+  HIP_CHECK(hipStreamSynchronize(s));
 }
 
 // map_candidates (single.rs:86-162) for every read of the batch: seed,
@@ -522,11 +530,19 @@ static void batch_candidates(pbw_ctx* c, Batch& b) {
   sa.nhits = c->d_nhits.p;
   sa.rcode = c->d_rcode.p;
   sa.ctr = c->d_ctr.p;
+  HIP_CHECK(hipEventRecord(c->ev[0], s));
   launch_seed(sa, s);
   HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipEventRecord(c->ev[1], s));
   std::vector<uint32_t> nh(B);
   HIP_CHECK(hipMemcpyAsync(nh.data(), c->d_nhits.p, (size_t)B * 4, hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
+  {
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->stats.ms_k_seed += ms;
+    c->stats.n_k_seed += 1;
+  }
   // overflow: rerun the affected reads with exact capacity into a second arena
   std::vector<uint32_t> ov;
   uint64_t ov_total = 0;
@@ -640,11 +656,19 @@ static void batch_candidates(pbw_ctx* c, Batch& b) {
   da.max_m = b.max_m;
   da.max_n = b.max_n;
   if (2 * band + 1 <= 63) {
+    HIP_CHECK(hipEventRecord(c->ev[2], s));
     launch_dp_band(da, s);
     HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(c->ev[3], s));
     std::vector<uint8_t> gen(nslots);
     HIP_CHECK(hipMemcpyAsync(gen.data(), c->d_generic.p, nslots, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
+    {
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+      c->stats.ms_k_dp += ms;
+      c->stats.n_k_dp += 1;
+    }
     std::vector<uint32_t> list;
     for (size_t k = 0; k < nslots; k++)
       if (gen[k]) list.push_back((uint32_t)k);
@@ -724,10 +748,11 @@ static void collect_counters(pbw_ctx* c, uint32_t reads) {
 }
 
 // SingleEndMapper::map (single.rs:164-207) for every read of a batch.
-static void map_se_batch(pbw_ctx* c, const pbw_read_batch* in, uint64_t b0, uint64_t b1, HostRecords& hr) {
+static void map_se_batch(pbw_ctx* c, const pbw_read_batch* in, uint64_t b0, uint64_t b1, HostRecords& hr,
+                         bool resident) {
   hipStream_t s = c->stream;
   Batch b;
-  batch_upload_slice(c, b, in, b0, b1);
+  batch_upload_slice(c, b, in, b0, b1, resident);
   HIP_CHECK(hipMemsetAsync(c->d_ctr.p, 0, sizeof(Counters), s));
   batch_candidates(c, b);
   auto t0 = std::chrono::steady_clock::now();
@@ -1045,6 +1070,8 @@ void pbw_ctx_destroy(pbw_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1058,7 +1085,22 @@ int pbw_map_se(pbw_ctx* c, const pbw_read_batch* reads, pbw_record_batch* out) {
     HostRecords hr;
     hr.rec.reserve(reads->n);
     uint64_t bs = c->opt.batch_reads ? c->opt.batch_reads : 131072;
-    for (uint64_t b = 0; b < reads->n; b += bs) map_se_batch(c, reads, b, std::min(reads->n, b + bs), hr);
+    for (uint64_t b = 0; b < reads->n; b += bs) map_se_batch(c, reads, b, std::min(reads->n, b + bs), hr, false);
+    emit_batch(hr, out);
+    c->stats.ms_total = ms_since(t0);
+  });
+}
+
+int pbw_map_se_resident(pbw_ctx* c, const pbw_read_batch* reads, pbw_record_batch* out) {
+  if (!c || !reads || !out) return PBW_ERR_ARG;
+  return guarded(c, [&] {
+    HIP_CHECK(hipSetDevice(c->device));
+    c->stats = pbw_stats{};
+    auto t0 = std::chrono::steady_clock::now();
+    HostRecords hr;
+    hr.rec.reserve(reads->n);
+    uint64_t bs = c->opt.batch_reads ? c->opt.batch_reads : 131072;
+    for (uint64_t b = 0; b < reads->n; b += bs) map_se_batch(c, reads, b, std::min(reads->n, b + bs), hr, true);
     emit_batch(hr, out);
     c->stats.ms_total = ms_since(t0);
   });

@@ -126,36 +126,42 @@ def reads_probe_se(contigs, n=20_000, L=120, seed=11):
     return _flat(seqs, quals), truth
 
 
-def reads_uneven_se(contigs, n=2_000_000, seed=12, lmin=80, lmax=150):
-    """Uneven single-end reads: L ~ U{80..150}, decaying Phred qualities, Phred-rate errors."""
+def reads_uneven_se(contigs, n=2_000_000, seed=12, lmin=80, lmax=150, chunk=65536):
+    """Uneven single-end reads: L ~ U{80..150}, Q(pos) = 38 - 16 pos/(L-1) + N(0,3)
+    rounded and clamped to [2,41], substitutions at the Phred-implied rate,
+    strand 50/50 (reverse reads are the reverse complement, quals reversed).
+    Vectorised in chunks; returns flat (seq, seq_off, qual, qual_off)."""
     rng = np.random.default_rng(np.random.PCG64(seed))
+    genome = np.concatenate([c[1] for c in contigs])
     lens = np.array([len(c[1]) for c in contigs], np.int64)
-    cum = np.cumsum(lens)
+    starts = np.concatenate([[0], np.cumsum(lens)[:-1]])
     L = rng.integers(lmin, lmax + 1, n)
-    gpos = (rng.random(n) * (cum[-1] - lmax - 1)).astype(np.int64)
-    cid = np.searchsorted(cum, gpos, side="right")
-    start = gpos - np.concatenate([[0], cum[:-1]])[cid]
+    cid = rng.choice(len(contigs), n, p=lens / lens.sum())
+    st = (rng.random(n) * (lens[cid] - L)).astype(np.int64) + starts[cid]
     rev = rng.random(n) < 0.5
-    seqs, quals = [], []
-    for i in range(n):
-        c = contigs[cid[i]][1]
-        st = int(min(start[i], len(c) - L[i]))
-        s = c[st:st + L[i]]
-        pos = np.arange(L[i])
-        q = np.rint(38.0 - 16.0 * pos / (L[i] - 1) + rng.normal(0.0, 3.0, L[i]))
+    so = np.zeros(n + 1, np.uint64)
+    so[1:] = np.cumsum(L)
+    seq = np.zeros(int(so[-1]) + 1, np.uint8)
+    qual = np.zeros(int(so[-1]) + 1, np.uint8)
+    col = np.arange(lmax)
+    for c0 in range(0, n, chunk):
+        c1 = min(n, c0 + chunk)
+        Lc, sc, rc = L[c0:c1], st[c0:c1], rev[c0:c1]
+        valid = col[None, :] < Lc[:, None]
+        # forward-strand bases of the template; reverse reads read it backwards
+        idx = np.where(rc[:, None], sc[:, None] + Lc[:, None] - 1 - col[None, :], sc[:, None] + col[None, :])
+        idx = np.where(valid, idx, 0)
+        b = genome[idx]
+        b = np.where(rc[:, None], COMP[b], b)
+        q = np.rint(38.0 - 16.0 * col[None, :] / (Lc[:, None] - 1) + rng.normal(0.0, 3.0, (c1 - c0, lmax)))
         q = np.clip(q, 2, 41).astype(np.uint8)
-        err = rng.random(L[i]) < np.power(10.0, -q / 10.0)
-        s = s.copy()
-        if err.any():
-            idx = np.nonzero(err)[0]
-            codes = np.searchsorted(BASES, s[idx])
-            s[idx] = BASES[(codes + rng.integers(1, 4, len(idx))) % 4]
-        if rev[i]:
-            s = revcomp(s)
-            q = q[::-1].copy()
-        seqs.append(s)
-        quals.append(q)
-    return _flat(seqs, quals)
+        err = rng.random((c1 - c0, lmax)) < np.power(10.0, -q.astype(np.float64) / 10.0)
+        codes = np.searchsorted(BASES, b)
+        b = np.where(err, BASES[(codes + rng.integers(1, 4, (c1 - c0, lmax))) % 4], b)
+        o0 = int(so[c0])
+        seq[o0:int(so[c1])] = b[valid]
+        qual[o0:int(so[c1])] = q[valid]
+    return seq, so, qual, so.copy()
 
 
 def reads_pairs(contigs, n=10_000, L=150, seed=13, repeat_spans=()):
