@@ -7,7 +7,8 @@ reads with lengths U{80..150}, Phred qualities decaying 38 -> 22 along the
 read (+ N(0,3) noise), substitutions at the Phred-implied rate.
 
 A step maps one batch of `--batch` reads that are already resident in HBM
-(`pbw_map_se_resident`): seed -> cluster -> banded DP -> MAPQ, records back
+(`pbw_reads_upload` once, then `pbw_map_se_device` per step; calls return
+after the records are in host memory): seed -> cluster -> banded DP -> MAPQ, records back
 in host memory.  `value` = reads mapped per second over the K timed steps.
 
 Also reported: the dominant kernel's roofline position (HIP-event timed inside
@@ -70,23 +71,18 @@ def main():
     ctx = P.Context(gp, batch_reads=args.batch, device=0)
     log("context (index build + upload) %.1f s, %d index entries" % (time.time() - t0, ctx.total_entries))
 
-    import torch
-
-    dseq = torch.from_numpy(seq).to("cuda:0")
-    dqual = torch.from_numpy(qual).to("cuda:0")
-    torch.cuda.synchronize()
+    dreads = ctx.upload_reads(P.ReadBatch.from_arrays(seq, so, qual, qo))
     n = args.reads
     B = min(args.batch, n)
 
     def step(s):
         lo = (s * B) % n
         hi = min(n, lo + B)
-        return ctx.map_se_resident(dseq.data_ptr(), so[lo:hi + 1], dqual.data_ptr(), qo[lo:hi + 1]), ctx.stats()
+        return ctx.map_se_device(dreads, lo, hi), ctx.stats()
 
     for w in range(args.warmup):
         step(w)
         log("warmup step %d done" % w)
-    torch.cuda.synchronize()
     agg = {}
     first = None
     t0 = time.perf_counter()
@@ -97,7 +93,6 @@ def main():
         for key, v in st.items():
             agg[key] = agg.get(key, 0) + v
         log("step %d: %d reads in %.1f ms" % (k, st["reads"], st["ms_total"]))
-    torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     reads_done = int(agg["reads"])
     value = reads_done / dt

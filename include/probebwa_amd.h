@@ -149,10 +149,14 @@ const char* pbw_global_error(void);
  *      records in input order ---- */
 int pbw_map_se(pbw_ctx* ctx, const pbw_read_batch* reads, pbw_record_batch* out);
 
-/* ---- same as pbw_map_se for reads already resident in device memory (HBM):
- *      seq/qual are device pointers, seq_off/qual_off are host arrays.  No
+/* ---- reads kept resident in device memory (HBM) across calls: upload once,
+ *      then map any [begin, end) range without host-to-device copies.  No
  *      counterpart in the reference; for callers that keep reads on the GPU. ---- */
-int pbw_map_se_resident(pbw_ctx* ctx, const pbw_read_batch* reads, pbw_record_batch* out);
+typedef struct pbw_device_reads pbw_device_reads;
+int pbw_reads_upload(pbw_ctx* ctx, const pbw_read_batch* reads, pbw_device_reads** out);
+int pbw_map_se_device(pbw_ctx* ctx, const pbw_device_reads* reads, uint64_t begin, uint64_t end,
+                      pbw_record_batch* out);
+void pbw_reads_free(pbw_device_reads* reads);
 
 /* ---- paired-end: map_paired (src/lib.rs:145-185) incl. the 500-pair serial
  *      warm-up and 2000-pair frozen-snapshot batches; records interleaved

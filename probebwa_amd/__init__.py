@@ -44,8 +44,8 @@ CIGAR_CHARS = "MIDNSHP=X"
 EXPORTED_SYMBOLS = (
     "pbw_genome_create", "pbw_genome_add_contig", "pbw_genome_add_fasta", "pbw_genome_n_contigs",
     "pbw_genome_contig_name", "pbw_genome_contig_length", "pbw_genome_destroy", "pbw_default_options",
-    "pbw_ctx_create", "pbw_ctx_destroy", "pbw_last_error", "pbw_global_error", "pbw_map_se", "pbw_map_se_resident",
-    "pbw_map_pe",
+    "pbw_ctx_create", "pbw_ctx_destroy", "pbw_last_error", "pbw_global_error", "pbw_map_se", "pbw_reads_upload",
+    "pbw_map_se_device", "pbw_reads_free", "pbw_map_pe",
     "pbw_align_batch", "pbw_index_lookup", "pbw_index_total_entries", "pbw_records_release", "pbw_get_stats",
     "pbw_format_sam", "pbw_free_text",
 )
@@ -138,7 +138,9 @@ def lib():
             "pbw_last_error": (C.c_char_p, [vp]),
             "pbw_global_error": (C.c_char_p, []),
             "pbw_map_se": (i32, [vp, C.POINTER(_ReadBatch), C.POINTER(_RecordBatch)]),
-            "pbw_map_se_resident": (i32, [vp, C.POINTER(_ReadBatch), C.POINTER(_RecordBatch)]),
+            "pbw_reads_upload": (i32, [vp, C.POINTER(_ReadBatch), C.POINTER(vp)]),
+            "pbw_map_se_device": (i32, [vp, vp, u64, u64, C.POINTER(_RecordBatch)]),
+            "pbw_reads_free": (None, [vp]),
             "pbw_map_pe": (i32, [vp, C.POINTER(_ReadBatch), C.POINTER(_ReadBatch), C.POINTER(InsertModel),
                                  C.POINTER(_RecordBatch)]),
             "pbw_align_batch": (i32, [vp, u32, u32, C.POINTER(_AlignJob), u64, C.POINTER(_AlignResult), vp, u64]),
@@ -290,14 +292,15 @@ class Context:
         _check(lib().pbw_map_se(self.h, C.byref(rs), C.byref(out)), self.h)
         return _take(out)
 
-    def map_se_resident(self, seq_ptr, seq_off, qual_ptr, qual_off):
-        """Single-end map of reads resident in device memory (device pointers
-        seq_ptr/qual_ptr, host offset arrays)."""
-        so = np.ascontiguousarray(seq_off, dtype=np.uint64)
-        qo = np.ascontiguousarray(qual_off, dtype=np.uint64)
-        rs = _ReadBatch(len(so) - 1, seq_ptr, so.ctypes.data, qual_ptr, qo.ctypes.data)
+    def upload_reads(self, reads):
+        """Copies a read batch into device memory once (`pbw_reads_upload`)."""
+        return DeviceReads(self, reads)
+
+    def map_se_device(self, dreads, begin=0, end=None):
+        """Single-end map of reads [begin, end) already resident in HBM."""
+        end = dreads.n if end is None else end
         out = _RecordBatch()
-        _check(lib().pbw_map_se_resident(self.h, C.byref(rs), C.byref(out)), self.h)
+        _check(lib().pbw_map_se_device(self.h, dreads.h, begin, end, C.byref(out)), self.h)
         return _take(out)
 
     def map_pe(self, r1, r2, mean=None, sd=None, n_observed=0):
@@ -349,6 +352,23 @@ class Context:
         s = Stats()
         _check(lib().pbw_get_stats(self.h, C.byref(s)), self.h)
         return s.as_dict()
+
+
+class DeviceReads:
+    """Reads resident in device memory (`pbw_reads_upload` / `pbw_reads_free`)."""
+
+    def __init__(self, ctx, reads):
+        self.ctx = ctx  # keeps the context (and its device) alive
+        self.n = reads.n
+        h = C.c_void_p()
+        rs = reads.cstruct()
+        _check(lib().pbw_reads_upload(ctx.h, C.byref(rs), C.byref(h)), ctx.h)
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().pbw_reads_free(self.h)
+            self.h = None
 
 
 def format_sam(genome, reads, records, command_line="probebwa", read_group=None):

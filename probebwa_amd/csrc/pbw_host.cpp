@@ -608,13 +608,13 @@ static void batch_candidates(pbw_ctx* c, Batch& b) {
   HIP_CHECK(hipMemcpyAsync(&nt2, c->d_ntier2.p, 4, hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
   if (nt2) {
-    uint64_t p2 = 1;
-    while (p2 < max_hits) p2 <<= 1;
-    uint32_t blocks = std::min<uint32_t>(nt2, 64);
-    ca.scratch_per_block = 6 * p2 + 16;
+    // one 256-thread block per read, per-block scratch sized for the largest read, <= 1 GiB in all
+    ca.scratch_per_block = cluster_scratch_words(max_hits);
+    const uint64_t fit = std::max<uint64_t>(1, (1ull << 27) / ca.scratch_per_block);
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>({(uint64_t)nt2, 2048, fit});
     c->d_scratch.ensure((size_t)blocks * ca.scratch_per_block);
     ca.scratch = c->d_scratch.p;
-    launch_cluster_tier2(ca, c->d_tier2.p, nt2, s);
+    launch_cluster_tier2(ca, c->d_tier2.p, nt2, blocks, s);
     HIP_CHECK(hipGetLastError());
     c->stats.tier2_cluster_reads += nt2;
   }
@@ -1091,16 +1091,53 @@ int pbw_map_se(pbw_ctx* c, const pbw_read_batch* reads, pbw_record_batch* out) {
   });
 }
 
-int pbw_map_se_resident(pbw_ctx* c, const pbw_read_batch* reads, pbw_record_batch* out) {
+struct pbw_device_reads {
+  int device;
+  uint64_t n;
+  DevBuf<uint8_t> seq, qual;
+  std::vector<uint64_t> seq_off, qual_off;
+};
+
+int pbw_reads_upload(pbw_ctx* c, const pbw_read_batch* reads, pbw_device_reads** out) {
   if (!c || !reads || !out) return PBW_ERR_ARG;
+  return guarded(c, [&] {
+    HIP_CHECK(hipSetDevice(c->device));
+    auto* d = new pbw_device_reads();
+    d->device = c->device;
+    d->n = reads->n;
+    d->seq_off.assign(reads->seq_off, reads->seq_off + reads->n + 1);
+    d->qual_off.assign(reads->qual_off, reads->qual_off + reads->n + 1);
+    try {
+      d->seq.upload(reads->seq + reads->seq_off[0], reads->seq_off[reads->n] - reads->seq_off[0], c->stream);
+      d->qual.upload(reads->qual + reads->qual_off[0], reads->qual_off[reads->n] - reads->qual_off[0], c->stream);
+      HIP_CHECK(hipStreamSynchronize(c->stream));
+    } catch (...) {
+      delete d;
+      throw;
+    }
+    for (auto& x : d->seq_off) x -= reads->seq_off[0];
+    for (auto& x : d->qual_off) x -= reads->qual_off[0];
+    *out = d;
+  });
+}
+
+void pbw_reads_free(pbw_device_reads* d) {
+  if (!d) return;
+  (void)hipSetDevice(d->device);
+  delete d;
+}
+
+int pbw_map_se_device(pbw_ctx* c, const pbw_device_reads* d, uint64_t begin, uint64_t end, pbw_record_batch* out) {
+  if (!c || !d || !out || begin > end || end > d->n) return PBW_ERR_ARG;
   return guarded(c, [&] {
     HIP_CHECK(hipSetDevice(c->device));
     c->stats = pbw_stats{};
     auto t0 = std::chrono::steady_clock::now();
     HostRecords hr;
-    hr.rec.reserve(reads->n);
+    hr.rec.reserve(end - begin);
+    pbw_read_batch view{d->n, d->seq.p, d->seq_off.data(), d->qual.p, d->qual_off.data()};
     uint64_t bs = c->opt.batch_reads ? c->opt.batch_reads : 131072;
-    for (uint64_t b = 0; b < reads->n; b += bs) map_se_batch(c, reads, b, std::min(reads->n, b + bs), hr, true);
+    for (uint64_t b = begin; b < end; b += bs) map_se_batch(c, &view, b, std::min(end, b + bs), hr, true);
     emit_batch(hr, out);
     c->stats.ms_total = ms_since(t0);
   });

@@ -22,7 +22,7 @@ constexpr int64_t DIAG_BIAS = 1ll << 40;
 // ---- limits of this implementation ----
 constexpr int MAX_READ_LEN = 1024;       // longer reads are rejected with PBW_ERR_LIMIT
 constexpr int HIT_CAP = 1024;            // hits per read in the first seed pass (overflow -> exact rerun)
-constexpr int CLUSTER_LDS_CAP = 1024;    // hits per read sorted in LDS by one wave (else tier 2)
+constexpr int CLUSTER_LDS_CAP = 512;     // hits per read clustered in LDS by one wave (else tier 2)
 constexpr int LN_INT_MAX = 4096;         // ln(k) table for phred_n_choose_m
 constexpr int GC_LUT_MAX = 1024;         // (gc, total) table for the GC-adjusted alphabet size
 
@@ -131,7 +131,8 @@ struct ClusterArgs {
   uint64_t scratch_per_block;
 };
 void launch_cluster(const ClusterArgs& a, hipStream_t s);
-void launch_cluster_tier2(const ClusterArgs& a, const uint32_t* list, uint32_t n, hipStream_t s);
+uint64_t cluster_scratch_words(uint32_t max_hits);  // per-block scratch of the block tier (u64 words)
+void launch_cluster_tier2(const ClusterArgs& a, const uint32_t* list, uint32_t n, uint32_t blocks, hipStream_t s);
 
 struct JobArgs {
   DevGenome G;
@@ -166,9 +167,11 @@ struct DpArgs {
   uint64_t* tb;            // per wave: (max_m + 1) * tiles * 4
   uint32_t tiles;          // ceil((max_n + 1) / 64)
   uint32_t wpb;            // waves per block (banded kernel; set by the launcher)
+  uint32_t* tbw;           // lane kernel: per-wave traceback words, (max_m + 1) * 8 * 64 each
 };
 void launch_dp_band(const DpArgs& a, hipStream_t s);
 void launch_dp_tiled(const DpArgs& a, uint32_t nwaves, hipStream_t s);
+bool launch_dp_lanes(const DpArgs& a, uint32_t nwaves, hipStream_t s);
 
 struct MapqArgs {
   DevGenome G;

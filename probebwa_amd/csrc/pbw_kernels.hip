@@ -56,22 +56,38 @@ __device__ __forceinline__ int64_t shfl_xor_i64(int64_t v, int m) {
   return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
-  int l = lane_id();
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    uint32_t t = (uint32_t)__shfl_up((int)x, (unsigned)o, 64);
-    if (l >= o) x += t;
-  }
-  return x;
+// DPP cross-lane moves (GFX9 encodings; all VALU, no LDS traffic).
+// wave_shr:1 -> lane i reads lane i-1; wave_shl:1 -> lane i reads lane i+1;
+// lanes without a source keep `old`.
+__device__ __forceinline__ int32_t dpp_wave_shr1(int32_t old, int32_t x) {
+  return __builtin_amdgcn_update_dpp(old, x, 0x138, 0xf, 0xf, false);
 }
+__device__ __forceinline__ int32_t dpp_wave_shl1(int32_t old, int32_t x) {
+  return __builtin_amdgcn_update_dpp(old, x, 0x130, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int32_t readlane_i32(int32_t x, int lane) { return __builtin_amdgcn_readlane(x, lane); }
+
+// Inclusive prefix sum over the wave: row_shr 1,2,4,8 inside rows of 16,
+// then row_bcast 15 / 31 across rows.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+  int32_t v = (int32_t)x;
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
+  return (uint32_t)v;
+}
+// Inclusive prefix max over the wave (same pattern, identity INT32_MIN).
 __device__ __forceinline__ int32_t wave_incl_max(int32_t x) {
-  int l = lane_id();
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    int32_t t = shfl_up_i32(x, o);
-    if (l >= o) x = max(x, t);
-  }
+  const int32_t I = INT32_MIN;
+  x = max(x, __builtin_amdgcn_update_dpp(I, x, 0x111, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(I, x, 0x112, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(I, x, 0x114, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(I, x, 0x118, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(I, x, 0x142, 0xa, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(I, x, 0x143, 0xc, 0xf, false));
   return x;
 }
 __device__ __forceinline__ int32_t wave_suffix_min(int32_t x) {
@@ -257,7 +273,7 @@ __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
       S.nbrep[lane] = 0;
       uint32_t incl = wave_incl_scan(nlook);
       S.lbase[lane] = incl - nlook;
-      const uint32_t T = shfl_u32(incl, 63);
+      const uint32_t T = (uint32_t)readlane_i32((int32_t)incl, 63);
       wave_sync();
       // lookups of this chunk in emission order: window, exact, then the 45
       // neighbours in (position, base) order (seeds.rs:107-147, :291-309)
@@ -292,7 +308,7 @@ __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
           c_pos += lcnt;
         }
         uint32_t pincl = wave_incl_scan(lcnt);
-        const uint32_t PT = shfl_u32(pincl, 63);
+        const uint32_t PT = (uint32_t)readlane_i32((int32_t)pincl, 63);
         S.r_incl[lane] = pincl;
         S.r_off[lane] = loff;
         S.r_w[lane] = (uint8_t)w;
@@ -409,301 +425,379 @@ __device__ uint32_t sw_find_slot(const int32_t* tab, uint32_t nb, uint64_t h) {
   }
 }
 
-// Scratch used by the (single-lane) group/cluster pass.  All pointers are
-// generic: LDS for the wave tier, global memory for the block tier.
-struct ClusterWork {
-  uint32_t* g_key;    // group key (contig*2 + strand)
-  uint32_t* g_min;    // group's first hit index (emission order)
-  uint32_t* g_rank;   // iteration rank of the group in the emulated FxHashMap
-  uint32_t* g_sort;   // (first hit << 16 | group) sorted
-  int32_t* table;     // emulated SwissTable buckets
-  int32_t* tmp;       // resize re-insertion list
-  uint32_t table_cap; // capacity of table[]
-};
+// ---- emulated FxHashMap iteration order of the (contig, strand) groups ----
+// Groups are inserted in first-appearance order (candidates.rs:36-43); the
+// map iterates in ascending bucket index.
 
-// Lane-0 part 1: groups from the sorted keys.
-__device__ uint32_t cluster_groups(const uint64_t* key, const uint32_t* idx, uint32_t n, ClusterWork& W) {
-  uint32_t G = 0;
-  uint32_t prev = 0xFFFFFFFFu;
-  for (uint32_t e = 0; e < n; e++) {
-    uint32_t gk = (uint32_t)(key[e] >> DIAG_BITS);
-    if (gk != prev) {
-      W.g_key[G] = gk;
-      W.g_min[G] = idx[e];
-      G++;
-      prev = gk;
-    } else if (idx[e] < W.g_min[G - 1]) {
-      W.g_min[G - 1] = idx[e];
-    }
-  }
-  return G;
-}
-
-// Lane-0 part 2: iteration rank of each group in the emulated FxHashMap
-// (keys inserted in first-appearance order; iteration = ascending bucket).
-__device__ void cluster_group_ranks(uint32_t G, ClusterWork& W) {
+// General path (any number of groups): one thread walks an explicit table.
+// g_sort[t] = group inserted t-th; writes g_rank[group].
+__device__ void group_ranks_table(uint32_t G, const uint32_t* g_key, const uint32_t* g_sort, uint32_t* g_rank,
+                                  int32_t* table, int32_t* tmp) {
   uint32_t nb = 0, items = 0, growth_left = 0;
   for (uint32_t t = 0; t < G; t++) {
-    uint32_t gi = W.g_sort[t] & 0xFFFFu;
-    if (G > 0xFFFF) gi = W.g_sort[t];  // block tier stores plain indices
-    uint64_t h = fx_hash_group(W.g_key[gi] >> 1, W.g_key[gi] & 1u);
+    const uint32_t gi = g_sort[t];
+    const uint64_t h = fx_hash_group(g_key[gi] >> 1, g_key[gi] & 1u);
     if (growth_left == 0) {
-      uint32_t cap = max(items + 1, sw_capacity(nb) + 1);
-      uint32_t nnb = sw_buckets_for(cap);
+      const uint32_t nnb = sw_buckets_for(max(items + 1, sw_capacity(nb) + 1));
       uint32_t cnt = 0;
       for (uint32_t b = 0; b < nb; b++)
-        if (W.table[b] >= 0) W.tmp[cnt++] = W.table[b];
-      for (uint32_t b = 0; b < nnb; b++) W.table[b] = -1;
+        if (table[b] >= 0) tmp[cnt++] = table[b];
+      for (uint32_t b = 0; b < nnb; b++) table[b] = -1;
       nb = nnb;
       for (uint32_t c = 0; c < cnt; c++) {
-        int32_t gj = W.tmp[c];
-        uint64_t hj = fx_hash_group(W.g_key[gj] >> 1, W.g_key[gj] & 1u);
-        W.table[sw_find_slot(W.table, nb, hj)] = gj;
+        const int32_t gj = tmp[c];
+        table[sw_find_slot(table, nb, fx_hash_group(g_key[gj] >> 1, g_key[gj] & 1u))] = gj;
       }
       growth_left = sw_capacity(nb) - items;
     }
-    W.table[sw_find_slot(W.table, nb, h)] = (int32_t)gi;
+    table[sw_find_slot(table, nb, h)] = (int32_t)gi;
     growth_left--;
     items++;
   }
   uint32_t rank = 0;
   for (uint32_t b = 0; b < nb; b++)
-    if (W.table[b] >= 0) W.g_rank[W.table[b]] = rank++;
+    if (table[b] >= 0) g_rank[table[b]] = rank++;
 }
 
-// Lane-0 part 3: clusters (sliding merge, candidates.rs:51-71) and the
-// top-20 of the stable sort by score (:76): key (count desc, group rank, order).
-__device__ uint32_t cluster_select(const uint64_t* key, uint32_t n, uint32_t L, const ClusterWork& W, Cand* out) {
-  uint64_t top_k[TOP_CANDIDATES];
-  Cand top_c[TOP_CANDIDATES];
-  uint32_t nt = 0;
-  uint32_t gi = 0, start = 0;
-  int64_t sum = 0;
-  uint32_t gk_prev = (uint32_t)(key[0] >> DIAG_BITS);
-  int64_t d_prev = (int64_t)(key[0] & ((1ull << DIAG_BITS) - 1)) - DIAG_BIAS;
-  sum = d_prev;
-  for (uint32_t e = 1; e <= n; e++) {
-    bool boundary;
-    uint32_t gk = 0;
-    int64_t d = 0;
-    if (e == n) {
-      boundary = true;
-    } else {
-      gk = (uint32_t)(key[e] >> DIAG_BITS);
-      d = (int64_t)(key[e] & ((1ull << DIAG_BITS) - 1)) - DIAG_BIAS;
-      boundary = gk != gk_prev || d - d_prev > MERGE_DISTANCE;
-    }
-    if (boundary) {
-      uint32_t count = e - start;
-      int64_t avg = sum / (int64_t)count;
-      bool rev = gk_prev & 1u;
-      int64_t position = rev ? avg - (int64_t)L + K : avg;
-      if (position < 0) position = 0;
-      uint64_t sk = ((uint64_t)(0x3FFFFFu - min(count, 0x3FFFFFu)) << 42) |
-                    ((uint64_t)min(W.g_rank[gi], 0xFFFFFu) << 22) | (uint64_t)min(start, 0x3FFFFFu);
-      if (nt < (uint32_t)TOP_CANDIDATES || sk < top_k[nt - 1]) {
-        uint32_t pos = nt < (uint32_t)TOP_CANDIDATES ? nt : nt - 1;
-        while (pos > 0 && top_k[pos - 1] > sk) {
-          top_k[pos] = top_k[pos - 1];
-          top_c[pos] = top_c[pos - 1];
-          pos--;
+// Insert slot of sw_find_slot for a table of at most 64 buckets whose
+// occupancy is the bit mask `occ` (wave-uniform).
+__device__ __forceinline__ uint32_t sw_find_slot_mask(uint64_t occ, uint32_t nb, uint32_t h) {
+  const uint32_t mask = nb - 1;
+  uint32_t pos = h & mask, stride = 0;
+  for (;;) {
+    for (uint32_t b = 0; b < 16; b++) {
+      const uint32_t i = pos + b;
+      bool empty;
+      if (i < nb) empty = !((occ >> i) & 1ull);
+      else if (nb < 16) empty = i < 16 ? true : !((occ >> (i - 16)) & 1ull);
+      else empty = !((occ >> (i - nb)) & 1ull);
+      if (empty) {
+        const uint32_t index = (pos + b) & mask;
+        if ((occ >> index) & 1ull) {
+          for (uint32_t c = 0; c < 16; c++)
+            if (c >= nb || !((occ >> c) & 1ull)) return c;
         }
-        top_k[pos] = sk;
-        Cand c;
-        c.pos = (uint64_t)position;
-        c.contig = gk_prev >> 1;
-        c.strand_score = (rev ? 0x80000000u : 0u) | min(count, 0x7FFFFFFFu);
-        top_c[pos] = c;
-        if (nt < (uint32_t)TOP_CANDIDATES) nt++;
-      }
-      if (e < n) {
-        if (gk != gk_prev) gi++;
-        start = e;
-        sum = 0;
+        return index;
       }
     }
-    if (e < n) {
-      sum += d;
-      gk_prev = gk;
-      d_prev = d;
+    stride += 16;
+    pos = (pos + stride) & mask;
+  }
+}
+
+// Register path for G <= 56 groups (final table <= 64 buckets): one wave,
+// lane b holds the group stored in bucket b, occupancy is a uniform mask.
+// All 64 lanes of the calling wave must be active.
+__device__ void group_ranks_wave(uint32_t G, const uint32_t* g_key, const uint32_t* g_sort, uint32_t* g_rank) {
+  const int lane = lane_id();
+  // lane g: low word of the group's hash; lane t: group inserted t-th
+  uint32_t hl = 0, ins = 0;
+  if ((uint32_t)lane < G) {
+    hl = (uint32_t)fx_hash_group(g_key[lane] >> 1, g_key[lane] & 1u);
+    ins = g_sort[lane];
+  }
+  int32_t tab = -1;
+  uint64_t occ = 0;
+  uint32_t nb = 0, items = 0, growth_left = 0;
+  for (uint32_t t = 0; t < G; t++) {
+    const uint32_t gi = (uint32_t)readlane_i32((int32_t)ins, (int)t);
+    const uint32_t h = (uint32_t)readlane_i32((int32_t)hl, (int)gi);
+    if (growth_left == 0) {
+      const uint32_t nnb = sw_buckets_for(max(items + 1, sw_capacity(nb) + 1));
+      uint64_t old = occ, nocc = 0;
+      int32_t ntab = -1;
+      while (old) {  // re-insert in ascending old bucket order
+        const uint32_t b = (uint32_t)__builtin_ctzll(old);
+        old &= old - 1;
+        const int32_t gj = readlane_i32(tab, (int)b);
+        const uint32_t s = sw_find_slot_mask(nocc, nnb, (uint32_t)readlane_i32((int32_t)hl, gj));
+        nocc |= 1ull << s;
+        if ((uint32_t)lane == s) ntab = gj;
+      }
+      occ = nocc;
+      tab = ntab;
+      nb = nnb;
+      growth_left = sw_capacity(nb) - items;
+    }
+    const uint32_t s = sw_find_slot_mask(occ, nb, h);
+    occ |= 1ull << s;
+    if ((uint32_t)lane == s) tab = (int32_t)gi;
+    growth_left--;
+    items++;
+  }
+  if ((occ >> lane) & 1ull) g_rank[tab] = (uint32_t)__builtin_popcountll(occ & lanemask_lt());
+}
+
+// ---- team primitives: T = 64 (one wave) or a block of T threads ----
+template <int T>
+__device__ __forceinline__ void team_sync() {
+  if constexpr (T == 64) wave_sync();
+  else __syncthreads();
+}
+// Exclusive prefix sum of one value per thread; `total` = team sum.
+// sh: T/64 words of LDS (unused for T = 64).
+template <int T>
+__device__ __forceinline__ uint32_t team_excl_scan(uint32_t v, uint32_t& total, uint32_t* sh) {
+  const uint32_t incl = wave_incl_scan(v);
+  const uint32_t wtot = (uint32_t)readlane_i32((int32_t)incl, 63);
+  if constexpr (T == 64) {
+    total = wtot;
+    return incl - v;
+  } else {
+    const uint32_t w = threadIdx.x >> 6;
+    if (lane_id() == 0) sh[w] = wtot;
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+    for (uint32_t k = 0; k < (uint32_t)(T / 64); k++) {
+      const uint32_t x = sh[k];
+      if (k < w) before += x;
+      tot += x;
+    }
+    __syncthreads();
+    total = tot;
+    return before + incl - v;
+  }
+}
+// Bitonic sort of p2 (power of two) 64-bit keys; `val` (optional) follows.
+template <int T>
+__device__ void team_bitonic(uint64_t* key, uint32_t* val, uint32_t p2, uint32_t tid) {
+  for (uint32_t k = 2; k <= p2; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = tid; i < p2; i += T) {
+        const uint32_t ixj = i ^ j;
+        if (ixj > i) {
+          const bool up = (i & k) == 0;
+          const uint64_t x = key[i], y = key[ixj];
+          if ((x > y) == up) {
+            key[i] = y;
+            key[ixj] = x;
+            if (val) {
+              const uint32_t t = val[i];
+              val[i] = val[ixj];
+              val[ixj] = t;
+            }
+          }
+        }
+      }
+      team_sync<T>();
     }
   }
-  for (uint32_t t = 0; t < nt; t++) out[t] = top_c[t];
-  return nt;
 }
 
-struct ClusterLds {
-  uint64_t key[CLUSTER_LDS_CAP];
-  uint32_t idx[CLUSTER_LDS_CAP];
-  uint32_t g_key[CLUSTER_LDS_CAP], g_min[CLUSTER_LDS_CAP], g_rank[CLUSTER_LDS_CAP], g_sort[CLUSTER_LDS_CAP];
-  int32_t table[2048], tmp[CLUSTER_LDS_CAP];
-  uint32_t G;
+// Working arrays of one read's clustering (LDS for the wave tier, global
+// scratch for the block tier).  Sizes: key/idx p2, cl_* n (+1), g_* gcap,
+// table tcap, tmp gcap.
+struct ClusterMem {
+  uint64_t* key;       // hit keys; later the cluster sort keys
+  uint32_t* idx;       // emission index of each hit; later cluster ids
+  unsigned long long* cl_sum;  // per cluster: sum of diagonals (two's complement)
+  uint32_t* cl_start;  // per cluster: first sorted hit; [C] = n
+  uint32_t* cl_gid;    // per cluster: group id
+  uint32_t* g_key;     // per group: contig*2 + strand
+  uint32_t* g_min;     // per group: first emission index
+  uint32_t* g_rank;    // per group: FxHashMap iteration rank
+  uint32_t* g_sort;    // groups in insertion (first-appearance) order
+  int32_t* table;      // general-path hash table
+  int32_t* tmp;
+  uint32_t gcap;       // capacity of the g_* arrays and tmp
 };
 
-__device__ void wave_bitonic_u64(uint64_t* key, uint32_t* idx, uint32_t p2) {
-  for (uint32_t k = 2; k <= p2; k <<= 1) {
-    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      for (uint32_t i = lane_id(); i < p2; i += 64) {
-        uint32_t ixj = i ^ j;
-        if (ixj > i) {
-          bool up = (i & k) == 0;
-          uint64_t x = key[i], y = key[ixj];
-          if ((x > y) == up) {
-            key[i] = y;
-            key[ixj] = x;
-            uint32_t t = idx[i];
-            idx[i] = idx[ixj];
-            idx[ixj] = t;
-          }
-        }
-      }
-      wave_sync();
+__device__ __forceinline__ int64_t hit_diag(uint64_t k) {
+  return (int64_t)(k & ((1ull << DIAG_BITS) - 1)) - DIAG_BIAS;
+}
+
+// find_candidates (candidates.rs:30-78) of one read by a team of T threads.
+// Returns false (uniformly) when the read has more groups than M.gcap; the
+// caller then hands it to the block tier.
+template <int T>
+__device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const ClusterMem& M, Cand* out,
+                             uint8_t* ncand_out, uint32_t* sh) {
+  const uint32_t tid = threadIdx.x % T;
+  uint32_t p2 = 1;
+  while (p2 < n) p2 <<= 1;
+  for (uint32_t i = tid; i < p2; i += T) {
+    M.key[i] = i < n ? hits[i] : ~0ull;
+    M.idx[i] = i;
+    if (i < n) M.cl_sum[i] = 0ull;
+  }
+  team_sync<T>();
+  // group by (contig, strand), sort each group by diagonal (:36-49): one sort by (group, diagonal)
+  team_bitonic<T>(M.key, M.idx, p2, tid);
+  // group and cluster starts (sliding merge, :51-71), one contiguous chunk per thread
+  const uint32_t chunk = (n + T - 1) / T;
+  const uint32_t c0 = min(n, tid * chunk), c1 = min(n, c0 + chunk);
+  uint32_t ng = 0, nc = 0;
+  {
+    uint64_t kp = c0 > 0 ? M.key[c0 - 1] : 0;
+    for (uint32_t e = c0; e < c1; e++) {
+      const uint64_t k = M.key[e];
+      const bool gs = e == 0 || (k >> DIAG_BITS) != (kp >> DIAG_BITS);
+      ng += gs;
+      nc += gs || hit_diag(k) - hit_diag(kp) > MERGE_DISTANCE;
+      kp = k;
     }
   }
-}
-__device__ void wave_bitonic_u32(uint32_t* key, uint32_t p2) {
-  for (uint32_t k = 2; k <= p2; k <<= 1) {
-    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      for (uint32_t i = lane_id(); i < p2; i += 64) {
-        uint32_t ixj = i ^ j;
-        if (ixj > i) {
-          bool up = (i & k) == 0;
-          uint32_t x = key[i], y = key[ixj];
-          if ((x > y) == up) {
-            key[i] = y;
-            key[ixj] = x;
-          }
-        }
+  uint32_t G, C;
+  const uint32_t gbase = team_excl_scan<T>(ng, G, sh);
+  const uint32_t cbase = team_excl_scan<T>(nc, C, sh);
+  if (G > M.gcap) return false;
+  for (uint32_t g = tid; g < G; g += T) M.g_min[g] = 0xFFFFFFFFu;
+  if (tid == 0) M.cl_start[C] = n;
+  team_sync<T>();
+  if (c1 > c0) {
+    int32_t gid = (int32_t)gbase - 1, cid = (int32_t)cbase - 1;
+    uint64_t kp = c0 > 0 ? M.key[c0 - 1] : 0;
+    int64_t psum = 0;
+    uint32_t pmin = 0xFFFFFFFFu;
+    for (uint32_t e = c0; e < c1; e++) {
+      const uint64_t k = M.key[e];
+      const bool gs = e == 0 || (k >> DIAG_BITS) != (kp >> DIAG_BITS);
+      const bool cs = gs || hit_diag(k) - hit_diag(kp) > MERGE_DISTANCE;
+      if (gs) {
+        if (gid >= 0 && pmin != 0xFFFFFFFFu) atomicMin(&M.g_min[gid], pmin);
+        pmin = 0xFFFFFFFFu;
+        gid++;
+        M.g_key[gid] = (uint32_t)(k >> DIAG_BITS);
       }
-      wave_sync();
+      if (cs) {
+        if (cid >= 0 && e > c0) atomicAdd(&M.cl_sum[cid], (unsigned long long)psum);
+        psum = 0;
+        cid++;
+        M.cl_start[cid] = e;
+        M.cl_gid[cid] = (uint32_t)gid;
+      }
+      psum += hit_diag(k);
+      pmin = min(pmin, M.idx[e]);
+      kp = k;
     }
+    atomicAdd(&M.cl_sum[cid], (unsigned long long)psum);
+    atomicMin(&M.g_min[gid], pmin);
   }
+  team_sync<T>();
+  // insertion order of the groups = order of first appearance in the hit list
+  if (G <= 1024) {
+    for (uint32_t g = tid; g < G; g += T) {
+      const uint32_t mg = M.g_min[g];
+      uint32_t r = 0;
+      for (uint32_t u = 0; u < G; u++) r += M.g_min[u] < mg;
+      M.g_sort[r] = g;
+    }
+    team_sync<T>();
+  } else {  // hit keys are no longer needed: sort (first index, group) in key[]
+    uint32_t g2 = 1;
+    while (g2 < G) g2 <<= 1;
+    for (uint32_t g = tid; g < g2; g += T) M.key[g] = g < G ? (((uint64_t)M.g_min[g] << 32) | g) : ~0ull;
+    team_sync<T>();
+    team_bitonic<T>(M.key, nullptr, g2, tid);
+    for (uint32_t g = tid; g < G; g += T) M.g_sort[g] = (uint32_t)M.key[g];
+    team_sync<T>();
+  }
+  if (G == 1) {
+    if (tid == 0) M.g_rank[0] = 0;
+  } else if (G <= 56) {
+    if (tid < 64) group_ranks_wave(G, M.g_key, M.g_sort, M.g_rank);
+  } else if (tid == 0) {
+    group_ranks_table(G, M.g_key, M.g_sort, M.g_rank, M.table, M.tmp);
+  }
+  team_sync<T>();
+  // stable sort by score (:76): key (count desc, group rank, position in the group)
+  uint32_t pc = 1;
+  while (pc < C) pc <<= 1;
+  for (uint32_t c = tid; c < pc; c += T) {
+    uint64_t sk = ~0ull;
+    if (c < C) {
+      const uint32_t s = M.cl_start[c], cnt = M.cl_start[c + 1] - s;
+      sk = ((uint64_t)(0x3FFFFFu - min(cnt, 0x3FFFFFu)) << 42) | ((uint64_t)min(M.g_rank[M.cl_gid[c]], 0xFFFFFu) << 22) |
+           (uint64_t)s;
+    }
+    M.key[c] = sk;
+    M.idx[c] = c;
+  }
+  team_sync<T>();
+  team_bitonic<T>(M.key, M.idx, pc, tid);
+  const uint32_t nt = min(C, (uint32_t)TOP_CANDIDATES);
+  if (tid < nt) {
+    const uint32_t c = M.idx[tid];
+    const uint32_t s = M.cl_start[c], cnt = M.cl_start[c + 1] - s;
+    const uint32_t gk = M.g_key[M.cl_gid[c]];
+    const int64_t avg = (int64_t)M.cl_sum[c] / (int64_t)cnt;
+    const bool rev = gk & 1u;
+    int64_t position = rev ? avg - (int64_t)L + K : avg;
+    if (position < 0) position = 0;
+    Cand cd;
+    cd.pos = (uint64_t)position;
+    cd.contig = gk >> 1;
+    cd.strand_score = (rev ? 0x80000000u : 0u) | min(cnt, 0x7FFFFFFFu);
+    out[tid] = cd;
+  }
+  if (tid == 0) *ncand_out = (uint8_t)nt;
+  team_sync<T>();
+  return true;
 }
+
+// Wave tier: one wave per read, all arrays in LDS.
+constexpr uint32_t CL_NCAP = CLUSTER_LDS_CAP;  // hits
+constexpr uint32_t CL_GCAP = 128;              // groups
+struct ClusterLds {
+  uint64_t key[CL_NCAP];
+  unsigned long long cl_sum[CL_NCAP];
+  uint32_t idx[CL_NCAP];
+  uint32_t cl_start[CL_NCAP + 1];
+  uint32_t cl_gid[CL_NCAP];
+  uint32_t g_key[CL_GCAP], g_min[CL_GCAP], g_rank[CL_GCAP], g_sort[CL_GCAP];
+  int32_t table[2 * CL_GCAP], tmp[CL_GCAP];
+};
 
 __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
   __shared__ ClusterLds S;
-  const int lane = lane_id();
+  const ClusterMem M{S.key, S.idx, S.cl_sum, S.cl_start, S.cl_gid, S.g_key, S.g_min, S.g_rank, S.g_sort, S.table,
+                     S.tmp, CL_GCAP};
   for (uint32_t r = blockIdx.x; r < a.R.n; r += gridDim.x) {
     const uint32_t n = a.nhits[r];
-    const uint32_t L = (uint32_t)(a.R.seq_off[r + 1] - a.R.seq_off[r]);
     if (n == 0) {
-      if (lane == 0) a.ncand[r] = 0;
+      if (lane_id() == 0) a.ncand[r] = 0;
       continue;
     }
-    if (n > (uint32_t)CLUSTER_LDS_CAP) {
-      if (lane == 0) {
-        uint32_t slot = atomicAdd(a.ntier2, 1u);
-        a.tier2[slot] = r;
-      }
-      continue;
+    bool done = false;
+    if (n <= CL_NCAP) {
+      const uint32_t L = (uint32_t)(a.R.seq_off[r + 1] - a.R.seq_off[r]);
+      done = cluster_read<64>(a.hits + a.hoff[r], n, L, M, a.cands + (uint64_t)r * TOP_CANDIDATES, a.ncand + r,
+                              nullptr);
     }
-    uint32_t p2 = 1;
-    while (p2 < n) p2 <<= 1;
-    const uint64_t* h = a.hits + a.hoff[r];
-    for (uint32_t i = lane; i < p2; i += 64) {
-      S.key[i] = i < n ? h[i] : ~0ull;
-      S.idx[i] = i;
+    if (!done && lane_id() == 0) {
+      const uint32_t slot = atomicAdd(a.ntier2, 1u);
+      a.tier2[slot] = r;
     }
-    wave_sync();
-    wave_bitonic_u64(S.key, S.idx, p2);
-    ClusterWork W{S.g_key, S.g_min, S.g_rank, S.g_sort, S.table, S.tmp, 2048};
-    if (lane == 0) S.G = cluster_groups(S.key, S.idx, n, W);
-    wave_sync();
-    const uint32_t G = S.G;
-    uint32_t g2 = 1;
-    while (g2 < G) g2 <<= 1;
-    for (uint32_t i = lane; i < g2; i += 64) S.g_sort[i] = i < G ? ((S.g_min[i] << 16) | i) : 0xFFFFFFFFu;
-    wave_sync();
-    wave_bitonic_u32(S.g_sort, g2);
-    if (lane == 0) {
-      cluster_group_ranks(G, W);
-      a.ncand[r] = (uint8_t)cluster_select(S.key, n, L, W, a.cands + (uint64_t)r * TOP_CANDIDATES);
-    }
-    wave_sync();
   }
 }
 
-// Block tier for reads with more hits than one wave sorts in LDS: bitonic
-// sort in global scratch by a 256-thread block, then the same lane-0 passes.
+// Block tier: reads with more hits (or groups) than the wave tier holds; a
+// block of 256 threads per read, arrays in per-block global scratch of
+// cluster_scratch_words(max hits) words.
 __global__ __launch_bounds__(256) void k_cluster_tier2(ClusterArgs a, const uint32_t* list, uint32_t nlist) {
-  uint64_t* base = a.scratch + (uint64_t)blockIdx.x * a.scratch_per_block;
+  __shared__ uint32_t sh[4];
   for (uint32_t li = blockIdx.x; li < nlist; li += gridDim.x) {
     const uint32_t r = list[li];
     const uint32_t n = a.nhits[r];
     const uint32_t L = (uint32_t)(a.R.seq_off[r + 1] - a.R.seq_off[r]);
     uint32_t p2 = 1;
     while (p2 < n) p2 <<= 1;
-    uint64_t* key = base;
-    uint32_t* idx = (uint32_t*)(base + p2);
-    uint32_t* g_key = idx + p2;
-    uint32_t* g_min = g_key + p2;
-    uint32_t* g_rank = g_min + p2;
-    uint32_t* g_sort = g_rank + p2;
-    int32_t* tmp = (int32_t*)(g_sort + p2);
-    int32_t* table = tmp + p2;  // up to 4*p2 buckets
-    const uint64_t* h = a.hits + a.hoff[r];
-    for (uint32_t i = threadIdx.x; i < p2; i += blockDim.x) {
-      key[i] = i < n ? h[i] : ~0ull;
-      idx[i] = i;
-    }
-    __syncthreads();
-    for (uint32_t k = 2; k <= p2; k <<= 1) {
-      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-        for (uint32_t i = threadIdx.x; i < p2; i += blockDim.x) {
-          uint32_t ixj = i ^ j;
-          if (ixj > i) {
-            bool up = (i & k) == 0;
-            uint64_t x = key[i], y = key[ixj];
-            if ((x > y) == up) {
-              key[i] = y;
-              key[ixj] = x;
-              uint32_t t = idx[i];
-              idx[i] = idx[ixj];
-              idx[ixj] = t;
-            }
-          }
-        }
-        __syncthreads();
-      }
-    }
-    __shared__ uint32_t Gs;
-    ClusterWork W{g_key, g_min, g_rank, g_sort, table, tmp, 4 * p2};
-    if (threadIdx.x == 0) Gs = cluster_groups(key, idx, n, W);
-    __syncthreads();
-    const uint32_t G = Gs;
-    uint32_t g2 = 1;
-    while (g2 < G) g2 <<= 1;
-    // sort groups by first appearance: keys (g_min, index) as u64 in key[] (hits no longer needed
-    // after cluster_groups only if we re-read them; keep hits: use g_sort pairs with a stable
-    // O(G log G) bitonic over 64-bit packed keys stored in g_rank/g_sort temporarily)
-    uint64_t* gk64 = (uint64_t*)table;  // table area is free until cluster_group_ranks
-    for (uint32_t i = threadIdx.x; i < g2; i += blockDim.x) gk64[i] = i < G ? (((uint64_t)g_min[i] << 32) | i) : ~0ull;
-    __syncthreads();
-    for (uint32_t k = 2; k <= g2; k <<= 1) {
-      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-        for (uint32_t i = threadIdx.x; i < g2; i += blockDim.x) {
-          uint32_t ixj = i ^ j;
-          if (ixj > i) {
-            bool up = (i & k) == 0;
-            uint64_t x = gk64[i], y = gk64[ixj];
-            if ((x > y) == up) {
-              gk64[i] = y;
-              gk64[ixj] = x;
-            }
-          }
-        }
-        __syncthreads();
-      }
-    }
-    for (uint32_t i = threadIdx.x; i < G; i += blockDim.x) g_sort[i] = (uint32_t)(gk64[i] & 0xFFFFFFFFu);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      if (G <= 0xFFFF)
-        for (uint32_t i = 0; i < G; i++) g_sort[i] = (g_sort[i] & 0xFFFFu);
-      cluster_group_ranks(G, W);
-      a.ncand[r] = (uint8_t)cluster_select(key, n, L, W, a.cands + (uint64_t)r * TOP_CANDIDATES);
-    }
-    __syncthreads();
+    uint64_t* base = a.scratch + (uint64_t)blockIdx.x * a.scratch_per_block;
+    ClusterMem M;
+    M.key = base;
+    M.cl_sum = (unsigned long long*)(base + p2);
+    uint32_t* w = (uint32_t*)(base + 2 * (uint64_t)p2);
+    M.idx = w;
+    M.cl_start = w + p2;
+    M.cl_gid = w + 2 * (uint64_t)p2 + 1;
+    M.g_key = w + 3 * (uint64_t)p2 + 1;
+    M.g_min = M.g_key + p2;
+    M.g_rank = M.g_min + p2;
+    M.g_sort = M.g_rank + p2;
+    M.tmp = (int32_t*)(M.g_sort + p2);
+    M.table = M.tmp + p2;  // 4 * p2 buckets
+    M.gcap = p2;
+    cluster_read<256>(a.hits + a.hoff[r], n, L, M, a.cands + (uint64_t)r * TOP_CANDIDATES, a.ncand + r, sh);
   }
 }
 
@@ -712,9 +806,15 @@ void launch_cluster(const ClusterArgs& a, hipStream_t s) {
   uint32_t blocks = a.R.n < 65536 ? a.R.n : 65536;
   hipLaunchKernelGGL(k_cluster, dim3(blocks), dim3(64), 0, s, a);
 }
-void launch_cluster_tier2(const ClusterArgs& a, const uint32_t* list, uint32_t n, hipStream_t s) {
+uint64_t cluster_scratch_words(uint32_t max_hits) {
+  uint64_t p2 = 1;
+  while (p2 < max_hits) p2 <<= 1;
+  // key + cl_sum: 2 p2 words; idx, cl_start (+1), cl_gid, g_key, g_min, g_rank, g_sort, tmp
+  // (p2 u32 each) and table (4 p2 u32): 12 p2 + 1 u32
+  return 2 * p2 + 6 * p2 + 1 + 8;
+}
+void launch_cluster_tier2(const ClusterArgs& a, const uint32_t* list, uint32_t n, uint32_t blocks, hipStream_t s) {
   if (n == 0) return;
-  uint32_t blocks = n < 64 ? n : 64;
   hipLaunchKernelGGL(k_cluster_tier2, dim3(blocks), dim3(256), 0, s, a, list, n);
 }
 
@@ -981,12 +1081,11 @@ __global__ __launch_bounds__(64 * DP_WPB) void k_dp_band(DpArgs a) {
       const int32_t sc = (rb == g) ? MATCH_SCORE : mis;
       const int32_t fm = Mp + sc, fi = Ip + sc, fd = Dp + sc;
       int32_t M = max(max(fm, fi), fd);
-      const int32_t Mup = shfl_down_i32(Mp, 1), Iup = shfl_down_i32(Ip, 1);
+      const int32_t Mup = dpp_wave_shl1(NEG_INF, Mp), Iup = dpp_wave_shl1(NEG_INF, Ip);
       const int32_t io = Mup + go, ie = Iup + ge;
       int32_t I = max(io, ie);
       if (!valid) M = NEG_INF;
-      int32_t Ml = shfl_up_i32(M, 1);
-      if (lane == 0) Ml = NEG_INF;
+      const int32_t Ml = dpp_wave_shr1(NEG_INF, M);
       const int32_t A = Ml + go;
       const int32_t d0 = max(0, 1 - (i + hb));
       const int32_t Sent = NEG_INF - (d0 - 1) * ge;
@@ -1081,6 +1180,270 @@ void launch_dp_band(const DpArgs& a0, hipStream_t s) {
   hipLaunchKernelGGL(k_dp_band, dim3(blocks), dim3(64 * a.wpb), lds, s, a);
 }
 
+// ---------------------------------------------------------------------------
+// DP, one lane per job (inter-task): smith_waterman.rs:163-345 for narrow
+// bands (2*band+1 = W <= 63) and shapes with no skipped rows.  Lane-local
+// band state (M, I, D of the previous row) lives in registers as three
+// W-wide arrays indexed by the band column d (j = i + hint - band + d), so
+// the diagonal predecessor is d and the vertical one d+1; the row is swept
+// left to right, so D is a scalar carry.  The per-lane ref window (base and
+// homopolymer gap-open per column) is staged in LDS lane-interleaved; the
+// traceback nibbles (tb_m 2 bits, tb_i, tb_d) go to a per-wave global
+// region, coalesced across lanes.  Columns left of j = 1 are forced to the
+// reset value (NEG_INF) while they are inside the band; columns right of
+// j = n are computed but never read by a cell with j <= n, never reach the
+// endpoint scan and never lie on the traceback path.
+// ---------------------------------------------------------------------------
+template <int W>
+__global__ __launch_bounds__(64) void k_dp_lanes(DpArgs a) {
+  extern __shared__ __align__(16) uint8_t dyn[];
+  int16_t* refw = (int16_t*)dyn;  // [max_n][64]: (go << 8) | base byte
+  const int lane = lane_id();
+  uint32_t* TB = a.tbw + (uint64_t)blockIdx.x * (uint64_t)(a.max_m + 1) * 8 * 64;
+  unsigned long long c_cells = 0;
+  for (uint32_t g = blockIdx.x; g * 64u < a.njobs; g += gridDim.x) {
+    const uint32_t t = g * 64u + (uint32_t)lane;
+    bool active = t < a.njobs;
+    const uint32_t jid = active ? a.jlist[t] : 0u;
+    DpJob job{};
+    if (active) job = a.jobs[jid];
+    uint32_t m = 0, n = 0, qlen = 0;
+    uint64_t so = 0, qo = 0;
+    int32_t h = 0, b = 0;
+    if (active) {
+      so = a.R.seq_off[job.read];
+      qo = a.R.qual_off[job.read];
+      m = (uint32_t)(a.R.seq_off[job.read + 1] - so);
+      qlen = (uint32_t)(a.R.qual_off[job.read + 1] - qo);
+      n = job.n;
+      h = job.hint;
+      b = job.band;
+      // shapes this kernel does not take: hand them to the banded kernel
+      const bool regular = m >= 1 && n >= 1 && 2 * b + 1 == W && h + b >= 0 && (int64_t)m + h - b <= (int64_t)n &&
+                           m <= a.max_m && n <= a.max_n;
+      if (!regular) {
+        a.needs_generic[jid] = 1;
+        active = false;
+        m = 0;
+      }
+    }
+    const int32_t hb = h - b;
+    // stage the ref window: base byte + homopolymer gap-open (smith_waterman.rs:288-291, :348-363);
+    // W entries of padding on both sides hold NOMATCH so every band column reads LDS unconditionally
+    {
+      for (int y = 0; y < W; y++) {
+        refw[y * 64 + lane] = (int16_t)0x00FF;
+        refw[(n + W + y) * 64 + lane] = (int16_t)0x00FF;
+      }
+      int16_t* refx = refw + W * 64;
+      uint32_t x = 0;
+      while (x < n) {
+        const uint8_t base = job.explicit_ref ? a.ref_bytes[job.ref_gb + x] : genome_base(a.G, job.ref_gb + x);
+        uint32_t e = x + 1;  // end of this run within the window
+        while (e < n) {
+          const uint8_t nb = job.explicit_ref ? a.ref_bytes[job.ref_gb + e] : genome_base(a.G, job.ref_gb + e);
+          if (nb != base) break;
+          e++;
+        }
+        int32_t run = (int32_t)(e - x);
+        if (run > 65535) run = 65535;
+        const int32_t go = min(a.gap_open + (run - 1), -1);
+        const int16_t packed = (int16_t)((go << 8) | (int32_t)base);
+        for (uint32_t y = x; y < e; y++) refx[y * 64 + lane] = packed;
+        x = e;
+      }
+    }
+    // wave-uniform row count
+    uint32_t mmax = m;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mmax = max(mmax, (uint32_t)__shfl_xor((int)mmax, o, 64));
+    int32_t Mp[W], Ip[W], Dp[W];
+#pragma unroll
+    for (int d = 0; d < W; d++) {  // row 0 (M = 0 on [0, n]) in row-0 lane coordinates
+      const int32_t col = hb + d;
+      Mp[d] = (col >= 0 && col <= (int32_t)n) ? 0 : NEG_INF;
+      Ip[d] = NEG_INF;
+      Dp[d] = NEG_INF;
+    }
+    const int32_t ge = a.gap_extend;
+    const int16_t* refx = refw + W * 64 + lane;  // entry of ref index x at refx[x * 64]
+    uint64_t cells = 0;
+    // One row of the band; EDGE: some band columns have j < 1 and are forced
+    // to the reset value (no traceback bits).
+    auto row_step = [&](uint32_t i, auto edge_tag) {
+      constexpr bool EDGE = decltype(edge_tag)::value;
+      const int32_t rb = a.R.seq[so + (job.rc ? m - i : i - 1)];
+      const int32_t rbase = job.rc ? comp_base((uint8_t)rb) : rb;
+      const uint32_t qv = (i - 1) < qlen ? a.R.qual[qo + (job.rc ? qlen - i : i - 1)] : 30u;
+      const int32_t mis = a.T.mis_pen[qv];
+      const int32_t base_j = (int32_t)i + hb;  // column of d = 0
+      const int32_t dlo = 1 - base_j;          // first d with j >= 1
+      const int16_t* rrow = refx + (base_j - 1) * 64;
+      int32_t Ml = NEG_INF, Dl = NEG_INF;
+      uint32_t word = 0;
+      uint32_t* row = TB + (uint64_t)i * 8 * 64 + lane;
+#pragma unroll
+      for (int d = 0; d < W; d++) {
+        const int32_t rv = rrow[d * 64];
+        const int32_t go = rv >> 8;
+        const int32_t sc = ((rv & 0xFF) == rbase) ? MATCH_SCORE : mis;
+        const int32_t fm = Mp[d] + sc, fi = Ip[d] + sc, fd = Dp[d] + sc;
+        int32_t M = max(max(fm, fi), fd);
+        const int32_t mu = (d + 1 < W) ? Mp[d + 1] : NEG_INF;
+        const int32_t iu = (d + 1 < W) ? Ip[d + 1] : NEG_INF;
+        const int32_t io = mu + go;
+        int32_t I = max(io, iu + ge);
+        const int32_t dO = Ml + go;
+        int32_t D = max(dO, Dl + ge);
+        uint32_t nib = (M == fm ? 0u : (M == fi ? 1u : 2u)) | (I == io ? 0u : 4u) | (D == dO ? 0u : 8u);
+        if (EDGE && d < dlo) {
+          M = NEG_INF;
+          I = NEG_INF;
+          D = NEG_INF;
+          nib = 0;
+        }
+        Mp[d] = M;
+        Ip[d] = I;
+        Dp[d] = D;
+        Ml = M;
+        Dl = D;
+        word |= nib << (4 * (d & 7));
+        if ((d & 7) == 7 || d == W - 1) {
+          row[(d >> 3) * 64] = word;
+          word = 0;
+        }
+      }
+      const int32_t js = max(base_j, 1), je = min((int32_t)i + h + b, (int32_t)n);
+      cells += (uint64_t)(je - js + 1);
+    };
+    for (uint32_t i = 1; i <= mmax; i++) {
+      const bool live = active && i <= m;
+      const bool edge = live && (1 - ((int32_t)i + hb)) > 0;
+      if (__ballot(edge)) {
+        if (live) row_step(i, std::true_type{});
+      } else {
+        if (live) row_step(i, std::false_type{});
+      }
+    }
+    __threadfence_block();
+    if (active) {
+      // endpoint: last maximum over (j asc; M, I) of row m
+      const int32_t base_j = (int32_t)m + hb;
+      int64_t key = INT64_MIN;
+#pragma unroll
+      for (int d = 0; d < W; d++) {
+        const int32_t j = base_j + d;
+        if (j >= 1 && j <= (int32_t)n) {
+          const int64_t km = end_key(Mp[d], (uint32_t)(2 * j));
+          const int64_t ki = end_key(Ip[d], (uint32_t)(2 * j + 1));
+          const int64_t k2 = km > ki ? km : ki;
+          key = key > k2 ? key : k2;
+        }
+      }
+      const int32_t js = max(base_j, 1), je = max(min((int32_t)m + h + b, (int32_t)n), 0);
+      const int32_t joob = je < (int32_t)n ? (int32_t)n : js - 1;
+      const int64_t koob = end_key(NEG_INF, (uint32_t)(2 * joob + 1));
+      key = key > koob ? key : koob;
+      const int32_t best_score = (int32_t)(key >> 32);
+      uint32_t jj = ((uint32_t)key) >> 1, state = ((uint32_t)key) & 1u;
+      // traceback (smith_waterman.rs:299-345); runs written in reverse into the job's slot
+      uint32_t* cig = a.cigar + (uint64_t)jid * a.cigar_cap;
+      uint32_t ii = m, nr = 0, cur_op = 0xFF, cur_len = 0;
+      uint32_t span = 0;
+      while (ii > 0) {
+        uint32_t op;
+        if (jj == 0) {
+          op = 1;
+          ii--;
+        } else {
+          const int32_t dd = (int32_t)jj - ((int32_t)ii + hb);
+          uint32_t bits = 0;
+          if (dd >= 0 && dd < W) bits = (TB[((uint64_t)ii * 8 + (uint32_t)(dd >> 3)) * 64 + lane] >> (4 * (dd & 7))) & 15u;
+          if (state == 0) {
+            op = 0;
+            ii--;
+            jj--;
+            state = bits & 3u;
+          } else if (state == 1) {
+            op = 1;
+            ii--;
+            state = (bits & 4u) ? 1u : 0u;
+          } else {
+            op = 2;
+            jj--;
+            state = (bits & 8u) ? 2u : 0u;
+          }
+        }
+        if (op != 1) span++;
+        if (op == cur_op) {
+          cur_len++;
+        } else {
+          if (cur_len) cig[nr++] = (cur_len << 4) | cur_op;
+          cur_op = op;
+          cur_len = 1;
+        }
+      }
+      if (cur_len) cig[nr++] = (cur_len << 4) | cur_op;
+      for (uint32_t u = 0; u < nr / 2; u++) {  // forward order
+        const uint32_t tmp = cig[u];
+        cig[u] = cig[nr - 1 - u];
+        cig[nr - 1 - u] = tmp;
+      }
+      const uint32_t off = jj;
+      // alignment_log_likelihood_bases (bayesian.rs:142-179)
+      double lk = 0.0;
+      uint32_t qi = 0, ri = 0;
+      for (uint32_t u = 0; u < nr; u++) {
+        const uint32_t w = cig[u], op = w & 15u, len = w >> 4;
+        if (op == 0) {
+          for (uint32_t v = 0; v < len; v++) {
+            if (qi >= m || ri >= span) break;
+            const uint32_t q = qi < qlen ? a.R.qual[qo + (job.rc ? qlen - 1 - qi : qi)] : 30u;
+            const uint8_t rr = a.R.seq[so + (job.rc ? m - 1 - qi : qi)];
+            const uint8_t qb = job.rc ? comp_base(rr) : rr;
+            const uint8_t gbse = (uint8_t)(refx[(off + ri) * 64] & 0xFF);
+            lk += (qb == gbse) ? a.T.lk_match[q] : a.T.lk_mis[q];
+            qi++;
+            ri++;
+          }
+        } else if (op == 1) {
+          for (uint32_t v = 0; v < len; v++) {
+            if (qi >= m) break;
+            lk += a.T.ln_quarter;
+            qi++;
+          }
+        } else if (op == 2) {
+          ri += len;
+        }
+      }
+      DpOut o;
+      o.log_lk = lk;
+      o.ref_off = off;
+      o.score = best_score;
+      o.cigar_n = nr;
+      o.span = span;
+      o.cells = cells;
+      a.out[jid] = o;
+      c_cells += cells;
+    }
+    __threadfence_block();
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c_cells += (unsigned long long)__shfl_xor((long long)c_cells, o, 64);
+  if (lane == 0 && a.ctr) atomicAdd(&a.ctr->cells, c_cells);
+}
+
+// Lane kernel for the default band (30 -> 61 columns); other widths use k_dp_band.
+bool launch_dp_lanes(const DpArgs& a, uint32_t nwaves, hipStream_t s) {
+  if (a.njobs == 0) return true;
+  size_t lds = (size_t)(a.max_n + 2 * 61) * 64 * 2;
+  if (lds > 64 * 1024) return false;
+  uint32_t blocks = (a.njobs + 63) / 64;
+  if (blocks > nwaves) blocks = nwaves;
+  hipLaunchKernelGGL(k_dp_lanes<61>, dim3(blocks), dim3(64), lds, s, a);
+  return true;
+}
+
 // Any band width: the band of row i, [js, je], is processed in 64-column
 // tiles; the previous row lives in per-wave global scratch (M, I, D arrays
 // over columns 0..n); traceback bits per (row, tile) in global scratch.
@@ -1171,15 +1534,10 @@ __global__ __launch_bounds__(64) void k_dp_tiled(DpArgs a) {
           uI = RI[j];
           uD = RD[j];
         }
-        int32_t dM = shfl_up_i32(uM, 1), dI = shfl_up_i32(uI, 1), dD = shfl_up_i32(uD, 1);
-        if (lane == 0) {
-          dM = cM;
-          dI = cI;
-          dD = cD;
-        }
-        cM = __shfl(uM, 63, 64);
-        cI = __shfl(uI, 63, 64);
-        cD = __shfl(uD, 63, 64);
+        const int32_t dM = dpp_wave_shr1(cM, uM), dI = dpp_wave_shr1(cI, uI), dD = dpp_wave_shr1(cD, uD);
+        cM = readlane_i32(uM, 63);
+        cI = readlane_i32(uI, 63);
+        cD = readlane_i32(uD, 63);
         uint8_t g = 0;
         int32_t go = 0;
         if (valid) {
@@ -1192,8 +1550,7 @@ __global__ __launch_bounds__(64) void k_dp_tiled(DpArgs a) {
         const int32_t io = uM + go, ie = uI + ge;
         const int32_t I = max(io, ie);
         if (!valid) M = NEG_INF;
-        int32_t Ml = shfl_up_i32(M, 1);
-        if (lane == 0) Ml = curMl;
+        const int32_t Ml = dpp_wave_shr1(curMl, M);
         const int32_t A = Ml + go;
         const int32_t Sent = curDl + ge;  // D(tile_start - 1) + ge
         int32_t B = valid ? A - lane * ge : INT32_MIN;
@@ -1213,8 +1570,8 @@ __global__ __launch_bounds__(64) void k_dp_tiled(DpArgs a) {
         }
         // last valid lane's current-row values feed the next tile
         const int lastl = (int)min64((int64_t)63, je - (js + (int64_t)tt * 64));
-        curMl = __shfl(M, lastl, 64);
-        curDl = __shfl(D, lastl, 64);
+        curMl = readlane_i32(M, lastl);
+        curDl = readlane_i32(D, lastl);
         if (valid) {
           RM[j] = M;
           RI[j] = I;

@@ -235,6 +235,29 @@ def test_map_se_repeat_heavy_reads():
     assert st["tier2_cluster_reads"] > 0 or st["seed_overflow_reads"] > 0
 
 
+@pytest.mark.parametrize("n_contigs,lmax", [(60, 30), (150, 60)])
+def test_map_se_many_contig_groups(n_contigs, lmax):
+    """A 400 bp unit in every contig, in alternating orientation: short reads
+    from it hit > 56 and > 128 (contig, strand) groups, which takes the table
+    and block-tier paths of the emulated FxHashMap iteration order."""
+    rng = np.random.default_rng(61 + n_contigs)
+    unit = synth.random_bases(rng, 400)
+    contigs = []
+    for c in range(n_contigs):
+        u = synth.mutate(rng, unit, 0.01)
+        if c % 2:
+            u = synth.revcomp(u)
+        flank = lambda: synth.random_bases(rng, 300)
+        contigs.append(("c%d" % c, np.concatenate([flank(), u, flank()])))
+    go, gp = pair_of_genomes(contigs)
+    seq, so, qual, qo = synth.sample_reads(contigs, 400, 161 + n_contigs, lmin=18, lmax=lmax)
+    ro_b, rp_b = both_reads(seq, so, qual, qo)
+    ro = O.Mapper(go, O.HashTable(go)).map_se(ro_b)
+    ctx = P.Context(gp)
+    rp = ctx.map_se(rp_b)
+    compare_records(ro, rp, label="%d contigs" % n_contigs)
+
+
 def test_map_se_edge_reads():
     """Empty reads, reads shorter than k, all-N reads, quality tracks of other lengths."""
     contigs = synth.small_genome(51, n_contigs=1, length=20000)
