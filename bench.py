@@ -107,7 +107,7 @@ def main():
     gather_gbps = gbytes_per_launch / (ms_seed * 1e-3) / 1e9 if ms_seed > 0 else 0.0
     if agg["ms_k_dp"] >= agg["ms_k_seed"]:
         ach = dp_gcups * DP_OPS_PER_CELL / 1000.0
-        roof = {"kernel": "k_dp_band", "bound": "valu-int32", "achieved": round(ach, 3), "peak": round(PEAK_INT32_TOPS, 2),
+        roof = {"kernel": "k_dp_quad", "bound": "valu-int32", "achieved": round(ach, 3), "peak": round(PEAK_INT32_TOPS, 2),
                 "unit": "Tops/s", "frac": round(ach / PEAK_INT32_TOPS, 4), "traffic": None,
                 "per_launch": {"cells": cells_per_launch, "ms": ms_dp, "ops_per_cell": DP_OPS_PER_CELL}}
     else:
@@ -169,7 +169,8 @@ def main():
         "sample_sam_concordance": concord,
         "stage_ms": {k: round(agg[k], 1) for k in ("ms_seed", "ms_cluster", "ms_dp", "ms_mapq", "ms_total")},
         "counters": {k: int(agg[k]) for k in ("reads", "lookups", "positions", "filtered_hits", "dp_cells",
-                                               "seed_overflow_reads", "tier2_cluster_reads", "generic_dp_jobs")},
+                                               "seed_overflow_reads", "tier2_cluster_reads", "band_dp_jobs",
+                                               "generic_dp_jobs")},
     }
     print(json.dumps(out), flush=True)
 

@@ -119,9 +119,10 @@ typedef struct {
   uint64_t wide_cells;
   uint64_t seed_overflow_reads;
   uint64_t tier2_cluster_reads;
-  uint64_t generic_dp_jobs;
+  uint64_t generic_dp_jobs;  /* jobs handed to the tiled (any-width) DP kernel */
+  uint64_t band_dp_jobs;     /* jobs handed to the one-job-per-wave banded DP kernel */
   double ms_seed, ms_cluster, ms_dp, ms_mapq, ms_pair, ms_total;  /* host wall time per stage */
-  double ms_k_seed, ms_k_dp;  /* HIP-event time of the seed and banded-DP kernels */
+  double ms_k_seed, ms_k_dp;  /* HIP-event time of the seed kernel and the main banded-DP launch */
   uint64_t n_k_seed, n_k_dp;  /* launches timed */
 } pbw_stats;
 

@@ -99,7 +99,7 @@ class _AlignResult(C.Structure):
 class Stats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("reads", "lookups", "positions", "filtered_hits", "dp_jobs", "dp_cells",
                                            "wide_jobs", "wide_cells", "seed_overflow_reads", "tier2_cluster_reads",
-                                           "generic_dp_jobs")] + \
+                                           "generic_dp_jobs", "band_dp_jobs")] + \
                [(n, C.c_double) for n in ("ms_seed", "ms_cluster", "ms_dp", "ms_mapq", "ms_pair", "ms_total",
                                            "ms_k_seed", "ms_k_dp")] + \
                [(n, C.c_uint64) for n in ("n_k_seed", "n_k_dp")]

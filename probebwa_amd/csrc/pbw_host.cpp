@@ -656,8 +656,14 @@ static void batch_candidates(pbw_ctx* c, Batch& b) {
   da.max_m = b.max_m;
   da.max_n = b.max_n;
   if (2 * band + 1 <= 63) {
+    // four jobs per wave for the regular shapes; the flagged rest goes to the
+    // one-job-per-wave banded kernel, and what that flags to the tiled one
+    DpArgs qa = da;
+    qa.max_m = std::min<uint32_t>(b.max_m, QUAD_MAX_M);
+    qa.max_n = std::min<uint32_t>(b.max_n, qa.max_m + (uint32_t)right_pad + LEFT_PAD);
     HIP_CHECK(hipEventRecord(c->ev[2], s));
-    launch_dp_band(da, s);
+    const bool quad = launch_dp_quad(qa, s);
+    if (!quad) launch_dp_band(da, s);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(c->ev[3], s));
     std::vector<uint8_t> gen(nslots);
@@ -672,6 +678,22 @@ static void batch_candidates(pbw_ctx* c, Batch& b) {
     std::vector<uint32_t> list;
     for (size_t k = 0; k < nslots; k++)
       if (gen[k]) list.push_back((uint32_t)k);
+    if (quad && !list.empty()) {
+      c->stats.band_dp_jobs += list.size();
+      HIP_CHECK(hipMemsetAsync(c->d_generic.p, 0, nslots, s));
+      c->d_list.upload(list.data(), list.size(), s);
+      DpArgs ba = da;
+      ba.jlist = c->d_list.p;
+      ba.njobs = (uint32_t)list.size();
+      ba.job_valid = nullptr;
+      launch_dp_band(ba, s);
+      HIP_CHECK(hipGetLastError());
+      HIP_CHECK(hipMemcpyAsync(gen.data(), c->d_generic.p, nslots, hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      list.clear();
+      for (size_t k = 0; k < nslots; k++)
+        if (gen[k]) list.push_back((uint32_t)k);
+    }
     run_dp_tiled(c, da, list);
     c->stats.generic_dp_jobs += list.size();
   } else {

@@ -25,6 +25,7 @@ constexpr int HIT_CAP = 1024;            // hits per read in the first seed pass
 constexpr int CLUSTER_LDS_CAP = 512;     // hits per read clustered in LDS by one wave (else tier 2)
 constexpr int LN_INT_MAX = 4096;         // ln(k) table for phred_n_choose_m
 constexpr int GC_LUT_MAX = 1024;         // (gc, total) table for the GC-adjusted alphabet size
+constexpr int QUAD_MAX_M = 320;         // longest read the four-jobs-per-wave DP kernel takes
 
 struct DevGenome {
   const uint64_t* g2b;    // 2-bit bases, 32 per word, LSB first (A0 C1 G2 T3), contigs back to back
@@ -167,11 +168,10 @@ struct DpArgs {
   uint64_t* tb;            // per wave: (max_m + 1) * tiles * 4
   uint32_t tiles;          // ceil((max_n + 1) / 64)
   uint32_t wpb;            // waves per block (banded kernel; set by the launcher)
-  uint32_t* tbw;           // lane kernel: per-wave traceback words, (max_m + 1) * 8 * 64 each
 };
 void launch_dp_band(const DpArgs& a, hipStream_t s);
 void launch_dp_tiled(const DpArgs& a, uint32_t nwaves, hipStream_t s);
-bool launch_dp_lanes(const DpArgs& a, uint32_t nwaves, hipStream_t s);
+bool launch_dp_quad(const DpArgs& a, hipStream_t s);  // false: layout too large, nothing launched
 
 struct MapqArgs {
   DevGenome G;

@@ -1181,36 +1181,117 @@ void launch_dp_band(const DpArgs& a0, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// DP, one lane per job (inter-task): smith_waterman.rs:163-345 for narrow
-// bands (2*band+1 = W <= 63) and shapes with no skipped rows.  Lane-local
-// band state (M, I, D of the previous row) lives in registers as three
-// W-wide arrays indexed by the band column d (j = i + hint - band + d), so
-// the diagonal predecessor is d and the vertical one d+1; the row is swept
-// left to right, so D is a scalar carry.  The per-lane ref window (base and
-// homopolymer gap-open per column) is staged in LDS lane-interleaved; the
-// traceback nibbles (tb_m 2 bits, tb_i, tb_d) go to a per-wave global
-// region, coalesced across lanes.  Columns left of j = 1 are forced to the
-// reset value (NEG_INF) while they are inside the band; columns right of
-// j = n are computed but never read by a cell with j <= n, never reach the
-// endpoint scan and never lie on the traceback path.
+// DP, four jobs per wave (k_dp_quad): smith_waterman.rs:163-345 for bands of
+// W = 2*band+1 <= 63 columns whose rows are all computed (no skipped row),
+// genome-sourced reference windows and reads of at most QUAD_MAX_M bases.
+// Each job owns one 16-lane DPP row of the wave; lane L holds the columns
+// d' = 4L..4L+3 of a 64-column window that puts the band at [s, s+W),
+// s = 63 - W, and a guard column at d' = 63.  Column d' of row i is the
+// reference column j = i + o + d', o = hint + band - 62, so the diagonal
+// predecessor keeps d', the vertical one is d'+1 and the horizontal one d'-1.
+//  - M and I of the previous row stay in registers, 4 per lane.
+//  - D is the max-plus prefix scan D[d'] = max_k (M[k-1] + go[k] + (d'-k) ge)
+//    over the band, exact in int32: a lane-local prefix over 4 columns, then
+//    an exclusive scan over the 16 lanes with row_shr DPP moves.  The row
+//    keeps P = D - d'*ge.
+//  - Columns left of the band never feed it (the scan excludes them).  The
+//    guard column scores -2^28 per row, so from row 1 on it is unreachable,
+//    as the reference's NEG_INF fill makes it; row 0 keeps 0 there.  Cells
+//    with j <= 0 are reset to NEG_INF in the rows where the band reaches
+//    them.  Cells with j > n are computed but never read by a cell with
+//    j <= n, never reach the endpoint scan and never lie on the traceback
+//    path.  Unreachable values stay within [-2^31, -2^29).
+//  - Traceback bits go into four words per lane with v_alignbit (planes:
+//    M came from I or D; M came from D; tb_i; tb_d) and to LDS every 8 rows.
+//    One lane per job walks them, taking runs of diagonal M steps from one
+//    word, and pushes CIGAR runs into the consumed top of its traceback area.
 // ---------------------------------------------------------------------------
-template <int W>
-__global__ __launch_bounds__(64) void k_dp_lanes(DpArgs a) {
+constexpr int32_t QUAD_GUARD = -(1 << 28);
+constexpr int32_t QUAD_EXCL = -(1 << 29);
+
+struct QuadLayout {  // byte offsets inside one job's LDS area
+  uint32_t tb, go, code, scw, q, qq, ref, stride, tb_words;
+};
+__host__ __device__ inline QuadLayout quad_layout(uint32_t max_m, uint32_t max_n) {
+  QuadLayout l;
+  const uint32_t blocks = (max_m + 7) / 8 + 1;  // + one block for the CIGAR runs
+  l.tb_words = 64 * blocks;
+  l.tb = 0;
+  l.go = l.tb + 4 * l.tb_words;
+  l.code = l.go + 4 * (max_m + 64);
+  l.scw = l.code + 4 * (max_m + 64);
+  l.q = l.scw + 4 * ((max_m > 128 ? max_m : 128) + 4);
+  l.qq = l.q + ((max_m + 15) & ~15u);
+  l.ref = l.qq + ((max_m + 15) & ~15u);
+  l.stride = l.ref + ((max_n + 15) & ~15u);
+  return l;
+}
+
+template <int CTRL>
+__device__ __forceinline__ int32_t dpp_mov(int32_t old, int32_t x) {
+  return __builtin_amdgcn_update_dpp(old, x, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+__device__ __forceinline__ int32_t dpp_mov_z(int32_t x) {  // lanes without a source read 0
+  return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xf, 0xf, true);
+}
+constexpr int DPP_ROW_SHL1 = 0x101;
+constexpr int DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112, DPP_ROW_SHR4 = 0x114, DPP_ROW_SHR8 = 0x118;
+constexpr int DPP_ROW_SHL2 = 0x102, DPP_ROW_SHL4 = 0x104, DPP_ROW_SHL8 = 0x108;
+
+__device__ __forceinline__ uint32_t tb_push_bit(uint32_t w, int32_t x) {  // (w << 1) | sign(x)
+  return __builtin_amdgcn_alignbit(w, (uint32_t)x, 31);
+}
+__device__ __forceinline__ int32_t bfe_i32(int32_t x, int32_t off, int32_t w) {
+  return __builtin_amdgcn_sbfe(x, (uint32_t)off, (uint32_t)w);
+}
+
+// Sum over rows 1..m of the reference's computed cells per row,
+// min(i+h+b, n) - max(i+h-b, 1) + 1 (all rows computed).
+__device__ uint64_t quad_cells(int64_t m, int64_t h, int64_t b, int64_t n) {
+  const int64_t c = h + b;
+  int64_t t = n - c;
+  t = t < 0 ? 0 : (t > m ? m : t);
+  const int64_t hi = t * (t + 1) / 2 + c * t + (m - t) * n;
+  const int64_t c2 = h - b;
+  int64_t u = -c2;
+  u = u < 0 ? 0 : (u > m ? m : u);
+  const int64_t lo = u + (m * (m + 1) / 2 - u * (u + 1) / 2) + c2 * (m - u);
+  return (uint64_t)(hi - lo + m);
+}
+
+__global__ __launch_bounds__(64) void k_dp_quad(DpArgs a) {
   extern __shared__ __align__(16) uint8_t dyn[];
-  int16_t* refw = (int16_t*)dyn;  // [max_n][64]: (go << 8) | base byte
   const int lane = lane_id();
-  uint32_t* TB = a.tbw + (uint64_t)blockIdx.x * (uint64_t)(a.max_m + 1) * 8 * 64;
+  const int g = lane >> 4, L = lane & 15;
+  const QuadLayout Y = quad_layout(a.max_m, a.max_n);
+  uint8_t* jb = dyn + (size_t)g * Y.stride;
+  uint32_t* TB = (uint32_t*)(jb + Y.tb);
+  int32_t* GOA = (int32_t*)(jb + Y.go);    // gap open of window column c at [c - (1 + o)]
+  int32_t* CODA = (int32_t*)(jb + Y.code);  // 6 * (A, C, G, T, N -> 0..4) of the same column
+  int32_t* SCW = (int32_t*)(jb + Y.scw);
+  uint8_t* Q = jb + Y.q;
+  uint8_t* QQ = jb + Y.qq;
+  uint8_t* REF = jb + Y.ref;
+  const int32_t ge = a.gap_extend;
   unsigned long long c_cells = 0;
-  for (uint32_t g = blockIdx.x; g * 64u < a.njobs; g += gridDim.x) {
-    const uint32_t t = g * 64u + (uint32_t)lane;
-    bool active = t < a.njobs;
-    const uint32_t jid = active ? a.jlist[t] : 0u;
+  const uint32_t nwaves = gridDim.x;
+  const uint32_t nquads = (a.njobs + 3) / 4;
+  for (uint32_t wq = blockIdx.x; wq < nquads; wq += nwaves) {
+    // ---- this group's job and its shape ----
+    const uint32_t t = wq * 4 + (uint32_t)g;
+    bool valid = t < a.njobs;
+    uint32_t jid = 0;
+    if (valid) {
+      jid = a.jlist ? a.jlist[t] : t;
+      if (a.job_valid && (jid % TOP_CANDIDATES) >= a.job_valid[jid / TOP_CANDIDATES]) valid = false;
+    }
     DpJob job{};
-    if (active) job = a.jobs[jid];
-    uint32_t m = 0, n = 0, qlen = 0;
+    uint32_t m = 0, qlen = 0, n = 0;
     uint64_t so = 0, qo = 0;
     int32_t h = 0, b = 0;
-    if (active) {
+    if (valid) {
+      job = a.jobs[jid];
       so = a.R.seq_off[job.read];
       qo = a.R.qual_off[job.read];
       m = (uint32_t)(a.R.seq_off[job.read + 1] - so);
@@ -1218,229 +1299,436 @@ __global__ __launch_bounds__(64) void k_dp_lanes(DpArgs a) {
       n = job.n;
       h = job.hint;
       b = job.band;
-      // shapes this kernel does not take: hand them to the banded kernel
-      const bool regular = m >= 1 && n >= 1 && 2 * b + 1 == W && h + b >= 0 && (int64_t)m + h - b <= (int64_t)n &&
-                           m <= a.max_m && n <= a.max_n;
+      const bool regular = m >= 1 && n >= 1 && m <= a.max_m && n <= a.max_n && b >= 0 && b <= 31 &&
+                           !job.explicit_ref && (int64_t)h + b >= 0 && (int64_t)m + h - b <= (int64_t)n;
       if (!regular) {
-        a.needs_generic[jid] = 1;
-        active = false;
-        m = 0;
+        if (L == 0) a.needs_generic[jid] = 1;
+        valid = false;
       }
     }
-    const int32_t hb = h - b;
-    // stage the ref window: base byte + homopolymer gap-open (smith_waterman.rs:288-291, :348-363);
-    // W entries of padding on both sides hold NOMATCH so every band column reads LDS unconditionally
+    if (!valid) {
+      m = 0;
+      n = 0;
+      h = 0;
+      b = 0;
+    }
+    const int32_t W = 2 * b + 1, s = 63 - W, o = h + b - 62;
+    const uint32_t ql = qlen < m ? qlen : m;
+    // ---- staging: query (reverse complement for reverse-strand jobs, single.rs:101-103) ----
+    for (uint32_t x = L; x < m; x += 16) Q[x] = job.rc ? comp_base(a.R.seq[so + m - 1 - x]) : a.R.seq[so + x];
+    for (uint32_t x = L; x < ql; x += 16) QQ[x] = job.rc ? a.R.qual[qo + qlen - 1 - x] : a.R.qual[qo + x];
+    // reference window: 2-bit words and N-mask words through LDS, then bytes
+    uint64_t* gw = (uint64_t*)SCW;
+    const uint64_t gb0 = job.ref_gb;
+    const uint32_t nw = m ? (uint32_t)(((gb0 & 31) + n + 31) >> 5) : 0;
+    const uint32_t nwm = m ? (uint32_t)(((gb0 & 63) + n + 63) >> 6) : 0;
+    for (uint32_t x = L; x < nw; x += 16) gw[x] = a.G.g2b[(gb0 >> 5) + x];
+    for (uint32_t x = L; x < nwm; x += 16) gw[nw + x] = a.G.nmask[(gb0 >> 6) + x];
+    wave_sync();
+    for (uint32_t x = L; x < n && m; x += 16) {
+      const uint64_t p = (gb0 & 31) + x, pm = (gb0 & 63) + x;
+      const uint32_t c2 = (uint32_t)(gw[p >> 5] >> ((p & 31) * 2)) & 3u;
+      const bool isn = (gw[nw + (pm >> 6)] >> (pm & 63)) & 1ull;
+      REF[x] = isn ? (uint8_t)'N' : (uint8_t)("ACGT"[c2]);
+    }
+    wave_sync();
+    // homopolymer run of every column (smith_waterman.rs:348-363): each lane
+    // scans a contiguous segment; run starts carry in by a prefix max and run
+    // ends by a suffix min over the 16 lanes
     {
-      for (int y = 0; y < W; y++) {
-        refw[y * 64 + lane] = (int16_t)0x00FF;
-        refw[(n + W + y) * 64 + lane] = (int16_t)0x00FF;
+      const uint32_t seg = (n + 15) / 16;
+      const uint32_t x0 = min(n, L * seg), x1 = min(n, x0 + seg);
+      int32_t lb = -1, fe = INT32_MAX;
+      for (uint32_t x = x0; x < x1; x++) {
+        if (x == 0 || REF[x] != REF[x - 1]) lb = (int32_t)x;
+        if (fe == INT32_MAX && (x == n - 1 || REF[x] != REF[x + 1])) fe = (int32_t)x;
       }
-      int16_t* refx = refw + W * 64;
-      uint32_t x = 0;
-      while (x < n) {
-        const uint8_t base = job.explicit_ref ? a.ref_bytes[job.ref_gb + x] : genome_base(a.G, job.ref_gb + x);
-        uint32_t e = x + 1;  // end of this run within the window
-        while (e < n) {
-          const uint8_t nb = job.explicit_ref ? a.ref_bytes[job.ref_gb + e] : genome_base(a.G, job.ref_gb + e);
-          if (nb != base) break;
-          e++;
+      int32_t pm = lb;
+      pm = max(pm, dpp_mov<DPP_ROW_SHR1>(-1, pm));
+      pm = max(pm, dpp_mov<DPP_ROW_SHR2>(-1, pm));
+      pm = max(pm, dpp_mov<DPP_ROW_SHR4>(-1, pm));
+      pm = max(pm, dpp_mov<DPP_ROW_SHR8>(-1, pm));
+      int32_t start = dpp_mov<DPP_ROW_SHR1>(-1, pm);
+      int32_t sm = fe;
+      sm = min(sm, dpp_mov<DPP_ROW_SHL1>(INT32_MAX, sm));
+      sm = min(sm, dpp_mov<DPP_ROW_SHL2>(INT32_MAX, sm));
+      sm = min(sm, dpp_mov<DPP_ROW_SHL4>(INT32_MAX, sm));
+      sm = min(sm, dpp_mov<DPP_ROW_SHL8>(INT32_MAX, sm));
+      int32_t end = dpp_mov<DPP_ROW_SHL1>(INT32_MAX, sm);
+      // window columns c = x + 1 at index c - (1 + o); default for c outside [1, n]
+      const int32_t c_lo = 1 + o;
+      const int32_t nent = (int32_t)m + 63;
+      for (int32_t e = L; e < nent && m; e += 16) {
+        GOA[e] = -1;
+        CODA[e] = 24;
+      }
+      wave_sync();
+      for (uint32_t x = x0; x < x1; x++) {  // run starts, ascending
+        if (x == 0 || REF[x] != REF[x - 1]) start = (int32_t)x;
+        const int32_t e = (int32_t)x + 1 - c_lo;
+        if (e >= 0 && e < nent) GOA[e] = start;  // temporarily the run start
+      }
+      for (int32_t x = (int32_t)x1 - 1; x >= (int32_t)x0; x--) {  // run ends, descending
+        if (x == (int32_t)n - 1 || REF[x] != REF[x + 1]) end = x;
+        const int32_t e = x + 1 - c_lo;
+        if (e >= 0 && e < nent) {
+          int32_t run = end - GOA[e] + 1;
+          if (run > 65535) run = 65535;
+          const int32_t go = min(a.gap_open + (run - 1), -1);
+          const uint8_t rb = REF[x];
+          const int32_t code6 = rb == 'A' ? 0 : rb == 'C' ? 6 : rb == 'G' ? 12 : rb == 'T' ? 18 : 24;
+          GOA[e] = go;
+          CODA[e] = code6;
         }
-        int32_t run = (int32_t)(e - x);
-        if (run > 65535) run = 65535;
-        const int32_t go = min(a.gap_open + (run - 1), -1);
-        const int16_t packed = (int16_t)((go << 8) | (int32_t)base);
-        for (uint32_t y = x; y < e; y++) refx[y * 64 + lane] = packed;
-        x = e;
       }
     }
-    // wave-uniform row count
-    uint32_t mmax = m;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) mmax = max(mmax, (uint32_t)__shfl_xor((int)mmax, o, 64));
-    int32_t Mp[W], Ip[W], Dp[W];
-#pragma unroll
-    for (int d = 0; d < W; d++) {  // row 0 (M = 0 on [0, n]) in row-0 lane coordinates
-      const int32_t col = hb + d;
-      Mp[d] = (col >= 0 && col <= (int32_t)n) ? 0 : NEG_INF;
-      Ip[d] = NEG_INF;
-      Dp[d] = NEG_INF;
-    }
-    const int32_t ge = a.gap_extend;
-    const int16_t* refx = refw + W * 64 + lane;  // entry of ref index x at refx[x * 64]
-    uint64_t cells = 0;
-    // One row of the band; EDGE: some band columns have j < 1 and are forced
-    // to the reset value (no traceback bits).
-    auto row_step = [&](uint32_t i, auto edge_tag) {
-      constexpr bool EDGE = decltype(edge_tag)::value;
-      const int32_t rb = a.R.seq[so + (job.rc ? m - i : i - 1)];
-      const int32_t rbase = job.rc ? comp_base((uint8_t)rb) : rb;
-      const uint32_t qv = (i - 1) < qlen ? a.R.qual[qo + (job.rc ? qlen - i : i - 1)] : 30u;
+    wave_sync();
+    // per-row score word: 6-bit signed fields for reference A, C, G, T, N
+    for (uint32_t x = L; x < m; x += 16) {
+      const uint8_t rb = Q[x];
+      const uint32_t qv = x < qlen ? QQ[x] : 30u;
       const int32_t mis = a.T.mis_pen[qv];
-      const int32_t base_j = (int32_t)i + hb;  // column of d = 0
-      const int32_t dlo = 1 - base_j;          // first d with j >= 1
-      const int16_t* rrow = refx + (base_j - 1) * 64;
-      int32_t Ml = NEG_INF, Dl = NEG_INF;
-      uint32_t word = 0;
-      uint32_t* row = TB + (uint64_t)i * 8 * 64 + lane;
+      uint32_t w = 0;
+      w |= (uint32_t)((rb == 'A' ? MATCH_SCORE : mis) & 63) << 0;
+      w |= (uint32_t)((rb == 'C' ? MATCH_SCORE : mis) & 63) << 6;
+      w |= (uint32_t)((rb == 'G' ? MATCH_SCORE : mis) & 63) << 12;
+      w |= (uint32_t)((rb == 'T' ? MATCH_SCORE : mis) & 63) << 18;
+      w |= (uint32_t)((rb == 'N' ? MATCH_SCORE : mis) & 63) << 24;
+      SCW[x] = (int32_t)w;
+    }
+    wave_sync();
+    // ---- row 0 and per-column constants ----
+    int32_t Mo[4], Io[4], Po[4], kge[4], cst[4];
+    const int32_t cp = (L == 15) ? QUAD_GUARD : 0;
 #pragma unroll
-      for (int d = 0; d < W; d++) {
-        const int32_t rv = rrow[d * 64];
-        const int32_t go = rv >> 8;
-        const int32_t sc = ((rv & 0xFF) == rbase) ? MATCH_SCORE : mis;
-        const int32_t fm = Mp[d] + sc, fi = Ip[d] + sc, fd = Dp[d] + sc;
-        int32_t M = max(max(fm, fi), fd);
-        const int32_t mu = (d + 1 < W) ? Mp[d + 1] : NEG_INF;
-        const int32_t iu = (d + 1 < W) ? Ip[d + 1] : NEG_INF;
-        const int32_t io = mu + go;
-        int32_t I = max(io, iu + ge);
-        const int32_t dO = Ml + go;
-        int32_t D = max(dO, Dl + ge);
-        uint32_t nib = (M == fm ? 0u : (M == fi ? 1u : 2u)) | (I == io ? 0u : 4u) | (D == dO ? 0u : 8u);
-        if (EDGE && d < dlo) {
-          M = NEG_INF;
-          I = NEG_INF;
-          D = NEG_INF;
-          nib = 0;
-        }
-        Mp[d] = M;
-        Ip[d] = I;
-        Dp[d] = D;
-        Ml = M;
-        Dl = D;
-        word |= nib << (4 * (d & 7));
-        if ((d & 7) == 7 || d == W - 1) {
-          row[(d >> 3) * 64] = word;
-          word = 0;
-        }
-      }
-      const int32_t js = max(base_j, 1), je = min((int32_t)i + h + b, (int32_t)n);
-      cells += (uint64_t)(je - js + 1);
+    for (int k = 0; k < 4; k++) {
+      const int32_t d = 4 * L + k;
+      kge[k] = d * ge;
+      cst[k] = -d * ge + (d <= s ? QUAD_EXCL : 0);
+      const int32_t j0 = o + d;
+      Mo[k] = (d >= s && j0 >= 0 && j0 <= (int32_t)n) ? 0 : NEG_INF;
+      Io[k] = NEG_INF;
+      Po[k] = NEG_INF - kge[k];
+    }
+    const int32_t kge3 = kge[3] + cp;
+    uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+    uint32_t mmax = m;
+    int32_t emax = m ? b - h : 0;  // rows i <= band - hint reach columns j <= 0
+    for (int gg = 0; gg < 64; gg += 16) {
+      mmax = max(mmax, (uint32_t)readlane_i32((int32_t)m, gg));
+      emax = max(emax, readlane_i32(m ? b - h : 0, gg));
+    }
+    // Inputs of one row: score word and, per column of this lane, the gap
+    // open and the base code.  Row i+1's inputs are loaded while row i runs.
+    struct RowIn {
+      int32_t scw, g0, g1, g2, g3, c0, c1, c2, c3;
     };
-    for (uint32_t i = 1; i <= mmax; i++) {
-      const bool live = active && i <= m;
-      const bool edge = live && (1 - ((int32_t)i + hb)) > 0;
-      if (__ballot(edge)) {
-        if (live) row_step(i, std::true_type{});
-      } else {
-        if (live) row_step(i, std::false_type{});
+    auto load_row = [&](RowIn& x, uint32_t i) {  // in bounds for i <= m + 1
+      const uint32_t e = i - 1 + 4 * (uint32_t)L;
+      x.scw = SCW[i - 1];
+      x.g0 = GOA[e];
+      x.g1 = GOA[e + 1];
+      x.g2 = GOA[e + 2];
+      x.g3 = GOA[e + 3];
+      x.c0 = CODA[e];
+      x.c1 = CODA[e + 1];
+      x.c2 = CODA[e + 2];
+      x.c3 = CODA[e + 3];
+    };
+    const int32_t cp2 = (L == 15) ? QUAD_GUARD : 0;  // the guard column's vertical neighbour
+    // One row of the band.  EDGE: reset cells with j <= 0 (rows i <= band - hint).
+    auto row = [&](uint32_t i, const RowIn& R, auto edge_tag) {
+      constexpr bool EDGE = decltype(edge_tag)::value;
+      // out-of-row lanes read 0: the guard column adds its sentinel, lane 0's
+      // left neighbour is excluded from the scan
+      const int32_t mu3 = dpp_mov_z<DPP_ROW_SHL1>(Mo[0]);
+      const int32_t iu3 = dpp_mov_z<DPP_ROW_SHL1>(Io[0]);
+      int32_t M0, M1, M2, M3, I0, I1, I2, I3;
+      {
+        const int32_t sc = bfe_i32(R.scw, R.c0, 6);
+        const int32_t fm = Mo[0] + sc, fi = Io[0] + sc, fd = Po[0] + kge[0] + sc;
+        const int32_t tt = max(fm, fi);
+        M0 = max(tt, fd);
+        const int32_t io = Mo[1] + R.g0;
+        I0 = max(io, Io[1] + ge);
+        w0 = tb_push_bit(w0, fm - M0);
+        w1 = tb_push_bit(w1, tt - M0);
+        w2 = tb_push_bit(w2, io - I0);
+      }
+      {
+        const int32_t sc = bfe_i32(R.scw, R.c1, 6);
+        const int32_t fm = Mo[1] + sc, fi = Io[1] + sc, fd = Po[1] + kge[1] + sc;
+        const int32_t tt = max(fm, fi);
+        M1 = max(tt, fd);
+        const int32_t io = Mo[2] + R.g1;
+        I1 = max(io, Io[2] + ge);
+        w0 = tb_push_bit(w0, fm - M1);
+        w1 = tb_push_bit(w1, tt - M1);
+        w2 = tb_push_bit(w2, io - I1);
+      }
+      {
+        const int32_t sc = bfe_i32(R.scw, R.c2, 6);
+        const int32_t fm = Mo[2] + sc, fi = Io[2] + sc, fd = Po[2] + kge[2] + sc;
+        const int32_t tt = max(fm, fi);
+        M2 = max(tt, fd);
+        const int32_t io = Mo[3] + R.g2;
+        I2 = max(io, Io[3] + ge);
+        w0 = tb_push_bit(w0, fm - M2);
+        w1 = tb_push_bit(w1, tt - M2);
+        w2 = tb_push_bit(w2, io - I2);
+      }
+      {
+        const int32_t sc = bfe_i32(R.scw, R.c3, 6);
+        const int32_t fm = Mo[3] + sc + cp, fi = Io[3] + sc + cp, fd = Po[3] + kge3 + sc;
+        const int32_t tt = max(fm, fi);
+        M3 = max(tt, fd);
+        const int32_t io = mu3 + R.g3 + cp2;
+        I3 = max(io, iu3 + ge + cp2);
+        w0 = tb_push_bit(w0, fm - M3);
+        w1 = tb_push_bit(w1, tt - M3);
+        w2 = tb_push_bit(w2, io - I3);
+      }
+      const int32_t jl = (int32_t)i + o + 4 * L;  // j of column k is jl + k
+      if (EDGE) {
+        if (jl + 0 <= 0) M0 = I0 = NEG_INF;
+        if (jl + 1 <= 0) M1 = I1 = NEG_INF;
+        if (jl + 2 <= 0) M2 = I2 = NEG_INF;
+        if (jl + 3 <= 0) M3 = I3 = NEG_INF;
+      }
+      // D: max-plus prefix scan along the row
+      const int32_t ml0 = dpp_mov_z<DPP_ROW_SHR1>(M3);
+      const int32_t B0 = ml0 + R.g0 + cst[0], B1 = M0 + R.g1 + cst[1], B2 = M1 + R.g2 + cst[2],
+                    B3 = M2 + R.g3 + cst[3];
+      const int32_t L0 = B0, L1 = max(L0, B1), L2 = max(L1, B2), L3 = max(L2, B3);
+      int32_t x = L3;
+      x = max(x, dpp_mov<DPP_ROW_SHR1>(INT32_MIN, x));
+      x = max(x, dpp_mov<DPP_ROW_SHR2>(INT32_MIN, x));
+      x = max(x, dpp_mov<DPP_ROW_SHR4>(INT32_MIN, x));
+      x = max(x, dpp_mov<DPP_ROW_SHR8>(INT32_MIN, x));
+      const int32_t carry = dpp_mov<DPP_ROW_SHR1>(INT32_MIN, x);
+      int32_t P0 = max(L0, carry), P1 = max(L1, carry), P2 = max(L2, carry), P3 = max(L3, carry);
+      w3 = tb_push_bit(w3, B0 - P0);
+      w3 = tb_push_bit(w3, B1 - P1);
+      w3 = tb_push_bit(w3, B2 - P2);
+      w3 = tb_push_bit(w3, B3 - P3);
+      if (EDGE) {
+        if (jl + 0 <= 0) P0 = NEG_INF - kge[0];
+        if (jl + 1 <= 0) P1 = NEG_INF - kge[1];
+        if (jl + 2 <= 0) P2 = NEG_INF - kge[2];
+        if (jl + 3 <= 0) P3 = NEG_INF - kge[3];
+      }
+      Mo[0] = M0;
+      Mo[1] = M1;
+      Mo[2] = M2;
+      Mo[3] = M3;
+      Io[0] = I0;
+      Io[1] = I1;
+      Io[2] = I2;
+      Io[3] = I3;
+      Po[0] = P0;
+      Po[1] = P1;
+      Po[2] = P2;
+      Po[3] = P3;
+      if ((i & 7u) == 0) {
+        uint32_t* tw = TB + (((i - 1) >> 3) * 4) * 16 + L;
+        tw[0] = w0;
+        tw[16] = w1;
+        tw[32] = w2;
+        tw[48] = w3;
+        w0 = w1 = w2 = w3 = 0;
+      }
+    };
+    RowIn RA, RB;
+    load_row(RA, 1);
+    uint32_t i = 1;
+    // rows the band reaches j <= 0 in, then the rest; two rows per trip so the
+    // prefetch alternates between RA and RB without copies
+    const uint32_t elim = (uint32_t)min((int32_t)mmax, emax);
+    for (; i + 1 <= elim; i += 2) {
+      load_row(RB, i + 1);
+      if (i <= m) row(i, RA, std::true_type{});
+      load_row(RA, i + 2);
+      if (i + 1 <= m) row(i + 1, RB, std::true_type{});
+    }
+    if (i <= elim) {
+      load_row(RB, i + 1);
+      if (i <= m) row(i, RA, std::true_type{});
+      RA = RB;
+      i++;
+    }
+    for (; i + 1 <= mmax; i += 2) {
+      load_row(RB, i + 1);
+      if (i <= m) row(i, RA, std::false_type{});
+      load_row(RA, i + 2);
+      if (i + 1 <= m) row(i + 1, RB, std::false_type{});
+    }
+    if (i <= mmax) {
+      load_row(RB, i + 1);
+      if (i <= m) row(i, RA, std::false_type{});
+    }
+    if (m & 7u) {
+      const uint32_t sh = 4 * (8 - (m & 7u));
+      uint32_t* tw = TB + (((m - 1) >> 3) * 4) * 16 + L;
+      tw[0] = w0 << sh;
+      tw[16] = w1 << sh;
+      tw[32] = w2 << sh;
+      tw[48] = w3 << sh;
+    }
+    // ---- endpoint: last maximum over (j asc; M then I) of row m ----
+    int64_t key = end_key(NEG_INF, 2 * n + 1);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int32_t d = 4 * L + k, j = (int32_t)m + o + d;
+      if (m && d >= s && d < s + W && j >= 1 && j <= (int32_t)n) {
+        const int64_t km = end_key(Mo[k], (uint32_t)(2 * j)), ki = end_key(Io[k], (uint32_t)(2 * j + 1));
+        key = key > km ? key : km;
+        key = key > ki ? key : ki;
       }
     }
-    __threadfence_block();
-    if (active) {
-      // endpoint: last maximum over (j asc; M, I) of row m
-      const int32_t base_j = (int32_t)m + hb;
-      int64_t key = INT64_MIN;
-#pragma unroll
-      for (int d = 0; d < W; d++) {
-        const int32_t j = base_j + d;
-        if (j >= 1 && j <= (int32_t)n) {
-          const int64_t km = end_key(Mp[d], (uint32_t)(2 * j));
-          const int64_t ki = end_key(Ip[d], (uint32_t)(2 * j + 1));
-          const int64_t k2 = km > ki ? km : ki;
-          key = key > k2 ? key : k2;
+    wave_sync();
+    int64_t* keys = (int64_t*)SCW;
+    keys[L] = key;
+    wave_sync();
+    // ---- traceback (smith_waterman.rs:299-345) by lane 0 of the group ----
+    // Each step takes a whole run from one traceback word: diagonal M steps
+    // (rows r, r-1, .. of a column), horizontal D steps (columns k, k-1, ..
+    // of a row) or vertical I steps (cells (r, k), (r-1, k+1), ..).  The
+    // selection is branch-free; the pending CIGAR run is rewritten in place
+    // every step and the slot advances when the operation changes.
+    uint32_t nruns = 0, off = 0;
+    int64_t best = 0;
+    uint32_t* runs_top = TB + Y.tb_words;
+    if (L == 0 && m) {
+      best = keys[0];
+      for (int u = 1; u < 16; u++) best = best > keys[u] ? best : keys[u];
+      uint32_t i = m, j = ((uint32_t)best) >> 1, st = ((uint32_t)best) & 1u;
+      uint32_t cur_op = 0xFFu, cur_len = 0;
+      int32_t cb = -1, cl = -1;
+      uint32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+      while (i > 0) {
+        int32_t d = (int32_t)j - (int32_t)i - o;
+        d = d < 0 ? 0 : (d > 63 ? 63 : d);  // cells on the path always lie in the band
+        const int32_t lq = d >> 2, rb = (int32_t)(i - 1) >> 3;
+        if (rb != cb || lq != cl) {
+          const uint32_t* tw = TB + (rb * 4) * 16 + lq;
+          v0 = tw[0];
+          v1 = tw[16];
+          v2 = tw[32];
+          v3 = tw[48];
+          cb = rb;
+          cl = lq;
         }
+        const uint32_t k = (uint32_t)d & 3u, r = (i - 1) & 7u;
+        const uint32_t pos = 31u - (4u * r + k);
+        // M: stops after the first row whose M came from I or D
+        const uint32_t x = (v0 >> pos) & 0x11111111u;
+        const uint32_t f = x ? ((uint32_t)__builtin_ctz(x) >> 2) : 32u;
+        const uint32_t mst = min(min(f + 1u, r + 1u), j);
+        const uint32_t mpar = (mst == f + 1u) ? 1u + ((v1 >> ((pos + 4u * f) & 31u)) & 1u) : 0u;
+        // D: stops after the first column whose D was opened
+        const uint32_t y = ~(v3 >> pos) & ((2u << k) - 1u);
+        const uint32_t fz = y ? (uint32_t)__builtin_ctz(y) : 32u;
+        const uint32_t dst = min(min(fz + 1u, k + 1u), j);
+        const uint32_t dpar = (dst == fz + 1u) ? 0u : 2u;
+        // I: stops after the first cell whose I was opened (bits pos, pos+3, ..)
+        const uint32_t tmax = min(3u - k, r);
+        const uint32_t zi = ~(v2 >> pos) & 0x249u & ((8u << (3u * tmax)) - 1u);
+        const uint32_t fi = zi ? (uint32_t)__builtin_ctz(zi) / 3u : 32u;
+        const uint32_t ist = min(fi + 1u, tmax + 1u);
+        const uint32_t ipar = (ist == fi + 1u) ? 0u : 1u;
+        // select: reference column 0 forces a single I step with the state kept
+        const uint32_t sel = (j == 0) ? 1u : st;
+        const uint32_t isM = 0u - (uint32_t)(sel == 0u), isD = 0u - (uint32_t)(sel == 2u);
+        const uint32_t isI = ~(isM | isD);
+        const uint32_t jz = 0u - (uint32_t)(j == 0);
+        const uint32_t len = (mst & isM) | (dst & isD) | (((ist & ~jz) | (1u & jz)) & isI);
+        const uint32_t nst = (mpar & isM) | (dpar & isD) | (((ipar & ~jz) | (st & jz)) & isI);
+        i -= len & ~isD;
+        j -= len & ~isI;
+        st = nst;
+        const uint32_t fresh = 0u - (uint32_t)(sel != cur_op);
+        nruns += (fresh & (uint32_t)(cur_len != 0)) ? 1u : 0u;
+        cur_len = (len & fresh) | ((cur_len + len) & ~fresh);
+        cur_op = sel;
+        runs_top[-1 - (int32_t)nruns] = (cur_len << 4) | cur_op;
       }
-      const int32_t js = max(base_j, 1), je = max(min((int32_t)m + h + b, (int32_t)n), 0);
-      const int32_t joob = je < (int32_t)n ? (int32_t)n : js - 1;
-      const int64_t koob = end_key(NEG_INF, (uint32_t)(2 * joob + 1));
-      key = key > koob ? key : koob;
-      const int32_t best_score = (int32_t)(key >> 32);
-      uint32_t jj = ((uint32_t)key) >> 1, state = ((uint32_t)key) & 1u;
-      // traceback (smith_waterman.rs:299-345); runs written in reverse into the job's slot
-      uint32_t* cig = a.cigar + (uint64_t)jid * a.cigar_cap;
-      uint32_t ii = m, nr = 0, cur_op = 0xFF, cur_len = 0;
-      uint32_t span = 0;
-      while (ii > 0) {
-        uint32_t op;
-        if (jj == 0) {
-          op = 1;
-          ii--;
-        } else {
-          const int32_t dd = (int32_t)jj - ((int32_t)ii + hb);
-          uint32_t bits = 0;
-          if (dd >= 0 && dd < W) bits = (TB[((uint64_t)ii * 8 + (uint32_t)(dd >> 3)) * 64 + lane] >> (4 * (dd & 7))) & 15u;
-          if (state == 0) {
-            op = 0;
-            ii--;
-            jj--;
-            state = bits & 3u;
-          } else if (state == 1) {
-            op = 1;
-            ii--;
-            state = (bits & 4u) ? 1u : 0u;
-          } else {
-            op = 2;
-            jj--;
-            state = (bits & 8u) ? 2u : 0u;
-          }
+      nruns += cur_len != 0 ? 1u : 0u;
+      off = j;
+    }
+    wave_sync();
+    nruns = (uint32_t)__shfl((int)nruns, lane & ~15, 64);
+    off = (uint32_t)__shfl((int)off, lane & ~15, 64);
+    const uint32_t* runs = runs_top - nruns;
+    // ---- alignment log-likelihood terms (single.rs:228-262), one per read
+    //      position, in parallel into the consumed bottom of the traceback area:
+    //      M columns by base match, I columns ln(1/4) ----
+    double* terms = (double*)TB;
+    {
+      uint32_t u = 0, qs = 0, rs = 0;
+      for (uint32_t qi = L; qi < m; qi += 16) {
+        uint32_t op = 1u;
+        while (u < nruns) {  // the run that holds read position qi
+          const uint32_t w = runs[u], len = w >> 4;
+          op = w & 15u;
+          const uint32_t qc = op <= 1u ? len : 0u;
+          if (qi < qs + qc) break;
+          qs += qc;
+          rs += (op == 0u || op == 2u) ? len : 0u;
+          u++;
         }
-        if (op != 1) span++;
-        if (op == cur_op) {
-          cur_len++;
-        } else {
-          if (cur_len) cig[nr++] = (cur_len << 4) | cur_op;
-          cur_op = op;
-          cur_len = 1;
+        double tv = a.T.ln_quarter;
+        if (op == 0u) {
+          const uint32_t qv = qi < qlen ? QQ[qi] : 30u;
+          tv = (Q[qi] == REF[off + rs + (qi - qs)]) ? a.T.lk_match[qv] : a.T.lk_mis[qv];
         }
+        terms[qi] = tv;
       }
-      if (cur_len) cig[nr++] = (cur_len << 4) | cur_op;
-      for (uint32_t u = 0; u < nr / 2; u++) {  // forward order
-        const uint32_t tmp = cig[u];
-        cig[u] = cig[nr - 1 - u];
-        cig[nr - 1 - u] = tmp;
-      }
-      const uint32_t off = jj;
-      // alignment_log_likelihood_bases (bayesian.rs:142-179)
+    }
+    wave_sync();
+    if (L == 0 && m) {
       double lk = 0.0;
-      uint32_t qi = 0, ri = 0;
-      for (uint32_t u = 0; u < nr; u++) {
-        const uint32_t w = cig[u], op = w & 15u, len = w >> 4;
-        if (op == 0) {
-          for (uint32_t v = 0; v < len; v++) {
-            if (qi >= m || ri >= span) break;
-            const uint32_t q = qi < qlen ? a.R.qual[qo + (job.rc ? qlen - 1 - qi : qi)] : 30u;
-            const uint8_t rr = a.R.seq[so + (job.rc ? m - 1 - qi : qi)];
-            const uint8_t qb = job.rc ? comp_base(rr) : rr;
-            const uint8_t gbse = (uint8_t)(refx[(off + ri) * 64] & 0xFF);
-            lk += (qb == gbse) ? a.T.lk_match[q] : a.T.lk_mis[q];
-            qi++;
-            ri++;
-          }
-        } else if (op == 1) {
-          for (uint32_t v = 0; v < len; v++) {
-            if (qi >= m) break;
-            lk += a.T.ln_quarter;
-            qi++;
-          }
-        } else if (op == 2) {
-          ri += len;
-        }
+      for (uint32_t qi = 0; qi < m; qi++) lk += terms[qi];  // read order, as the reference sums
+      uint32_t span = 0;
+      for (uint32_t u = 0; u < nruns; u++) {
+        const uint32_t w = runs[u], op = w & 15u;
+        if (op == 0u || op == 2u) span += w >> 4;
       }
-      DpOut o;
-      o.log_lk = lk;
-      o.ref_off = off;
-      o.score = best_score;
-      o.cigar_n = nr;
-      o.span = span;
-      o.cells = cells;
-      a.out[jid] = o;
+      const uint64_t cells = quad_cells(m, h, b, n);
       c_cells += cells;
+      DpOut oo;
+      oo.log_lk = lk;
+      oo.ref_off = off;
+      oo.score = (int32_t)(best >> 32);
+      oo.cigar_n = nruns;
+      oo.span = span;
+      oo.cells = cells;
+      a.out[jid] = oo;
     }
-    __threadfence_block();
+    // ---- CIGAR runs to the job's slot, forward order ----
+    {
+      const uint32_t ncopy = nruns < a.cigar_cap ? nruns : a.cigar_cap;
+      for (uint32_t u = L; u < ncopy; u += 16) a.cigar[(uint64_t)jid * a.cigar_cap + u] = runs[u];
+    }
+    wave_sync();
   }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) c_cells += (unsigned long long)__shfl_xor((long long)c_cells, o, 64);
+  c_cells = __shfl((long long)c_cells, 0, 64) + __shfl((long long)c_cells, 16, 64) +
+            __shfl((long long)c_cells, 32, 64) + __shfl((long long)c_cells, 48, 64);
   if (lane == 0 && a.ctr) atomicAdd(&a.ctr->cells, c_cells);
 }
 
-// Lane kernel for the default band (30 -> 61 columns); other widths use k_dp_band.
-bool launch_dp_lanes(const DpArgs& a, uint32_t nwaves, hipStream_t s) {
+// Launches k_dp_quad; returns false (nothing launched) when one wave's four
+// job areas do not fit in 64 KiB of LDS.
+bool launch_dp_quad(const DpArgs& a, hipStream_t s) {
   if (a.njobs == 0) return true;
-  size_t lds = (size_t)(a.max_n + 2 * 61) * 64 * 2;
-  if (lds > 64 * 1024) return false;
-  uint32_t blocks = (a.njobs + 63) / 64;
-  if (blocks > nwaves) blocks = nwaves;
-  hipLaunchKernelGGL(k_dp_lanes<61>, dim3(blocks), dim3(64), lds, s, a);
+  const QuadLayout Y = quad_layout(a.max_m, a.max_n);
+  const size_t lds = 4 * (size_t)Y.stride;
+  if (a.max_m > (uint32_t)QUAD_MAX_M || lds > 64 * 1024) return false;
+  // genome-word staging reuses the score-word area
+  const size_t need = 8 * ((size_t)((a.max_n + 62) >> 5) + ((a.max_n + 126) >> 6));
+  if (need > Y.q - Y.scw) return false;
+  uint32_t blocks = (a.njobs + 3) / 4;
+  if (blocks > 65536 * 4) blocks = 65536 * 4;
+  hipLaunchKernelGGL(k_dp_quad, dim3(blocks), dim3(64), lds, s, a);
   return true;
 }
 
