@@ -39,6 +39,22 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_traffic_uneven_se_100mbp.txt")
+TRAFFIC_BATCH = 262144  # reads per launch the profile was taken at
+
+
+def measured_traffic(kernel, batch):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, copy-calibrated) from the
+    committed rocprofv3 counter summary, when it was taken at this batch size; else None."""
+    if batch != TRAFFIC_BATCH or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        f = line.strip().split(",")
+        if len(f) == 7 and f[0] == kernel:
+            return int(float(f[5]))
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,14 +124,15 @@ def main():
     if agg["ms_k_dp"] >= agg["ms_k_seed"]:
         ach = dp_gcups * DP_OPS_PER_CELL / 1000.0
         roof = {"kernel": "k_dp_quad", "bound": "valu-int32", "achieved": round(ach, 3), "peak": round(PEAK_INT32_TOPS, 2),
-                "unit": "Tops/s", "frac": round(ach / PEAK_INT32_TOPS, 4), "traffic": None,
+                "unit": "Tops/s", "frac": round(ach / PEAK_INT32_TOPS, 4), "traffic": measured_traffic("pbw::k_dp_quad", B),
                 "per_launch": {"cells": cells_per_launch, "ms": ms_dp, "ops_per_cell": DP_OPS_PER_CELL}}
     else:
         roof = {"kernel": "k_seed", "bound": "hbm", "achieved": round(gather_gbps, 2), "peak": PEAK_HBM_GBPS,
-                "unit": "GB/s", "frac": round(gather_gbps / PEAK_HBM_GBPS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(gather_gbps / PEAK_HBM_GBPS, 4), "traffic": measured_traffic("pbw::k_seed", B),
                 "per_launch": {"bytes": gbytes_per_launch, "ms": ms_seed}}
     gather_roof = {"kernel": "k_seed", "bound": "hbm", "achieved": round(gather_gbps, 2), "peak": PEAK_HBM_GBPS,
-                   "unit": "GB/s", "frac": round(gather_gbps / PEAK_HBM_GBPS, 4)}
+                   "unit": "GB/s", "frac": round(gather_gbps / PEAK_HBM_GBPS, 4),
+                   "traffic": measured_traffic("pbw::k_seed", B)}
 
     cpu = None
     concord = None

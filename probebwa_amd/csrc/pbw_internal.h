@@ -26,6 +26,7 @@ constexpr int CLUSTER_LDS_CAP = 512;     // hits per read clustered in LDS by on
 constexpr int LN_INT_MAX = 4096;         // ln(k) table for phred_n_choose_m
 constexpr int GC_LUT_MAX = 1024;         // (gc, total) table for the GC-adjusted alphabet size
 constexpr int QUAD_MAX_M = 320;         // longest read the four-jobs-per-wave DP kernel takes
+constexpr int QUAD_MAX_N = QUAD_MAX_M + 64;  // longest reference window it takes
 
 struct DevGenome {
   const uint64_t* g2b;    // 2-bit bases, 32 per word, LSB first (A0 C1 G2 T3), contigs back to back

@@ -1299,7 +1299,7 @@ __global__ __launch_bounds__(64) void k_dp_quad(DpArgs a) {
       n = job.n;
       h = job.hint;
       b = job.band;
-      const bool regular = m >= 1 && n >= 1 && m <= a.max_m && n <= a.max_n && b >= 0 && b <= 31 &&
+      const bool regular = m >= 1 && n >= 1 && m <= a.max_m && n <= a.max_n && n <= (uint32_t)QUAD_MAX_N && b >= 0 && b <= 31 &&
                            !job.explicit_ref && (int64_t)h + b >= 0 && (int64_t)m + h - b <= (int64_t)n;
       if (!regular) {
         if (L == 0) a.needs_generic[jid] = 1;
@@ -1315,6 +1315,7 @@ __global__ __launch_bounds__(64) void k_dp_quad(DpArgs a) {
     const int32_t W = 2 * b + 1, s = 63 - W, o = h + b - 62;
     const uint32_t ql = qlen < m ? qlen : m;
     // ---- staging: query (reverse complement for reverse-strand jobs, single.rs:101-103) ----
+    // Loads first, at clamped addresses so they are all in flight together, then the stores.
     for (uint32_t x = L; x < m; x += 16) Q[x] = job.rc ? comp_base(a.R.seq[so + m - 1 - x]) : a.R.seq[so + x];
     for (uint32_t x = L; x < ql; x += 16) QQ[x] = job.rc ? a.R.qual[qo + qlen - 1 - x] : a.R.qual[qo + x];
     // reference window: 2-bit words and N-mask words through LDS, then bytes
@@ -1332,16 +1333,33 @@ __global__ __launch_bounds__(64) void k_dp_quad(DpArgs a) {
       REF[x] = isn ? (uint8_t)'N' : (uint8_t)("ACGT"[c2]);
     }
     wave_sync();
+    constexpr int NR = QUAD_MAX_N / 16;
     // homopolymer run of every column (smith_waterman.rs:348-363): each lane
-    // scans a contiguous segment; run starts carry in by a prefix max and run
-    // ends by a suffix min over the 16 lanes
+    // takes a contiguous segment in registers; run starts carry in by a prefix
+    // max and run ends by a suffix min over the 16 lanes
     {
-      const uint32_t seg = (n + 15) / 16;
-      const uint32_t x0 = min(n, L * seg), x1 = min(n, x0 + seg);
+      const uint32_t seg = (n + 15) / 16;  // <= NR
+      const uint32_t x0 = min(n, L * seg), cnt = min(n, x0 + seg) - x0;
+      uint32_t bb[NR + 1];
+      const uint32_t prevb = (x0 > 0 && x0 < n) ? REF[x0 - 1] : 0x100u;  // no byte equals 0x100
+#pragma unroll
+      for (int u = 0; u <= NR; u++) {
+        const uint32_t x = x0 + u;
+        const uint32_t v = REF[n ? min(x, n - 1) : 0];
+        bb[u] = x < n ? v : 0x101u;
+      }
       int32_t lb = -1, fe = INT32_MAX;
-      for (uint32_t x = x0; x < x1; x++) {
-        if (x == 0 || REF[x] != REF[x - 1]) lb = (int32_t)x;
-        if (fe == INT32_MAX && (x == n - 1 || REF[x] != REF[x + 1])) fe = (int32_t)x;
+#pragma unroll
+      for (int u = NR - 1; u >= 0; u--) {
+        if ((uint32_t)u < cnt) {
+          if (bb[u] != bb[u + 1]) fe = (int32_t)(x0 + u);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < NR; u++) {
+        if ((uint32_t)u < cnt) {
+          if (bb[u] != (u == 0 ? prevb : bb[u - 1])) lb = (int32_t)(x0 + u);
+        }
       }
       int32_t pm = lb;
       pm = max(pm, dpp_mov<DPP_ROW_SHR1>(-1, pm));
@@ -1362,32 +1380,36 @@ __global__ __launch_bounds__(64) void k_dp_quad(DpArgs a) {
         GOA[e] = -1;
         CODA[e] = 24;
       }
-      wave_sync();
-      for (uint32_t x = x0; x < x1; x++) {  // run starts, ascending
-        if (x == 0 || REF[x] != REF[x - 1]) start = (int32_t)x;
-        const int32_t e = (int32_t)x + 1 - c_lo;
-        if (e >= 0 && e < nent) GOA[e] = start;  // temporarily the run start
+      int32_t st[NR];
+#pragma unroll
+      for (int u = 0; u < NR; u++) {
+        if (bb[u] != (u == 0 ? prevb : bb[u - 1])) start = (int32_t)(x0 + u);
+        st[u] = start;
       }
-      for (int32_t x = (int32_t)x1 - 1; x >= (int32_t)x0; x--) {  // run ends, descending
-        if (x == (int32_t)n - 1 || REF[x] != REF[x + 1]) end = x;
-        const int32_t e = x + 1 - c_lo;
-        if (e >= 0 && e < nent) {
-          int32_t run = end - GOA[e] + 1;
-          if (run > 65535) run = 65535;
-          const int32_t go = min(a.gap_open + (run - 1), -1);
-          const uint8_t rb = REF[x];
-          const int32_t code6 = rb == 'A' ? 0 : rb == 'C' ? 6 : rb == 'G' ? 12 : rb == 'T' ? 18 : 24;
-          GOA[e] = go;
-          CODA[e] = code6;
+      wave_sync();
+#pragma unroll
+      for (int u = NR - 1; u >= 0; u--) {
+        const int32_t x = (int32_t)x0 + u;
+        if ((uint32_t)u < cnt) {
+          if (bb[u] != bb[u + 1]) end = x;
+          const int32_t e = x + 1 - c_lo;
+          if (e >= 0 && e < nent) {
+            int32_t run = end - st[u] + 1;
+            if (run > 65535) run = 65535;
+            const uint32_t rb = bb[u];
+            GOA[e] = min(a.gap_open + (run - 1), -1);
+            CODA[e] = rb == 'A' ? 0 : rb == 'C' ? 6 : rb == 'G' ? 12 : rb == 'T' ? 18 : 24;
+          }
         }
       }
     }
     wave_sync();
-    // per-row score word: 6-bit signed fields for reference A, C, G, T, N
+    // per-row score word: 6-bit signed fields for reference A, C, G, T, N;
+    // mismatch_penalty(q) = round(-30 clamp(q/30, 0.2, 1)) = -clamp(q, 6, 30)
     for (uint32_t x = L; x < m; x += 16) {
       const uint8_t rb = Q[x];
       const uint32_t qv = x < qlen ? QQ[x] : 30u;
-      const int32_t mis = a.T.mis_pen[qv];
+      const int32_t mis = -(int32_t)min(max(qv, 6u), 30u);
       uint32_t w = 0;
       w |= (uint32_t)((rb == 'A' ? MATCH_SCORE : mis) & 63) << 0;
       w |= (uint32_t)((rb == 'C' ? MATCH_SCORE : mis) & 63) << 6;
@@ -1590,11 +1612,12 @@ __global__ __launch_bounds__(64) void k_dp_quad(DpArgs a) {
     keys[L] = key;
     wave_sync();
     // ---- traceback (smith_waterman.rs:299-345) by lane 0 of the group ----
-    // Each step takes a whole run from one traceback word: diagonal M steps
-    // (rows r, r-1, .. of a column), horizontal D steps (columns k, k-1, ..
-    // of a row) or vertical I steps (cells (r, k), (r-1, k+1), ..).  The
+    // Each step takes a whole run from at most two traceback words: diagonal M
+    // steps (rows r, r-1, .. of a column, into the block below), horizontal D
+    // steps (columns k, k-1, .., into the lane to the left) or vertical I steps
+    // (cells (r, k), (r-1, k+1), .., into the lane to the right).  The
     // selection is branch-free; the pending CIGAR run is rewritten in place
-    // every step and the slot advances when the operation changes.
+    // every step and its slot advances when the operation changes.
     uint32_t nruns = 0, off = 0;
     int64_t best = 0;
     uint32_t* runs_top = TB + Y.tb_words;
@@ -1604,37 +1627,49 @@ __global__ __launch_bounds__(64) void k_dp_quad(DpArgs a) {
       uint32_t i = m, j = ((uint32_t)best) >> 1, st = ((uint32_t)best) & 1u;
       uint32_t cur_op = 0xFFu, cur_len = 0;
       int32_t cb = -1, cl = -1;
-      uint32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+      uint32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0, p0 = 0, p1 = 0, n2 = 0, l3 = 0;
       while (i > 0) {
         int32_t d = (int32_t)j - (int32_t)i - o;
         d = d < 0 ? 0 : (d > 63 ? 63 : d);  // cells on the path always lie in the band
         const int32_t lq = d >> 2, rb = (int32_t)(i - 1) >> 3;
         if (rb != cb || lq != cl) {
           const uint32_t* tw = TB + (rb * 4) * 16 + lq;
+          const uint32_t* pw = TB + ((rb > 0 ? rb - 1 : 0) * 4) * 16 + lq;
           v0 = tw[0];
           v1 = tw[16];
           v2 = tw[32];
           v3 = tw[48];
+          p0 = pw[0];
+          p1 = pw[16];
+          n2 = tw[32 + (lq < 15 ? 1 : 0)];
+          l3 = tw[48 - (lq > 0 ? 1 : 0)];
           cb = rb;
           cl = lq;
         }
         const uint32_t k = (uint32_t)d & 3u, r = (i - 1) & 7u;
         const uint32_t pos = 31u - (4u * r + k);
         // M: stops after the first row whose M came from I or D
-        const uint32_t x = (v0 >> pos) & 0x11111111u;
-        const uint32_t f = x ? ((uint32_t)__builtin_ctz(x) >> 2) : 32u;
-        const uint32_t mst = min(min(f + 1u, r + 1u), j);
-        const uint32_t mpar = (mst == f + 1u) ? 1u + ((v1 >> ((pos + 4u * f) & 31u)) & 1u) : 0u;
+        const uint32_t mav = r + 1u + (rb > 0 ? 8u : 0u);
+        const uint64_t xm = ((((uint64_t)p0 << 32) | v0) >> pos) & 0x1111111111111111ull &
+                            (mav >= 16u ? ~0ull : ((1ull << (4u * mav)) - 1ull));
+        const uint32_t f = xm ? ((uint32_t)__builtin_ctzll(xm) >> 2) : 64u;
+        const uint32_t mst = min(min(f + 1u, mav), j);
+        const uint64_t x1 = (((uint64_t)p1 << 32) | v1) >> pos;
+        const uint32_t mpar = (mst == f + 1u) ? 1u + (uint32_t)((x1 >> ((4u * f) & 63u)) & 1ull) : 0u;
         // D: stops after the first column whose D was opened
-        const uint32_t y = ~(v3 >> pos) & ((2u << k) - 1u);
+        const uint32_t dav = k + 1u + (lq > 0 ? 4u : 0u);
+        const uint32_t yd = ((v3 >> pos) & ((2u << k) - 1u)) | (((l3 >> (28u - 4u * r)) & 0xFu) << (k + 1u));
+        const uint32_t y = ~yd & ((1u << dav) - 1u);
         const uint32_t fz = y ? (uint32_t)__builtin_ctz(y) : 32u;
-        const uint32_t dst = min(min(fz + 1u, k + 1u), j);
+        const uint32_t dst = min(min(fz + 1u, dav), j);
         const uint32_t dpar = (dst == fz + 1u) ? 0u : 2u;
-        // I: stops after the first cell whose I was opened (bits pos, pos+3, ..)
-        const uint32_t tmax = min(3u - k, r);
-        const uint32_t zi = ~(v2 >> pos) & 0x249u & ((8u << (3u * tmax)) - 1u);
+        // I: stops after the first cell whose I was opened (stride-3 bits)
+        const uint32_t tav = min((lq < 15 ? 7u : 3u) - k, r);
+        const uint32_t ma = 0x249249u & ((1u << (3u * (3u - k) + 1u)) - 1u);
+        const uint32_t zv = ((v2 >> pos) & ma) | ((r > 0 ? (n2 >> ((pos + 4u) & 31u)) : 0u) & 0x249249u & ~ma);
+        const uint32_t zi = ~zv & 0x249249u & ((1u << (3u * tav + 1u)) - 1u);
         const uint32_t fi = zi ? (uint32_t)__builtin_ctz(zi) / 3u : 32u;
-        const uint32_t ist = min(fi + 1u, tmax + 1u);
+        const uint32_t ist = min(fi + 1u, tav + 1u);
         const uint32_t ipar = (ist == fi + 1u) ? 0u : 1u;
         // select: reference column 0 forces a single I step with the state kept
         const uint32_t sel = (j == 0) ? 1u : st;
@@ -1661,7 +1696,8 @@ __global__ __launch_bounds__(64) void k_dp_quad(DpArgs a) {
     const uint32_t* runs = runs_top - nruns;
     // ---- alignment log-likelihood terms (single.rs:228-262), one per read
     //      position, in parallel into the consumed bottom of the traceback area:
-    //      M columns by base match, I columns ln(1/4) ----
+    //      M columns by base match, I columns ln(1/4).  The table reads of a
+    //      lane are issued together. ----
     double* terms = (double*)TB;
     {
       uint32_t u = 0, qs = 0, rs = 0;
@@ -1687,7 +1723,15 @@ __global__ __launch_bounds__(64) void k_dp_quad(DpArgs a) {
     wave_sync();
     if (L == 0 && m) {
       double lk = 0.0;
-      for (uint32_t qi = 0; qi < m; qi++) lk += terms[qi];  // read order, as the reference sums
+      uint32_t qi = 0;
+      for (; qi + 8 <= m; qi += 8) {  // read order, as the reference sums
+        double t8[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) t8[u] = terms[qi + u];
+#pragma unroll
+        for (int u = 0; u < 8; u++) lk += t8[u];
+      }
+      for (; qi < m; qi++) lk += terms[qi];
       uint32_t span = 0;
       for (uint32_t u = 0; u < nruns; u++) {
         const uint32_t w = runs[u], op = w & 15u;

@@ -105,7 +105,8 @@ def brute_align(read, qual, ref, band, hint, go=-90, ge=-1):
 
 
 def test_mismatch_penalty_closed_form():  # smith_waterman.rs:272-275
-    for q in range(0, 94):
+    # every byte a quality can hold: the DP kernel uses -clamp(q, 6, 30) directly
+    for q in range(0, 256):
         want = -6 if q <= 6 else (-q if q <= 30 else -30)
         assert O.mismatch_penalty(q) == want, q
 
