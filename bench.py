@@ -55,6 +55,19 @@ def measured_traffic(kernel, batch):
     return None
 
 
+def sam_identical(a, b):
+    """Per record: POS, RNAME, FLAG, MAPQ, strand and CIGAR identical (records in the same order)."""
+    ra, rb = a.rec, b.rec
+    same = ((ra["pos"] == rb["pos"]) & (ra["contig"] == rb["contig"]) & (ra["flag"] == rb["flag"])
+            & (ra["mapq"] == rb["mapq"]) & (ra["strand"] == rb["strand"]) & (ra["cigar_n"] == rb["cigar_n"]))
+    if same.all() and len(a.cigar) == len(b.cigar):
+        eq = np.asarray(a.cigar) == np.asarray(b.cigar)
+        starts = np.asarray(ra["cigar_off"], dtype=np.int64)
+        ok = np.logical_and.reduceat(eq, starts) if len(eq) else np.ones(len(ra), bool)
+        return same & ok
+    return same & np.array([a.cigar_words(i) == b.cigar_words(i) for i in range(len(ra))])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,9 +75,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=262144)
     ap.add_argument("--reads", type=int, default=2_000_000)
-    ap.add_argument("--cpu-sample", type=int, default=16384)
+    ap.add_argument("--cpu-sample", type=int, default=131072)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--train-reads", type=int, default=131072)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
@@ -134,6 +149,54 @@ def main():
                    "unit": "GB/s", "frac": round(gather_gbps / PEAK_HBM_GBPS, 4),
                    "traffic": measured_traffic("pbw::k_seed", B)}
 
+    learned = None
+    if not args.no_learned:
+        # learned candidate prefilter: trained on a separate read set of the same genome (labels from
+        # the plain DP over every candidate), then the same steps timed with it on
+        t1 = time.time()
+        tseq, tso, tqual, tqo = synth.reads_uneven_se(contigs, n=args.train_reads, seed=99)
+        rep = ctx.prefilter_train(P.ReadBatch.from_arrays(tseq, tso, tqual, tqo), label_nats=40.0, recall=0.9999,
+                                  epochs=20, batch=4096)
+        log("prefilter trained in %.1f s on %d rows (%d positive), held-out skip %.3f recall %.5f"
+            % (time.time() - t1, rep["rows"], rep["positives"], rep["val_skip"], rep["val_recall"]))
+        ctx.set_prefilter(enabled=1, audit_period=64)
+        for w in range(args.warmup):
+            step(w)
+        agg2 = {}
+        first2 = None
+        t1 = time.perf_counter()
+        for k in range(args.steps):
+            recs, st = step(k)
+            if k == 0:
+                first2 = recs
+            for key, v in st.items():
+                agg2[key] = agg2.get(key, 0) + v
+            log("learned step %d: %d reads in %.1f ms" % (k, st["reads"], st["ms_total"]))
+        dt2 = time.perf_counter() - t1
+        ctx.set_prefilter(enabled=0)
+        same = sam_identical(first, first2)
+        learned = {
+            "reads_per_s": round(int(agg2["reads"]) / dt2, 1),
+            "ms_per_step": round(dt2 * 1000.0 / args.steps, 3),
+            "speedup_vs_plain": round((int(agg2["reads"]) / dt2) / value, 3),
+            "sam_identical_first_step": round(float(same.mean()), 6),
+            "sam_differing_first_step": int((~same).sum()),
+            "audit_reads": int(agg2["pf_audit_reads"]),
+            "audit_mismatches": int(agg2["pf_audit_mismatches"]),
+            "fallback_reads": int(agg2["pf_fallback_reads"]),
+            "fallback_rate": round(agg2["pf_fallback_reads"] / max(1, agg2["reads"]), 5),
+            "dp_jobs_aligned_frac": round(agg2["pf_jobs_aligned"] / max(1, agg2["pf_jobs"]), 5),
+            "dp_cells": int(agg2["dp_cells"]),
+            "dp_cells_skipped_frac": round(1.0 - agg2["dp_cells"] / max(1, agg["dp_cells"]), 5),
+            "stage_ms": {k: round(agg2[k], 1) for k in ("ms_seed", "ms_cluster", "ms_pf", "ms_dp", "ms_mapq",
+                                                         "ms_total")},
+            "training": {"reads": args.train_reads, "rows": int(rep["rows"]), "positives": int(rep["positives"]),
+                         "label": "log-likelihood within 40 nats of the read's best", "epochs": 20,
+                         "batch": 4096, "lr": 1e-3, "threshold": round(float(rep["threshold"]), 5),
+                         "heldout_recall": round(rep["val_recall"], 6), "heldout_skip": round(rep["val_skip"], 5),
+                         "loss_first_epoch": round(rep["loss_first"], 6), "loss_last_epoch": round(rep["loss_last"], 6)},
+        }
+
     cpu = None
     concord = None
     if not args.no_cpu:
@@ -154,13 +217,11 @@ def main():
         cpu = {"value": round(S / cdt, 1), "unit": "reads/s", "cores": args.cpu_threads, "kind": "port",
                "sample": "first %d reads of the workload, oracle (C++ restatement) on %d threads, %.1f s"
                          % (S, args.cpu_threads, cdt)}
-        same = 0
-        for i in range(S):
-            a, b = ro.rec[i], first.rec[i]
-            if (a["pos"] == b["pos"] and a["contig"] == b["contig"] and a["flag"] == b["flag"] and a["mapq"] == b["mapq"]
-                    and ro.cigar_words(i) == first.cigar_words(i)):
-                same += 1
-        concord = same / S
+        ra, rg = ro.rec, first.rec[:S]
+        same = ((ra["pos"] == rg["pos"]) & (ra["contig"] == rg["contig"]) & (ra["flag"] == rg["flag"])
+                & (ra["mapq"] == rg["mapq"]))
+        same &= np.array([ro.cigar_words(i) == first.cigar_words(i) for i in range(S)])
+        concord = float(same.mean())
 
     out = {
         "metric": "reads/s on 1 GPU, learned vs plain @≥99.9% SAM concordance; DP GCUPS",
@@ -182,6 +243,8 @@ def main():
         "gather_gbps": round(gather_gbps, 2),
         "roofline": roof,
         "roofline_gather": gather_roof,
+        "learned": learned,
+        "learned_vs_plain": learned["speedup_vs_plain"] if learned else None,
         "cpu_baseline": cpu,
         "sample_sam_concordance": concord,
         "stage_ms": {k: round(agg[k], 1) for k in ("ms_seed", "ms_cluster", "ms_dp", "ms_mapq", "ms_total")},

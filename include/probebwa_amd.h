@@ -124,6 +124,13 @@ typedef struct {
   double ms_seed, ms_cluster, ms_dp, ms_mapq, ms_pair, ms_total;  /* host wall time per stage */
   double ms_k_seed, ms_k_dp;  /* HIP-event time of the seed kernel and the main banded-DP launch */
   uint64_t n_k_seed, n_k_dp;  /* launches timed */
+  /* candidate prefilter (zero when it is off) */
+  uint64_t pf_jobs;             /* job slots that plain mapping would align */
+  uint64_t pf_jobs_aligned;     /* job slots aligned (kept + re-run) */
+  uint64_t pf_fallback_reads;   /* reads re-run plain as ambiguous */
+  uint64_t pf_audit_reads;      /* reads re-run plain as the audit sample */
+  uint64_t pf_audit_mismatches; /* audited reads whose SAM fields differ from the plain run */
+  double ms_pf;                 /* host wall time of scoring and selection */
 } pbw_stats;
 
 /* ---- genome: GenomeIndexBuilder (src/index/genome.rs:101-147) ---- */
@@ -173,6 +180,67 @@ int pbw_align_batch(pbw_ctx* ctx, uint32_t gap_open_phred, uint32_t gap_extend_p
  *      copies up to cap packed positions, returns the bucket size (or <0) ---- */
 int64_t pbw_index_lookup(pbw_ctx* ctx, uint32_t code, uint64_t* out, uint64_t cap);
 uint64_t pbw_index_total_entries(const pbw_ctx* ctx);
+
+/* ---- candidate prefilter network: MLP 32 -> 128 -> 128 -> 1 (ReLU), BF16 matrix-core
+ *      operands with FP32 accumulation; FP32 master parameters and Adam.  No counterpart
+ *      in the reference: it ranks the <= 20 candidates of a read before the DP
+ *      (between cluster_hits, src/mapper/single.rs:91, and the DP loop, :114-150).
+ *      Parameter layout (floats): W1[32][128] | b1[128] | W2[128][128] | b2[128] | W3[128] | b3.
+ *      Inputs are bf16 bit patterns, n rows of 32. ---- */
+#define PBW_MLP_INPUTS 32
+#define PBW_MLP_PARAMS 20865
+typedef struct {
+  uint32_t epochs;   /* 20 */
+  uint32_t batch;    /* 4096 rows */
+  float lr;          /* Adam 1e-3 */
+  float beta1, beta2, eps;
+  float pos_weight;  /* BCE weight of label-1 rows */
+  uint64_t seed;     /* shuffle seed */
+} pbw_train_options;
+void pbw_default_train_options(pbw_train_options* o);
+
+/* Prefilter in the mapping path (pbw_map_se / pbw_map_se_device).  A candidate is
+ * aligned when its rank (cluster-score order) is below top_k or its network score
+ * is >= threshold.  A read is re-run plain when its prefiltered MAPQ is below
+ * mapq_fallback or a kept second candidate is within close_nats of the best; every
+ * audit_period-th read is re-run plain and compared (pbw_stats.pf_audit_*). */
+typedef struct {
+  int32_t enabled;
+  float threshold;
+  uint32_t top_k;          /* 1 */
+  uint32_t mapq_fallback;  /* 30 */
+  float close_nats;        /* 40 */
+  uint32_t audit_period;   /* 64 */
+} pbw_prefilter_options;
+void pbw_default_prefilter_options(pbw_prefilter_options* o);
+int pbw_prefilter_set(pbw_ctx* ctx, const pbw_prefilter_options* o);
+int pbw_prefilter_get(const pbw_ctx* ctx, pbw_prefilter_options* o);
+
+/* Training rows from plain mapping of `reads`: per aligned candidate its 32 features
+ * (BF16), label 1 when its log-likelihood is within label_nats of the read's best, and
+ * its rank.  Returns the row count (rows beyond cap are counted, not written). */
+int pbw_prefilter_dataset(pbw_ctx* ctx, const pbw_read_batch* reads, float label_nats, uint16_t* x, float* y,
+                          uint8_t* rank, uint64_t cap, uint64_t* n);
+typedef struct {
+  uint64_t rows, positives, val_rows, val_positives;
+  float threshold;   /* calibrated and stored in the context's prefilter options */
+  double val_recall; /* validation positives kept (by rank or score) */
+  double val_skip;   /* validation rows not aligned */
+  double loss_first, loss_last;
+} pbw_prefilter_report;
+/* Trains the network on rows of `reads` (the last 10% held out) and calibrates the
+ * threshold so that `recall` of the held-out positives ranked >= top_k score above it. */
+int pbw_prefilter_train(pbw_ctx* ctx, const pbw_read_batch* reads, const pbw_train_options* o, float label_nats,
+                        float recall, pbw_prefilter_report* out);
+int pbw_mlp_init(pbw_ctx* ctx, uint64_t seed);
+int pbw_mlp_set_params(pbw_ctx* ctx, const float* theta, uint64_t n);
+int pbw_mlp_get_params(pbw_ctx* ctx, float* theta, uint64_t n);
+int pbw_mlp_forward(pbw_ctx* ctx, const uint16_t* x_bf16, uint64_t n, float* logits);
+/* gradient of mean_i w_i BCE(sigmoid(z_i), y_i), w = pos_weight for y = 1 */
+int pbw_mlp_gradient(pbw_ctx* ctx, const uint16_t* x_bf16, const float* y, uint64_t n, float pos_weight, float* grad,
+                     double* loss);
+int pbw_mlp_train(pbw_ctx* ctx, const uint16_t* x_bf16, const float* y, uint64_t n, const pbw_train_options* o,
+                  double* epoch_loss);
 
 void pbw_records_release(pbw_record_batch* b);
 int pbw_get_stats(const pbw_ctx* ctx, pbw_stats* out);

@@ -47,8 +47,59 @@ EXPORTED_SYMBOLS = (
     "pbw_ctx_create", "pbw_ctx_destroy", "pbw_last_error", "pbw_global_error", "pbw_map_se", "pbw_reads_upload",
     "pbw_map_se_device", "pbw_reads_free", "pbw_map_pe",
     "pbw_align_batch", "pbw_index_lookup", "pbw_index_total_entries", "pbw_records_release", "pbw_get_stats",
-    "pbw_format_sam", "pbw_free_text",
+    "pbw_format_sam", "pbw_free_text", "pbw_default_train_options", "pbw_mlp_init", "pbw_mlp_set_params",
+    "pbw_mlp_get_params", "pbw_mlp_forward", "pbw_mlp_gradient", "pbw_mlp_train", "pbw_default_prefilter_options",
+    "pbw_prefilter_set", "pbw_prefilter_get", "pbw_prefilter_dataset", "pbw_prefilter_train",
 )
+
+MLP_INPUTS = 32
+MLP_PARAMS = 20865
+
+
+class TrainOptions(C.Structure):
+    """Prefilter training options (Adam over shuffled mini-batches)."""
+
+    _fields_ = [("epochs", C.c_uint32), ("batch", C.c_uint32), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float), ("pos_weight", C.c_float), ("seed", C.c_uint64)]
+
+    def __init__(self, **kw):
+        super().__init__()
+        lib().pbw_default_train_options(C.byref(self))
+        for k, v in kw.items():
+            setattr(self, k, v)
+
+
+class PrefilterOptions(C.Structure):
+    """Candidate prefilter in the mapping path (see include/probebwa_amd.h)."""
+
+    _fields_ = [("enabled", C.c_int32), ("threshold", C.c_float), ("top_k", C.c_uint32),
+                ("mapq_fallback", C.c_uint32), ("close_nats", C.c_float), ("audit_period", C.c_uint32)]
+
+    def __init__(self, **kw):
+        super().__init__()
+        lib().pbw_default_prefilter_options(C.byref(self))
+        for k, v in kw.items():
+            setattr(self, k, v)
+
+
+class PrefilterReport(C.Structure):
+    _fields_ = [("rows", C.c_uint64), ("positives", C.c_uint64), ("val_rows", C.c_uint64),
+                ("val_positives", C.c_uint64), ("threshold", C.c_float), ("val_recall", C.c_double),
+                ("val_skip", C.c_double), ("loss_first", C.c_double), ("loss_last", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def to_bf16(a):
+    """float array -> bf16 bit patterns (uint16), round to nearest even."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16)
+
+
+def from_bf16(h):
+    return (np.asarray(h, dtype=np.uint16).astype(np.uint32) << 16).view(np.float32)
 
 
 class MapperOptions(C.Structure):
@@ -102,7 +153,9 @@ class Stats(C.Structure):
                                            "generic_dp_jobs", "band_dp_jobs")] + \
                [(n, C.c_double) for n in ("ms_seed", "ms_cluster", "ms_dp", "ms_mapq", "ms_pair", "ms_total",
                                            "ms_k_seed", "ms_k_dp")] + \
-               [(n, C.c_uint64) for n in ("n_k_seed", "n_k_dp")]
+               [(n, C.c_uint64) for n in ("n_k_seed", "n_k_dp", "pf_jobs", "pf_jobs_aligned", "pf_fallback_reads",
+                                           "pf_audit_reads", "pf_audit_mismatches")] + \
+               [("ms_pf", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -151,6 +204,18 @@ def lib():
             "pbw_format_sam": (vp, [vp, C.POINTER(_ReadBatch), C.c_char_p, C.POINTER(_RecordBatch), C.c_char_p,
                                     C.c_char_p, C.POINTER(u64)]),
             "pbw_free_text": (None, [vp]),
+            "pbw_default_train_options": (None, [vp]),
+            "pbw_mlp_init": (i32, [vp, u64]),
+            "pbw_mlp_set_params": (i32, [vp, vp, u64]),
+            "pbw_mlp_get_params": (i32, [vp, vp, u64]),
+            "pbw_mlp_forward": (i32, [vp, vp, u64, vp]),
+            "pbw_mlp_gradient": (i32, [vp, vp, vp, u64, C.c_float, vp, C.POINTER(C.c_double)]),
+            "pbw_mlp_train": (i32, [vp, vp, vp, u64, vp, vp]),
+            "pbw_default_prefilter_options": (None, [vp]),
+            "pbw_prefilter_set": (i32, [vp, vp]),
+            "pbw_prefilter_get": (i32, [vp, vp]),
+            "pbw_prefilter_dataset": (i32, [vp, C.POINTER(_ReadBatch), C.c_float, vp, vp, vp, u64, C.POINTER(u64)]),
+            "pbw_prefilter_train": (i32, [vp, C.POINTER(_ReadBatch), vp, C.c_float, C.c_float, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -232,6 +297,13 @@ class ReadBatch:
     def cstruct(self):
         return _ReadBatch(self.n, self.seq.ctypes.data, self.seq_off.ctypes.data, self.qual.ctypes.data,
                           self.qual_off.ctypes.data)
+
+    def subset(self, begin, end):
+        """Reads [begin, end) as a new batch."""
+        so, qo = self.seq_off, self.qual_off
+        return ReadBatch.from_arrays(self.seq[so[begin]:so[end]], so[begin:end + 1] - so[begin],
+                                     self.qual[qo[begin]:qo[end]], qo[begin:end + 1] - qo[begin],
+                                     self.ids[begin:end])
 
     def seq_of(self, i):
         return self.seq[self.seq_off[i]:self.seq_off[i + 1]].tobytes()
@@ -352,6 +424,83 @@ class Context:
         s = Stats()
         _check(lib().pbw_get_stats(self.h, C.byref(s)), self.h)
         return s.as_dict()
+
+    # ---- candidate prefilter network ----
+    def mlp_init(self, seed=1):
+        _check(lib().pbw_mlp_init(self.h, seed), self.h)
+
+    @property
+    def mlp_params(self):
+        th = np.zeros(MLP_PARAMS, np.float32)
+        _check(lib().pbw_mlp_get_params(self.h, th.ctypes.data, MLP_PARAMS), self.h)
+        return th
+
+    @mlp_params.setter
+    def mlp_params(self, theta):
+        th = np.ascontiguousarray(theta, dtype=np.float32)
+        _check(lib().pbw_mlp_set_params(self.h, th.ctypes.data, len(th)), self.h)
+
+    def mlp_forward(self, x_bf16):
+        x = np.ascontiguousarray(x_bf16, dtype=np.uint16).reshape(-1, MLP_INPUTS)
+        z = np.zeros(len(x), np.float32)
+        _check(lib().pbw_mlp_forward(self.h, x.ctypes.data, len(x), z.ctypes.data), self.h)
+        return z
+
+    def mlp_gradient(self, x_bf16, y, pos_weight=1.0):
+        x = np.ascontiguousarray(x_bf16, dtype=np.uint16).reshape(-1, MLP_INPUTS)
+        yy = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_double()
+        _check(lib().pbw_mlp_gradient(self.h, x.ctypes.data, yy.ctypes.data, len(x), pos_weight, g.ctypes.data,
+                                      C.byref(loss)), self.h)
+        return g, loss.value
+
+    # ---- prefilter in the mapping path ----
+    @property
+    def prefilter(self):
+        o = PrefilterOptions()
+        _check(lib().pbw_prefilter_get(self.h, C.byref(o)), self.h)
+        return o
+
+    @prefilter.setter
+    def prefilter(self, o):
+        _check(lib().pbw_prefilter_set(self.h, C.byref(o)), self.h)
+
+    def set_prefilter(self, **kw):
+        o = self.prefilter
+        for k, v in kw.items():
+            setattr(o, k, v)
+        self.prefilter = o
+        return o
+
+    def prefilter_dataset(self, reads, label_nats=40.0):
+        """(x_bf16 [n, 32], y [n], rank [n]) training rows from plain mapping of `reads`."""
+        rs = reads.cstruct()
+        n = C.c_uint64()
+        _check(lib().pbw_prefilter_dataset(self.h, C.byref(rs), label_nats, None, None, None, 0, C.byref(n)), self.h)
+        x = np.zeros((max(n.value, 1), MLP_INPUTS), np.uint16)
+        y = np.zeros(max(n.value, 1), np.float32)
+        r = np.zeros(max(n.value, 1), np.uint8)
+        _check(lib().pbw_prefilter_dataset(self.h, C.byref(rs), label_nats, x.ctypes.data, y.ctypes.data,
+                                           r.ctypes.data, n.value, C.byref(n)), self.h)
+        return x[:n.value], y[:n.value], r[:n.value]
+
+    def prefilter_train(self, reads, label_nats=40.0, recall=0.9999, **kw):
+        """Trains the network on `reads` (plain-DP labels) and calibrates the threshold."""
+        rs = reads.cstruct()
+        o = TrainOptions(**kw)
+        rep = PrefilterReport()
+        _check(lib().pbw_prefilter_train(self.h, C.byref(rs), C.byref(o), label_nats, recall, C.byref(rep)), self.h)
+        return rep.as_dict()
+
+    def mlp_train(self, x_bf16, y, **kw):
+        x = np.ascontiguousarray(x_bf16, dtype=np.uint16).reshape(-1, MLP_INPUTS)
+        yy = np.ascontiguousarray(y, dtype=np.float32)
+        o = TrainOptions(**kw)
+        losses = np.zeros(o.epochs, np.float64)
+        _check(lib().pbw_mlp_train(self.h, x.ctypes.data, yy.ctypes.data, len(x), C.byref(o), losses.ctypes.data),
+               self.h)
+        return losses
 
 
 class DeviceReads:

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <random>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -278,6 +279,37 @@ struct pbw_ctx {
   DevBuf<uint32_t> d_plan, d_wcigar, d_name_class;
   DevBuf<uint64_t> d_cig_src;
   DevBuf<double> d_obs;
+  // candidate prefilter network
+  DevBuf<float> d_mlp_f;     // theta | m | v | b1 | b2 | b3
+  DevBuf<uint16_t> d_mlp_h;  // w1t | w2t | w2 | w3
+  DevBuf<float> d_mlp_part, d_mlp_loss, d_mlp_y, d_mlp_z, d_mlp_grad;
+  DevBuf<uint16_t> d_mlp_x;
+  DevBuf<uint32_t> d_mlp_perm;
+  bool mlp_ready = false;
+  uint64_t mlp_step = 0;
+  // prefilter in the mapping path
+  pbw_prefilter_options pf{};
+  DevBuf<uint8_t> d_hflag, d_hflag_ov, d_verdict, d_rec2;
+  DevBuf<uint16_t> d_feat;
+  DevBuf<uint32_t> d_pf_keep, d_pf_list, d_pf_n, d_pf_rlist, d_best2;
+  DevBuf<unsigned long long> d_pf_ctr;
+
+  MlpDev mlp() {
+    d_mlp_f.ensure(3 * (size_t)MLP_NP + 2 * MLP_H + 1);
+    d_mlp_h.ensure((size_t)MLP_IN * MLP_H + 2 * (size_t)MLP_H * MLP_H + MLP_H);
+    MlpDev P;
+    P.theta = d_mlp_f.p;
+    P.m = P.theta + MLP_NP;
+    P.v = P.m + MLP_NP;
+    P.b1 = P.v + MLP_NP;
+    P.b2 = P.b1 + MLP_H;
+    P.b3 = P.b2 + MLP_H;
+    P.w1t = d_mlp_h.p;
+    P.w2t = P.w1t + MLP_IN * MLP_H;
+    P.w2 = P.w2t + MLP_H * MLP_H;
+    P.w3 = P.w2 + MLP_H * MLP_H;
+    return P;
+  }
 
   DevGenome dev_genome() const {
     DevGenome G;
@@ -507,7 +539,7 @@ static void batch_upload_slice(pbw_ctx* c, Batch& b, const pbw_read_batch* in, u
 
 // map_candidates (single.rs:86-162) for every read of the batch: seed,
 // cluster, DP jobs, banded DP.  Results stay on the device.
-static void batch_candidates(pbw_ctx* c, Batch& b) {
+static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false) {
   hipStream_t s = c->stream;
   const uint32_t B = b.B;
   const DevReads R = b.R;
@@ -530,6 +562,10 @@ static void batch_candidates(pbw_ctx* c, Batch& b) {
   sa.nhits = c->d_nhits.p;
   sa.rcode = c->d_rcode.p;
   sa.ctr = c->d_ctr.p;
+  if (feat) {
+    c->d_hflag.ensure((size_t)B * HIT_CAP);
+    sa.hflag = c->d_hflag.p;
+  }
   HIP_CHECK(hipEventRecord(c->ev[0], s));
   launch_seed(sa, s);
   HIP_CHECK(hipGetLastError());
@@ -558,6 +594,10 @@ static void batch_candidates(pbw_ctx* c, Batch& b) {
   if (!ov.empty()) {
     c->d_hits_ov.ensure((size_t)B * HIT_CAP + ov_total);
     HIP_CHECK(hipMemcpyAsync(c->d_hits_ov.p, c->d_hits.p, (size_t)B * HIT_CAP * 8, hipMemcpyDeviceToDevice, s));
+    if (feat) {
+      c->d_hflag_ov.ensure((size_t)B * HIT_CAP + ov_total);
+      HIP_CHECK(hipMemcpyAsync(c->d_hflag_ov.p, c->d_hflag.p, (size_t)B * HIT_CAP, hipMemcpyDeviceToDevice, s));
+    }
     std::vector<uint64_t> ovoff(ov.size());
     std::vector<uint32_t> ovcap(ov.size());
     uint64_t cur = (uint64_t)B * HIT_CAP;
@@ -578,6 +618,7 @@ static void batch_candidates(pbw_ctx* c, Batch& b) {
     sb.hcap = c->d_hcap.p;
     sb.hits = c->d_hits_ov.p;
     sb.ctr = nullptr;  // counted once, in the first pass
+    sb.hflag = feat ? c->d_hflag_ov.p : nullptr;
     launch_seed(sb, s);
     HIP_CHECK(hipGetLastError());
     c->d_hoff.upload(hoff.data(), B, s);
@@ -602,6 +643,12 @@ static void batch_candidates(pbw_ctx* c, Batch& b) {
   ca.ncand = c->d_ncand.p;
   ca.tier2 = c->d_tier2.p;
   ca.ntier2 = c->d_ntier2.p;
+  if (feat) {
+    c->d_feat.ensure((size_t)B * TOP_CANDIDATES * MLP_IN);
+    ca.hflag = hits_base == c->d_hits_ov.p ? c->d_hflag_ov.p : c->d_hflag.p;
+    ca.rcode = c->d_rcode.p;
+    ca.feat = c->d_feat.p;
+  }
   launch_cluster(ca, s);
   HIP_CHECK(hipGetLastError());
   uint32_t nt2 = 0;
@@ -639,16 +686,27 @@ static void batch_candidates(pbw_ctx* c, Batch& b) {
   HIP_CHECK(hipGetLastError());
   b.max_n = b.max_m + (uint32_t)right_pad + LEFT_PAD;
   b.cap = b.max_m + b.max_n + 2;
+  c->d_out.ensure((size_t)B * TOP_CANDIDATES);
+  c->d_cigar.ensure((size_t)B * TOP_CANDIDATES * b.cap);
+}
+
+// Banded DP (single.rs:114-150) of the batch's job slots: every valid slot
+// (jlist == nullptr) or the nlist slots of a device list.
+static void batch_dp(pbw_ctx* c, Batch& b, const uint32_t* jlist, uint32_t nlist) {
+  hipStream_t s = c->stream;
+  const uint32_t B = b.B;
+  const DevReads R = b.R;
+  auto t0 = std::chrono::steady_clock::now();
+  const int32_t band = (int32_t)c->opt.max_indel_len;
+  const int32_t right_pad = (int32_t)std::max<uint32_t>(c->opt.max_indel_len, 10);  // single.rs:110
   const size_t nslots = (size_t)B * TOP_CANDIDATES;
-  c->d_out.ensure(nslots);
-  c->d_cigar.ensure(nslots * b.cap);
   c->d_generic.ensure(nslots);
   HIP_CHECK(hipMemsetAsync(c->d_generic.p, 0, nslots, s));
   DpArgs da = dp_args_base(c, R);
   da.jobs = c->d_jobs.p;
-  da.jlist = nullptr;
-  da.njobs = (uint32_t)nslots;
-  da.job_valid = c->d_njobs.p;
+  da.jlist = jlist;
+  da.njobs = jlist ? nlist : (uint32_t)nslots;
+  da.job_valid = jlist ? nullptr : c->d_njobs.p;
   da.cigar_cap = b.cap;
   da.cigar = c->d_cigar.p;
   da.out = c->d_out.p;
@@ -697,12 +755,18 @@ static void batch_candidates(pbw_ctx* c, Batch& b) {
     run_dp_tiled(c, da, list);
     c->stats.generic_dp_jobs += list.size();
   } else {
-    std::vector<uint8_t> nj(B);
-    HIP_CHECK(hipMemcpyAsync(nj.data(), c->d_njobs.p, B, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
     std::vector<uint32_t> list;
-    for (uint32_t r = 0; r < B; r++)
-      for (uint32_t k = 0; k < nj[r]; k++) list.push_back(r * TOP_CANDIDATES + k);
+    if (jlist) {
+      list.resize(nlist);
+      if (nlist) HIP_CHECK(hipMemcpyAsync(list.data(), jlist, (size_t)nlist * 4, hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+    } else {
+      std::vector<uint8_t> nj(B);
+      HIP_CHECK(hipMemcpyAsync(nj.data(), c->d_njobs.p, B, hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      for (uint32_t r = 0; r < B; r++)
+        for (uint32_t k = 0; k < nj[r]; k++) list.push_back(r * TOP_CANDIDATES + k);
+    }
     run_dp_tiled(c, da, list);
     c->stats.generic_dp_jobs += list.size();
   }
@@ -769,18 +833,10 @@ static void collect_counters(pbw_ctx* c, uint32_t reads) {
   HIP_CHECK(hipMemsetAsync(c->d_ctr.p, 0, sizeof(Counters), c->stream));
 }
 
-// SingleEndMapper::map (single.rs:164-207) for every read of a batch.
-static void map_se_batch(pbw_ctx* c, const pbw_read_batch* in, uint64_t b0, uint64_t b1, HostRecords& hr,
-                         bool resident) {
-  hipStream_t s = c->stream;
-  Batch b;
-  batch_upload_slice(c, b, in, b0, b1, resident);
-  HIP_CHECK(hipMemsetAsync(c->d_ctr.p, 0, sizeof(Counters), s));
-  batch_candidates(c, b);
-  auto t0 = std::chrono::steady_clock::now();
-  const uint32_t B = b.B;
-  c->d_rec.ensure((size_t)B * sizeof(pbw_record));
-  c->d_best.ensure(B);
+static void mlp_forward_dev(pbw_ctx* c, const uint16_t* dx, uint64_t n);
+
+// MAPQ arguments of a batch (records to d_rec, best slots to d_best).
+static MapqArgs mapq_args(pbw_ctx* c, const Batch& b) {
   MapqArgs ma{};
   ma.G = c->dev_genome();
   ma.R = b.R;
@@ -794,8 +850,127 @@ static void map_se_batch(pbw_ctx* c, const pbw_read_batch* in, uint64_t b0, uint
   ma.rcode = c->d_rcode.p;
   ma.rec = c->d_rec.p;
   ma.best = c->d_best.p;
+  return ma;
+}
+
+static uint32_t download_u32(pbw_ctx* c, const uint32_t* d) {
+  uint32_t v = 0;
+  HIP_CHECK(hipMemcpyAsync(&v, d, 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  return v;
+}
+
+// Candidate prefilter between clustering and the DP (no counterpart in the
+// reference): the network scores every candidate, the DP aligns the kept ones,
+// MAPQ uses the kept ones, and ambiguous or audited reads are re-run plain.
+static void prefilter_dp_mapq(pbw_ctx* c, Batch& b, uint64_t read_base) {
+  hipStream_t s = c->stream;
+  const uint32_t B = b.B;
+  const uint64_t nslots = (uint64_t)B * TOP_CANDIDATES;
+  auto tp = std::chrono::steady_clock::now();
+  mlp_forward_dev(c, c->d_feat.p, nslots);
+  c->d_pf_keep.ensure(B);
+  c->d_pf_list.ensure(nslots);
+  c->d_pf_n.ensure(2);
+  c->d_pf_ctr.ensure(8);
+  HIP_CHECK(hipMemsetAsync(c->d_pf_n.p, 0, 8, s));
+  HIP_CHECK(hipMemsetAsync(c->d_pf_ctr.p, 0, 8 * sizeof(unsigned long long), s));
+  PfArgs pa{};
+  pa.R = b.R;
+  pa.jobs = c->d_jobs.p;
+  pa.out = c->d_out.p;
+  pa.njobs = c->d_njobs.p;
+  pa.logit = c->d_mlp_z.p;
+  pa.threshold = c->pf.threshold;
+  pa.top_k = c->pf.top_k;
+  pa.keep = c->d_pf_keep.p;
+  pa.list = c->d_pf_list.p;
+  pa.nlist = c->d_pf_n.p;
+  pa.ctr = c->d_pf_ctr.p;
+  launch_pf_select(pa, s);
+  HIP_CHECK(hipGetLastError());
+  const uint32_t nk = download_u32(c, c->d_pf_n.p);
+  c->stats.ms_pf += ms_since(tp);
+  batch_dp(c, b, c->d_pf_list.p, nk);
+  auto t0 = std::chrono::steady_clock::now();
+  c->d_rec.ensure((size_t)B * sizeof(pbw_record));
+  c->d_best.ensure(B);
+  MapqArgs ma = mapq_args(c, b);
+  ma.keep = c->d_pf_keep.p;
   launch_mapq(ma, s);
   HIP_CHECK(hipGetLastError());
+  // re-run verdicts
+  c->d_verdict.ensure(B);
+  c->d_pf_rlist.ensure(B);
+  HIP_CHECK(hipMemsetAsync(c->d_pf_n.p, 0, 8, s));
+  pa.rec = c->d_rec.p;
+  pa.mapq_min = c->pf.mapq_fallback;
+  pa.close_nats = c->pf.close_nats;
+  pa.audit_period = c->pf.audit_period;
+  pa.read_base = read_base;
+  pa.verdict = c->d_verdict.p;
+  pa.rlist = c->d_pf_rlist.p;
+  pa.nrlist = c->d_pf_n.p + 1;
+  pa.audit = c->d_pf_ctr.p + 4;
+  launch_pf_verdict(pa, s);
+  HIP_CHECK(hipGetLastError());
+  uint32_t cnt[2];
+  HIP_CHECK(hipMemcpyAsync(cnt, c->d_pf_n.p, 8, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  c->stats.ms_mapq += ms_since(t0);
+  if (cnt[0]) batch_dp(c, b, c->d_pf_list.p, cnt[0]);
+  if (cnt[1]) {
+    t0 = std::chrono::steady_clock::now();
+    c->d_rec2.ensure((size_t)B * sizeof(pbw_record));
+    c->d_best2.ensure(B);
+    MapqArgs m2 = mapq_args(c, b);
+    m2.rec = c->d_rec2.p;
+    m2.best = c->d_best2.p;
+    m2.rlist = c->d_pf_rlist.p;
+    m2.nlist = cnt[1];
+    launch_mapq(m2, s);
+    pa.rec_out = c->d_rec.p;
+    pa.rec2 = c->d_rec2.p;
+    pa.best = c->d_best.p;
+    pa.best2 = c->d_best2.p;
+    pa.cigar = c->d_cigar.p;
+    pa.cigar_cap = b.cap;
+    launch_pf_merge(pa, cnt[1], s);
+    HIP_CHECK(hipGetLastError());
+    c->stats.ms_mapq += ms_since(t0);
+  }
+  unsigned long long ctr[8];
+  HIP_CHECK(hipMemcpyAsync(ctr, c->d_pf_ctr.p, sizeof(ctr), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  c->stats.pf_jobs += ctr[0];
+  c->stats.pf_jobs_aligned += ctr[1] + ctr[3];
+  c->stats.pf_audit_reads += ctr[4];
+  c->stats.pf_audit_mismatches += ctr[5];
+  c->stats.pf_fallback_reads += ctr[6];
+}
+
+// SingleEndMapper::map (single.rs:164-207) for every read of a batch.
+static void map_se_batch(pbw_ctx* c, const pbw_read_batch* in, uint64_t b0, uint64_t b1, HostRecords& hr,
+                         bool resident) {
+  hipStream_t s = c->stream;
+  Batch b;
+  batch_upload_slice(c, b, in, b0, b1, resident);
+  HIP_CHECK(hipMemsetAsync(c->d_ctr.p, 0, sizeof(Counters), s));
+  const bool pf = c->pf.enabled && c->mlp_ready;
+  batch_candidates(c, b, pf);
+  const uint32_t B = b.B;
+  if (pf) {
+    prefilter_dp_mapq(c, b, b0);
+  } else {
+    batch_dp(c, b, nullptr, 0);
+    auto t0 = std::chrono::steady_clock::now();
+    c->d_rec.ensure((size_t)B * sizeof(pbw_record));
+    c->d_best.ensure(B);
+    launch_mapq(mapq_args(c, b), s);
+    HIP_CHECK(hipGetLastError());
+    c->stats.ms_mapq += ms_since(t0);
+  }
+  auto t0 = std::chrono::steady_clock::now();
   std::vector<uint32_t> best(B);
   HIP_CHECK(hipMemcpyAsync(best.data(), c->d_best.p, (size_t)B * 4, hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
@@ -850,6 +1025,7 @@ static void map_pe_batch(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_ba
   Batch b;
   batch_upload(c, b, sp, sl, qp, ql);
   batch_candidates(c, b);
+  batch_dp(c, b, nullptr, 0);
   auto t0 = std::chrono::steady_clock::now();
   // cross-alignment plan
   PairArgs pa{};
@@ -993,6 +1169,186 @@ static int guarded(pbw_ctx* c, F&& f) {
 }
 
 // ===========================================================================
+// candidate prefilter network: parameter upload, forward, gradient, training
+// ===========================================================================
+static void mlp_set(pbw_ctx* c, const float* theta) {
+  hipStream_t s = c->stream;
+  MlpDev P = c->mlp();
+  HIP_CHECK(hipMemcpyAsync(P.theta, theta, MLP_NP * sizeof(float), hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemsetAsync(P.m, 0, 2 * (size_t)MLP_NP * sizeof(float), s));
+  launch_mlp_publish(P, s);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(s));
+  c->mlp_ready = true;
+  c->mlp_step = 0;
+}
+
+// He-uniform weights (U(-sqrt(6/fan_in), +)), zero biases; splitmix64 stream.
+static void mlp_init_host(uint64_t seed, std::vector<float>& th) {
+  th.assign(MLP_NP, 0.f);
+  uint64_t x = seed * 0x9E3779B97F4A7C15ull + 1;
+  auto uni = [&]() {
+    x += 0x9E3779B97F4A7C15ull;
+    uint64_t z = x;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  };
+  const double a1 = std::sqrt(6.0 / MLP_IN), a2 = std::sqrt(6.0 / MLP_H);
+  for (int p = MLP_OFF_W1; p < MLP_OFF_B1; p++) th[p] = (float)((2 * uni() - 1) * a1);
+  for (int p = MLP_OFF_W2; p < MLP_OFF_B2; p++) th[p] = (float)((2 * uni() - 1) * a2);
+  for (int p = MLP_OFF_W3; p < MLP_OFF_B3; p++) th[p] = (float)((2 * uni() - 1) * a2);
+}
+
+static void mlp_require(pbw_ctx* c) {
+  if (!c->mlp_ready) throw PbwError{PBW_ERR_ARG, "prefilter network has no parameters (pbw_mlp_init or pbw_mlp_set_params)"};
+}
+
+// Forward over n rows already on the device; logits to d_mlp_z.
+static void mlp_forward_dev(pbw_ctx* c, const uint16_t* dx, uint64_t n) {
+  c->d_mlp_z.ensure(n);
+  launch_mlp_forward(c->mlp(), dx, n, c->d_mlp_z.p, c->stream);
+  HIP_CHECK(hipGetLastError());
+}
+
+// One Adam epoch over the device dataset (d_mlp_x, d_mlp_y) in a shuffled order; returns the mean loss.
+static double mlp_epoch(pbw_ctx* c, uint64_t n, const pbw_train_options& o, uint64_t epoch) {
+  hipStream_t s = c->stream;
+  std::vector<uint32_t> perm(n);
+  for (uint64_t i = 0; i < n; i++) perm[i] = (uint32_t)i;
+  std::mt19937_64 rng(o.seed * 1000003ull + epoch);
+  std::shuffle(perm.begin(), perm.end(), rng);
+  c->d_mlp_perm.upload(perm.data(), n, s);
+  const uint64_t B = o.batch;
+  const uint64_t steps = (n + B - 1) / B;
+  const uint32_t tiles_max = (uint32_t)((B + 31) / 32);
+  c->d_mlp_part.ensure((size_t)tiles_max * MLP_NP);
+  c->d_mlp_loss.ensure((size_t)steps * tiles_max);
+  HIP_CHECK(hipMemsetAsync(c->d_mlp_loss.p, 0, (size_t)steps * tiles_max * sizeof(float), s));
+  MlpDev P = c->mlp();
+  for (uint64_t st = 0; st < steps; st++) {
+    const uint64_t r0 = st * B, nr = std::min<uint64_t>(B, n - r0);
+    launch_mlp_grad(P, c->d_mlp_x.p, c->d_mlp_y.p, c->d_mlp_perm.p, r0, nr, n, o.pos_weight, 1.0f / (float)nr,
+                    c->d_mlp_part.p, c->d_mlp_loss.p + st * tiles_max, s);
+    c->mlp_step++;
+    const float bc1 = 1.f - std::pow(o.beta1, (float)c->mlp_step), bc2 = 1.f - std::pow(o.beta2, (float)c->mlp_step);
+    launch_mlp_reduce_adam(P, c->d_mlp_part.p, (uint32_t)((nr + 31) / 32), nullptr, 1, o.lr, o.beta1, o.beta2, o.eps,
+                           bc1, bc2, s);
+  }
+  HIP_CHECK(hipGetLastError());
+  std::vector<float> lp((size_t)steps * tiles_max);
+  HIP_CHECK(hipMemcpyAsync(lp.data(), c->d_mlp_loss.p, lp.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  double tot = 0.0;
+  for (float v : lp) tot += v;
+  return tot / (double)n;
+}
+
+// Training rows of the prefilter: plain mapping with features on; per aligned
+// candidate its features, label (log-likelihood within label_nats of the
+// read's best) and rank.
+struct PfRows {
+  std::vector<uint16_t> x;
+  std::vector<float> y;
+  std::vector<uint8_t> rank;
+};
+static void prefilter_rows(pbw_ctx* c, const pbw_read_batch* in, float label_nats, PfRows& out) {
+  hipStream_t s = c->stream;
+  const uint64_t bs = c->opt.batch_reads ? c->opt.batch_reads : 131072;
+  for (uint64_t b0 = 0; b0 < in->n; b0 += bs) {
+    const uint64_t b1 = std::min<uint64_t>(in->n, b0 + bs);
+    Batch b;
+    batch_upload_slice(c, b, in, b0, b1, false);
+    HIP_CHECK(hipMemsetAsync(c->d_ctr.p, 0, sizeof(Counters), s));
+    batch_candidates(c, b, true);
+    batch_dp(c, b, nullptr, 0);
+    const uint32_t B = b.B;
+    const size_t ns = (size_t)B * TOP_CANDIDATES;
+    std::vector<uint8_t> nj(B);
+    std::vector<DpJob> jobs(ns);
+    std::vector<DpOut> dout(ns);
+    std::vector<uint16_t> feat(ns * MLP_IN);
+    HIP_CHECK(hipMemcpyAsync(nj.data(), c->d_njobs.p, B, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(jobs.data(), c->d_jobs.p, ns * sizeof(DpJob), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(dout.data(), c->d_out.p, ns * sizeof(DpOut), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(feat.data(), c->d_feat.p, ns * MLP_IN * 2, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    for (uint32_t r = 0; r < B; r++) {
+      const size_t s0 = (size_t)r * TOP_CANDIDATES;
+      if (!nj[r]) continue;
+      double best = dout[s0].log_lk;
+      for (uint32_t k = 1; k < nj[r]; k++) best = std::max(best, dout[s0 + k].log_lk);
+      for (uint32_t k = 0; k < nj[r]; k++) {
+        const uint32_t cand = jobs[s0 + k].cand;
+        out.x.insert(out.x.end(), feat.begin() + (size_t)cand * MLP_IN, feat.begin() + (size_t)(cand + 1) * MLP_IN);
+        out.y.push_back(dout[s0 + k].log_lk >= best - (double)label_nats ? 1.f : 0.f);
+        out.rank.push_back((uint8_t)(cand - s0));
+      }
+    }
+  }
+  HIP_CHECK(hipMemsetAsync(c->d_ctr.p, 0, sizeof(Counters), s));
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+static void prefilter_train(pbw_ctx* c, const pbw_read_batch* in, const pbw_train_options& o, float label_nats,
+                            float recall, pbw_prefilter_report& rep) {
+  PfRows rows;
+  prefilter_rows(c, in, label_nats, rows);
+  const uint64_t n = rows.y.size();
+  if (n < 20) throw PbwError{PBW_ERR_ARG, "too few candidate rows to train the prefilter"};
+  const uint64_t nval = n / 10, ntr = n - nval;
+  if (!c->mlp_ready) {
+    std::vector<float> th;
+    mlp_init_host(o.seed, th);
+    mlp_set(c, th.data());
+  }
+  hipStream_t s = c->stream;
+  c->d_mlp_x.upload(rows.x.data(), ntr * MLP_IN, s);
+  c->d_mlp_y.upload(rows.y.data(), ntr, s);
+  for (uint32_t e = 0; e < o.epochs; e++) {
+    const double l = mlp_epoch(c, ntr, o, e);
+    if (e == 0) rep.loss_first = l;
+    rep.loss_last = l;
+  }
+  // threshold: the (1 - recall) quantile of held-out positives the rank rule does not keep
+  c->d_mlp_x.upload(rows.x.data() + ntr * MLP_IN, nval * MLP_IN, s);
+  mlp_forward_dev(c, c->d_mlp_x.p, nval);
+  std::vector<float> z(nval);
+  if (nval) HIP_CHECK(hipMemcpyAsync(z.data(), c->d_mlp_z.p, nval * 4, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  std::vector<float> pos;
+  uint64_t npos = 0;
+  for (uint64_t i = 0; i < nval; i++) {
+    const bool p = rows.y[ntr + i] > 0.5f;
+    npos += p;
+    if (p && rows.rank[ntr + i] >= c->pf.top_k) pos.push_back(z[i]);
+  }
+  std::sort(pos.begin(), pos.end());
+  float thr = 3.0e38f;
+  if (!pos.empty()) {
+    const size_t idx = (size_t)std::floor((1.0 - (double)recall) * (double)pos.size());
+    thr = pos[std::min(idx, pos.size() - 1)];
+  }
+  uint64_t kept_pos = 0, skipped = 0;
+  for (uint64_t i = 0; i < nval; i++) {
+    const bool keep = rows.rank[ntr + i] < c->pf.top_k || z[i] >= thr;
+    if (rows.y[ntr + i] > 0.5f && keep) kept_pos++;
+    if (!keep) skipped++;
+  }
+  uint64_t tot_pos = 0;
+  for (float v : rows.y) tot_pos += v > 0.5f;
+  rep.rows = n;
+  rep.positives = tot_pos;
+  rep.val_rows = nval;
+  rep.val_positives = npos;
+  rep.threshold = thr;
+  rep.val_recall = npos ? (double)kept_pos / (double)npos : 1.0;
+  rep.val_skip = nval ? (double)skipped / (double)nval : 0.0;
+  c->pf.threshold = thr;
+}
+
+// ===========================================================================
 // C ABI
 // ===========================================================================
 extern "C" {
@@ -1076,6 +1432,7 @@ int pbw_ctx_create(const pbw_genome* g, const pbw_options* o, int device, pbw_ct
   if (o) c->opt = *o;
   else pbw_default_options(&c->opt);
   c->device = device;
+  pbw_default_prefilter_options(&c->pf);
   int rc = guarded(c, [&] { ctx_build(c); });
   if (rc != PBW_OK) {
     g_error = c->error;
@@ -1393,5 +1750,145 @@ char* pbw_format_sam(const pbw_genome* g, const pbw_read_batch* rd, const char* 
   return t;
 }
 void pbw_free_text(char* t) { free(t); }
+
+void pbw_default_train_options(pbw_train_options* o) {
+  if (!o) return;
+  o->epochs = 20;
+  o->batch = 4096;
+  o->lr = 1e-3f;
+  o->beta1 = 0.9f;
+  o->beta2 = 0.999f;
+  o->eps = 1e-8f;
+  o->pos_weight = 1.0f;
+  o->seed = 1;
+}
+
+void pbw_default_prefilter_options(pbw_prefilter_options* o) {
+  if (!o) return;
+  o->enabled = 0;
+  o->threshold = 0.f;
+  o->top_k = 1;
+  o->mapq_fallback = 20;
+  o->close_nats = 10.f;
+  o->audit_period = 64;
+}
+
+int pbw_prefilter_set(pbw_ctx* c, const pbw_prefilter_options* o) {
+  if (!c || !o) return PBW_ERR_ARG;
+  if (o->enabled && !c->mlp_ready) {
+    c->error = "prefilter enabled without network parameters";
+    return PBW_ERR_ARG;
+  }
+  c->pf = *o;
+  return PBW_OK;
+}
+
+int pbw_prefilter_get(const pbw_ctx* c, pbw_prefilter_options* o) {
+  if (!c || !o) return PBW_ERR_ARG;
+  *o = c->pf;
+  return PBW_OK;
+}
+
+int pbw_prefilter_dataset(pbw_ctx* c, const pbw_read_batch* reads, float label_nats, uint16_t* x, float* y,
+                          uint8_t* rank, uint64_t cap, uint64_t* n) {
+  if (!c || !reads || !n) return PBW_ERR_ARG;
+  return guarded(c, [&] {
+    PfRows rows;
+    prefilter_rows(c, reads, label_nats, rows);
+    const uint64_t m = rows.y.size(), w = std::min<uint64_t>(m, cap);
+    if (x && w) std::memcpy(x, rows.x.data(), w * MLP_IN * 2);
+    if (y && w) std::memcpy(y, rows.y.data(), w * 4);
+    if (rank && w) std::memcpy(rank, rows.rank.data(), w);
+    *n = m;
+  });
+}
+
+int pbw_prefilter_train(pbw_ctx* c, const pbw_read_batch* reads, const pbw_train_options* o, float label_nats,
+                        float recall, pbw_prefilter_report* out) {
+  if (!c || !reads || !o || !o->batch) return PBW_ERR_ARG;
+  return guarded(c, [&] {
+    pbw_prefilter_report rep{};
+    prefilter_train(c, reads, *o, label_nats, recall, rep);
+    if (out) *out = rep;
+  });
+}
+
+int pbw_mlp_init(pbw_ctx* c, uint64_t seed) {
+  if (!c) return PBW_ERR_ARG;
+  return guarded(c, [&] {
+    std::vector<float> th;
+    mlp_init_host(seed, th);
+    mlp_set(c, th.data());
+  });
+}
+
+int pbw_mlp_set_params(pbw_ctx* c, const float* theta, uint64_t n) {
+  if (!c || !theta || n != (uint64_t)MLP_NP) return PBW_ERR_ARG;
+  return guarded(c, [&] { mlp_set(c, theta); });
+}
+
+int pbw_mlp_get_params(pbw_ctx* c, float* theta, uint64_t n) {
+  if (!c || !theta || n != (uint64_t)MLP_NP) return PBW_ERR_ARG;
+  return guarded(c, [&] {
+    mlp_require(c);
+    HIP_CHECK(hipMemcpyAsync(theta, c->mlp().theta, MLP_NP * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int pbw_mlp_forward(pbw_ctx* c, const uint16_t* x, uint64_t n, float* logits) {
+  if (!c || (n && (!x || !logits))) return PBW_ERR_ARG;
+  return guarded(c, [&] {
+    mlp_require(c);
+    if (!n) return;
+    c->d_mlp_x.upload(x, n * MLP_IN, c->stream);
+    mlp_forward_dev(c, c->d_mlp_x.p, n);
+    HIP_CHECK(hipMemcpyAsync(logits, c->d_mlp_z.p, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int pbw_mlp_gradient(pbw_ctx* c, const uint16_t* x, const float* y, uint64_t n, float pos_weight, float* grad,
+                     double* loss) {
+  if (!c || !n || !x || !y || !grad) return PBW_ERR_ARG;
+  return guarded(c, [&] {
+    mlp_require(c);
+    hipStream_t s = c->stream;
+    c->d_mlp_x.upload(x, n * MLP_IN, s);
+    c->d_mlp_y.upload(y, n, s);
+    const uint32_t tiles = (uint32_t)((n + 31) / 32);
+    c->d_mlp_part.ensure((size_t)tiles * MLP_NP);
+    c->d_mlp_loss.ensure(tiles);
+    c->d_mlp_grad.ensure(MLP_NP);
+    MlpDev P = c->mlp();
+    launch_mlp_grad(P, c->d_mlp_x.p, c->d_mlp_y.p, nullptr, 0, n, n, pos_weight, 1.0f / (float)n, c->d_mlp_part.p,
+                    c->d_mlp_loss.p, s);
+    launch_mlp_reduce_adam(P, c->d_mlp_part.p, tiles, c->d_mlp_grad.p, 0, 0.f, 0.f, 0.f, 0.f, 1.f, 1.f, s);
+    HIP_CHECK(hipGetLastError());
+    std::vector<float> lp(tiles);
+    HIP_CHECK(hipMemcpyAsync(grad, c->d_mlp_grad.p, MLP_NP * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(lp.data(), c->d_mlp_loss.p, tiles * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (loss) {
+      double t = 0.0;
+      for (float v : lp) t += v;
+      *loss = t / (double)n;
+    }
+  });
+}
+
+int pbw_mlp_train(pbw_ctx* c, const uint16_t* x, const float* y, uint64_t n, const pbw_train_options* o,
+                  double* epoch_loss) {
+  if (!c || !n || !x || !y || !o || !o->batch || n > 0xFFFFFFFFull) return PBW_ERR_ARG;
+  return guarded(c, [&] {
+    mlp_require(c);
+    c->d_mlp_x.upload(x, n * MLP_IN, c->stream);
+    c->d_mlp_y.upload(y, n, c->stream);
+    for (uint32_t e = 0; e < o->epochs; e++) {
+      const double l = mlp_epoch(c, n, *o, e);
+      if (epoch_loss) epoch_loss[e] = l;
+    }
+  });
+}
 
 }  // extern "C"

@@ -117,6 +117,7 @@ struct SeedArgs {
   uint32_t* nhits;         // per read (indexed by read id)
   uint8_t* rcode;          // per window repeat code, indexed by seq_off[read] + p
   Counters* ctr;
+  uint8_t* hflag;          // optional, parallel to hits: bit 7 neighbour probe, bits 0..3 bucket-size class
 };
 void launch_seed(const SeedArgs& a, hipStream_t s);
 
@@ -131,6 +132,11 @@ struct ClusterArgs {
   uint32_t* ntier2;
   uint64_t* scratch;       // tier 2: per-block scratch (keys)
   uint64_t scratch_per_block;
+  // prefilter features (optional): per hit flags from the seed pass, per
+  // candidate slot (read * 20 + rank) 32 BF16 features
+  const uint8_t* hflag;
+  const uint8_t* rcode;
+  uint16_t* feat;
 };
 void launch_cluster(const ClusterArgs& a, hipStream_t s);
 uint64_t cluster_scratch_words(uint32_t max_hits);  // per-block scratch of the block tier (u64 words)
@@ -187,8 +193,44 @@ struct MapqArgs {
   const uint8_t* rcode;
   uint8_t* rec;            // pbw_record array (64 B each)
   uint32_t* best;          // per read best job slot or UINT32_MAX
+  const uint32_t* keep;    // optional per read mask of the job slots to use (prefilter)
+  const uint32_t* rlist;   // optional list of reads to finalize (nullptr = all)
+  uint32_t nlist;
 };
 void launch_mapq(const MapqArgs& a, hipStream_t s);
+
+// Prefilter selection, re-run verdict and merge (pbw_kernels.hip).
+struct PfArgs {
+  DevReads R;
+  const DpJob* jobs;
+  const DpOut* out;
+  const uint8_t* njobs;
+  const float* logit;        // per candidate slot (read * 20 + rank)
+  float threshold;           // keep a candidate whose logit >= threshold
+  uint32_t top_k;            // always keep ranks < top_k
+  uint32_t* keep;            // per read: mask of kept job slots
+  uint32_t* list;            // job slots to align
+  uint32_t* nlist;
+  unsigned long long* ctr;   // [0] jobs, [1] kept jobs, [2] cells skipped (reference count), [3] re-run jobs
+  const uint8_t* rec;        // prefiltered records
+  uint32_t mapq_min;         // re-run when the prefiltered MAPQ is below this
+  float close_nats;          // .. or a kept second candidate is within this of the best
+  uint32_t audit_period;     // re-run every audit_period-th read to measure concordance (0: never)
+  uint64_t read_base;        // global index of the batch's first read
+  uint8_t* verdict;          // per read: 0 as is, 1 fallback, 2 audit
+  uint32_t* rlist;           // re-run reads
+  uint32_t* nrlist;
+  uint8_t* rec_out;          // records to patch (fallback) or compare (audit)
+  const uint8_t* rec2;       // plain records of the re-run reads
+  uint32_t* best;
+  const uint32_t* best2;
+  const uint32_t* cigar;
+  uint32_t cigar_cap;
+  unsigned long long* audit; // [0] audited reads, [1] audited reads whose SAM fields differ, [2] fallback reads
+};
+void launch_pf_select(const PfArgs& a, hipStream_t s);
+void launch_pf_verdict(const PfArgs& a, hipStream_t s);
+void launch_pf_merge(const PfArgs& a, uint32_t n, hipStream_t s);
 
 struct PairArgs {
   DevGenome G;
@@ -226,4 +268,41 @@ void gather_cigar(const uint32_t* cig, const uint64_t* src, const uint32_t* off,
                   uint32_t* out, hipStream_t s);
 void launch_fill_koff(uint32_t* koff, const uint32_t* keys, const uint32_t* kstart, uint64_t nkeys, uint32_t total,
                       hipStream_t s);
+
+__device__ __forceinline__ uint16_t float_to_bf16(float f) {  // round to nearest even
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+// ---- candidate prefilter network (pbw_mlp.hip) ----
+constexpr int MLP_IN = 32, MLP_H = 128;
+constexpr int MLP_OFF_W1 = 0;                          // W1 [f][k], 32 x 128
+constexpr int MLP_OFF_B1 = MLP_OFF_W1 + MLP_IN * MLP_H;
+constexpr int MLP_OFF_W2 = MLP_OFF_B1 + MLP_H;         // W2 [k][n], 128 x 128
+constexpr int MLP_OFF_B2 = MLP_OFF_W2 + MLP_H * MLP_H;
+constexpr int MLP_OFF_W3 = MLP_OFF_B2 + MLP_H;         // W3 [n]
+constexpr int MLP_OFF_B3 = MLP_OFF_W3 + MLP_H;
+constexpr int MLP_NP = MLP_OFF_B3 + 1;                 // 20865 parameters
+struct MlpDev {
+  float* theta;   // FP32 master parameters, layout above
+  float* m;       // Adam moments
+  float* v;
+  uint16_t* w1t;  // BF16 W1 as [k][f]
+  uint16_t* w2t;  // BF16 W2 as [n][k]
+  uint16_t* w2;   // BF16 W2 as [k][n]
+  uint16_t* w3;   // BF16 W3 [n]
+  float* b1;
+  float* b2;
+  float* b3;
+};
+size_t mlp_lds_bytes();
+void launch_mlp_forward(const MlpDev& P, const uint16_t* X, uint64_t N, float* z, hipStream_t s);
+void launch_mlp_grad(const MlpDev& P, const uint16_t* X, const float* Y, const uint32_t* perm, uint64_t row0,
+                     uint64_t nrows, uint64_t N, float pos_weight, float inv_b, float* part, float* loss_part,
+                     hipStream_t s);
+void launch_mlp_reduce_adam(const MlpDev& P, const float* part, uint32_t ntiles, float* grad_out, int adam, float lr,
+                            float b1, float b2, float eps, float bc1, float bc2, hipStream_t s);
+void launch_mlp_publish(const MlpDev& P, hipStream_t s);
+
 }  // namespace pbw

@@ -318,6 +318,7 @@ __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
           const uint32_t e = pb + lane;
           bool pass = false;
           uint64_t key = 0;
+          uint32_t hfl = 0;
           if (e < PT) {
             int lo = 0, hi = 63;  // first l with r_incl[l] > e
             while (lo < hi) {
@@ -336,10 +337,15 @@ __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
             const int64_t rp = (int64_t)(cb + ww);
             const int64_t diag = rev ? (int64_t)pos + rp : (int64_t)pos - rp;  // candidates.rs:37-40
             key = ((uint64_t)(contig * 2u + (rev ? 1u : 0u)) << DIAG_BITS) | (uint64_t)(diag + DIAG_BIAS);
+            // prefilter flags: neighbour probe, bit length of the bucket size
+            hfl = (S.r_nb[l] ? 0x80u : 0u) | (uint32_t)(32 - __builtin_clz((S.r_incl[l] - excl) | 1u));
           }
           const uint64_t m = __ballot(pass);
           const uint32_t pre = (uint32_t)__popcll(m & lanemask_lt());
-          if (pass && hcount + pre < cap) a.hits[hbase + hcount + pre] = key;
+          if (pass && hcount + pre < cap) {
+            a.hits[hbase + hcount + pre] = key;
+            if (a.hflag) a.hflag[hbase + hcount + pre] = (uint8_t)hfl;
+          }
           hcount += (uint32_t)__popcll(m);
           c_filt += pass ? 1 : 0;
         }
@@ -609,9 +615,22 @@ __device__ __forceinline__ int64_t hit_diag(uint64_t k) {
 // find_candidates (candidates.rs:30-78) of one read by a team of T threads.
 // Returns false (uniformly) when the read has more groups than M.gcap; the
 // caller then hands it to the block tier.
-template <int T>
+// Prefilter features of one read (only for the FEAT instances of the cluster kernels).
+struct ClusterFeat {
+  const uint8_t* hflag;  // this read's hit flags, emission order
+  const uint8_t* qual;
+  uint32_t qlen;
+  const uint8_t* rcode;  // this read's window repeat codes
+  uint16_t* out;         // this read's 20 feature rows of 32 BF16
+  uint32_t* f1;          // per cluster: neighbour hits | min bucket class << 24
+  uint32_t* f2;          // per cluster: sum of bucket classes
+  uint32_t* f3;          // per cluster: diagonal span
+  uint32_t* cnt;         // team counters: clusters tied at the top count, quality sum, q < 10, q < 20, repeat windows
+};
+
+template <int T, bool FEAT>
 __device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const ClusterMem& M, Cand* out,
-                             uint8_t* ncand_out, uint32_t* sh) {
+                             uint8_t* ncand_out, uint32_t* sh, const ClusterFeat& F) {
   const uint32_t tid = threadIdx.x % T;
   uint32_t p2 = 1;
   while (p2 < n) p2 <<= 1;
@@ -674,6 +693,38 @@ __device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const
     atomicMin(&M.g_min[gid], pmin);
   }
   team_sync<T>();
+  if constexpr (FEAT) {
+    // per-cluster aggregates over its sorted hits, before the keys are overwritten
+    for (uint32_t c = tid; c < C; c += T) {
+      const uint32_t s0 = M.cl_start[c], e0 = M.cl_start[c + 1];
+      uint32_t nb = 0, ms = 0, mn = 15;
+      for (uint32_t x = s0; x < e0; x++) {
+        const uint32_t f = F.hflag[M.idx[x]];
+        nb += f >> 7;
+        ms += f & 15u;
+        mn = min(mn, f & 15u);
+      }
+      F.f1[c] = nb | (mn << 24);
+      F.f2[c] = ms;
+      F.f3[c] = (uint32_t)min((int64_t)0xFFFFFFF, hit_diag(M.key[e0 - 1]) - hit_diag(M.key[s0]));
+    }
+    if (tid < 5) F.cnt[tid] = 0;
+    team_sync<T>();
+    uint32_t qs = 0, q10 = 0, q20 = 0, rep = 0;
+    for (uint32_t x = tid; x < F.qlen; x += T) {
+      const uint32_t q = F.qual[x];
+      qs += q;
+      q10 += q < 10;
+      q20 += q < 20;
+    }
+    const uint32_t nwin = L >= (uint32_t)K ? L - K + 1 : 0;
+    for (uint32_t x = tid; x < nwin; x += T) rep += F.rcode[x] != 0;
+    if (qs) atomicAdd(&F.cnt[1], qs);
+    if (q10) atomicAdd(&F.cnt[2], q10);
+    if (q20) atomicAdd(&F.cnt[3], q20);
+    if (rep) atomicAdd(&F.cnt[4], rep);
+  }
+  team_sync<T>();
   // insertion order of the groups = order of first appearance in the hit list
   if (G <= 1024) {
     for (uint32_t g = tid; g < G; g += T) {
@@ -716,6 +767,52 @@ __device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const
   team_sync<T>();
   team_bitonic<T>(M.key, M.idx, pc, tid);
   const uint32_t nt = min(C, (uint32_t)TOP_CANDIDATES);
+  if constexpr (FEAT) {
+    const uint64_t top = M.key[0] >> 42;
+    for (uint32_t c = tid; c < C; c += T)
+      if ((M.key[c] >> 42) == top) atomicAdd(&F.cnt[0], 1u);
+    team_sync<T>();
+    if (tid < nt) {
+      const uint32_t c = M.idx[tid];
+      const float cnt = (float)(M.cl_start[c + 1] - M.cl_start[c]);
+      const float c0 = (float)(M.cl_start[M.idx[0] + 1] - M.cl_start[M.idx[0]]);
+      const float c1 = C > 1 ? (float)(M.cl_start[M.idx[1] + 1] - M.cl_start[M.idx[1]]) : 0.f;
+      const float nb = (float)(F.f1[c] & 0xFFFFFFu), mn = (float)(F.f1[c] >> 24);
+      const float ms = (float)F.f2[c], span = (float)F.f3[c];
+      const float nwin = (float)(L >= (uint32_t)K ? L - K + 1 : 1);
+      const float ql = (float)max(F.qlen, 1u);
+      float f[32];
+      f[0] = log2f(1.f + cnt);
+      f[1] = cnt / c0;
+      f[2] = log2f(1.f + nb);
+      f[3] = nb / cnt;
+      f[4] = (cnt - nb) / cnt;
+      f[5] = log2f(1.f + span);
+      f[6] = span / fmaxf(1.f, cnt - 1.f);
+      f[7] = ms / cnt / 9.f;
+      f[8] = mn / 9.f;
+      f[9] = (float)tid / 20.f;
+      f[10] = tid == 0 ? 1.f : 0.f;
+      f[11] = (M.g_key[M.cl_gid[c]] & 1u) ? 1.f : 0.f;
+      f[12] = log2f(1.f + c0);
+      f[13] = log2f(1.f + c1);
+      f[14] = cnt == c0 ? 1.f : 0.f;
+      f[15] = (float)F.cnt[0] / 20.f;
+      f[16] = log2f(1.f + (float)C) / 10.f;
+      f[17] = log2f(1.f + (float)n) / 10.f;
+      f[18] = (float)L / 150.f;
+      f[19] = (float)F.cnt[1] / ql / 40.f;
+      f[20] = (float)F.cnt[2] / ql;
+      f[21] = (float)F.cnt[3] / ql;
+      f[22] = cnt / nwin;
+      f[23] = (float)F.cnt[4] / nwin;
+      f[24] = (float)G / 20.f;
+      f[25] = f[0] - f[12];
+      f[26] = (c0 - c1) / c0;
+      for (int u = 27; u < 32; u++) f[u] = 0.f;
+      for (int u = 0; u < 32; u++) F.out[tid * 32 + u] = float_to_bf16(f[u]);
+    }
+  }
   if (tid < nt) {
     const uint32_t c = M.idx[tid];
     const uint32_t s = M.cl_start[c], cnt = M.cl_start[c + 1] - s;
@@ -748,8 +845,10 @@ struct ClusterLds {
   int32_t table[2 * CL_GCAP], tmp[CL_GCAP];
 };
 
+template <bool FEAT>
 __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
   __shared__ ClusterLds S;
+  __shared__ uint32_t FX[FEAT ? 3 * CL_NCAP + 8 : 1];
   const ClusterMem M{S.key, S.idx, S.cl_sum, S.cl_start, S.cl_gid, S.g_key, S.g_min, S.g_rank, S.g_sort, S.table,
                      S.tmp, CL_GCAP};
   for (uint32_t r = blockIdx.x; r < a.R.n; r += gridDim.x) {
@@ -761,8 +860,21 @@ __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
     bool done = false;
     if (n <= CL_NCAP) {
       const uint32_t L = (uint32_t)(a.R.seq_off[r + 1] - a.R.seq_off[r]);
-      done = cluster_read<64>(a.hits + a.hoff[r], n, L, M, a.cands + (uint64_t)r * TOP_CANDIDATES, a.ncand + r,
-                              nullptr);
+      ClusterFeat F{};
+      if (FEAT) {
+        const uint64_t so = a.R.seq_off[r], qo = a.R.qual_off[r];
+        F.hflag = a.hflag + a.hoff[r];
+        F.qual = a.R.qual + qo;
+        F.qlen = (uint32_t)(a.R.qual_off[r + 1] - qo);
+        F.rcode = a.rcode + so;
+        F.out = a.feat + (uint64_t)r * TOP_CANDIDATES * MLP_IN;
+        F.f1 = FX;
+        F.f2 = FX + CL_NCAP;
+        F.f3 = FX + 2 * CL_NCAP;
+        F.cnt = FX + 3 * CL_NCAP;
+      }
+      done = cluster_read<64, FEAT>(a.hits + a.hoff[r], n, L, M, a.cands + (uint64_t)r * TOP_CANDIDATES,
+                                    a.ncand + r, nullptr, F);
     }
     if (!done && lane_id() == 0) {
       const uint32_t slot = atomicAdd(a.ntier2, 1u);
@@ -774,6 +886,7 @@ __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
 // Block tier: reads with more hits (or groups) than the wave tier holds; a
 // block of 256 threads per read, arrays in per-block global scratch of
 // cluster_scratch_words(max hits) words.
+template <bool FEAT>
 __global__ __launch_bounds__(256) void k_cluster_tier2(ClusterArgs a, const uint32_t* list, uint32_t nlist) {
   __shared__ uint32_t sh[4];
   for (uint32_t li = blockIdx.x; li < nlist; li += gridDim.x) {
@@ -797,25 +910,42 @@ __global__ __launch_bounds__(256) void k_cluster_tier2(ClusterArgs a, const uint
     M.tmp = (int32_t*)(M.g_sort + p2);
     M.table = M.tmp + p2;  // 4 * p2 buckets
     M.gcap = p2;
-    cluster_read<256>(a.hits + a.hoff[r], n, L, M, a.cands + (uint64_t)r * TOP_CANDIDATES, a.ncand + r, sh);
+    ClusterFeat F{};
+    if (FEAT) {
+      const uint64_t so = a.R.seq_off[r], qo = a.R.qual_off[r];
+      uint32_t* fx = (uint32_t*)(M.table + 4 * p2);
+      F.hflag = a.hflag + a.hoff[r];
+      F.qual = a.R.qual + qo;
+      F.qlen = (uint32_t)(a.R.qual_off[r + 1] - qo);
+      F.rcode = a.rcode + so;
+      F.out = a.feat + (uint64_t)r * TOP_CANDIDATES * MLP_IN;
+      F.f1 = fx;
+      F.f2 = fx + p2;
+      F.f3 = fx + 2 * p2;
+      F.cnt = fx + 3 * p2;
+    }
+    cluster_read<256, FEAT>(a.hits + a.hoff[r], n, L, M, a.cands + (uint64_t)r * TOP_CANDIDATES, a.ncand + r, sh,
+                            F);
   }
 }
 
 void launch_cluster(const ClusterArgs& a, hipStream_t s) {
   if (a.R.n == 0) return;
   uint32_t blocks = a.R.n < 65536 ? a.R.n : 65536;
-  hipLaunchKernelGGL(k_cluster, dim3(blocks), dim3(64), 0, s, a);
+  if (a.feat) hipLaunchKernelGGL(k_cluster<true>, dim3(blocks), dim3(64), 0, s, a);
+  else hipLaunchKernelGGL(k_cluster<false>, dim3(blocks), dim3(64), 0, s, a);
 }
 uint64_t cluster_scratch_words(uint32_t max_hits) {
   uint64_t p2 = 1;
   while (p2 < max_hits) p2 <<= 1;
   // key + cl_sum: 2 p2 words; idx, cl_start (+1), cl_gid, g_key, g_min, g_rank, g_sort, tmp
-  // (p2 u32 each) and table (4 p2 u32): 12 p2 + 1 u32
-  return 2 * p2 + 6 * p2 + 1 + 8;
+  // (p2 u32 each), table (4 p2 u32) and the feature arrays (3 p2 + 8 u32): 15 p2 + 9 u32
+  return 2 * p2 + 8 * p2 + 16;
 }
 void launch_cluster_tier2(const ClusterArgs& a, const uint32_t* list, uint32_t n, uint32_t blocks, hipStream_t s) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_cluster_tier2, dim3(blocks), dim3(256), 0, s, a, list, n);
+  if (a.feat) hipLaunchKernelGGL(k_cluster_tier2<true>, dim3(blocks), dim3(256), 0, s, a, list, n);
+  else hipLaunchKernelGGL(k_cluster_tier2<false>, dim3(blocks), dim3(256), 0, s, a, list, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -2158,7 +2288,12 @@ static_assert(sizeof(RecOut) == 64, "record layout");
 
 __global__ void k_mapq(MapqArgs a) {
   uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= a.R.n) return;
+  if (a.rlist) {
+    if (r >= a.nlist) return;
+    r = a.rlist[r];
+  } else if (r >= a.R.n) {
+    return;
+  }
   const uint64_t so = a.R.seq_off[r], qo = a.R.qual_off[r];
   const uint32_t L = (uint32_t)(a.R.seq_off[r + 1] - so), qlen = (uint32_t)(a.R.qual_off[r + 1] - qo);
   RecOut o;
@@ -2183,9 +2318,14 @@ __global__ void k_mapq(MapqArgs a) {
     return;
   }
   const uint64_t s0 = (uint64_t)r * TOP_CANDIDATES;
-  double mx = a.out[s0].log_lk;
-  uint32_t bi = 0;
+  // slots in use: all, or the prefilter's kept ones (never empty: the top ranks are kept)
+  uint32_t use = nj >= 32 ? 0xFFFFFFFFu : ((1u << nj) - 1u);
+  if (a.keep && (a.keep[r] & use)) use &= a.keep[r];
+  const uint32_t k0 = (uint32_t)__builtin_ctz(use);
+  double mx = a.out[s0 + k0].log_lk;
+  uint32_t bi = k0;
   for (uint32_t k = 0; k < nj; k++) {  // select_best: last maximum
+    if (!((use >> k) & 1u)) continue;
     double v = a.out[s0 + k].log_lk;
     if (v >= mx) {
       mx = v;
@@ -2193,7 +2333,8 @@ __global__ void k_mapq(MapqArgs a) {
     }
   }
   double se = -0.0;
-  for (uint32_t k = 0; k < nj; k++) se += exp(a.out[s0 + k].log_lk - mx);
+  for (uint32_t k = 0; k < nj; k++)
+    if ((use >> k) & 1u) se += exp(a.out[s0 + k].log_lk - mx);
   const DpJob job = a.jobs[s0 + bi];
   const DpOut dout = a.out[s0 + bi];
   QueryView Q{a.R.seq + so, a.R.qual + qo, L, qlen, job.rc != 0};
@@ -2217,8 +2358,125 @@ __global__ void k_mapq(MapqArgs a) {
 }
 
 void launch_mapq(const MapqArgs& a, hipStream_t s) {
+  const uint32_t n = a.rlist ? a.nlist : a.R.n;
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_mapq, dim3((n + 127) / 128), dim3(128), 0, s, a);
+}
+
+// ---------------------------------------------------------------------------
+// Candidate prefilter in the mapping path (no counterpart in the reference).
+// It sits between cluster_hits (single.rs:91) and the DP loop (:114-150).
+// ---------------------------------------------------------------------------
+// Per read: keep the job slots whose candidate ranks < top_k or scores
+// >= threshold; list them for the DP.
+__global__ void k_pf_select(PfArgs a) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= a.R.n) return;
+  const uint32_t nj = a.njobs[r];
+  const uint32_t m = (uint32_t)(a.R.seq_off[r + 1] - a.R.seq_off[r]);
+  const uint64_t s0 = (uint64_t)r * TOP_CANDIDATES;
+  uint32_t mask = 0, nk = 0;
+  unsigned long long skipped = 0;
+  for (uint32_t k = 0; k < nj; k++) {
+    const DpJob job = a.jobs[s0 + k];
+    const uint32_t rank = (uint32_t)(job.cand - s0);
+    if (rank < a.top_k || a.logit[job.cand] >= a.threshold) {
+      mask |= 1u << k;
+      nk++;
+    } else if (m >= 1 && job.n >= 1 && (int64_t)job.hint + job.band >= 0 &&
+               (int64_t)m + job.hint - job.band <= (int64_t)job.n) {
+      skipped += quad_cells(m, job.hint, job.band, job.n);
+    }
+  }
+  a.keep[r] = mask;
+  if (nk) {
+    uint32_t pos = atomicAdd(a.nlist, nk);
+    for (uint32_t k = 0; k < nj; k++)
+      if ((mask >> k) & 1u) a.list[pos++] = (uint32_t)(s0 + k);
+  }
+  if (nj) atomicAdd(&a.ctr[0], (unsigned long long)nj);
+  if (nk) atomicAdd(&a.ctr[1], (unsigned long long)nk);
+  if (skipped) atomicAdd(&a.ctr[2], skipped);
+}
+
+// Per read with skipped slots: re-run it plain when its prefiltered result is
+// ambiguous (MAPQ below mapq_min, or a kept second candidate within
+// close_nats of the best), or when it falls on the audit sample.
+__global__ void k_pf_verdict(PfArgs a) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= a.R.n) return;
+  const uint32_t nj = a.njobs[r];
+  const uint32_t full = nj >= 32 ? 0xFFFFFFFFu : ((1u << nj) - 1u);
+  const uint32_t mask = a.keep[r] & full;
+  a.verdict[r] = 0;
+  if (mask == full) return;
+  const uint64_t s0 = (uint64_t)r * TOP_CANDIDATES;
+  double mx = -HUGE_VAL, sec = -HUGE_VAL;
+  uint32_t bi = 0;
+  for (uint32_t k = 0; k < nj; k++) {
+    if (!((mask >> k) & 1u)) continue;
+    const double v = a.out[s0 + k].log_lk;
+    if (v >= mx) {
+      if (mx != -HUGE_VAL) sec = mx > sec ? mx : sec;
+      mx = v;
+      bi = k;
+    } else if (v > sec) {
+      sec = v;
+    }
+  }
+  (void)bi;
+  const uint8_t mapq = ((const RecOut*)a.rec)[r].mapq;
+  const bool fallback = mapq < a.mapq_min || sec >= mx - (double)a.close_nats;
+  const bool audit = !fallback && a.audit_period && ((a.read_base + r) % a.audit_period) == 0;
+  if (!fallback && !audit) return;
+  a.verdict[r] = fallback ? 1 : 2;
+  const uint32_t p = atomicAdd(a.nrlist, 1u);
+  a.rlist[p] = r;
+  const uint32_t ns = (uint32_t)__builtin_popcount(full & ~mask);
+  uint32_t pos = atomicAdd(a.nlist, ns);
+  for (uint32_t k = 0; k < nj; k++)
+    if (!((mask >> k) & 1u)) a.list[pos++] = (uint32_t)(s0 + k);
+  atomicAdd(&a.ctr[3], (unsigned long long)ns);
+  if (fallback) atomicAdd(&a.audit[2], 1ull);
+}
+
+// Fallback reads take the plain record; audited reads are compared field by
+// field (POS, RNAME, FLAG, MAPQ, strand and the CIGAR words).
+__global__ void k_pf_merge(PfArgs a, uint32_t n) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint32_t r = a.rlist[t];
+  const RecOut* p2 = (const RecOut*)(a.rec2) + r;
+  RecOut* p1 = (RecOut*)(a.rec_out) + r;
+  if (a.verdict[r] == 1) {
+    *p1 = *p2;
+    a.best[r] = a.best2[r];
+    return;
+  }
+  const RecOut x = *p1, y = *p2;
+  bool same = x.pos == y.pos && x.contig == y.contig && x.flag == y.flag && x.mapq == y.mapq && x.strand == y.strand &&
+              x.cigar_n == y.cigar_n;
+  const uint32_t b1 = a.best[r], b2 = a.best2[r];
+  if (same && b1 != b2 && b1 != 0xFFFFFFFFu && b2 != 0xFFFFFFFFu) {
+    const uint32_t* c1 = a.cigar + (uint64_t)b1 * a.cigar_cap;
+    const uint32_t* c2 = a.cigar + (uint64_t)b2 * a.cigar_cap;
+    for (uint32_t u = 0; u < x.cigar_n && u < a.cigar_cap; u++) same &= c1[u] == c2[u];
+  }
+  atomicAdd(&a.audit[0], 1ull);
+  if (!same) atomicAdd(&a.audit[1], 1ull);
+}
+
+void launch_pf_select(const PfArgs& a, hipStream_t s) {
   if (a.R.n == 0) return;
-  hipLaunchKernelGGL(k_mapq, dim3((a.R.n + 127) / 128), dim3(128), 0, s, a);
+  hipLaunchKernelGGL(k_pf_select, dim3((a.R.n + 127) / 128), dim3(128), 0, s, a);
+}
+void launch_pf_verdict(const PfArgs& a, hipStream_t s) {
+  if (a.R.n == 0) return;
+  hipLaunchKernelGGL(k_pf_verdict, dim3((a.R.n + 127) / 128), dim3(128), 0, s, a);
+}
+void launch_pf_merge(const PfArgs& a, uint32_t n, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_pf_merge, dim3((n + 127) / 128), dim3(128), 0, s, a, n);
 }
 
 // Winning CIGARs into one compact array (one wave per record).
