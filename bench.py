@@ -1,22 +1,31 @@
-"""Benchmark: single-end mapping throughput of the HIP path on one MI355X.
+"""Benchmark: single-end mapping throughput of the HIP path on one MI355X, plain and
+with the learned candidate prefilter.
 
-Workload (uneven_se_100mbp, probebwa_amd/synth.py): a 100 Mbp synthetic genome
-over 10 contigs (uniform background, 5% segmental duplications at 0.5-3%
-divergence, 5-15 bp homopolymer runs every ~2 kb) and 2,000,000 single-end
-reads with lengths U{80..150}, Phred qualities decaying 38 -> 22 along the
-read (+ N(0,3) noise), substitutions at the Phred-implied rate.
+Workloads (probebwa_amd/synth.py):
+- uneven_se_100mbp (default): a 100 Mbp synthetic genome over 10 contigs
+  (uniform background, 5% segmental duplications at 0.5-3% divergence, 5-15 bp
+  homopolymer runs every ~2 kb) and 2,000,000 single-end reads with lengths
+  U{80..150}, Phred qualities decaying 38 -> 22 along the read (+ N(0,3) noise),
+  substitutions at the Phred-implied rate.  The prefilter trains on 131,072
+  other reads of the same generator.
+- prefilter_se_2mbp: the 2.1 Mbp probe genome (2 Mbp uniform + 50 copies of a
+  2 kb unit at 1% divergence); the prefilter trains on 500,000 reads and is
+  evaluated on 2,000,000 held-out reads (L ~ U{80..150}, 1% substitutions,
+  0.05% indels, Q ~ U{20..40}).
 
 A step maps one batch of `--batch` reads that are already resident in HBM
 (`pbw_reads_upload` once, then `pbw_map_se_device` per step; calls return
-after the records are in host memory): seed -> cluster -> banded DP -> MAPQ, records back
-in host memory.  `value` = reads mapped per second over the K timed steps.
+after the records are in host memory): seed -> cluster -> banded DP -> MAPQ.
+`value` = reads mapped per second over the K timed steps of the plain path.
+The same K steps are then timed with the trained prefilter on (`learned`),
+and every record of them is compared with the plain path's.
 
 Also reported: the dominant kernel's roofline position (HIP-event timed inside
 the library on its own stream), and the CPU oracle's reads/s on a bounded
 sample of the same reads on the host cores, with the sample's SAM-field
 concordance between the HIP path and the oracle.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
 import argparse
 import json
@@ -55,17 +64,23 @@ def measured_traffic(kernel, batch):
     return None
 
 
-def sam_identical(a, b):
-    """Per record: POS, RNAME, FLAG, MAPQ, strand and CIGAR identical (records in the same order)."""
-    ra, rb = a.rec, b.rec
-    same = ((ra["pos"] == rb["pos"]) & (ra["contig"] == rb["contig"]) & (ra["flag"] == rb["flag"])
-            & (ra["mapq"] == rb["mapq"]) & (ra["strand"] == rb["strand"]) & (ra["cigar_n"] == rb["cigar_n"]))
-    if same.all() and len(a.cigar) == len(b.cigar):
-        eq = np.asarray(a.cigar) == np.asarray(b.cigar)
-        starts = np.asarray(ra["cigar_off"], dtype=np.int64)
-        ok = np.logical_and.reduceat(eq, starts) if len(eq) else np.ones(len(ra), bool)
-        return same & ok
-    return same & np.array([a.cigar_words(i) == b.cigar_words(i) for i in range(len(ra))])
+WORKLOADS = {
+    "uneven_se_100mbp": {"reads": 2_000_000, "train_reads": 131072,
+                         "desc": "single-end, 100 Mbp / 10 contigs, 2M reads of 80-150 bp"},
+    "prefilter_se_2mbp": {"reads": 2_000_000, "train_reads": 500_000,
+                          "desc": "single-end, 2.1 Mbp probe genome, 2M held-out reads of 80-150 bp"},
+}
+
+
+def make_workload(name, n, n_train):
+    from probebwa_amd import synth
+    if name == "uneven_se_100mbp":
+        contigs = synth.genome_uneven_100mbp()
+        return (contigs, lambda: synth.reads_uneven_se(contigs, n=n, seed=12),
+                lambda: synth.reads_uneven_se(contigs, n=n_train, seed=99), "seeds 2/12/99")
+    contigs = synth.genome_probe_se_2mbp()
+    return (contigs, lambda: synth.reads_probe_uneven_se(contigs, n=n, seed=42),
+            lambda: synth.reads_probe_uneven_se(contigs, n=n_train, seed=41), "seeds 1/42/41")
 
 
 def main():
@@ -74,12 +89,15 @@ def main():
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=262144)
-    ap.add_argument("--reads", type=int, default=2_000_000)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="uneven_se_100mbp")
+    ap.add_argument("--reads", type=int, default=None)
     ap.add_argument("--cpu-sample", type=int, default=131072)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
-    ap.add_argument("--train-reads", type=int, default=131072)
+    ap.add_argument("--train-reads", type=int, default=None)
+    ap.add_argument("--label", choices=["loglk", "score"], default="loglk")
+    ap.add_argument("--margin", type=float, default=40.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
@@ -87,23 +105,25 @@ def main():
         sys.exit(2)
 
     import probebwa_amd as P
-    from probebwa_amd import synth
 
+    wl = WORKLOADS[args.workload]
+    n_reads = args.reads or wl["reads"]
+    n_train = args.train_reads or wl["train_reads"]
     t0 = time.time()
-    contigs = synth.genome_uneven_100mbp()
+    contigs, gen_reads, gen_train, seeds = make_workload(args.workload, n_reads, n_train)
     gp = P.Genome()
     for name, s in contigs:
         gp.add_contig(name, s.tobytes())
     log("genome %.1f s" % (time.time() - t0))
     t0 = time.time()
-    seq, so, qual, qo = synth.reads_uneven_se(contigs, n=args.reads)
-    log("reads %.1f s (%d reads, %d bases)" % (time.time() - t0, args.reads, int(so[-1])))
+    seq, so, qual, qo = gen_reads()
+    log("reads %.1f s (%d reads, %d bases)" % (time.time() - t0, n_reads, int(so[-1])))
     t0 = time.time()
     ctx = P.Context(gp, batch_reads=args.batch, device=0)
     log("context (index build + upload) %.1f s, %d index entries" % (time.time() - t0, ctx.total_entries))
 
     dreads = ctx.upload_reads(P.ReadBatch.from_arrays(seq, so, qual, qo))
-    n = args.reads
+    n = n_reads
     B = min(args.batch, n)
 
     def step(s):
@@ -115,12 +135,11 @@ def main():
         step(w)
         log("warmup step %d done" % w)
     agg = {}
-    first = None
+    plain_recs = []
     t0 = time.perf_counter()
     for k in range(args.steps):
         recs, st = step(k)
-        if k == 0:
-            first = recs
+        plain_recs.append(recs)
         for key, v in st.items():
             agg[key] = agg.get(key, 0) + v
         log("step %d: %d reads in %.1f ms" % (k, st["reads"], st["ms_total"]))
@@ -154,33 +173,37 @@ def main():
         # learned candidate prefilter: trained on a separate read set of the same genome (labels from
         # the plain DP over every candidate), then the same steps timed with it on
         t1 = time.time()
-        tseq, tso, tqual, tqo = synth.reads_uneven_se(contigs, n=args.train_reads, seed=99)
-        rep = ctx.prefilter_train(P.ReadBatch.from_arrays(tseq, tso, tqual, tqo), label_nats=40.0, recall=0.9999,
-                                  epochs=20, batch=4096)
+        tseq, tso, tqual, tqo = gen_train()
+        rep = ctx.prefilter_train(P.ReadBatch.from_arrays(tseq, tso, tqual, tqo), label=args.label, margin=args.margin,
+                                  recall=0.9999, epochs=20, batch=4096, lr=1e-3)
+        del tseq, tqual
         log("prefilter trained in %.1f s on %d rows (%d positive), held-out skip %.3f recall %.5f"
             % (time.time() - t1, rep["rows"], rep["positives"], rep["val_skip"], rep["val_recall"]))
         ctx.set_prefilter(enabled=1, audit_period=64)
         for w in range(args.warmup):
             step(w)
         agg2 = {}
-        first2 = None
+        learned_recs = []
         t1 = time.perf_counter()
         for k in range(args.steps):
             recs, st = step(k)
-            if k == 0:
-                first2 = recs
+            learned_recs.append(recs)
             for key, v in st.items():
                 agg2[key] = agg2.get(key, 0) + v
             log("learned step %d: %d reads in %.1f ms" % (k, st["reads"], st["ms_total"]))
         dt2 = time.perf_counter() - t1
         ctx.set_prefilter(enabled=0)
-        same = sam_identical(first, first2)
+        same = np.concatenate([P.primary_identical(a, b) for a, b in zip(plain_recs, learned_recs)])
+        del learned_recs
+        label = ("log-likelihood within %g nats of the read's best" if args.label == "loglk"
+                 else "DP score within %g units of the read's best") % args.margin
         learned = {
             "reads_per_s": round(int(agg2["reads"]) / dt2, 1),
             "ms_per_step": round(dt2 * 1000.0 / args.steps, 3),
             "speedup_vs_plain": round((int(agg2["reads"]) / dt2) / value, 3),
-            "sam_identical_first_step": round(float(same.mean()), 6),
-            "sam_differing_first_step": int((~same).sum()),
+            "sam_identical": round(float(same.mean()), 6),
+            "sam_differing_reads": int((~same).sum()),
+            "reads_compared": int(len(same)),
             "audit_reads": int(agg2["pf_audit_reads"]),
             "audit_mismatches": int(agg2["pf_audit_mismatches"]),
             "fallback_reads": int(agg2["pf_fallback_reads"]),
@@ -190,11 +213,11 @@ def main():
             "dp_cells_skipped_frac": round(1.0 - agg2["dp_cells"] / max(1, agg["dp_cells"]), 5),
             "stage_ms": {k: round(agg2[k], 1) for k in ("ms_seed", "ms_cluster", "ms_pf", "ms_dp", "ms_mapq",
                                                          "ms_total")},
-            "training": {"reads": args.train_reads, "rows": int(rep["rows"]), "positives": int(rep["positives"]),
-                         "label": "log-likelihood within 40 nats of the read's best", "epochs": 20,
-                         "batch": 4096, "lr": 1e-3, "threshold": round(float(rep["threshold"]), 5),
-                         "heldout_recall": round(rep["val_recall"], 6), "heldout_skip": round(rep["val_skip"], 5),
-                         "loss_first_epoch": round(rep["loss_first"], 6), "loss_last_epoch": round(rep["loss_last"], 6)},
+            "training": {"reads": n_train, "rows": int(rep["rows"]), "positives": int(rep["positives"]),
+                         "label": label, "epochs": 20, "batch": 4096, "lr": 1e-3,
+                         "threshold": round(float(rep["threshold"]), 5), "heldout_recall": round(rep["val_recall"], 6),
+                         "heldout_skip": round(rep["val_skip"], 5), "loss_first_epoch": round(rep["loss_first"], 6),
+                         "loss_last_epoch": round(rep["loss_last"], 6)},
         }
 
     cpu = None
@@ -217,11 +240,8 @@ def main():
         cpu = {"value": round(S / cdt, 1), "unit": "reads/s", "cores": args.cpu_threads, "kind": "port",
                "sample": "first %d reads of the workload, oracle (C++ restatement) on %d threads, %.1f s"
                          % (S, args.cpu_threads, cdt)}
-        ra, rg = ro.rec, first.rec[:S]
-        same = ((ra["pos"] == rg["pos"]) & (ra["contig"] == rg["contig"]) & (ra["flag"] == rg["flag"])
-                & (ra["mapq"] == rg["mapq"]))
-        same &= np.array([ro.cigar_words(i) == first.cigar_words(i) for i in range(S)])
-        concord = float(same.mean())
+        first = plain_recs[0]
+        concord = float(P.primary_identical(ro, P.Records(first.rec[:S], first.cigar)).mean())
 
     out = {
         "metric": "reads/s on 1 GPU, learned vs plain @≥99.9% SAM concordance; DP GCUPS",
@@ -235,10 +255,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32+f64",
-        "data": "synthetic (uneven_se_100mbp generator, seeds 2/12)",
-        "config": {"workload": "uneven_se_100mbp: plain GPU path, single-end, 100 Mbp / 10 contigs, "
-                               "2M reads of 80-150 bp", "batch_reads": B, "reads_per_step": B,
-                   "path": "plain (no learned prefilter)"},
+        "data": "synthetic (%s generator, %s)" % (args.workload, seeds),
+        "config": {"workload": "%s: plain GPU path, %s" % (args.workload, wl["desc"]), "batch_reads": B,
+                   "reads_per_step": B, "path": "plain (no learned prefilter)"},
         "dp_gcups": round(dp_gcups, 2),
         "gather_gbps": round(gather_gbps, 2),
         "roofline": roof,

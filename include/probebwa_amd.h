@@ -208,19 +208,25 @@ typedef struct {
   int32_t enabled;
   float threshold;
   uint32_t top_k;          /* 1 */
-  uint32_t mapq_fallback;  /* 30 */
-  float close_nats;        /* 40 */
+  uint32_t mapq_fallback;  /* 20 */
+  float close_nats;        /* 10 */
   uint32_t audit_period;   /* 64 */
 } pbw_prefilter_options;
 void pbw_default_prefilter_options(pbw_prefilter_options* o);
 int pbw_prefilter_set(pbw_ctx* ctx, const pbw_prefilter_options* o);
 int pbw_prefilter_get(const pbw_ctx* ctx, pbw_prefilter_options* o);
 
+/* Training label of a candidate, from the plain DP over every candidate of its read:
+ * PBW_LABEL_LOGLK  1 when its log-likelihood is within `margin` nats of the read's best;
+ * PBW_LABEL_SCORE  1 when its DP score is within `margin` score units of the read's best. */
+#define PBW_LABEL_LOGLK 0
+#define PBW_LABEL_SCORE 1
+
 /* Training rows from plain mapping of `reads`: per aligned candidate its 32 features
- * (BF16), label 1 when its log-likelihood is within label_nats of the read's best, and
- * its rank.  Returns the row count (rows beyond cap are counted, not written). */
-int pbw_prefilter_dataset(pbw_ctx* ctx, const pbw_read_batch* reads, float label_nats, uint16_t* x, float* y,
-                          uint8_t* rank, uint64_t cap, uint64_t* n);
+ * (BF16), its label and its rank.  Returns the row count (rows beyond cap are counted,
+ * not written). */
+int pbw_prefilter_dataset(pbw_ctx* ctx, const pbw_read_batch* reads, int label_kind, float margin, uint16_t* x,
+                          float* y, uint8_t* rank, uint64_t cap, uint64_t* n);
 typedef struct {
   uint64_t rows, positives, val_rows, val_positives;
   float threshold;   /* calibrated and stored in the context's prefilter options */
@@ -230,8 +236,8 @@ typedef struct {
 } pbw_prefilter_report;
 /* Trains the network on rows of `reads` (the last 10% held out) and calibrates the
  * threshold so that `recall` of the held-out positives ranked >= top_k score above it. */
-int pbw_prefilter_train(pbw_ctx* ctx, const pbw_read_batch* reads, const pbw_train_options* o, float label_nats,
-                        float recall, pbw_prefilter_report* out);
+int pbw_prefilter_train(pbw_ctx* ctx, const pbw_read_batch* reads, const pbw_train_options* o, int label_kind,
+                        float margin, float recall, pbw_prefilter_report* out);
 int pbw_mlp_init(pbw_ctx* ctx, uint64_t seed);
 int pbw_mlp_set_params(pbw_ctx* ctx, const float* theta, uint64_t n);
 int pbw_mlp_get_params(pbw_ctx* ctx, float* theta, uint64_t n);

@@ -82,6 +82,10 @@ class PrefilterOptions(C.Structure):
             setattr(self, k, v)
 
 
+# training label rules (PBW_LABEL_LOGLK, PBW_LABEL_SCORE)
+LABEL_KINDS = {"loglk": 0, "score": 1}
+
+
 class PrefilterReport(C.Structure):
     _fields_ = [("rows", C.c_uint64), ("positives", C.c_uint64), ("val_rows", C.c_uint64),
                 ("val_positives", C.c_uint64), ("threshold", C.c_float), ("val_recall", C.c_double),
@@ -214,8 +218,9 @@ def lib():
             "pbw_default_prefilter_options": (None, [vp]),
             "pbw_prefilter_set": (i32, [vp, vp]),
             "pbw_prefilter_get": (i32, [vp, vp]),
-            "pbw_prefilter_dataset": (i32, [vp, C.POINTER(_ReadBatch), C.c_float, vp, vp, vp, u64, C.POINTER(u64)]),
-            "pbw_prefilter_train": (i32, [vp, C.POINTER(_ReadBatch), vp, C.c_float, C.c_float, vp]),
+            "pbw_prefilter_dataset": (i32, [vp, C.POINTER(_ReadBatch), i32, C.c_float, vp, vp, vp, u64,
+                                            C.POINTER(u64)]),
+            "pbw_prefilter_train": (i32, [vp, C.POINTER(_ReadBatch), vp, i32, C.c_float, C.c_float, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -330,13 +335,32 @@ class Records:
         return "".join("%d%s" % (w >> 4, CIGAR_CHARS[w & 15]) for w in self.cigar_words(i))
 
 
+class _RecordMemory:
+    """Owns the arrays of one pbw_record_batch; frees them when no view is left."""
+
+    def __init__(self, rb):
+        self.rb = rb
+
+    def __del__(self):
+        rb, self.rb = self.rb, None
+        if rb is not None:
+            try:
+                lib().pbw_records_release(C.byref(rb))
+            except Exception:  # interpreter shutdown
+                pass
+
+
+def _view(keeper, addr, nbytes, dtype):
+    buf = (C.c_char * nbytes).from_address(addr)
+    buf._keeper = keeper  # the numpy view holds buf, buf holds the owner
+    return np.frombuffer(buf, dtype=dtype)
+
+
 def _take(rb):
-    n = rb.n
-    rec = np.frombuffer((C.c_char * (n * 64)).from_address(rb.rec), dtype=RECORD_DTYPE).copy() if n else \
-        np.zeros(0, RECORD_DTYPE)
-    cig = np.frombuffer((C.c_char * (rb.n_cigar * 4)).from_address(rb.cigar), dtype=np.uint32).copy() \
-        if rb.n_cigar else np.zeros(0, np.uint32)
-    lib().pbw_records_release(C.byref(rb))
+    """Records of a pbw_record_batch as numpy views of the library's arrays (no copy)."""
+    keeper = _RecordMemory(rb)
+    rec = _view(keeper, rb.rec, rb.n * 64, RECORD_DTYPE) if rb.n else np.zeros(0, RECORD_DTYPE)
+    cig = _view(keeper, rb.cigar, rb.n_cigar * 4, np.uint32) if rb.n_cigar else np.zeros(0, np.uint32)
     return Records(rec, cig)
 
 
@@ -473,24 +497,30 @@ class Context:
         self.prefilter = o
         return o
 
-    def prefilter_dataset(self, reads, label_nats=40.0):
-        """(x_bf16 [n, 32], y [n], rank [n]) training rows from plain mapping of `reads`."""
+    def prefilter_dataset(self, reads, label="loglk", margin=40.0):
+        """(x_bf16 [n, 32], y [n], rank [n]) training rows from plain mapping of `reads`.
+        label "loglk": y = 1 when the candidate's log-likelihood is within `margin` nats of the read's
+        best; "score": when its DP score is within `margin` score units."""
         rs = reads.cstruct()
+        kind = LABEL_KINDS[label]
         n = C.c_uint64()
-        _check(lib().pbw_prefilter_dataset(self.h, C.byref(rs), label_nats, None, None, None, 0, C.byref(n)), self.h)
+        _check(lib().pbw_prefilter_dataset(self.h, C.byref(rs), kind, margin, None, None, None, 0, C.byref(n)),
+               self.h)
         x = np.zeros((max(n.value, 1), MLP_INPUTS), np.uint16)
         y = np.zeros(max(n.value, 1), np.float32)
         r = np.zeros(max(n.value, 1), np.uint8)
-        _check(lib().pbw_prefilter_dataset(self.h, C.byref(rs), label_nats, x.ctypes.data, y.ctypes.data,
+        _check(lib().pbw_prefilter_dataset(self.h, C.byref(rs), kind, margin, x.ctypes.data, y.ctypes.data,
                                            r.ctypes.data, n.value, C.byref(n)), self.h)
         return x[:n.value], y[:n.value], r[:n.value]
 
-    def prefilter_train(self, reads, label_nats=40.0, recall=0.9999, **kw):
-        """Trains the network on `reads` (plain-DP labels) and calibrates the threshold."""
+    def prefilter_train(self, reads, label="loglk", margin=40.0, recall=0.9999, **kw):
+        """Trains the network on `reads` (plain-DP labels, as in prefilter_dataset) and calibrates
+        the threshold; `kw` are TrainOptions fields."""
         rs = reads.cstruct()
         o = TrainOptions(**kw)
         rep = PrefilterReport()
-        _check(lib().pbw_prefilter_train(self.h, C.byref(rs), C.byref(o), label_nats, recall, C.byref(rep)), self.h)
+        _check(lib().pbw_prefilter_train(self.h, C.byref(rs), C.byref(o), LABEL_KINDS[label], margin, recall,
+                                         C.byref(rep)), self.h)
         return rep.as_dict()
 
     def mlp_train(self, x_bf16, y, **kw):
@@ -518,6 +548,26 @@ class DeviceReads:
         if getattr(self, "h", None):
             lib().pbw_reads_free(self.h)
             self.h = None
+
+
+def primary_identical(a, b):
+    """Per record of two record sets for the same reads (same order): True where the SAM
+    fields of the primary alignment agree (RNAME, POS, FLAG, MAPQ, strand and every CIGAR run)."""
+    ra, rb = a.rec, b.rec
+    if len(ra) != len(rb):
+        raise ValueError("record sets differ in length")
+    same = ((ra["pos"] == rb["pos"]) & (ra["contig"] == rb["contig"]) & (ra["flag"] == rb["flag"])
+            & (ra["mapq"] == rb["mapq"]) & (ra["strand"] == rb["strand"]) & (ra["cigar_n"] == rb["cigar_n"]))
+    n = np.where(same, ra["cigar_n"], 0).astype(np.int64)
+    tot = int(n.sum())
+    if tot:
+        rid = np.repeat(np.arange(len(ra)), n)
+        first = np.cumsum(n) - n
+        k = np.arange(tot) - np.repeat(first, n)
+        ca, cb = np.asarray(a.cigar), np.asarray(b.cigar)
+        diff = ca[ra["cigar_off"].astype(np.int64)[rid] + k] != cb[rb["cigar_off"].astype(np.int64)[rid] + k]
+        same[rid[diff]] = False
+    return same
 
 
 def format_sam(genome, reads, records, command_line="probebwa", read_group=None):

@@ -277,7 +277,8 @@ struct pbw_ctx {
   DevBuf<DpOut> d_wout;
   DevBuf<uint8_t> d_wvalid, d_slcand, d_has_obs;
   DevBuf<uint32_t> d_plan, d_wcigar, d_name_class;
-  DevBuf<uint64_t> d_cig_src;
+  DevBuf<uint64_t> d_cig_src, d_rec_src;
+  DevBuf<uint32_t> d_rec_bsum;
   DevBuf<double> d_obs;
   // candidate prefilter network
   DevBuf<float> d_mlp_f;     // theta | m | v | b1 | b2 | b3
@@ -429,9 +430,44 @@ static void ctx_build(pbw_ctx* c) {
 // ---------------------------------------------------------------------------
 // single-end batch: seed -> cluster -> jobs -> DP -> MAPQ
 // ---------------------------------------------------------------------------
+// A malloc'd array that an output batch takes over (pbw_records_release frees it).
+template <typename T>
+struct HostArr {
+  T* p = nullptr;
+  size_t n = 0, cap = 0;
+  HostArr() = default;
+  HostArr(const HostArr&) = delete;
+  HostArr& operator=(const HostArr&) = delete;
+  ~HostArr() { free(p); }
+  void reserve(size_t want) {
+    if (want <= cap) return;
+    T* q = (T*)realloc(p, want * sizeof(T));
+    if (!q) throw PbwError{PBW_ERR_LIMIT, "host allocation of " + std::to_string(want * sizeof(T)) + " bytes failed"};
+    p = q;
+    cap = want;
+  }
+  T* grow(size_t add) {  // appends `add` uninitialised elements
+    if (n + add > cap) reserve(std::max(n + add, cap * 2));
+    T* at = p + n;
+    n += add;
+    return at;
+  }
+  void push_back(const T& v) { *grow(1) = v; }
+  size_t size() const { return n; }
+  bool empty() const { return n == 0; }
+  T& operator[](size_t i) { return p[i]; }
+  T* release() {
+    reserve(1);  // never hand out a null array
+    T* q = p;
+    p = nullptr;
+    n = cap = 0;
+    return q;
+  }
+};
+
 struct HostRecords {
-  std::vector<pbw_record> rec;
-  std::vector<uint32_t> cigar;
+  HostArr<pbw_record> rec;
+  HostArr<uint32_t> cigar;
 };
 
 static double ms_since(std::chrono::steady_clock::time_point t0) {
@@ -776,50 +812,40 @@ static void batch_dp(pbw_ctx* c, Batch& b, const uint32_t* jlist, uint32_t nlist
 
 // Downloads records (already in d_rec) and gathers their CIGARs from one or
 // two pools (src bit 63 selects the second); unmapped records get {L}S.
-static void download_records(pbw_ctx* c, const Batch& b, uint32_t nrec, const std::vector<uint64_t>& src,
-                             const uint32_t* pool1, HostRecords& hr) {
+// Records [0, nrec) of a batch (in d_rec) and their CIGAR runs to host memory.
+// d_src[i]: first word of record i's runs in its pool (bit 63: pool1), ~0 when
+// unmapped.  Word counts and offsets are scanned on the device; the host reads
+// the batch's word count and copies records and runs straight into the arrays
+// the output batch takes over.
+static void download_records(pbw_ctx* c, const Batch& b, uint32_t nrec, const uint64_t* d_src, const uint32_t* pool1,
+                             HostRecords& hr) {
   hipStream_t s = c->stream;
-  size_t r0 = hr.rec.size();
-  hr.rec.resize(r0 + nrec);
-  HIP_CHECK(hipMemcpyAsync(hr.rec.data() + r0, c->d_rec.p, (size_t)nrec * sizeof(pbw_record), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
-  std::vector<uint64_t> gsrc;
-  std::vector<uint32_t> cnt, offs;
-  uint64_t total = 0;
-  for (uint32_t i = 0; i < nrec; i++) {
-    if (src[i] == ~0ull) continue;
-    gsrc.push_back(src[i]);
-    cnt.push_back(hr.rec[r0 + i].cigar_n);
-    offs.push_back((uint32_t)total);
-    total += hr.rec[r0 + i].cigar_n;
-  }
-  std::vector<uint32_t> words(total);
-  if (total) {
-    DevBuf<uint64_t> dsrc;
-    DevBuf<uint32_t> dcnt;
-    dsrc.upload(gsrc.data(), gsrc.size(), s);
-    dcnt.upload(cnt.data(), cnt.size(), s);
-    c->d_cigar_off.upload(offs.data(), offs.size(), s);
-    c->d_cigar_out.ensure(total);
-    gather_cigar2(c->d_cigar.p, pool1 ? pool1 : c->d_cigar.p, dsrc.p, c->d_cigar_off.p, dcnt.p, (uint32_t)gsrc.size(),
-                  c->d_cigar_out.p, s);
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipMemcpyAsync(words.data(), c->d_cigar_out.p, total * 4, hipMemcpyDeviceToHost, s));
+  if (!nrec) {
     HIP_CHECK(hipStreamSynchronize(s));
+    return;
   }
-  uint64_t w = 0;
-  for (uint32_t i = 0; i < nrec; i++) {
-    pbw_record& rr = hr.rec[r0 + i];
-    rr.cigar_off = (uint32_t)hr.cigar.size();
-    if (src[i] == ~0ull) {
-      uint64_t L = b.so[i + 1] - b.so[i];
-      hr.cigar.push_back((uint32_t)(L << 4) | 4u);  // SoftClip(L), single.rs:368
-      rr.cigar_n = 1;
-    } else {
-      hr.cigar.insert(hr.cigar.end(), words.begin() + w, words.begin() + w + rr.cigar_n);
-      w += rr.cigar_n;
-    }
-  }
+  if (hr.cigar.size() > 0xFFFFFFFFull)
+    throw PbwError{PBW_ERR_LIMIT, "more than 2^32 CIGAR words in one call; map fewer reads per call"};
+  const uint32_t base = (uint32_t)hr.cigar.size();
+  const uint32_t nb = (nrec + REC_BLOCK - 1) / REC_BLOCK;
+  c->d_rec_bsum.ensure(nb + 1);
+  uint32_t* d_total = c->d_rec_bsum.p + nb;
+  launch_rec_offsets(c->d_rec.p, d_src, nrec, base, c->d_rec_bsum.p, d_total, s);
+  HIP_CHECK(hipGetLastError());
+  pbw_record* hrec = hr.rec.grow(nrec);
+  uint32_t total = 0;
+  HIP_CHECK(hipMemcpyAsync(hrec, c->d_rec.p, (size_t)nrec * sizeof(pbw_record), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(&total, d_total, 4, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  if ((uint64_t)base + total > 0xFFFFFFFFull)
+    throw PbwError{PBW_ERR_LIMIT, "more than 2^32 CIGAR words in one call; map fewer reads per call"};
+  c->d_cigar_out.ensure(std::max<uint32_t>(total, 1));
+  launch_rec_cigar(c->d_rec.p, d_src, c->d_cigar.p, pool1 ? pool1 : c->d_cigar.p, b.R.seq_off, nrec, base,
+                   c->d_cigar_out.p, s);
+  HIP_CHECK(hipGetLastError());
+  uint32_t* hc = hr.cigar.grow(total);
+  HIP_CHECK(hipMemcpyAsync(hc, c->d_cigar_out.p, (size_t)total * 4, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
 }
 
 static void collect_counters(pbw_ctx* c, uint32_t reads) {
@@ -971,12 +997,9 @@ static void map_se_batch(pbw_ctx* c, const pbw_read_batch* in, uint64_t b0, uint
     c->stats.ms_mapq += ms_since(t0);
   }
   auto t0 = std::chrono::steady_clock::now();
-  std::vector<uint32_t> best(B);
-  HIP_CHECK(hipMemcpyAsync(best.data(), c->d_best.p, (size_t)B * 4, hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
-  std::vector<uint64_t> src(B);
-  for (uint32_t i = 0; i < B; i++) src[i] = best[i] == 0xFFFFFFFFu ? ~0ull : (uint64_t)best[i] * b.cap;
-  download_records(c, b, B, src, nullptr, hr);
+  c->d_rec_src.ensure(B);
+  launch_rec_src_se(c->d_best.p, B, b.cap, c->d_rec_src.p, s);
+  download_records(c, b, B, c->d_rec_src.p, nullptr, hr);
   c->stats.ms_mapq += ms_since(t0);
   collect_counters(c, B);
 }
@@ -1104,16 +1127,11 @@ static void map_pe_batch(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_ba
   pa.has_obs = c->d_has_obs.p;
   launch_pair_finalize(pa, s);
   HIP_CHECK(hipGetLastError());
-  std::vector<uint64_t> src(2 * (size_t)P);
   obs_out.resize(P);
   has_out.resize(P);
-  HIP_CHECK(hipMemcpyAsync(src.data(), c->d_cig_src.p, 2 * (size_t)P * 8, hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipMemcpyAsync(obs_out.data(), c->d_obs.p, (size_t)P * 8, hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipMemcpyAsync(has_out.data(), c->d_has_obs.p, P, hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
-  for (auto& x : src)
-    if (x != ~0ull && (x >> 63)) x = (1ull << 63) | (x & ~(1ull << 63));
-  download_records(c, b, 2 * P, src, c->d_wcigar.p, hr);
+  download_records(c, b, 2 * P, c->d_cig_src.p, c->d_wcigar.p, hr);
   c->stats.ms_pair += ms_since(t0);
   collect_counters(c, 2 * P);
 }
@@ -1145,11 +1163,9 @@ static void map_pe_all(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_batc
 
 static void emit_batch(HostRecords& hr, pbw_record_batch* out) {
   out->n = hr.rec.size();
-  out->rec = (pbw_record*)malloc(std::max<size_t>(hr.rec.size(), 1) * sizeof(pbw_record));
-  out->cigar = (uint32_t*)malloc(std::max<size_t>(hr.cigar.size(), 1) * 4);
   out->n_cigar = hr.cigar.size();
-  if (!hr.rec.empty()) memcpy(out->rec, hr.rec.data(), hr.rec.size() * sizeof(pbw_record));
-  if (!hr.cigar.empty()) memcpy(out->cigar, hr.cigar.data(), hr.cigar.size() * 4);
+  out->rec = hr.rec.release();
+  out->cigar = hr.cigar.release();
 }
 
 template <typename F>
@@ -1246,14 +1262,16 @@ static double mlp_epoch(pbw_ctx* c, uint64_t n, const pbw_train_options& o, uint
 }
 
 // Training rows of the prefilter: plain mapping with features on; per aligned
-// candidate its features, label (log-likelihood within label_nats of the
-// read's best) and rank.
+// candidate its features, label (log-likelihood within `margin` nats, or DP
+// score within `margin` units, of the read's best) and rank.
 struct PfRows {
   std::vector<uint16_t> x;
   std::vector<float> y;
   std::vector<uint8_t> rank;
 };
-static void prefilter_rows(pbw_ctx* c, const pbw_read_batch* in, float label_nats, PfRows& out) {
+static void prefilter_rows(pbw_ctx* c, const pbw_read_batch* in, int label_kind, float margin, PfRows& out) {
+  if (label_kind != PBW_LABEL_LOGLK && label_kind != PBW_LABEL_SCORE)
+    throw PbwError{PBW_ERR_ARG, "unknown prefilter label kind"};
   hipStream_t s = c->stream;
   const uint64_t bs = c->opt.batch_reads ? c->opt.batch_reads : 131072;
   for (uint64_t b0 = 0; b0 < in->n; b0 += bs) {
@@ -1277,12 +1295,14 @@ static void prefilter_rows(pbw_ctx* c, const pbw_read_batch* in, float label_nat
     for (uint32_t r = 0; r < B; r++) {
       const size_t s0 = (size_t)r * TOP_CANDIDATES;
       if (!nj[r]) continue;
-      double best = dout[s0].log_lk;
-      for (uint32_t k = 1; k < nj[r]; k++) best = std::max(best, dout[s0 + k].log_lk);
+      const bool by_score = label_kind == PBW_LABEL_SCORE;
+      auto key = [&](uint32_t k) { return by_score ? (double)dout[s0 + k].score : dout[s0 + k].log_lk; };
+      double best = key(0);
+      for (uint32_t k = 1; k < nj[r]; k++) best = std::max(best, key(k));
       for (uint32_t k = 0; k < nj[r]; k++) {
         const uint32_t cand = jobs[s0 + k].cand;
         out.x.insert(out.x.end(), feat.begin() + (size_t)cand * MLP_IN, feat.begin() + (size_t)(cand + 1) * MLP_IN);
-        out.y.push_back(dout[s0 + k].log_lk >= best - (double)label_nats ? 1.f : 0.f);
+        out.y.push_back(key(k) >= best - (double)margin ? 1.f : 0.f);
         out.rank.push_back((uint8_t)(cand - s0));
       }
     }
@@ -1291,10 +1311,10 @@ static void prefilter_rows(pbw_ctx* c, const pbw_read_batch* in, float label_nat
   HIP_CHECK(hipStreamSynchronize(s));
 }
 
-static void prefilter_train(pbw_ctx* c, const pbw_read_batch* in, const pbw_train_options& o, float label_nats,
-                            float recall, pbw_prefilter_report& rep) {
+static void prefilter_train(pbw_ctx* c, const pbw_read_batch* in, const pbw_train_options& o, int label_kind,
+                            float margin, float recall, pbw_prefilter_report& rep) {
   PfRows rows;
-  prefilter_rows(c, in, label_nats, rows);
+  prefilter_rows(c, in, label_kind, margin, rows);
   const uint64_t n = rows.y.size();
   if (n < 20) throw PbwError{PBW_ERR_ARG, "too few candidate rows to train the prefilter"};
   const uint64_t nval = n / 10, ntr = n - nval;
@@ -1789,12 +1809,12 @@ int pbw_prefilter_get(const pbw_ctx* c, pbw_prefilter_options* o) {
   return PBW_OK;
 }
 
-int pbw_prefilter_dataset(pbw_ctx* c, const pbw_read_batch* reads, float label_nats, uint16_t* x, float* y,
-                          uint8_t* rank, uint64_t cap, uint64_t* n) {
+int pbw_prefilter_dataset(pbw_ctx* c, const pbw_read_batch* reads, int label_kind, float margin, uint16_t* x,
+                          float* y, uint8_t* rank, uint64_t cap, uint64_t* n) {
   if (!c || !reads || !n) return PBW_ERR_ARG;
   return guarded(c, [&] {
     PfRows rows;
-    prefilter_rows(c, reads, label_nats, rows);
+    prefilter_rows(c, reads, label_kind, margin, rows);
     const uint64_t m = rows.y.size(), w = std::min<uint64_t>(m, cap);
     if (x && w) std::memcpy(x, rows.x.data(), w * MLP_IN * 2);
     if (y && w) std::memcpy(y, rows.y.data(), w * 4);
@@ -1803,12 +1823,12 @@ int pbw_prefilter_dataset(pbw_ctx* c, const pbw_read_batch* reads, float label_n
   });
 }
 
-int pbw_prefilter_train(pbw_ctx* c, const pbw_read_batch* reads, const pbw_train_options* o, float label_nats,
-                        float recall, pbw_prefilter_report* out) {
+int pbw_prefilter_train(pbw_ctx* c, const pbw_read_batch* reads, const pbw_train_options* o, int label_kind,
+                        float margin, float recall, pbw_prefilter_report* out) {
   if (!c || !reads || !o || !o->batch) return PBW_ERR_ARG;
   return guarded(c, [&] {
     pbw_prefilter_report rep{};
-    prefilter_train(c, reads, *o, label_nats, recall, rep);
+    prefilter_train(c, reads, *o, label_kind, margin, recall, rep);
     if (out) *out = rep;
   });
 }

@@ -261,6 +261,13 @@ struct PairArgs {
 };
 void launch_pair_plan(const PairArgs& a, hipStream_t s);
 void launch_pair_finalize(const PairArgs& a, hipStream_t s);
+// records of a batch to host memory (device-side CIGAR offsets and runs)
+constexpr uint32_t REC_BLOCK = 1024;
+void launch_rec_src_se(const uint32_t* best, uint32_t n, uint32_t cap, uint64_t* src, hipStream_t s);
+void launch_rec_offsets(uint8_t* rec, const uint64_t* src, uint32_t n, uint32_t base, uint32_t* bsum,
+                        uint32_t* total, hipStream_t s);
+void launch_rec_cigar(const uint8_t* rec, const uint64_t* src, const uint32_t* cig0, const uint32_t* cig1,
+                      const uint64_t* seq_off, uint32_t n, uint32_t base, uint32_t* out, hipStream_t s);
 void gather_cigar2(const uint32_t* cig0, const uint32_t* cig1, const uint64_t* src, const uint32_t* off,
                    const uint32_t* cnt, uint32_t n, uint32_t* out, hipStream_t s);
 

@@ -2492,6 +2492,95 @@ void gather_cigar(const uint32_t* cig, const uint64_t* src, const uint32_t* off,
   hipLaunchKernelGGL(k_gather_cigar, dim3((n + 3) / 4), dim3(256), 0, s, cig, src, off, cnt, n, out);
 }
 
+// ---------------------------------------------------------------------------
+// records to host memory: CIGAR word counts, offsets and runs of one batch.
+// src[i] is the first CIGAR word of record i in its pool (bit 63 selects the
+// second pool), or ~0 for an unmapped read, whose CIGAR is the single run
+// SoftClip(L) (single.rs:368).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t rec_words(const RecOut* rec, const uint64_t* src, uint32_t i) {
+  return src[i] == ~0ull ? 1u : rec[i].cigar_n;
+}
+
+__global__ __launch_bounds__(REC_BLOCK) void k_rec_block_sums(const RecOut* rec, const uint64_t* src, uint32_t n,
+                                                              uint32_t* bsum) {
+  __shared__ uint32_t sh[REC_BLOCK / 64];
+  const uint32_t i = blockIdx.x * REC_BLOCK + threadIdx.x;
+  uint32_t tot;
+  team_excl_scan<REC_BLOCK>(i < n ? rec_words(rec, src, i) : 0u, tot, sh);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// Exclusive scan of the block sums, starting at `base`; the batch's word count to total[0].
+__global__ __launch_bounds__(REC_BLOCK) void k_rec_scan_sums(uint32_t* bsum, uint32_t nb, uint32_t base,
+                                                             uint32_t* total) {
+  __shared__ uint32_t sh[REC_BLOCK / 64];
+  uint32_t carry = base;
+  for (uint32_t b0 = 0; b0 < nb; b0 += REC_BLOCK) {
+    const uint32_t b = b0 + threadIdx.x;
+    uint32_t tot;
+    const uint32_t ex = team_excl_scan<REC_BLOCK>(b < nb ? bsum[b] : 0u, tot, sh);
+    if (b < nb) bsum[b] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) total[0] = carry - base;
+}
+
+__global__ __launch_bounds__(REC_BLOCK) void k_rec_offsets(RecOut* rec, const uint64_t* src, uint32_t n,
+                                                           const uint32_t* bsum) {
+  __shared__ uint32_t sh[REC_BLOCK / 64];
+  const uint32_t i = blockIdx.x * REC_BLOCK + threadIdx.x;
+  const uint32_t v = i < n ? rec_words(rec, src, i) : 0u;
+  uint32_t tot;
+  const uint32_t ex = team_excl_scan<REC_BLOCK>(v, tot, sh);
+  if (i < n) {
+    rec[i].cigar_off = bsum[blockIdx.x] + ex;
+    rec[i].cigar_n = v;
+  }
+}
+
+// Thread per record: its runs to out[cigar_off - base ...].
+__global__ void k_rec_cigar(const RecOut* rec, const uint64_t* src, const uint32_t* cig0, const uint32_t* cig1,
+                            const uint64_t* seq_off, uint32_t n, uint32_t base, uint32_t* out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t sv = src[i];
+  uint32_t* o = out + (rec[i].cigar_off - base);
+  if (sv == ~0ull) {
+    o[0] = (uint32_t)((seq_off[i + 1] - seq_off[i]) << 4) | 4u;
+    return;
+  }
+  const uint32_t* pool = (sv >> 63) ? cig1 : cig0;
+  const uint64_t at = sv & ~(1ull << 63);
+  const uint32_t cnt = rec[i].cigar_n;
+  for (uint32_t t = 0; t < cnt; t++) o[t] = pool[at + t];
+}
+
+// src of single-end records: slot `best[i]` of the DP output pool (cap words per slot).
+__global__ void k_rec_src_se(const uint32_t* best, uint32_t n, uint32_t cap, uint64_t* src) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) src[i] = best[i] == 0xFFFFFFFFu ? ~0ull : (uint64_t)best[i] * cap;
+}
+
+void launch_rec_src_se(const uint32_t* best, uint32_t n, uint32_t cap, uint64_t* src, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_rec_src_se, dim3((n + 255) / 256), dim3(256), 0, s, best, n, cap, src);
+}
+void launch_rec_offsets(uint8_t* recb, const uint64_t* src, uint32_t n, uint32_t base, uint32_t* bsum,
+                        uint32_t* total, hipStream_t s) {
+  RecOut* rec = reinterpret_cast<RecOut*>(recb);
+  const uint32_t nb = (n + REC_BLOCK - 1) / REC_BLOCK;
+  if (nb) hipLaunchKernelGGL(k_rec_block_sums, dim3(nb), dim3(REC_BLOCK), 0, s, rec, src, n, bsum);
+  hipLaunchKernelGGL(k_rec_scan_sums, dim3(1), dim3(REC_BLOCK), 0, s, bsum, nb, base, total);
+  if (nb) hipLaunchKernelGGL(k_rec_offsets, dim3(nb), dim3(REC_BLOCK), 0, s, rec, src, n, bsum);
+}
+void launch_rec_cigar(const uint8_t* recb, const uint64_t* src, const uint32_t* cig0, const uint32_t* cig1,
+                      const uint64_t* seq_off, uint32_t n, uint32_t base, uint32_t* out, hipStream_t s) {
+  const RecOut* rec = reinterpret_cast<const RecOut*>(recb);
+  if (n)
+    hipLaunchKernelGGL(k_rec_cigar, dim3((n + 255) / 256), dim3(256), 0, s, rec, src, cig0, cig1, seq_off, n, base,
+                       out);
+}
+
 // Same, from two pools (bit 63 of src selects the second).
 __global__ void k_gather_cigar2(const uint32_t* cig0, const uint32_t* cig1, const uint64_t* src, const uint32_t* off,
                                 const uint32_t* cnt, uint32_t n, uint32_t* out) {

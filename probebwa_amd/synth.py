@@ -164,6 +164,65 @@ def reads_uneven_se(contigs, n=2_000_000, seed=12, lmin=80, lmax=150, chunk=6553
     return seq, so, qual, so.copy()
 
 
+def reads_probe_uneven_se(contigs, n=2_000_000, seed=31, lmin=80, lmax=150, chunk=65536):
+    """Probe-style single-end reads of uneven length: L ~ U{lmin..lmax}, start uniform over
+    the contigs, strand 50/50, 0.05% per-base indel events of length U{1,2,3} (ins/del 50/50),
+    then 1% substitutions, qualities i.i.d. U{20..40}.  Vectorised in chunks (only the reads
+    with an indel event take a per-read pass); returns flat (seq, seq_off, qual, qual_off)."""
+    rng = np.random.default_rng(np.random.PCG64(seed))
+    genome = np.concatenate([c[1] for c in contigs])
+    lens = np.array([len(c[1]) for c in contigs], np.int64)
+    starts = np.concatenate([[0], np.cumsum(lens)[:-1]])
+    pad = 12  # template slack: deletions consume bases past the read's end
+    L = rng.integers(lmin, lmax + 1, n)
+    cid = rng.choice(len(contigs), n, p=lens / lens.sum())
+    W = lmax + pad
+    st = (rng.random(n) * (lens[cid] - W)).astype(np.int64) + starts[cid]  # template inside its contig
+    rev = rng.random(n) < 0.5
+    so = np.zeros(n + 1, np.uint64)
+    so[1:] = np.cumsum(L)
+    seq = np.zeros(int(so[-1]) + 1, np.uint8)
+    qual = np.zeros(int(so[-1]) + 1, np.uint8)
+    col = np.arange(W)
+    for c0 in range(0, n, chunk):
+        c1 = min(n, c0 + chunk)
+        m = c1 - c0
+        Lc, sc, rc = L[c0:c1], st[c0:c1], rev[c0:c1]
+        t = genome[sc[:, None] + col[None, :]]
+        ev = (rng.random((m, W)) < 0.0005) & (col[None, :] < Lc[:, None] + 3)
+        for r in np.nonzero(ev.any(axis=1))[0]:
+            parts, i = [], 0
+            for p in np.nonzero(ev[r])[0]:
+                if p < i:
+                    continue
+                parts.append(t[r, i:p])
+                ln = int(rng.integers(1, 4))
+                if rng.random() < 0.5:
+                    parts.append(random_bases(rng, ln))
+                    i = p
+                else:
+                    i = min(W, p + ln)
+            parts.append(t[r, i:])
+            row = np.concatenate(parts)
+            if len(row) < W:
+                row = np.concatenate([row, random_bases(rng, W - len(row))])
+            t[r] = row[:W]
+        b = t[:, :lmax]
+        codes = np.searchsorted(BASES, b)
+        err = rng.random((m, lmax)) < 0.01
+        b = np.where(err, BASES[(codes + rng.integers(1, 4, (m, lmax))) % 4], b)
+        # reverse reads are the reverse complement of the first L template bases
+        ridx = np.where(rc[:, None], Lc[:, None] - 1 - col[None, :lmax], col[None, :lmax])
+        valid = col[None, :lmax] < Lc[:, None]
+        b = np.take_along_axis(b, np.where(valid, ridx, 0), axis=1)
+        b = np.where(rc[:, None], COMP[b], b)
+        q = rng.integers(20, 41, (m, lmax)).astype(np.uint8)
+        o0 = int(so[c0])
+        seq[o0:int(so[c1])] = b[valid]
+        qual[o0:int(so[c1])] = q[valid]
+    return seq, so, qual, so.copy()
+
+
 def reads_pairs(contigs, n=10_000, L=150, seed=13, repeat_spans=()):
     """FR pairs: fragment ~ N(350, 50) clipped to [200, 600]; 10% anchored in a repeat span."""
     rng = np.random.default_rng(np.random.PCG64(seed))

@@ -86,14 +86,35 @@ def test_mlp_training_learns_a_linear_rule(ctx):
     assert acc > 0.95 and acc > acc0 + 0.2, (acc, acc0)
 
 
+def test_adam_steps_match_reference(ctx):
+    """Two training steps (one batch each) vs Adam applied to the GPU's own gradient.
+    The gradient of a step is summed in a shuffled row order, so it can differ from
+    pbw_mlp_gradient's in the last FP32 bits; Adam's first steps are about lr*sign(g),
+    so only an element whose gradient is ~0 may move by up to 2 lr."""
+    rng = np.random.default_rng(31)
+    n, lr = 512, 1e-3
+    xb, _ = rand_rows(rng, n)
+    y = (rng.random(n) < 0.3).astype(np.float32)
+    ctx.mlp_init(seed=13)  # resets the Adam moments and step count
+    th = ctx.mlp_params.astype(np.float64)
+    m = np.zeros(P.MLP_PARAMS)
+    v = np.zeros(P.MLP_PARAMS)
+    for t in (1, 2):
+        g, _ = ctx.mlp_gradient(xb, y, 2.0)
+        ctx.mlp_train(xb, y, epochs=1, batch=n, lr=lr, pos_weight=2.0, seed=t)
+        th, m, v = R.adam_step(th, m, v, g.astype(np.float64), t, lr=lr)
+        got = ctx.mlp_params.astype(np.float64)
+        err = np.abs(got - th)
+        tol = 1e-6 * np.abs(th) + 1e-3 * lr
+        assert np.mean(err > tol) < 1e-3, (t, np.mean(err > tol))
+        assert np.all(err <= 2 * t * lr + 1e-6 * np.abs(th)), (t, err.max())
+        th = got  # continue from the GPU's parameters
+
+
 # ---------------------------------------------------------------------------
 # prefilter in the mapping path: features from clustering, labels from plain DP
 # ---------------------------------------------------------------------------
-def sam_identical(a, b):
-    same = ((a.rec["pos"] == b.rec["pos"]) & (a.rec["contig"] == b.rec["contig"]) & (a.rec["flag"] == b.rec["flag"])
-            & (a.rec["mapq"] == b.rec["mapq"]) & (a.rec["strand"] == b.rec["strand"]))
-    cig = np.array([a.cigar_words(i) == b.cigar_words(i) for i in range(len(a))])
-    return same & cig
+sam_identical = P.primary_identical
 
 
 @pytest.fixture(scope="module")
@@ -111,7 +132,7 @@ def test_prefilter_dataset_rows_and_labels(probe_genome):
     g, train, _ = probe_genome
     ctx = P.Context(g)
     sub = train.subset(0, 2000)
-    x, y, rank = ctx.prefilter_dataset(sub, label_nats=40.0)
+    x, y, rank = ctx.prefilter_dataset(sub, label="loglk", margin=40.0)
     plain = ctx.map_se(sub)
     # one row per aligned candidate; the best candidate of every mapped read is a positive
     assert len(x) == len(y) == len(rank) and len(x) >= int((plain.rec["pos"] > 0).sum())
@@ -124,10 +145,30 @@ def test_prefilter_dataset_rows_and_labels(probe_genome):
     assert np.all((f[:, 10] == 1) == (rank == 0))  # feature 10 flags the first candidate
 
 
+def test_prefilter_label_rules(probe_genome):
+    g, train, _ = probe_genome
+    ctx = P.Context(g)
+    sub = train.subset(0, 1500)
+    x, y_nats, rank = ctx.prefilter_dataset(sub, label="loglk", margin=40.0)
+    x2, y_score, rank2 = ctx.prefilter_dataset(sub, label="score", margin=10.0)
+    assert np.array_equal(x, x2) and np.array_equal(rank, rank2)  # same rows, other labels
+    _, y_all, _ = ctx.prefilter_dataset(sub, label="score", margin=1e9)
+    _, y_tight, _ = ctx.prefilter_dataset(sub, label="score", margin=0.0)
+    assert y_all.min() == 1.0
+    # rows of a read are contiguous and start at rank 0: every read has a best, so
+    # at least one positive at margin 0, and widening the margin only adds positives
+    starts = np.nonzero(rank == 0)[0]
+    assert np.all(np.add.reduceat(y_tight, starts) >= 1)
+    assert np.all(y_score >= y_tight) and np.all(y_all >= y_score)
+    assert y_nats.sum() > y_tight.sum()
+    with pytest.raises(KeyError):
+        ctx.prefilter_dataset(sub, label="depth")
+
+
 def test_prefilter_path_is_sam_identical_and_skips_dp(probe_genome):
     g, train, held = probe_genome
     ctx = P.Context(g)
-    rep = ctx.prefilter_train(train, label_nats=40.0, recall=0.9999, epochs=20, batch=4096)
+    rep = ctx.prefilter_train(train, label="loglk", margin=40.0, recall=0.9999, epochs=20, batch=4096)
     assert rep["loss_last"] < rep["loss_first"] and rep["val_recall"] >= 0.999
     plain = ctx.map_se(held)
     cells_plain = ctx.stats()["dp_cells"]
@@ -150,7 +191,7 @@ def test_trained_network_beats_the_rank_rule(probe_genome):
     keeps SAM identity with a large skip."""
     g, train, held = probe_genome
     ctx = P.Context(g)
-    ctx.prefilter_train(train, label_nats=40.0, recall=0.9999, epochs=20, batch=4096)
+    ctx.prefilter_train(train, label="loglk", margin=40.0, recall=0.9999, epochs=20, batch=4096)
     plain = ctx.map_se(held)
     thr = ctx.prefilter.threshold
     ctx.set_prefilter(enabled=1, audit_period=0)
