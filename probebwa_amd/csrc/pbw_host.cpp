@@ -254,6 +254,7 @@ struct pbw_ctx {
   // genome + index
   DevBuf<uint64_t> d_g2b, d_nmask, d_coff, d_clen;
   DevBuf<uint32_t> d_koff;
+  DevBuf<uint64_t> d_pres;
   DevBuf<uint64_t> d_kpos;
   uint64_t total_entries = 0;
   // LUTs
@@ -363,6 +364,8 @@ static void ctx_build(pbw_ctx* c) {
   dstart.upload(hi.kstart.data(), hi.kstart.size(), s);
   c->d_koff.ensure(CODE_SPACE + 1);
   launch_fill_koff(c->d_koff.p, dkeys.p, dstart.p, hi.keys.size(), (uint32_t)hi.pos.size(), s);
+  c->d_pres.ensure(CODE_SPACE / 64);
+  launch_fill_pres(c->d_koff.p, c->d_pres.p, s);
   HIP_CHECK(hipGetLastError());
   // LUTs (bayesian.rs:103-110, :142-179, :405-439; seeds.rs:133-134; smith_waterman.rs:272-275)
   const pbw_options& o = c->opt;
@@ -588,7 +591,7 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false) {
   c->d_nhits.ensure(B);
   SeedArgs sa{};
   sa.G = c->dev_genome();
-  sa.X = DevIndex{c->d_koff.p, c->d_kpos.p};
+  sa.X = DevIndex{c->d_koff.p, c->d_kpos.p, c->d_pres.p};
   sa.R = R;
   sa.rlist = nullptr;
   sa.nlist = B;

@@ -39,6 +39,7 @@ struct DevGenome {
 struct DevIndex {
   const uint32_t* koff;  // [2^30 + 1] bucket start by 15-mer code; bucket = [koff[c], koff[c+1])
   const uint64_t* kpos;  // packed positions: bit 63 strand, 62..40 contig, 39..0 position
+  const uint64_t* pres;  // [2^24] bit c%64 of word c/64 set when bucket c is non-empty
 };
 
 struct DevReads {
@@ -273,6 +274,7 @@ void gather_cigar2(const uint32_t* cig0, const uint32_t* cig1, const uint64_t* s
 
 void gather_cigar(const uint32_t* cig, const uint64_t* src, const uint32_t* off, const uint32_t* cnt, uint32_t n,
                   uint32_t* out, hipStream_t s);
+void launch_fill_pres(const uint32_t* koff, uint64_t* pres, hipStream_t s);
 void launch_fill_koff(uint32_t* koff, const uint32_t* keys, const uint32_t* kstart, uint64_t nkeys, uint32_t total,
                       hipStream_t s);
 

@@ -300,9 +300,12 @@ __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
             uint32_t orig = (wc >> sh) & 3u;
             uint32_t nbase = alt < orig ? alt : alt + 1;
             uint32_t ncode = (wc & ~(3u << sh)) | (nbase << sh);
-            loff = a.X.koff[ncode];
-            lcnt = a.X.koff[ncode + 1] - loff;
-            if (lcnt >= (uint32_t)REPEAT_SKIP) S.nbrep[w] = 1;
+            // presence bitmap (128 MiB) first: most neighbour codes have empty buckets
+            if ((a.X.pres[ncode >> 6] >> (ncode & 63)) & 1ull) {
+              loff = a.X.koff[ncode];
+              lcnt = a.X.koff[ncode + 1] - loff;
+              if (lcnt >= (uint32_t)REPEAT_SKIP) S.nbrep[w] = 1;
+            }
           }
           c_look++;
           c_pos += lcnt;
@@ -2616,6 +2619,18 @@ __global__ void k_fill_koff(uint32_t* koff, const uint32_t* keys, const uint32_t
     koff[c] = lo < nkeys ? kstart[lo] : total;
   }
 }
+// Presence bitmap of the bucket table: one wave per 64 consecutive codes.
+__global__ void k_fill_pres(const uint32_t* koff, uint64_t* pres) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < CODE_SPACE; c += stride) {
+    const uint64_t m = __ballot(koff[c + 1] > koff[c]);
+    if (lane_id() == 0) pres[c >> 6] = m;
+  }
+}
+void launch_fill_pres(const uint32_t* koff, uint64_t* pres, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_pres, dim3(65536), dim3(256), 0, s, koff, pres);
+}
+
 void launch_fill_koff(uint32_t* koff, const uint32_t* keys, const uint32_t* kstart, uint64_t nkeys, uint32_t total,
                       hipStream_t s) {
   hipLaunchKernelGGL(k_fill_koff, dim3(65536), dim3(256), 0, s, koff, keys, kstart, nkeys, total);
