@@ -71,7 +71,7 @@ typedef struct {
   const uint64_t* qual_off; /* n+1 */
 } pbw_read_batch;
 
-/* Library-owned output arena; free with pbw_records_release. */
+/* Library-owned output arena in pinned host memory; free with pbw_records_release. */
 typedef struct {
   uint64_t n;
   pbw_record* rec;

@@ -267,7 +267,8 @@ struct pbw_ctx {
   DevBuf<uint64_t> d_seq_off, d_qual_off, d_hoff, d_hits, d_hits_ov, d_scratch;
   DevBuf<uint32_t> d_nhits, d_list, d_hcap, d_tier2, d_ntier2, d_best, d_cigar, d_cigar_out, d_cigar_off;
   DevBuf<Cand> d_cands;
-  DevBuf<uint8_t> d_ncand, d_njobs, d_generic, d_rec, d_refbytes;
+  DevBuf<uint8_t> d_ncand, d_njobs, d_rec, d_refbytes;
+  DevBuf<uint32_t> d_flag_a, d_flag_b, d_flag_c, d_flag_n;
   DevBuf<DpJob> d_jobs;
   DevBuf<DpOut> d_out;
   DevBuf<Counters> d_ctr;
@@ -433,7 +434,9 @@ static void ctx_build(pbw_ctx* c) {
 // ---------------------------------------------------------------------------
 // single-end batch: seed -> cluster -> jobs -> DP -> MAPQ
 // ---------------------------------------------------------------------------
-// A malloc'd array that an output batch takes over (pbw_records_release frees it).
+// A pinned host array that an output batch takes over (pbw_records_release
+// frees it with hipHostFree).  Device-to-host copies into pinned memory run at
+// the link's rate; into fresh pageable pages they run several times slower.
 template <typename T>
 struct HostArr {
   T* p = nullptr;
@@ -441,11 +444,16 @@ struct HostArr {
   HostArr() = default;
   HostArr(const HostArr&) = delete;
   HostArr& operator=(const HostArr&) = delete;
-  ~HostArr() { free(p); }
+  ~HostArr() {
+    if (p) (void)hipHostFree(p);
+  }
   void reserve(size_t want) {
     if (want <= cap) return;
-    T* q = (T*)realloc(p, want * sizeof(T));
-    if (!q) throw PbwError{PBW_ERR_LIMIT, "host allocation of " + std::to_string(want * sizeof(T)) + " bytes failed"};
+    T* q = nullptr;
+    if (hipHostMalloc((void**)&q, want * sizeof(T), hipHostMallocDefault) != hipSuccess || !q)
+      throw PbwError{PBW_ERR_LIMIT, "pinned host allocation of " + std::to_string(want * sizeof(T)) + " bytes failed"};
+    if (n) memcpy(q, p, n * sizeof(T));
+    if (p) (void)hipHostFree(p);
     p = q;
     cap = want;
   }
@@ -489,14 +497,28 @@ static DpArgs dp_args_base(pbw_ctx* c, const DevReads& R) {
 }
 
 // Runs the tiled DP kernel over a job list (wide windows, odd shapes).
-static void run_dp_tiled(pbw_ctx* c, DpArgs a, const std::vector<uint32_t>& list) {
-  if (list.empty()) return;
+// One device word to the host (synchronises the stream).
+static uint32_t download_u32(pbw_ctx* c, const uint32_t* d) {
+  uint32_t v = 0;
+  HIP_CHECK(hipMemcpyAsync(&v, d, 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  return v;
+}
+
+// Jobs of a device list through the tiled kernel.  It flags only jobs beyond the
+// batch's max_m / max_n, which the callers never pass; any flag is an error.
+static void run_dp_tiled_dev(pbw_ctx* c, DpArgs a, const uint32_t* dlist, uint32_t n) {
+  if (!n) return;
   hipStream_t s = c->stream;
-  c->d_list.upload(list.data(), list.size(), s);
-  a.jlist = c->d_list.p;
-  a.njobs = (uint32_t)list.size();
+  a.jlist = dlist;
+  a.njobs = n;
   a.job_valid = nullptr;
-  uint32_t nwaves = std::min<uint32_t>((uint32_t)list.size(), 2048);
+  c->d_flag_n.ensure(4);
+  c->d_flag_c.ensure(n);
+  a.flag_list = c->d_flag_c.p;
+  a.flag_n = c->d_flag_n.p + 2;
+  HIP_CHECK(hipMemsetAsync(a.flag_n, 0, 4, s));
+  uint32_t nwaves = std::min<uint32_t>(n, 2048);
   a.tiles = (a.max_n + 1 + 63) / 64;
   c->d_rows.ensure((size_t)nwaves * 3 * (a.max_n + 1));
   c->d_tb.ensure((size_t)nwaves * (a.max_m + 1) * a.tiles * 4);
@@ -504,6 +526,12 @@ static void run_dp_tiled(pbw_ctx* c, DpArgs a, const std::vector<uint32_t>& list
   a.tb = c->d_tb.p;
   launch_dp_tiled(a, nwaves, s);
   HIP_CHECK(hipGetLastError());
+  if (download_u32(c, a.flag_n)) throw PbwError{PBW_ERR_LIMIT, "alignment window beyond the tiled kernel's shape"};
+}
+static void run_dp_tiled(pbw_ctx* c, DpArgs a, const std::vector<uint32_t>& list) {
+  if (list.empty()) return;
+  c->d_list.upload(list.data(), list.size(), c->stream);
+  run_dp_tiled_dev(c, a, c->d_list.p, (uint32_t)list.size());
 }
 
 // Device state of one batch between the stages.
@@ -739,8 +767,11 @@ static void batch_dp(pbw_ctx* c, Batch& b, const uint32_t* jlist, uint32_t nlist
   const int32_t band = (int32_t)c->opt.max_indel_len;
   const int32_t right_pad = (int32_t)std::max<uint32_t>(c->opt.max_indel_len, 10);  // single.rs:110
   const size_t nslots = (size_t)B * TOP_CANDIDATES;
-  c->d_generic.ensure(nslots);
-  HIP_CHECK(hipMemsetAsync(c->d_generic.p, 0, nslots, s));
+  // flagged-job lists: quad -> band (a), band -> tiled (b); counters [0], [1]
+  c->d_flag_a.ensure(nslots);
+  c->d_flag_b.ensure(nslots);
+  c->d_flag_n.ensure(4);
+  HIP_CHECK(hipMemsetAsync(c->d_flag_n.p, 0, 8, s));
   DpArgs da = dp_args_base(c, R);
   da.jobs = c->d_jobs.p;
   da.jlist = jlist;
@@ -749,7 +780,6 @@ static void batch_dp(pbw_ctx* c, Batch& b, const uint32_t* jlist, uint32_t nlist
   da.cigar_cap = b.cap;
   da.cigar = c->d_cigar.p;
   da.out = c->d_out.p;
-  da.needs_generic = c->d_generic.p;
   da.max_m = b.max_m;
   da.max_n = b.max_n;
   if (2 * band + 1 <= 63) {
@@ -758,41 +788,35 @@ static void batch_dp(pbw_ctx* c, Batch& b, const uint32_t* jlist, uint32_t nlist
     DpArgs qa = da;
     qa.max_m = std::min<uint32_t>(b.max_m, QUAD_MAX_M);
     qa.max_n = std::min<uint32_t>(b.max_n, qa.max_m + (uint32_t)right_pad + LEFT_PAD);
+    qa.flag_list = c->d_flag_a.p;
+    qa.flag_n = c->d_flag_n.p;
+    DpArgs ba = da;
+    ba.flag_list = c->d_flag_b.p;
+    ba.flag_n = c->d_flag_n.p + 1;
     HIP_CHECK(hipEventRecord(c->ev[2], s));
     const bool quad = launch_dp_quad(qa, s);
-    if (!quad) launch_dp_band(da, s);
+    if (!quad) launch_dp_band(ba, s);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(c->ev[3], s));
-    std::vector<uint8_t> gen(nslots);
-    HIP_CHECK(hipMemcpyAsync(gen.data(), c->d_generic.p, nslots, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
+    const uint32_t nflag = quad ? download_u32(c, qa.flag_n) : 0u;
     {
       float ms = 0.f;
+      HIP_CHECK(hipEventSynchronize(c->ev[3]));
       HIP_CHECK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
       c->stats.ms_k_dp += ms;
       c->stats.n_k_dp += 1;
     }
-    std::vector<uint32_t> list;
-    for (size_t k = 0; k < nslots; k++)
-      if (gen[k]) list.push_back((uint32_t)k);
-    if (quad && !list.empty()) {
-      c->stats.band_dp_jobs += list.size();
-      HIP_CHECK(hipMemsetAsync(c->d_generic.p, 0, nslots, s));
-      c->d_list.upload(list.data(), list.size(), s);
-      DpArgs ba = da;
-      ba.jlist = c->d_list.p;
-      ba.njobs = (uint32_t)list.size();
+    if (nflag) {
+      c->stats.band_dp_jobs += nflag;
+      ba.jlist = c->d_flag_a.p;
+      ba.njobs = nflag;
       ba.job_valid = nullptr;
       launch_dp_band(ba, s);
       HIP_CHECK(hipGetLastError());
-      HIP_CHECK(hipMemcpyAsync(gen.data(), c->d_generic.p, nslots, hipMemcpyDeviceToHost, s));
-      HIP_CHECK(hipStreamSynchronize(s));
-      list.clear();
-      for (size_t k = 0; k < nslots; k++)
-        if (gen[k]) list.push_back((uint32_t)k);
     }
-    run_dp_tiled(c, da, list);
-    c->stats.generic_dp_jobs += list.size();
+    const uint32_t ntiled = download_u32(c, ba.flag_n);
+    run_dp_tiled_dev(c, da, c->d_flag_b.p, ntiled);
+    c->stats.generic_dp_jobs += ntiled;
   } else {
     std::vector<uint32_t> list;
     if (jlist) {
@@ -880,13 +904,6 @@ static MapqArgs mapq_args(pbw_ctx* c, const Batch& b) {
   ma.rec = c->d_rec.p;
   ma.best = c->d_best.p;
   return ma;
-}
-
-static uint32_t download_u32(pbw_ctx* c, const uint32_t* d) {
-  uint32_t v = 0;
-  HIP_CHECK(hipMemcpyAsync(&v, d, 4, hipMemcpyDeviceToHost, c->stream));
-  HIP_CHECK(hipStreamSynchronize(c->stream));
-  return v;
 }
 
 // Candidate prefilter between clustering and the DP (no counterpart in the
@@ -1102,8 +1119,6 @@ static void map_pe_batch(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_ba
   da.cigar_cap = wcap;
   da.cigar = c->d_wcigar.p;
   da.out = c->d_wout.p;
-  c->d_generic.ensure(nws);
-  da.needs_generic = c->d_generic.p;
   da.max_m = b.max_m;
   da.max_n = (uint32_t)wmax;
   c->stats.wide_jobs += wlist.size();
@@ -1611,8 +1626,9 @@ int pbw_align_batch(pbw_ctx* c, uint32_t gop, uint32_t gep, const pbw_align_job*
     const uint32_t cap = max_m + max_n + 2;
     c->d_out.ensure(n);
     c->d_cigar.ensure(n * cap);
-    c->d_generic.ensure(n);
-    HIP_CHECK(hipMemsetAsync(c->d_generic.p, 0, n, s));
+    c->d_flag_a.ensure(n);
+    c->d_flag_n.ensure(4);
+    HIP_CHECK(hipMemsetAsync(c->d_flag_n.p, 0, 4, s));
     HIP_CHECK(hipMemsetAsync(c->d_ctr.p, 0, sizeof(Counters), s));
     DpArgs da = dp_args_base(c, R);
     da.gap_open = std::min(h_f64_i32(std::round(-90.0 * (double)gop / 40.0)), -1);
@@ -1622,7 +1638,8 @@ int pbw_align_batch(pbw_ctx* c, uint32_t gop, uint32_t gep, const pbw_align_job*
     da.cigar_cap = cap;
     da.cigar = c->d_cigar.p;
     da.out = c->d_out.p;
-    da.needs_generic = c->d_generic.p;
+    da.flag_list = c->d_flag_a.p;
+    da.flag_n = c->d_flag_n.p;
     // banded kernel for narrow bands and moderate windows, tiled for the rest
     std::vector<uint32_t> band_list, tiled_list;
     uint32_t bm = 0, bn = 0;
@@ -1645,11 +1662,11 @@ int pbw_align_batch(pbw_ctx* c, uint32_t gop, uint32_t gep, const pbw_align_job*
       db.max_n = bn;
       launch_dp_band(db, s);
       HIP_CHECK(hipGetLastError());
-      std::vector<uint8_t> gen(n);
-      HIP_CHECK(hipMemcpyAsync(gen.data(), c->d_generic.p, n, hipMemcpyDeviceToHost, s));
+      const uint32_t nf = download_u32(c, c->d_flag_n.p);
+      std::vector<uint32_t> fl(nf);
+      if (nf) HIP_CHECK(hipMemcpyAsync(fl.data(), c->d_flag_a.p, (size_t)nf * 4, hipMemcpyDeviceToHost, s));
       HIP_CHECK(hipStreamSynchronize(s));
-      for (uint32_t i : band_list)
-        if (gen[i]) tiled_list.push_back(i);
+      tiled_list.insert(tiled_list.end(), fl.begin(), fl.end());
     }
     da.max_m = max_m;
     da.max_n = max_n;
@@ -1690,8 +1707,8 @@ uint64_t pbw_index_total_entries(const pbw_ctx* c) { return c ? c->total_entries
 
 void pbw_records_release(pbw_record_batch* b) {
   if (!b) return;
-  free(b->rec);
-  free(b->cigar);
+  if (b->rec) (void)hipHostFree(b->rec);
+  if (b->cigar) (void)hipHostFree(b->cigar);
   b->rec = nullptr;
   b->cigar = nullptr;
   b->n = b->n_cigar = 0;

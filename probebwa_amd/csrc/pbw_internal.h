@@ -168,7 +168,8 @@ struct DpArgs {
   uint32_t cigar_cap;
   uint32_t* cigar;         // job * cigar_cap
   DpOut* out;
-  uint8_t* needs_generic;  // per job: 1 = shape not handled by the banded kernel
+  uint32_t* flag_list;  // jobs this kernel does not handle, appended in any order (flag_n counts them)
+  uint32_t* flag_n;
   Counters* ctr;
   uint32_t max_m, max_n;   // LDS sizing
   // tiled kernel scratch

@@ -1114,6 +1114,12 @@ __device__ double dp_log_lk(const DevLuts& T, const DpStage& st, uint32_t m, uin
 }
 
 // Copies the CIGAR (forward order) and fills DpOut; lane-parallel copy.
+// Hands job `jid` to the next kernel of the chain (one lane calls it).
+__device__ __forceinline__ void dp_flag(const DpArgs& a, uint32_t jid) {
+  const uint32_t at = atomicAdd(a.flag_n, 1u);
+  a.flag_list[at] = jid;
+}
+
 __device__ void dp_emit(const DpArgs& a, uint32_t jid, const DpStage& st, uint32_t nruns, uint32_t off, uint32_t span,
                         int32_t score, double lk, uint64_t cells) {
   const int lane = lane_id();
@@ -1181,7 +1187,7 @@ __global__ __launch_bounds__(64 * DP_WPB) void k_dp_band(DpArgs a) {
       continue;
     }
     if (m > a.max_m || n > a.max_n || 2 * b + 1 > 63 || b < 0) {
-      if (lane == 0) a.needs_generic[jid] = 1;
+      if (lane == 0) dp_flag(a, jid);
       continue;
     }
     dp_stage_job(a, job, m, qlen, so, qo, st);
@@ -1435,7 +1441,7 @@ __global__ __launch_bounds__(64) void k_dp_quad(DpArgs a) {
       const bool regular = m >= 1 && n >= 1 && m <= a.max_m && n <= a.max_n && n <= (uint32_t)QUAD_MAX_N && b >= 0 && b <= 31 &&
                            !job.explicit_ref && (int64_t)h + b >= 0 && (int64_t)m + h - b <= (int64_t)n;
       if (!regular) {
-        if (L == 0) a.needs_generic[jid] = 1;
+        if (L == 0) dp_flag(a, jid);
         valid = false;
       }
     }
@@ -1954,7 +1960,7 @@ __global__ __launch_bounds__(64) void k_dp_tiled(DpArgs a) {
       continue;
     }
     if (m > a.max_m || n > a.max_n) {
-      if (lane == 0) a.needs_generic[jid] = 2;  // host reports PBW_ERR_LIMIT
+      if (lane == 0) dp_flag(a, jid);  // the host reports PBW_ERR_LIMIT
       continue;
     }
     dp_stage_job(a, job, m, qlen, so, qo, st);

@@ -120,17 +120,82 @@ __device__ void mlp_load_x(const uint16_t* X, const uint32_t* perm, uint64_t row
   __syncthreads();
 }
 
-__global__ __launch_bounds__(64) void k_mlp_forward(MlpDev P, const uint16_t* X, uint64_t N, float* z) {
-  extern __shared__ __align__(16) uint8_t dyn[];
-  MlpLds& S = *(MlpLds*)dyn;
-  const int l = threadIdx.x;
-  for (uint64_t t0 = (uint64_t)blockIdx.x * 32; t0 < N; t0 += (uint64_t)gridDim.x * 32) {
-    const uint64_t nrows = N - t0 < 32 ? N - t0 : 32;
-    mlp_load_x(X, nullptr, t0, nrows, N, S);
-    uint64_t m1 = 0, m2 = 0;
-    mlp_forward_tile(P, S, m1, m2);
-    if ((uint64_t)l < nrows) z[t0 + l] = S.z[l];
-    __syncthreads();
+// Inference forward.  Every layer is computed transposed, out^T = W^T . in^T:
+// the accumulator of lane l then holds sample l % 32 and hidden units
+// 32 t + 8 g + 4 (l / 32) + j (element 4 g + j of output tile t), which is
+// exactly the B operand of k-step 4 t + g of the next layer.  So activations
+// stay in registers; only W1^T and W2^T go through LDS, staged once per block
+// with rows padded by 4 BF16 so the 32 rows a k-step reads sit in distinct
+// banks.  Each of the block's waves takes 32-row tiles in turn.
+constexpr int MLPI_WAVES = 4;
+constexpr int W1T_LD = 32 + 4, W2T_LD = 128 + 4;
+
+__device__ __forceinline__ s4v lds_op(const uint16_t* base, int ld, int s) {
+  const int l = threadIdx.x & 63;
+  return *(const s4v*)(base + (l & 31) * ld + 8 * s + 4 * (l >> 5));
+}
+
+__global__ __launch_bounds__(64 * MLPI_WAVES) void k_mlp_infer(MlpDev P, const uint16_t* X, uint64_t N, float* z) {
+  __shared__ __align__(16) uint16_t w1t[128 * W1T_LD];
+  __shared__ __align__(16) uint16_t w2t[128 * W2T_LD];
+  __shared__ float b1s[128], b2s[128], w3s[128];
+  for (int e = threadIdx.x; e < 128 * 32 / 4; e += 64 * MLPI_WAVES) {  // 4 BF16 per move
+    const int n = e >> 3, f = (e & 7) * 4;
+    *(uint64_t*)(w1t + n * W1T_LD + f) = *(const uint64_t*)(P.w1t + n * 32 + f);
+  }
+  for (int e = threadIdx.x; e < 128 * 128 / 4; e += 64 * MLPI_WAVES) {
+    const int n = e >> 5, k = (e & 31) * 4;
+    *(uint64_t*)(w2t + n * W2T_LD + k) = *(const uint64_t*)(P.w2t + n * 128 + k);
+  }
+  if (threadIdx.x < 128) {
+    b1s[threadIdx.x] = P.b1[threadIdx.x];
+    b2s[threadIdx.x] = P.b2[threadIdx.x];
+    w3s[threadIdx.x] = bf2f(P.w3[threadIdx.x]);
+  }
+  __syncthreads();
+  const int l = threadIdx.x & 63, half = l >> 5;
+  const float b3 = P.b3[0];
+  const uint64_t ntiles = (N + 31) / 32;
+  for (uint64_t tile = (uint64_t)blockIdx.x * MLPI_WAVES + (threadIdx.x >> 6); tile < ntiles;
+       tile += (uint64_t)gridDim.x * MLPI_WAVES) {
+    const uint64_t row = tile * 32 + (l & 31);
+    s4v xb[4];
+#pragma unroll
+    for (int s = 0; s < 4; s++) xb[s] = row < N ? *(const s4v*)(X + row * 32 + 8 * s + 4 * half) : s4v{0, 0, 0, 0};
+    // layer 1: h1^T = bf16(relu(W1^T x^T + b1)), kept as layer 2's B operands
+    s4v hb[16];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      f16v acc = {};
+#pragma unroll
+      for (int s = 0; s < 4; s++) acc = mfma(lds_op(w1t + 32 * W1T_LD * t, W1T_LD, s), xb[s], acc);
+#pragma unroll
+      for (int g = 0; g < 4; g++) {
+        s4v h;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const float a = acc[4 * g + j] + b1s[32 * t + 8 * g + 4 * half + j];
+          h[j] = (short)f2bf(a > 0.f ? a : 0.f);
+        }
+        hb[4 * t + g] = h;
+      }
+    }
+    // layer 2 and 3: z = bf16(relu(W2^T h1^T + b2)) . W3 + b3
+    float zp = 0.f;
+#pragma unroll 1
+    for (int t = 0; t < 4; t++) {
+      f16v acc = {};
+#pragma unroll
+      for (int s = 0; s < 16; s++) acc = mfma(lds_op(w2t + 32 * W2T_LD * t, W2T_LD, s), hb[s], acc);
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int n = 32 * t + 8 * (r >> 2) + 4 * half + (r & 3);
+        const float a = acc[r] + b2s[n];
+        zp += bf2f(f2bf(a > 0.f ? a : 0.f)) * w3s[n];
+      }
+    }
+    zp += __shfl_xor(zp, 32, 64);
+    if (half == 0 && row < N) z[row] = zp + b3;
   }
 }
 
@@ -308,9 +373,10 @@ size_t mlp_lds_bytes() { return sizeof(MlpLds); }
 
 void launch_mlp_forward(const MlpDev& P, const uint16_t* X, uint64_t N, float* z, hipStream_t s) {
   if (N == 0) return;
-  uint64_t tiles = (N + 31) / 32;
-  uint32_t blocks = (uint32_t)(tiles < 65536 ? tiles : 65536);
-  hipLaunchKernelGGL(k_mlp_forward, dim3(blocks), dim3(64), sizeof(MlpLds), s, P, X, N, z);
+  const uint64_t tiles = (N + 31) / 32;
+  const uint64_t want = (tiles + MLPI_WAVES - 1) / MLPI_WAVES;
+  const uint32_t blocks = (uint32_t)(want < 256 * 2 ? want : 256 * 2);  // 188 VGPRs: two 4-wave blocks per CU
+  hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(64 * MLPI_WAVES), 0, s, P, X, N, z);
 }
 void launch_mlp_grad(const MlpDev& P, const uint16_t* X, const float* Y, const uint32_t* perm, uint64_t row0,
                      uint64_t nrows, uint64_t N, float pos_weight, float inv_b, float* part, float* loss_part,
