@@ -268,7 +268,7 @@ struct pbw_ctx {
   DevBuf<uint32_t> d_nhits, d_list, d_hcap, d_tier2, d_ntier2, d_best, d_cigar, d_cigar_out, d_cigar_off;
   DevBuf<Cand> d_cands;
   DevBuf<uint8_t> d_ncand, d_njobs, d_rec, d_refbytes;
-  DevBuf<uint32_t> d_flag_a, d_flag_b, d_flag_c, d_flag_n;
+  DevBuf<uint32_t> d_flag_a, d_flag_b, d_flag_c, d_flag_n, d_sorted, d_sort_hist, d_tbq;
   DevBuf<DpJob> d_jobs;
   DevBuf<DpOut> d_out;
   DevBuf<Counters> d_ctr;
@@ -785,11 +785,24 @@ static void batch_dp(pbw_ctx* c, Batch& b, const uint32_t* jlist, uint32_t nlist
   if (2 * band + 1 <= 63) {
     // four jobs per wave for the regular shapes; the flagged rest goes to the
     // one-job-per-wave banded kernel, and what that flags to the tiled one
+    // valid jobs only, longest read first
+    c->d_sorted.ensure(nslots);
+    c->d_sort_hist.ensure(MAX_READ_LEN + 1);
+    launch_job_sort(da, c->d_sort_hist.p, c->d_sorted.p, c->d_flag_n.p + 3, s);
+    HIP_CHECK(hipGetLastError());
+    const uint32_t nsorted = download_u32(c, c->d_flag_n.p + 3);
     DpArgs qa = da;
+    qa.jlist = c->d_sorted.p;
+    qa.njobs = nsorted;
+    qa.job_valid = nullptr;
     qa.max_m = std::min<uint32_t>(b.max_m, QUAD_MAX_M);
     qa.max_n = std::min<uint32_t>(b.max_n, qa.max_m + (uint32_t)right_pad + LEFT_PAD);
     qa.flag_list = c->d_flag_a.p;
     qa.flag_n = c->d_flag_n.p;
+    if (qa.max_m <= QUAD_MAX_M) {
+      c->d_tbq.ensure(quad_scratch_words(qa.max_m, qa.max_n, nsorted));
+      qa.tbq = c->d_tbq.p;
+    }
     DpArgs ba = da;
     ba.flag_list = c->d_flag_b.p;
     ba.flag_n = c->d_flag_n.p + 1;

@@ -177,10 +177,16 @@ struct DpArgs {
   uint64_t* tb;            // per wave: (max_m + 1) * tiles * 4
   uint32_t tiles;          // ceil((max_n + 1) / 64)
   uint32_t wpb;            // waves per block (banded kernel; set by the launcher)
+  uint32_t* tbq;           // k_dp_quad traceback scratch: quad_scratch_words() words
 };
 void launch_dp_band(const DpArgs& a, hipStream_t s);
 void launch_dp_tiled(const DpArgs& a, uint32_t nwaves, hipStream_t s);
 bool launch_dp_quad(const DpArgs& a, hipStream_t s);  // false: layout too large, nothing launched
+constexpr uint32_t QUAD_MAX_WAVES = 8192;  // persistent grid of k_dp_quad
+size_t quad_scratch_words(uint32_t max_m, uint32_t max_n, uint32_t njobs);
+// Jobs of `a` (jlist / njobs / job_valid) sorted longest read first into `out`, count
+// to total[0]; `hist` holds MAX_READ_LEN + 1 words of scratch.
+void launch_job_sort(const DpArgs& a, uint32_t* hist, uint32_t* out, uint32_t* total, hipStream_t s);
 
 struct MapqArgs {
   DevGenome G;

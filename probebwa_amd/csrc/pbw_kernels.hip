@@ -42,6 +42,14 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Orders a wave's global-memory stores before later loads of the same data by
+// other lanes of the wave (a block of one wave: workgroup scope suffices).
+__device__ __forceinline__ void mem_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
 
 __device__ __forceinline__ int32_t shfl_up_i32(int32_t v, int d) { return __shfl_up(v, (unsigned)d, 64); }
@@ -1354,9 +1362,9 @@ struct QuadLayout {  // byte offsets inside one job's LDS area
 __host__ __device__ inline QuadLayout quad_layout(uint32_t max_m, uint32_t max_n) {
   QuadLayout l;
   const uint32_t blocks = (max_m + 7) / 8 + 1;  // + one block for the CIGAR runs
-  l.tb_words = 64 * blocks;
+  l.tb_words = 64 * blocks;  // per job, in global scratch (DpArgs::tbq)
   l.tb = 0;
-  l.go = l.tb + 4 * l.tb_words;
+  l.go = 0;
   l.code = l.go + 4 * (max_m + 64);
   l.scw = l.code + 4 * (max_m + 64);
   l.q = l.scw + 4 * ((max_m > 128 ? max_m : 128) + 4);
@@ -1405,7 +1413,9 @@ __global__ __launch_bounds__(64) void k_dp_quad(DpArgs a) {
   const int g = lane >> 4, L = lane & 15;
   const QuadLayout Y = quad_layout(a.max_m, a.max_n);
   uint8_t* jb = dyn + (size_t)g * Y.stride;
-  uint32_t* TB = (uint32_t*)(jb + Y.tb);
+  // traceback planes, then CIGAR runs and log-likelihood terms, in this group's
+  // global scratch: LDS keeps only the per-row inputs, so more waves fit per CU
+  uint32_t* TB = a.tbq + ((size_t)blockIdx.x * 4 + (size_t)g) * Y.tb_words;
   int32_t* GOA = (int32_t*)(jb + Y.go);    // gap open of window column c at [c - (1 + o)]
   int32_t* CODA = (int32_t*)(jb + Y.code);  // 6 * (A, C, G, T, N -> 0..4) of the same column
   int32_t* SCW = (int32_t*)(jb + Y.scw);
@@ -1750,6 +1760,7 @@ __global__ __launch_bounds__(64) void k_dp_quad(DpArgs a) {
     int64_t* keys = (int64_t*)SCW;
     keys[L] = key;
     wave_sync();
+    mem_sync();  // the traceback words stored by every lane
     // ---- traceback (smith_waterman.rs:299-345) by lane 0 of the group ----
     // Each step takes a whole run from at most two traceback words: diagonal M
     // steps (rows r, r-1, .. of a column, into the block below), horizontal D
@@ -1829,7 +1840,7 @@ __global__ __launch_bounds__(64) void k_dp_quad(DpArgs a) {
       nruns += cur_len != 0 ? 1u : 0u;
       off = j;
     }
-    wave_sync();
+    mem_sync();  // the CIGAR runs the walker stored
     nruns = (uint32_t)__shfl((int)nruns, lane & ~15, 64);
     off = (uint32_t)__shfl((int)off, lane & ~15, 64);
     const uint32_t* runs = runs_top - nruns;
@@ -1859,7 +1870,7 @@ __global__ __launch_bounds__(64) void k_dp_quad(DpArgs a) {
         terms[qi] = tv;
       }
     }
-    wave_sync();
+    mem_sync();  // the terms of all 16 lanes
     if (L == 0 && m) {
       double lk = 0.0;
       uint32_t qi = 0;
@@ -1901,17 +1912,95 @@ __global__ __launch_bounds__(64) void k_dp_quad(DpArgs a) {
 
 // Launches k_dp_quad; returns false (nothing launched) when one wave's four
 // job areas do not fit in 64 KiB of LDS.
+__global__ void k_job_zero(uint32_t* p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Counting sort of the DP jobs by read length, longest first: the four jobs
+// that share a wave of k_dp_quad then run the same number of rows, and slots
+// without a job (past the read's job count) drop out.  Results do not depend
+// on the order (each job writes only its own slot); inside one length the
+// order is whatever the atomics give.
+// ---------------------------------------------------------------------------
+constexpr uint32_t SORT_KEYS = MAX_READ_LEN + 1, SORT_BLOCK = 1024;
+
+__device__ __forceinline__ bool sort_job(const DpArgs& a, uint32_t t, uint32_t& jid, uint32_t& key) {
+  if (t >= a.njobs) return false;
+  jid = a.jlist ? a.jlist[t] : t;
+  if (a.job_valid && (jid % TOP_CANDIDATES) >= a.job_valid[jid / TOP_CANDIDATES]) return false;
+  const uint32_t r = a.jobs[jid].read;
+  const uint32_t m = (uint32_t)(a.R.seq_off[r + 1] - a.R.seq_off[r]);
+  key = (uint32_t)MAX_READ_LEN - min(m, (uint32_t)MAX_READ_LEN);  // 0 = longest
+  return true;
+}
+
+__global__ __launch_bounds__(SORT_BLOCK) void k_job_hist(DpArgs a, uint32_t* hist) {
+  __shared__ uint32_t h[SORT_KEYS];
+  for (uint32_t i = threadIdx.x; i < SORT_KEYS; i += SORT_BLOCK) h[i] = 0;
+  __syncthreads();
+  uint32_t jid, key;
+  if (sort_job(a, blockIdx.x * SORT_BLOCK + threadIdx.x, jid, key)) atomicAdd(&h[key], 1u);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < SORT_KEYS; i += SORT_BLOCK)
+    if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+// Exclusive scan of the key counts in place; the job count to total[0].
+__global__ __launch_bounds__(SORT_BLOCK) void k_job_scan(uint32_t* hist, uint32_t* total) {
+  __shared__ uint32_t sh[SORT_BLOCK / 64];
+  uint32_t carry = 0;
+  for (uint32_t k0 = 0; k0 < SORT_KEYS; k0 += SORT_BLOCK) {
+    const uint32_t k = k0 + threadIdx.x;
+    uint32_t tot;
+    const uint32_t ex = team_excl_scan<SORT_BLOCK>(k < SORT_KEYS ? hist[k] : 0u, tot, sh);
+    if (k < SORT_KEYS) hist[k] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) total[0] = carry;
+}
+
+// Each block reserves one range per key with one global atomic, then places its
+// jobs inside it by their LDS rank.
+__global__ __launch_bounds__(SORT_BLOCK) void k_job_scatter(DpArgs a, uint32_t* off, uint32_t* out) {
+  __shared__ uint32_t h[SORT_KEYS];
+  for (uint32_t i = threadIdx.x; i < SORT_KEYS; i += SORT_BLOCK) h[i] = 0;
+  __syncthreads();
+  uint32_t jid = 0, key = 0, rank = 0;
+  const bool has = sort_job(a, blockIdx.x * SORT_BLOCK + threadIdx.x, jid, key);
+  if (has) rank = atomicAdd(&h[key], 1u);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < SORT_KEYS; i += SORT_BLOCK)
+    if (h[i]) h[i] = atomicAdd(&off[i], h[i]);
+  __syncthreads();
+  if (has) out[h[key] + rank] = jid;
+}
+
+void launch_job_sort(const DpArgs& a, uint32_t* hist, uint32_t* out, uint32_t* total, hipStream_t s) {
+  const uint32_t blocks = (a.njobs + SORT_BLOCK - 1) / SORT_BLOCK;
+  hipLaunchKernelGGL(k_job_zero, dim3((SORT_KEYS + 255) / 256), dim3(256), 0, s, hist, SORT_KEYS);
+  if (blocks) hipLaunchKernelGGL(k_job_hist, dim3(blocks), dim3(SORT_BLOCK), 0, s, a, hist);
+  hipLaunchKernelGGL(k_job_scan, dim3(1), dim3(SORT_BLOCK), 0, s, hist, total);
+  if (blocks) hipLaunchKernelGGL(k_job_scatter, dim3(blocks), dim3(SORT_BLOCK), 0, s, a, hist, out);
+}
+
+uint32_t quad_blocks(uint32_t njobs) {
+  const uint32_t q = (njobs + 3) / 4;
+  return q < QUAD_MAX_WAVES ? q : QUAD_MAX_WAVES;
+}
+size_t quad_scratch_words(uint32_t max_m, uint32_t max_n, uint32_t njobs) {
+  return (size_t)quad_blocks(njobs) * 4 * quad_layout(max_m, max_n).tb_words;
+}
 bool launch_dp_quad(const DpArgs& a, hipStream_t s) {
   if (a.njobs == 0) return true;
   const QuadLayout Y = quad_layout(a.max_m, a.max_n);
   const size_t lds = 4 * (size_t)Y.stride;
-  if (a.max_m > (uint32_t)QUAD_MAX_M || lds > 64 * 1024) return false;
+  if (a.max_m > (uint32_t)QUAD_MAX_M || lds > 64 * 1024 || !a.tbq) return false;
   // genome-word staging reuses the score-word area
   const size_t need = 8 * ((size_t)((a.max_n + 62) >> 5) + ((a.max_n + 126) >> 6));
   if (need > Y.q - Y.scw) return false;
-  uint32_t blocks = (a.njobs + 3) / 4;
-  if (blocks > 65536 * 4) blocks = 65536 * 4;
-  hipLaunchKernelGGL(k_dp_quad, dim3(blocks), dim3(64), lds, s, a);
+  hipLaunchKernelGGL(k_dp_quad, dim3(quad_blocks(a.njobs)), dim3(64), lds, s, a);
   return true;
 }
 
