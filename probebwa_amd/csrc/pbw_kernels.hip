@@ -629,13 +629,15 @@ __device__ __forceinline__ int64_t hit_diag(uint64_t k) {
 // Prefilter features of one read (only for the FEAT instances of the cluster kernels).
 struct ClusterFeat {
   const uint8_t* hflag;  // this read's hit flags, emission order
+  uint8_t* hf;           // working copy of hflag (LDS in the wave tier; hflag itself in the block tier)
   const uint8_t* qual;
   uint32_t qlen;
   const uint8_t* rcode;  // this read's window repeat codes
   uint16_t* out;         // this read's 20 feature rows of 32 BF16
-  uint32_t* f1;          // per cluster: neighbour hits | min bucket class << 24
+  uint32_t* f1;          // per cluster: neighbour hits
   uint32_t* f2;          // per cluster: sum of bucket classes
   uint32_t* f3;          // per cluster: diagonal span
+  uint32_t* f4;          // per cluster: minimum bucket class
   uint32_t* cnt;         // team counters: clusters tied at the top count, quality sum, q < 10, q < 20, repeat windows
 };
 
@@ -649,6 +651,9 @@ __device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const
     M.key[i] = i < n ? hits[i] : ~0ull;
     M.idx[i] = i;
     if (i < n) M.cl_sum[i] = 0ull;
+    if constexpr (FEAT) {
+      if (i < n && F.hf != F.hflag) F.hf[i] = F.hflag[i];
+    }
   }
   team_sync<T>();
   // group by (contig, strand), sort each group by diagonal (:36-49): one sort by (group, diagonal)
@@ -672,6 +677,13 @@ __device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const
   const uint32_t cbase = team_excl_scan<T>(nc, C, sh);
   if (G > M.gcap) return false;
   for (uint32_t g = tid; g < G; g += T) M.g_min[g] = 0xFFFFFFFFu;
+  if constexpr (FEAT) {
+    for (uint32_t c = tid; c < C; c += T) {
+      F.f1[c] = 0;
+      F.f2[c] = 0;
+      F.f4[c] = 15;
+    }
+  }
   if (tid == 0) M.cl_start[C] = n;
   team_sync<T>();
   if (c1 > c0) {
@@ -679,6 +691,14 @@ __device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const
     uint64_t kp = c0 > 0 ? M.key[c0 - 1] : 0;
     int64_t psum = 0;
     uint32_t pmin = 0xFFFFFFFFu;
+    uint32_t fnb = 0, fms = 0, fmn = 15;  // feature sums of the current run of a cluster
+    auto flush_feat = [&](int32_t c) {
+      if constexpr (FEAT) {
+        if (fnb) atomicAdd(&F.f1[c], fnb);
+        atomicAdd(&F.f2[c], fms);
+        atomicMin(&F.f4[c], fmn);
+      }
+    };
     for (uint32_t e = c0; e < c1; e++) {
       const uint64_t k = M.key[e];
       const bool gs = e == 0 || (k >> DIAG_BITS) != (kp >> DIAG_BITS);
@@ -690,33 +710,37 @@ __device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const
         M.g_key[gid] = (uint32_t)(k >> DIAG_BITS);
       }
       if (cs) {
-        if (cid >= 0 && e > c0) atomicAdd(&M.cl_sum[cid], (unsigned long long)psum);
+        if (cid >= 0 && e > c0) {
+          atomicAdd(&M.cl_sum[cid], (unsigned long long)psum);
+          flush_feat(cid);
+        }
         psum = 0;
+        fnb = fms = 0;
+        fmn = 15;
         cid++;
         M.cl_start[cid] = e;
         M.cl_gid[cid] = (uint32_t)gid;
       }
       psum += hit_diag(k);
-      pmin = min(pmin, M.idx[e]);
+      const uint32_t ei = M.idx[e];
+      pmin = min(pmin, ei);
+      if constexpr (FEAT) {
+        const uint32_t f = F.hf[ei];
+        fnb += f >> 7;
+        fms += f & 15u;
+        fmn = min(fmn, f & 15u);
+      }
       kp = k;
     }
     atomicAdd(&M.cl_sum[cid], (unsigned long long)psum);
+    flush_feat(cid);
     atomicMin(&M.g_min[gid], pmin);
   }
   team_sync<T>();
   if constexpr (FEAT) {
-    // per-cluster aggregates over its sorted hits, before the keys are overwritten
+    // diagonal span of every cluster, before the keys are overwritten
     for (uint32_t c = tid; c < C; c += T) {
       const uint32_t s0 = M.cl_start[c], e0 = M.cl_start[c + 1];
-      uint32_t nb = 0, ms = 0, mn = 15;
-      for (uint32_t x = s0; x < e0; x++) {
-        const uint32_t f = F.hflag[M.idx[x]];
-        nb += f >> 7;
-        ms += f & 15u;
-        mn = min(mn, f & 15u);
-      }
-      F.f1[c] = nb | (mn << 24);
-      F.f2[c] = ms;
       F.f3[c] = (uint32_t)min((int64_t)0xFFFFFFF, hit_diag(M.key[e0 - 1]) - hit_diag(M.key[s0]));
     }
     if (tid < 5) F.cnt[tid] = 0;
@@ -788,7 +812,7 @@ __device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const
       const float cnt = (float)(M.cl_start[c + 1] - M.cl_start[c]);
       const float c0 = (float)(M.cl_start[M.idx[0] + 1] - M.cl_start[M.idx[0]]);
       const float c1 = C > 1 ? (float)(M.cl_start[M.idx[1] + 1] - M.cl_start[M.idx[1]]) : 0.f;
-      const float nb = (float)(F.f1[c] & 0xFFFFFFu), mn = (float)(F.f1[c] >> 24);
+      const float nb = (float)F.f1[c], mn = (float)F.f4[c];
       const float ms = (float)F.f2[c], span = (float)F.f3[c];
       const float nwin = (float)(L >= (uint32_t)K ? L - K + 1 : 1);
       const float ql = (float)max(F.qlen, 1u);
@@ -859,7 +883,7 @@ struct ClusterLds {
 template <bool FEAT>
 __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
   __shared__ ClusterLds S;
-  __shared__ uint32_t FX[FEAT ? 3 * CL_NCAP + 8 : 1];
+  __shared__ uint32_t FX[FEAT ? 4 * CL_NCAP + 8 + CL_NCAP / 4 : 1];
   const ClusterMem M{S.key, S.idx, S.cl_sum, S.cl_start, S.cl_gid, S.g_key, S.g_min, S.g_rank, S.g_sort, S.table,
                      S.tmp, CL_GCAP};
   for (uint32_t r = blockIdx.x; r < a.R.n; r += gridDim.x) {
@@ -882,7 +906,9 @@ __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
         F.f1 = FX;
         F.f2 = FX + CL_NCAP;
         F.f3 = FX + 2 * CL_NCAP;
-        F.cnt = FX + 3 * CL_NCAP;
+        F.f4 = FX + 3 * CL_NCAP;
+        F.cnt = FX + 4 * CL_NCAP;
+        F.hf = (uint8_t*)(FX + 4 * CL_NCAP + 8);
       }
       done = cluster_read<64, FEAT>(a.hits + a.hoff[r], n, L, M, a.cands + (uint64_t)r * TOP_CANDIDATES,
                                     a.ncand + r, nullptr, F);
@@ -926,6 +952,7 @@ __global__ __launch_bounds__(256) void k_cluster_tier2(ClusterArgs a, const uint
       const uint64_t so = a.R.seq_off[r], qo = a.R.qual_off[r];
       uint32_t* fx = (uint32_t*)(M.table + 4 * p2);
       F.hflag = a.hflag + a.hoff[r];
+      F.hf = const_cast<uint8_t*>(F.hflag);  // read in place
       F.qual = a.R.qual + qo;
       F.qlen = (uint32_t)(a.R.qual_off[r + 1] - qo);
       F.rcode = a.rcode + so;
@@ -933,7 +960,8 @@ __global__ __launch_bounds__(256) void k_cluster_tier2(ClusterArgs a, const uint
       F.f1 = fx;
       F.f2 = fx + p2;
       F.f3 = fx + 2 * p2;
-      F.cnt = fx + 3 * p2;
+      F.f4 = fx + 3 * p2;
+      F.cnt = fx + 4 * p2;
     }
     cluster_read<256, FEAT>(a.hits + a.hoff[r], n, L, M, a.cands + (uint64_t)r * TOP_CANDIDATES, a.ncand + r, sh,
                             F);
@@ -950,7 +978,7 @@ uint64_t cluster_scratch_words(uint32_t max_hits) {
   uint64_t p2 = 1;
   while (p2 < max_hits) p2 <<= 1;
   // key + cl_sum: 2 p2 words; idx, cl_start (+1), cl_gid, g_key, g_min, g_rank, g_sort, tmp
-  // (p2 u32 each), table (4 p2 u32) and the feature arrays (3 p2 + 8 u32): 15 p2 + 9 u32
+  // (p2 u32 each), table (4 p2 u32) and the feature arrays (4 p2 + 8 u32): 16 p2 + 9 u32
   return 2 * p2 + 8 * p2 + 16;
 }
 void launch_cluster_tier2(const ClusterArgs& a, const uint32_t* list, uint32_t n, uint32_t blocks, hipStream_t s) {
@@ -1407,7 +1435,9 @@ __device__ uint64_t quad_cells(int64_t m, int64_t h, int64_t b, int64_t n) {
   return (uint64_t)(hi - lo + m);
 }
 
-__global__ __launch_bounds__(64) void k_dp_quad(DpArgs a) {
+// At most 168 VGPRs: three waves per SIMD (the latency-bound phases of one wave
+// overlap the row work of the others).
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_dp_quad(DpArgs a) {
   extern __shared__ __align__(16) uint8_t dyn[];
   const int lane = lane_id();
   const int g = lane >> 4, L = lane & 15;
