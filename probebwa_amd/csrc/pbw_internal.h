@@ -182,7 +182,7 @@ struct DpArgs {
 void launch_dp_band(const DpArgs& a, hipStream_t s);
 void launch_dp_tiled(const DpArgs& a, uint32_t nwaves, hipStream_t s);
 bool launch_dp_quad(const DpArgs& a, hipStream_t s);  // false: layout too large, nothing launched
-constexpr uint32_t QUAD_MAX_WAVES = 8192;  // persistent grid of k_dp_quad
+constexpr uint32_t QUAD_MAX_WAVES = 32768;  // grid cap of k_dp_quad (more blocks than resident waves: dynamic balance)
 size_t quad_scratch_words(uint32_t max_m, uint32_t max_n, uint32_t njobs);
 // Jobs of `a` (jlist / njobs / job_valid) sorted longest read first into `out`, count
 // to total[0]; `hist` holds MAX_READ_LEN + 1 words of scratch.

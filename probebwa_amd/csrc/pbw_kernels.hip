@@ -1398,7 +1398,7 @@ __host__ __device__ inline QuadLayout quad_layout(uint32_t max_m, uint32_t max_n
   l.q = l.scw + 4 * ((max_m > 128 ? max_m : 128) + 4);
   l.qq = l.q + ((max_m + 15) & ~15u);
   l.ref = l.qq + ((max_m + 15) & ~15u);
-  l.stride = l.ref + ((max_n + 15) & ~15u);
+  l.stride = (l.ref + ((max_n + 15) & ~15u) + 15) & ~15u;  // 8-byte keys and terms in every group's area
   return l;
 }
 
@@ -1875,10 +1875,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     off = (uint32_t)__shfl((int)off, lane & ~15, 64);
     const uint32_t* runs = runs_top - nruns;
     // ---- alignment log-likelihood terms (single.rs:228-262), one per read
-    //      position, in parallel into the consumed bottom of the traceback area:
-    //      M columns by base match, I columns ln(1/4).  The table reads of a
-    //      lane are issued together. ----
-    double* terms = (double*)TB;
+    //      position, in parallel into the LDS of the per-row inputs (dead after
+    //      the rows; it holds 8 max_m bytes): M columns by base match, I columns
+    //      ln(1/4).  The table reads of a lane are issued together. ----
+    double* terms = (double*)(jb + Y.go);
     {
       uint32_t u = 0, qs = 0, rs = 0;
       for (uint32_t qi = L; qi < m; qi += 16) {
@@ -1900,7 +1900,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
         terms[qi] = tv;
       }
     }
-    mem_sync();  // the terms of all 16 lanes
+    wave_sync();  // the terms of all 16 lanes
     if (L == 0 && m) {
       double lk = 0.0;
       uint32_t qi = 0;
