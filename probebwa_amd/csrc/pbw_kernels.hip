@@ -2281,6 +2281,51 @@ struct QueryView {  // query bases/quals in the aligned orientation
   __device__ uint8_t q(uint32_t x) const { return rc ? qual[qlen - 1 - x] : qual[x]; }
 };
 
+// Sequential reads of the 2-bit genome: one word load per 32 bases.
+struct GenomeCursor {
+  const DevGenome& G;
+  uint64_t w = ~0ull, g = 0;
+  uint32_t n = 0;  // N bits of the 32 bases of word w
+  __device__ explicit GenomeCursor(const DevGenome& g_) : G(g_) {}
+  __device__ __forceinline__ uint8_t at(uint64_t p) {  // same value as genome_base(G, p)
+    const uint64_t ww = p >> 5;
+    if (ww != w) {
+      w = ww;
+      g = G.g2b[ww];
+      n = (uint32_t)(G.nmask[ww >> 1] >> ((ww & 1) * 32));
+    }
+    const uint32_t i = (uint32_t)(p & 31);
+    return ((n >> i) & 1u) ? (uint8_t)'N' : (uint8_t)("ACGT"[(g >> (2 * i)) & 3u]);
+  }
+};
+
+// gc_content (bayesian.rs:445-456) of genome[gb, gb+span): G/C bases and
+// non-N bases, counted 32 at a time (C = 01 and G = 10 are the codes whose
+// two bits differ).
+__device__ void gc_count(const DevGenome& G, uint64_t gb, uint32_t span, uint32_t& gc, uint32_t& total) {
+  gc = total = 0;
+  const uint64_t end = gb + span;
+  for (uint64_t p = gb; p < end;) {
+    const uint64_t w = p >> 5;
+    const uint32_t b0 = (uint32_t)(p & 31);
+    const uint32_t b1 = (uint32_t)min<uint64_t>(32, end - (w << 5));
+    const uint32_t cnt = b1 - b0;
+    const uint32_t sel = (cnt == 32 ? 0xFFFFFFFFu : ((1u << cnt) - 1u)) << b0;
+    const uint64_t g = G.g2b[w];
+    uint64_t x = (g ^ (g >> 1)) & 0x5555555555555555ull;  // bit 2i: the two bits of base i differ
+    x = (x | (x >> 1)) & 0x3333333333333333ull;
+    x = (x | (x >> 2)) & 0x0F0F0F0F0F0F0F0Full;
+    x = (x | (x >> 4)) & 0x00FF00FF00FF00FFull;
+    x = (x | (x >> 8)) & 0x0000FFFF0000FFFFull;
+    x = (x | (x >> 16)) & 0x00000000FFFFFFFFull;
+    const uint32_t nb = (uint32_t)(G.nmask[w >> 1] >> ((w & 1) * 32));
+    const uint32_t ok = sel & ~nb;
+    gc += (uint32_t)__popc((uint32_t)x & ok);
+    total += (uint32_t)__popc(ok);
+    p = (w << 5) + b1;
+  }
+}
+
 // bayesian.rs:306-373 log_entropy; ref_window = genome[gb, gb+span)
 __device__ double log_entropy(const DevGenome& G, const DevLuts& T, const uint32_t* cig, uint32_t ncig,
                               const QueryView& Q, uint64_t gb, uint32_t span) {
@@ -2288,6 +2333,7 @@ __device__ double log_entropy(const DevGenome& G, const DevLuts& T, const uint32
   uint32_t readlen = 0, refpos = 0;
   int64_t indel_runs = 0, deletion_runs = 0;
   uint32_t curtype = 0;
+  GenomeCursor gcur(G);
   for (uint32_t c = 0; c < ncig; c++) {
     uint32_t op = cig[c] & 15u, len = cig[c] >> 4;
     if (op == 1) {
@@ -2313,7 +2359,7 @@ __device__ double log_entropy(const DevGenome& G, const DevLuts& T, const uint32
         uint32_t q = readlen < Q.qlen ? Q.q(readlen) : 0u;
         bool hq = q >= 10u;
         if (hq) n++;
-        uint8_t aa = Q.base(readlen), rr = genome_base(G, gb + refpos);
+        uint8_t aa = Q.base(readlen), rr = gcur.at(gb + refpos);
         if (aa != rr && aa != 'N' && rr != 'N' && hq) mm++;
         readlen++;
         refpos++;
@@ -2321,17 +2367,8 @@ __device__ double log_entropy(const DevGenome& G, const DevLuts& T, const uint32
     }
   }
   int64_t insertion_runs = indel_runs - deletion_runs;
-  // gc_content (bayesian.rs:445-456) of the ref window
-  uint32_t gc = 0, total = 0;
-  for (uint32_t x = 0; x < span; x++) {
-    uint8_t bb = genome_base(G, gb + x);
-    if (bb == 'G' || bb == 'C') {
-      gc++;
-      total++;
-    } else if (bb == 'A' || bb == 'T') {
-      total++;
-    }
-  }
+  uint32_t gc, total;
+  gc_count(G, gb, span, gc, total);
   double lnb, lnbm1;
   if (total <= (uint32_t)GC_LUT_MAX) {
     uint64_t idx = (uint64_t)total * (total + 1) / 2 + gc;
@@ -2356,28 +2393,51 @@ __device__ double log_entropy(const DevGenome& G, const DevLuts& T, const uint32
 // pass (0: 0.0, 1: 1.0, 2: 10^(-q/10) at the window start, seeds.rs:125-139)
 __device__ double missing_locus(const DevLuts& T, const uint8_t* qual, uint32_t qlen, const uint8_t* rcode,
                                 uint32_t nwin) {
-  const uint32_t k = K;
+  constexpr uint32_t k = K, PH = 5;
   if (qlen < k) return 0.5;
-  double offset_means = 0.0;
-  for (uint32_t phase = 0; phase < 5; phase++) {
-    double log_p = 0.0;
-    for (uint32_t start = phase; start + k <= qlen; start += k) {
-      double rep = 0.0;
-      if (start < nwin) {
-        uint8_t c = rcode[start];
-        if (c == 1) rep = 1.0;
-        else if (c == 2) rep = T.nrp[start < qlen ? qual[start] : 30];
-      }
-      double z = 1.0;
-      for (uint32_t t = start; t < start + k; t++) z = z * T.omq[qual[t]];
-      double p1 = -0.0;
-      for (uint32_t t = start; t < start + k; t++) p1 = p1 + T.qratio[qual[t]];
-      double p2 = fmin(fmax(1.0 - z * (1.0 + p1), 0.0), 1.0);
-      double pm = fmin(fmax(1.0 - (1.0 - p2) * (1.0 - rep), 1e-300), 1.0);
-      log_p += log(pm);
-    }
-    offset_means += exp(log_p);
+  // The reference loops phase { window { position } }.  One pass over the
+  // positions feeds the five phases together: each window's product and sum
+  // still run over its positions in order, and each phase still adds its
+  // windows' log terms in window order, so every double is the same.
+  double z[PH], p1[PH], log_p[PH];
+  uint32_t rel[PH];  // position inside the phase's current window
+#pragma unroll
+  for (uint32_t ph = 0; ph < PH; ph++) {
+    z[ph] = 1.0;
+    p1[ph] = -0.0;
+    log_p[ph] = 0.0;
+    rel[ph] = 0;
   }
+  for (uint32_t t = 0; t < qlen; t++) {
+    const uint32_t q = qual[t];
+    const double om = T.omq[q], qr = T.qratio[q];
+#pragma unroll
+    for (uint32_t ph = 0; ph < PH; ph++) {
+      if (t < ph) continue;
+      const uint32_t start = t - rel[ph];
+      if (start + k <= qlen) {  // windows past the end do not exist in the reference
+        z[ph] = z[ph] * om;
+        p1[ph] = p1[ph] + qr;
+        if (rel[ph] == k - 1) {
+          double rep = 0.0;
+          if (start < nwin) {
+            const uint8_t c = rcode[start];
+            if (c == 1) rep = 1.0;
+            else if (c == 2) rep = T.nrp[start < qlen ? qual[start] : 30];
+          }
+          const double p2 = fmin(fmax(1.0 - z[ph] * (1.0 + p1[ph]), 0.0), 1.0);
+          const double pm = fmin(fmax(1.0 - (1.0 - p2) * (1.0 - rep), 1e-300), 1.0);
+          log_p[ph] += log(pm);
+          z[ph] = 1.0;
+          p1[ph] = -0.0;
+        }
+      }
+      rel[ph] = rel[ph] == k - 1 ? 0u : rel[ph] + 1u;
+    }
+  }
+  double offset_means = 0.0;
+#pragma unroll
+  for (uint32_t ph = 0; ph < PH; ph++) offset_means += exp(log_p[ph]);
   return fmin(offset_means / 5.0, 0.99);
 }
 
