@@ -265,7 +265,7 @@ struct pbw_ctx {
   // batch workspace
   DevBuf<uint8_t> d_seq, d_qual, d_rcode;
   DevBuf<uint64_t> d_seq_off, d_qual_off, d_hoff, d_hits, d_hits_ov, d_scratch;
-  DevBuf<uint32_t> d_nhits, d_list, d_hcap, d_tier2, d_ntier2, d_best, d_cigar, d_cigar_out, d_cigar_off;
+  DevBuf<uint32_t> d_nhits, d_list, d_hcap, d_tier2, d_mid, d_ntier2, d_best, d_cigar, d_cigar_out, d_cigar_off;
   DevBuf<Cand> d_cands;
   DevBuf<uint8_t> d_ncand, d_njobs, d_rec, d_refbytes;
   DevBuf<uint32_t> d_flag_a, d_flag_b, d_flag_c, d_flag_n, d_sorted, d_sort_hist, d_tbq;
@@ -699,8 +699,9 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false) {
   c->d_cands.ensure((size_t)B * TOP_CANDIDATES);
   c->d_ncand.ensure(B);
   c->d_tier2.ensure(B);
-  c->d_ntier2.ensure(1);
-  HIP_CHECK(hipMemsetAsync(c->d_ntier2.p, 0, 4, s));
+  c->d_mid.ensure(B);
+  c->d_ntier2.ensure(2);  // [0] block tier, [1] second wave tier
+  HIP_CHECK(hipMemsetAsync(c->d_ntier2.p, 0, 8, s));
   ClusterArgs ca{};
   ca.R = R;
   ca.hits = hits_base;
@@ -710,6 +711,8 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false) {
   ca.ncand = c->d_ncand.p;
   ca.tier2 = c->d_tier2.p;
   ca.ntier2 = c->d_ntier2.p;
+  ca.mid = c->d_mid.p;
+  ca.nmid = c->d_ntier2.p + 1;
   if (feat) {
     c->d_feat.ensure((size_t)B * TOP_CANDIDATES * MLP_IN);
     ca.hflag = hits_base == c->d_hits_ov.p ? c->d_hflag_ov.p : c->d_hflag.p;

@@ -129,6 +129,8 @@ struct ClusterArgs {
   const uint32_t* nhits;
   Cand* cands;             // read * 20
   uint8_t* ncand;
+  uint32_t* mid;           // reads past the small wave tier (count at nmid, zero before launch)
+  uint32_t* nmid;
   uint32_t* tier2;         // list of reads needing the block-level path
   uint32_t* ntier2;
   uint64_t* scratch;       // tier 2: per-block scratch (keys)

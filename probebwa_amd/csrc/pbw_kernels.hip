@@ -601,6 +601,74 @@ __device__ void team_bitonic(uint64_t* key, uint32_t* val, uint32_t p2, uint32_t
   }
 }
 
+// The same sorting network over keys in global memory, staged through LDS
+// chunks of CH keys: every step whose partner distance is below the chunk runs
+// on a chunk in LDS, so only the few steps with distance >= CH touch global
+// memory (about 3 of the 78 steps at 4096 keys).
+template <int T, uint32_t CH>
+__device__ void team_bitonic_staged(uint64_t* key, uint32_t* val, uint32_t p2, uint32_t tid, uint64_t* sk,
+                                    uint32_t* sv) {
+  const uint32_t ch = p2 < CH ? p2 : CH;
+  auto chunk_pass = [&](uint32_t k_lo, uint32_t k_hi) {  // stages k_lo..k_hi, steps below ch
+    for (uint32_t c0 = 0; c0 < p2; c0 += ch) {
+      for (uint32_t i = tid; i < ch; i += T) {
+        sk[i] = key[c0 + i];
+        if (val) sv[i] = val[c0 + i];
+      }
+      team_sync<T>();
+      for (uint32_t k = k_lo; k <= k_hi; k <<= 1) {
+        for (uint32_t j = min(k >> 1, ch >> 1); j > 0; j >>= 1) {
+          for (uint32_t i = tid; i < ch; i += T) {
+            const uint32_t ixj = i ^ j;
+            if (ixj > i) {
+              const bool up = ((c0 + i) & k) == 0;
+              const uint64_t x = sk[i], y = sk[ixj];
+              if ((x > y) == up) {
+                sk[i] = y;
+                sk[ixj] = x;
+                if (val) {
+                  const uint32_t t = sv[i];
+                  sv[i] = sv[ixj];
+                  sv[ixj] = t;
+                }
+              }
+            }
+          }
+          team_sync<T>();
+        }
+      }
+      for (uint32_t i = tid; i < ch; i += T) {
+        key[c0 + i] = sk[i];
+        if (val) val[c0 + i] = sv[i];
+      }
+      team_sync<T>();
+    }
+  };
+  chunk_pass(2, ch);
+  for (uint32_t k = ch << 1; k <= p2; k <<= 1) {
+    for (uint32_t j = k >> 1; j >= ch; j >>= 1) {
+      for (uint32_t i = tid; i < p2; i += T) {
+        const uint32_t ixj = i ^ j;
+        if (ixj > i) {
+          const bool up = (i & k) == 0;
+          const uint64_t x = key[i], y = key[ixj];
+          if ((x > y) == up) {
+            key[i] = y;
+            key[ixj] = x;
+            if (val) {
+              const uint32_t t = val[i];
+              val[i] = val[ixj];
+              val[ixj] = t;
+            }
+          }
+        }
+      }
+      team_sync<T>();
+    }
+    chunk_pass(k, k);
+  }
+}
+
 // Working arrays of one read's clustering (LDS for the wave tier, global
 // scratch for the block tier).  Sizes: key/idx p2, cl_* n (+1), g_* gcap,
 // table tcap, tmp gcap.
@@ -617,7 +685,17 @@ struct ClusterMem {
   int32_t* table;      // general-path hash table
   int32_t* tmp;
   uint32_t gcap;       // capacity of the g_* arrays and tmp
+  uint64_t* stage_key;  // block tier: LDS chunks for the sorts (CLUSTER_STAGE keys)
+  uint32_t* stage_val;
 };
+constexpr uint32_t CLUSTER_STAGE = 1024;
+
+template <int T>
+__device__ __forceinline__ void cluster_sort(const ClusterMem& M, uint64_t* key, uint32_t* val, uint32_t p2,
+                                             uint32_t tid) {
+  if constexpr (T == 64) team_bitonic<T>(key, val, p2, tid);
+  else team_bitonic_staged<T, CLUSTER_STAGE>(key, val, p2, tid, M.stage_key, M.stage_val);
+}
 
 __device__ __forceinline__ int64_t hit_diag(uint64_t k) {
   return (int64_t)(k & ((1ull << DIAG_BITS) - 1)) - DIAG_BIAS;
@@ -657,7 +735,7 @@ __device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const
   }
   team_sync<T>();
   // group by (contig, strand), sort each group by diagonal (:36-49): one sort by (group, diagonal)
-  team_bitonic<T>(M.key, M.idx, p2, tid);
+  cluster_sort<T>(M, M.key, M.idx, p2, tid);
   // group and cluster starts (sliding merge, :51-71), one contiguous chunk per thread
   const uint32_t chunk = (n + T - 1) / T;
   const uint32_t c0 = min(n, tid * chunk), c1 = min(n, c0 + chunk);
@@ -774,7 +852,7 @@ __device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const
     while (g2 < G) g2 <<= 1;
     for (uint32_t g = tid; g < g2; g += T) M.key[g] = g < G ? (((uint64_t)M.g_min[g] << 32) | g) : ~0ull;
     team_sync<T>();
-    team_bitonic<T>(M.key, nullptr, g2, tid);
+    cluster_sort<T>(M, M.key, nullptr, g2, tid);
     for (uint32_t g = tid; g < G; g += T) M.g_sort[g] = (uint32_t)M.key[g];
     team_sync<T>();
   }
@@ -800,7 +878,7 @@ __device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const
     M.idx[c] = c;
   }
   team_sync<T>();
-  team_bitonic<T>(M.key, M.idx, pc, tid);
+  cluster_sort<T>(M, M.key, M.idx, pc, tid);
   const uint32_t nt = min(C, (uint32_t)TOP_CANDIDATES);
   if constexpr (FEAT) {
     const uint64_t top = M.key[0] >> 42;
@@ -870,30 +948,40 @@ __device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const
 // Wave tier: one wave per read, all arrays in LDS.
 constexpr uint32_t CL_NCAP = CLUSTER_LDS_CAP;  // hits
 constexpr uint32_t CL_GCAP = 128;              // groups
+constexpr uint32_t CL_SMALL = 128;             // hits of the small wave tier (most reads)
+template <uint32_t NC>
 struct ClusterLds {
-  uint64_t key[CL_NCAP];
-  unsigned long long cl_sum[CL_NCAP];
-  uint32_t idx[CL_NCAP];
-  uint32_t cl_start[CL_NCAP + 1];
-  uint32_t cl_gid[CL_NCAP];
+  uint64_t key[NC];
+  unsigned long long cl_sum[NC];
+  uint32_t idx[NC];
+  uint32_t cl_start[NC + 1];
+  uint32_t cl_gid[NC];
   uint32_t g_key[CL_GCAP], g_min[CL_GCAP], g_rank[CL_GCAP], g_sort[CL_GCAP];
   int32_t table[2 * CL_GCAP], tmp[CL_GCAP];
 };
 
-template <bool FEAT>
-__global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
-  __shared__ ClusterLds S;
-  __shared__ uint32_t FX[FEAT ? 4 * CL_NCAP + 8 + CL_NCAP / 4 : 1];
+// Wave tier: one wave per read, every array in LDS for up to NC hits.  The
+// reads of `list` (all reads when null; count at list_n[0]) that hold more
+// hits, or more groups than CL_GCAP, go to (defer, defer_n) for the next tier.
+// Two sizes run in turn: NC = 128 takes most reads at a small LDS footprint
+// (more waves per CU), NC = 512 the rest.
+template <bool FEAT, uint32_t NC>
+__global__ __launch_bounds__(64) void k_cluster(ClusterArgs a, const uint32_t* list, const uint32_t* list_n,
+                                                uint32_t* defer, uint32_t* defer_n) {
+  __shared__ ClusterLds<NC> S;
+  __shared__ uint32_t FX[FEAT ? 4 * NC + 8 + NC / 4 : 1];
   const ClusterMem M{S.key, S.idx, S.cl_sum, S.cl_start, S.cl_gid, S.g_key, S.g_min, S.g_rank, S.g_sort, S.table,
-                     S.tmp, CL_GCAP};
-  for (uint32_t r = blockIdx.x; r < a.R.n; r += gridDim.x) {
+                     S.tmp, CL_GCAP, nullptr, nullptr};
+  const uint32_t total = list ? list_n[0] : a.R.n;
+  for (uint32_t li = blockIdx.x; li < total; li += gridDim.x) {
+    const uint32_t r = list ? list[li] : li;
     const uint32_t n = a.nhits[r];
     if (n == 0) {
       if (lane_id() == 0) a.ncand[r] = 0;
       continue;
     }
     bool done = false;
-    if (n <= CL_NCAP) {
+    if (n <= NC) {
       const uint32_t L = (uint32_t)(a.R.seq_off[r + 1] - a.R.seq_off[r]);
       ClusterFeat F{};
       if (FEAT) {
@@ -904,18 +992,18 @@ __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
         F.rcode = a.rcode + so;
         F.out = a.feat + (uint64_t)r * TOP_CANDIDATES * MLP_IN;
         F.f1 = FX;
-        F.f2 = FX + CL_NCAP;
-        F.f3 = FX + 2 * CL_NCAP;
-        F.f4 = FX + 3 * CL_NCAP;
-        F.cnt = FX + 4 * CL_NCAP;
-        F.hf = (uint8_t*)(FX + 4 * CL_NCAP + 8);
+        F.f2 = FX + NC;
+        F.f3 = FX + 2 * NC;
+        F.f4 = FX + 3 * NC;
+        F.cnt = FX + 4 * NC;
+        F.hf = (uint8_t*)(FX + 4 * NC + 8);
       }
       done = cluster_read<64, FEAT>(a.hits + a.hoff[r], n, L, M, a.cands + (uint64_t)r * TOP_CANDIDATES,
                                     a.ncand + r, nullptr, F);
     }
     if (!done && lane_id() == 0) {
-      const uint32_t slot = atomicAdd(a.ntier2, 1u);
-      a.tier2[slot] = r;
+      const uint32_t slot = atomicAdd(defer_n, 1u);
+      defer[slot] = r;
     }
   }
 }
@@ -926,6 +1014,8 @@ __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
 template <bool FEAT>
 __global__ __launch_bounds__(256) void k_cluster_tier2(ClusterArgs a, const uint32_t* list, uint32_t nlist) {
   __shared__ uint32_t sh[4];
+  __shared__ uint64_t stage_key[CLUSTER_STAGE];
+  __shared__ uint32_t stage_val[CLUSTER_STAGE];
   for (uint32_t li = blockIdx.x; li < nlist; li += gridDim.x) {
     const uint32_t r = list[li];
     const uint32_t n = a.nhits[r];
@@ -947,6 +1037,8 @@ __global__ __launch_bounds__(256) void k_cluster_tier2(ClusterArgs a, const uint
     M.tmp = (int32_t*)(M.g_sort + p2);
     M.table = M.tmp + p2;  // 4 * p2 buckets
     M.gcap = p2;
+    M.stage_key = stage_key;
+    M.stage_val = stage_val;
     ClusterFeat F{};
     if (FEAT) {
       const uint64_t so = a.R.seq_off[r], qo = a.R.qual_off[r];
@@ -970,9 +1062,17 @@ __global__ __launch_bounds__(256) void k_cluster_tier2(ClusterArgs a, const uint
 
 void launch_cluster(const ClusterArgs& a, hipStream_t s) {
   if (a.R.n == 0) return;
-  uint32_t blocks = a.R.n < 65536 ? a.R.n : 65536;
-  if (a.feat) hipLaunchKernelGGL(k_cluster<true>, dim3(blocks), dim3(64), 0, s, a);
-  else hipLaunchKernelGGL(k_cluster<false>, dim3(blocks), dim3(64), 0, s, a);
+  const uint32_t blocks = a.R.n < 65536 ? a.R.n : 65536;
+  const uint32_t mid_blocks = a.R.n < 8192 ? a.R.n : 8192;  // the count is on the device
+  if (a.feat) {
+    hipLaunchKernelGGL((k_cluster<true, CL_SMALL>), dim3(blocks), dim3(64), 0, s, a, nullptr, nullptr, a.mid, a.nmid);
+    hipLaunchKernelGGL((k_cluster<true, CL_NCAP>), dim3(mid_blocks), dim3(64), 0, s, a, a.mid, a.nmid, a.tier2,
+                       a.ntier2);
+  } else {
+    hipLaunchKernelGGL((k_cluster<false, CL_SMALL>), dim3(blocks), dim3(64), 0, s, a, nullptr, nullptr, a.mid, a.nmid);
+    hipLaunchKernelGGL((k_cluster<false, CL_NCAP>), dim3(mid_blocks), dim3(64), 0, s, a, a.mid, a.nmid, a.tier2,
+                       a.ntier2);
+  }
 }
 uint64_t cluster_scratch_words(uint32_t max_hits) {
   uint64_t p2 = 1;
