@@ -615,7 +615,9 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false) {
   std::vector<uint64_t> hoff(B);
   for (uint32_t i = 0; i < B; i++) hoff[i] = (uint64_t)i * HIT_CAP;
   c->d_hoff.upload(hoff.data(), B, s);
-  c->d_hits.ensure((size_t)B * HIT_CAP);
+  // fixed HIT_CAP slots per read, then slack for the overflow pass's exact-size slots
+  const size_t base_cap = (size_t)B * HIT_CAP, slack = (size_t)B * 64;
+  c->d_hits.ensure(base_cap + slack);
   c->d_nhits.ensure(B);
   SeedArgs sa{};
   sa.G = c->dev_genome();
@@ -630,7 +632,7 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false) {
   sa.rcode = c->d_rcode.p;
   sa.ctr = c->d_ctr.p;
   if (feat) {
-    c->d_hflag.ensure((size_t)B * HIT_CAP);
+    c->d_hflag.ensure(base_cap + slack);
     sa.hflag = c->d_hflag.p;
   }
   HIP_CHECK(hipEventRecord(c->ev[0], s));
@@ -658,12 +660,21 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false) {
     }
   }
   const uint64_t* hits_base = c->d_hits.p;
+  const uint8_t* hflag_base = feat ? c->d_hflag.p : nullptr;
   if (!ov.empty()) {
-    c->d_hits_ov.ensure((size_t)B * HIT_CAP + ov_total);
-    HIP_CHECK(hipMemcpyAsync(c->d_hits_ov.p, c->d_hits.p, (size_t)B * HIT_CAP * 8, hipMemcpyDeviceToDevice, s));
-    if (feat) {
-      c->d_hflag_ov.ensure((size_t)B * HIT_CAP + ov_total);
-      HIP_CHECK(hipMemcpyAsync(c->d_hflag_ov.p, c->d_hflag.p, (size_t)B * HIT_CAP, hipMemcpyDeviceToDevice, s));
+    // overflow slots after the fixed ones: in place when the slack holds them,
+    // else in a second arena that starts with a copy of the fixed slots
+    uint64_t* arena = c->d_hits.p;
+    uint8_t* farena = feat ? c->d_hflag.p : nullptr;
+    if (ov_total > slack) {
+      c->d_hits_ov.ensure(base_cap + ov_total);
+      HIP_CHECK(hipMemcpyAsync(c->d_hits_ov.p, c->d_hits.p, base_cap * 8, hipMemcpyDeviceToDevice, s));
+      arena = c->d_hits_ov.p;
+      if (feat) {
+        c->d_hflag_ov.ensure(base_cap + ov_total);
+        HIP_CHECK(hipMemcpyAsync(c->d_hflag_ov.p, c->d_hflag.p, base_cap, hipMemcpyDeviceToDevice, s));
+        farena = c->d_hflag_ov.p;
+      }
     }
     std::vector<uint64_t> ovoff(ov.size());
     std::vector<uint32_t> ovcap(ov.size());
@@ -683,14 +694,15 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false) {
     sb.nlist = (uint32_t)ov.size();
     sb.hoff = dovoff.p;
     sb.hcap = c->d_hcap.p;
-    sb.hits = c->d_hits_ov.p;
+    sb.hits = arena;
     sb.ctr = nullptr;  // counted once, in the first pass
-    sb.hflag = feat ? c->d_hflag_ov.p : nullptr;
+    sb.hflag = farena;
     launch_seed(sb, s);
     HIP_CHECK(hipGetLastError());
     c->d_hoff.upload(hoff.data(), B, s);
     HIP_CHECK(hipStreamSynchronize(s));
-    hits_base = c->d_hits_ov.p;
+    hits_base = arena;
+    hflag_base = farena;
     c->stats.seed_overflow_reads += ov.size();
   }
   c->stats.ms_seed += ms_since(t0);
@@ -715,7 +727,7 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false) {
   ca.nmid = c->d_ntier2.p + 1;
   if (feat) {
     c->d_feat.ensure((size_t)B * TOP_CANDIDATES * MLP_IN);
-    ca.hflag = hits_base == c->d_hits_ov.p ? c->d_hflag_ov.p : c->d_hflag.p;
+    ca.hflag = hflag_base;
     ca.rcode = c->d_rcode.p;
     ca.feat = c->d_feat.p;
   }
