@@ -64,6 +64,63 @@ def measured_traffic(kernel, batch):
     return None
 
 
+def bench_pe(args):
+    """Paired-end joint posterior on the probe genome: 1M FR pairs of 2x150 bp (10% with one
+    mate inside a repeat copy), one pbw_map_pe call over all of them (500 serial warm-up
+    pairs, then 2000-pair batches on a frozen insert-size snapshot, as the reference)."""
+    import probebwa_amd as P
+    from probebwa_amd import synth
+    n = args.reads or 1_000_000
+    t0 = time.time()
+    contigs = synth.genome_probe_se_2mbp()
+    spans = synth.probe_repeat_spans()
+    g = P.Genome()
+    for name, s in contigs:
+        g.add_contig(name, s.tobytes())
+    m1, m2 = synth.reads_pairs_probe(contigs, n=n, seed=3, repeat_spans=spans)
+    r1, r2 = P.ReadBatch.from_arrays(*m1), P.ReadBatch.from_arrays(*m2)
+    log("pairs %.1f s (%d pairs)" % (time.time() - t0, n))
+    ctx = P.Context(g, device=0)
+    w = min(n, 10000)
+    ctx.map_pe(r1.subset(0, w), r2.subset(0, w))
+    t1 = time.perf_counter()
+    recs = ctx.map_pe(r1, r2)
+    dt = time.perf_counter() - t1
+    st = ctx.stats()
+    log("mapped %d pairs in %.2f s" % (n, dt))
+    cpu = None
+    concord = None
+    if not args.no_cpu:
+        from oracle import pbw_oracle as O
+        S = min(n, 20000)
+        go = O.Genome()
+        for name, s in contigs:
+            go.add_contig(name, s.tobytes())
+        m = O.Mapper(go, O.HashTable(go), threads=args.cpu_threads)
+        a = O.ReadBatch.from_arrays(*[x for x in m1[:4]]).subset(0, S)
+        b = O.ReadBatch.from_arrays(*[x for x in m2[:4]]).subset(0, S)
+        t2 = time.perf_counter()
+        ro = m.map_pe(a, b)
+        cdt = time.perf_counter() - t2
+        cpu = {"value": round(S / cdt, 1), "unit": "pairs/s", "cores": args.cpu_threads, "kind": "port",
+               "sample": "first %d pairs of the workload, oracle (C++ restatement) on %d threads, %.1f s"
+                         % (S, args.cpu_threads, cdt)}
+        concord = float(P.primary_identical(ro, P.Records(recs.rec[:2 * S], recs.cigar)).mean())
+    out = {
+        "metric": "pairs/s on 1 GPU, paired-end joint posterior",
+        "value": round(n / dt, 1), "unit": "pairs/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(dt * 1000.0, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32+f64", "data": "synthetic (probe genome seed 1, pairs seed 3)",
+        "config": {"workload": "probe_pe_2mbp: 2.1 Mbp probe genome, %d FR pairs of 2x150 bp, 10%% anchored in "
+                               "repeat copies" % n, "pair_batch": 2000, "serial_warmup_pairs": 500},
+        "reads_per_s": round(2 * n / dt, 1),
+        "dp_cells": int(st["dp_cells"]), "wide_jobs": int(st["wide_jobs"]), "wide_cells": int(st["wide_cells"]),
+        "stage_ms": {k: round(st[k], 1) for k in ("ms_seed", "ms_cluster", "ms_dp", "ms_pair", "ms_total")},
+        "cpu_baseline": cpu, "sample_sam_concordance": concord,
+    }
+    print(json.dumps(out), flush=True)
+
+
 WORKLOADS = {
     "uneven_se_100mbp": {"reads": 2_000_000, "train_reads": 131072,
                          "desc": "single-end, 100 Mbp / 10 contigs, 2M reads of 80-150 bp"},
@@ -89,7 +146,7 @@ def main():
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=262144)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="uneven_se_100mbp")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["probe_pe_2mbp"], default="uneven_se_100mbp")
     ap.add_argument("--reads", type=int, default=None)
     ap.add_argument("--cpu-sample", type=int, default=131072)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
@@ -104,6 +161,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "probe_pe_2mbp":
+        bench_pe(args)
+        return
     import probebwa_amd as P
 
     wl = WORKLOADS[args.workload]

@@ -56,6 +56,16 @@ def genome_probe_se_2mbp(seed=1):
     return [("chr1", np.concatenate(pieces))]
 
 
+def probe_repeat_spans(seed=1):
+    """[start, end) of the 50 inserted 2 kb copies in genome_probe_se_2mbp(seed): the
+    same draws replayed up to the insertion points (copy k lands after k earlier copies)."""
+    rng = np.random.default_rng(np.random.PCG64(seed))
+    random_bases(rng, 2_000_000)
+    random_bases(rng, 2000)
+    pos = np.sort(rng.choice(2_000_000, 50, replace=False))
+    return [(int(p) + 2000 * k, int(p) + 2000 * k + 2000) for k, p in enumerate(pos)]
+
+
 def genome_uneven_100mbp(seed=2, n_contigs=10, contig_len=10_000_000):
     rng = np.random.default_rng(np.random.PCG64(seed))
     contigs = []
@@ -221,6 +231,76 @@ def reads_probe_uneven_se(contigs, n=2_000_000, seed=31, lmin=80, lmax=150, chun
         seq[o0:int(so[c1])] = b[valid]
         qual[o0:int(so[c1])] = q[valid]
     return seq, so, qual, so.copy()
+
+
+def _probe_errors(rng, t, L):
+    """Rows of template bases (each at least L + 12 long) -> rows of L bases under the
+    probe read model: 0.05% per-base indel events of length U{1,2,3} (ins/del 50/50),
+    then 1% substitutions."""
+    m, W = t.shape
+    ev = (rng.random((m, W)) < 0.0005) & (np.arange(W)[None, :] < L + 3)
+    t = t.copy()
+    for r in np.nonzero(ev.any(axis=1))[0]:
+        parts, i = [], 0
+        for p in np.nonzero(ev[r])[0]:
+            if p < i:
+                continue
+            parts.append(t[r, i:p])
+            ln = int(rng.integers(1, 4))
+            if rng.random() < 0.5:
+                parts.append(random_bases(rng, ln))
+                i = p
+            else:
+                i = min(W, p + ln)
+        parts.append(t[r, i:])
+        row = np.concatenate(parts)
+        if len(row) < W:
+            row = np.concatenate([row, random_bases(rng, W - len(row))])
+        t[r] = row[:W]
+    b = t[:, :L]
+    err = rng.random((m, L)) < 0.01
+    codes = np.searchsorted(BASES, b)
+    return np.where(err, BASES[(codes + rng.integers(1, 4, (m, L))) % 4], b)
+
+
+def reads_pairs_probe(contigs, n=1_000_000, L=150, seed=3, repeat_spans=(), chunk=65536):
+    """FR pairs on the probe genome: fragment ~ N(350, 50) rounded and clipped to
+    [200, 600]; 10% of the pairs placed so that one mate lies inside a repeat copy;
+    mates under the probe read model (indels, 1% substitutions, Q ~ U{20..40});
+    mate 2 is the reverse complement of the fragment's end; mates swapped for half
+    of the pairs.  Vectorised; returns ((seq, so, qual, qo) of mate 1, ... of mate 2)."""
+    rng = np.random.default_rng(np.random.PCG64(seed))
+    g = contigs[0][1]
+    pad = 12
+    frag = np.clip(np.rint(rng.normal(350, 50, n)), 200, 600).astype(np.int64)
+    start = (rng.random(n) * (len(g) - frag - 1 - pad)).astype(np.int64)
+    if repeat_spans:
+        anch = np.nonzero(rng.random(n) < 0.1)[0]
+        sp = np.asarray(repeat_spans, np.int64)[rng.integers(0, len(repeat_spans), len(anch))]
+        off = (rng.random(len(anch)) * np.maximum(1, sp[:, 1] - sp[:, 0] - L)).astype(np.int64)
+        start[anch] = np.clip(sp[:, 0] + off - frag[anch] + L, 0, len(g) - frag[anch] - 1 - pad)
+    swap = rng.random(n) < 0.5
+    so = np.arange(n + 1, dtype=np.uint64) * np.uint64(L)
+    out = [np.zeros(n * L + 1, np.uint8) for _ in range(4)]
+    col = np.arange(L + pad)
+    for c0 in range(0, n, chunk):
+        c1 = min(n, c0 + chunk)
+        st, fr = start[c0:c1], frag[c0:c1]
+        a = _probe_errors(rng, g[st[:, None] + col[None, :]], L)
+        # mate 2: reverse complement of the fragment's last L bases; its template runs
+        # backwards from the fragment end so deletions consume bases beyond its 5' end
+        e = st + fr - 1
+        tb = COMP[g[np.maximum(e[:, None] - col[None, :], 0)]]
+        b2 = _probe_errors(rng, tb, L)
+        sw = swap[c0:c1][:, None]
+        m1, m2 = np.where(sw, b2, a), np.where(sw, a, b2)
+        q1 = rng.integers(20, 41, (c1 - c0, L)).astype(np.uint8)
+        q2 = rng.integers(20, 41, (c1 - c0, L)).astype(np.uint8)
+        out[0][c0 * L:c1 * L] = m1.ravel()
+        out[1][c0 * L:c1 * L] = q1.ravel()
+        out[2][c0 * L:c1 * L] = m2.ravel()
+        out[3][c0 * L:c1 * L] = q2.ravel()
+    return (out[0], so, out[1], so.copy()), (out[2], so.copy(), out[3], so.copy())
 
 
 def reads_pairs(contigs, n=10_000, L=150, seed=13, repeat_spans=()):
