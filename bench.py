@@ -52,6 +52,22 @@ TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_traffic_uneven_se_100mbp.t
 TRAFFIC_BATCH = 262144  # reads per launch the profile was taken at
 
 
+VALU_PROFILE = os.path.join(ROOT, "profiles", "pmc_valu_uneven_se_100mbp.txt")
+
+
+def measured_valu(kernel, batch):
+    """VALU issue counters of `kernel` from the committed rocprofv3 summary (taken at the
+    default batch): fraction of the VALU issue capacity used, int32 share of it, active lanes."""
+    if batch != TRAFFIC_BATCH or not os.path.exists(VALU_PROFILE):
+        return None
+    for line in open(VALU_PROFILE):
+        f = line.strip().split(",")
+        if len(f) == 10 and f[0] == kernel:
+            return {"valu_busy": float(f[7]), "int32_util": float(f[8]), "lanes_active": float(f[9]),
+                    "valu_insts": int(f[1]), "int32_insts": int(f[2])}
+    return None
+
+
 def measured_traffic(kernel, batch):
     """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, copy-calibrated) from the
     committed rocprofv3 counter summary, when it was taken at this batch size; else None."""
@@ -219,7 +235,8 @@ def main():
         ach = dp_gcups * DP_OPS_PER_CELL / 1000.0
         roof = {"kernel": "k_dp_quad", "bound": "valu-int32", "achieved": round(ach, 3), "peak": round(PEAK_INT32_TOPS, 2),
                 "unit": "Tops/s", "frac": round(ach / PEAK_INT32_TOPS, 4), "traffic": measured_traffic("pbw::k_dp_quad", B),
-                "per_launch": {"cells": cells_per_launch, "ms": ms_dp, "ops_per_cell": DP_OPS_PER_CELL}}
+                "per_launch": {"cells": cells_per_launch, "ms": ms_dp, "ops_per_cell": DP_OPS_PER_CELL},
+                "counters": measured_valu("pbw::k_dp_quad", B)}
     else:
         roof = {"kernel": "k_seed", "bound": "hbm", "achieved": round(gather_gbps, 2), "peak": PEAK_HBM_GBPS,
                 "unit": "GB/s", "frac": round(gather_gbps / PEAK_HBM_GBPS, 4), "traffic": measured_traffic("pbw::k_seed", B),
