@@ -166,6 +166,33 @@ int pbw_map_se_device(pbw_ctx* ctx, const pbw_device_reads* reads, uint64_t begi
                       pbw_record_batch* out);
 void pbw_reads_free(pbw_device_reads* reads);
 
+/* ---- Autotuner (no counterpart in the reference): launch shapes that change
+ *      the speed of a mapping call, never its records. ---- */
+typedef struct {
+  uint32_t batch_reads; /* reads per device batch (as pbw_options.batch_reads) */
+  uint32_t dp_grid;     /* waves of a four-jobs-per-wave DP launch (jobs beyond loop) */
+  uint32_t split_len;   /* 0: one DP launch for every read length; else reads longer than
+                           split_len run in a launch of their own and the rest in a launch
+                           whose per-job LDS tile is sized for split_len bases */
+  uint32_t long_waves;  /* reads longer than split_len: 0 = four jobs per wave, 1 = one job
+                           per wave (64 lanes per alignment) */
+} pbw_tune;
+int pbw_tune_get(const pbw_ctx* ctx, pbw_tune* out);
+int pbw_tune_set(pbw_ctx* ctx, const pbw_tune* t);
+#define PBW_TUNE_MAX_TRIALS 48
+typedef struct {
+  pbw_tune best;
+  double best_ms;  /* mapping time of the sample with `best` */
+  uint32_t trials;
+  pbw_tune tried[PBW_TUNE_MAX_TRIALS];
+  double ms[PBW_TUNE_MAX_TRIALS];
+} pbw_autotune_report;
+/* Maps reads [begin, end) of `reads` under candidate settings (coordinate descent
+ * over dp_grid, then split_len x long_waves, then batch_reads; each timed twice,
+ * the faster run counts), keeps the fastest in the context and reports every trial. */
+int pbw_autotune(pbw_ctx* ctx, const pbw_device_reads* reads, uint64_t begin, uint64_t end,
+                 pbw_autotune_report* out);
+
 /* ---- paired-end: map_paired (src/lib.rs:145-185) incl. the 500-pair serial
  *      warm-up and 2000-pair frozen-snapshot batches; records interleaved
  *      r1,r2; the insert model is read and updated in place ---- */

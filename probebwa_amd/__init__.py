@@ -50,6 +50,7 @@ EXPORTED_SYMBOLS = (
     "pbw_format_sam", "pbw_free_text", "pbw_default_train_options", "pbw_mlp_init", "pbw_mlp_set_params",
     "pbw_mlp_get_params", "pbw_mlp_forward", "pbw_mlp_gradient", "pbw_mlp_train", "pbw_default_prefilter_options",
     "pbw_prefilter_set", "pbw_prefilter_get", "pbw_prefilter_dataset", "pbw_prefilter_train",
+    "pbw_tune_get", "pbw_tune_set", "pbw_autotune",
 )
 
 MLP_INPUTS = 32
@@ -80,6 +81,28 @@ class PrefilterOptions(C.Structure):
         lib().pbw_default_prefilter_options(C.byref(self))
         for k, v in kw.items():
             setattr(self, k, v)
+
+
+class Tune(C.Structure):
+    """Launch shapes of a mapping call (`pbw_tune`): they change its speed, never its records."""
+
+    _fields_ = [("batch_reads", C.c_uint32), ("dp_grid", C.c_uint32), ("split_len", C.c_uint32),
+                ("long_waves", C.c_uint32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+TUNE_MAX_TRIALS = 48
+
+
+class AutotuneReport(C.Structure):
+    _fields_ = [("best", Tune), ("best_ms", C.c_double), ("trials", C.c_uint32),
+                ("tried", Tune * TUNE_MAX_TRIALS), ("ms", C.c_double * TUNE_MAX_TRIALS)]
+
+    def as_dict(self):
+        return {"best": self.best.as_dict(), "best_ms": self.best_ms,
+                "trials": [dict(self.tried[i].as_dict(), ms=self.ms[i]) for i in range(self.trials)]}
 
 
 # training label rules (PBW_LABEL_LOGLK, PBW_LABEL_SCORE)
@@ -198,6 +221,9 @@ def lib():
             "pbw_reads_upload": (i32, [vp, C.POINTER(_ReadBatch), C.POINTER(vp)]),
             "pbw_map_se_device": (i32, [vp, vp, u64, u64, C.POINTER(_RecordBatch)]),
             "pbw_reads_free": (None, [vp]),
+            "pbw_tune_get": (i32, [vp, vp]),
+            "pbw_tune_set": (i32, [vp, vp]),
+            "pbw_autotune": (i32, [vp, vp, u64, u64, vp]),
             "pbw_map_pe": (i32, [vp, C.POINTER(_ReadBatch), C.POINTER(_ReadBatch), C.POINTER(InsertModel),
                                  C.POINTER(_RecordBatch)]),
             "pbw_align_batch": (i32, [vp, u32, u32, C.POINTER(_AlignJob), u64, C.POINTER(_AlignResult), vp, u64]),
@@ -391,6 +417,28 @@ class Context:
     def upload_reads(self, reads):
         """Copies a read batch into device memory once (`pbw_reads_upload`)."""
         return DeviceReads(self, reads)
+
+    @property
+    def tune(self):
+        t = Tune()
+        _check(lib().pbw_tune_get(self.h, C.byref(t)), self.h)
+        return t
+
+    @tune.setter
+    def tune(self, t):
+        if isinstance(t, dict):
+            cur = self.tune
+            for k, v in t.items():
+                setattr(cur, k, v)
+            t = cur
+        _check(lib().pbw_tune_set(self.h, C.byref(t)), self.h)
+
+    def autotune(self, dreads, begin=0, end=None):
+        """Times candidate launch shapes on reads [begin, end) of `dreads` and keeps the fastest."""
+        end = dreads.n if end is None else end
+        rep = AutotuneReport()
+        _check(lib().pbw_autotune(self.h, dreads.h, begin, end, C.byref(rep)), self.h)
+        return rep.as_dict()
 
     def map_se_device(self, dreads, begin=0, end=None):
         """Single-end map of reads [begin, end) already resident in HBM."""

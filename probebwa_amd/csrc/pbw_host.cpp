@@ -269,6 +269,7 @@ struct pbw_ctx {
   DevBuf<Cand> d_cands;
   DevBuf<uint8_t> d_ncand, d_njobs, d_rec, d_refbytes;
   DevBuf<uint32_t> d_flag_a, d_flag_b, d_flag_c, d_flag_n, d_sorted, d_sort_hist, d_tbq;
+  pbw_tune tune{0, 0, 0, 0};  // batch_reads mirrors opt.batch_reads; zeros elsewhere: defaults
   DevBuf<DpJob> d_jobs;
   DevBuf<DpOut> d_out;
   DevBuf<Counters> d_ctr;
@@ -806,27 +807,55 @@ static void batch_dp(pbw_ctx* c, Batch& b, const uint32_t* jlist, uint32_t nlist
     launch_job_sort(da, c->d_sort_hist.p, c->d_sorted.p, c->d_flag_n.p + 3, s);
     HIP_CHECK(hipGetLastError());
     const uint32_t nsorted = download_u32(c, c->d_flag_n.p + 3);
-    DpArgs qa = da;
-    qa.jlist = c->d_sorted.p;
-    qa.njobs = nsorted;
-    qa.job_valid = nullptr;
-    qa.max_m = std::min<uint32_t>(b.max_m, QUAD_MAX_M);
-    qa.max_n = std::min<uint32_t>(b.max_n, qa.max_m + (uint32_t)right_pad + LEFT_PAD);
-    qa.flag_list = c->d_flag_a.p;
-    qa.flag_n = c->d_flag_n.p;
-    if (qa.max_m <= QUAD_MAX_M) {
-      c->d_tbq.ensure(quad_scratch_words(qa.max_m, qa.max_n, nsorted));
-      qa.tbq = c->d_tbq.p;
-    }
+    // length split (pbw_tune): sorted jobs [0, n_long) are the reads longer than split_len;
+    // after the scatter, hist[k] is the end of key k (key = MAX_READ_LEN - read length)
+    const pbw_tune& tn = c->tune;
+    const bool split = tn.split_len > 0 && tn.split_len < b.max_m;
+    const uint32_t n_long = split ? download_u32(c, c->d_sort_hist.p + (MAX_READ_LEN - tn.split_len - 1)) : 0u;
+    auto quad_args = [&](uint32_t lo, uint32_t cnt, uint32_t mm) {
+      DpArgs q = da;
+      q.jlist = c->d_sorted.p + lo;
+      q.njobs = cnt;
+      q.job_valid = nullptr;
+      q.max_m = std::min<uint32_t>(mm, QUAD_MAX_M);
+      q.max_n = std::min<uint32_t>(b.max_n, q.max_m + (uint32_t)right_pad + LEFT_PAD);
+      q.flag_list = c->d_flag_a.p;
+      q.flag_n = c->d_flag_n.p;
+      q.quad_grid = tn.dp_grid;
+      return q;
+    };
     DpArgs ba = da;
     ba.flag_list = c->d_flag_b.p;
     ba.flag_n = c->d_flag_n.p + 1;
+    auto band_args = [&](uint32_t lo, uint32_t cnt) {
+      DpArgs q = ba;
+      q.jlist = c->d_sorted.p + lo;
+      q.njobs = cnt;
+      q.job_valid = nullptr;
+      return q;
+    };
+    std::vector<DpArgs> quads, bands;
+    if (!split) {
+      quads.push_back(quad_args(0, nsorted, b.max_m));
+    } else {
+      if (tn.long_waves) bands.push_back(band_args(0, n_long));
+      else quads.push_back(quad_args(0, n_long, b.max_m));
+      quads.push_back(quad_args(n_long, nsorted - n_long, tn.split_len));
+    }
+    size_t tbw = 0;
+    for (const DpArgs& q : quads)
+      if (q.max_m <= QUAD_MAX_M) tbw = std::max(tbw, quad_scratch_words(q.max_m, q.max_n, q.njobs, q.quad_grid));
+    c->d_tbq.ensure(std::max<size_t>(tbw, 1));
     HIP_CHECK(hipEventRecord(c->ev[2], s));
-    const bool quad = launch_dp_quad(qa, s);
-    if (!quad) launch_dp_band(ba, s);
+    for (DpArgs& q : quads) {
+      q.tbq = c->d_tbq.p;
+      // a launch whose tile does not fit LDS goes to the one-job-per-wave kernel
+      if (!launch_dp_quad(q, s)) bands.push_back(band_args((uint32_t)(q.jlist - c->d_sorted.p), q.njobs));
+    }
+    for (const DpArgs& q : bands) launch_dp_band(q, s);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(c->ev[3], s));
-    const uint32_t nflag = quad ? download_u32(c, qa.flag_n) : 0u;
+    const uint32_t nflag = download_u32(c, c->d_flag_n.p);
     {
       float ms = 0.f;
       HIP_CHECK(hipEventSynchronize(c->ev[3]));
@@ -834,6 +863,7 @@ static void batch_dp(pbw_ctx* c, Batch& b, const uint32_t* jlist, uint32_t nlist
       c->stats.ms_k_dp += ms;
       c->stats.n_k_dp += 1;
     }
+    if (split && tn.long_waves) c->stats.band_dp_jobs += n_long;
     if (nflag) {
       c->stats.band_dp_jobs += nflag;
       ba.jlist = c->d_flag_a.p;
@@ -1585,6 +1615,82 @@ int pbw_map_se_device(pbw_ctx* c, const pbw_device_reads* d, uint64_t begin, uin
     for (uint64_t b = begin; b < end; b += bs) map_se_batch(c, &view, b, std::min(end, b + bs), hr, true);
     emit_batch(hr, out);
     c->stats.ms_total = ms_since(t0);
+  });
+}
+
+int pbw_tune_get(const pbw_ctx* c, pbw_tune* out) {
+  if (!c || !out) return PBW_ERR_ARG;
+  *out = c->tune;
+  out->batch_reads = c->opt.batch_reads ? c->opt.batch_reads : 131072;
+  if (!out->dp_grid) out->dp_grid = QUAD_MAX_WAVES;
+  return PBW_OK;
+}
+
+int pbw_tune_set(pbw_ctx* c, const pbw_tune* t) {
+  if (!c || !t || t->long_waves > 1 || t->split_len > (uint32_t)MAX_READ_LEN) return PBW_ERR_ARG;
+  c->tune = *t;
+  if (t->batch_reads) c->opt.batch_reads = t->batch_reads;
+  return PBW_OK;
+}
+
+// Coordinate descent over the launch shapes; every trial maps the same reads, so
+// the only difference between trials is time (the records are identical).
+int pbw_autotune(pbw_ctx* c, const pbw_device_reads* d, uint64_t begin, uint64_t end, pbw_autotune_report* out) {
+  if (!c || !d || !out || begin >= end || end > d->n) return PBW_ERR_ARG;
+  return guarded(c, [&] {
+    HIP_CHECK(hipSetDevice(c->device));
+    const pbw_stats saved = c->stats;
+    pbw_read_batch view{d->n, d->seq.p, d->seq_off.data(), d->qual.p, d->qual_off.data()};
+    uint32_t max_len = 0;
+    for (uint64_t r = begin; r < end; r++) max_len = std::max<uint32_t>(max_len, (uint32_t)(d->seq_off[r + 1] - d->seq_off[r]));
+    pbw_autotune_report rep{};
+    auto trial = [&](const pbw_tune& t) {
+      c->tune = t;
+      c->opt.batch_reads = t.batch_reads;
+      double best = 1e30;
+      for (int rep_i = 0; rep_i < 2; rep_i++) {
+        HostRecords hr;
+        hr.rec.reserve(end - begin);
+        c->stats = pbw_stats{};
+        auto t0 = std::chrono::steady_clock::now();
+        for (uint64_t b0 = begin; b0 < end; b0 += t.batch_reads)
+          map_se_batch(c, &view, b0, std::min<uint64_t>(end, b0 + t.batch_reads), hr, true);
+        best = std::min(best, ms_since(t0));
+      }
+      if (rep.trials < PBW_TUNE_MAX_TRIALS) {
+        rep.tried[rep.trials] = t;
+        rep.ms[rep.trials] = best;
+        rep.trials++;
+      }
+      return best;
+    };
+    pbw_tune cur;
+    pbw_tune_get(c, &cur);
+    double cur_ms = trial(cur);
+    auto consider = [&](pbw_tune t) {
+      const double ms = trial(t);
+      if (ms < cur_ms) {
+        cur = t;
+        cur_ms = ms;
+      }
+    };
+    const pbw_tune base0 = cur;
+    for (uint32_t g : {8192u, 16384u, 32768u, 65536u})
+      if (g != base0.dp_grid) consider(pbw_tune{cur.batch_reads, g, cur.split_len, cur.long_waves});
+    const pbw_tune base1 = cur;
+    for (uint32_t sl : {96u, 128u, 160u, 192u, 256u})
+      if (sl < max_len)
+        for (uint32_t lw : {0u, 1u}) consider(pbw_tune{base1.batch_reads, base1.dp_grid, sl, lw});
+    const pbw_tune base2 = cur;
+    for (uint32_t bs : {65536u, 131072u, 262144u, 524288u})
+      if (bs != base2.batch_reads && bs <= std::max<uint64_t>(end - begin, 65536))
+        consider(pbw_tune{bs, base2.dp_grid, base2.split_len, base2.long_waves});
+    c->tune = cur;
+    c->opt.batch_reads = cur.batch_reads;
+    c->stats = saved;
+    rep.best = cur;
+    rep.best_ms = cur_ms;
+    *out = rep;
   });
 }
 

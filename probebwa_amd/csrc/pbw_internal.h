@@ -180,12 +180,13 @@ struct DpArgs {
   uint32_t tiles;          // ceil((max_n + 1) / 64)
   uint32_t wpb;            // waves per block (banded kernel; set by the launcher)
   uint32_t* tbq;           // k_dp_quad traceback scratch: quad_scratch_words() words
+  uint32_t quad_grid;      // k_dp_quad grid cap (0: QUAD_MAX_WAVES)
 };
 void launch_dp_band(const DpArgs& a, hipStream_t s);
 void launch_dp_tiled(const DpArgs& a, uint32_t nwaves, hipStream_t s);
 bool launch_dp_quad(const DpArgs& a, hipStream_t s);  // false: layout too large, nothing launched
 constexpr uint32_t QUAD_MAX_WAVES = 32768;  // grid cap of k_dp_quad (more blocks than resident waves: dynamic balance)
-size_t quad_scratch_words(uint32_t max_m, uint32_t max_n, uint32_t njobs);
+size_t quad_scratch_words(uint32_t max_m, uint32_t max_n, uint32_t njobs, uint32_t grid_cap);
 // Jobs of `a` (jlist / njobs / job_valid) sorted longest read first into `out`, count
 // to total[0]; `hist` holds MAX_READ_LEN + 1 words of scratch.
 void launch_job_sort(const DpArgs& a, uint32_t* hist, uint32_t* out, uint32_t* total, hipStream_t s);

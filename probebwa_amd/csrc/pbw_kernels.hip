@@ -2115,12 +2115,13 @@ void launch_job_sort(const DpArgs& a, uint32_t* hist, uint32_t* out, uint32_t* t
   if (blocks) hipLaunchKernelGGL(k_job_scatter, dim3(blocks), dim3(SORT_BLOCK), 0, s, a, hist, out);
 }
 
-uint32_t quad_blocks(uint32_t njobs) {
+uint32_t quad_blocks(uint32_t njobs, uint32_t grid_cap) {
+  const uint32_t cap = grid_cap ? grid_cap : QUAD_MAX_WAVES;
   const uint32_t q = (njobs + 3) / 4;
-  return q < QUAD_MAX_WAVES ? q : QUAD_MAX_WAVES;
+  return q < cap ? q : cap;
 }
-size_t quad_scratch_words(uint32_t max_m, uint32_t max_n, uint32_t njobs) {
-  return (size_t)quad_blocks(njobs) * 4 * quad_layout(max_m, max_n).tb_words;
+size_t quad_scratch_words(uint32_t max_m, uint32_t max_n, uint32_t njobs, uint32_t grid_cap) {
+  return (size_t)quad_blocks(njobs, grid_cap) * 4 * quad_layout(max_m, max_n).tb_words;
 }
 bool launch_dp_quad(const DpArgs& a, hipStream_t s) {
   if (a.njobs == 0) return true;
@@ -2130,7 +2131,7 @@ bool launch_dp_quad(const DpArgs& a, hipStream_t s) {
   // genome-word staging reuses the score-word area
   const size_t need = 8 * ((size_t)((a.max_n + 62) >> 5) + ((a.max_n + 126) >> 6));
   if (need > Y.q - Y.scw) return false;
-  hipLaunchKernelGGL(k_dp_quad, dim3(quad_blocks(a.njobs)), dim3(64), lds, s, a);
+  hipLaunchKernelGGL(k_dp_quad, dim3(quad_blocks(a.njobs, a.quad_grid)), dim3(64), lds, s, a);
   return true;
 }
 
