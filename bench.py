@@ -168,6 +168,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune launch shapes on the first 524288 reads before the timed steps")
     ap.add_argument("--train-reads", type=int, default=None)
     ap.add_argument("--label", choices=["loglk", "score"], default="loglk")
     ap.add_argument("--margin", type=float, default=40.0)
@@ -201,6 +203,15 @@ def main():
     dreads = ctx.upload_reads(P.ReadBatch.from_arrays(seq, so, qual, qo))
     n = n_reads
     B = min(args.batch, n)
+
+    tuned = None
+    if args.autotune:
+        t1 = time.time()
+        rep = ctx.autotune(dreads, 0, min(n, 524288))
+        tuned = {"best": rep["best"], "best_ms": round(rep["best_ms"], 2), "trials": len(rep["trials"]),
+                 "default_ms": round(rep["trials"][0]["ms"], 2), "seconds": round(time.time() - t1, 1)}
+        B = min(ctx.tune.batch_reads, n) if args.batch == 262144 else B
+        log("autotune %.1f s: %s" % (time.time() - t1, rep["best"]))
 
     def step(s):
         lo = (s * B) % n
@@ -339,6 +350,7 @@ def main():
         "gather_gbps": round(gather_gbps, 2),
         "roofline": roof,
         "roofline_gather": gather_roof,
+        "autotune": tuned,
         "learned": learned,
         "learned_vs_plain": learned["speedup_vs_plain"] if learned else None,
         "cpu_baseline": cpu,
