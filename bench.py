@@ -55,10 +55,13 @@ TRAFFIC_BATCH = 262144  # reads per launch the profile was taken at
 VALU_PROFILE = os.path.join(ROOT, "profiles", "pmc_valu_uneven_se_100mbp.txt")
 
 
-def measured_valu(kernel, batch):
+PROFILED_WORKLOAD = "uneven_se_100mbp"  # the workload the committed counter summaries were taken on
+
+
+def measured_valu(kernel, batch, workload=PROFILED_WORKLOAD):
     """VALU issue counters of `kernel` from the committed rocprofv3 summary (taken at the
     default batch): fraction of the VALU issue capacity used, int32 share of it, active lanes."""
-    if batch != TRAFFIC_BATCH or not os.path.exists(VALU_PROFILE):
+    if workload != PROFILED_WORKLOAD or batch != TRAFFIC_BATCH or not os.path.exists(VALU_PROFILE):
         return None
     for line in open(VALU_PROFILE):
         f = line.strip().split(",")
@@ -68,10 +71,10 @@ def measured_valu(kernel, batch):
     return None
 
 
-def measured_traffic(kernel, batch):
+def measured_traffic(kernel, batch, workload=PROFILED_WORKLOAD):
     """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, copy-calibrated) from the
-    committed rocprofv3 counter summary, when it was taken at this batch size; else None."""
-    if batch != TRAFFIC_BATCH or not os.path.exists(TRAFFIC_PROFILE):
+    committed rocprofv3 counter summary, when it was taken on this workload and batch size; else None."""
+    if workload != PROFILED_WORKLOAD or batch != TRAFFIC_BATCH or not os.path.exists(TRAFFIC_PROFILE):
         return None
     for line in open(TRAFFIC_PROFILE):
         f = line.strip().split(",")
@@ -142,6 +145,9 @@ WORKLOADS = {
                          "desc": "single-end, 100 Mbp / 10 contigs, 2M reads of 80-150 bp"},
     "prefilter_se_2mbp": {"reads": 2_000_000, "train_reads": 500_000,
                           "desc": "single-end, 2.1 Mbp probe genome, 2M held-out reads of 80-150 bp"},
+    "genome_se_3gbp": {"reads": 10_000_000, "train_reads": 131072,
+                       "desc": "single-end, 3.1 Gbp / 24 contigs (45% interspersed repeats, 5% segmental "
+                               "duplications), 10M reads of 60-250 bp"},
 }
 
 
@@ -151,6 +157,10 @@ def make_workload(name, n, n_train):
         contigs = synth.genome_uneven_100mbp()
         return (contigs, lambda: synth.reads_uneven_se(contigs, n=n, seed=12),
                 lambda: synth.reads_uneven_se(contigs, n=n_train, seed=99), "seeds 2/12/99")
+    if name == "genome_se_3gbp":
+        contigs = synth.genome_scale_3gbp()
+        return (contigs, lambda: synth.reads_uneven_se(contigs, n=n, seed=55, lmin=60, lmax=250),
+                lambda: synth.reads_uneven_se(contigs, n=n_train, seed=59, lmin=60, lmax=250), "seeds 5/55/59")
     contigs = synth.genome_probe_se_2mbp()
     return (contigs, lambda: synth.reads_probe_uneven_se(contigs, n=n, seed=42),
             lambda: synth.reads_probe_uneven_se(contigs, n=n_train, seed=41), "seeds 1/42/41")
@@ -245,16 +255,16 @@ def main():
     if agg["ms_k_dp"] >= agg["ms_k_seed"]:
         ach = dp_gcups * DP_OPS_PER_CELL / 1000.0
         roof = {"kernel": "k_dp_quad", "bound": "valu-int32", "achieved": round(ach, 3), "peak": round(PEAK_INT32_TOPS, 2),
-                "unit": "Tops/s", "frac": round(ach / PEAK_INT32_TOPS, 4), "traffic": measured_traffic("pbw::k_dp_quad", B),
+                "unit": "Tops/s", "frac": round(ach / PEAK_INT32_TOPS, 4), "traffic": measured_traffic("pbw::k_dp_quad", B, args.workload),
                 "per_launch": {"cells": cells_per_launch, "ms": ms_dp, "ops_per_cell": DP_OPS_PER_CELL},
-                "counters": measured_valu("pbw::k_dp_quad", B)}
+                "counters": measured_valu("pbw::k_dp_quad", B, args.workload)}
     else:
         roof = {"kernel": "k_seed", "bound": "hbm", "achieved": round(gather_gbps, 2), "peak": PEAK_HBM_GBPS,
-                "unit": "GB/s", "frac": round(gather_gbps / PEAK_HBM_GBPS, 4), "traffic": measured_traffic("pbw::k_seed", B),
+                "unit": "GB/s", "frac": round(gather_gbps / PEAK_HBM_GBPS, 4), "traffic": measured_traffic("pbw::k_seed", B, args.workload),
                 "per_launch": {"bytes": gbytes_per_launch, "ms": ms_seed}}
     gather_roof = {"kernel": "k_seed", "bound": "hbm", "achieved": round(gather_gbps, 2), "peak": PEAK_HBM_GBPS,
                    "unit": "GB/s", "frac": round(gather_gbps / PEAK_HBM_GBPS, 4),
-                   "traffic": measured_traffic("pbw::k_seed", B)}
+                   "traffic": measured_traffic("pbw::k_seed", B, args.workload)}
 
     learned = None
     if not args.no_learned:
@@ -318,7 +328,7 @@ def main():
         go = O.Genome()
         for name, s in contigs:
             go.add_contig(name, s.tobytes())
-        ht = O.HashTable(go)
+        ht = O.HashTable(go, progress=True)
         m = O.Mapper(go, ht, threads=args.cpu_threads)
         log("oracle index %.1f s" % (time.time() - t1))
         rb = O.ReadBatch.from_arrays(seq[:int(so[S])], so[:S + 1], qual[:int(qo[S])], qo[:S + 1])

@@ -287,6 +287,9 @@ static void stable_sort_by_hash(std::vector<uint32_t>& h, std::vector<uint64_t>&
   }
 }
 
+// Progress lines of long index builds (genome-scale references), on stderr.
+static int g_progress = 0;
+
 // hashtable.rs:248-322 HashTableBuilder::build
 static HashTable* hash_build(const Genome& g, size_t k, size_t stride, size_t max_count) {
   std::vector<uint32_t> ph;
@@ -298,6 +301,7 @@ static HashTable* hash_build(const Genome& g, size_t k, size_t stride, size_t ma
   std::vector<uint8_t> seq;
   for (size_t cid = 0; cid < g.contigs.size(); cid++) {
     const Contig& c = g.contigs[cid];
+    if (g_progress) fprintf(stderr, "[oracle] index: scanning contig %zu/%zu\n", cid + 1, g.contigs.size());
     contig_slice(c, 0, c.length, seq);
     if (seq.size() < k) continue;
     // forward strand: hashtable.rs:266-275
@@ -329,7 +333,9 @@ static HashTable* hash_build(const Genome& g, size_t k, size_t stride, size_t ma
     }
   }
   seen = U32Map();
+  if (g_progress) fprintf(stderr, "[oracle] index: sorting %zu entries\n", ph.size());
   stable_sort_by_hash(ph, pp);
+  if (g_progress) fprintf(stderr, "[oracle] index: building buckets\n");
   HashTable* ht = new HashTable();
   ht->k = k;
   ht->stride = stride;
@@ -1747,6 +1753,7 @@ void orc_unpack_position(uint64_t packed, uint64_t* c, uint64_t* p, int* s) {
   *s = st == Forward ? 0 : 1;
 }
 int orc_is_acgt_only(const uint8_t* k, uint64_t len) { return is_acgt_only(k, len) ? 1 : 0; }
+void orc_set_progress(int on) { g_progress = on; }
 void* orc_hash_build(const void* g, int k, int stride, int max_count) {
   return hash_build(*(const Genome*)g, (size_t)k, (size_t)stride, (size_t)max_count);
 }

@@ -144,6 +144,7 @@ def lib():
             ),
             "orc_records_release": (None, [C.POINTER(_Records)]),
             "orc_set_group_order": (None, [i32]),
+            "orc_set_progress": (None, [i32]),
             "orc_read_reads": (i32, [C.c_char_p, i32, C.c_uint8, C.c_char_p, C.POINTER(_Parsed)]),
             "orc_parsed_reads_free": (None, [C.POINTER(_Parsed)]),
             "orc_format_sam": (C.c_void_p, [vp, C.POINTER(_Parsed), C.POINTER(_Records), C.c_char_p, C.c_char_p,
@@ -229,9 +230,10 @@ def is_acgt_only(kmer):
 class HashTable:
     """`HashTableBuilder::new(&genome).with_k(k).with_stride(s).with_max_count(c).build()` (hashtable.rs:248)."""
 
-    def __init__(self, genome, k=15, stride=5, max_count=300):
+    def __init__(self, genome, k=15, stride=5, max_count=300, progress=False):
         self.genome = genome
         self.k = k
+        lib().orc_set_progress(1 if progress else 0)
         self.h = lib().orc_hash_build(genome.h, k, stride, max_count)
 
     def __del__(self):

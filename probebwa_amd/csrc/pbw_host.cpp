@@ -738,10 +738,11 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false) {
   HIP_CHECK(hipMemcpyAsync(&nt2, c->d_ntier2.p, 4, hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
   if (nt2) {
-    // one 256-thread block per read, per-block scratch sized for the largest read, <= 1 GiB in all
+    // one 256-thread block per read, per-block scratch sized for the largest read; up to
+    // 4096 blocks within 16 GiB (repeat-rich genomes send most reads here, with 10^4 hits each)
     ca.scratch_per_block = cluster_scratch_words(max_hits);
-    const uint64_t fit = std::max<uint64_t>(1, (1ull << 27) / ca.scratch_per_block);
-    const uint32_t blocks = (uint32_t)std::min<uint64_t>({(uint64_t)nt2, 2048, fit});
+    const uint64_t fit = std::max<uint64_t>(1, (1ull << 31) / ca.scratch_per_block);
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>({(uint64_t)nt2, 4096, fit});
     c->d_scratch.ensure((size_t)blocks * ca.scratch_per_block);
     ca.scratch = c->d_scratch.p;
     launch_cluster_tier2(ca, c->d_tier2.p, nt2, blocks, s);

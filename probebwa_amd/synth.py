@@ -92,6 +92,66 @@ def genome_uneven_100mbp(seed=2, n_contigs=10, contig_len=10_000_000):
     return [("chr%d" % (c + 1), contigs[c]) for c in range(n_contigs)]
 
 
+def _mutated_copies(rng, consensus, n, dmin, dmax):
+    """n copies of `consensus` with per-copy substitution divergence U[dmin, dmax], half of
+    them reverse-complemented.  Returns an (n, len) uint8 array."""
+    L = len(consensus)
+    out = np.empty((n, L), np.uint8)
+    for c0 in range(0, n, 65536):
+        c1 = min(n, c0 + 65536)
+        m = c1 - c0
+        x = np.broadcast_to(consensus, (m, L)).copy()
+        d = rng.uniform(dmin, dmax, m)
+        err = rng.random((m, L)) < d[:, None]
+        codes = np.searchsorted(BASES, x)
+        x = np.where(err, BASES[(codes + rng.integers(1, 4, (m, L), dtype=np.uint8)) % 4], x)
+        rc = rng.random(m) < 0.5
+        x[rc] = COMP[x[rc, ::-1]]
+        out[c0:c1] = x
+    return out
+
+
+def genome_scale_3gbp(seed=5, n_contigs=24, total=3_100_000_000):
+    """Genome-scale synthetic reference: `n_contigs` contigs of equal length summing to
+    `total` bases; a uniform background with 45% of the bases in interspersed repeats from
+    two families (a 300 bp consensus whose copies diverge 10-20%, a 6 kb consensus at
+    5-15%; half of the repeat bases each, both strands, in random order with exponential
+    background gaps) and 5% in segmental duplications (copies of 1-10 kb segments of the
+    same contig at 0.5-3% divergence).  Built contig by contig in uint8."""
+    rng = np.random.default_rng(np.random.PCG64(seed))
+    fam_s = random_bases(rng, 300)
+    fam_l = random_bases(rng, 6000)
+    L = total // n_contigs
+    contigs = []
+    for ci in range(n_contigs):
+        g = BASES[rng.integers(0, 4, L, dtype=np.uint8)]
+        ns = int(0.225 * L / 300)
+        nl = int(0.225 * L / 6000)
+        lens = np.concatenate([np.full(ns, 300, np.int64), np.full(nl, 6000, np.int64)])
+        kind = np.concatenate([np.zeros(ns, np.int8), np.ones(nl, np.int8)])
+        order = rng.permutation(len(lens))
+        lens, kind = lens[order], kind[order]
+        gaps = rng.exponential(1.0, len(lens) + 1)
+        gaps = np.floor(gaps / gaps.sum() * (L - lens.sum())).astype(np.int64)
+        starts = np.cumsum(gaps[:-1]) + np.concatenate([[0], np.cumsum(lens)[:-1]])
+        for k, cons, dlo, dhi in ((0, fam_s, 0.10, 0.20), (1, fam_l, 0.05, 0.15)):
+            sel = np.nonzero(kind == k)[0]
+            cp = _mutated_copies(rng, cons, len(sel), dlo, dhi)
+            idx = starts[sel][:, None] + np.arange(len(cons))[None, :]
+            g[idx] = cp
+        # segmental duplications: 5% of the contig
+        done = 0
+        while done < 0.05 * L:
+            ln = int(rng.integers(1000, 10001))
+            src = int(rng.integers(0, L - ln))
+            dst = int(rng.integers(0, L - ln))
+            seg = mutate(rng, g[src:src + ln].copy(), float(rng.uniform(0.005, 0.03)))
+            g[dst:dst + ln] = seg
+            done += ln
+        contigs.append(("chr%d" % (ci + 1), g))
+    return contigs
+
+
 def _flat(seqs, quals):
     so = np.zeros(len(seqs) + 1, np.uint64)
     qo = np.zeros(len(quals) + 1, np.uint64)
