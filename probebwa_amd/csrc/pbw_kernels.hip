@@ -687,8 +687,12 @@ struct ClusterMem {
   uint32_t gcap;       // capacity of the g_* arrays and tmp
   uint64_t* stage_key;  // block tier: LDS chunks for the sorts (CLUSTER_STAGE keys)
   uint32_t* stage_val;
+  uint32_t* gt_key;     // block tier: LDS table of the read's distinct groups (CLUSTER_GT slots)
+  uint32_t* gt_cnt;     //   hits per group, then the group's scatter cursor
+  uint32_t* gt_misc;    //   [0] distinct groups, [1] listed groups
 };
 constexpr uint32_t CLUSTER_STAGE = 1024;
+constexpr uint32_t CLUSTER_GT = 1024, CLUSTER_GT_MAX = 512;  // table slots, groups handled
 
 template <int T>
 __device__ __forceinline__ void cluster_sort(const ClusterMem& M, uint64_t* key, uint32_t* val, uint32_t p2,
@@ -719,23 +723,151 @@ struct ClusterFeat {
   uint32_t* cnt;         // team counters: clusters tied at the top count, quality sum, q < 10, q < 20, repeat windows
 };
 
+__device__ __forceinline__ uint32_t gt_slot(uint32_t g) { return (g * 0x9E3779B1u) >> 22; }  // 10 bits
+
+// Block tier: the (group, diagonal) sort as a segmented sort.  The hits are
+// counted per group in an LDS table, the groups ranked in ascending order, the
+// hits scattered into one segment per group (key[] / idx[], emission index as
+// the value), and then every run of consecutive whole segments that fits
+// CLUSTER_STAGE keys is sorted in LDS (the group sits in the key's high bits,
+// so sorting several whole segments together keeps them apart); a segment
+// longer than that is sorted through the staged network in the scratch that
+// cl_sum / cl_start provide until the clustering needs them.  Ties between
+// equal keys may land in any order, as in the full sort: the clustering only
+// sums diagonals and takes minimum emission indices.  Returns false (nothing
+// written) when the read has more than CLUSTER_GT_MAX groups.
+template <int T>
+__device__ bool group_segment_sort(const uint64_t* hits, uint32_t n, const ClusterMem& M, uint32_t tid) {
+  for (uint32_t i = tid; i < CLUSTER_GT; i += T) {
+    M.gt_key[i] = 0xFFFFFFFFu;
+    M.gt_cnt[i] = 0;
+  }
+  if (tid < 2) M.gt_misc[tid] = 0;
+  team_sync<T>();
+  for (uint32_t i = tid; i < n; i += T) {
+    const uint32_t g = (uint32_t)(hits[i] >> DIAG_BITS);
+    uint32_t sl = gt_slot(g);
+    for (uint32_t probe = 0; probe < CLUSTER_GT; probe++) {
+      const uint32_t prev = atomicCAS(&M.gt_key[sl], 0xFFFFFFFFu, g);
+      if (prev == 0xFFFFFFFFu) atomicAdd(&M.gt_misc[0], 1u);
+      if (prev == 0xFFFFFFFFu || prev == g) {
+        atomicAdd(&M.gt_cnt[sl], 1u);
+        break;
+      }
+      sl = (sl + 1) & (CLUSTER_GT - 1);
+    }
+  }
+  team_sync<T>();
+  const uint32_t G = M.gt_misc[0];
+  if (G > CLUSTER_GT_MAX) return false;
+  // groups ascending: (group << 32 | slot) keys sorted in LDS
+  uint32_t g2 = 1;
+  while (g2 < G) g2 <<= 1;
+  for (uint32_t i = tid; i < CLUSTER_GT; i += T)
+    if (M.gt_key[i] != 0xFFFFFFFFu) {
+      const uint32_t at = atomicAdd(&M.gt_misc[1], 1u);
+      M.stage_key[at] = ((uint64_t)M.gt_key[i] << 32) | i;
+    }
+  team_sync<T>();
+  for (uint32_t i = G + tid; i < g2; i += T) M.stage_key[i] = ~0ull;
+  team_sync<T>();
+  team_bitonic<T>(M.stage_key, nullptr, g2, tid);
+  // segment starts in group order: the scatter cursor of every slot; the start of
+  // group rank r stays in stage_val[r] (stage_val[G] = n)
+  uint32_t tot;
+  const uint32_t per = (G + T - 1) / T;  // ranks per thread (G <= 512: at most 2)
+  uint32_t local = 0;
+  for (uint32_t r = tid * per; r < min(G, tid * per + per); r++) local += M.gt_cnt[(uint32_t)M.stage_key[r]];
+  uint32_t run = team_excl_scan<T>(local, tot, M.gt_misc + 2 /* scan scratch */);
+  team_sync<T>();
+  for (uint32_t r = tid * per; r < min(G, tid * per + per); r++) {
+    const uint32_t sl = (uint32_t)M.stage_key[r];
+    M.stage_val[r] = run;
+    run += M.gt_cnt[sl];
+    M.gt_cnt[sl] = M.stage_val[r];
+  }
+  if (tid == 0) M.stage_val[G] = n;
+  team_sync<T>();
+  for (uint32_t i = tid; i < n; i += T) {
+    const uint64_t k = hits[i];
+    const uint32_t g = (uint32_t)(k >> DIAG_BITS);
+    uint32_t sl = gt_slot(g);
+    while (M.gt_key[sl] != g) sl = (sl + 1) & (CLUSTER_GT - 1);
+    const uint32_t pos = atomicAdd(&M.gt_cnt[sl], 1u);
+    M.key[pos] = k;
+    M.idx[pos] = i;
+  }
+  team_sync<T>();
+  // segment boundaries (ranks 0..G) move from stage_val to the group table so
+  // stage_* can hold the chunks
+  for (uint32_t r = tid; r <= G; r += T) M.gt_key[r] = M.stage_val[r];
+  team_sync<T>();
+  uint32_t r = 0;
+  while (r < G) {
+    const uint32_t a0 = M.gt_key[r];
+    uint32_t r1 = r + 1;
+    while (r1 < G && M.gt_key[r1 + 1] - a0 <= CLUSTER_STAGE) r1++;
+    const uint32_t cnt = M.gt_key[r1] - a0;
+    if (cnt <= CLUSTER_STAGE) {  // whole segments r..r1-1 in one LDS sort
+      uint32_t p2 = 1;
+      while (p2 < cnt) p2 <<= 1;
+      for (uint32_t i = tid; i < p2; i += T) {
+        M.stage_key[i] = i < cnt ? M.key[a0 + i] : ~0ull;
+        M.stage_val[i] = i < cnt ? M.idx[a0 + i] : 0u;
+      }
+      team_sync<T>();
+      team_bitonic<T>(M.stage_key, M.stage_val, p2, tid);
+      for (uint32_t i = tid; i < cnt; i += T) {
+        M.key[a0 + i] = M.stage_key[i];
+        M.idx[a0 + i] = M.stage_val[i];
+      }
+      team_sync<T>();
+    } else {  // one segment longer than a chunk: padded copy in cl_sum / cl_start
+      uint32_t p2 = 1;
+      while (p2 < cnt) p2 <<= 1;
+      uint64_t* tk = reinterpret_cast<uint64_t*>(M.cl_sum);
+      uint32_t* tv = M.cl_start;
+      for (uint32_t i = tid; i < p2; i += T) {
+        tk[i] = i < cnt ? M.key[a0 + i] : ~0ull;
+        tv[i] = i < cnt ? M.idx[a0 + i] : 0u;
+      }
+      team_sync<T>();
+      team_bitonic_staged<T, CLUSTER_STAGE>(tk, tv, p2, tid, M.stage_key, M.stage_val);
+      for (uint32_t i = tid; i < cnt; i += T) {
+        M.key[a0 + i] = tk[i];
+        M.idx[a0 + i] = tv[i];
+      }
+      team_sync<T>();
+    }
+    r = r1;
+  }
+  return true;
+}
+
 template <int T, bool FEAT>
 __device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const ClusterMem& M, Cand* out,
                              uint8_t* ncand_out, uint32_t* sh, const ClusterFeat& F) {
   const uint32_t tid = threadIdx.x % T;
   uint32_t p2 = 1;
   while (p2 < n) p2 <<= 1;
-  for (uint32_t i = tid; i < p2; i += T) {
-    M.key[i] = i < n ? hits[i] : ~0ull;
-    M.idx[i] = i;
-    if (i < n) M.cl_sum[i] = 0ull;
+  // group by (contig, strand), sort each group by diagonal (:36-49): one sort by (group, diagonal)
+  bool sorted = false;
+  if constexpr (T != 64) sorted = group_segment_sort<T>(hits, n, M, tid);
+  if (!sorted) {
+    for (uint32_t i = tid; i < p2; i += T) {
+      M.key[i] = i < n ? hits[i] : ~0ull;
+      M.idx[i] = i;
+    }
+    team_sync<T>();
+    cluster_sort<T>(M, M.key, M.idx, p2, tid);
+  }
+  for (uint32_t i = tid; i < n; i += T) {
+    M.cl_sum[i] = 0ull;
     if constexpr (FEAT) {
-      if (i < n && F.hf != F.hflag) F.hf[i] = F.hflag[i];
+      if (F.hf != F.hflag) F.hf[i] = F.hflag[i];
     }
   }
   team_sync<T>();
-  // group by (contig, strand), sort each group by diagonal (:36-49): one sort by (group, diagonal)
-  cluster_sort<T>(M, M.key, M.idx, p2, tid);
   // group and cluster starts (sliding merge, :51-71), one contiguous chunk per thread
   const uint32_t chunk = (n + T - 1) / T;
   const uint32_t c0 = min(n, tid * chunk), c1 = min(n, c0 + chunk);
@@ -971,7 +1103,7 @@ __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a, const uint32_t* l
   __shared__ ClusterLds<NC> S;
   __shared__ uint32_t FX[FEAT ? 4 * NC + 8 + NC / 4 : 1];
   const ClusterMem M{S.key, S.idx, S.cl_sum, S.cl_start, S.cl_gid, S.g_key, S.g_min, S.g_rank, S.g_sort, S.table,
-                     S.tmp, CL_GCAP, nullptr, nullptr};
+                     S.tmp, CL_GCAP, nullptr, nullptr, nullptr, nullptr, nullptr};
   const uint32_t total = list ? list_n[0] : a.R.n;
   for (uint32_t li = blockIdx.x; li < total; li += gridDim.x) {
     const uint32_t r = list ? list[li] : li;
@@ -1016,6 +1148,7 @@ __global__ __launch_bounds__(256) void k_cluster_tier2(ClusterArgs a, const uint
   __shared__ uint32_t sh[4];
   __shared__ uint64_t stage_key[CLUSTER_STAGE];
   __shared__ uint32_t stage_val[CLUSTER_STAGE];
+  __shared__ uint32_t gt_key[CLUSTER_GT], gt_cnt[CLUSTER_GT], gt_misc[2 + 256 / 64];
   for (uint32_t li = blockIdx.x; li < nlist; li += gridDim.x) {
     const uint32_t r = list[li];
     const uint32_t n = a.nhits[r];
@@ -1039,6 +1172,9 @@ __global__ __launch_bounds__(256) void k_cluster_tier2(ClusterArgs a, const uint
     M.gcap = p2;
     M.stage_key = stage_key;
     M.stage_val = stage_val;
+    M.gt_key = gt_key;
+    M.gt_cnt = gt_cnt;
+    M.gt_misc = gt_misc;
     ClusterFeat F{};
     if (FEAT) {
       const uint64_t so = a.R.seq_off[r], qo = a.R.qual_off[r];
