@@ -844,6 +844,74 @@ __device__ bool group_segment_sort(const uint64_t* hits, uint32_t n, const Clust
   return true;
 }
 
+// Block tier, many clusters: only the nt smallest of the C (distinct) score keys
+// in key[0..C) matter.  An MSB-first radix select (one LDS histogram of a key
+// byte per pass) finds the nt-th smallest key; the keys up to it are sorted and
+// swapped to the front of key[] / idx[], so key[] still holds every cluster's key
+// (the prefilter features count ties over all of them).  idx[c] must be c.
+template <int T>
+__device__ void top_clusters(const ClusterMem& M, uint32_t C, uint32_t nt, uint32_t tid) {
+  uint32_t* hist = M.gt_cnt;
+  uint32_t* sh = M.gt_misc;
+  uint64_t prefix = 0, mask = 0;
+  uint32_t need = nt;
+  for (int byte = 7; byte >= 0; byte--) {
+    const int sh8 = 8 * byte;
+    for (uint32_t i = tid; i < 256; i += T) hist[i] = 0;
+    team_sync<T>();
+    for (uint32_t c = tid; c < C; c += T) {
+      const uint64_t k = M.key[c];
+      if ((k & mask) == prefix) atomicAdd(&hist[(uint32_t)(k >> sh8) & 255u], 1u);
+    }
+    team_sync<T>();
+    if (tid == 0) {
+      uint32_t acc = 0, d = 0;
+      for (; d < 255; d++) {
+        if (acc + hist[d] >= need) break;
+        acc += hist[d];
+      }
+      sh[0] = d;
+      sh[1] = need - acc;
+    }
+    team_sync<T>();
+    prefix |= (uint64_t)sh[0] << sh8;
+    mask |= 0xFFull << sh8;
+    need = sh[1];
+    team_sync<T>();
+  }
+  if (tid == 0) sh[2] = 0;
+  team_sync<T>();
+  for (uint32_t c = tid; c < C; c += T)
+    if (M.key[c] <= prefix) {
+      const uint32_t at = atomicAdd(&sh[2], 1u);
+      M.stage_key[at] = M.key[c];
+      M.stage_val[at] = c;
+    }
+  team_sync<T>();
+  uint32_t p2 = 1;
+  while (p2 < nt) p2 <<= 1;
+  for (uint32_t i = nt + tid; i < p2; i += T) M.stage_key[i] = ~0ull;
+  team_sync<T>();
+  team_bitonic<T>(M.stage_key, M.stage_val, p2, tid);
+  if (tid == 0) {
+    uint32_t pos[TOP_CANDIDATES];
+    for (uint32_t j = 0; j < nt; j++) pos[j] = M.stage_val[j];
+    for (uint32_t j = 0; j < nt; j++) {
+      const uint32_t q = pos[j];
+      if (q == j) continue;
+      const uint64_t tk = M.key[j];
+      M.key[j] = M.key[q];
+      M.key[q] = tk;
+      const uint32_t tv = M.idx[j];
+      M.idx[j] = M.idx[q];
+      M.idx[q] = tv;
+      for (uint32_t m2 = j + 1; m2 < nt; m2++)
+        if (pos[m2] == j) pos[m2] = q;
+    }
+  }
+  team_sync<T>();
+}
+
 template <int T, bool FEAT>
 __device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const ClusterMem& M, Cand* out,
                              uint8_t* ncand_out, uint32_t* sh, const ClusterFeat& F) {
@@ -1010,8 +1078,13 @@ __device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const
     M.idx[c] = c;
   }
   team_sync<T>();
-  cluster_sort<T>(M, M.key, M.idx, pc, tid);
   const uint32_t nt = min(C, (uint32_t)TOP_CANDIDATES);
+  if constexpr (T != 64) {
+    if (C > 256) top_clusters<T>(M, C, nt, tid);
+    else cluster_sort<T>(M, M.key, M.idx, pc, tid);
+  } else {
+    cluster_sort<T>(M, M.key, M.idx, pc, tid);
+  }
   if constexpr (FEAT) {
     const uint64_t top = M.key[0] >> 42;
     for (uint32_t c = tid; c < C; c += T)
