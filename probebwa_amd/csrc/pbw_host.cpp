@@ -366,8 +366,10 @@ static void ctx_build(pbw_ctx* c) {
   dstart.upload(hi.kstart.data(), hi.kstart.size(), s);
   c->d_koff.ensure(CODE_SPACE + 1);
   launch_fill_koff(c->d_koff.p, dkeys.p, dstart.p, hi.keys.size(), (uint32_t)hi.pos.size(), s);
-  c->d_pres.ensure(CODE_SPACE / 64);
-  launch_fill_pres(c->d_koff.p, c->d_pres.p, s);
+  c->d_pres.ensure(3 * CODE_SPACE / 64);  // natural, rotated by 10, rotated by 20
+  launch_fill_pres(c->d_koff.p, c->d_pres.p, 0, s);
+  launch_fill_pres(c->d_koff.p, c->d_pres.p + CODE_SPACE / 64, 10, s);
+  launch_fill_pres(c->d_koff.p, c->d_pres.p + 2 * CODE_SPACE / 64, 20, s);
   HIP_CHECK(hipGetLastError());
   // LUTs (bayesian.rs:103-110, :142-179, :405-439; seeds.rs:133-134; smith_waterman.rs:272-275)
   const pbw_options& o = c->opt;
@@ -519,11 +521,18 @@ static void run_dp_tiled_dev(pbw_ctx* c, DpArgs a, const uint32_t* dlist, uint32
   a.flag_list = c->d_flag_c.p;
   a.flag_n = c->d_flag_n.p + 2;
   HIP_CHECK(hipMemsetAsync(a.flag_n, 0, 4, s));
-  uint32_t nwaves = std::min<uint32_t>(n, 2048);
   a.tiles = (a.max_n + 1 + 63) / 64;
-  c->d_rows.ensure((size_t)nwaves * 3 * (a.max_n + 1));
+  // previous row in LDS when it fits (windows up to ~2000 columns): no global
+  // round trip per row, and no per-wave row scratch, so more waves
+  const bool lds_rows = dp_tiled_rows_in_lds(a.max_m, a.max_n);
+  const uint32_t nwaves = std::min<uint32_t>(n, lds_rows ? 8192 : 2048);
+  if (lds_rows) {
+    a.rows = nullptr;
+  } else {
+    c->d_rows.ensure((size_t)nwaves * 3 * (a.max_n + 1));
+    a.rows = c->d_rows.p;
+  }
   c->d_tb.ensure((size_t)nwaves * (a.max_m + 1) * a.tiles * 4);
-  a.rows = c->d_rows.p;
   a.tb = c->d_tb.p;
   launch_dp_tiled(a, nwaves, s);
   HIP_CHECK(hipGetLastError());
@@ -622,7 +631,8 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false) {
   c->d_nhits.ensure(B);
   SeedArgs sa{};
   sa.G = c->dev_genome();
-  sa.X = DevIndex{c->d_koff.p, c->d_kpos.p, c->d_pres.p};
+  sa.X = DevIndex{c->d_koff.p, c->d_kpos.p, c->d_pres.p, c->d_pres.p + CODE_SPACE / 64,
+                  c->d_pres.p + 2 * CODE_SPACE / 64};
   sa.R = R;
   sa.rlist = nullptr;
   sa.nlist = B;

@@ -40,6 +40,11 @@ struct DevIndex {
   const uint32_t* koff;  // [2^30 + 1] bucket start by 15-mer code; bucket = [koff[c], koff[c+1])
   const uint64_t* kpos;  // packed positions: bit 63 strand, 62..40 contig, 39..0 position
   const uint64_t* pres;  // [2^24] bit c%64 of word c/64 set when bucket c is non-empty
+  // the same bitmap indexed by the code rotated right by 10 and by 20 bits, so that
+  // the 1-mismatch neighbours at window positions 5-9 (resp. 0-4) share the
+  // original code's 1024-bit line, as positions 10-14 do in `pres`
+  const uint64_t* pres_r10;
+  const uint64_t* pres_r20;
 };
 
 struct DevReads {
@@ -184,6 +189,8 @@ struct DpArgs {
 };
 void launch_dp_band(const DpArgs& a, hipStream_t s);
 void launch_dp_tiled(const DpArgs& a, uint32_t nwaves, hipStream_t s);
+// true: the tiled kernel keeps its previous row in LDS (pass rows = nullptr)
+bool dp_tiled_rows_in_lds(uint32_t max_m, uint32_t max_n);
 bool launch_dp_quad(const DpArgs& a, hipStream_t s);  // false: layout too large, nothing launched
 constexpr uint32_t QUAD_MAX_WAVES = 32768;  // grid cap of k_dp_quad (more blocks than resident waves: dynamic balance)
 size_t quad_scratch_words(uint32_t max_m, uint32_t max_n, uint32_t njobs, uint32_t grid_cap);
@@ -284,7 +291,7 @@ void gather_cigar2(const uint32_t* cig0, const uint32_t* cig1, const uint64_t* s
 
 void gather_cigar(const uint32_t* cig, const uint64_t* src, const uint32_t* off, const uint32_t* cnt, uint32_t n,
                   uint32_t* out, hipStream_t s);
-void launch_fill_pres(const uint32_t* koff, uint64_t* pres, hipStream_t s);
+void launch_fill_pres(const uint32_t* koff, uint64_t* pres, uint32_t rot, hipStream_t s);
 void launch_fill_koff(uint32_t* koff, const uint32_t* keys, const uint32_t* kstart, uint64_t nkeys, uint32_t total,
                       hipStream_t s);
 

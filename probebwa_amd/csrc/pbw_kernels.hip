@@ -145,6 +145,11 @@ __device__ __forceinline__ uint8_t genome_base(const DevGenome& G, uint64_t gb) 
   return isn ? (uint8_t)'N' : (uint8_t)("ACGT"[c]);
 }
 
+// Rotation of a 30-bit 15-mer code (index of the rotated presence bitmaps).
+__host__ __device__ __forceinline__ uint32_t rotr30(uint32_t x, uint32_t r) {
+  return r ? (((x >> r) | (x << (30 - r))) & 0x3FFFFFFFu) : x;
+}
+
 // ---------------------------------------------------------------------------
 // seed kernel: seeds.rs:91-151
 // ---------------------------------------------------------------------------
@@ -308,8 +313,11 @@ __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
             uint32_t orig = (wc >> sh) & 3u;
             uint32_t nbase = alt < orig ? alt : alt + 1;
             uint32_t ncode = (wc & ~(3u << sh)) | (nbase << sh);
-            // presence bitmap (128 MiB) first: most neighbour codes have empty buckets
-            if ((a.X.pres[ncode >> 6] >> (ncode & 63)) & 1ull) {
+            // presence bitmap first (most neighbour codes have empty buckets), in the
+            // copy whose 1024-bit lines hold this position's neighbours together
+            const uint32_t pi = i >= 10 ? ncode : rotr30(ncode, i >= 5 ? 10u : 20u);
+            const uint64_t* pb = i >= 10 ? a.X.pres : (i >= 5 ? a.X.pres_r10 : a.X.pres_r20);
+            if ((pb[pi >> 6] >> (pi & 63)) & 1ull) {
               loff = a.X.koff[ncode];
               lcnt = a.X.koff[ncode + 1] - loff;
               if (lcnt >= (uint32_t)REPEAT_SKIP) S.nbrep[w] = 1;
@@ -2366,7 +2374,10 @@ __global__ __launch_bounds__(64) void k_dp_tiled(DpArgs a) {
   st.ref = st.qq + ((a.max_m + 15) & ~15u);
   st.go = (int16_t*)(st.ref + ((a.max_n + 15) & ~15u));
   st.runs = (uint32_t*)((uint8_t*)st.go + (((a.max_n * 2) + 15) & ~15u));
-  int32_t* RM = a.rows + (uint64_t)blockIdx.x * 3 * (a.max_n + 1);
+  // the previous row's M, I, D: in LDS after the staging arrays when the launch
+  // gave it room (a.rows null), else in this wave's global scratch
+  int32_t* RM = a.rows ? a.rows + (uint64_t)blockIdx.x * 3 * (a.max_n + 1)
+                       : (int32_t*)(dyn + dp_tiled_lds_bytes(a.max_m, a.max_n));
   int32_t* RI = RM + (a.max_n + 1);
   int32_t* RD = RI + (a.max_n + 1);
   uint64_t* TB = a.tb + (uint64_t)blockIdx.x * (uint64_t)(a.max_m + 1) * a.tiles * 4;
@@ -2539,10 +2550,13 @@ __global__ __launch_bounds__(64) void k_dp_tiled(DpArgs a) {
   if (lane == 0 && a.ctr) atomicAdd(&a.ctr->cells, c_cells);
 }
 
+bool dp_tiled_rows_in_lds(uint32_t max_m, uint32_t max_n) {
+  return dp_tiled_lds_bytes(max_m, max_n) + 12ull * (max_n + 1) <= 48 * 1024;
+}
 void launch_dp_tiled(const DpArgs& a, uint32_t nwaves, hipStream_t s) {
   if (a.njobs == 0) return;
   uint32_t blocks = nwaves < a.njobs ? nwaves : a.njobs;
-  size_t lds = dp_tiled_lds_bytes(a.max_m, a.max_n);
+  size_t lds = dp_tiled_lds_bytes(a.max_m, a.max_n) + (a.rows ? 0 : 12ull * (a.max_n + 1));
   hipLaunchKernelGGL(k_dp_tiled, dim3(blocks), dim3(64), lds, s, a);
 }
 
@@ -3114,16 +3128,18 @@ __global__ void k_fill_koff(uint32_t* koff, const uint32_t* keys, const uint32_t
     koff[c] = lo < nkeys ? kstart[lo] : total;
   }
 }
-// Presence bitmap of the bucket table: one wave per 64 consecutive codes.
-__global__ void k_fill_pres(const uint32_t* koff, uint64_t* pres) {
+// Presence bitmap of the bucket table, bit i = bucket of code rotl30(i, rot)
+// non-empty: one wave per 64 consecutive bits.
+__global__ void k_fill_pres(const uint32_t* koff, uint64_t* pres, uint32_t rot) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < CODE_SPACE; c += stride) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < CODE_SPACE; i += stride) {
+    const uint32_t c = rot ? rotr30((uint32_t)i, 30 - rot) : (uint32_t)i;  // rotl by rot
     const uint64_t m = __ballot(koff[c + 1] > koff[c]);
-    if (lane_id() == 0) pres[c >> 6] = m;
+    if (lane_id() == 0) pres[i >> 6] = m;
   }
 }
-void launch_fill_pres(const uint32_t* koff, uint64_t* pres, hipStream_t s) {
-  hipLaunchKernelGGL(k_fill_pres, dim3(65536), dim3(256), 0, s, koff, pres);
+void launch_fill_pres(const uint32_t* koff, uint64_t* pres, uint32_t rot, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_pres, dim3(65536), dim3(256), 0, s, koff, pres, rot);
 }
 
 void launch_fill_koff(uint32_t* koff, const uint32_t* keys, const uint32_t* kstart, uint64_t nkeys, uint32_t total,
