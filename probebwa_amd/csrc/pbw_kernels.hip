@@ -155,14 +155,22 @@ __host__ __device__ __forceinline__ uint32_t rotr30(uint32_t x, uint32_t r) {
 // ---------------------------------------------------------------------------
 constexpr int SEED_WPB = 4;
 
+// Pending-bucket queue: fewer than 64 positions are carried from one lookup
+// round to the next, so at most 63 buckets wait when a round appends its 64.
+constexpr int SEED_QCAP = 128;
+
 struct SeedLds {
   uint8_t seq[MAX_READ_LEN];
   uint32_t wcode[64], woff[64], wcnt[64], lbase[64];
-  uint32_t r_incl[64], r_off[64];
   uint16_t fpF[64], fpR[64];
   uint8_t wflags[64];  // 1 valid, 2 probed, 4 fullF, 8 fullR, 16 exact-repetitive
   uint8_t nbrep[64];
-  uint8_t r_w[64], r_nb[64];
+  // looked-up buckets holding at least one position, first in first out in
+  // emission order; q_incl is the inclusive prefix of their sizes
+  uint32_t q_incl[SEED_QCAP], q_off[SEED_QCAP], q_fp[SEED_QCAP];  // q_fp: fpF | fpR << 16
+  uint16_t q_pos[SEED_QCAP];                                        // window start in the read
+  uint8_t q_fl[SEED_QCAP];   // 1 neighbour probe, 4 fullF, 8 fullR
+  uint8_t q_hfl[SEED_QCAP];  // prefilter flags of the hits from this bucket
 };
 
 // seeds.rs:276-286
@@ -217,6 +225,45 @@ __device__ __forceinline__ bool seed_filter(const DevGenome& G, uint32_t contig,
   return dist <= (neighbour ? 8 : 10);
 }
 
+// One round of the 64 queued positions from queue position qc on: flank
+// filter, candidate key and compaction of the passing hits in emission order
+// (seeds.rs:120-147, candidates.rs:37-40).
+__device__ __forceinline__ void seed_positions(const SeedArgs& a, const SeedLds& S, uint32_t qn, uint32_t qc,
+                                               uint32_t qtot, uint64_t hbase, uint32_t cap, uint32_t& hcount,
+                                               unsigned long long& c_filt) {
+  const uint32_t e = qc + lane_id();
+  bool pass = false;
+  uint64_t key = 0;
+  uint32_t hfl = 0;
+  if (e < qtot) {
+    int lo = 0, hi = (int)qn - 1;  // first bucket l with q_incl[l] > e
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (S.q_incl[mid] > e) hi = mid;
+      else lo = mid + 1;
+    }
+    const uint32_t excl = lo ? S.q_incl[lo - 1] : 0u;
+    const uint64_t packed = a.X.kpos[(uint64_t)S.q_off[lo] + (e - excl)];
+    const uint32_t contig = (uint32_t)((packed >> 40) & ((1ull << 23) - 1));
+    const uint64_t pos = packed & ((1ull << 40) - 1);
+    const bool rev = (packed >> 63) & 1ull;
+    const uint32_t fl = S.q_fl[lo], fp = S.q_fp[lo];
+    pass = seed_filter(a.G, contig, pos, rev, (fl & 1u) != 0, (uint8_t)fl, (uint16_t)fp, (uint16_t)(fp >> 16));
+    const int64_t rp = (int64_t)S.q_pos[lo];
+    const int64_t diag = rev ? (int64_t)pos + rp : (int64_t)pos - rp;
+    key = ((uint64_t)(contig * 2u + (rev ? 1u : 0u)) << DIAG_BITS) | (uint64_t)(diag + DIAG_BIAS);
+    hfl = S.q_hfl[lo];
+  }
+  const uint64_t m = __ballot(pass);
+  const uint32_t pre = (uint32_t)__popcll(m & lanemask_lt());
+  if (pass && hcount + pre < cap) {
+    a.hits[hbase + hcount + pre] = key;
+    if (a.hflag) a.hflag[hbase + hcount + pre] = (uint8_t)hfl;
+  }
+  hcount += (uint32_t)__popcll(m);
+  c_filt += pass ? 1 : 0;
+}
+
 __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
   __shared__ SeedLds lds_all[SEED_WPB];
   const int wib = threadIdx.x >> 6, lane = lane_id();
@@ -233,6 +280,7 @@ __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
     const uint64_t hbase = a.hoff[li];
     const uint32_t cap = a.hcap ? a.hcap[li] : (uint32_t)HIT_CAP;
     uint32_t hcount = 0;
+    uint32_t qn = 0, qtot = 0, qc = 0;  // queued buckets, their positions, positions consumed
     if (L < (uint32_t)K) {
       if (lane == 0) a.nhits[r] = 0;
       continue;
@@ -326,49 +374,61 @@ __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
           c_look++;
           c_pos += lcnt;
         }
-        uint32_t pincl = wave_incl_scan(lcnt);
+        // append the buckets with positions to the queue in emission order
+        const uint32_t pincl = wave_incl_scan(lcnt);
         const uint32_t PT = (uint32_t)readlane_i32((int32_t)pincl, 63);
-        S.r_incl[lane] = pincl;
-        S.r_off[lane] = loff;
-        S.r_w[lane] = (uint8_t)w;
-        S.r_nb[lane] = (uint8_t)(j != 0);
-        wave_sync();
-        for (uint32_t pb = 0; pb < PT; pb += 64) {
-          const uint32_t e = pb + lane;
-          bool pass = false;
-          uint64_t key = 0;
-          uint32_t hfl = 0;
-          if (e < PT) {
-            int lo = 0, hi = 63;  // first l with r_incl[l] > e
-            while (lo < hi) {
-              int mid = (lo + hi) >> 1;
-              if (S.r_incl[mid] > e) hi = mid;
-              else lo = mid + 1;
-            }
-            const int l = lo;
-            const uint32_t excl = l ? S.r_incl[l - 1] : 0u;
-            const uint64_t packed = a.X.kpos[(uint64_t)S.r_off[l] + (e - excl)];
-            const uint32_t contig = (uint32_t)((packed >> 40) & ((1ull << 23) - 1));
-            const uint64_t pos = packed & ((1ull << 40) - 1);
-            const bool rev = (packed >> 63) & 1ull;
-            const uint32_t ww = S.r_w[l];
-            pass = seed_filter(a.G, contig, pos, rev, S.r_nb[l] != 0, S.wflags[ww], S.fpF[ww], S.fpR[ww]);
-            const int64_t rp = (int64_t)(cb + ww);
-            const int64_t diag = rev ? (int64_t)pos + rp : (int64_t)pos - rp;  // candidates.rs:37-40
-            key = ((uint64_t)(contig * 2u + (rev ? 1u : 0u)) << DIAG_BITS) | (uint64_t)(diag + DIAG_BIAS);
-            // prefilter flags: neighbour probe, bit length of the bucket size
-            hfl = (S.r_nb[l] ? 0x80u : 0u) | (uint32_t)(32 - __builtin_clz((S.r_incl[l] - excl) | 1u));
-          }
-          const uint64_t m = __ballot(pass);
-          const uint32_t pre = (uint32_t)__popcll(m & lanemask_lt());
-          if (pass && hcount + pre < cap) {
-            a.hits[hbase + hcount + pre] = key;
-            if (a.hflag) a.hflag[hbase + hcount + pre] = (uint8_t)hfl;
-          }
-          hcount += (uint32_t)__popcll(m);
-          c_filt += pass ? 1 : 0;
+        const uint64_t qm = __ballot(lcnt != 0);
+        if (lcnt) {
+          const uint32_t qi = qn + (uint32_t)__popcll(qm & lanemask_lt());
+          S.q_incl[qi] = qtot + pincl;
+          S.q_off[qi] = loff;
+          S.q_fp[qi] = (uint32_t)S.fpF[w] | ((uint32_t)S.fpR[w] << 16);
+          S.q_pos[qi] = (uint16_t)(cb + w);
+          S.q_fl[qi] = (uint8_t)((S.wflags[w] & 12u) | (j != 0 ? 1u : 0u));
+          // prefilter flags: neighbour probe, bit length of the bucket size
+          S.q_hfl[qi] = (uint8_t)((j != 0 ? 0x80u : 0u) | (uint32_t)(32 - __builtin_clz(lcnt)));
         }
+        qn += (uint32_t)__popcll(qm);
+        qtot += PT;
         wave_sync();
+        if (qtot - qc >= 64) {
+          do {
+            seed_positions(a, S, qn, qc, qtot, hbase, cap, hcount, c_filt);
+            qc += 64;
+          } while (qtot - qc >= 64);
+          // drop the buckets consumed whole and rebase the prefix sums; fewer
+          // than 64 positions remain, so fewer than 64 buckets move
+          const uint64_t d0 = __ballot(lane < qn && S.q_incl[lane] <= qc);
+          const uint64_t d1 = __ballot(lane + 64 < qn && S.q_incl[lane + 64] <= qc);
+          const uint32_t l0 = (uint32_t)(__popcll(d0) + __popcll(d1));
+          const uint32_t base = l0 ? S.q_incl[l0 - 1] : 0u;
+          const bool mv = l0 + lane < qn;
+          uint32_t vi = 0, vo = 0, vf = 0;
+          uint16_t vp = 0;
+          uint8_t vfl = 0, vh = 0;
+          if (mv) {
+            const uint32_t sidx = l0 + lane;
+            vi = S.q_incl[sidx];
+            vo = S.q_off[sidx];
+            vf = S.q_fp[sidx];
+            vp = S.q_pos[sidx];
+            vfl = S.q_fl[sidx];
+            vh = S.q_hfl[sidx];
+          }
+          wave_sync();
+          if (mv) {
+            S.q_incl[lane] = vi - base;
+            S.q_off[lane] = vo;
+            S.q_fp[lane] = vf;
+            S.q_pos[lane] = vp;
+            S.q_fl[lane] = vfl;
+            S.q_hfl[lane] = vh;
+          }
+          qn -= l0;
+          qtot -= base;
+          qc -= base;
+          wave_sync();
+        }
       }
       if (act) {
         uint8_t rc = 0;
@@ -378,6 +438,8 @@ __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
       }
       wave_sync();
     }
+    // the positions still queued at the end of the read
+    for (; qc < qtot; qc += 64) seed_positions(a, S, qn, qc, qtot, hbase, cap, hcount, c_filt);
     if (lane == 0) a.nhits[r] = hcount;
   }
   // per-wave counter reduction
