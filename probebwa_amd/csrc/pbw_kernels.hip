@@ -1764,16 +1764,15 @@ constexpr int32_t QUAD_GUARD = -(1 << 28);
 constexpr int32_t QUAD_EXCL = -(1 << 29);
 
 struct QuadLayout {  // byte offsets inside one job's LDS area
-  uint32_t tb, go, code, scw, q, qq, ref, stride, tb_words;
+  uint32_t tb, gc, scw, q, qq, ref, stride, tb_words;
 };
 __host__ __device__ inline QuadLayout quad_layout(uint32_t max_m, uint32_t max_n) {
   QuadLayout l;
   const uint32_t blocks = (max_m + 7) / 8 + 1;  // + one block for the CIGAR runs
   l.tb_words = 64 * blocks;  // per job, in global scratch (DpArgs::tbq)
   l.tb = 0;
-  l.go = 0;
-  l.code = l.go + 4 * (max_m + 64);
-  l.scw = l.code + 4 * (max_m + 64);
+  l.gc = 0;
+  l.scw = l.gc + 8 * (max_m + 64);
   l.q = l.scw + 4 * ((max_m > 128 ? max_m : 128) + 4);
   l.qq = l.q + ((max_m + 15) & ~15u);
   l.ref = l.qq + ((max_m + 15) & ~15u);
@@ -1795,6 +1794,9 @@ constexpr int DPP_ROW_SHL2 = 0x102, DPP_ROW_SHL4 = 0x104, DPP_ROW_SHL8 = 0x108;
 
 __device__ __forceinline__ uint32_t tb_push_bit(uint32_t w, int32_t x) {  // (w << 1) | sign(x)
   return __builtin_amdgcn_alignbit(w, (uint32_t)x, 31);
+}
+__device__ __forceinline__ int32_t sub_sat(int32_t x, int32_t y) {  // x - y clamped to int32
+  return __builtin_elementwise_sub_sat(x, y);
 }
 __device__ __forceinline__ int32_t bfe_i32(int32_t x, int32_t off, int32_t w) {
   return __builtin_amdgcn_sbfe(x, (uint32_t)off, (uint32_t)w);
@@ -1825,13 +1827,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   // traceback planes, then CIGAR runs and log-likelihood terms, in this group's
   // global scratch: LDS keeps only the per-row inputs, so more waves fit per CU
   uint32_t* TB = a.tbq + ((size_t)blockIdx.x * 4 + (size_t)g) * Y.tb_words;
-  int32_t* GOA = (int32_t*)(jb + Y.go);    // gap open of window column c at [c - (1 + o)]
-  int32_t* CODA = (int32_t*)(jb + Y.code);  // 6 * (A, C, G, T, N -> 0..4) of the same column
+  // per window column c at [c - (1 + o)]: gap open, then 6 * (A, C, G, T, N -> 0..4)
+  int2* GC = (int2*)(jb + Y.gc);
   int32_t* SCW = (int32_t*)(jb + Y.scw);
   uint8_t* Q = jb + Y.q;
   uint8_t* QQ = jb + Y.qq;
   uint8_t* REF = jb + Y.ref;
   const int32_t ge = a.gap_extend;
+  int32_t gev;  // the same value held per lane
+  asm volatile("v_mov_b32 %0, %1" : "=v"(gev) : "s"(ge));
   unsigned long long c_cells = 0;
   const uint32_t nwaves = gridDim.x;
   const uint32_t nquads = (a.njobs + 3) / 4;
@@ -1934,10 +1938,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
       // window columns c = x + 1 at index c - (1 + o); default for c outside [1, n]
       const int32_t c_lo = 1 + o;
       const int32_t nent = (int32_t)m + 63;
-      for (int32_t e = L; e < nent && m; e += 16) {
-        GOA[e] = -1;
-        CODA[e] = 24;
-      }
+      for (int32_t e = L; e < nent && m; e += 16) GC[e] = make_int2(-1, 24);
       int32_t st[NR];
 #pragma unroll
       for (int u = 0; u < NR; u++) {
@@ -1955,8 +1956,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
             int32_t run = end - st[u] + 1;
             if (run > 65535) run = 65535;
             const uint32_t rb = bb[u];
-            GOA[e] = min(a.gap_open + (run - 1), -1);
-            CODA[e] = rb == 'A' ? 0 : rb == 'C' ? 6 : rb == 'G' ? 12 : rb == 'T' ? 18 : 24;
+            GC[e] = make_int2(min(a.gap_open + (run - 1), -1),
+                              rb == 'A' ? 0 : rb == 'C' ? 6 : rb == 'G' ? 12 : rb == 'T' ? 18 : 24);
           }
         }
       }
@@ -1990,7 +1991,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
       Io[k] = NEG_INF;
       Po[k] = NEG_INF - kge[k];
     }
-    const int32_t kge3 = kge[3] + cp;
     uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
     uint32_t mmax = m;
     int32_t emax = m ? b - h : 0;  // rows i <= band - hint reach columns j <= 0
@@ -2004,16 +2004,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
       int32_t scw, g0, g1, g2, g3, c0, c1, c2, c3;
     };
     auto load_row = [&](RowIn& x, uint32_t i) {  // in bounds for i <= m + 1
-      const uint32_t e = i - 1 + 4 * (uint32_t)L;
+      const int2* gc = GC + (i - 1 + 4 * (uint32_t)L);
       x.scw = SCW[i - 1];
-      x.g0 = GOA[e];
-      x.g1 = GOA[e + 1];
-      x.g2 = GOA[e + 2];
-      x.g3 = GOA[e + 3];
-      x.c0 = CODA[e];
-      x.c1 = CODA[e + 1];
-      x.c2 = CODA[e + 2];
-      x.c3 = CODA[e + 3];
+      const int2 v0 = gc[0], v1 = gc[1], v2 = gc[2], v3 = gc[3];
+      x.g0 = v0.x;
+      x.g1 = v1.x;
+      x.g2 = v2.x;
+      x.g3 = v3.x;
+      x.c0 = v0.y;
+      x.c1 = v1.y;
+      x.c2 = v2.y;
+      x.c3 = v3.y;
     };
     const int32_t cp2 = (L == 15) ? QUAD_GUARD : 0;  // the guard column's vertical neighbour
     // One row of the band.  EDGE: reset cells with j <= 0 (rows i <= band - hint).
@@ -2024,49 +2025,44 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
       const int32_t mu3 = dpp_mov_z<DPP_ROW_SHL1>(Mo[0]);
       const int32_t iu3 = dpp_mov_z<DPP_ROW_SHL1>(Io[0]);
       int32_t M0, M1, M2, M3, I0, I1, I2, I3;
+      // M = max(M, I, D of the diagonal predecessor) + score: the predecessor
+      // whose value is below the maximum sets its plane bit (w0: M, w1: I; the
+      // row flush keeps w0 & w1, "M came from D")
       {
-        const int32_t sc = bfe_i32(R.scw, R.c0, 6);
-        const int32_t fm = Mo[0] + sc, fi = Io[0] + sc, fd = Po[0] + kge[0] + sc;
-        const int32_t tt = max(fm, fi);
-        M0 = max(tt, fd);
+        const int32_t X = max(max(Mo[0], Io[0]), Po[0] + kge[0]);
+        M0 = X + bfe_i32(R.scw, R.c0, 6);
         const int32_t io = Mo[1] + R.g0;
-        I0 = max(io, Io[1] + ge);
-        w0 = tb_push_bit(w0, fm - M0);
-        w1 = tb_push_bit(w1, tt - M0);
-        w2 = tb_push_bit(w2, io - I0);
+        I0 = max(io, Io[1] + gev);
+        w0 = tb_push_bit(w0, sub_sat(Mo[0], X));
+        w1 = tb_push_bit(w1, sub_sat(Io[0], X));
+        w2 = tb_push_bit(w2, sub_sat(io, I0));
       }
       {
-        const int32_t sc = bfe_i32(R.scw, R.c1, 6);
-        const int32_t fm = Mo[1] + sc, fi = Io[1] + sc, fd = Po[1] + kge[1] + sc;
-        const int32_t tt = max(fm, fi);
-        M1 = max(tt, fd);
+        const int32_t X = max(max(Mo[1], Io[1]), Po[1] + kge[1]);
+        M1 = X + bfe_i32(R.scw, R.c1, 6);
         const int32_t io = Mo[2] + R.g1;
-        I1 = max(io, Io[2] + ge);
-        w0 = tb_push_bit(w0, fm - M1);
-        w1 = tb_push_bit(w1, tt - M1);
-        w2 = tb_push_bit(w2, io - I1);
+        I1 = max(io, Io[2] + gev);
+        w0 = tb_push_bit(w0, sub_sat(Mo[1], X));
+        w1 = tb_push_bit(w1, sub_sat(Io[1], X));
+        w2 = tb_push_bit(w2, sub_sat(io, I1));
       }
       {
-        const int32_t sc = bfe_i32(R.scw, R.c2, 6);
-        const int32_t fm = Mo[2] + sc, fi = Io[2] + sc, fd = Po[2] + kge[2] + sc;
-        const int32_t tt = max(fm, fi);
-        M2 = max(tt, fd);
+        const int32_t X = max(max(Mo[2], Io[2]), Po[2] + kge[2]);
+        M2 = X + bfe_i32(R.scw, R.c2, 6);
         const int32_t io = Mo[3] + R.g2;
-        I2 = max(io, Io[3] + ge);
-        w0 = tb_push_bit(w0, fm - M2);
-        w1 = tb_push_bit(w1, tt - M2);
-        w2 = tb_push_bit(w2, io - I2);
+        I2 = max(io, Io[3] + gev);
+        w0 = tb_push_bit(w0, sub_sat(Mo[2], X));
+        w1 = tb_push_bit(w1, sub_sat(Io[2], X));
+        w2 = tb_push_bit(w2, sub_sat(io, I2));
       }
       {
-        const int32_t sc = bfe_i32(R.scw, R.c3, 6);
-        const int32_t fm = Mo[3] + sc + cp, fi = Io[3] + sc + cp, fd = Po[3] + kge3 + sc;
-        const int32_t tt = max(fm, fi);
-        M3 = max(tt, fd);
+        const int32_t X = max(max(Mo[3], Io[3]), Po[3] + kge[3]);
+        M3 = X + bfe_i32(R.scw, R.c3, 6) + cp;
         const int32_t io = mu3 + R.g3 + cp2;
-        I3 = max(io, iu3 + ge + cp2);
-        w0 = tb_push_bit(w0, fm - M3);
-        w1 = tb_push_bit(w1, tt - M3);
-        w2 = tb_push_bit(w2, io - I3);
+        I3 = max(io, iu3 + gev + cp2);
+        w0 = tb_push_bit(w0, sub_sat(Mo[3], X));
+        w1 = tb_push_bit(w1, sub_sat(Io[3], X));
+        w2 = tb_push_bit(w2, sub_sat(io, I3));
       }
       const int32_t jl = (int32_t)i + o + 4 * L;  // j of column k is jl + k
       if (EDGE) {
@@ -2087,10 +2083,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
       x = max(x, dpp_mov<DPP_ROW_SHR8>(INT32_MIN, x));
       const int32_t carry = dpp_mov<DPP_ROW_SHR1>(INT32_MIN, x);
       int32_t P0 = max(L0, carry), P1 = max(L1, carry), P2 = max(L2, carry), P3 = max(L3, carry);
-      w3 = tb_push_bit(w3, B0 - P0);
-      w3 = tb_push_bit(w3, B1 - P1);
-      w3 = tb_push_bit(w3, B2 - P2);
-      w3 = tb_push_bit(w3, B3 - P3);
+      w3 = tb_push_bit(w3, sub_sat(B0, P0));
+      w3 = tb_push_bit(w3, sub_sat(B1, P1));
+      w3 = tb_push_bit(w3, sub_sat(B2, P2));
+      w3 = tb_push_bit(w3, sub_sat(B3, P3));
       if (EDGE) {
         if (jl + 0 <= 0) P0 = NEG_INF - kge[0];
         if (jl + 1 <= 0) P1 = NEG_INF - kge[1];
@@ -2112,7 +2108,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
       if ((i & 7u) == 0) {
         uint32_t* tw = TB + (((i - 1) >> 3) * 4) * 16 + L;
         tw[0] = w0;
-        tw[16] = w1;
+        tw[16] = w0 & w1;
         tw[32] = w2;
         tw[48] = w3;
         w0 = w1 = w2 = w3 = 0;
@@ -2150,7 +2146,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
       const uint32_t sh = 4 * (8 - (m & 7u));
       uint32_t* tw = TB + (((m - 1) >> 3) * 4) * 16 + L;
       tw[0] = w0 << sh;
-      tw[16] = w1 << sh;
+      tw[16] = (w0 & w1) << sh;
       tw[32] = w2 << sh;
       tw[48] = w3 << sh;
     }
@@ -2257,7 +2253,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     //      position, in parallel into the LDS of the per-row inputs (dead after
     //      the rows; it holds 8 max_m bytes): M columns by base match, I columns
     //      ln(1/4).  The table reads of a lane are issued together. ----
-    double* terms = (double*)(jb + Y.go);
+    double* terms = (double*)(jb + Y.gc);
     {
       uint32_t u = 0, qs = 0, rs = 0;
       for (uint32_t qi = L; qi < m; qi += 16) {
