@@ -547,25 +547,27 @@ __device__ void group_ranks_table(uint32_t G, const uint32_t* g_key, const uint3
 }
 
 // Insert slot of sw_find_slot for a table of at most 64 buckets whose
-// occupancy is the bit mask `occ` (wave-uniform).
+// occupancy is the bit mask `occ` (wave-uniform).  A probe group's 16 control
+// positions pos..pos+15 are a 16-bit window of the occupancy: rotated within
+// the nb buckets when nb >= 16 (positions past the end mirror the first
+// buckets), and for nb < 16 the mask followed by empty padding up to 16 and
+// a copy of the mask.  The first empty position of the window is the slot;
+// in a small table it can fall on the copy of an occupied bucket, and the
+// slot is then the first empty position of group 0.
 __device__ __forceinline__ uint32_t sw_find_slot_mask(uint64_t occ, uint32_t nb, uint32_t h) {
   const uint32_t mask = nb - 1;
+  const uint64_t lowm = nb >= 64 ? ~0ull : ((1ull << nb) - 1ull);
+  const uint64_t o = occ & lowm;
+  const uint32_t g0 = ~(uint32_t)o & 0xFFFFu;  // empty positions of group 0
   uint32_t pos = h & mask, stride = 0;
   for (;;) {
-    for (uint32_t b = 0; b < 16; b++) {
-      const uint32_t i = pos + b;
-      bool empty;
-      if (i < nb) empty = !((occ >> i) & 1ull);
-      else if (nb < 16) empty = i < 16 ? true : !((occ >> (i - 16)) & 1ull);
-      else empty = !((occ >> (i - nb)) & 1ull);
-      if (empty) {
-        const uint32_t index = (pos + b) & mask;
-        if ((occ >> index) & 1ull) {
-          for (uint32_t c = 0; c < 16; c++)
-            if (c >= nb || !((occ >> c) & 1ull)) return c;
-        }
-        return index;
-      }
+    uint32_t win;
+    if (nb >= 16) win = (uint32_t)((pos ? (((o >> pos) | (o << (nb - pos))) & lowm) : o) & 0xFFFFull);
+    else win = (uint32_t)(((o | (o << 16)) >> pos) & 0xFFFFull);
+    const uint32_t empty = ~win & 0xFFFFu;
+    if (empty) {
+      const uint32_t index = (pos + (uint32_t)__builtin_ctz(empty)) & mask;
+      return ((o >> index) & 1ull) ? (uint32_t)__builtin_ctz(g0) : index;
     }
     stride += 16;
     pos = (pos + stride) & mask;
@@ -764,11 +766,82 @@ struct ClusterMem {
 constexpr uint32_t CLUSTER_STAGE = 1024;
 constexpr uint32_t CLUSTER_GT = 1024, CLUSTER_GT_MAX = 512;  // table slots, groups handled
 
+// The same network for p2 <= 64 * E keys held by one wave in registers:
+// element e * 64 + lane sits in slot e of lane `lane`; partners at distance
+// j < 64 are exchanged across lanes, at j >= 64 between slots of a lane.
+// Every compare-exchange takes the same decision as team_bitonic's, ties
+// included, so the (key, val) arrangement is the same.
+template <int E>
+__device__ void wave_bitonic_reg(uint64_t* key, uint32_t* val, uint32_t p2) {
+  const uint32_t lane = lane_id();
+  uint64_t kr[E];
+  uint32_t vr[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const uint32_t i = (uint32_t)e * 64u + lane;
+    kr[e] = i < p2 ? key[i] : ~0ull;
+    vr[e] = (val && i < p2) ? val[i] : 0u;
+  }
+  for (uint32_t k = 2; k <= p2; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      if (j >= 64) {
+        const uint32_t js = j >> 6;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          const uint32_t ep = (uint32_t)e ^ js;
+          if ((uint32_t)e < ep && ep < (uint32_t)E) {
+            const bool up = (((uint32_t)e * 64u) & k) == 0;
+            const uint64_t x = kr[e], y = kr[ep];
+            if ((x > y) == up) {
+              kr[e] = y;
+              kr[ep] = x;
+              const uint32_t t = vr[e];
+              vr[e] = vr[ep];
+              vr[ep] = t;
+            }
+          }
+        }
+      } else {
+        const bool lower = (lane & j) == 0;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          const uint32_t i = (uint32_t)e * 64u + lane;
+          const uint64_t pk = (uint64_t)__shfl_xor((long long)kr[e], (int)j, 64);
+          const uint32_t pv = (uint32_t)__shfl_xor((int)vr[e], (int)j, 64);
+          const bool up = (i & k) == 0;
+          // the pair (x at the lower index, y at the upper) swaps when (x > y) == up
+          const bool sw = lower ? ((kr[e] > pk) == up) : ((pk > kr[e]) == up);
+          if (sw) {
+            kr[e] = pk;
+            vr[e] = pv;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const uint32_t i = (uint32_t)e * 64u + lane;
+    if (i < p2) {
+      key[i] = kr[e];
+      if (val) val[i] = vr[e];
+    }
+  }
+  wave_sync();
+}
+
 template <int T>
 __device__ __forceinline__ void cluster_sort(const ClusterMem& M, uint64_t* key, uint32_t* val, uint32_t p2,
                                              uint32_t tid) {
-  if constexpr (T == 64) team_bitonic<T>(key, val, p2, tid);
-  else team_bitonic_staged<T, CLUSTER_STAGE>(key, val, p2, tid, M.stage_key, M.stage_val);
+  if constexpr (T == 64) {
+    if (p2 <= 64) wave_bitonic_reg<1>(key, val, p2);
+    else if (p2 <= 128) wave_bitonic_reg<2>(key, val, p2);
+    else if (p2 <= 256) wave_bitonic_reg<4>(key, val, p2);
+    else if (p2 <= 512) wave_bitonic_reg<8>(key, val, p2);
+    else team_bitonic<T>(key, val, p2, tid);
+  } else {
+    team_bitonic_staged<T, CLUSTER_STAGE>(key, val, p2, tid, M.stage_key, M.stage_val);
+  }
 }
 
 __device__ __forceinline__ int64_t hit_diag(uint64_t k) {
