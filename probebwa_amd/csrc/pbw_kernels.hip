@@ -163,6 +163,7 @@ struct SeedLds {
   uint8_t seq[MAX_READ_LEN];
   uint32_t wcode[64], woff[64], wcnt[64], lbase[64];
   uint16_t fpF[64], fpR[64];
+  uint64_t wmask[64];  // per window: present neighbours (bit 3i + alt)
   uint8_t wflags[64];  // 1 valid, 2 probed, 4 fullF, 8 fullR, 16 exact-repetitive
   uint8_t nbrep[64];
   // looked-up buckets holding at least one position, first in first out in
@@ -223,6 +224,61 @@ __device__ __forceinline__ bool seed_filter(const DevGenome& G, uint32_t contig,
 #pragma unroll
   for (int c = 0; c < 4; c++) dist += abs((int)((rf >> (4 * c)) & 15) - cnt[c]);
   return dist <= (neighbour ? 8 : 10);
+}
+
+// Which of the 45 single-substitution neighbours of a 15-mer have a non-empty
+// bucket: bit 3i + alt for position i and the alt-th other base, the lookup
+// order of seeds.rs:291-309.  The five positions of a group (10..14, 5..9,
+// 0..4) differ in bits 0..9 of the presence copy rotated by 0, 10 or 20 bits,
+// so a group reads its code's word (bits 0..5) and three words for each of
+// the two positions in bits 6..9: 21 words per window, all issued together.
+__device__ __forceinline__ uint64_t neighbour_presence(const DevIndex& X, uint32_t code) {
+  uint64_t mask = 0;
+#pragma unroll
+  for (int g = 0; g < 3; g++) {
+    const uint32_t rg = 10u * (uint32_t)g;
+    const uint64_t* pb = g == 0 ? X.pres : (g == 1 ? X.pres_r10 : X.pres_r20);
+    const uint32_t base = rotr30(code, rg), W = base >> 6, off = base & 63u;
+    const uint32_t o3 = (base >> 6) & 3u, o4 = (base >> 8) & 3u;
+    const uint64_t w0 = pb[W];
+    uint64_t w3[3], w4[3];
+#pragma unroll
+    for (uint32_t alt = 0; alt < 3; alt++) {
+      w3[alt] = pb[(W & ~3u) | (alt < o3 ? alt : alt + 1)];
+      w4[alt] = pb[(W & ~12u) | ((alt < o4 ? alt : alt + 1) << 2)];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 5; k++) {
+      const uint32_t i = 14u - rg / 2u - k, ls = 2u * k, orig = (base >> ls) & 3u;
+#pragma unroll
+      for (uint32_t alt = 0; alt < 3; alt++) {
+        const uint32_t nb = alt < orig ? alt : alt + 1;
+        uint64_t bit;
+        if (k < 3) bit = (w0 >> ((off & ~(3u << ls)) | (nb << ls))) & 1ull;
+        else if (k == 3) bit = (w3[alt] >> off) & 1ull;
+        else bit = (w4[alt] >> off) & 1ull;
+        mask |= bit << (3u * i + alt);
+      }
+    }
+  }
+  return mask;
+}
+
+// Position of the k-th (from 0) set bit of m; m has more than k set bits.
+__device__ __forceinline__ uint32_t select_set_bit(uint64_t m, uint32_t k) {
+  uint32_t pos = 0, c;
+  c = (uint32_t)__popcll(m & 0xFFFFFFFFull);
+  if (k >= c) { k -= c; m >>= 32; pos += 32; }
+  c = (uint32_t)__popcll(m & 0xFFFFull);
+  if (k >= c) { k -= c; m >>= 16; pos += 16; }
+  c = (uint32_t)__popcll(m & 0xFFull);
+  if (k >= c) { k -= c; m >>= 8; pos += 8; }
+  c = (uint32_t)__popcll(m & 0xFull);
+  if (k >= c) { k -= c; m >>= 4; pos += 4; }
+  c = (uint32_t)__popcll(m & 0x3ull);
+  if (k >= c) { k -= c; m >>= 2; pos += 2; }
+  if (k >= (uint32_t)(m & 1ull)) pos += 1;
+  return pos;
 }
 
 // One round of the 64 queued positions from queue position qc on: flank
@@ -292,6 +348,7 @@ __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
       const uint32_t p = cb + lane;
       const bool act = p < nwin;
       uint32_t code = 0, off = 0, cnt = 0, nlook = 0;
+      uint64_t nbm = 0;
       uint8_t fl = 0;
       uint16_t fpF = 0, fpR = 0;
       if (act) {
@@ -307,7 +364,11 @@ __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
           cnt = a.X.koff[code + 1] - off;
           if (cnt >= (uint32_t)REPEAT_SKIP) fl |= 16;
           else if (neighbour_probed(p, L)) fl |= 2;
-          nlook = 1 + ((fl & 2) ? 45u : 0u);
+          // the reference looks up the window and, when probed, its 45 neighbours;
+          // only the neighbours with a non-empty bucket are enumerated below
+          c_look += 1 + ((fl & 2) ? 45u : 0u);
+          if (fl & 2) nbm = neighbour_presence(a.X, code);
+          nlook = 1 + (uint32_t)__popcll(nbm);
         }
         // read-side fingerprints (seeds.rs:181-211) for complete flanks only
         if (p >= 4 && p + 23 <= L) {
@@ -329,6 +390,7 @@ __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
       S.woff[lane] = off;
       S.wcnt[lane] = cnt;
       S.wflags[lane] = fl;
+      S.wmask[lane] = nbm;
       S.fpF[lane] = fpF;
       S.fpR[lane] = fpR;
       S.nbrep[lane] = 0;
@@ -336,8 +398,9 @@ __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
       S.lbase[lane] = incl - nlook;
       const uint32_t T = (uint32_t)readlane_i32((int32_t)incl, 63);
       wave_sync();
-      // lookups of this chunk in emission order: window, exact, then the 45
-      // neighbours in (position, base) order (seeds.rs:107-147, :291-309)
+      // lookups of this chunk in emission order: window, exact, then the
+      // neighbours with a non-empty bucket in (position, base) order
+      // (seeds.rs:107-147, :291-309)
       for (uint32_t ub = 0; ub < T; ub += 64) {
         const uint32_t u = ub + lane;
         const bool act2 = u < T;
@@ -355,23 +418,17 @@ __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
             loff = S.woff[w];
             lcnt = S.wcnt[w];
           } else {
-            uint32_t wc = S.wcode[w];
-            uint32_t i = (j - 1) / 3, alt = (j - 1) % 3;
-            uint32_t sh = 2 * (14 - i);
-            uint32_t orig = (wc >> sh) & 3u;
-            uint32_t nbase = alt < orig ? alt : alt + 1;
-            uint32_t ncode = (wc & ~(3u << sh)) | (nbase << sh);
-            // presence bitmap first (most neighbour codes have empty buckets), in the
-            // copy whose 1024-bit lines hold this position's neighbours together
-            const uint32_t pi = i >= 10 ? ncode : rotr30(ncode, i >= 5 ? 10u : 20u);
-            const uint64_t* pb = i >= 10 ? a.X.pres : (i >= 5 ? a.X.pres_r10 : a.X.pres_r20);
-            if ((pb[pi >> 6] >> (pi & 63)) & 1ull) {
-              loff = a.X.koff[ncode];
-              lcnt = a.X.koff[ncode + 1] - loff;
-              if (lcnt >= (uint32_t)REPEAT_SKIP) S.nbrep[w] = 1;
-            }
+            const uint32_t wc = S.wcode[w];
+            const uint32_t t = select_set_bit(S.wmask[w], j - 1);  // neighbour 3i + alt
+            const uint32_t i = t / 3, alt = t % 3;
+            const uint32_t sh = 2 * (14 - i);
+            const uint32_t orig = (wc >> sh) & 3u;
+            const uint32_t nbase = alt < orig ? alt : alt + 1;
+            const uint32_t ncode = (wc & ~(3u << sh)) | (nbase << sh);
+            loff = a.X.koff[ncode];
+            lcnt = a.X.koff[ncode + 1] - loff;
+            if (lcnt >= (uint32_t)REPEAT_SKIP) S.nbrep[w] = 1;
           }
-          c_look++;
           c_pos += lcnt;
         }
         // append the buckets with positions to the queue in emission order
