@@ -234,8 +234,8 @@ __device__ __forceinline__ bool seed_filter(const DevGenome& G, uint32_t contig,
 // the two positions in bits 6..9: 21 words per window, all issued together.
 __device__ __forceinline__ uint64_t neighbour_presence(const DevIndex& X, uint32_t code) {
   uint64_t mask = 0;
-#pragma unroll
-  for (int g = 0; g < 3; g++) {
+#pragma unroll 1
+  for (int g = 0; g < 3; g++) {  // one group's seven words in flight at a time
     const uint32_t rg = 10u * (uint32_t)g;
     const uint64_t* pb = g == 0 ? X.pres : (g == 1 ? X.pres_r10 : X.pres_r20);
     const uint32_t base = rotr30(code, rg), W = base >> 6, off = base & 63u;
