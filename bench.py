@@ -183,11 +183,24 @@ def main():
     ap.add_argument("--train-reads", type=int, default=None)
     ap.add_argument("--label", choices=["loglk", "score"], default="loglk")
     ap.add_argument("--margin", type=float, default=40.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device to device three times (the known-size run that scales "
+                         "rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print("copied %d bytes x3" % (4 << 28), flush=True)
+        return
 
     if args.workload == "probe_pe_2mbp":
         bench_pe(args)
