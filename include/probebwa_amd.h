@@ -176,6 +176,10 @@ typedef struct {
                            whose per-job LDS tile is sized for split_len bases */
   uint32_t long_waves;  /* reads longer than split_len: 0 = four jobs per wave, 1 = one job
                            per wave (64 lanes per alignment) */
+  uint32_t dp_pack16;   /* 1: the four-jobs-per-wave launches run eight jobs per wave, two per
+                           register in saturating 16-bit halves, when the gap penalties allow
+                           (gap extend >= -4, gap open >= -400); jobs whose best score is below
+                           -18000 are re-run in 32 bits */
 } pbw_tune;
 int pbw_tune_get(const pbw_ctx* ctx, pbw_tune* out);
 int pbw_tune_set(pbw_ctx* ctx, const pbw_tune* t);
@@ -188,7 +192,7 @@ typedef struct {
   double ms[PBW_TUNE_MAX_TRIALS];
 } pbw_autotune_report;
 /* Maps reads [begin, end) of `reads` under candidate settings (coordinate descent
- * over dp_grid, then split_len x long_waves, then batch_reads; each timed twice,
+ * over dp_grid, dp_pack16, then split_len x long_waves, then batch_reads; each timed twice,
  * the faster run counts), keeps the fastest in the context and reports every trial. */
 int pbw_autotune(pbw_ctx* ctx, const pbw_device_reads* reads, uint64_t begin, uint64_t end,
                  pbw_autotune_report* out);

@@ -833,6 +833,7 @@ static void batch_dp(pbw_ctx* c, Batch& b, const uint32_t* jlist, uint32_t nlist
       q.flag_list = c->d_flag_a.p;
       q.flag_n = c->d_flag_n.p;
       q.quad_grid = tn.dp_grid;
+      q.pack16 = tn.dp_pack16;
       return q;
     };
     DpArgs ba = da;
@@ -1638,7 +1639,7 @@ int pbw_tune_get(const pbw_ctx* c, pbw_tune* out) {
 }
 
 int pbw_tune_set(pbw_ctx* c, const pbw_tune* t) {
-  if (!c || !t || t->long_waves > 1 || t->split_len > (uint32_t)MAX_READ_LEN) return PBW_ERR_ARG;
+  if (!c || !t || t->long_waves > 1 || t->dp_pack16 > 1 || t->split_len > (uint32_t)MAX_READ_LEN) return PBW_ERR_ARG;
   c->tune = *t;
   if (t->batch_reads) c->opt.batch_reads = t->batch_reads;
   return PBW_OK;
@@ -1687,15 +1688,16 @@ int pbw_autotune(pbw_ctx* c, const pbw_device_reads* d, uint64_t begin, uint64_t
     };
     const pbw_tune base0 = cur;
     for (uint32_t g : {8192u, 16384u, 32768u, 65536u})
-      if (g != base0.dp_grid) consider(pbw_tune{cur.batch_reads, g, cur.split_len, cur.long_waves});
+      if (g != base0.dp_grid) consider(pbw_tune{cur.batch_reads, g, cur.split_len, cur.long_waves, cur.dp_pack16});
+    consider(pbw_tune{cur.batch_reads, cur.dp_grid, cur.split_len, cur.long_waves, 1u - cur.dp_pack16});
     const pbw_tune base1 = cur;
     for (uint32_t sl : {96u, 128u, 160u, 192u, 256u})
       if (sl < max_len)
-        for (uint32_t lw : {0u, 1u}) consider(pbw_tune{base1.batch_reads, base1.dp_grid, sl, lw});
+        for (uint32_t lw : {0u, 1u}) consider(pbw_tune{base1.batch_reads, base1.dp_grid, sl, lw, base1.dp_pack16});
     const pbw_tune base2 = cur;
     for (uint32_t bs : {65536u, 131072u, 262144u, 524288u})
       if (bs != base2.batch_reads && bs <= std::max<uint64_t>(end - begin, 65536))
-        consider(pbw_tune{bs, base2.dp_grid, base2.split_len, base2.long_waves});
+        consider(pbw_tune{bs, base2.dp_grid, base2.split_len, base2.long_waves, base2.dp_pack16});
     c->tune = cur;
     c->opt.batch_reads = cur.batch_reads;
     c->stats = saved;

@@ -186,6 +186,7 @@ struct DpArgs {
   uint32_t wpb;            // waves per block (banded kernel; set by the launcher)
   uint32_t* tbq;           // k_dp_quad traceback scratch: quad_scratch_words() words
   uint32_t quad_grid;      // k_dp_quad grid cap (0: QUAD_MAX_WAVES)
+  uint32_t pack16;         // 1: eight jobs per wave in 16-bit halves (k_dp_pair16) where allowed
 };
 void launch_dp_band(const DpArgs& a, hipStream_t s);
 void launch_dp_tiled(const DpArgs& a, uint32_t nwaves, hipStream_t s);

@@ -2445,6 +2445,635 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   if (lane == 0 && a.ctr) atomicAdd(&a.ctr->cells, c_cells);
 }
 
+// ---------------------------------------------------------------------------
+// DP, eight jobs per wave in 16-bit halves (k_dp_pair16): the k_dp_quad
+// recurrence with two jobs per 16-lane DPP row, job A in the low and job B
+// in the high half of every 32-bit register, so one v_pk_* instruction
+// advances two cells and one DPP move serves both scans.
+//  - Saturating 16-bit arithmetic.  A cell on the optimal path whose score is
+//    s has every argmax predecessor >= s - 20 per row, so for a job whose best
+//    score is >= PAIR16_MIN_SCORE all those values stay >= -25,000 (reads of
+//    at most 320 bases) and are exact, while every value derived from a
+//    saturated one stays <= -32,768 + 320 * 20 = -26,368 below them: the best
+//    score, its endpoint and every traceback decision on the path equal the
+//    32-bit kernel's.  A job whose best is lower is flagged to the
+//    one-job-per-wave kernel.  The launcher takes this kernel only when
+//    -4 <= gap extend and gap open >= -400, which bound the shifts of the D
+//    scan's representation.
+//  - Plane bits: the signs of both halves of a saturating difference, pushed
+//    into one word per plane (bit 0 and bit 16); every 4 rows (16 pushes) the
+//    words are split per job into 16-bit plane fields, 8 bytes per lane and
+//    4-row block.  The walk joins two blocks into the 32-bit 8-row words of
+//    k_dp_quad and runs k_dp_quad's walker unchanged.
+//  - Walk, log-likelihood terms, ordered sum and outputs per job as in
+//    k_dp_quad; lane h of a row walks and sums its job h, so eight lanes of
+//    the wave do the serial work.
+// ---------------------------------------------------------------------------
+constexpr int32_t PAIR16_MIN_SCORE = -18000;
+
+// The shapes k_dp_quad runs (the rest is flagged to the one-job-per-wave kernel).
+__device__ __forceinline__ bool quad_regular(const DpArgs& a, const DpJob& job, uint32_t m, uint32_t n) {
+  const int32_t h = job.hint, b = job.band;
+  return m >= 1 && n >= 1 && m <= a.max_m && n <= a.max_n && n <= (uint32_t)QUAD_MAX_N && b >= 0 && b <= 31 &&
+         !job.explicit_ref && (int64_t)h + b >= 0 && (int64_t)m + h - b <= (int64_t)n;
+}
+constexpr uint32_t PK_NEG = 0x80008000u;  // -32768 in both halves
+
+typedef short s16x2_t __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pk_add(uint32_t x, uint32_t y) {
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_add_sat(__builtin_bit_cast(s16x2_t, x), __builtin_bit_cast(s16x2_t, y)));
+}
+__device__ __forceinline__ uint32_t pk_sub(uint32_t x, uint32_t y) {
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_sub_sat(__builtin_bit_cast(s16x2_t, x), __builtin_bit_cast(s16x2_t, y)));
+}
+__device__ __forceinline__ uint32_t pk_max(uint32_t x, uint32_t y) {
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2_t, x), __builtin_bit_cast(s16x2_t, y)));
+}
+// (w << 1) with the sign of d's low half at bit 0 and of its high half at bit 16
+__device__ __forceinline__ uint32_t pk_push(uint32_t w, uint32_t d) {
+  const uint32_t sg = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2_t, d) >> (unsigned short)15);
+  return (w << 1) | sg;
+}
+__device__ __forceinline__ uint32_t pk_make(int32_t lo, int32_t hi) {
+  return ((uint32_t)lo & 0xFFFFu) | ((uint32_t)hi << 16);
+}
+__device__ __forceinline__ int32_t pk_lo(uint32_t v) { return (int32_t)(int16_t)(v & 0xFFFFu); }
+__device__ __forceinline__ int32_t pk_hi(uint32_t v) { return (int32_t)(int16_t)(v >> 16); }
+
+struct Pair16Layout {  // byte offsets inside one row's (two jobs') LDS area
+  uint32_t gc, scw, q, qq, ref, keys, stride, tb_words;
+  uint32_t scw_len, q_len, ref_len;
+};
+__host__ __device__ inline Pair16Layout pair16_layout(uint32_t max_m, uint32_t max_n) {
+  Pair16Layout l;
+  const uint32_t blocks = (max_m + 3) / 4 + 2;  // 4-row blocks + two for the CIGAR runs
+  l.tb_words = 32 * blocks;                     // per job, in global scratch (DpArgs::tbq)
+  l.gc = 0;                                     // (max_m + 64) x {gap open A | B, code A | code B << 8}
+  l.scw_len = 4 * ((max_m > 128 ? max_m : 128) + 4);
+  l.scw = l.gc + 8 * (max_m + 64);              // score words of job A, then of job B
+  l.q_len = (max_m + 15) & ~15u;
+  l.q = l.scw + 2 * l.scw_len;                  // Q A, Q B
+  l.qq = l.q + 2 * l.q_len;                     // QQ A, QQ B
+  l.ref_len = (max_n + 15) & ~15u;
+  l.ref = l.qq + 2 * l.q_len;                   // REF A, REF B
+  l.keys = (l.ref + 2 * l.ref_len + 7) & ~7u;    // 16 endpoint keys of job A, then of job B
+  l.stride = (l.keys + 256 + 15) & ~15u;
+  return l;
+}
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_dp_pair16(DpArgs a) {
+  extern __shared__ __align__(16) uint8_t dyn[];
+  const int lane = lane_id();
+  const int g = lane >> 4, L = lane & 15;
+  const Pair16Layout Y = pair16_layout(a.max_m, a.max_n);
+  uint8_t* jb = dyn + (size_t)g * Y.stride;
+  uint2* GC = (uint2*)(jb + Y.gc);
+  const int32_t ge = a.gap_extend;
+  int32_t gev;  // the same value held per lane
+  asm volatile("v_mov_b32 %0, %1" : "=v"(gev) : "s"(ge));
+  const uint32_t ge_pk = pk_make(gev, gev);
+  unsigned long long c_cells = 0;
+  const uint32_t nwaves = gridDim.x;
+  const uint32_t noct = (a.njobs + 7) / 8;
+  for (uint32_t wq = blockIdx.x; wq < noct; wq += nwaves) {
+    // ---- the row's two jobs and their shapes ----
+    uint32_t jidv[2], mv[2], qlenv[2], nv[2];
+    uint64_t sov[2], qov[2], gbv[2];
+    int32_t hv[2], bv[2], ov[2], sv[2], Wv[2];
+    bool rcv[2];
+#pragma unroll
+    for (int hh = 0; hh < 2; hh++) {
+      const uint32_t t = wq * 8 + (uint32_t)g * 2 + (uint32_t)hh;
+      bool valid = t < a.njobs;
+      uint32_t jid = 0;
+      if (valid) {
+        jid = a.jlist[t];
+        if (a.job_valid && (jid % TOP_CANDIDATES) >= a.job_valid[jid / TOP_CANDIDATES]) valid = false;
+      }
+      DpJob job{};
+      uint32_t m = 0, qlen = 0, n = 0;
+      uint64_t so = 0, qo = 0;
+      if (valid) {
+        job = a.jobs[jid];
+        so = a.R.seq_off[job.read];
+        qo = a.R.qual_off[job.read];
+        m = (uint32_t)(a.R.seq_off[job.read + 1] - so);
+        qlen = (uint32_t)(a.R.qual_off[job.read + 1] - qo);
+        n = job.n;
+        if (!quad_regular(a, job, m, n)) {
+          if (L == 0) dp_flag(a, jid);
+          valid = false;
+        }
+      }
+      if (!valid) {
+        m = 0;
+        n = 0;
+        job.hint = 0;
+        job.band = 0;
+      }
+      jidv[hh] = jid;
+      mv[hh] = m;
+      qlenv[hh] = qlen;
+      nv[hh] = n;
+      sov[hh] = so;
+      qov[hh] = qo;
+      gbv[hh] = job.ref_gb;
+      hv[hh] = job.hint;
+      bv[hh] = job.band;
+      rcv[hh] = job.rc != 0;
+      Wv[hh] = 2 * job.band + 1;
+      sv[hh] = 63 - Wv[hh];
+      ov[hh] = job.hint + job.band - 62;
+    }
+    // ---- staging, one job after the other with the row's 16 lanes ----
+    for (int32_t e = L; e < (int32_t)max(mv[0], mv[1]) + 63; e += 16) GC[e] = make_uint2(0xFFFFFFFFu, 24u | (24u << 8));
+    wave_sync();
+#pragma unroll
+    for (int hh = 0; hh < 2; hh++) {
+      const uint32_t m = mv[hh], qlen = qlenv[hh], n = nv[hh];
+      const uint64_t so = sov[hh], qo = qov[hh];
+      const bool rc = rcv[hh];
+      const int32_t o = ov[hh];
+      int32_t* SCW = (int32_t*)(jb + Y.scw + hh * Y.scw_len);
+      uint8_t* Q = jb + Y.q + hh * Y.q_len;
+      uint8_t* QQ = jb + Y.qq + hh * Y.q_len;
+      uint8_t* REF = jb + Y.ref + hh * Y.ref_len;
+      const uint32_t ql = qlen < m ? qlen : m;
+      for (uint32_t x = L; x < m; x += 16) Q[x] = rc ? comp_base(a.R.seq[so + m - 1 - x]) : a.R.seq[so + x];
+      for (uint32_t x = L; x < ql; x += 16) QQ[x] = rc ? a.R.qual[qo + qlen - 1 - x] : a.R.qual[qo + x];
+      uint64_t* gw = (uint64_t*)SCW;
+      const uint64_t gb0 = gbv[hh];
+      const uint32_t nw = m ? (uint32_t)(((gb0 & 31) + n + 31) >> 5) : 0;
+      const uint32_t nwm = m ? (uint32_t)(((gb0 & 63) + n + 63) >> 6) : 0;
+      for (uint32_t x = L; x < nw; x += 16) gw[x] = a.G.g2b[(gb0 >> 5) + x];
+      for (uint32_t x = L; x < nwm; x += 16) gw[nw + x] = a.G.nmask[(gb0 >> 6) + x];
+      wave_sync();
+      for (uint32_t x = L; x < n && m; x += 16) {
+        const uint64_t p = (gb0 & 31) + x, pm = (gb0 & 63) + x;
+        const uint32_t c2 = (uint32_t)(gw[p >> 5] >> ((p & 31) * 2)) & 3u;
+        const bool isn = (gw[nw + (pm >> 6)] >> (pm & 63)) & 1ull;
+        REF[x] = isn ? (uint8_t)'N' : (uint8_t)("ACGT"[c2]);
+      }
+      wave_sync();
+      constexpr int NR = QUAD_MAX_N / 16;
+      // homopolymer run of every column (smith_waterman.rs:348-363), as in k_dp_quad
+      {
+        const uint32_t seg = (n + 15) / 16;
+        const uint32_t x0 = min(n, L * seg), cnt = min(n, x0 + seg) - x0;
+        uint32_t bb[NR + 1];
+        const uint32_t prevb = (x0 > 0 && x0 < n) ? REF[x0 - 1] : 0x100u;
+#pragma unroll
+        for (int u = 0; u <= NR; u++) {
+          const uint32_t x = x0 + u;
+          const uint32_t v = REF[n ? min(x, n - 1) : 0];
+          bb[u] = x < n ? v : 0x101u;
+        }
+        int32_t lb = -1, fe = INT32_MAX;
+#pragma unroll
+        for (int u = NR - 1; u >= 0; u--) {
+          if ((uint32_t)u < cnt) {
+            if (bb[u] != bb[u + 1]) fe = (int32_t)(x0 + u);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < NR; u++) {
+          if ((uint32_t)u < cnt) {
+            if (bb[u] != (u == 0 ? prevb : bb[u - 1])) lb = (int32_t)(x0 + u);
+          }
+        }
+        int32_t pm = lb;
+        pm = max(pm, dpp_mov<DPP_ROW_SHR1>(-1, pm));
+        pm = max(pm, dpp_mov<DPP_ROW_SHR2>(-1, pm));
+        pm = max(pm, dpp_mov<DPP_ROW_SHR4>(-1, pm));
+        pm = max(pm, dpp_mov<DPP_ROW_SHR8>(-1, pm));
+        int32_t start = dpp_mov<DPP_ROW_SHR1>(-1, pm);
+        int32_t sm = fe;
+        sm = min(sm, dpp_mov<DPP_ROW_SHL1>(INT32_MAX, sm));
+        sm = min(sm, dpp_mov<DPP_ROW_SHL2>(INT32_MAX, sm));
+        sm = min(sm, dpp_mov<DPP_ROW_SHL4>(INT32_MAX, sm));
+        sm = min(sm, dpp_mov<DPP_ROW_SHL8>(INT32_MAX, sm));
+        int32_t end = dpp_mov<DPP_ROW_SHL1>(INT32_MAX, sm);
+        const int32_t c_lo = 1 + o;
+        const int32_t nent = (int32_t)m + 63;
+        int32_t st[NR];
+#pragma unroll
+        for (int u = 0; u < NR; u++) {
+          if (bb[u] != (u == 0 ? prevb : bb[u - 1])) start = (int32_t)(x0 + u);
+          st[u] = start;
+        }
+        int16_t* gch = (int16_t*)GC;  // gap-open halves: [4e + hh]
+        uint8_t* gcc = (uint8_t*)GC;  // code bytes: [8e + 4 + hh]
+#pragma unroll
+        for (int u = NR - 1; u >= 0; u--) {
+          const int32_t x = (int32_t)x0 + u;
+          if ((uint32_t)u < cnt) {
+            if (bb[u] != bb[u + 1]) end = x;
+            const int32_t e = x + 1 - c_lo;
+            if (e >= 0 && e < nent) {
+              int32_t run = end - st[u] + 1;
+              if (run > 65535) run = 65535;
+              const uint32_t rb = bb[u];
+              gch[4 * e + hh] = (int16_t)min(a.gap_open + (run - 1), -1);
+              gcc[8 * e + 4 + hh] = (uint8_t)(rb == 'A' ? 0 : rb == 'C' ? 6 : rb == 'G' ? 12 : rb == 'T' ? 18 : 24);
+            }
+          }
+        }
+      }
+      wave_sync();
+      // per-row score word (6-bit fields for reference A, C, G, T, N)
+      for (uint32_t x = L; x < m; x += 16) {
+        const uint8_t rb = Q[x];
+        const uint32_t qv = x < qlen ? QQ[x] : 30u;
+        const int32_t mis = -(int32_t)min(max(qv, 6u), 30u);
+        uint32_t w = 0;
+        w |= (uint32_t)((rb == 'A' ? MATCH_SCORE : mis) & 63) << 0;
+        w |= (uint32_t)((rb == 'C' ? MATCH_SCORE : mis) & 63) << 6;
+        w |= (uint32_t)((rb == 'G' ? MATCH_SCORE : mis) & 63) << 12;
+        w |= (uint32_t)((rb == 'T' ? MATCH_SCORE : mis) & 63) << 18;
+        w |= (uint32_t)((rb == 'N' ? MATCH_SCORE : mis) & 63) << 24;
+        SCW[x] = (int32_t)w;
+      }
+      wave_sync();
+    }
+    const int32_t* SCWA = (const int32_t*)(jb + Y.scw);
+    const int32_t* SCWB = (const int32_t*)(jb + Y.scw + Y.scw_len);
+    // ---- row 0 and per-column constants ----
+    uint32_t Mo[4], Io[4], Po[4], kge[4], cst[4];
+    const uint32_t cp = (L == 15) ? PK_NEG : 0u;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int32_t d = 4 * L + k;
+      kge[k] = pk_make(d * gev, d * gev);
+      cst[k] = pk_make(d <= sv[0] ? -32768 : -d * gev, d <= sv[1] ? -32768 : -d * gev);
+      const int32_t j0a = ov[0] + d, j0b = ov[1] + d;
+      Mo[k] = pk_make((d >= sv[0] && j0a >= 0 && j0a <= (int32_t)nv[0]) ? 0 : -32768,
+                      (d >= sv[1] && j0b >= 0 && j0b <= (int32_t)nv[1]) ? 0 : -32768);
+      Io[k] = PK_NEG;
+      Po[k] = pk_sub(PK_NEG, kge[k]);
+    }
+    uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+    const uint32_t mrow = max(mv[0], mv[1]);
+    uint32_t mmax = mrow;
+    int32_t emax = max(mv[0] ? bv[0] - hv[0] : 0, mv[1] ? bv[1] - hv[1] : 0);
+    for (int gg = 0; gg < 64; gg += 16) {
+      mmax = max(mmax, (uint32_t)readlane_i32((int32_t)mrow, gg));
+      emax = max(emax, readlane_i32(emax, gg));
+    }
+    uint32_t* TBA = a.tbq + (((size_t)blockIdx.x * 4 + (size_t)g) * 2) * Y.tb_words;
+    uint32_t* TBB = TBA + Y.tb_words;
+    int64_t* keys = (int64_t*)(jb + Y.keys);
+    keys[L] = end_key(NEG_INF, 2 * nv[0] + 1);
+    keys[16 + L] = end_key(NEG_INF, 2 * nv[1] + 1);
+    // endpoint of job hh at its last row: last maximum over (j asc; M then I)
+    auto endpoint = [&](int hh) {
+      int64_t key = end_key(NEG_INF, 2 * nv[hh] + 1);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int32_t d = 4 * L + k, j = (int32_t)mv[hh] + ov[hh] + d;
+        if (d >= sv[hh] && d < sv[hh] + Wv[hh] && j >= 1 && j <= (int32_t)nv[hh]) {
+          const int32_t Mk = hh ? pk_hi(Mo[k]) : pk_lo(Mo[k]), Ik = hh ? pk_hi(Io[k]) : pk_lo(Io[k]);
+          const int64_t km = end_key(Mk, (uint32_t)(2 * j)), ki = end_key(Ik, (uint32_t)(2 * j + 1));
+          key = key > km ? key : km;
+          key = key > ki ? key : ki;
+        }
+      }
+      keys[16 * hh + L] = key;
+    };
+    struct RowIn {
+      int32_t sa, sb;
+      uint32_t g0, g1, g2, g3, c0, c1, c2, c3;
+    };
+    auto load_row = [&](RowIn& x, uint32_t i) {  // in bounds for i <= max_m + 1
+      const uint2* gc = GC + (i - 1 + 4 * (uint32_t)L);
+      x.sa = SCWA[i - 1];
+      x.sb = SCWB[i - 1];
+      const uint2 v0 = gc[0], v1 = gc[1], v2 = gc[2], v3 = gc[3];
+      x.g0 = v0.x;
+      x.g1 = v1.x;
+      x.g2 = v2.x;
+      x.g3 = v3.x;
+      x.c0 = v0.y;
+      x.c1 = v1.y;
+      x.c2 = v2.y;
+      x.c3 = v3.y;
+    };
+    // scores of both jobs' cells of a column: code A in bits 0..4, code B in 8..12
+    auto score = [&](const RowIn& R, uint32_t c) {
+      return __builtin_amdgcn_perm((uint32_t)bfe_i32(R.sb, (int32_t)(c >> 8), 6), (uint32_t)bfe_i32(R.sa, (int32_t)c, 6),
+                                   0x05040100u);
+    };
+    auto row = [&](uint32_t i, const RowIn& R, auto edge_tag) {
+      constexpr bool EDGE = decltype(edge_tag)::value;
+      const uint32_t mu3 = (uint32_t)dpp_mov_z<DPP_ROW_SHL1>((int32_t)Mo[0]);
+      const uint32_t iu3 = (uint32_t)dpp_mov_z<DPP_ROW_SHL1>((int32_t)Io[0]);
+      uint32_t M0, M1, M2, M3, I0, I1, I2, I3;
+      {
+        const uint32_t X = pk_max(pk_max(Mo[0], Io[0]), pk_add(Po[0], kge[0]));
+        M0 = pk_add(X, score(R, R.c0));
+        const uint32_t io = pk_add(Mo[1], R.g0);
+        I0 = pk_max(io, pk_add(Io[1], ge_pk));
+        w0 = pk_push(w0, pk_sub(Mo[0], X));
+        w1 = pk_push(w1, pk_sub(Io[0], X));
+        w2 = pk_push(w2, pk_sub(io, I0));
+      }
+      {
+        const uint32_t X = pk_max(pk_max(Mo[1], Io[1]), pk_add(Po[1], kge[1]));
+        M1 = pk_add(X, score(R, R.c1));
+        const uint32_t io = pk_add(Mo[2], R.g1);
+        I1 = pk_max(io, pk_add(Io[2], ge_pk));
+        w0 = pk_push(w0, pk_sub(Mo[1], X));
+        w1 = pk_push(w1, pk_sub(Io[1], X));
+        w2 = pk_push(w2, pk_sub(io, I1));
+      }
+      {
+        const uint32_t X = pk_max(pk_max(Mo[2], Io[2]), pk_add(Po[2], kge[2]));
+        M2 = pk_add(X, score(R, R.c2));
+        const uint32_t io = pk_add(Mo[3], R.g2);
+        I2 = pk_max(io, pk_add(Io[3], ge_pk));
+        w0 = pk_push(w0, pk_sub(Mo[2], X));
+        w1 = pk_push(w1, pk_sub(Io[2], X));
+        w2 = pk_push(w2, pk_sub(io, I2));
+      }
+      {
+        const uint32_t X = pk_max(pk_max(Mo[3], Io[3]), pk_add(Po[3], kge[3]));
+        M3 = pk_add(pk_add(X, score(R, R.c3)), cp);
+        const uint32_t io = pk_add(pk_add(mu3, R.g3), cp);
+        I3 = pk_max(io, pk_add(pk_add(iu3, ge_pk), cp));
+        w0 = pk_push(w0, pk_sub(Mo[3], X));
+        w1 = pk_push(w1, pk_sub(Io[3], X));
+        w2 = pk_push(w2, pk_sub(io, I3));
+      }
+      const int32_t jla = (int32_t)i + ov[0] + 4 * L, jlb = (int32_t)i + ov[1] + 4 * L;
+      uint32_t em[4] = {0, 0, 0, 0};
+      if (EDGE) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) em[k] = (jla + k <= 0 ? 0xFFFFu : 0u) | (jlb + k <= 0 ? 0xFFFF0000u : 0u);
+        M0 = (M0 & ~em[0]) | (PK_NEG & em[0]);
+        I0 = (I0 & ~em[0]) | (PK_NEG & em[0]);
+        M1 = (M1 & ~em[1]) | (PK_NEG & em[1]);
+        I1 = (I1 & ~em[1]) | (PK_NEG & em[1]);
+        M2 = (M2 & ~em[2]) | (PK_NEG & em[2]);
+        I2 = (I2 & ~em[2]) | (PK_NEG & em[2]);
+        M3 = (M3 & ~em[3]) | (PK_NEG & em[3]);
+        I3 = (I3 & ~em[3]) | (PK_NEG & em[3]);
+      }
+      // D: max-plus prefix scan along the row
+      const uint32_t ml0 = (uint32_t)dpp_mov_z<DPP_ROW_SHR1>((int32_t)M3);
+      const uint32_t B0 = pk_add(pk_add(ml0, R.g0), cst[0]), B1 = pk_add(pk_add(M0, R.g1), cst[1]),
+                     B2 = pk_add(pk_add(M1, R.g2), cst[2]), B3 = pk_add(pk_add(M2, R.g3), cst[3]);
+      const uint32_t L0 = B0, L1 = pk_max(L0, B1), L2 = pk_max(L1, B2), L3 = pk_max(L2, B3);
+      uint32_t x = L3;
+      x = pk_max(x, (uint32_t)dpp_mov<DPP_ROW_SHR1>((int32_t)PK_NEG, (int32_t)x));
+      x = pk_max(x, (uint32_t)dpp_mov<DPP_ROW_SHR2>((int32_t)PK_NEG, (int32_t)x));
+      x = pk_max(x, (uint32_t)dpp_mov<DPP_ROW_SHR4>((int32_t)PK_NEG, (int32_t)x));
+      x = pk_max(x, (uint32_t)dpp_mov<DPP_ROW_SHR8>((int32_t)PK_NEG, (int32_t)x));
+      const uint32_t carry = (uint32_t)dpp_mov<DPP_ROW_SHR1>((int32_t)PK_NEG, (int32_t)x);
+      uint32_t P0 = pk_max(L0, carry), P1 = pk_max(L1, carry), P2 = pk_max(L2, carry), P3 = pk_max(L3, carry);
+      w3 = pk_push(w3, pk_sub(B0, P0));
+      w3 = pk_push(w3, pk_sub(B1, P1));
+      w3 = pk_push(w3, pk_sub(B2, P2));
+      w3 = pk_push(w3, pk_sub(B3, P3));
+      if (EDGE) {
+        P0 = (P0 & ~em[0]) | (pk_sub(PK_NEG, kge[0]) & em[0]);
+        P1 = (P1 & ~em[1]) | (pk_sub(PK_NEG, kge[1]) & em[1]);
+        P2 = (P2 & ~em[2]) | (pk_sub(PK_NEG, kge[2]) & em[2]);
+        P3 = (P3 & ~em[3]) | (pk_sub(PK_NEG, kge[3]) & em[3]);
+      }
+      Mo[0] = M0;
+      Mo[1] = M1;
+      Mo[2] = M2;
+      Mo[3] = M3;
+      Io[0] = I0;
+      Io[1] = I1;
+      Io[2] = I2;
+      Io[3] = I3;
+      Po[0] = P0;
+      Po[1] = P1;
+      Po[2] = P2;
+      Po[3] = P3;
+      if (i == mv[0]) endpoint(0);
+      if (i == mv[1]) endpoint(1);
+      if ((i & 3u) == 0) {  // 16 pushes: one 4-row block per job
+        const uint32_t wd = w0 & w1;
+        const uint32_t q = (i - 1) >> 2;
+        *(uint2*)(TBA + q * 32 + 2 * L) = make_uint2(__builtin_amdgcn_perm(wd, w0, 0x05040100u), __builtin_amdgcn_perm(w3, w2, 0x05040100u));
+        *(uint2*)(TBB + q * 32 + 2 * L) = make_uint2(__builtin_amdgcn_perm(wd, w0, 0x07060302u), __builtin_amdgcn_perm(w3, w2, 0x07060302u));
+        w0 = w1 = w2 = w3 = 0;
+      }
+    };
+    RowIn RA, RB;
+    load_row(RA, 1);
+    uint32_t i = 1;
+    const uint32_t elim = (uint32_t)min((int32_t)mmax, emax);
+    for (; i + 1 <= elim; i += 2) {
+      load_row(RB, i + 1);
+      if (i <= mrow) row(i, RA, std::true_type{});
+      load_row(RA, i + 2);
+      if (i + 1 <= mrow) row(i + 1, RB, std::true_type{});
+    }
+    if (i <= elim) {
+      load_row(RB, i + 1);
+      if (i <= mrow) row(i, RA, std::true_type{});
+      RA = RB;
+      i++;
+    }
+    for (; i + 1 <= mmax; i += 2) {
+      load_row(RB, i + 1);
+      if (i <= mrow) row(i, RA, std::false_type{});
+      load_row(RA, i + 2);
+      if (i + 1 <= mrow) row(i + 1, RB, std::false_type{});
+    }
+    if (i <= mmax) {
+      load_row(RB, i + 1);
+      if (i <= mrow) row(i, RA, std::false_type{});
+    }
+    if (mrow & 3u) {  // the row's last, partial block: left-align both halves' fields
+      const uint32_t sh = 4 * (4 - (mrow & 3u));
+      const u16x2_t shv = {(unsigned short)sh, (unsigned short)sh};
+      const uint32_t a0 = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2_t, w0) << shv);
+      const uint32_t a1 = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2_t, w0 & w1) << shv);
+      const uint32_t a2 = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2_t, w2) << shv);
+      const uint32_t a3 = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2_t, w3) << shv);
+      const uint32_t q = (mrow - 1) >> 2;
+      *(uint2*)(TBA + q * 32 + 2 * L) = make_uint2(__builtin_amdgcn_perm(a1, a0, 0x05040100u), __builtin_amdgcn_perm(a3, a2, 0x05040100u));
+      *(uint2*)(TBB + q * 32 + 2 * L) = make_uint2(__builtin_amdgcn_perm(a1, a0, 0x07060302u), __builtin_amdgcn_perm(a3, a2, 0x07060302u));
+    }
+    wave_sync();
+    mem_sync();  // the traceback words stored by every lane
+    // ---- traceback by lane hh of the row, for job hh (smith_waterman.rs:299-345) ----
+    uint32_t nruns = 0, off = 0;
+    int64_t best = 0;
+    bool wvalid = false;
+    if (L < 2) {
+      const int hh = L;
+      const uint32_t m = mv[hh];
+      if (m) {
+        best = keys[16 * hh];
+        for (int u = 1; u < 16; u++) best = best > keys[16 * hh + u] ? best : keys[16 * hh + u];
+        wvalid = (int32_t)(best >> 32) >= PAIR16_MIN_SCORE;
+        if (!wvalid) dp_flag(a, jidv[hh]);
+      }
+      if (wvalid) {
+        const uint32_t* TB = hh ? TBB : TBA;
+        uint32_t* runs_top = (hh ? TBB : TBA) + Y.tb_words;
+        const int32_t o = ov[hh];
+        uint32_t i = m, j = ((uint32_t)best) >> 1, st = ((uint32_t)best) & 1u;
+        uint32_t cur_op = 0xFFu, cur_len = 0;
+        int32_t cb = -1, cl = -1;
+        uint32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0, p0 = 0, p1 = 0, n2 = 0, l3 = 0;
+        // the 32-bit 8-row word of plane (field) f of lane ln and 8-row block rb:
+        // 4-row block 2rb in the high half, 2rb + 1 in the low half
+        while (i > 0) {
+          int32_t d = (int32_t)j - (int32_t)i - o;
+          d = d < 0 ? 0 : (d > 63 ? 63 : d);
+          const int32_t lq = d >> 2, rb = (int32_t)(i - 1) >> 3;
+          if (rb != cb || lq != cl) {
+            const uint2 h0 = *(const uint2*)(TB + (2 * rb) * 32 + 2 * lq);
+            const uint2 h1 = *(const uint2*)(TB + (2 * rb + 1) * 32 + 2 * lq);
+            const int32_t pb = rb > 0 ? rb - 1 : 0;
+            const uint32_t ph0 = TB[(2 * pb) * 32 + 2 * lq], ph1 = TB[(2 * pb + 1) * 32 + 2 * lq];
+            const int32_t ln2 = lq < 15 ? lq + 1 : lq, ll3 = lq > 0 ? lq - 1 : lq;
+            const uint32_t n20 = TB[(2 * rb) * 32 + 2 * ln2 + 1], n21 = TB[(2 * rb + 1) * 32 + 2 * ln2 + 1];
+            const uint32_t l30 = TB[(2 * rb) * 32 + 2 * ll3 + 1], l31 = TB[(2 * rb + 1) * 32 + 2 * ll3 + 1];
+            v0 = __builtin_amdgcn_perm(h0.x, h1.x, 0x05040100u);  // {h0.x lo16, h1.x lo16}: plane 0
+            v1 = __builtin_amdgcn_perm(h0.x, h1.x, 0x07060302u);  // plane 1
+            v2 = __builtin_amdgcn_perm(h0.y, h1.y, 0x05040100u);  // plane 2
+            v3 = __builtin_amdgcn_perm(h0.y, h1.y, 0x07060302u);  // plane 3
+            p0 = __builtin_amdgcn_perm(ph0, ph1, 0x05040100u);
+            p1 = __builtin_amdgcn_perm(ph0, ph1, 0x07060302u);
+            n2 = __builtin_amdgcn_perm(n20, n21, 0x05040100u);
+            l3 = __builtin_amdgcn_perm(l30, l31, 0x07060302u);
+            cb = rb;
+            cl = lq;
+          }
+          const uint32_t k = (uint32_t)d & 3u, r = (i - 1) & 7u;
+          const uint32_t pos = 31u - (4u * r + k);
+          const uint32_t mav = r + 1u + (rb > 0 ? 8u : 0u);
+          const uint64_t xm = ((((uint64_t)p0 << 32) | v0) >> pos) & 0x1111111111111111ull &
+                              (mav >= 16u ? ~0ull : ((1ull << (4u * mav)) - 1ull));
+          const uint32_t f = xm ? ((uint32_t)__builtin_ctzll(xm) >> 2) : 64u;
+          const uint32_t mst = min(min(f + 1u, mav), j);
+          const uint64_t x1 = (((uint64_t)p1 << 32) | v1) >> pos;
+          const uint32_t mpar = (mst == f + 1u) ? 1u + (uint32_t)((x1 >> ((4u * f) & 63u)) & 1ull) : 0u;
+          const uint32_t dav = k + 1u + (lq > 0 ? 4u : 0u);
+          const uint32_t yd = ((v3 >> pos) & ((2u << k) - 1u)) | (((l3 >> (28u - 4u * r)) & 0xFu) << (k + 1u));
+          const uint32_t y = ~yd & ((1u << dav) - 1u);
+          const uint32_t fz = y ? (uint32_t)__builtin_ctz(y) : 32u;
+          const uint32_t dst = min(min(fz + 1u, dav), j);
+          const uint32_t dpar = (dst == fz + 1u) ? 0u : 2u;
+          const uint32_t tav = min((lq < 15 ? 7u : 3u) - k, r);
+          const uint32_t ma = 0x249249u & ((1u << (3u * (3u - k) + 1u)) - 1u);
+          const uint32_t zv = ((v2 >> pos) & ma) | ((r > 0 ? (n2 >> ((pos + 4u) & 31u)) : 0u) & 0x249249u & ~ma);
+          const uint32_t zi = ~zv & 0x249249u & ((1u << (3u * tav + 1u)) - 1u);
+          const uint32_t fi = zi ? (uint32_t)__builtin_ctz(zi) / 3u : 32u;
+          const uint32_t ist = min(fi + 1u, tav + 1u);
+          const uint32_t ipar = (ist == fi + 1u) ? 0u : 1u;
+          const uint32_t sel = (j == 0) ? 1u : st;
+          const uint32_t isM = 0u - (uint32_t)(sel == 0u), isD = 0u - (uint32_t)(sel == 2u);
+          const uint32_t isI = ~(isM | isD);
+          const uint32_t jz = 0u - (uint32_t)(j == 0);
+          const uint32_t len = (mst & isM) | (dst & isD) | (((ist & ~jz) | (1u & jz)) & isI);
+          const uint32_t nst = (mpar & isM) | (dpar & isD) | (((ipar & ~jz) | (st & jz)) & isI);
+          i -= len & ~isD;
+          j -= len & ~isI;
+          st = nst;
+          const uint32_t fresh = 0u - (uint32_t)(sel != cur_op);
+          nruns += (fresh & (uint32_t)(cur_len != 0)) ? 1u : 0u;
+          cur_len = (len & fresh) | ((cur_len + len) & ~fresh);
+          cur_op = sel;
+          runs_top[-1 - (int32_t)nruns] = (cur_len << 4) | cur_op;
+        }
+        nruns += cur_len != 0 ? 1u : 0u;
+        off = j;
+      }
+    }
+    mem_sync();  // the CIGAR runs the walkers stored
+    // ---- per job: log-likelihood terms (single.rs:228-262), ordered sum, outputs ----
+    double* terms0 = (double*)(jb + Y.gc);  // dead per-row inputs: job A's terms, then job B's
+#pragma unroll
+    for (int hh = 0; hh < 2; hh++) {
+      const uint32_t nr = (uint32_t)__shfl((int)nruns, (lane & ~15) + hh, 64);
+      const uint32_t of = (uint32_t)__shfl((int)off, (lane & ~15) + hh, 64);
+      const bool ok = __shfl((int)wvalid, (lane & ~15) + hh, 64) != 0;
+      const uint32_t rd = a.jobs[jidv[hh]].read;
+      const uint32_t m = ok ? mv[hh] : 0u, qlen = (uint32_t)(a.R.qual_off[rd + 1] - a.R.qual_off[rd]);
+      const uint32_t* runs = (hh ? TBB : TBA) + Y.tb_words - nr;
+      const uint8_t* Q = jb + Y.q + hh * Y.q_len;
+      const uint8_t* QQ = jb + Y.qq + hh * Y.q_len;
+      const uint8_t* REF = jb + Y.ref + hh * Y.ref_len;
+      double* terms = terms0 + (hh ? a.max_m : 0u);
+      uint32_t u = 0, qs = 0, rs = 0;
+      for (uint32_t qi = L; qi < m; qi += 16) {
+        uint32_t op = 1u;
+        while (u < nr) {
+          const uint32_t w = runs[u], len = w >> 4;
+          op = w & 15u;
+          const uint32_t qc = op <= 1u ? len : 0u;
+          if (qi < qs + qc) break;
+          qs += qc;
+          rs += (op == 0u || op == 2u) ? len : 0u;
+          u++;
+        }
+        double tv = a.T.ln_quarter;
+        if (op == 0u) {
+          const uint32_t qv = qi < qlen ? QQ[qi] : 30u;
+          tv = (Q[qi] == REF[of + rs + (qi - qs)]) ? a.T.lk_match[qv] : a.T.lk_mis[qv];
+        }
+        terms[qi] = tv;
+      }
+    }
+    wave_sync();  // the terms of all 16 lanes
+    if (L < 2 && wvalid) {
+      const int hh = L;
+      const uint32_t m = mv[hh];
+      const double* terms = terms0 + (hh ? a.max_m : 0u);
+      double lk = 0.0;
+      uint32_t qi = 0;
+      for (; qi + 8 <= m; qi += 8) {  // read order, as the reference sums
+        double t8[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) t8[u] = terms[qi + u];
+#pragma unroll
+        for (int u = 0; u < 8; u++) lk += t8[u];
+      }
+      for (; qi < m; qi++) lk += terms[qi];
+      const uint32_t* runs = (hh ? TBB : TBA) + Y.tb_words - nruns;
+      uint32_t span = 0;
+      for (uint32_t u = 0; u < nruns; u++) {
+        const uint32_t w = runs[u], op = w & 15u;
+        if (op == 0u || op == 2u) span += w >> 4;
+      }
+      const DpJob jh = a.jobs[jidv[hh]];
+      const uint64_t cells = quad_cells(m, jh.hint, jh.band, nv[hh]);
+      c_cells += cells;
+      DpOut oo;
+      oo.log_lk = lk;
+      oo.ref_off = off;
+      oo.score = (int32_t)(best >> 32);
+      oo.cigar_n = nruns;
+      oo.span = span;
+      oo.cells = cells;
+      a.out[jidv[hh]] = oo;
+    }
+    // ---- CIGAR runs to each job's slot, forward order ----
+#pragma unroll
+    for (int hh = 0; hh < 2; hh++) {
+      const uint32_t nr = (uint32_t)__shfl((int)nruns, (lane & ~15) + hh, 64);
+      const bool ok = __shfl((int)wvalid, (lane & ~15) + hh, 64) != 0;
+      if (ok) {
+        const uint32_t* runs = (hh ? TBB : TBA) + Y.tb_words - nr;
+        const uint32_t ncopy = nr < a.cigar_cap ? nr : a.cigar_cap;
+        for (uint32_t u = L; u < ncopy; u += 16) a.cigar[(uint64_t)jidv[hh] * a.cigar_cap + u] = runs[u];
+      }
+    }
+    wave_sync();
+  }
+#pragma unroll
+  for (int sft = 32; sft > 0; sft >>= 1) c_cells += (unsigned long long)__shfl_xor((long long)c_cells, sft, 64);
+  if (lane == 0 && a.ctr && c_cells) atomicAdd(&a.ctr->cells, c_cells);
+}
+
 // Launches k_dp_quad; returns false (nothing launched) when one wave's four
 // job areas do not fit in 64 KiB of LDS.
 __global__ void k_job_zero(uint32_t* p, uint32_t n) {
@@ -2525,11 +3154,29 @@ uint32_t quad_blocks(uint32_t njobs, uint32_t grid_cap) {
   const uint32_t q = (njobs + 3) / 4;
   return q < cap ? q : cap;
 }
+static uint32_t pair16_blocks(uint32_t njobs, uint32_t grid_cap) {
+  const uint32_t cap = grid_cap ? grid_cap : QUAD_MAX_WAVES;
+  const uint32_t q = (njobs + 7) / 8;
+  return q < cap ? q : cap;
+}
 size_t quad_scratch_words(uint32_t max_m, uint32_t max_n, uint32_t njobs, uint32_t grid_cap) {
-  return (size_t)quad_blocks(njobs, grid_cap) * 4 * quad_layout(max_m, max_n).tb_words;
+  const size_t quad = (size_t)quad_blocks(njobs, grid_cap) * 4 * quad_layout(max_m, max_n).tb_words;
+  const size_t pair = (size_t)pair16_blocks(njobs, grid_cap) * 8 * pair16_layout(max_m, max_n).tb_words;
+  return quad > pair ? quad : pair;
 }
 bool launch_dp_quad(const DpArgs& a, hipStream_t s) {
   if (a.njobs == 0) return true;
+  // eight jobs per wave in 16-bit halves while the gap penalties keep the D
+  // scan's column shifts small (see k_dp_pair16)
+  if (a.pack16 && a.max_m <= (uint32_t)QUAD_MAX_M && a.gap_extend >= -4 && a.gap_open >= -400 && a.tbq && a.jlist) {
+    const Pair16Layout P = pair16_layout(a.max_m, a.max_n);
+    const size_t lds = 4 * (size_t)P.stride;
+    const size_t need = 8 * ((size_t)((a.max_n + 62) >> 5) + ((a.max_n + 126) >> 6));
+    if (lds <= 64 * 1024 && need <= P.scw_len) {
+      hipLaunchKernelGGL(k_dp_pair16, dim3(pair16_blocks(a.njobs, a.quad_grid)), dim3(64), lds, s, a);
+      return true;
+    }
+  }
   const QuadLayout Y = quad_layout(a.max_m, a.max_n);
   const size_t lds = 4 * (size_t)Y.stride;
   if (a.max_m > (uint32_t)QUAD_MAX_M || lds > 64 * 1024 || !a.tbq) return false;
