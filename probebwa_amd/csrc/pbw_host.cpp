@@ -264,7 +264,9 @@ struct pbw_ctx {
   int32_t gap_open = -90, gap_extend = -1;
   // batch workspace
   DevBuf<uint8_t> d_seq, d_qual, d_rcode;
-  DevBuf<uint64_t> d_seq_off, d_qual_off, d_hoff, d_hits, d_hits_ov, d_scratch;
+  DevBuf<uint64_t> d_seq_off, d_qual_off, d_hoff, d_hits, d_hits_ov, d_scratch, d_ovoff;
+  uint32_t* h_nh = nullptr;  // pinned: per-read hit counts of the seed pass
+  size_t h_nh_cap = 0;
   DevBuf<uint32_t> d_nhits, d_list, d_hcap, d_tier2, d_mid, d_ntier2, d_best, d_cigar, d_cigar_out, d_cigar_off;
   DevBuf<Cand> d_cands;
   DevBuf<uint8_t> d_ncand, d_njobs, d_rec, d_refbytes;
@@ -622,10 +624,9 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false) {
   const DevReads R = b.R;
   auto t0 = std::chrono::steady_clock::now();
   // ---- seed ----
-  std::vector<uint64_t> hoff(B);
-  for (uint32_t i = 0; i < B; i++) hoff[i] = (uint64_t)i * HIT_CAP;
-  c->d_hoff.upload(hoff.data(), B, s);
   // fixed HIT_CAP slots per read, then slack for the overflow pass's exact-size slots
+  c->d_hoff.ensure(B);
+  launch_fill_stride(c->d_hoff.p, B, (uint64_t)HIT_CAP, s);
   const size_t base_cap = (size_t)B * HIT_CAP, slack = (size_t)B * 64;
   c->d_hits.ensure(base_cap + slack);
   c->d_nhits.ensure(B);
@@ -650,8 +651,16 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false) {
   launch_seed(sa, s);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipEventRecord(c->ev[1], s));
-  std::vector<uint32_t> nh(B);
-  HIP_CHECK(hipMemcpyAsync(nh.data(), c->d_nhits.p, (size_t)B * 4, hipMemcpyDeviceToHost, s));
+  if (c->h_nh_cap < B) {
+    if (c->h_nh) (void)hipHostFree(c->h_nh);
+    c->h_nh = nullptr;
+    c->h_nh_cap = 0;
+    if (hipHostMalloc((void**)&c->h_nh, (size_t)B * 4, hipHostMallocDefault) != hipSuccess || !c->h_nh)
+      throw PbwError{PBW_ERR_LIMIT, "pinned host allocation of " + std::to_string((size_t)B * 4) + " bytes failed"};
+    c->h_nh_cap = B;
+  }
+  const uint32_t* nh = c->h_nh;
+  HIP_CHECK(hipMemcpyAsync(c->h_nh, c->d_nhits.p, (size_t)B * 4, hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
   {
     float ms = 0.f;
@@ -693,24 +702,24 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false) {
     for (size_t k = 0; k < ov.size(); k++) {
       ovoff[k] = cur;
       ovcap[k] = nh[ov[k]];
-      hoff[ov[k]] = cur;
       cur += nh[ov[k]];
     }
-    DevBuf<uint64_t> dovoff;
-    dovoff.upload(ovoff.data(), ov.size(), s);
+    c->d_ovoff.upload(ovoff.data(), ov.size(), s);
     c->d_list.upload(ov.data(), ov.size(), s);
     c->d_hcap.upload(ovcap.data(), ov.size(), s);
     SeedArgs sb = sa;
     sb.rlist = c->d_list.p;
     sb.nlist = (uint32_t)ov.size();
-    sb.hoff = dovoff.p;
+    sb.hoff = c->d_ovoff.p;
     sb.hcap = c->d_hcap.p;
     sb.hits = arena;
     sb.ctr = nullptr;  // counted once, in the first pass
     sb.hflag = farena;
     launch_seed(sb, s);
     HIP_CHECK(hipGetLastError());
-    c->d_hoff.upload(hoff.data(), B, s);
+    // the overflow reads' slots replace their fixed ones in the per-read offsets
+    launch_scatter_u64(c->d_hoff.p, c->d_list.p, c->d_ovoff.p, (uint32_t)ov.size(), s);
+    HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(s));
     hits_base = arena;
     hflag_base = farena;
@@ -1560,6 +1569,7 @@ void pbw_ctx_destroy(pbw_ctx* c) {
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->h_nh) (void)hipHostFree(c->h_nh);
   delete c;
 }
 

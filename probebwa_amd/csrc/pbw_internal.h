@@ -293,6 +293,10 @@ void gather_cigar2(const uint32_t* cig0, const uint32_t* cig1, const uint64_t* s
 void gather_cigar(const uint32_t* cig, const uint64_t* src, const uint32_t* off, const uint32_t* cnt, uint32_t n,
                   uint32_t* out, hipStream_t s);
 void launch_fill_pres(const uint32_t* koff, uint64_t* pres, uint32_t rot, hipStream_t s);
+// out[i] = i * stride for i < n
+void launch_fill_stride(uint64_t* out, uint32_t n, uint64_t stride, hipStream_t s);
+// out[idx[k]] = val[k] for k < n
+void launch_scatter_u64(uint64_t* out, const uint32_t* idx, const uint64_t* val, uint32_t n, hipStream_t s);
 void launch_fill_koff(uint32_t* koff, const uint32_t* keys, const uint32_t* kstart, uint64_t nkeys, uint32_t total,
                       hipStream_t s);
 

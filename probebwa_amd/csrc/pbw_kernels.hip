@@ -3973,6 +3973,21 @@ __global__ void k_fill_pres(const uint32_t* koff, uint64_t* pres, uint32_t rot) 
     if (lane_id() == 0) pres[i >> 6] = m;
   }
 }
+__global__ void k_fill_stride(uint64_t* out, uint32_t n, uint64_t stride) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (uint64_t)i * stride;
+}
+void launch_fill_stride(uint64_t* out, uint32_t n, uint64_t stride, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_fill_stride, dim3((n + 255) / 256), dim3(256), 0, s, out, n, stride);
+}
+__global__ void k_scatter_u64(uint64_t* out, const uint32_t* idx, const uint64_t* val, uint32_t n) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[idx[k]] = val[k];
+}
+void launch_scatter_u64(uint64_t* out, const uint32_t* idx, const uint64_t* val, uint32_t n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_scatter_u64, dim3((n + 255) / 256), dim3(256), 0, s, out, idx, val, n);
+}
+
 void launch_fill_pres(const uint32_t* koff, uint64_t* pres, uint32_t rot, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_pres, dim3(65536), dim3(256), 0, s, koff, pres, rot);
 }
