@@ -484,6 +484,7 @@ struct HostArr {
 struct HostRecords {
   HostArr<pbw_record> rec;
   HostArr<uint32_t> cigar;
+  size_t expect = 0;  // records the call will produce (sizes the first reservation)
 };
 
 static double ms_since(std::chrono::steady_clock::time_point t0) {
@@ -618,7 +619,8 @@ static void batch_upload_slice(pbw_ctx* c, Batch& b, const pbw_read_batch* in, u
 
 // map_candidates (single.rs:86-162) for every read of the batch: seed,
 // cluster, DP jobs, banded DP.  Results stay on the device.
-static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false) {
+// `pre` (optional): output arrays to reserve on the host while the seed kernel runs.
+static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecords* pre = nullptr) {
   hipStream_t s = c->stream;
   const uint32_t B = b.B;
   const DevReads R = b.R;
@@ -651,6 +653,12 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false) {
   launch_seed(sa, s);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipEventRecord(c->ev[1], s));
+  if (pre) {  // pinned allocations overlap the seed kernel: one record and ~4 CIGAR runs per read
+    const size_t wr = std::max(pre->rec.n + B, pre->expect);
+    if (wr > pre->rec.cap) pre->rec.reserve(std::max(wr, pre->rec.cap * 2));
+    const size_t wc = std::max(pre->cigar.n + (size_t)B * 4, pre->expect * 4);
+    if (wc > pre->cigar.cap) pre->cigar.reserve(std::max(wc, pre->cigar.cap * 2));
+  }
   if (c->h_nh_cap < B) {
     if (c->h_nh) (void)hipHostFree(c->h_nh);
     c->h_nh = nullptr;
@@ -1082,7 +1090,7 @@ static void map_se_batch(pbw_ctx* c, const pbw_read_batch* in, uint64_t b0, uint
   batch_upload_slice(c, b, in, b0, b1, resident);
   HIP_CHECK(hipMemsetAsync(c->d_ctr.p, 0, sizeof(Counters), s));
   const bool pf = c->pf.enabled && c->mlp_ready;
-  batch_candidates(c, b, pf);
+  batch_candidates(c, b, pf, &hr);
   const uint32_t B = b.B;
   if (pf) {
     prefilter_dp_mapq(c, b, b0);
@@ -1579,8 +1587,8 @@ int pbw_map_se(pbw_ctx* c, const pbw_read_batch* reads, pbw_record_batch* out) {
     HIP_CHECK(hipSetDevice(c->device));
     c->stats = pbw_stats{};
     auto t0 = std::chrono::steady_clock::now();
-    HostRecords hr;
-    hr.rec.reserve(reads->n);
+    HostRecords hr;  // reserved while the first batch's seed kernel runs
+    hr.expect = reads->n;
     uint64_t bs = c->opt.batch_reads ? c->opt.batch_reads : 131072;
     for (uint64_t b = 0; b < reads->n; b += bs) map_se_batch(c, reads, b, std::min(reads->n, b + bs), hr, false);
     emit_batch(hr, out);
@@ -1630,8 +1638,8 @@ int pbw_map_se_device(pbw_ctx* c, const pbw_device_reads* d, uint64_t begin, uin
     HIP_CHECK(hipSetDevice(c->device));
     c->stats = pbw_stats{};
     auto t0 = std::chrono::steady_clock::now();
-    HostRecords hr;
-    hr.rec.reserve(end - begin);
+    HostRecords hr;  // reserved while the first batch's seed kernel runs
+    hr.expect = end - begin;
     pbw_read_batch view{d->n, d->seq.p, d->seq_off.data(), d->qual.p, d->qual_off.data()};
     uint64_t bs = c->opt.batch_reads ? c->opt.batch_reads : 131072;
     for (uint64_t b = begin; b < end; b += bs) map_se_batch(c, &view, b, std::min(end, b + bs), hr, true);
