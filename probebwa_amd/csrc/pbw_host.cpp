@@ -267,6 +267,7 @@ struct pbw_ctx {
   DevBuf<uint64_t> d_seq_off, d_qual_off, d_hoff, d_hits, d_hits_ov, d_scratch, d_ovoff;
   uint32_t* h_nh = nullptr;  // pinned: per-read hit counts of the seed pass
   size_t h_nh_cap = 0;
+  uint32_t hit_cap = HIT_CAP;  // first-pass hits per read, adapted batch to batch
   DevBuf<uint32_t> d_nhits, d_list, d_hcap, d_tier2, d_mid, d_ntier2, d_best, d_cigar, d_cigar_out, d_cigar_off;
   DevBuf<Cand> d_cands;
   DevBuf<uint8_t> d_ncand, d_njobs, d_rec, d_refbytes;
@@ -626,10 +627,11 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
   const DevReads R = b.R;
   auto t0 = std::chrono::steady_clock::now();
   // ---- seed ----
-  // fixed HIT_CAP slots per read, then slack for the overflow pass's exact-size slots
+  // fixed slots of cap1 hits per read, then slack for the overflow pass's exact-size slots
+  const uint32_t cap1 = c->hit_cap;
   c->d_hoff.ensure(B);
-  launch_fill_stride(c->d_hoff.p, B, (uint64_t)HIT_CAP, s);
-  const size_t base_cap = (size_t)B * HIT_CAP, slack = (size_t)B * 64;
+  launch_fill_stride(c->d_hoff.p, B, (uint64_t)cap1, s);
+  const size_t base_cap = (size_t)B * cap1, slack = (size_t)B * 64;
   c->d_hits.ensure(base_cap + slack);
   c->d_nhits.ensure(B);
   SeedArgs sa{};
@@ -641,6 +643,7 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
   sa.nlist = B;
   sa.hoff = c->d_hoff.p;
   sa.hcap = nullptr;
+  sa.hit_cap = cap1;
   sa.hits = c->d_hits.p;
   sa.nhits = c->d_nhits.p;
   sa.rcode = c->d_rcode.p;
@@ -682,10 +685,23 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
   uint32_t max_hits = 0;
   for (uint32_t i = 0; i < B; i++) {
     max_hits = std::max(max_hits, nh[i]);
-    if (nh[i] > (uint32_t)HIT_CAP) {
+    if (nh[i] > cap1) {
       ov.push_back(i);
       ov_total += nh[i];
     }
+  }
+  if ((uint64_t)ov.size() * 20 > B && cap1 < (uint32_t)HIT_CAP_MAX) {
+    // more than 5 % of the reads overflowed: the next batches' first pass takes
+    // the least power of two that would have left at most 5 % (within 24 GiB of
+    // hit slots and flags)
+    uint32_t over[4] = {0, 0, 0, 0};  // reads with more than HIT_CAP << k hits
+    for (uint32_t r : ov)
+      for (int k = 0; k < 4; k++) over[k] += nh[r] > ((uint32_t)HIT_CAP << k);
+    uint32_t want = cap1;
+    for (int k = 0; k < 4; k++)
+      if ((uint64_t)over[k] * 20 > B) want = std::max<uint32_t>(want, (uint32_t)HIT_CAP << (k + 1));
+    while (want > cap1 && (uint64_t)B * want * 9 > (24ull << 30)) want >>= 1;
+    c->hit_cap = std::min<uint32_t>(want, HIT_CAP_MAX);
   }
   const uint64_t* hits_base = c->d_hits.p;
   const uint8_t* hflag_base = feat ? c->d_hflag.p : nullptr;
@@ -706,7 +722,7 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
     }
     std::vector<uint64_t> ovoff(ov.size());
     std::vector<uint32_t> ovcap(ov.size());
-    uint64_t cur = (uint64_t)B * HIT_CAP;
+    uint64_t cur = (uint64_t)B * cap1;
     for (size_t k = 0; k < ov.size(); k++) {
       ovoff[k] = cur;
       ovcap[k] = nh[ov[k]];

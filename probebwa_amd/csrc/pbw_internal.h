@@ -21,7 +21,8 @@ constexpr int64_t DIAG_BIAS = 1ll << 40;
 
 // ---- limits of this implementation ----
 constexpr int MAX_READ_LEN = 1024;       // longer reads are rejected with PBW_ERR_LIMIT
-constexpr int HIT_CAP = 1024;            // hits per read in the first seed pass (overflow -> exact rerun)
+constexpr int HIT_CAP = 1024;            // least hits per read in the first seed pass (overflow -> exact rerun)
+constexpr int HIT_CAP_MAX = 16384;       // most: the first-pass capacity adapts between the two
 constexpr int CLUSTER_LDS_CAP = 512;     // hits per read clustered in LDS by one wave (else tier 2)
 constexpr int LN_INT_MAX = 4096;         // ln(k) table for phred_n_choose_m
 constexpr int GC_LUT_MAX = 1024;         // (gc, total) table for the GC-adjusted alphabet size
@@ -118,7 +119,8 @@ struct SeedArgs {
   const uint32_t* rlist;   // reads to process (nullptr = 0..n-1)
   uint32_t nlist;
   const uint64_t* hoff;    // per list entry: hit buffer offset
-  const uint32_t* hcap;    // per list entry: capacity (nullptr = HIT_CAP)
+  const uint32_t* hcap;    // per list entry: capacity (nullptr = hit_cap)
+  uint32_t hit_cap;        // capacity of every read when hcap is null
   uint64_t* hits;
   uint32_t* nhits;         // per read (indexed by read id)
   uint8_t* rcode;          // per window repeat code, indexed by seq_off[read] + p

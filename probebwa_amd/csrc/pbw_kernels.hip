@@ -334,7 +334,7 @@ __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
     const uint32_t qlen = (uint32_t)(a.R.qual_off[r + 1] - qo);
     (void)qlen;
     const uint64_t hbase = a.hoff[li];
-    const uint32_t cap = a.hcap ? a.hcap[li] : (uint32_t)HIT_CAP;
+    const uint32_t cap = a.hcap ? a.hcap[li] : a.hit_cap;
     uint32_t hcount = 0;
     uint32_t qn = 0, qtot = 0, qc = 0;  // queued buckets, their positions, positions consumed
     if (L < (uint32_t)K) {

@@ -235,6 +235,27 @@ def test_map_se_repeat_heavy_reads():
     assert st["tier2_cluster_reads"] > 0 or st["seed_overflow_reads"] > 0
 
 
+def test_map_se_first_pass_capacity_adapts():
+    """When more than 5 % of a batch's reads overflow the first seed pass's hit slots, the
+    later batches take larger slots; the records stay the oracle's in every batch and call."""
+    rng = np.random.default_rng(43)
+    unit = synth.random_bases(rng, 41)
+    seqs = [("rep", np.concatenate([synth.mutate(rng, unit, 0.03) for _ in range(1200)])),
+            ("uniq", synth.random_bases(rng, 30000))]
+    go, gp = pair_of_genomes(seqs)
+    seq, so, qual, qo = synth.sample_reads(seqs, 240, 143, lmin=60, lmax=200)
+    ro_b, rp_b = both_reads(seq, so, qual, qo)
+    ro = O.Mapper(go, O.HashTable(go)).map_se(ro_b)
+    ctx = P.Context(gp, batch_reads=40)
+    rp1 = ctx.map_se(rp_b)
+    ov1 = ctx.stats()["seed_overflow_reads"]
+    rp2 = ctx.map_se(rp_b)
+    ov2 = ctx.stats()["seed_overflow_reads"]
+    compare_records(ro, rp1, label="first call")
+    compare_records(ro, rp2, label="second call")
+    assert ov1 > 0 and ov2 <= ov1, (ov1, ov2)
+
+
 @pytest.mark.parametrize("n_contigs,lmax", [(60, 30), (150, 60)])
 def test_map_se_many_contig_groups(n_contigs, lmax):
     """A 400 bp unit in every contig, in alternating orientation: short reads
