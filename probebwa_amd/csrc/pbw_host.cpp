@@ -55,6 +55,18 @@ struct DevBuf {
     HIP_CHECK(hipMalloc((void**)&p, bytes));
     n = std::max<size_t>(count, 1);
   }
+  void grow_keep(size_t count, hipStream_t s) {  // as ensure, keeping the current contents
+    if (count <= n && p) return;
+    T* q = nullptr;
+    HIP_CHECK(hipMalloc((void**)&q, std::max<size_t>(count, 1) * sizeof(T)));
+    if (p && n) {
+      HIP_CHECK(hipMemcpyAsync(q, p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+    }
+    release();
+    p = q;
+    n = std::max<size_t>(count, 1);
+  }
   void upload(const T* h, size_t count, hipStream_t s) {
     ensure(count);
     if (count) HIP_CHECK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
@@ -268,6 +280,8 @@ struct pbw_ctx {
   uint32_t* h_nh = nullptr;  // pinned: per-read hit counts of the seed pass
   size_t h_nh_cap = 0;
   uint32_t hit_cap = HIT_CAP;  // first-pass hits per read, adapted batch to batch
+  size_t pe_nws = 0;           // wide mate-window slots of the current pair batch
+  uint32_t pe_wcap = 0;        // CIGAR capacity per wide slot in that batch
   DevBuf<uint32_t> d_nhits, d_list, d_hcap, d_tier2, d_mid, d_ntier2, d_best, d_cigar, d_cigar_out, d_cigar_off;
   DevBuf<Cand> d_cands;
   DevBuf<uint8_t> d_ncand, d_njobs, d_rec, d_refbytes;
@@ -1150,10 +1164,11 @@ static uint64_t rust_f64_to_usize(double x) {  // `as usize`: saturating, NaN ->
   return (uint64_t)x;
 }
 
-static void map_pe_batch(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_batch* m2, uint64_t p0, uint64_t p1,
-                         const HostInsert& snap, HostRecords& hr, std::vector<double>& obs_out,
-                         std::vector<uint8_t>& has_out) {
-  hipStream_t s = c->stream;
+// The part of a batch of pairs that does not depend on the insert-size model:
+// both mates' reads (interleaved r1, r2) through seeding, clustering and the
+// single-end DP.
+static void pe_prepare(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_batch* m2, uint64_t p0, uint64_t p1,
+                       Batch& b) {
   const uint32_t P = (uint32_t)(p1 - p0);
   std::vector<const uint8_t*> sp(2 * P), qp(2 * P);
   std::vector<uint64_t> sl(2 * P), ql(2 * P);
@@ -1168,12 +1183,28 @@ static void map_pe_batch(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_ba
       ql[2 * i + k] = x->qual_off[r + 1] - x->qual_off[r];
     }
   }
-  Batch b;
   batch_upload(c, b, sp, sl, qp, ql);
   batch_candidates(c, b);
   batch_dp(c, b, nullptr, 0);
-  auto t0 = std::chrono::steady_clock::now();
-  // cross-alignment plan
+  const size_t nws = (size_t)P * 40;
+  c->d_wjobs.ensure(nws);
+  c->d_wvalid.ensure(nws);
+  c->d_slcand.ensure(nws);
+  c->d_plan.ensure(P);
+  c->d_wout.ensure(nws);
+  c->d_rec.ensure((size_t)2 * P * sizeof(pbw_record));
+  c->d_cig_src.ensure(2 * (size_t)P);
+  c->d_obs.ensure(P);
+  c->d_has_obs.ensure(P);
+}
+
+// Pairs [q0, q1) of a prepared batch under the model snapshot `snap`: cross-
+// alignment plan, wide mate windows, joint posterior and records (on the
+// device); their observations to obs_out / has_out.
+static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, const HostInsert& snap,
+                     std::vector<double>& obs_out, std::vector<uint8_t>& has_out) {
+  hipStream_t s = c->stream;
+  const uint32_t P = q1 - q0;
   PairArgs pa{};
   pa.G = c->dev_genome();
   pa.R = b.R;
@@ -1185,11 +1216,6 @@ static void map_pe_batch(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_ba
   pa.njobs = c->d_njobs.p;
   pa.cands = c->d_cands.p;
   pa.rcode = c->d_rcode.p;
-  const size_t nws = (size_t)P * 40;
-  c->d_wjobs.ensure(nws);
-  c->d_wvalid.ensure(nws);
-  c->d_slcand.ensure(nws);
-  c->d_plan.ensure(P);
   pa.wjobs = c->d_wjobs.p;
   pa.wvalid = c->d_wvalid.p;
   pa.slcand = c->d_slcand.p;
@@ -1202,46 +1228,52 @@ static void map_pe_batch(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_ba
   pa.radius_base = rust_f64_to_usize(std::ceil(5.0 * snap.sd));
   pa.name_class = c->d_name_class.p;
   pa.P = P;
+  pa.p_base = q0;
   launch_pair_plan(pa, s);
   HIP_CHECK(hipGetLastError());
+  const size_t nws = (size_t)P * 40, ws0 = (size_t)q0 * 40;
   std::vector<uint8_t> wv(nws);
-  HIP_CHECK(hipMemcpyAsync(wv.data(), c->d_wvalid.p, nws, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(wv.data(), c->d_wvalid.p + ws0, nws, hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
   std::vector<uint32_t> wlist;
   for (size_t k = 0; k < nws; k++)
-    if (wv[k]) wlist.push_back((uint32_t)k);
+    if (wv[k]) wlist.push_back((uint32_t)(ws0 + k));
   // wide DP (align_at_window, single.rs:220-275): window <= 2 radius + L
   const uint64_t wmax = 2 * (pa.radius_base + b.max_m) + b.max_m;
   if (!wlist.empty() && wmax > 8000)
     throw PbwError{PBW_ERR_LIMIT, "insert-size search window wider than 8000 bases"};
   const uint32_t wcap = b.max_m + (uint32_t)wmax + 2;
-  c->d_wout.ensure(nws);
-  c->d_wcigar.ensure(nws * wcap);
+  // the wide CIGAR pool holds the batch's slots at one stride, which only grows
+  // within a batch: slots written earlier keep their places (a record's CIGAR
+  // source is an absolute offset), and a larger stride never overlaps them
+  if (wcap > c->pe_wcap) {
+    if (c->pe_wcap) c->d_wcigar.grow_keep(c->pe_nws * (size_t)wcap, s);
+    else c->d_wcigar.ensure(c->pe_nws * (size_t)wcap);
+    c->pe_wcap = wcap;
+  }
   DpArgs da = dp_args_base(c, b.R);
   da.jobs = c->d_wjobs.p;
-  da.cigar_cap = wcap;
+  da.cigar_cap = c->pe_wcap;
   da.cigar = c->d_wcigar.p;
   da.out = c->d_wout.p;
   da.max_m = b.max_m;
   da.max_n = (uint32_t)wmax;
   c->stats.wide_jobs += wlist.size();
-  Counters before{};
-  HIP_CHECK(hipMemcpyAsync(&before, c->d_ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
-  run_dp_tiled(c, da, wlist);
-  Counters after{};
-  HIP_CHECK(hipMemcpyAsync(&after, c->d_ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
-  c->stats.wide_cells += after.cells - before.cells;
-  HIP_CHECK(hipMemcpyAsync(c->d_ctr.p, &before, sizeof(Counters), hipMemcpyHostToDevice, s));
+  if (!wlist.empty()) {
+    Counters before{};
+    HIP_CHECK(hipMemcpyAsync(&before, c->d_ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    run_dp_tiled(c, da, wlist);
+    Counters after{};
+    HIP_CHECK(hipMemcpyAsync(&after, c->d_ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    c->stats.wide_cells += after.cells - before.cells;
+    HIP_CHECK(hipMemcpyAsync(c->d_ctr.p, &before, sizeof(Counters), hipMemcpyHostToDevice, s));
+  }
   // joint posterior + records
-  c->d_rec.ensure((size_t)2 * P * sizeof(pbw_record));
-  c->d_cig_src.ensure(2 * (size_t)P);
-  c->d_obs.ensure(P);
-  c->d_has_obs.ensure(P);
   pa.wout = c->d_wout.p;
   pa.wcigar = c->d_wcigar.p;
-  pa.wcap = wcap;
+  pa.wcap = c->pe_wcap;
   pa.rec = c->d_rec.p;
   pa.cig_src = c->d_cig_src.p;
   pa.obs = c->d_obs.p;
@@ -1250,8 +1282,22 @@ static void map_pe_batch(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_ba
   HIP_CHECK(hipGetLastError());
   obs_out.resize(P);
   has_out.resize(P);
-  HIP_CHECK(hipMemcpyAsync(obs_out.data(), c->d_obs.p, (size_t)P * 8, hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipMemcpyAsync(has_out.data(), c->d_has_obs.p, P, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(obs_out.data(), c->d_obs.p + q0, (size_t)P * 8, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(has_out.data(), c->d_has_obs.p + q0, P, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// Pairs [p0, p1) in one batch on the model snapshot `snap`.
+static void map_pe_batch(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_batch* m2, uint64_t p0, uint64_t p1,
+                         const HostInsert& snap, HostRecords& hr, std::vector<double>& obs_out,
+                         std::vector<uint8_t>& has_out) {
+  const uint32_t P = (uint32_t)(p1 - p0);
+  Batch b;
+  pe_prepare(c, m1, m2, p0, p1, b);
+  c->pe_nws = (size_t)P * 40;
+  c->pe_wcap = 0;
+  auto t0 = std::chrono::steady_clock::now();
+  pe_pairs(c, b, 0, P, snap, obs_out, has_out);
   download_records(c, b, 2 * P, c->d_cig_src.p, c->d_wcigar.p, hr);
   c->stats.ms_pair += ms_since(t0);
   collect_counters(c, 2 * P);
@@ -1265,10 +1311,23 @@ static void map_pe_all(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_batc
   std::vector<double> obs;
   std::vector<uint8_t> has;
   const uint64_t warm = std::min<uint64_t>(n, 500);
-  for (uint64_t i = 0; i < warm; i++) {
-    HostInsert snap = im;
-    map_pe_batch(c, m1, m2, i, i + 1, snap, hr, obs, has);
-    if (has[0]) im.update(obs[0]);
+  if (warm) {
+    // the warm-up pairs go through seeding, clustering and the single-end DP
+    // together (none of it reads the model); then pair by pair, each on the
+    // model its predecessors left, through the model-dependent stages
+    Batch b;
+    pe_prepare(c, m1, m2, 0, warm, b);
+    c->pe_nws = (size_t)warm * 40;
+    c->pe_wcap = 0;
+    auto t0 = std::chrono::steady_clock::now();
+    for (uint64_t i = 0; i < warm; i++) {
+      HostInsert snap = im;
+      pe_pairs(c, b, (uint32_t)i, (uint32_t)i + 1, snap, obs, has);
+      if (has[0]) im.update(obs[0]);
+    }
+    download_records(c, b, 2 * (uint32_t)warm, c->d_cig_src.p, c->d_wcigar.p, hr);
+    c->stats.ms_pair += ms_since(t0);
+    collect_counters(c, 2 * (uint32_t)warm);
   }
   for (uint64_t b0 = warm; b0 < n; b0 += 2000) {
     uint64_t b1 = std::min<uint64_t>(b0 + 2000, n);

@@ -278,7 +278,8 @@ struct PairArgs {
   uint64_t* cig_src;       // per record CIGAR location (bit 63: cross-alignment pool)
   double* obs;
   uint8_t* has_obs;
-  uint32_t P;
+  uint32_t P;              // pairs handled by this launch: p_base .. p_base + P - 1 of the batch
+  uint32_t p_base;
 };
 void launch_pair_plan(const PairArgs& a, hipStream_t s);
 void launch_pair_finalize(const PairArgs& a, hipStream_t s);
