@@ -1,20 +1,28 @@
 // HIP kernels (gfx950) of the probebwa map path.
 //
 //   k_seed         seeds.rs:91-151 + hashtable.rs:90-124 — wavefront per read:
-//                  15-mer codes in LDS, exact + 1-mismatch-neighbour probes of
-//                  the dense code-indexed bucket table, coalesced bucket reads,
-//                  similarity filter on the 2-bit genome, hits compacted in
+//                  15-mer codes in LDS, exact probes of the dense code-indexed
+//                  bucket table, the 45 one-mismatch neighbours' presence from
+//                  rotated presence bitmaps, lookups of the present ones, a
+//                  queue of buckets whose positions are read in full rounds of
+//                  64, similarity filter on the 2-bit genome, hits compacted in
 //                  emission order with ballots.
 //   k_cluster      candidates.rs:30-78 — wavefront per read: bitonic sort of
-//                  (group, diagonal) keys in LDS, segmented merge, FxHashMap
-//                  group order emulation, top-20 selection.
+//                  (group, diagonal) keys in registers, segmented merge,
+//                  FxHashMap group order emulation, top-20 selection; a block
+//                  per read (k_cluster_tier2) for reads with many hits.
 //   k_build_jobs   single.rs:105-133 — candidate windows -> DP jobs.
 //   k_dp_band      smith_waterman.rs:163-255 — wavefront per job, one lane per
 //                  band column, row state in registers, the deletion recurrence
 //                  as an exact max-plus prefix scan across lanes, traceback
 //                  bits in LDS, traceback + log-likelihood on lane 0.
+//   k_dp_quad      the same recurrence, four jobs per wavefront (one 16-lane
+//                  DPP row each, four band columns per lane): the main DP.
+//   k_dp_pair16    k_dp_quad with two jobs per row in saturating 16-bit halves
+//                  (a tune option).
 //   k_dp_tiled     the same recurrence for any band width (64-column tiles,
-//                  row state in global scratch): wide windows of align_at_window.
+//                  row state in LDS or global scratch): wide windows of
+//                  align_at_window.
 //   k_mapq         bayesian.rs:142-439 — FP64 select_best + finalize_mapq per read.
 //
 // All FP64 code is compiled with -ffp-contract=off: the reference (Rust)
@@ -1886,7 +1894,8 @@ void launch_dp_band(const DpArgs& a0, hipStream_t s) {
 //    j <= n, never reach the endpoint scan and never lie on the traceback
 //    path.  Unreachable values stay within [-2^31, -2^29).
 //  - Traceback bits go into four words per lane with v_alignbit (planes:
-//    M came from I or D; M came from D; tb_i; tb_d) and to LDS every 8 rows.
+//    M came from I or D; M came from D; tb_i; tb_d) and to the job's area in
+//    global scratch every 8 rows.
 //    One lane per job walks them, taking runs of diagonal M steps from one
 //    word, and pushes CIGAR runs into the consumed top of its traceback area.
 // ---------------------------------------------------------------------------
