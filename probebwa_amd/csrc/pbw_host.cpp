@@ -285,7 +285,7 @@ struct pbw_ctx {
   DevBuf<uint32_t> d_nhits, d_list, d_hcap, d_tier2, d_mid, d_ntier2, d_best, d_cigar, d_cigar_out, d_cigar_off;
   DevBuf<Cand> d_cands;
   DevBuf<uint8_t> d_ncand, d_njobs, d_rec, d_refbytes;
-  DevBuf<uint32_t> d_flag_a, d_flag_b, d_flag_c, d_flag_n, d_sorted, d_sort_hist, d_tbq;
+  DevBuf<uint32_t> d_flag_a, d_flag_b, d_flag_c, d_flag_n, d_sorted, d_sort_hist, d_tbq, d_flag_w, d_flag_wn;
   pbw_tune tune{0, 0, 0, 0};  // batch_reads mirrors opt.batch_reads; zeros elsewhere: defaults
   DevBuf<DpJob> d_jobs;
   DevBuf<DpOut> d_out;
@@ -558,8 +558,31 @@ static void run_dp_tiled_dev(pbw_ctx* c, DpArgs a, const uint32_t* dlist, uint32
 }
 static void run_dp_tiled(pbw_ctx* c, DpArgs a, const std::vector<uint32_t>& list) {
   if (list.empty()) return;
-  c->d_list.upload(list.data(), list.size(), c->stream);
-  run_dp_tiled_dev(c, a, c->d_list.p, (uint32_t)list.size());
+  hipStream_t s = c->stream;
+  const uint32_t n = (uint32_t)list.size();
+  c->d_list.upload(list.data(), n, s);
+  const uint32_t T = dp_wide_tiles(a.max_n);
+  if (!T) {
+    run_dp_tiled_dev(c, a, c->d_list.p, n);
+    return;
+  }
+  // whole-window rectangles in k_dp_wide; the jobs it flags go to k_dp_tiled
+  DpArgs w = a;
+  w.jlist = c->d_list.p;
+  w.njobs = n;
+  w.job_valid = nullptr;
+  c->d_flag_w.ensure(n);
+  c->d_flag_wn.ensure(1);
+  HIP_CHECK(hipMemsetAsync(c->d_flag_wn.p, 0, 4, s));
+  w.flag_list = c->d_flag_w.p;
+  w.flag_n = c->d_flag_wn.p;
+  const uint32_t nwaves = std::min<uint32_t>(n, 8192);
+  c->d_tb.ensure((size_t)nwaves * (a.max_m + 1) * T * 16);  // 16 bits per lane, tile and row
+  w.tb = c->d_tb.p;
+  launch_dp_wide(w, nwaves, s);
+  HIP_CHECK(hipGetLastError());
+  const uint32_t nflag = download_u32(c, c->d_flag_wn.p);
+  if (nflag) run_dp_tiled_dev(c, a, c->d_flag_w.p, nflag);
 }
 
 // Device state of one batch between the stages.

@@ -192,6 +192,10 @@ struct DpArgs {
 };
 void launch_dp_band(const DpArgs& a, hipStream_t s);
 void launch_dp_tiled(const DpArgs& a, uint32_t nwaves, hipStream_t s);
+// Windows whose band covers every row (paired-end mate windows) up to 1024
+// columns: k_dp_wide, which flags every other job to (flag_list, flag_n).
+uint32_t dp_wide_tiles(uint32_t max_n);  // 256-column tiles for max_n, 0: too wide
+void launch_dp_wide(const DpArgs& a, uint32_t nwaves, hipStream_t s);
 // true: the tiled kernel keeps its previous row in LDS (pass rows = nullptr)
 bool dp_tiled_rows_in_lds(uint32_t max_m, uint32_t max_n);
 bool launch_dp_quad(const DpArgs& a, hipStream_t s);  // false: layout too large, nothing launched

@@ -3405,6 +3405,204 @@ void launch_dp_tiled(const DpArgs& a, uint32_t nwaves, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
+// Wide windows whose band covers every row entirely (the align_at_window mate
+// windows of paired-end: js = 1 and je = n on every row, single.rs:220-258):
+// k_dp_tiled's recurrence with four columns per lane, 256-column tiles and
+// the previous row's M, I and D in registers (T tiles, n <= 256 T).
+//  - Up neighbour: the same register; diagonal: the lane's previous column or,
+//    for its first column, the left lane's last one by a wave_shr DPP move
+//    (lane 0 takes the previous tile's last column by readlane).
+//  - D: lane-local max-plus prefix over four columns, then an inclusive max
+//    scan over the wave of (A - position * ge), exactly as k_dp_tiled's 64
+//    lanes but one scan per 256 columns.
+//  - Traceback bits as k_dp_tiled's (M from I, M from D, I extended, D
+//    extended), 16 bits per lane, tile and row in global scratch.
+// Jobs of another shape, or wider than 256 T columns, are flagged to
+// k_dp_tiled.
+// ---------------------------------------------------------------------------
+template <int T>
+__global__ __launch_bounds__(64) void k_dp_wide(DpArgs a) {
+  extern __shared__ __align__(16) uint8_t dyn[];
+  const int lane = lane_id();
+  DpStage st;
+  st.q = dyn;
+  st.qq = st.q + ((a.max_m + 15) & ~15u);
+  st.ref = st.qq + ((a.max_m + 15) & ~15u);
+  st.go = (int16_t*)(st.ref + ((a.max_n + 15) & ~15u));
+  st.runs = (uint32_t*)((uint8_t*)st.go + (((a.max_n * 2) + 15) & ~15u));
+  uint16_t* TB = (uint16_t*)a.tb + (uint64_t)blockIdx.x * (uint64_t)(a.max_m + 1) * T * 64;
+  const int32_t ge = a.gap_extend;
+  unsigned long long c_cells = 0;
+  for (uint32_t t = blockIdx.x; t < a.njobs; t += gridDim.x) {
+    const uint32_t jid = a.jlist ? a.jlist[t] : t;
+    const DpJob job = a.jobs[jid];
+    const uint64_t so = a.R.seq_off[job.read], qo = a.R.qual_off[job.read];
+    const uint32_t m = (uint32_t)(a.R.seq_off[job.read + 1] - so);
+    const uint32_t qlen = (uint32_t)(a.R.qual_off[job.read + 1] - qo);
+    const uint32_t n = job.n;
+    const int64_t b = job.band, h = job.hint;
+    const bool rect = m >= 1 && n >= 1 && m <= a.max_m && n <= a.max_n && n <= (uint32_t)(256 * T) &&
+                      !job.explicit_ref && (int64_t)m + h - b <= 1 && 1 + h + b >= (int64_t)n;
+    if (!rect) {
+      if (lane == 0) dp_flag(a, jid);
+      continue;
+    }
+    dp_stage_job(a, job, m, qlen, so, qo, st);
+    // per-column constants: reference byte and gap open of column c = 256 tt + 4 lane + k + 1
+    uint32_t rbyte[T][4];
+    int32_t gop[T][4];
+    int32_t PM[T][4], PI[T][4], PD[T][4];
+#pragma unroll
+    for (int tt = 0; tt < T; tt++) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const uint32_t c = 256u * tt + 4u * lane + k + 1u;
+        const bool v = c <= n;
+        rbyte[tt][k] = v ? st.ref[c - 1] : 0x100u;
+        gop[tt][k] = v ? (int32_t)st.go[c - 1] : -1;
+        PM[tt][k] = 0;  // row 0: M = 0 on [0, n], I = D = NEG_INF
+        PI[tt][k] = NEG_INF;
+        PD[tt][k] = NEG_INF;
+      }
+    }
+    for (uint32_t i = 1; i <= m; i++) {
+      const uint32_t rb = st.q[i - 1];
+      const uint32_t qv = (i - 1) < qlen ? st.qq[i - 1] : 30u;
+      const int32_t mis = a.T.mis_pen[qv];
+      // previous row at the column left of the tile (column 0 is in the band only in row 0)
+      int32_t cM = i == 1 ? 0 : NEG_INF, cI = NEG_INF, cD = NEG_INF;
+      int32_t curMl = NEG_INF, curDl = NEG_INF;  // current row, column left of the tile
+#pragma unroll
+      for (int tt = 0; tt < T; tt++) {
+        const int32_t dM0 = dpp_wave_shr1(cM, PM[tt][3]);
+        const int32_t dI0 = dpp_wave_shr1(cI, PI[tt][3]);
+        const int32_t dD0 = dpp_wave_shr1(cD, PD[tt][3]);
+        cM = readlane_i32(PM[tt][3], 63);
+        cI = readlane_i32(PI[tt][3], 63);
+        cD = readlane_i32(PD[tt][3], 63);
+        int32_t M[4], I[4];
+        uint32_t bits = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const uint32_t c = 256u * tt + 4u * lane + k + 1u;
+          const bool v = c <= n;
+          const int32_t dM = k ? PM[tt][k - 1] : dM0, dI = k ? PI[tt][k - 1] : dI0, dD = k ? PD[tt][k - 1] : dD0;
+          const int32_t sc = (rb == rbyte[tt][k]) ? MATCH_SCORE : mis;
+          const int32_t fm = dM + sc, fi = dI + sc, fd = dD + sc;
+          int32_t Mk = max(max(fm, fi), fd);
+          if (!v) Mk = NEG_INF;
+          const int32_t io = PM[tt][k] + gop[tt][k], ie = PI[tt][k] + ge;
+          const int32_t Ik = max(io, ie);
+          M[k] = Mk;
+          I[k] = Ik;
+          const uint32_t b0 = (v && Mk != fm && Mk == fi) ? 1u : 0u;
+          const uint32_t b1 = (v && Mk != fm && Mk != fi) ? 2u : 0u;
+          const uint32_t b2 = (v && Ik != io) ? 4u : 0u;
+          bits |= (b0 | b1 | b2) << (4 * k);
+        }
+        // D: A(c) = M(c - 1) + go(c); D(c) = max(A(c), D(c - 1) + ge)
+        const int32_t Ml0 = dpp_wave_shr1(curMl, M[3]);
+        int32_t A[4], key[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const uint32_t c = 256u * tt + 4u * lane + k + 1u;
+          const int32_t pos = 4 * lane + k;
+          A[k] = (k ? M[k - 1] : Ml0) + gop[tt][k];
+          key[k] = c <= n ? A[k] - pos * ge : INT32_MIN;
+        }
+        const int32_t l1 = max(key[0], key[1]), l2 = max(l1, key[2]), l3 = max(l2, key[3]);
+        const int32_t incl = wave_incl_max(l3);
+        const int32_t ex = dpp_wave_shr1(INT32_MIN, incl);  // lanes left of this one
+        const int32_t Sent = curDl + ge;
+        const int32_t base = max(ex, Sent);
+        int32_t D[4];
+        D[0] = 4 * lane * ge + max(key[0], base);
+        D[1] = (4 * lane + 1) * ge + max(l1, base);
+        D[2] = (4 * lane + 2) * ge + max(l2, base);
+        D[3] = (4 * lane + 3) * ge + max(l3, base);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const uint32_t c = 256u * tt + 4u * lane + k + 1u;
+          bits |= ((c <= n && D[k] != A[k]) ? 8u : 0u) << (4 * k);
+        }
+        curMl = readlane_i32(M[3], 63);
+        curDl = readlane_i32(D[3], 63);
+        TB[((uint64_t)i * T + tt) * 64 + lane] = (uint16_t)bits;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          PM[tt][k] = M[k];
+          PI[tt][k] = I[k];
+          PD[tt][k] = D[k];
+        }
+      }
+    }
+    // endpoint over row m: last maximum over (j asc; M then I)
+    int64_t key = INT64_MIN;
+#pragma unroll
+    for (int tt = 0; tt < T; tt++) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const uint32_t c = 256u * tt + 4u * lane + k + 1u;
+        if (c <= n) {
+          const int64_t km = end_key(PM[tt][k], 2u * c), ki = end_key(PI[tt][k], 2u * c + 1u);
+          const int64_t k2 = km > ki ? km : ki;
+          key = key > k2 ? key : k2;
+        }
+      }
+    }
+    key = wave_max_i64(key);
+    {
+      const int64_t koob = end_key(NEG_INF, 1u);  // column 0 of the last row lies outside its band
+      key = key > koob ? key : koob;
+    }
+    const int32_t best_score = (int32_t)(key >> 32);
+    const uint32_t best_j = ((uint32_t)key) >> 1, best_state = ((uint32_t)key) & 1u;
+    mem_sync();  // the traceback words of every lane
+    uint32_t nruns = 0, off = 0, span = 0;
+    double lk = 0.0;
+    if (lane == 0) {
+      auto tb = [&](uint32_t ii, uint32_t jj) -> uint32_t {
+        const uint32_t rel = jj - 1, tt = rel >> 8, ln = (rel >> 2) & 63u, k = rel & 3u;
+        return ((uint32_t)TB[((uint64_t)ii * T + tt) * 64 + ln] >> (4 * k)) & 15u;
+      };
+      dp_traceback(m, best_j, best_state, tb, st, nruns, off);
+      for (uint32_t t2 = 0; t2 < nruns; t2++) {
+        const uint32_t op = st.runs[t2] & 15u;
+        if (op == 0 || op == 2) span += st.runs[t2] >> 4;
+      }
+      lk = dp_log_lk(a.T, st, m, qlen, nruns, off, span);
+    }
+    nruns = shfl_u32(nruns, 0);
+    off = shfl_u32(off, 0);
+    span = shfl_u32(span, 0);
+    wave_sync();
+    const uint64_t cells = (uint64_t)m * n;
+    dp_emit(a, jid, st, nruns, off, span, best_score, lk, cells);
+    c_cells += cells;
+    wave_sync();
+  }
+  if (lane == 0 && a.ctr) atomicAdd(&a.ctr->cells, c_cells);
+}
+
+// Tiles of k_dp_wide for windows of up to max_n columns (0: too wide for it).
+uint32_t dp_wide_tiles(uint32_t max_n) {
+  const uint32_t t = (max_n + 255) / 256;
+  return t <= 4 ? (t ? t : 1) : 0;
+}
+void launch_dp_wide(const DpArgs& a, uint32_t nwaves, hipStream_t s) {
+  if (a.njobs == 0) return;
+  const uint32_t blocks = nwaves < a.njobs ? nwaves : a.njobs;
+  const size_t lds = dp_tiled_lds_bytes(a.max_m, a.max_n);
+  switch (dp_wide_tiles(a.max_n)) {
+    case 1: hipLaunchKernelGGL(k_dp_wide<1>, dim3(blocks), dim3(64), lds, s, a); break;
+    case 2: hipLaunchKernelGGL(k_dp_wide<2>, dim3(blocks), dim3(64), lds, s, a); break;
+    case 3: hipLaunchKernelGGL(k_dp_wide<3>, dim3(blocks), dim3(64), lds, s, a); break;
+    case 4: hipLaunchKernelGGL(k_dp_wide<4>, dim3(blocks), dim3(64), lds, s, a); break;
+    default: break;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // MAPQ: bayesian.rs:193-439 (thread per read, FP64)
 // ---------------------------------------------------------------------------
 constexpr double LN_10 = 2.302585092994045684017991454684364208;
