@@ -3442,7 +3442,7 @@ __global__ __launch_bounds__(64) void k_dp_wide(DpArgs a) {
     const uint32_t n = job.n;
     const int64_t b = job.band, h = job.hint;
     const bool rect = m >= 1 && n >= 1 && m <= a.max_m && n <= a.max_n && n <= (uint32_t)(256 * T) &&
-                      !job.explicit_ref && (int64_t)m + h - b <= 1 && 1 + h + b >= (int64_t)n;
+                      (int64_t)m + h - b <= 1 && 1 + h + b >= (int64_t)n;
     if (!rect) {
       if (lane == 0) dp_flag(a, jid);
       continue;
