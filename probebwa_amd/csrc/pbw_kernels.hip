@@ -20,9 +20,10 @@
 //                  DPP row each, four band columns per lane): the main DP.
 //   k_dp_pair16    k_dp_quad with two jobs per row in saturating 16-bit halves
 //                  (a tune option).
+//   k_dp_wide      windows whose band covers every row (paired-end mate
+//                  windows): four columns per lane, previous row in registers.
 //   k_dp_tiled     the same recurrence for any band width (64-column tiles,
-//                  row state in LDS or global scratch): wide windows of
-//                  align_at_window.
+//                  row state in LDS or global scratch): the remaining shapes.
 //   k_mapq         bayesian.rs:142-439 — FP64 select_best + finalize_mapq per read.
 //
 // All FP64 code is compiled with -ffp-contract=off: the reference (Rust)
