@@ -985,7 +985,7 @@ static void batch_dp(pbw_ctx* c, Batch& b, const uint32_t* jlist, uint32_t nlist
 // the batch's word count and copies records and runs straight into the arrays
 // the output batch takes over.
 static void download_records(pbw_ctx* c, const Batch& b, uint32_t nrec, const uint64_t* d_src, const uint32_t* pool1,
-                             HostRecords& hr) {
+                             HostRecords& hr, uint32_t rec0 = 0) {
   hipStream_t s = c->stream;
   if (!nrec) {
     HIP_CHECK(hipStreamSynchronize(s));
@@ -997,17 +997,19 @@ static void download_records(pbw_ctx* c, const Batch& b, uint32_t nrec, const ui
   const uint32_t nb = (nrec + REC_BLOCK - 1) / REC_BLOCK;
   c->d_rec_bsum.ensure(nb + 1);
   uint32_t* d_total = c->d_rec_bsum.p + nb;
-  launch_rec_offsets(c->d_rec.p, d_src, nrec, base, c->d_rec_bsum.p, d_total, s);
+  uint8_t* d_rec = c->d_rec.p + (size_t)rec0 * sizeof(pbw_record);
+  d_src += rec0;
+  launch_rec_offsets(d_rec, d_src, nrec, base, c->d_rec_bsum.p, d_total, s);
   HIP_CHECK(hipGetLastError());
   pbw_record* hrec = hr.rec.grow(nrec);
   uint32_t total = 0;
-  HIP_CHECK(hipMemcpyAsync(hrec, c->d_rec.p, (size_t)nrec * sizeof(pbw_record), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(hrec, d_rec, (size_t)nrec * sizeof(pbw_record), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipMemcpyAsync(&total, d_total, 4, hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
   if ((uint64_t)base + total > 0xFFFFFFFFull)
     throw PbwError{PBW_ERR_LIMIT, "more than 2^32 CIGAR words in one call; map fewer reads per call"};
   c->d_cigar_out.ensure(std::max<uint32_t>(total, 1));
-  launch_rec_cigar(c->d_rec.p, d_src, c->d_cigar.p, pool1 ? pool1 : c->d_cigar.p, b.R.seq_off, nrec, base,
+  launch_rec_cigar(d_rec, d_src, c->d_cigar.p, pool1 ? pool1 : c->d_cigar.p, b.R.seq_off + rec0, nrec, base,
                    c->d_cigar_out.p, s);
   HIP_CHECK(hipGetLastError());
   uint32_t* hc = hr.cigar.grow(total);
@@ -1223,8 +1225,9 @@ static void pe_prepare(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_batc
 
 // Pairs [q0, q1) of a prepared batch under the model snapshot `snap`: cross-
 // alignment plan, wide mate windows, joint posterior and records (on the
-// device); their observations to obs_out / has_out.
-static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, const HostInsert& snap,
+// device); their observations to obs_out / has_out.  The cross-alignment
+// CIGAR pool holds slots wbase .. wbase + pe_nws - 1.
+static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, uint64_t wbase, const HostInsert& snap,
                      std::vector<double>& obs_out, std::vector<uint8_t>& has_out) {
   hipStream_t s = c->stream;
   const uint32_t P = q1 - q0;
@@ -1278,6 +1281,7 @@ static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, const
   da.jobs = c->d_wjobs.p;
   da.cigar_cap = c->pe_wcap;
   da.cigar = c->d_wcigar.p;
+  da.cigar_base = wbase;
   da.out = c->d_wout.p;
   da.max_m = b.max_m;
   da.max_n = (uint32_t)wmax;
@@ -1297,6 +1301,7 @@ static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, const
   pa.wout = c->d_wout.p;
   pa.wcigar = c->d_wcigar.p;
   pa.wcap = c->pe_wcap;
+  pa.wbase = wbase;
   pa.rec = c->d_rec.p;
   pa.cig_src = c->d_cig_src.p;
   pa.obs = c->d_obs.p;
@@ -1308,22 +1313,6 @@ static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, const
   HIP_CHECK(hipMemcpyAsync(obs_out.data(), c->d_obs.p + q0, (size_t)P * 8, hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipMemcpyAsync(has_out.data(), c->d_has_obs.p + q0, P, hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
-}
-
-// Pairs [p0, p1) in one batch on the model snapshot `snap`.
-static void map_pe_batch(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_batch* m2, uint64_t p0, uint64_t p1,
-                         const HostInsert& snap, HostRecords& hr, std::vector<double>& obs_out,
-                         std::vector<uint8_t>& has_out) {
-  const uint32_t P = (uint32_t)(p1 - p0);
-  Batch b;
-  pe_prepare(c, m1, m2, p0, p1, b);
-  c->pe_nws = (size_t)P * 40;
-  c->pe_wcap = 0;
-  auto t0 = std::chrono::steady_clock::now();
-  pe_pairs(c, b, 0, P, snap, obs_out, has_out);
-  download_records(c, b, 2 * P, c->d_cig_src.p, c->d_wcigar.p, hr);
-  c->stats.ms_pair += ms_since(t0);
-  collect_counters(c, 2 * P);
 }
 
 // lib.rs:145-185: 500 serial warm-up pairs, then 2000-pair batches
@@ -1345,19 +1334,37 @@ static void map_pe_all(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_batc
     auto t0 = std::chrono::steady_clock::now();
     for (uint64_t i = 0; i < warm; i++) {
       HostInsert snap = im;
-      pe_pairs(c, b, (uint32_t)i, (uint32_t)i + 1, snap, obs, has);
+      pe_pairs(c, b, (uint32_t)i, (uint32_t)i + 1, 0, snap, obs, has);
       if (has[0]) im.update(obs[0]);
     }
     download_records(c, b, 2 * (uint32_t)warm, c->d_cig_src.p, c->d_wcigar.p, hr);
     c->stats.ms_pair += ms_since(t0);
     collect_counters(c, 2 * (uint32_t)warm);
   }
-  for (uint64_t b0 = warm; b0 < n; b0 += 2000) {
-    uint64_t b1 = std::min<uint64_t>(b0 + 2000, n);
-    HostInsert snap = im;
-    map_pe_batch(c, m1, m2, b0, b1, snap, hr, obs, has);
-    for (uint64_t k = 0; k < b1 - b0; k++)
-      if (has[k]) im.update(obs[k]);
+  // the 2000-pair batches: seeding, clustering and the single-end DP for many
+  // of them at once (whole batches, so the boundaries stay warm + 2000 k); then
+  // batch by batch, each on the model the batches before it left, the
+  // model-dependent stages and that batch's records
+  const uint64_t bs = c->opt.batch_reads ? c->opt.batch_reads : 131072;
+  const uint64_t per_prep = std::max<uint64_t>(1, bs / 2 / 2000) * 2000;
+  for (uint64_t s0 = warm; s0 < n; s0 += per_prep) {
+    const uint64_t s1 = std::min<uint64_t>(s0 + per_prep, n);
+    Batch b;
+    pe_prepare(c, m1, m2, s0, s1, b);
+    for (uint64_t b0 = s0; b0 < s1; b0 += 2000) {
+      const uint64_t b1 = std::min<uint64_t>(b0 + 2000, s1);
+      const uint32_t q0 = (uint32_t)(b0 - s0), q1 = (uint32_t)(b1 - s0);
+      HostInsert snap = im;
+      c->pe_nws = (size_t)(q1 - q0) * 40;
+      c->pe_wcap = 0;
+      auto t0 = std::chrono::steady_clock::now();
+      pe_pairs(c, b, q0, q1, (uint64_t)q0 * 40, snap, obs, has);
+      download_records(c, b, 2 * (q1 - q0), c->d_cig_src.p, c->d_wcigar.p, hr, 2 * q0);
+      c->stats.ms_pair += ms_since(t0);
+      for (uint64_t k = 0; k < b1 - b0; k++)
+        if (has[k]) im.update(obs[k]);
+    }
+    collect_counters(c, 2 * (uint32_t)(s1 - s0));
   }
   model->mean = im.mean;
   model->sd = im.sd;

@@ -175,7 +175,8 @@ struct DpArgs {
   const uint8_t* ref_bytes;  // explicit-ref source
   int32_t gap_open, gap_extend;
   uint32_t cigar_cap;
-  uint32_t* cigar;         // job * cigar_cap
+  uint32_t* cigar;         // job jid at (jid - cigar_base) * cigar_cap
+  uint64_t cigar_base;     // first job id the CIGAR pool holds
   DpOut* out;
   uint32_t* flag_list;  // jobs this kernel does not handle, appended in any order (flag_n counts them)
   uint32_t* flag_n;
@@ -272,8 +273,9 @@ struct PairArgs {
   uint8_t* wvalid;
   uint8_t* slcand;         // shortlist candidate of each slot
   const DpOut* wout;
-  const uint32_t* wcigar;
+  const uint32_t* wcigar;  // slot s at (s - wbase) * wcap
   uint32_t wcap;
+  uint64_t wbase;          // first slot the cross-alignment CIGAR pool holds
   uint32_t* plan;
   double mean, sd, ln_norm, sv_prior_log;  // frozen insert-size snapshot
   uint64_t mean_usize, radius_base;        // mean.max(0) as usize, ceil(5 sd) as usize

@@ -1678,7 +1678,7 @@ __device__ void dp_emit(const DpArgs& a, uint32_t jid, const DpStage& st, uint32
                         int32_t score, double lk, uint64_t cells) {
   const int lane = lane_id();
   uint32_t ncopy = nruns < a.cigar_cap ? nruns : a.cigar_cap;
-  for (uint32_t t = lane; t < ncopy; t += 64) a.cigar[(uint64_t)jid * a.cigar_cap + t] = st.runs[nruns - 1 - t];
+  for (uint32_t t = lane; t < ncopy; t += 64) a.cigar[((uint64_t)jid - a.cigar_base) * a.cigar_cap + t] = st.runs[nruns - 1 - t];
   if (lane == 0) {
     DpOut o;
     o.log_lk = lk;
@@ -2446,7 +2446,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     // ---- CIGAR runs to the job's slot, forward order ----
     {
       const uint32_t ncopy = nruns < a.cigar_cap ? nruns : a.cigar_cap;
-      for (uint32_t u = L; u < ncopy; u += 16) a.cigar[(uint64_t)jid * a.cigar_cap + u] = runs[u];
+      for (uint32_t u = L; u < ncopy; u += 16) a.cigar[((uint64_t)jid - a.cigar_base) * a.cigar_cap + u] = runs[u];
     }
     wave_sync();
   }
@@ -3074,7 +3074,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
       if (ok) {
         const uint32_t* runs = (hh ? TBB : TBA) + Y.tb_words - nr;
         const uint32_t ncopy = nr < a.cigar_cap ? nr : a.cigar_cap;
-        for (uint32_t u = L; u < ncopy; u += 16) a.cigar[(uint64_t)jidv[hh] * a.cigar_cap + u] = runs[u];
+        for (uint32_t u = L; u < ncopy; u += 16) a.cigar[((uint64_t)jidv[hh] - a.cigar_base) * a.cigar_cap + u] = runs[u];
       }
     }
     wave_sync();
@@ -3808,21 +3808,21 @@ __device__ double missing_locus(const DevLuts& T, const uint8_t* qual, uint32_t 
   return fmin(offset_means / 5.0, 0.99);
 }
 
-// bayesian.rs:251-282 finalize_mapq
-__device__ void finalize_mapq(const DevGenome& G, const DevLuts& T, double sum_exp, const uint8_t* read_qual,
-                              uint32_t read_qlen, const uint8_t* rcode, uint32_t nwin, const uint32_t* cig,
-                              uint32_t ncig, const QueryView& Q, uint64_t gb, uint32_t span, uint8_t& mapq,
-                              double& posterior) {
+// bayesian.rs:251-282 finalize_mapq, given its two long parts: log_ent =
+// log_entropy(..) and missing = missing_locus(..), the latter read only when
+// the entropy term is positive (as the reference computes it only then).
+__device__ void finalize_mapq_from(const DevLuts& T, double sum_exp, double log_ent, double missing, uint8_t& mapq,
+                                   double& posterior) {
   double bp = 1.0 / fmax(sum_exp, 1.0);
   double pw = fmax(1.0 - bp, T.posterior_floor);
   double wrongmapping = py_int_round(log(pw) / DB);
-  double entropy = log_entropy(G, T, cig, ncig, Q, gb, span) - 3.0;
+  double entropy = log_ent - 3.0;
   if (entropy <= 0.0) {
     mapq = 0;
     posterior = 0.0;
     return;
   }
-  double p_missing = fmax(missing_locus(T, read_qual, read_qlen, rcode, nwin), 1e-30);
+  double p_missing = fmax(missing, 1e-30);
   double notfound = -10.0 * log10(p_missing);
   double combined = logadd(wrongmapping, notfound);
   double pp = fmin(fmax(logadd(entropy, combined), 0.0), 99.0);
@@ -3841,8 +3841,13 @@ struct RecOut {  // pbw_record layout
 };
 static_assert(sizeof(RecOut) == 64, "record layout");
 
+// Two lanes per read (2r + part), both in the same control flow: both take the
+// selection; part 0 walks the CIGAR for the log-entropy term, part 1 takes the
+// missing-locus term, and part 0 combines them and writes the record.
 __global__ void k_mapq(MapqArgs a) {
-  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t part = t & 1u;
+  uint32_t r = t >> 1;
   if (a.rlist) {
     if (r >= a.nlist) return;
     r = a.rlist[r];
@@ -3859,6 +3864,7 @@ __global__ void k_mapq(MapqArgs a) {
   o.pad[0] = o.pad[1] = o.pad[2] = 0;
   const uint32_t nj = a.njobs[r];
   if (nj == 0) {  // single.rs:362-383 unmapped_record
+    if (part) return;
     o.pos = 0;
     o.log_lk = 0.0;
     o.posterior = 0.0;
@@ -3894,10 +3900,14 @@ __global__ void k_mapq(MapqArgs a) {
   const DpOut dout = a.out[s0 + bi];
   QueryView Q{a.R.seq + so, a.R.qual + qo, L, qlen, job.rc != 0};
   const uint32_t* cig = a.cigar + (s0 + bi) * a.cigar_cap;
+  const double part_v = part == 0 ? log_entropy(a.G, a.T, cig, dout.cigar_n, Q, job.ref_gb + dout.ref_off, dout.span)
+                                  : missing_locus(a.T, a.R.qual + qo, qlen, a.rcode + so,
+                                                  L >= (uint32_t)K ? L - K + 1 : 0);
+  const double miss_v = __shfl_xor(part_v, 1, 64);  // part 0 receives the missing-locus term
+  if (part) return;
   uint8_t mapq;
   double post;
-  finalize_mapq(a.G, a.T, se, a.R.qual + qo, qlen, a.rcode + so, L >= (uint32_t)K ? L - K + 1 : 0, cig,
-                dout.cigar_n, Q, job.ref_gb + dout.ref_off, dout.span, mapq, post);
+  finalize_mapq_from(a.T, se, part_v, miss_v, mapq, post);
   const uint32_t contig = a.cands[job.cand].contig;
   o.pos = job.ref_gb - a.G.coff[contig] + dout.ref_off + 1;  // SAM is 1-based (single.rs:192)
   o.log_lk = mx;
@@ -3915,7 +3925,7 @@ __global__ void k_mapq(MapqArgs a) {
 void launch_mapq(const MapqArgs& a, hipStream_t s) {
   const uint32_t n = a.rlist ? a.nlist : a.R.n;
   if (n == 0) return;
-  hipLaunchKernelGGL(k_mapq, dim3((n + 127) / 128), dim3(128), 0, s, a);
+  hipLaunchKernelGGL(k_mapq, dim3((2 * n + 127) / 128), dim3(128), 0, s, a);
 }
 
 // ---------------------------------------------------------------------------

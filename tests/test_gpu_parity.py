@@ -351,3 +351,24 @@ def test_map_pe_matches_oracle(seed, n_pairs):
     compare_records(ro, rp, label="pairs seed %d" % seed)
     assert m.insert_model[2] == ctx.insert_model[2]
     assert m.insert_model[0] == ctx.insert_model[0] and m.insert_model[1] == ctx.insert_model[1]
+
+
+def test_map_pe_many_batches_matches_oracle():
+    """4700 pairs: the 500 warm-up pairs, then 2000-pair batches on frozen model
+    snapshots (lib.rs:145-185).  With the default read batch the three pair
+    batches share one seeding/clustering/DP pass; with 4000 reads per batch each
+    gets its own.  Both give the oracle's records and insert-size model."""
+    contigs = synth.small_genome(63, n_contigs=2, length=40000, repeat_copies=8)
+    go, gp = pair_of_genomes(contigs)
+    (a, b) = synth.reads_pairs(contigs, n=4700, L=100, seed=64, repeat_spans=())
+    o1, p1 = both_reads(*a)
+    o2, p2 = both_reads(*b)
+    m = O.Mapper(go, O.HashTable(go))
+    ro = m.map_pe(o1, o2)
+    for shape in (None, dict(batch_reads=4000)):
+        ctx = P.Context(gp)
+        if shape:
+            ctx.tune = shape
+        rp = ctx.map_pe(p1, p2)
+        compare_records(ro, rp, label="4700 pairs, tune %s" % shape)
+        assert tuple(m.insert_model) == tuple(ctx.insert_model)
