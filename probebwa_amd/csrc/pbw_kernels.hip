@@ -3669,28 +3669,52 @@ struct GenomeCursor {
 // gc_content (bayesian.rs:445-456) of genome[gb, gb+span): G/C bases and
 // non-N bases, counted 32 at a time (C = 01 and G = 10 are the codes whose
 // two bits differ).
+// Word w's share of genome[gb, end): G/C bases and non-N bases.
+__device__ __forceinline__ void gc_word(const DevGenome& G, uint64_t w, uint64_t gb, uint64_t end, uint32_t& gc,
+                                        uint32_t& total) {
+  const uint64_t lo = gb > (w << 5) ? gb : (w << 5), hi = end < ((w + 1) << 5) ? end : ((w + 1) << 5);
+  const uint32_t b0 = (uint32_t)(lo & 31), cnt = (uint32_t)(hi - lo);
+  const uint32_t sel = (cnt == 32 ? 0xFFFFFFFFu : ((1u << cnt) - 1u)) << b0;
+  const uint64_t g = G.g2b[w];
+  uint64_t x = (g ^ (g >> 1)) & 0x5555555555555555ull;  // bit 2i: the two bits of base i differ
+  x = (x | (x >> 1)) & 0x3333333333333333ull;
+  x = (x | (x >> 2)) & 0x0F0F0F0F0F0F0F0Full;
+  x = (x | (x >> 4)) & 0x00FF00FF00FF00FFull;
+  x = (x | (x >> 8)) & 0x0000FFFF0000FFFFull;
+  x = (x | (x >> 16)) & 0x00000000FFFFFFFFull;
+  const uint32_t nb = (uint32_t)(G.nmask[w >> 1] >> ((w & 1) * 32));
+  const uint32_t ok = sel & ~nb;
+  gc += (uint32_t)__popc((uint32_t)x & ok);
+  total += (uint32_t)__popc(ok);
+}
 __device__ void gc_count(const DevGenome& G, uint64_t gb, uint32_t span, uint32_t& gc, uint32_t& total) {
   gc = total = 0;
+  if (!span) return;
   const uint64_t end = gb + span;
-  for (uint64_t p = gb; p < end;) {
-    const uint64_t w = p >> 5;
-    const uint32_t b0 = (uint32_t)(p & 31);
-    const uint32_t b1 = (uint32_t)min<uint64_t>(32, end - (w << 5));
-    const uint32_t cnt = b1 - b0;
-    const uint32_t sel = (cnt == 32 ? 0xFFFFFFFFu : ((1u << cnt) - 1u)) << b0;
-    const uint64_t g = G.g2b[w];
-    uint64_t x = (g ^ (g >> 1)) & 0x5555555555555555ull;  // bit 2i: the two bits of base i differ
-    x = (x | (x >> 1)) & 0x3333333333333333ull;
-    x = (x | (x >> 2)) & 0x0F0F0F0F0F0F0F0Full;
-    x = (x | (x >> 4)) & 0x00FF00FF00FF00FFull;
-    x = (x | (x >> 8)) & 0x0000FFFF0000FFFFull;
-    x = (x | (x >> 16)) & 0x00000000FFFFFFFFull;
-    const uint32_t nb = (uint32_t)(G.nmask[w >> 1] >> ((w & 1) * 32));
-    const uint32_t ok = sel & ~nb;
-    gc += (uint32_t)__popc((uint32_t)x & ok);
-    total += (uint32_t)__popc(ok);
-    p = (w << 5) + b1;
+  for (uint64_t w = gb >> 5; w <= (end - 1) >> 5; w++) gc_word(G, w, gb, end, gc, total);
+}
+
+// The FP64 part of log_entropy (bayesian.rs:306-373) from its integer counts.
+__device__ double entropy_from_counts(const DevLuts& T, int64_t n, int64_t mm, uint32_t readlen, int64_t insertion_runs,
+                                      int64_t deletion_runs, uint32_t gc, uint32_t total) {
+  double lnb, lnbm1;
+  if (total <= (uint32_t)GC_LUT_MAX) {
+    uint64_t idx = (uint64_t)total * (total + 1) / 2 + gc;
+    lnb = T.gc_lnb[idx];
+    lnbm1 = T.gc_lnbm1[idx];
+  } else {
+    double f = total == 0 ? 0.5 : (double)gc / (double)total;
+    f = fmin(fmax(f, 1e-6), 1.0 - 1e-6);
+    double bb = exp(-f * log(f / 2.0) - (1.0 - f) * log((1.0 - f) / 2.0));
+    lnb = log(bb);
+    lnbm1 = log(fmax(bb - 1.0, 1e-6));
   }
+  double match_term = py_int_round(10.0 * ((double)n * lnb - T.ln_2g) / LN_10);
+  double mismatch_term = py_int_round(10.0 * (double)mm * lnbm1 / LN_10);
+  double position_choice = phred_n_choose_m(T, (int64_t)readlen, 2 * insertion_runs + deletion_runs);
+  double deletion_length = 0.0;
+  if (deletion_runs > 0) deletion_length = py_int_round((double)deletion_runs * 10.0 * T.ln_30 / LN_10);
+  return match_term - mismatch_term - phred_n_choose_m(T, n, mm) - position_choice - deletion_length;
 }
 
 // bayesian.rs:306-373 log_entropy; ref_window = genome[gb, gb+span)
@@ -3733,27 +3757,25 @@ __device__ double log_entropy(const DevGenome& G, const DevLuts& T, const uint32
       }
     }
   }
-  int64_t insertion_runs = indel_runs - deletion_runs;
   uint32_t gc, total;
   gc_count(G, gb, span, gc, total);
-  double lnb, lnbm1;
-  if (total <= (uint32_t)GC_LUT_MAX) {
-    uint64_t idx = (uint64_t)total * (total + 1) / 2 + gc;
-    lnb = T.gc_lnb[idx];
-    lnbm1 = T.gc_lnbm1[idx];
-  } else {
-    double f = total == 0 ? 0.5 : (double)gc / (double)total;
-    f = fmin(fmax(f, 1e-6), 1.0 - 1e-6);
-    double bb = exp(-f * log(f / 2.0) - (1.0 - f) * log((1.0 - f) / 2.0));
-    lnb = log(bb);
-    lnbm1 = log(fmax(bb - 1.0, 1e-6));
+  return entropy_from_counts(T, n, mm, readlen, indel_runs - deletion_runs, deletion_runs, gc, total);
+}
+
+// The log term of one missing-locus window starting at `start`, from its
+// quality product z and ratio sum p1 (bayesian.rs:418-433).
+__device__ __forceinline__ double missing_window_log(const DevLuts& T, double z, double p1, const uint8_t* qual,
+                                                     uint32_t qlen, const uint8_t* rcode, uint32_t nwin,
+                                                     uint32_t start) {
+  double rep = 0.0;
+  if (start < nwin) {
+    const uint8_t c = rcode[start];
+    if (c == 1) rep = 1.0;
+    else if (c == 2) rep = T.nrp[start < qlen ? qual[start] : 30];
   }
-  double match_term = py_int_round(10.0 * ((double)n * lnb - T.ln_2g) / LN_10);
-  double mismatch_term = py_int_round(10.0 * (double)mm * lnbm1 / LN_10);
-  double position_choice = phred_n_choose_m(T, (int64_t)readlen, 2 * insertion_runs + deletion_runs);
-  double deletion_length = 0.0;
-  if (deletion_runs > 0) deletion_length = py_int_round((double)deletion_runs * 10.0 * T.ln_30 / LN_10);
-  return match_term - mismatch_term - phred_n_choose_m(T, n, mm) - position_choice - deletion_length;
+  const double p2 = fmin(fmax(1.0 - z * (1.0 + p1), 0.0), 1.0);
+  const double pm = fmin(fmax(1.0 - (1.0 - p2) * (1.0 - rep), 1e-300), 1.0);
+  return log(pm);
 }
 
 // bayesian.rs:405-439 missing_locus_probability; repeat mask from the seed
@@ -3786,15 +3808,7 @@ __device__ double missing_locus(const DevLuts& T, const uint8_t* qual, uint32_t 
         z[ph] = z[ph] * om;
         p1[ph] = p1[ph] + qr;
         if (rel[ph] == k - 1) {
-          double rep = 0.0;
-          if (start < nwin) {
-            const uint8_t c = rcode[start];
-            if (c == 1) rep = 1.0;
-            else if (c == 2) rep = T.nrp[start < qlen ? qual[start] : 30];
-          }
-          const double p2 = fmin(fmax(1.0 - z[ph] * (1.0 + p1[ph]), 0.0), 1.0);
-          const double pm = fmin(fmax(1.0 - (1.0 - p2) * (1.0 - rep), 1e-300), 1.0);
-          log_p[ph] += log(pm);
+          log_p[ph] += missing_window_log(T, z[ph], p1[ph], qual, qlen, rcode, nwin, start);
           z[ph] = 1.0;
           p1[ph] = -0.0;
         }
@@ -3805,6 +3819,109 @@ __device__ double missing_locus(const DevLuts& T, const uint8_t* qual, uint32_t 
   double offset_means = 0.0;
 #pragma unroll
   for (uint32_t ph = 0; ph < PH; ph++) offset_means += exp(log_p[ph]);
+  return fmin(offset_means / 5.0, 0.99);
+}
+
+// ---- the same two terms over a whole wave (one read, all 64 lanes active) ----
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t x) {
+  return (uint32_t)readlane_i32((int32_t)wave_incl_scan(x), 63);
+}
+
+// log_entropy with the per-base and per-word counting spread over the lanes:
+// integer counts (exact in any order), then the same FP64 tail.  The operation
+// scan runs on every lane alike; an M run's loop in log_entropy stops at the
+// read's or the window's end, which bounds the bases counted here the same way.
+__device__ double wave_log_entropy(const DevGenome& G, const DevLuts& T, const uint32_t* cig, uint32_t ncig,
+                                   const QueryView& Q, uint64_t gb, uint32_t span) {
+  const uint32_t lane = (uint32_t)lane_id();
+  int64_t indel_runs = 0, deletion_runs = 0;
+  uint32_t readlen = 0, refpos = 0, curtype = 0, n_l = 0, mm_l = 0;
+  for (uint32_t c = 0; c < ncig; c++) {
+    const uint32_t op = cig[c] & 15u, len = cig[c] >> 4;
+    if (len == 0) continue;
+    if (op == 1) {
+      if (curtype != 1) indel_runs++;
+      curtype = 1;
+      readlen += len;
+    } else if (op == 2) {
+      if (curtype != 2) {
+        deletion_runs++;
+        indel_runs++;
+      }
+      curtype = 2;
+      refpos += len;
+    } else if (op == 0) {
+      if (curtype != 0) indel_runs++;
+      curtype = 0;
+      const uint32_t ra = readlen < Q.L ? Q.L - readlen : 0u, rb = refpos < span ? span - refpos : 0u;
+      const uint32_t cnt = min(len, min(ra, rb));
+      for (uint32_t u = lane; u < cnt; u += 64) {
+        const uint32_t x = readlen + u;
+        const uint32_t q = x < Q.qlen ? Q.q(x) : 0u;
+        const bool hq = q >= 10u;
+        const uint8_t aa = Q.base(x), rr = genome_base(G, gb + refpos + u);
+        n_l += hq ? 1u : 0u;
+        mm_l += (aa != rr && aa != 'N' && rr != 'N' && hq) ? 1u : 0u;
+      }
+      readlen += cnt;
+      refpos += cnt;
+    }
+  }
+  uint32_t gc_l = 0, tot_l = 0;
+  if (span) {
+    const uint64_t end = gb + span;
+    for (uint64_t w = (gb >> 5) + lane; w <= (end - 1) >> 5; w += 64) gc_word(G, w, gb, end, gc_l, tot_l);
+  }
+  const int64_t n = wave_sum_u32(n_l), mm = wave_sum_u32(mm_l);
+  return entropy_from_counts(T, n, mm, readlen, indel_runs - deletion_runs, deletion_runs, wave_sum_u32(gc_l),
+                             wave_sum_u32(tot_l));
+}
+
+// missing_locus with one window per lane: window 5 j + ph starts at ph + 15 j
+// and multiplies / adds its 15 positions in order, as there; lane ph then adds
+// phase ph's window terms in window order, and the five phases are summed in
+// order, so every double is the same.
+__device__ double wave_missing_locus(const DevLuts& T, const uint8_t* qual, uint32_t qlen, const uint8_t* rcode,
+                                     uint32_t nwin) {
+  constexpr uint32_t k = K, PH = 5;
+  if (qlen < k) return 0.5;
+  const uint32_t lane = (uint32_t)lane_id();
+  const uint32_t nids = PH * ((qlen - k) / k + 1);
+  double acc = 0.0;  // lanes 0..4: log_p of phase `lane`
+  for (uint32_t base = 0; base < nids; base += 64) {
+    const uint32_t id = base + lane, start = id % PH + k * (id / PH);
+    const bool valid = id < nids && start + k <= qlen;
+    double term = 0.0;
+    if (valid) {
+      double z = 1.0, p1 = -0.0;
+      for (uint32_t i = 0; i < k; i++) {
+        const uint32_t q = qual[start + i];
+        z = z * T.omq[q];
+        p1 = p1 + T.qratio[q];
+      }
+      term = missing_window_log(T, z, p1, qual, qlen, rcode, nwin, start);
+    }
+    const uint64_t vmask = __ballot(valid);
+    const uint32_t first = (lane + PH - base % PH) % PH;  // first offset in this pass of phase `lane`
+    for (uint32_t o0 = 0; o0 < 64; o0 += PH) {
+      const uint32_t o = first + o0;
+      const double v = __shfl(term, (int)(o < 64 ? o : 0), 64);
+      if (lane < PH && o < 64 && ((vmask >> o) & 1ull)) acc += v;
+    }
+  }
+  double offset_means = 0.0;
+  for (uint32_t ph = 0; ph < PH; ph++) offset_means += exp(readlane_f64(acc, (int)ph));
   return fmin(offset_means / 5.0, 0.99);
 }
 
