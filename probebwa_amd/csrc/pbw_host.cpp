@@ -55,18 +55,6 @@ struct DevBuf {
     HIP_CHECK(hipMalloc((void**)&p, bytes));
     n = std::max<size_t>(count, 1);
   }
-  void grow_keep(size_t count, hipStream_t s) {  // as ensure, keeping the current contents
-    if (count <= n && p) return;
-    T* q = nullptr;
-    HIP_CHECK(hipMalloc((void**)&q, std::max<size_t>(count, 1) * sizeof(T)));
-    if (p && n) {
-      HIP_CHECK(hipMemcpyAsync(q, p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
-      HIP_CHECK(hipStreamSynchronize(s));
-    }
-    release();
-    p = q;
-    n = std::max<size_t>(count, 1);
-  }
   void upload(const T* h, size_t count, hipStream_t s) {
     ensure(count);
     if (count) HIP_CHECK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
@@ -280,8 +268,6 @@ struct pbw_ctx {
   uint32_t* h_nh = nullptr;  // pinned: per-read hit counts of the seed pass
   size_t h_nh_cap = 0;
   uint32_t hit_cap = HIT_CAP;  // first-pass hits per read, adapted batch to batch
-  size_t pe_nws = 0;           // wide mate-window slots of the current pair batch
-  uint32_t pe_wcap = 0;        // CIGAR capacity per wide slot in that batch
   DevBuf<uint32_t> d_nhits, d_list, d_hcap, d_tier2, d_mid, d_ntier2, d_best, d_cigar, d_cigar_out, d_cigar_off;
   DevBuf<Cand> d_cands;
   DevBuf<uint8_t> d_ncand, d_njobs, d_rec, d_refbytes;
@@ -296,7 +282,10 @@ struct pbw_ctx {
   DevBuf<DpJob> d_wjobs;
   DevBuf<DpOut> d_wout;
   DevBuf<uint8_t> d_wvalid, d_slcand, d_has_obs;
-  DevBuf<uint32_t> d_plan, d_wcigar, d_name_class;
+  DevBuf<uint32_t> d_plan, d_wcigar, d_name_class, d_wlist, d_pe_cnt;
+  DevBuf<Counters> d_ctr_w;  // the cross-alignment DP's counters
+  uint8_t* h_pe = nullptr;   // pinned: a pair batch's counts and observations
+  size_t h_pe_cap = 0;
   DevBuf<uint64_t> d_cig_src, d_rec_src;
   DevBuf<uint32_t> d_rec_bsum;
   DevBuf<double> d_obs;
@@ -985,7 +974,7 @@ static void batch_dp(pbw_ctx* c, Batch& b, const uint32_t* jlist, uint32_t nlist
 // the batch's word count and copies records and runs straight into the arrays
 // the output batch takes over.
 static void download_records(pbw_ctx* c, const Batch& b, uint32_t nrec, const uint64_t* d_src, const uint32_t* pool1,
-                             HostRecords& hr, uint32_t rec0 = 0) {
+                             HostRecords& hr) {
   hipStream_t s = c->stream;
   if (!nrec) {
     HIP_CHECK(hipStreamSynchronize(s));
@@ -997,8 +986,7 @@ static void download_records(pbw_ctx* c, const Batch& b, uint32_t nrec, const ui
   const uint32_t nb = (nrec + REC_BLOCK - 1) / REC_BLOCK;
   c->d_rec_bsum.ensure(nb + 1);
   uint32_t* d_total = c->d_rec_bsum.p + nb;
-  uint8_t* d_rec = c->d_rec.p + (size_t)rec0 * sizeof(pbw_record);
-  d_src += rec0;
+  uint8_t* d_rec = c->d_rec.p;
   launch_rec_offsets(d_rec, d_src, nrec, base, c->d_rec_bsum.p, d_total, s);
   HIP_CHECK(hipGetLastError());
   pbw_record* hrec = hr.rec.grow(nrec);
@@ -1009,7 +997,7 @@ static void download_records(pbw_ctx* c, const Batch& b, uint32_t nrec, const ui
   if ((uint64_t)base + total > 0xFFFFFFFFull)
     throw PbwError{PBW_ERR_LIMIT, "more than 2^32 CIGAR words in one call; map fewer reads per call"};
   c->d_cigar_out.ensure(std::max<uint32_t>(total, 1));
-  launch_rec_cigar(d_rec, d_src, c->d_cigar.p, pool1 ? pool1 : c->d_cigar.p, b.R.seq_off + rec0, nrec, base,
+  launch_rec_cigar(d_rec, d_src, c->d_cigar.p, pool1 ? pool1 : c->d_cigar.p, b.R.seq_off, nrec, base,
                    c->d_cigar_out.p, s);
   HIP_CHECK(hipGetLastError());
   uint32_t* hc = hr.cigar.grow(total);
@@ -1223,14 +1211,71 @@ static void pe_prepare(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_batc
   c->d_has_obs.ensure(P);
 }
 
+// Cross-alignment windows from a device list whose length is on the device
+// (*d_n <= cap): k_dp_wide for the whole-window rectangles, k_dp_tiled for the
+// jobs it flags (or for all when the windows are wider than k_dp_wide's
+// tiles).  No host round trip: jobs beyond the tiled kernel's shape are counted
+// in *d_bad for the caller to check.
+static void run_dp_wide_dev(pbw_ctx* c, DpArgs a, const uint32_t* d_list, const uint32_t* d_n, uint32_t cap,
+                            uint32_t* d_flag_n, uint32_t* d_bad) {
+  if (!cap) return;
+  hipStream_t s = c->stream;
+  a.job_valid = nullptr;
+  const uint32_t T = dp_wide_tiles(a.max_n);
+  const uint32_t* tl = d_list;
+  const uint32_t* tn = d_n;
+  uint32_t tw = std::min<uint32_t>(cap, 2048);
+  if (T) {
+    DpArgs w = a;
+    w.jlist = d_list;
+    w.njobs = cap;
+    w.njobs_dev = d_n;
+    c->d_flag_w.ensure(cap);
+    w.flag_list = c->d_flag_w.p;
+    w.flag_n = d_flag_n;
+    const uint32_t nwaves = std::min<uint32_t>(cap, 8192);
+    c->d_tb.ensure((size_t)nwaves * (a.max_m + 1) * T * 16);  // 16 bits per lane, tile and row
+    w.tb = c->d_tb.p;
+    launch_dp_wide(w, nwaves, s);
+    HIP_CHECK(hipGetLastError());
+    tl = c->d_flag_w.p;
+    tn = d_flag_n;
+    tw = std::min<uint32_t>(cap, 64);  // the flagged jobs: few or none
+  }
+  a.jlist = tl;
+  a.njobs = cap;
+  a.njobs_dev = tn;
+  c->d_flag_c.ensure(cap);
+  a.flag_list = c->d_flag_c.p;
+  a.flag_n = d_bad;
+  a.tiles = (a.max_n + 1 + 63) / 64;
+  const bool lds_rows = dp_tiled_rows_in_lds(a.max_m, a.max_n);
+  if (lds_rows) {
+    a.rows = nullptr;
+  } else {
+    c->d_rows.ensure((size_t)tw * 3 * (a.max_n + 1));
+    a.rows = c->d_rows.p;
+  }
+  c->d_tb.ensure((size_t)tw * (a.max_m + 1) * a.tiles * 4);
+  a.tb = c->d_tb.p;
+  launch_dp_tiled(a, tw, s);
+  HIP_CHECK(hipGetLastError());
+}
+
 // Pairs [q0, q1) of a prepared batch under the model snapshot `snap`: cross-
-// alignment plan, wide mate windows, joint posterior and records (on the
-// device); their observations to obs_out / has_out.  The cross-alignment
-// CIGAR pool holds slots wbase .. wbase + pe_nws - 1.
-static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, uint64_t wbase, const HostInsert& snap,
+// alignment plan, wide mate windows, joint posterior and records, queued on the
+// stream.  One synchronisation brings back the records, the CIGAR total, the
+// counts and the observations (obs_out / has_out); the CIGAR runs follow
+// asynchronously and are in hr by the stream's next synchronisation.  The
+// cross-alignment CIGAR pool holds this call's slots only.
+static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, const HostInsert& snap, HostRecords& hr,
                      std::vector<double>& obs_out, std::vector<uint8_t>& has_out) {
   hipStream_t s = c->stream;
-  const uint32_t P = q1 - q0;
+  const uint32_t P = q1 - q0, nws = P * 40;
+  c->d_pe_cnt.ensure(4);
+  uint32_t* d_wn = c->d_pe_cnt.p;  // [0] windows listed, [1] flagged by k_dp_wide, [2] beyond the tiled shape
+  HIP_CHECK(hipMemsetAsync(d_wn, 0, 4 * sizeof(uint32_t), s));
+  c->d_wlist.ensure(nws);
   PairArgs pa{};
   pa.G = c->dev_genome();
   pa.R = b.R;
@@ -1244,6 +1289,8 @@ static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, uint6
   pa.rcode = c->d_rcode.p;
   pa.wjobs = c->d_wjobs.p;
   pa.wvalid = c->d_wvalid.p;
+  pa.wlist = c->d_wlist.p;
+  pa.wn = d_wn;
   pa.slcand = c->d_slcand.p;
   pa.plan = c->d_plan.p;
   pa.mean = snap.mean;
@@ -1257,50 +1304,28 @@ static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, uint6
   pa.p_base = q0;
   launch_pair_plan(pa, s);
   HIP_CHECK(hipGetLastError());
-  const size_t nws = (size_t)P * 40, ws0 = (size_t)q0 * 40;
-  std::vector<uint8_t> wv(nws);
-  HIP_CHECK(hipMemcpyAsync(wv.data(), c->d_wvalid.p + ws0, nws, hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
-  std::vector<uint32_t> wlist;
-  for (size_t k = 0; k < nws; k++)
-    if (wv[k]) wlist.push_back((uint32_t)(ws0 + k));
-  // wide DP (align_at_window, single.rs:220-275): window <= 2 radius + L
+  // wide DP (align_at_window, single.rs:220-275): window <= 2 radius + L; a
+  // window beyond 8000 bases is counted as beyond the tiled shape
   const uint64_t wmax = 2 * (pa.radius_base + b.max_m) + b.max_m;
-  if (!wlist.empty() && wmax > 8000)
-    throw PbwError{PBW_ERR_LIMIT, "insert-size search window wider than 8000 bases"};
-  const uint32_t wcap = b.max_m + (uint32_t)wmax + 2;
-  // the wide CIGAR pool holds the batch's slots at one stride, which only grows
-  // within a batch: slots written earlier keep their places (a record's CIGAR
-  // source is an absolute offset), and a larger stride never overlaps them
-  if (wcap > c->pe_wcap) {
-    if (c->pe_wcap) c->d_wcigar.grow_keep(c->pe_nws * (size_t)wcap, s);
-    else c->d_wcigar.ensure(c->pe_nws * (size_t)wcap);
-    c->pe_wcap = wcap;
-  }
+  const uint32_t max_n = (uint32_t)std::min<uint64_t>(wmax, 8000);
+  const uint32_t wcap = b.max_m + max_n + 2;
+  const uint64_t wbase = (uint64_t)q0 * 40;
+  c->d_wcigar.ensure((size_t)nws * wcap);
+  c->d_ctr_w.ensure(1);
   DpArgs da = dp_args_base(c, b.R);
   da.jobs = c->d_wjobs.p;
-  da.cigar_cap = c->pe_wcap;
+  da.cigar_cap = wcap;
   da.cigar = c->d_wcigar.p;
   da.cigar_base = wbase;
   da.out = c->d_wout.p;
   da.max_m = b.max_m;
-  da.max_n = (uint32_t)wmax;
-  c->stats.wide_jobs += wlist.size();
-  if (!wlist.empty()) {
-    Counters before{};
-    HIP_CHECK(hipMemcpyAsync(&before, c->d_ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    run_dp_tiled(c, da, wlist);
-    Counters after{};
-    HIP_CHECK(hipMemcpyAsync(&after, c->d_ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    c->stats.wide_cells += after.cells - before.cells;
-    HIP_CHECK(hipMemcpyAsync(c->d_ctr.p, &before, sizeof(Counters), hipMemcpyHostToDevice, s));
-  }
+  da.max_n = max_n;
+  da.ctr = c->d_ctr_w.p;
+  run_dp_wide_dev(c, da, c->d_wlist.p, d_wn, nws, d_wn + 1, d_wn + 2);
   // joint posterior + records
   pa.wout = c->d_wout.p;
   pa.wcigar = c->d_wcigar.p;
-  pa.wcap = c->pe_wcap;
+  pa.wcap = wcap;
   pa.wbase = wbase;
   pa.rec = c->d_rec.p;
   pa.cig_src = c->d_cig_src.p;
@@ -1308,11 +1333,61 @@ static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, uint6
   pa.has_obs = c->d_has_obs.p;
   launch_pair_finalize(pa, s);
   HIP_CHECK(hipGetLastError());
-  obs_out.resize(P);
-  has_out.resize(P);
-  HIP_CHECK(hipMemcpyAsync(obs_out.data(), c->d_obs.p + q0, (size_t)P * 8, hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipMemcpyAsync(has_out.data(), c->d_has_obs.p + q0, P, hipMemcpyDeviceToHost, s));
+  // records [2 q0, 2 q1): CIGAR offsets on the device, then one synchronisation
+  const uint32_t nrec = 2 * P, rec0 = 2 * q0;
+  if (hr.cigar.size() > 0xFFFFFFFFull)
+    throw PbwError{PBW_ERR_LIMIT, "more than 2^32 CIGAR words in one call; map fewer reads per call"};
+  const uint32_t base = (uint32_t)hr.cigar.size();
+  const uint32_t nb = (nrec + REC_BLOCK - 1) / REC_BLOCK;
+  c->d_rec_bsum.ensure(nb + 1);
+  uint32_t* d_total = c->d_rec_bsum.p + nb;
+  uint8_t* d_rec = c->d_rec.p + (size_t)rec0 * sizeof(pbw_record);
+  const uint64_t* d_src = c->d_cig_src.p + rec0;
+  launch_rec_offsets(d_rec, d_src, nrec, base, c->d_rec_bsum.p, d_total, s);
+  HIP_CHECK(hipGetLastError());
+  pbw_record* hrec = hr.rec.grow(nrec);
+  HIP_CHECK(hipMemcpyAsync(hrec, d_rec, (size_t)nrec * sizeof(pbw_record), hipMemcpyDeviceToHost, s));
+  const size_t stage = 16 + (size_t)P * 9;
+  if (c->h_pe_cap < stage) {
+    if (c->h_pe) (void)hipHostFree(c->h_pe);
+    c->h_pe = nullptr;
+    c->h_pe_cap = 0;
+    if (hipHostMalloc((void**)&c->h_pe, stage, hipHostMallocDefault) != hipSuccess || !c->h_pe)
+      throw PbwError{PBW_ERR_LIMIT, "pinned host allocation of " + std::to_string(stage) + " bytes failed"};
+    c->h_pe_cap = stage;
+  }
+  uint32_t* h_cnt = reinterpret_cast<uint32_t*>(c->h_pe);  // 3 counts + total
+  double* h_obs = reinterpret_cast<double*>(c->h_pe + 16);
+  uint8_t* h_has = c->h_pe + 16 + (size_t)P * 8;
+  HIP_CHECK(hipMemcpyAsync(h_cnt, d_wn, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(h_cnt + 3, d_total, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(h_obs, c->d_obs.p + q0, (size_t)P * 8, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(h_has, c->d_has_obs.p + q0, P, hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
+  if (h_cnt[2])
+    throw PbwError{PBW_ERR_LIMIT, wmax > 8000 ? "insert-size search window wider than 8000 bases"
+                                              : "alignment window beyond the tiled kernel's shape"};
+  c->stats.wide_jobs += h_cnt[0];
+  obs_out.assign(h_obs, h_obs + P);
+  has_out.assign(h_has, h_has + P);
+  const uint32_t total = h_cnt[3];
+  if ((uint64_t)base + total > 0xFFFFFFFFull)
+    throw PbwError{PBW_ERR_LIMIT, "more than 2^32 CIGAR words in one call; map fewer reads per call"};
+  c->d_cigar_out.ensure(std::max<uint32_t>(total, 1));
+  launch_rec_cigar(d_rec, d_src, c->d_cigar.p, c->d_wcigar.p, b.R.seq_off + rec0, nrec, base, c->d_cigar_out.p, s);
+  HIP_CHECK(hipGetLastError());
+  uint32_t* hc = hr.cigar.grow(total);
+  if (total) HIP_CHECK(hipMemcpyAsync(hc, c->d_cigar_out.p, (size_t)total * 4, hipMemcpyDeviceToHost, s));
+}
+
+// Cross-alignment DP cells to the statistics (synchronises the stream).
+static void collect_wide_cells(pbw_ctx* c) {
+  if (!c->d_ctr_w.p) return;
+  Counters w{};
+  HIP_CHECK(hipMemcpyAsync(&w, c->d_ctr_w.p, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  c->stats.wide_cells += w.cells;
+  HIP_CHECK(hipMemsetAsync(c->d_ctr_w.p, 0, sizeof(Counters), c->stream));
 }
 
 // lib.rs:145-185: 500 serial warm-up pairs, then 2000-pair batches
@@ -1329,17 +1404,16 @@ static void map_pe_all(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_batc
     // model its predecessors left, through the model-dependent stages
     Batch b;
     pe_prepare(c, m1, m2, 0, warm, b);
-    c->pe_nws = (size_t)warm * 40;
-    c->pe_wcap = 0;
     auto t0 = std::chrono::steady_clock::now();
     for (uint64_t i = 0; i < warm; i++) {
       HostInsert snap = im;
-      pe_pairs(c, b, (uint32_t)i, (uint32_t)i + 1, 0, snap, obs, has);
+      pe_pairs(c, b, (uint32_t)i, (uint32_t)i + 1, snap, hr, obs, has);
       if (has[0]) im.update(obs[0]);
     }
-    download_records(c, b, 2 * (uint32_t)warm, c->d_cig_src.p, c->d_wcigar.p, hr);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
     c->stats.ms_pair += ms_since(t0);
     collect_counters(c, 2 * (uint32_t)warm);
+    collect_wide_cells(c);
   }
   // the 2000-pair batches: seeding, clustering and the single-end DP for many
   // of them at once (whole batches, so the boundaries stay warm + 2000 k); then
@@ -1355,16 +1429,15 @@ static void map_pe_all(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_batc
       const uint64_t b1 = std::min<uint64_t>(b0 + 2000, s1);
       const uint32_t q0 = (uint32_t)(b0 - s0), q1 = (uint32_t)(b1 - s0);
       HostInsert snap = im;
-      c->pe_nws = (size_t)(q1 - q0) * 40;
-      c->pe_wcap = 0;
       auto t0 = std::chrono::steady_clock::now();
-      pe_pairs(c, b, q0, q1, (uint64_t)q0 * 40, snap, obs, has);
-      download_records(c, b, 2 * (q1 - q0), c->d_cig_src.p, c->d_wcigar.p, hr, 2 * q0);
+      pe_pairs(c, b, q0, q1, snap, hr, obs, has);
       c->stats.ms_pair += ms_since(t0);
       for (uint64_t k = 0; k < b1 - b0; k++)
         if (has[k]) im.update(obs[k]);
     }
+    HIP_CHECK(hipStreamSynchronize(c->stream));
     collect_counters(c, 2 * (uint32_t)(s1 - s0));
+    collect_wide_cells(c);
   }
   model->mean = im.mean;
   model->sd = im.sd;
@@ -1683,6 +1756,7 @@ void pbw_ctx_destroy(pbw_ctx* c) {
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   if (c->h_nh) (void)hipHostFree(c->h_nh);
+  if (c->h_pe) (void)hipHostFree(c->h_pe);
   delete c;
 }
 

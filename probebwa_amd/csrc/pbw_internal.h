@@ -171,6 +171,7 @@ struct DpArgs {
   const DpJob* jobs;
   const uint32_t* jlist;   // job indices to run (nullptr = 0..njobs-1)
   uint32_t njobs;
+  const uint32_t* njobs_dev;  // k_dp_wide / k_dp_tiled: when set, the job count (<= njobs, which sizes the launch)
   const uint8_t* job_valid;  // optional: per read count of valid slots (slot = read*20+k)
   const uint8_t* ref_bytes;  // explicit-ref source
   int32_t gap_open, gap_extend;
@@ -271,6 +272,8 @@ struct PairArgs {
   const uint8_t* rcode;
   DpJob* wjobs;            // cross-alignment jobs, slot = pair * 40 + s
   uint8_t* wvalid;
+  uint32_t* wlist;         // slots with a window, appended by the plan (count in *wn)
+  uint32_t* wn;
   uint8_t* slcand;         // shortlist candidate of each slot
   const DpOut* wout;
   const uint32_t* wcigar;  // slot s at (s - wbase) * wcap

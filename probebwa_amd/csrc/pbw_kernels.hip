@@ -3227,7 +3227,8 @@ __global__ __launch_bounds__(64) void k_dp_tiled(DpArgs a) {
   int32_t* RD = RI + (a.max_n + 1);
   uint64_t* TB = a.tb + (uint64_t)blockIdx.x * (uint64_t)(a.max_m + 1) * a.tiles * 4;
   unsigned long long c_cells = 0;
-  for (uint32_t t = blockIdx.x; t < a.njobs; t += gridDim.x) {
+  const uint32_t nj = a.njobs_dev ? *a.njobs_dev : a.njobs;
+  for (uint32_t t = blockIdx.x; t < nj; t += gridDim.x) {
     const uint32_t jid = a.jlist ? a.jlist[t] : t;
     if (a.job_valid) {
       if ((jid % TOP_CANDIDATES) >= a.job_valid[jid / TOP_CANDIDATES]) continue;
@@ -3434,7 +3435,8 @@ __global__ __launch_bounds__(64) void k_dp_wide(DpArgs a) {
   uint16_t* TB = (uint16_t*)a.tb + (uint64_t)blockIdx.x * (uint64_t)(a.max_m + 1) * T * 64;
   const int32_t ge = a.gap_extend;
   unsigned long long c_cells = 0;
-  for (uint32_t t = blockIdx.x; t < a.njobs; t += gridDim.x) {
+  const uint32_t nj = a.njobs_dev ? *a.njobs_dev : a.njobs;
+  for (uint32_t t = blockIdx.x; t < nj; t += gridDim.x) {
     const uint32_t jid = a.jlist ? a.jlist[t] : t;
     const DpJob job = a.jobs[jid];
     const uint64_t so = a.R.seq_off[job.read], qo = a.R.qual_off[job.read];
