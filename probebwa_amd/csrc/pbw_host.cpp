@@ -55,6 +55,18 @@ struct DevBuf {
     HIP_CHECK(hipMalloc((void**)&p, bytes));
     n = std::max<size_t>(count, 1);
   }
+  void grow_keep(size_t count, hipStream_t s) {  // as ensure, keeping the current contents
+    if (count <= n && p) return;
+    T* q = nullptr;
+    HIP_CHECK(hipMalloc((void**)&q, std::max<size_t>(count, 1) * sizeof(T)));
+    if (p && n) {
+      HIP_CHECK(hipMemcpyAsync(q, p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+    }
+    release();
+    p = q;
+    n = std::max<size_t>(count, 1);
+  }
   void upload(const T* h, size_t count, hipStream_t s) {
     ensure(count);
     if (count) HIP_CHECK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
@@ -1262,14 +1274,24 @@ static void run_dp_wide_dev(pbw_ctx* c, DpArgs a, const uint32_t* d_list, const 
   HIP_CHECK(hipGetLastError());
 }
 
+// The cross-alignment CIGAR pool: slots base .. base + nslots - 1 at one stride,
+// which only grows while the pool is in use (slots written earlier keep their
+// places, since a record's CIGAR source is an absolute offset, and slots are
+// filled in increasing order, so a larger stride never overlaps them).
+struct WidePool {
+  uint64_t base;
+  size_t nslots;
+  uint32_t stride;
+};
+
 // Pairs [q0, q1) of a prepared batch under the model snapshot `snap`: cross-
 // alignment plan, wide mate windows, joint posterior and records, queued on the
-// stream.  One synchronisation brings back the records, the CIGAR total, the
-// counts and the observations (obs_out / has_out); the CIGAR runs follow
-// asynchronously and are in hr by the stream's next synchronisation.  The
-// cross-alignment CIGAR pool holds this call's slots only.
-static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, const HostInsert& snap, HostRecords& hr,
-                     std::vector<double>& obs_out, std::vector<uint8_t>& has_out) {
+// stream.  One synchronisation brings back the counts and the observations
+// (obs_out / has_out) and, when hr is given, the records and the CIGAR total;
+// the CIGAR runs then follow asynchronously and are in hr by the stream's next
+// synchronisation.
+static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, const HostInsert& snap, WidePool& pool,
+                     HostRecords* hr, std::vector<double>& obs_out, std::vector<uint8_t>& has_out) {
   hipStream_t s = c->stream;
   const uint32_t P = q1 - q0, nws = P * 40;
   c->d_pe_cnt.ensure(4);
@@ -1309,12 +1331,16 @@ static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, const
   const uint64_t wmax = 2 * (pa.radius_base + b.max_m) + b.max_m;
   const uint32_t max_n = (uint32_t)std::min<uint64_t>(wmax, 8000);
   const uint32_t wcap = b.max_m + max_n + 2;
-  const uint64_t wbase = (uint64_t)q0 * 40;
-  c->d_wcigar.ensure((size_t)nws * wcap);
+  if (wcap > pool.stride) {
+    if (pool.stride) c->d_wcigar.grow_keep(pool.nslots * (size_t)wcap, s);
+    else c->d_wcigar.ensure(pool.nslots * (size_t)wcap);
+    pool.stride = wcap;
+  }
+  const uint64_t wbase = pool.base;
   c->d_ctr_w.ensure(1);
   DpArgs da = dp_args_base(c, b.R);
   da.jobs = c->d_wjobs.p;
-  da.cigar_cap = wcap;
+  da.cigar_cap = pool.stride;
   da.cigar = c->d_wcigar.p;
   da.cigar_base = wbase;
   da.out = c->d_wout.p;
@@ -1325,7 +1351,7 @@ static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, const
   // joint posterior + records
   pa.wout = c->d_wout.p;
   pa.wcigar = c->d_wcigar.p;
-  pa.wcap = wcap;
+  pa.wcap = pool.stride;
   pa.wbase = wbase;
   pa.rec = c->d_rec.p;
   pa.cig_src = c->d_cig_src.p;
@@ -1335,18 +1361,22 @@ static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, const
   HIP_CHECK(hipGetLastError());
   // records [2 q0, 2 q1): CIGAR offsets on the device, then one synchronisation
   const uint32_t nrec = 2 * P, rec0 = 2 * q0;
-  if (hr.cigar.size() > 0xFFFFFFFFull)
-    throw PbwError{PBW_ERR_LIMIT, "more than 2^32 CIGAR words in one call; map fewer reads per call"};
-  const uint32_t base = (uint32_t)hr.cigar.size();
-  const uint32_t nb = (nrec + REC_BLOCK - 1) / REC_BLOCK;
-  c->d_rec_bsum.ensure(nb + 1);
-  uint32_t* d_total = c->d_rec_bsum.p + nb;
+  uint32_t base = 0;
   uint8_t* d_rec = c->d_rec.p + (size_t)rec0 * sizeof(pbw_record);
   const uint64_t* d_src = c->d_cig_src.p + rec0;
-  launch_rec_offsets(d_rec, d_src, nrec, base, c->d_rec_bsum.p, d_total, s);
-  HIP_CHECK(hipGetLastError());
-  pbw_record* hrec = hr.rec.grow(nrec);
-  HIP_CHECK(hipMemcpyAsync(hrec, d_rec, (size_t)nrec * sizeof(pbw_record), hipMemcpyDeviceToHost, s));
+  uint32_t* d_total = nullptr;
+  if (hr) {
+    if (hr->cigar.size() > 0xFFFFFFFFull)
+      throw PbwError{PBW_ERR_LIMIT, "more than 2^32 CIGAR words in one call; map fewer reads per call"};
+    base = (uint32_t)hr->cigar.size();
+    const uint32_t nb = (nrec + REC_BLOCK - 1) / REC_BLOCK;
+    c->d_rec_bsum.ensure(nb + 1);
+    d_total = c->d_rec_bsum.p + nb;
+    launch_rec_offsets(d_rec, d_src, nrec, base, c->d_rec_bsum.p, d_total, s);
+    HIP_CHECK(hipGetLastError());
+    pbw_record* hrec = hr->rec.grow(nrec);
+    HIP_CHECK(hipMemcpyAsync(hrec, d_rec, (size_t)nrec * sizeof(pbw_record), hipMemcpyDeviceToHost, s));
+  }
   const size_t stage = 16 + (size_t)P * 9;
   if (c->h_pe_cap < stage) {
     if (c->h_pe) (void)hipHostFree(c->h_pe);
@@ -1360,7 +1390,7 @@ static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, const
   double* h_obs = reinterpret_cast<double*>(c->h_pe + 16);
   uint8_t* h_has = c->h_pe + 16 + (size_t)P * 8;
   HIP_CHECK(hipMemcpyAsync(h_cnt, d_wn, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipMemcpyAsync(h_cnt + 3, d_total, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  if (hr) HIP_CHECK(hipMemcpyAsync(h_cnt + 3, d_total, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipMemcpyAsync(h_obs, c->d_obs.p + q0, (size_t)P * 8, hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipMemcpyAsync(h_has, c->d_has_obs.p + q0, P, hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
@@ -1370,13 +1400,14 @@ static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, const
   c->stats.wide_jobs += h_cnt[0];
   obs_out.assign(h_obs, h_obs + P);
   has_out.assign(h_has, h_has + P);
+  if (!hr) return;
   const uint32_t total = h_cnt[3];
   if ((uint64_t)base + total > 0xFFFFFFFFull)
     throw PbwError{PBW_ERR_LIMIT, "more than 2^32 CIGAR words in one call; map fewer reads per call"};
   c->d_cigar_out.ensure(std::max<uint32_t>(total, 1));
   launch_rec_cigar(d_rec, d_src, c->d_cigar.p, c->d_wcigar.p, b.R.seq_off + rec0, nrec, base, c->d_cigar_out.p, s);
   HIP_CHECK(hipGetLastError());
-  uint32_t* hc = hr.cigar.grow(total);
+  uint32_t* hc = hr->cigar.grow(total);
   if (total) HIP_CHECK(hipMemcpyAsync(hc, c->d_cigar_out.p, (size_t)total * 4, hipMemcpyDeviceToHost, s));
 }
 
@@ -1405,12 +1436,13 @@ static void map_pe_all(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_batc
     Batch b;
     pe_prepare(c, m1, m2, 0, warm, b);
     auto t0 = std::chrono::steady_clock::now();
+    WidePool pool{0, (size_t)warm * 40, 0};
     for (uint64_t i = 0; i < warm; i++) {
       HostInsert snap = im;
-      pe_pairs(c, b, (uint32_t)i, (uint32_t)i + 1, snap, hr, obs, has);
+      pe_pairs(c, b, (uint32_t)i, (uint32_t)i + 1, snap, pool, nullptr, obs, has);
       if (has[0]) im.update(obs[0]);
     }
-    HIP_CHECK(hipStreamSynchronize(c->stream));
+    download_records(c, b, 2 * (uint32_t)warm, c->d_cig_src.p, c->d_wcigar.p, hr);
     c->stats.ms_pair += ms_since(t0);
     collect_counters(c, 2 * (uint32_t)warm);
     collect_wide_cells(c);
@@ -1430,7 +1462,8 @@ static void map_pe_all(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_batc
       const uint32_t q0 = (uint32_t)(b0 - s0), q1 = (uint32_t)(b1 - s0);
       HostInsert snap = im;
       auto t0 = std::chrono::steady_clock::now();
-      pe_pairs(c, b, q0, q1, snap, hr, obs, has);
+      WidePool pool{(uint64_t)q0 * 40, (size_t)(q1 - q0) * 40, 0};
+      pe_pairs(c, b, q0, q1, snap, pool, &hr, obs, has);
       c->stats.ms_pair += ms_since(t0);
       for (uint64_t k = 0; k < b1 - b0; k++)
         if (has[k]) im.update(obs[k]);

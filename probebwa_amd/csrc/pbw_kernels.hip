@@ -3422,6 +3422,7 @@ void launch_dp_tiled(const DpArgs& a, uint32_t nwaves, hipStream_t s) {
 // Jobs of another shape, or wider than 256 T columns, are flagged to
 // k_dp_tiled.
 // ---------------------------------------------------------------------------
+constexpr uint32_t WIDE_TB_ROWS = 16;  // traceback rows staged in LDS at a time
 template <int T>
 __global__ __launch_bounds__(64) void k_dp_wide(DpArgs a) {
   extern __shared__ __align__(16) uint8_t dyn[];
@@ -3433,6 +3434,8 @@ __global__ __launch_bounds__(64) void k_dp_wide(DpArgs a) {
   st.go = (int16_t*)(st.ref + ((a.max_n + 15) & ~15u));
   st.runs = (uint32_t*)((uint8_t*)st.go + (((a.max_n * 2) + 15) & ~15u));
   uint16_t* TB = (uint16_t*)a.tb + (uint64_t)blockIdx.x * (uint64_t)(a.max_m + 1) * T * 64;
+  // traceback staging: WIDE_TB_ROWS rows of every tile, walked from LDS
+  uint32_t* tbs = (uint32_t*)(dyn + dp_tiled_lds_bytes(a.max_m, a.max_n));
   const int32_t ge = a.gap_extend;
   unsigned long long c_cells = 0;
   const uint32_t nj = a.njobs_dev ? *a.njobs_dev : a.njobs;
@@ -3485,33 +3488,32 @@ __global__ __launch_bounds__(64) void k_dp_wide(DpArgs a) {
         cD = readlane_i32(PD[tt][3], 63);
         int32_t M[4], I[4];
         uint32_t bits = 0;
+        // Columns past n only feed columns past n (every dependency runs
+        // rightwards or down), so they are computed like the others and never
+        // read: no masks.  With one score added to all three sources, the
+        // maximum and the traceback choice (M, then I, then D on ties) follow
+        // from the sources alone.
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-          const uint32_t c = 256u * tt + 4u * lane + k + 1u;
-          const bool v = c <= n;
           const int32_t dM = k ? PM[tt][k - 1] : dM0, dI = k ? PI[tt][k - 1] : dI0, dD = k ? PD[tt][k - 1] : dD0;
           const int32_t sc = (rb == rbyte[tt][k]) ? MATCH_SCORE : mis;
-          const int32_t fm = dM + sc, fi = dI + sc, fd = dD + sc;
-          int32_t Mk = max(max(fm, fi), fd);
-          if (!v) Mk = NEG_INF;
+          const int32_t xid = max(dI, dD);
+          const int32_t Mk = max(dM, xid) + sc;
           const int32_t io = PM[tt][k] + gop[tt][k], ie = PI[tt][k] + ge;
           const int32_t Ik = max(io, ie);
           M[k] = Mk;
           I[k] = Ik;
-          const uint32_t b0 = (v && Mk != fm && Mk == fi) ? 1u : 0u;
-          const uint32_t b1 = (v && Mk != fm && Mk != fi) ? 2u : 0u;
-          const uint32_t b2 = (v && Ik != io) ? 4u : 0u;
-          bits |= (b0 | b1 | b2) << (4 * k);
+          const uint32_t tm = dM >= xid ? 0u : (dI >= dD ? 1u : 2u);
+          bits |= (tm | (ie > io ? 4u : 0u)) << (4 * k);
         }
         // D: A(c) = M(c - 1) + go(c); D(c) = max(A(c), D(c - 1) + ge)
         const int32_t Ml0 = dpp_wave_shr1(curMl, M[3]);
         int32_t A[4], key[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-          const uint32_t c = 256u * tt + 4u * lane + k + 1u;
           const int32_t pos = 4 * lane + k;
           A[k] = (k ? M[k - 1] : Ml0) + gop[tt][k];
-          key[k] = c <= n ? A[k] - pos * ge : INT32_MIN;
+          key[k] = A[k] - pos * ge;
         }
         const int32_t l1 = max(key[0], key[1]), l2 = max(l1, key[2]), l3 = max(l2, key[3]);
         const int32_t incl = wave_incl_max(l3);
@@ -3524,10 +3526,7 @@ __global__ __launch_bounds__(64) void k_dp_wide(DpArgs a) {
         D[2] = (4 * lane + 2) * ge + max(l2, base);
         D[3] = (4 * lane + 3) * ge + max(l3, base);
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-          const uint32_t c = 256u * tt + 4u * lane + k + 1u;
-          bits |= ((c <= n && D[k] != A[k]) ? 8u : 0u) << (4 * k);
-        }
+        for (int k = 0; k < 4; k++) bits |= (D[k] != A[k] ? 8u : 0u) << (4 * k);
         curMl = readlane_i32(M[3], 63);
         curDl = readlane_i32(D[3], 63);
         TB[((uint64_t)i * T + tt) * 64 + lane] = (uint16_t)bits;
@@ -3561,23 +3560,30 @@ __global__ __launch_bounds__(64) void k_dp_wide(DpArgs a) {
     const int32_t best_score = (int32_t)(key >> 32);
     const uint32_t best_j = ((uint32_t)key) >> 1, best_state = ((uint32_t)key) & 1u;
     mem_sync();  // the traceback words of every lane
+    // Every lane walks the same path (the same words, read from LDS); rows come
+    // in blocks of WIDE_TB_ROWS, loaded by the whole wave when the walk leaves
+    // the block it has.
     uint32_t nruns = 0, off = 0, span = 0;
-    double lk = 0.0;
-    if (lane == 0) {
-      auto tb = [&](uint32_t ii, uint32_t jj) -> uint32_t {
-        const uint32_t rel = jj - 1, tt = rel >> 8, ln = (rel >> 2) & 63u, k = rel & 3u;
-        return ((uint32_t)TB[((uint64_t)ii * T + tt) * 64 + ln] >> (4 * k)) & 15u;
-      };
-      dp_traceback(m, best_j, best_state, tb, st, nruns, off);
-      for (uint32_t t2 = 0; t2 < nruns; t2++) {
-        const uint32_t op = st.runs[t2] & 15u;
-        if (op == 0 || op == 2) span += st.runs[t2] >> 4;
+    uint32_t lo = m + 1;  // first row of the staged block (none yet)
+    const uint32_t* TB32 = (const uint32_t*)TB;
+    auto tb = [&](uint32_t ii, uint32_t jj) -> uint32_t {
+      if (ii < lo) {
+        lo = ii + 1 > (uint32_t)WIDE_TB_ROWS ? ii + 1 - (uint32_t)WIDE_TB_ROWS : 1u;
+        const uint32_t nw = (ii - lo + 1) * T * 32, w0 = lo * T * 32;
+        wave_sync();
+        for (uint32_t w = (uint32_t)lane; w < nw; w += 64) tbs[w] = TB32[w0 + w];
+        wave_sync();
       }
-      lk = dp_log_lk(a.T, st, m, qlen, nruns, off, span);
+      const uint32_t rel = jj - 1, tt = rel >> 8, ln = (rel >> 2) & 63u, k = rel & 3u;
+      const uint16_t h = ((const uint16_t*)tbs)[((ii - lo) * T + tt) * 64 + ln];
+      return ((uint32_t)h >> (4 * k)) & 15u;
+    };
+    dp_traceback(m, best_j, best_state, tb, st, nruns, off);
+    for (uint32_t t2 = 0; t2 < nruns; t2++) {
+      const uint32_t op = st.runs[t2] & 15u;
+      if (op == 0 || op == 2) span += st.runs[t2] >> 4;
     }
-    nruns = shfl_u32(nruns, 0);
-    off = shfl_u32(off, 0);
-    span = shfl_u32(span, 0);
+    const double lk = dp_log_lk(a.T, st, m, qlen, nruns, off, span);
     wave_sync();
     const uint64_t cells = (uint64_t)m * n;
     dp_emit(a, jid, st, nruns, off, span, best_score, lk, cells);
@@ -3595,8 +3601,9 @@ uint32_t dp_wide_tiles(uint32_t max_n) {
 void launch_dp_wide(const DpArgs& a, uint32_t nwaves, hipStream_t s) {
   if (a.njobs == 0) return;
   const uint32_t blocks = nwaves < a.njobs ? nwaves : a.njobs;
-  const size_t lds = dp_tiled_lds_bytes(a.max_m, a.max_n);
-  switch (dp_wide_tiles(a.max_n)) {
+  const uint32_t T = dp_wide_tiles(a.max_n);
+  const size_t lds = dp_tiled_lds_bytes(a.max_m, a.max_n) + (size_t)WIDE_TB_ROWS * T * 128;
+  switch (T) {
     case 1: hipLaunchKernelGGL(k_dp_wide<1>, dim3(blocks), dim3(64), lds, s, a); break;
     case 2: hipLaunchKernelGGL(k_dp_wide<2>, dim3(blocks), dim3(64), lds, s, a); break;
     case 3: hipLaunchKernelGGL(k_dp_wide<3>, dim3(blocks), dim3(64), lds, s, a); break;
