@@ -298,6 +298,8 @@ struct pbw_ctx {
   DevBuf<Counters> d_ctr_w;  // the cross-alignment DP's counters
   uint8_t* h_pe = nullptr;   // pinned: a pair batch's counts and observations
   size_t h_pe_cap = 0;
+  uint8_t* h_up = nullptr;   // pinned: bases and qualities of a batch on their way up
+  size_t h_up_cap = 0;
   DevBuf<uint64_t> d_cig_src, d_rec_src;
   DevBuf<uint32_t> d_rec_bsum;
   DevBuf<double> d_obs;
@@ -610,18 +612,32 @@ static void batch_upload(pbw_ctx* c, Batch& b, const std::vector<const uint8_t*>
     b.qo[i + 1] = b.qo[i] + ql[i];
     b.max_m = std::max<uint32_t>(b.max_m, (uint32_t)std::max(sl[i], ql[i]));
   }
-  std::vector<uint8_t> seq(b.so[B] + 1), qual(b.qo[B] + 1);
-  for (uint32_t i = 0; i < B; i++) {
-    if (sl[i]) memcpy(seq.data() + b.so[i], sp[i], sl[i]);
-    if (ql[i]) memcpy(qual.data() + b.qo[i], qp[i], ql[i]);
+  // packed into pinned staging (bases, then qualities), then two copies up
+  const size_t ns = b.so[B] + 1, nq = b.qo[B] + 1;
+  if (c->h_up_cap < ns + nq) {
+    const size_t want = std::max(ns + nq, (c->h_up_cap * 3) / 2);
+    if (c->h_up) (void)hipHostFree(c->h_up);
+    c->h_up = nullptr;
+    c->h_up_cap = 0;
+    if (hipHostMalloc((void**)&c->h_up, want, hipHostMallocDefault) != hipSuccess || !c->h_up)
+      throw PbwError{PBW_ERR_LIMIT, "pinned host allocation of " + std::to_string(want) + " bytes failed"};
+    c->h_up_cap = want;
   }
-  c->d_seq.upload(seq.data(), seq.size(), s);
-  c->d_qual.upload(qual.data(), qual.size(), s);
+  uint8_t* seq = c->h_up;
+  uint8_t* qual = c->h_up + ns;
+  for (uint32_t i = 0; i < B; i++) {
+    if (sl[i]) memcpy(seq + b.so[i], sp[i], sl[i]);
+    if (ql[i]) memcpy(qual + b.qo[i], qp[i], ql[i]);
+  }
+  seq[ns - 1] = 0;
+  qual[nq - 1] = 0;
+  c->d_seq.upload(seq, ns, s);
+  c->d_qual.upload(qual, nq, s);
   c->d_seq_off.upload(b.so.data(), B + 1, s);
   c->d_qual_off.upload(b.qo.data(), B + 1, s);
   b.R = DevReads{c->d_seq.p, c->d_seq_off.p, c->d_qual.p, c->d_qual_off.p, B};
   c->d_rcode.ensure(b.so[B] + 1);
-  HIP_CHECK(hipStreamSynchronize(s));  // host staging vectors die here
+  HIP_CHECK(hipStreamSynchronize(s));  // the staging buffer is reused by the next batch
 }
 
 // Uploads a contiguous slice [b0, b1) of a read batch.
@@ -1790,6 +1806,7 @@ void pbw_ctx_destroy(pbw_ctx* c) {
   if (c->stream) (void)hipStreamDestroy(c->stream);
   if (c->h_nh) (void)hipHostFree(c->h_nh);
   if (c->h_pe) (void)hipHostFree(c->h_pe);
+  if (c->h_up) (void)hipHostFree(c->h_up);
   delete c;
 }
 
@@ -1951,6 +1968,7 @@ int pbw_map_pe(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_batch* m2, p
     auto t0 = std::chrono::steady_clock::now();
     HostRecords hr;
     hr.rec.reserve(2 * m1->n);
+    hr.cigar.reserve(4 * m1->n);  // two runs per record; grows if more
     map_pe_all(c, m1, m2, model, hr);
     emit_batch(hr, out);
     c->stats.ms_total = ms_since(t0);

@@ -3409,24 +3409,34 @@ void launch_dp_tiled(const DpArgs& a, uint32_t nwaves, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // Wide windows whose band covers every row entirely (the align_at_window mate
 // windows of paired-end: js = 1 and je = n on every row, single.rs:220-258):
-// k_dp_tiled's recurrence with four columns per lane, 256-column tiles and
-// the previous row's M, I and D in registers (T tiles, n <= 256 T).
+// k_dp_tiled's recurrence with four columns per lane and 256-column tiles, one
+// wave per tile (a block of T waves per job, n <= 256 T), the previous row's
+// M, I and D in registers.
+//  - Wave w runs one row behind wave w - 1 (step s: row s - w) and takes the
+//    previous tile's last column of its row (M, I, D) from LDS: M and D as the
+//    current row's left neighbours, and, one step later, all three as the
+//    diagonal.  Two LDS slots per tile (row parity) and a block barrier per
+//    step keep writer and reader apart.
 //  - Up neighbour: the same register; diagonal: the lane's previous column or,
-//    for its first column, the left lane's last one by a wave_shr DPP move
-//    (lane 0 takes the previous tile's last column by readlane).
+//    for its first column, the left lane's last one by a wave_shr DPP move.
 //  - D: lane-local max-plus prefix over four columns, then an inclusive max
 //    scan over the wave of (A - position * ge), exactly as k_dp_tiled's 64
 //    lanes but one scan per 256 columns.
 //  - Traceback bits as k_dp_tiled's (M from I, M from D, I extended, D
-//    extended), 16 bits per lane, tile and row in global scratch.
+//    extended), 16 bits per lane, tile and row in global scratch; wave 0 walks
+//    them from LDS-staged row blocks.
 // Jobs of another shape, or wider than 256 T columns, are flagged to
 // k_dp_tiled.
 // ---------------------------------------------------------------------------
 constexpr uint32_t WIDE_TB_ROWS = 16;  // traceback rows staged in LDS at a time
+__host__ __device__ inline uint32_t dp_wide_lds_bytes(uint32_t max_m, uint32_t max_n, uint32_t T) {
+  return dp_tiled_lds_bytes(max_m, max_n) + WIDE_TB_ROWS * T * 128 + 32 * T;
+}
 template <int T>
-__global__ __launch_bounds__(64) void k_dp_wide(DpArgs a) {
+__global__ __launch_bounds__(64 * T) void k_dp_wide(DpArgs a) {
   extern __shared__ __align__(16) uint8_t dyn[];
   const int lane = lane_id();
+  const uint32_t w = threadIdx.x >> 6;  // this wave's tile
   DpStage st;
   st.q = dyn;
   st.qq = st.q + ((a.max_m + 15) & ~15u);
@@ -3434,8 +3444,9 @@ __global__ __launch_bounds__(64) void k_dp_wide(DpArgs a) {
   st.go = (int16_t*)(st.ref + ((a.max_n + 15) & ~15u));
   st.runs = (uint32_t*)((uint8_t*)st.go + (((a.max_n * 2) + 15) & ~15u));
   uint16_t* TB = (uint16_t*)a.tb + (uint64_t)blockIdx.x * (uint64_t)(a.max_m + 1) * T * 64;
-  // traceback staging: WIDE_TB_ROWS rows of every tile, walked from LDS
-  uint32_t* tbs = (uint32_t*)(dyn + dp_tiled_lds_bytes(a.max_m, a.max_n));
+  uint32_t* tbs = (uint32_t*)(dyn + dp_tiled_lds_bytes(a.max_m, a.max_n));  // traceback staging
+  int64_t* ekey = (int64_t*)(tbs + WIDE_TB_ROWS * T * 32);                  // per tile: endpoint key
+  int32_t* carry = (int32_t*)(ekey + T);                                    // [tile][row parity][M, I, D]
   const int32_t ge = a.gap_extend;
   unsigned long long c_cells = 0;
   const uint32_t nj = a.njobs_dev ? *a.njobs_dev : a.njobs;
@@ -3450,42 +3461,68 @@ __global__ __launch_bounds__(64) void k_dp_wide(DpArgs a) {
     const bool rect = m >= 1 && n >= 1 && m <= a.max_m && n <= a.max_n && n <= (uint32_t)(256 * T) &&
                       (int64_t)m + h - b <= 1 && 1 + h + b >= (int64_t)n;
     if (!rect) {
-      if (lane == 0) dp_flag(a, jid);
+      if (threadIdx.x == 0) dp_flag(a, jid);
       continue;
     }
-    dp_stage_job(a, job, m, qlen, so, qo, st);
-    // per-column constants: reference byte and gap open of column c = 256 tt + 4 lane + k + 1
-    uint32_t rbyte[T][4];
-    int32_t gop[T][4];
-    int32_t PM[T][4], PI[T][4], PD[T][4];
-#pragma unroll
-    for (int tt = 0; tt < T; tt++) {
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        const uint32_t c = 256u * tt + 4u * lane + k + 1u;
-        const bool v = c <= n;
-        rbyte[tt][k] = v ? st.ref[c - 1] : 0x100u;
-        gop[tt][k] = v ? (int32_t)st.go[c - 1] : -1;
-        PM[tt][k] = 0;  // row 0: M = 0 on [0, n], I = D = NEG_INF
-        PI[tt][k] = NEG_INF;
-        PD[tt][k] = NEG_INF;
-      }
+    // staging, then each row's mismatch penalty (st.runs is free until the
+    // traceback): no table load on the row loop's path
+    int32_t* rmis = (int32_t*)st.runs;
+    if (w == 0) {
+      dp_stage_job(a, job, m, qlen, so, qo, st);
+      for (uint32_t x = (uint32_t)lane; x < m; x += 64) rmis[x] = a.T.mis_pen[x < qlen ? st.qq[x] : 30u];
     }
-    for (uint32_t i = 1; i <= m; i++) {
-      const uint32_t rb = st.q[i - 1];
-      const uint32_t qv = (i - 1) < qlen ? st.qq[i - 1] : 30u;
-      const int32_t mis = a.T.mis_pen[qv];
-      // previous row at the column left of the tile (column 0 is in the band only in row 0)
-      int32_t cM = i == 1 ? 0 : NEG_INF, cI = NEG_INF, cD = NEG_INF;
-      int32_t curMl = NEG_INF, curDl = NEG_INF;  // current row, column left of the tile
+    __syncthreads();
+    // per-column constants: reference byte and gap open of column c = 256 w + 4 lane + k + 1
+    uint32_t rbyte[4];
+    int32_t gop[4];
+    int32_t PM[4], PI[4], PD[4];
 #pragma unroll
-      for (int tt = 0; tt < T; tt++) {
-        const int32_t dM0 = dpp_wave_shr1(cM, PM[tt][3]);
-        const int32_t dI0 = dpp_wave_shr1(cI, PI[tt][3]);
-        const int32_t dD0 = dpp_wave_shr1(cD, PD[tt][3]);
-        cM = readlane_i32(PM[tt][3], 63);
-        cI = readlane_i32(PI[tt][3], 63);
-        cD = readlane_i32(PD[tt][3], 63);
+    for (int k = 0; k < 4; k++) {
+      const uint32_t c = 256u * w + 4u * lane + k + 1u;
+      const bool v = c <= n;
+      rbyte[k] = v ? st.ref[c - 1] : 0x100u;
+      gop[k] = v ? (int32_t)st.go[c - 1] : -1;
+      PM[k] = 0;  // row 0: M = 0 on [0, n], I = D = NEG_INF
+      PI[k] = NEG_INF;
+      PD[k] = NEG_INF;
+    }
+    // previous row at column 256 w (row 0 to start with)
+    int32_t cM = 0, cI = NEG_INF, cD = NEG_INF;
+    // Row i reads the previous row from one register set and writes the other;
+    // the sets alternate by step parity (odd steps read P*, even steps N*), so
+    // no copies.  Both start as row 0: a wave's first row may fall on either.
+    int32_t NM[4], NI[4], ND[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      NM[k] = PM[k];
+      NI[k] = PI[k];
+      ND[k] = PD[k];
+    }
+    auto step = [&](uint32_t s, const int32_t (&PM)[4], const int32_t (&PI)[4], const int32_t (&PD)[4],
+                    int32_t (&NM)[4], int32_t (&NI)[4], int32_t (&ND)[4]) {
+      const int32_t i = (int32_t)s - (int32_t)w;
+      if (i >= 1 && i <= (int32_t)m) {
+        const uint32_t rb = st.q[i - 1];
+        const int32_t mis = rmis[i - 1];
+        int32_t dgM, dgI, dgD, lM, lD;  // column 256 w: previous row, current row
+        if (w == 0) {  // column 0 is in the band only in row 0
+          dgM = i == 1 ? 0 : NEG_INF;
+          dgI = dgD = NEG_INF;
+          lM = lD = NEG_INF;
+        } else {
+          const int32_t* cr = carry + ((w - 1) * 2 + ((uint32_t)i & 1u)) * 3;
+          dgM = cM;
+          dgI = cI;
+          dgD = cD;
+          cM = cr[0];
+          cI = cr[1];
+          cD = cr[2];
+          lM = cM;
+          lD = cD;
+        }
+        const int32_t dM0 = dpp_wave_shr1(dgM, PM[3]);
+        const int32_t dI0 = dpp_wave_shr1(dgI, PI[3]);
+        const int32_t dD0 = dpp_wave_shr1(dgD, PD[3]);
         int32_t M[4], I[4];
         uint32_t bits = 0;
         // Columns past n only feed columns past n (every dependency runs
@@ -3495,11 +3532,11 @@ __global__ __launch_bounds__(64) void k_dp_wide(DpArgs a) {
         // from the sources alone.
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-          const int32_t dM = k ? PM[tt][k - 1] : dM0, dI = k ? PI[tt][k - 1] : dI0, dD = k ? PD[tt][k - 1] : dD0;
-          const int32_t sc = (rb == rbyte[tt][k]) ? MATCH_SCORE : mis;
+          const int32_t dM = k ? PM[k - 1] : dM0, dI = k ? PI[k - 1] : dI0, dD = k ? PD[k - 1] : dD0;
+          const int32_t sc = (rb == rbyte[k]) ? MATCH_SCORE : mis;
           const int32_t xid = max(dI, dD);
           const int32_t Mk = max(dM, xid) + sc;
-          const int32_t io = PM[tt][k] + gop[tt][k], ie = PI[tt][k] + ge;
+          const int32_t io = PM[k] + gop[k], ie = PI[k] + ge;
           const int32_t Ik = max(io, ie);
           M[k] = Mk;
           I[k] = Ik;
@@ -3507,19 +3544,18 @@ __global__ __launch_bounds__(64) void k_dp_wide(DpArgs a) {
           bits |= (tm | (ie > io ? 4u : 0u)) << (4 * k);
         }
         // D: A(c) = M(c - 1) + go(c); D(c) = max(A(c), D(c - 1) + ge)
-        const int32_t Ml0 = dpp_wave_shr1(curMl, M[3]);
+        const int32_t Ml0 = dpp_wave_shr1(lM, M[3]);
         int32_t A[4], key[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
           const int32_t pos = 4 * lane + k;
-          A[k] = (k ? M[k - 1] : Ml0) + gop[tt][k];
+          A[k] = (k ? M[k - 1] : Ml0) + gop[k];
           key[k] = A[k] - pos * ge;
         }
         const int32_t l1 = max(key[0], key[1]), l2 = max(l1, key[2]), l3 = max(l2, key[3]);
         const int32_t incl = wave_incl_max(l3);
         const int32_t ex = dpp_wave_shr1(INT32_MIN, incl);  // lanes left of this one
-        const int32_t Sent = curDl + ge;
-        const int32_t base = max(ex, Sent);
+        const int32_t base = max(ex, lD + ge);
         int32_t D[4];
         D[0] = 4 * lane * ge + max(key[0], base);
         D[1] = (4 * lane + 1) * ge + max(l1, base);
@@ -3527,70 +3563,88 @@ __global__ __launch_bounds__(64) void k_dp_wide(DpArgs a) {
         D[3] = (4 * lane + 3) * ge + max(l3, base);
 #pragma unroll
         for (int k = 0; k < 4; k++) bits |= (D[k] != A[k] ? 8u : 0u) << (4 * k);
-        curMl = readlane_i32(M[3], 63);
-        curDl = readlane_i32(D[3], 63);
-        TB[((uint64_t)i * T + tt) * 64 + lane] = (uint16_t)bits;
+        TB[((uint64_t)i * T + w) * 64 + lane] = (uint16_t)bits;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-          PM[tt][k] = M[k];
-          PI[tt][k] = I[k];
-          PD[tt][k] = D[k];
+          NM[k] = M[k];
+          NI[k] = I[k];
+          ND[k] = D[k];
         }
+        if (w + 1 < (uint32_t)T && lane == 63) {  // the tile's last column to wave w + 1
+          int32_t* cw = carry + (w * 2 + ((uint32_t)i & 1u)) * 3;
+          cw[0] = M[3];
+          cw[1] = I[3];
+          cw[2] = D[3];
+        }
+      }
+    };
+    for (uint32_t s = 1; s < m + T; s += 2) {
+      step(s, PM, PI, PD, NM, NI, ND);
+      __syncthreads();
+      if (s + 1 < m + T) {
+        step(s + 1, NM, NI, ND, PM, PI, PD);
+        __syncthreads();
+      }
+    }
+    if ((m + w) & 1u) {  // row m was written to N* (an odd step)
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        PM[k] = NM[k];
+        PI[k] = NI[k];
       }
     }
     // endpoint over row m: last maximum over (j asc; M then I)
     int64_t key = INT64_MIN;
 #pragma unroll
-    for (int tt = 0; tt < T; tt++) {
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        const uint32_t c = 256u * tt + 4u * lane + k + 1u;
-        if (c <= n) {
-          const int64_t km = end_key(PM[tt][k], 2u * c), ki = end_key(PI[tt][k], 2u * c + 1u);
-          const int64_t k2 = km > ki ? km : ki;
-          key = key > k2 ? key : k2;
-        }
+    for (int k = 0; k < 4; k++) {
+      const uint32_t c = 256u * w + 4u * lane + k + 1u;
+      if (c <= n) {
+        const int64_t km = end_key(PM[k], 2u * c), ki = end_key(PI[k], 2u * c + 1u);
+        const int64_t k2 = km > ki ? km : ki;
+        key = key > k2 ? key : k2;
       }
     }
     key = wave_max_i64(key);
-    {
-      const int64_t koob = end_key(NEG_INF, 1u);  // column 0 of the last row lies outside its band
-      key = key > koob ? key : koob;
-    }
-    const int32_t best_score = (int32_t)(key >> 32);
-    const uint32_t best_j = ((uint32_t)key) >> 1, best_state = ((uint32_t)key) & 1u;
-    mem_sync();  // the traceback words of every lane
-    // Every lane walks the same path (the same words, read from LDS); rows come
-    // in blocks of WIDE_TB_ROWS, loaded by the whole wave when the walk leaves
-    // the block it has.
-    uint32_t nruns = 0, off = 0, span = 0;
-    uint32_t lo = m + 1;  // first row of the staged block (none yet)
-    const uint32_t* TB32 = (const uint32_t*)TB;
-    auto tb = [&](uint32_t ii, uint32_t jj) -> uint32_t {
-      if (ii < lo) {
-        lo = ii + 1 > (uint32_t)WIDE_TB_ROWS ? ii + 1 - (uint32_t)WIDE_TB_ROWS : 1u;
-        const uint32_t nw = (ii - lo + 1) * T * 32, w0 = lo * T * 32;
-        wave_sync();
-        for (uint32_t w = (uint32_t)lane; w < nw; w += 64) tbs[w] = TB32[w0 + w];
-        wave_sync();
+    if (lane == 0) ekey[w] = key;
+    __syncthreads();  // also orders every wave's traceback words before wave 0 reads them
+    if (w == 0) {
+      key = end_key(NEG_INF, 1u);  // column 0 of the last row lies outside its band
+#pragma unroll
+      for (int tt = 0; tt < T; tt++) key = key > ekey[tt] ? key : ekey[tt];
+      const int32_t best_score = (int32_t)(key >> 32);
+      const uint32_t best_j = ((uint32_t)key) >> 1, best_state = ((uint32_t)key) & 1u;
+      // Every lane walks the same path (the same words, read from LDS); rows
+      // come in blocks of WIDE_TB_ROWS, loaded by the whole wave when the walk
+      // leaves the block it has.
+      uint32_t nruns = 0, off = 0, span = 0;
+      uint32_t lo = m + 1;  // first row of the staged block (none yet)
+      const uint32_t* TB32 = (const uint32_t*)TB;
+      auto tb = [&](uint32_t ii, uint32_t jj) -> uint32_t {
+        if (ii < lo) {
+          lo = ii + 1 > WIDE_TB_ROWS ? ii + 1 - WIDE_TB_ROWS : 1u;
+          const uint32_t nw = (ii - lo + 1) * T * 32, w0 = lo * T * 32;
+          wave_sync();
+          for (uint32_t x = (uint32_t)lane; x < nw; x += 64) tbs[x] = TB32[w0 + x];
+          wave_sync();
+        }
+        const uint32_t rel = jj - 1, tt = rel >> 8, ln = (rel >> 2) & 63u, k = rel & 3u;
+        const uint16_t hw = ((const uint16_t*)tbs)[((ii - lo) * T + tt) * 64 + ln];
+        return ((uint32_t)hw >> (4 * k)) & 15u;
+      };
+      dp_traceback(m, best_j, best_state, tb, st, nruns, off);
+      for (uint32_t t2 = 0; t2 < nruns; t2++) {
+        const uint32_t op = st.runs[t2] & 15u;
+        if (op == 0 || op == 2) span += st.runs[t2] >> 4;
       }
-      const uint32_t rel = jj - 1, tt = rel >> 8, ln = (rel >> 2) & 63u, k = rel & 3u;
-      const uint16_t h = ((const uint16_t*)tbs)[((ii - lo) * T + tt) * 64 + ln];
-      return ((uint32_t)h >> (4 * k)) & 15u;
-    };
-    dp_traceback(m, best_j, best_state, tb, st, nruns, off);
-    for (uint32_t t2 = 0; t2 < nruns; t2++) {
-      const uint32_t op = st.runs[t2] & 15u;
-      if (op == 0 || op == 2) span += st.runs[t2] >> 4;
+      const double lk = dp_log_lk(a.T, st, m, qlen, nruns, off, span);
+      wave_sync();
+      const uint64_t cells = (uint64_t)m * n;
+      dp_emit(a, jid, st, nruns, off, span, best_score, lk, cells);
+      c_cells += cells;
     }
-    const double lk = dp_log_lk(a.T, st, m, qlen, nruns, off, span);
-    wave_sync();
-    const uint64_t cells = (uint64_t)m * n;
-    dp_emit(a, jid, st, nruns, off, span, best_score, lk, cells);
-    c_cells += cells;
-    wave_sync();
+    __syncthreads();
   }
-  if (lane == 0 && a.ctr) atomicAdd(&a.ctr->cells, c_cells);
+  if (threadIdx.x == 0 && a.ctr) atomicAdd(&a.ctr->cells, c_cells);
 }
 
 // Tiles of k_dp_wide for windows of up to max_n columns (0: too wide for it).
@@ -3602,12 +3656,12 @@ void launch_dp_wide(const DpArgs& a, uint32_t nwaves, hipStream_t s) {
   if (a.njobs == 0) return;
   const uint32_t blocks = nwaves < a.njobs ? nwaves : a.njobs;
   const uint32_t T = dp_wide_tiles(a.max_n);
-  const size_t lds = dp_tiled_lds_bytes(a.max_m, a.max_n) + (size_t)WIDE_TB_ROWS * T * 128;
+  const size_t lds = dp_wide_lds_bytes(a.max_m, a.max_n, T);
   switch (T) {
     case 1: hipLaunchKernelGGL(k_dp_wide<1>, dim3(blocks), dim3(64), lds, s, a); break;
-    case 2: hipLaunchKernelGGL(k_dp_wide<2>, dim3(blocks), dim3(64), lds, s, a); break;
-    case 3: hipLaunchKernelGGL(k_dp_wide<3>, dim3(blocks), dim3(64), lds, s, a); break;
-    case 4: hipLaunchKernelGGL(k_dp_wide<4>, dim3(blocks), dim3(64), lds, s, a); break;
+    case 2: hipLaunchKernelGGL(k_dp_wide<2>, dim3(blocks), dim3(128), lds, s, a); break;
+    case 3: hipLaunchKernelGGL(k_dp_wide<3>, dim3(blocks), dim3(192), lds, s, a); break;
+    case 4: hipLaunchKernelGGL(k_dp_wide<4>, dim3(blocks), dim3(256), lds, s, a); break;
     default: break;
   }
 }
