@@ -298,8 +298,8 @@ struct pbw_ctx {
   DevBuf<Counters> d_ctr_w;  // the cross-alignment DP's counters
   uint8_t* h_pe = nullptr;   // pinned: a pair batch's counts and observations
   size_t h_pe_cap = 0;
-  uint8_t* h_up = nullptr;   // pinned: bases and qualities of a batch on their way up
-  size_t h_up_cap = 0;
+  uint8_t* h_up[2] = {nullptr, nullptr};  // pinned: bases and qualities of a pair batch on their way up
+  size_t h_up_cap[2] = {0, 0};
   DevBuf<uint64_t> d_cig_src, d_rec_src;
   DevBuf<uint32_t> d_rec_bsum;
   DevBuf<double> d_obs;
@@ -596,48 +596,73 @@ struct Batch {
   DevReads R{};
 };
 
-// Uploads reads given as per-read (seq, len, qual, qlen) views.
-static void batch_upload(pbw_ctx* c, Batch& b, const std::vector<const uint8_t*>& sp, const std::vector<uint64_t>& sl,
-                         const std::vector<const uint8_t*>& qp, const std::vector<uint64_t>& ql) {
-  hipStream_t s = c->stream;
-  const uint32_t B = (uint32_t)sp.size();
-  b.B = B;
-  b.so.assign(B + 1, 0);
-  b.qo.assign(B + 1, 0);
-  b.max_m = 0;
+// Pairs [p0, p1) of two read batches, interleaved (read 2i = mate 1 and read
+// 2i + 1 = mate 2 of pair p0 + i): offsets, longest read, and bases then
+// qualities packed into the context's pinned staging slot `slot`.  Touches no
+// device state, so it may run on another thread while the stream works.
+struct PairPack {
+  std::vector<uint64_t> so, qo;
+  uint32_t B = 0, max_m = 0;
+  uint8_t* buf = nullptr;
+};
+static void pair_pack(pbw_ctx* c, int slot, const pbw_read_batch* m1, const pbw_read_batch* m2, uint64_t p0,
+                      uint64_t p1, PairPack& pk) {
+  const uint32_t P = (uint32_t)(p1 - p0), B = 2 * P;
+  const pbw_read_batch* mm[2] = {m1, m2};
+  pk.B = B;
+  pk.so.assign(B + 1, 0);
+  pk.qo.assign(B + 1, 0);
+  pk.max_m = 0;
   for (uint32_t i = 0; i < B; i++) {
-    if (sl[i] > (uint64_t)MAX_READ_LEN || ql[i] > (uint64_t)MAX_READ_LEN)
+    const pbw_read_batch* x = mm[i & 1];
+    const uint64_t r = p0 + i / 2;
+    const uint64_t sl = x->seq_off[r + 1] - x->seq_off[r], ql = x->qual_off[r + 1] - x->qual_off[r];
+    if (sl > (uint64_t)MAX_READ_LEN || ql > (uint64_t)MAX_READ_LEN)
       throw PbwError{PBW_ERR_LIMIT, "read longer than " + std::to_string(MAX_READ_LEN) + " bases"};
-    b.so[i + 1] = b.so[i] + sl[i];
-    b.qo[i + 1] = b.qo[i] + ql[i];
-    b.max_m = std::max<uint32_t>(b.max_m, (uint32_t)std::max(sl[i], ql[i]));
+    pk.so[i + 1] = pk.so[i] + sl;
+    pk.qo[i + 1] = pk.qo[i] + ql;
+    pk.max_m = std::max<uint32_t>(pk.max_m, (uint32_t)std::max(sl, ql));
   }
-  // packed into pinned staging (bases, then qualities), then two copies up
-  const size_t ns = b.so[B] + 1, nq = b.qo[B] + 1;
-  if (c->h_up_cap < ns + nq) {
-    const size_t want = std::max(ns + nq, (c->h_up_cap * 3) / 2);
-    if (c->h_up) (void)hipHostFree(c->h_up);
-    c->h_up = nullptr;
-    c->h_up_cap = 0;
-    if (hipHostMalloc((void**)&c->h_up, want, hipHostMallocDefault) != hipSuccess || !c->h_up)
+  const size_t ns = pk.so[B] + 1, nq = pk.qo[B] + 1;
+  if (c->h_up_cap[slot] < ns + nq) {
+    const size_t want = std::max(ns + nq, (c->h_up_cap[slot] * 3) / 2);
+    if (c->h_up[slot]) (void)hipHostFree(c->h_up[slot]);
+    c->h_up[slot] = nullptr;
+    c->h_up_cap[slot] = 0;
+    if (hipHostMalloc((void**)&c->h_up[slot], want, hipHostMallocDefault) != hipSuccess || !c->h_up[slot])
       throw PbwError{PBW_ERR_LIMIT, "pinned host allocation of " + std::to_string(want) + " bytes failed"};
-    c->h_up_cap = want;
+    c->h_up_cap[slot] = want;
   }
-  uint8_t* seq = c->h_up;
-  uint8_t* qual = c->h_up + ns;
+  pk.buf = c->h_up[slot];
+  uint8_t* seq = pk.buf;
+  uint8_t* qual = pk.buf + ns;
   for (uint32_t i = 0; i < B; i++) {
-    if (sl[i]) memcpy(seq + b.so[i], sp[i], sl[i]);
-    if (ql[i]) memcpy(qual + b.qo[i], qp[i], ql[i]);
+    const pbw_read_batch* x = mm[i & 1];
+    const uint64_t r = p0 + i / 2;
+    const uint64_t sl = pk.so[i + 1] - pk.so[i], ql = pk.qo[i + 1] - pk.qo[i];
+    if (sl) memcpy(seq + pk.so[i], x->seq + x->seq_off[r], sl);
+    if (ql) memcpy(qual + pk.qo[i], x->qual + x->qual_off[r], ql);
   }
   seq[ns - 1] = 0;
   qual[nq - 1] = 0;
-  c->d_seq.upload(seq, ns, s);
-  c->d_qual.upload(qual, nq, s);
+}
+
+// A packed pair batch to the device (the staging slot is free again on return).
+static void batch_send(pbw_ctx* c, Batch& b, const PairPack& pk) {
+  hipStream_t s = c->stream;
+  const uint32_t B = pk.B;
+  b.B = B;
+  b.so = pk.so;
+  b.qo = pk.qo;
+  b.max_m = pk.max_m;
+  const size_t ns = pk.so[B] + 1, nq = pk.qo[B] + 1;
+  c->d_seq.upload(pk.buf, ns, s);
+  c->d_qual.upload(pk.buf + ns, nq, s);
   c->d_seq_off.upload(b.so.data(), B + 1, s);
   c->d_qual_off.upload(b.qo.data(), B + 1, s);
   b.R = DevReads{c->d_seq.p, c->d_seq_off.p, c->d_qual.p, c->d_qual_off.p, B};
   c->d_rcode.ensure(b.so[B] + 1);
-  HIP_CHECK(hipStreamSynchronize(s));  // the staging buffer is reused by the next batch
+  HIP_CHECK(hipStreamSynchronize(s));
 }
 
 // Uploads a contiguous slice [b0, b1) of a read batch.
@@ -1208,24 +1233,10 @@ static uint64_t rust_f64_to_usize(double x) {  // `as usize`: saturating, NaN ->
 // The part of a batch of pairs that does not depend on the insert-size model:
 // both mates' reads (interleaved r1, r2) through seeding, clustering and the
 // single-end DP.
-static void pe_prepare(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_batch* m2, uint64_t p0, uint64_t p1,
-                       Batch& b) {
-  const uint32_t P = (uint32_t)(p1 - p0);
-  std::vector<const uint8_t*> sp(2 * P), qp(2 * P);
-  std::vector<uint64_t> sl(2 * P), ql(2 * P);
-  for (uint32_t i = 0; i < P; i++) {
-    const pbw_read_batch* mm[2] = {m1, m2};
-    for (int k = 0; k < 2; k++) {
-      const pbw_read_batch* x = mm[k];
-      uint64_t r = p0 + i;
-      sp[2 * i + k] = x->seq + x->seq_off[r];
-      sl[2 * i + k] = x->seq_off[r + 1] - x->seq_off[r];
-      qp[2 * i + k] = x->qual + x->qual_off[r];
-      ql[2 * i + k] = x->qual_off[r + 1] - x->qual_off[r];
-    }
-  }
-  batch_upload(c, b, sp, sl, qp, ql);
-  batch_candidates(c, b);
+static void pe_prepare(pbw_ctx* c, const PairPack& pk, Batch& b, HostRecords* pre = nullptr) {
+  const uint32_t P = pk.B / 2;
+  batch_send(c, b, pk);
+  batch_candidates(c, b, false, pre);
   batch_dp(c, b, nullptr, 0);
   const size_t nws = (size_t)P * 40;
   c->d_wjobs.ensure(nws);
@@ -1437,6 +1448,33 @@ static void collect_wide_cells(pbw_ctx* c) {
   HIP_CHECK(hipMemsetAsync(c->d_ctr_w.p, 0, sizeof(Counters), c->stream));
 }
 
+// Packs a pair batch on a worker thread while the caller keeps the stream
+// busy; wait() hands over the pack (or rethrows the worker's error).  The
+// destructor waits too, so an error on the caller's side never leaves the
+// worker running.
+struct PackAhead {
+  std::thread th;
+  std::exception_ptr err;
+  void start(pbw_ctx* c, int slot, const pbw_read_batch* m1, const pbw_read_batch* m2, uint64_t p0, uint64_t p1,
+             PairPack& pk) {
+    err = nullptr;
+    th = std::thread([this, c, slot, m1, m2, p0, p1, &pk] {
+      try {
+        pair_pack(c, slot, m1, m2, p0, p1, pk);
+      } catch (...) {
+        err = std::current_exception();
+      }
+    });
+  }
+  void wait() {
+    if (th.joinable()) th.join();
+    if (err) std::rethrow_exception(err);
+  }
+  ~PackAhead() {
+    if (th.joinable()) th.join();
+  }
+};
+
 // lib.rs:145-185: 500 serial warm-up pairs, then 2000-pair batches
 static void map_pe_all(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_batch* m2, pbw_insert_model* model,
                        HostRecords& hr) {
@@ -1445,12 +1483,20 @@ static void map_pe_all(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_batc
   std::vector<double> obs;
   std::vector<uint8_t> has;
   const uint64_t warm = std::min<uint64_t>(n, 500);
+  const uint64_t bs = c->opt.batch_reads ? c->opt.batch_reads : 131072;
+  const uint64_t per_prep = std::max<uint64_t>(1, bs / 2 / 2000) * 2000;
+  PairPack packs[2];
+  PackAhead ahead;
+  int cur = 0;
+  if (warm < n) ahead.start(c, 0, m1, m2, warm, std::min<uint64_t>(warm + per_prep, n), packs[0]);
   if (warm) {
     // the warm-up pairs go through seeding, clustering and the single-end DP
     // together (none of it reads the model); then pair by pair, each on the
     // model its predecessors left, through the model-dependent stages
+    PairPack wp;
+    pair_pack(c, 1, m1, m2, 0, warm, wp);
     Batch b;
-    pe_prepare(c, m1, m2, 0, warm, b);
+    pe_prepare(c, wp, b, &hr);  // the records' pinned space is reserved during the seed kernel
     auto t0 = std::chrono::steady_clock::now();
     WidePool pool{0, (size_t)warm * 40, 0};
     for (uint64_t i = 0; i < warm; i++) {
@@ -1466,13 +1512,15 @@ static void map_pe_all(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_batc
   // the 2000-pair batches: seeding, clustering and the single-end DP for many
   // of them at once (whole batches, so the boundaries stay warm + 2000 k); then
   // batch by batch, each on the model the batches before it left, the
-  // model-dependent stages and that batch's records
-  const uint64_t bs = c->opt.batch_reads ? c->opt.batch_reads : 131072;
-  const uint64_t per_prep = std::max<uint64_t>(1, bs / 2 / 2000) * 2000;
+  // model-dependent stages and that batch's records.  The next group of
+  // batches is packed for upload meanwhile, on a worker thread.
   for (uint64_t s0 = warm; s0 < n; s0 += per_prep) {
     const uint64_t s1 = std::min<uint64_t>(s0 + per_prep, n);
+    ahead.wait();
     Batch b;
-    pe_prepare(c, m1, m2, s0, s1, b);
+    pe_prepare(c, packs[cur], b, s0 == 0 ? &hr : nullptr);
+    if (s1 < n) ahead.start(c, cur ^ 1, m1, m2, s1, std::min<uint64_t>(s1 + per_prep, n), packs[cur ^ 1]);
+    cur ^= 1;
     for (uint64_t b0 = s0; b0 < s1; b0 += 2000) {
       const uint64_t b1 = std::min<uint64_t>(b0 + 2000, s1);
       const uint32_t q0 = (uint32_t)(b0 - s0), q1 = (uint32_t)(b1 - s0);
@@ -1806,7 +1854,8 @@ void pbw_ctx_destroy(pbw_ctx* c) {
   if (c->stream) (void)hipStreamDestroy(c->stream);
   if (c->h_nh) (void)hipHostFree(c->h_nh);
   if (c->h_pe) (void)hipHostFree(c->h_pe);
-  if (c->h_up) (void)hipHostFree(c->h_up);
+  for (uint8_t* p : c->h_up)
+    if (p) (void)hipHostFree(p);
   delete c;
 }
 
@@ -1967,8 +2016,7 @@ int pbw_map_pe(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_batch* m2, p
     c->stats = pbw_stats{};
     auto t0 = std::chrono::steady_clock::now();
     HostRecords hr;
-    hr.rec.reserve(2 * m1->n);
-    hr.cigar.reserve(4 * m1->n);  // two runs per record; grows if more
+    hr.expect = 2 * m1->n;
     map_pe_all(c, m1, m2, model, hr);
     emit_batch(hr, out);
     c->stats.ms_total = ms_since(t0);
