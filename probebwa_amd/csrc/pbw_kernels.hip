@@ -3667,7 +3667,7 @@ void launch_dp_wide(const DpArgs& a, uint32_t nwaves, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// MAPQ: bayesian.rs:193-439 (thread per read, FP64)
+// MAPQ: bayesian.rs:193-439 (FP64; a group of lanes per read)
 // ---------------------------------------------------------------------------
 constexpr double LN_10 = 2.302585092994045684017991454684364208;
 constexpr double DB = -0.1 * LN_10;
@@ -3897,17 +3897,24 @@ __device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int l) {
   const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
   return ((uint64_t)hi << 32) | lo;
 }
-__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t x) {
-  return (uint32_t)readlane_i32((int32_t)wave_incl_scan(x), 63);
+
+// The group of NL lanes (NL a power of two, 8..64, aligned in the wave) that
+// holds one read: every lane of it active, in the same control flow.
+template <int NL>
+__device__ __forceinline__ uint32_t group_sum_u32(uint32_t x) {
+#pragma unroll
+  for (int o = NL / 2; o > 0; o >>= 1) x += (uint32_t)__shfl_xor((int)x, o, 64);
+  return x;
 }
 
-// log_entropy with the per-base and per-word counting spread over the lanes:
+// log_entropy with the per-base and per-word counting spread over the group:
 // integer counts (exact in any order), then the same FP64 tail.  The operation
 // scan runs on every lane alike; an M run's loop in log_entropy stops at the
 // read's or the window's end, which bounds the bases counted here the same way.
-__device__ double wave_log_entropy(const DevGenome& G, const DevLuts& T, const uint32_t* cig, uint32_t ncig,
-                                   const QueryView& Q, uint64_t gb, uint32_t span) {
-  const uint32_t lane = (uint32_t)lane_id();
+template <int NL>
+__device__ double group_log_entropy(const DevGenome& G, const DevLuts& T, const uint32_t* cig, uint32_t ncig,
+                                    const QueryView& Q, uint64_t gb, uint32_t span) {
+  const uint32_t gl = (uint32_t)lane_id() & (uint32_t)(NL - 1);
   int64_t indel_runs = 0, deletion_runs = 0;
   uint32_t readlen = 0, refpos = 0, curtype = 0, n_l = 0, mm_l = 0;
   for (uint32_t c = 0; c < ncig; c++) {
@@ -3929,7 +3936,7 @@ __device__ double wave_log_entropy(const DevGenome& G, const DevLuts& T, const u
       curtype = 0;
       const uint32_t ra = readlen < Q.L ? Q.L - readlen : 0u, rb = refpos < span ? span - refpos : 0u;
       const uint32_t cnt = min(len, min(ra, rb));
-      for (uint32_t u = lane; u < cnt; u += 64) {
+      for (uint32_t u = gl; u < cnt; u += NL) {
         const uint32_t x = readlen + u;
         const uint32_t q = x < Q.qlen ? Q.q(x) : 0u;
         const bool hq = q >= 10u;
@@ -3944,26 +3951,28 @@ __device__ double wave_log_entropy(const DevGenome& G, const DevLuts& T, const u
   uint32_t gc_l = 0, tot_l = 0;
   if (span) {
     const uint64_t end = gb + span;
-    for (uint64_t w = (gb >> 5) + lane; w <= (end - 1) >> 5; w += 64) gc_word(G, w, gb, end, gc_l, tot_l);
+    for (uint64_t w = (gb >> 5) + gl; w <= (end - 1) >> 5; w += NL) gc_word(G, w, gb, end, gc_l, tot_l);
   }
-  const int64_t n = wave_sum_u32(n_l), mm = wave_sum_u32(mm_l);
-  return entropy_from_counts(T, n, mm, readlen, indel_runs - deletion_runs, deletion_runs, wave_sum_u32(gc_l),
-                             wave_sum_u32(tot_l));
+  const int64_t n = group_sum_u32<NL>(n_l), mm = group_sum_u32<NL>(mm_l);
+  return entropy_from_counts(T, n, mm, readlen, indel_runs - deletion_runs, deletion_runs, group_sum_u32<NL>(gc_l),
+                             group_sum_u32<NL>(tot_l));
 }
 
 // missing_locus with one window per lane: window 5 j + ph starts at ph + 15 j
-// and multiplies / adds its 15 positions in order, as there; lane ph then adds
-// phase ph's window terms in window order, and the five phases are summed in
-// order, so every double is the same.
-__device__ double wave_missing_locus(const DevLuts& T, const uint8_t* qual, uint32_t qlen, const uint8_t* rcode,
-                                     uint32_t nwin) {
+// and multiplies / adds its 15 positions in order, as there; lane ph of the
+// group then adds phase ph's window terms in window order, and the five phases
+// are summed in order, so every double is the same.
+template <int NL>
+__device__ double group_missing_locus(const DevLuts& T, const uint8_t* qual, uint32_t qlen, const uint8_t* rcode,
+                                      uint32_t nwin) {
+  static_assert(NL >= 8 && NL <= 64 && (NL & (NL - 1)) == 0, "group of 8..64 lanes");
   constexpr uint32_t k = K, PH = 5;
   if (qlen < k) return 0.5;
-  const uint32_t lane = (uint32_t)lane_id();
+  const uint32_t lane = (uint32_t)lane_id(), gl = lane & (uint32_t)(NL - 1), g0 = lane - gl;
   const uint32_t nids = PH * ((qlen - k) / k + 1);
-  double acc = 0.0;  // lanes 0..4: log_p of phase `lane`
-  for (uint32_t base = 0; base < nids; base += 64) {
-    const uint32_t id = base + lane, start = id % PH + k * (id / PH);
+  double acc = 0.0;  // lanes 0..4 of the group: log_p of phase `gl`
+  for (uint32_t base = 0; base < nids; base += NL) {
+    const uint32_t id = base + gl, start = id % PH + k * (id / PH);
     const bool valid = id < nids && start + k <= qlen;
     double term = 0.0;
     if (valid) {
@@ -3975,16 +3984,17 @@ __device__ double wave_missing_locus(const DevLuts& T, const uint8_t* qual, uint
       }
       term = missing_window_log(T, z, p1, qual, qlen, rcode, nwin, start);
     }
-    const uint64_t vmask = __ballot(valid);
-    const uint32_t first = (lane + PH - base % PH) % PH;  // first offset in this pass of phase `lane`
-    for (uint32_t o0 = 0; o0 < 64; o0 += PH) {
+    const uint32_t first = (gl + PH - base % PH) % PH;  // first offset in this pass of phase `gl`
+    for (uint32_t o0 = 0; o0 < (uint32_t)NL; o0 += PH) {
       const uint32_t o = first + o0;
-      const double v = __shfl(term, (int)(o < 64 ? o : 0), 64);
-      if (lane < PH && o < 64 && ((vmask >> o) & 1ull)) acc += v;
+      const int src = (int)(g0 + (o < (uint32_t)NL ? o : 0u));
+      const double v = __shfl(term, src, 64);
+      const int ok = __shfl(valid ? 1 : 0, src, 64);
+      if (gl < PH && o < (uint32_t)NL && ok) acc += v;
     }
   }
   double offset_means = 0.0;
-  for (uint32_t ph = 0; ph < PH; ph++) offset_means += exp(readlane_f64(acc, (int)ph));
+  for (uint32_t ph = 0; ph < PH; ph++) offset_means += exp(__shfl(acc, (int)(g0 + ph), 64));
   return fmin(offset_means / 5.0, 0.99);
 }
 
@@ -4021,13 +4031,14 @@ struct RecOut {  // pbw_record layout
 };
 static_assert(sizeof(RecOut) == 64, "record layout");
 
-// Two lanes per read (2r + part), both in the same control flow: both take the
-// selection; part 0 walks the CIGAR for the log-entropy term, part 1 takes the
-// missing-locus term, and part 0 combines them and writes the record.
+// MAPQ_LANES lanes per read, all in the same control flow: every lane takes
+// the selection; the group then computes the log-entropy and missing-locus
+// terms together, and its first lane combines them and writes the record.
+constexpr int MAPQ_LANES = 8;
 __global__ void k_mapq(MapqArgs a) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t part = t & 1u;
-  uint32_t r = t >> 1;
+  const uint32_t gl = t & (uint32_t)(MAPQ_LANES - 1);
+  uint32_t r = t / MAPQ_LANES;
   if (a.rlist) {
     if (r >= a.nlist) return;
     r = a.rlist[r];
@@ -4044,7 +4055,7 @@ __global__ void k_mapq(MapqArgs a) {
   o.pad[0] = o.pad[1] = o.pad[2] = 0;
   const uint32_t nj = a.njobs[r];
   if (nj == 0) {  // single.rs:362-383 unmapped_record
-    if (part) return;
+    if (gl) return;
     o.pos = 0;
     o.log_lk = 0.0;
     o.posterior = 0.0;
@@ -4080,14 +4091,15 @@ __global__ void k_mapq(MapqArgs a) {
   const DpOut dout = a.out[s0 + bi];
   QueryView Q{a.R.seq + so, a.R.qual + qo, L, qlen, job.rc != 0};
   const uint32_t* cig = a.cigar + (s0 + bi) * a.cigar_cap;
-  const double part_v = part == 0 ? log_entropy(a.G, a.T, cig, dout.cigar_n, Q, job.ref_gb + dout.ref_off, dout.span)
-                                  : missing_locus(a.T, a.R.qual + qo, qlen, a.rcode + so,
-                                                  L >= (uint32_t)K ? L - K + 1 : 0);
-  const double miss_v = __shfl_xor(part_v, 1, 64);  // part 0 receives the missing-locus term
-  if (part) return;
+  const double le =
+      group_log_entropy<MAPQ_LANES>(a.G, a.T, cig, dout.cigar_n, Q, job.ref_gb + dout.ref_off, dout.span);
+  const double miss = le - 3.0 <= 0.0 ? 1.0
+                                      : group_missing_locus<MAPQ_LANES>(a.T, a.R.qual + qo, qlen, a.rcode + so,
+                                                                        L >= (uint32_t)K ? L - K + 1 : 0);
+  if (gl) return;
   uint8_t mapq;
   double post;
-  finalize_mapq_from(a.T, se, part_v, miss_v, mapq, post);
+  finalize_mapq_from(a.T, se, le, miss, mapq, post);
   const uint32_t contig = a.cands[job.cand].contig;
   o.pos = job.ref_gb - a.G.coff[contig] + dout.ref_off + 1;  // SAM is 1-based (single.rs:192)
   o.log_lk = mx;
@@ -4105,7 +4117,7 @@ __global__ void k_mapq(MapqArgs a) {
 void launch_mapq(const MapqArgs& a, hipStream_t s) {
   const uint32_t n = a.rlist ? a.nlist : a.R.n;
   if (n == 0) return;
-  hipLaunchKernelGGL(k_mapq, dim3((2 * n + 127) / 128), dim3(128), 0, s, a);
+  hipLaunchKernelGGL(k_mapq, dim3(((uint64_t)MAPQ_LANES * n + 127) / 128), dim3(128), 0, s, a);
 }
 
 // ---------------------------------------------------------------------------
