@@ -277,9 +277,8 @@ struct pbw_ctx {
   // batch workspace
   DevBuf<uint8_t> d_seq, d_qual, d_rcode;
   DevBuf<uint64_t> d_seq_off, d_qual_off, d_hoff, d_hits, d_hits_ov, d_scratch, d_ovoff;
-  uint32_t* h_nh = nullptr;  // pinned: per-read hit counts of the seed pass
-  size_t h_nh_cap = 0;
   uint32_t hit_cap = HIT_CAP;  // first-pass hits per read, adapted batch to batch
+  DevBuf<uint32_t> d_seed_st;  // the seed pass's overflow counts
   DevBuf<uint32_t> d_nhits, d_list, d_hcap, d_tier2, d_mid, d_ntier2, d_best, d_cigar, d_cigar_out, d_cigar_off;
   DevBuf<Cand> d_cands;
   DevBuf<uint8_t> d_ncand, d_njobs, d_rec, d_refbytes;
@@ -741,16 +740,17 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
     const size_t wc = std::max(pre->cigar.n + (size_t)B * 4, pre->expect * 4);
     if (wc > pre->cigar.cap) pre->cigar.reserve(std::max(wc, pre->cigar.cap * 2));
   }
-  if (c->h_nh_cap < B) {
-    if (c->h_nh) (void)hipHostFree(c->h_nh);
-    c->h_nh = nullptr;
-    c->h_nh_cap = 0;
-    if (hipHostMalloc((void**)&c->h_nh, (size_t)B * 4, hipHostMallocDefault) != hipSuccess || !c->h_nh)
-      throw PbwError{PBW_ERR_LIMIT, "pinned host allocation of " + std::to_string((size_t)B * 4) + " bytes failed"};
-    c->h_nh_cap = B;
-  }
-  const uint32_t* nh = c->h_nh;
-  HIP_CHECK(hipMemcpyAsync(c->h_nh, c->d_nhits.p, (size_t)B * 4, hipMemcpyDeviceToHost, s));
+  // overflow reads listed on the device; their counts to the host
+  c->d_list.ensure(B);
+  c->d_ovoff.ensure(B);
+  c->d_hcap.ensure(B);
+  c->d_seed_st.ensure(8);
+  HIP_CHECK(hipMemsetAsync(c->d_seed_st.p, 0, 8 * sizeof(uint32_t), s));
+  launch_seed_overflow(c->d_nhits.p, B, cap1, (uint64_t)B * cap1, c->d_list.p, c->d_ovoff.p, c->d_hcap.p,
+                       c->d_seed_st.p, s);
+  HIP_CHECK(hipGetLastError());
+  uint32_t st[8];
+  HIP_CHECK(hipMemcpyAsync(st, c->d_seed_st.p, sizeof(st), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
   {
     float ms = 0.f;
@@ -758,24 +758,13 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
     c->stats.ms_k_seed += ms;
     c->stats.n_k_seed += 1;
   }
-  // overflow: rerun the affected reads with exact capacity into a second arena
-  std::vector<uint32_t> ov;
-  uint64_t ov_total = 0;
-  uint32_t max_hits = 0;
-  for (uint32_t i = 0; i < B; i++) {
-    max_hits = std::max(max_hits, nh[i]);
-    if (nh[i] > cap1) {
-      ov.push_back(i);
-      ov_total += nh[i];
-    }
-  }
-  if ((uint64_t)ov.size() * 20 > B && cap1 < (uint32_t)HIT_CAP_MAX) {
+  const uint32_t n_ov = st[0], max_hits = st[1];
+  const uint64_t ov_total = (uint64_t)st[6] | ((uint64_t)st[7] << 32);
+  if ((uint64_t)n_ov * 20 > B && cap1 < (uint32_t)HIT_CAP_MAX) {
     // more than 5 % of the reads overflowed: the next batches' first pass takes
     // the least power of two that would have left at most 5 % (within 24 GiB of
     // hit slots and flags)
-    uint32_t over[4] = {0, 0, 0, 0};  // reads with more than HIT_CAP << k hits
-    for (uint32_t r : ov)
-      for (int k = 0; k < 4; k++) over[k] += nh[r] > ((uint32_t)HIT_CAP << k);
+    const uint32_t* over = st + 2;  // reads with more than HIT_CAP << k hits
     uint32_t want = cap1;
     for (int k = 0; k < 4; k++)
       if ((uint64_t)over[k] * 20 > B) want = std::max<uint32_t>(want, (uint32_t)HIT_CAP << (k + 1));
@@ -784,7 +773,7 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
   }
   const uint64_t* hits_base = c->d_hits.p;
   const uint8_t* hflag_base = feat ? c->d_hflag.p : nullptr;
-  if (!ov.empty()) {
+  if (n_ov) {
     // overflow slots after the fixed ones: in place when the slack holds them,
     // else in a second arena that starts with a copy of the fixed slots
     uint64_t* arena = c->d_hits.p;
@@ -799,20 +788,9 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
         farena = c->d_hflag_ov.p;
       }
     }
-    std::vector<uint64_t> ovoff(ov.size());
-    std::vector<uint32_t> ovcap(ov.size());
-    uint64_t cur = (uint64_t)B * cap1;
-    for (size_t k = 0; k < ov.size(); k++) {
-      ovoff[k] = cur;
-      ovcap[k] = nh[ov[k]];
-      cur += nh[ov[k]];
-    }
-    c->d_ovoff.upload(ovoff.data(), ov.size(), s);
-    c->d_list.upload(ov.data(), ov.size(), s);
-    c->d_hcap.upload(ovcap.data(), ov.size(), s);
     SeedArgs sb = sa;
     sb.rlist = c->d_list.p;
-    sb.nlist = (uint32_t)ov.size();
+    sb.nlist = n_ov;
     sb.hoff = c->d_ovoff.p;
     sb.hcap = c->d_hcap.p;
     sb.hits = arena;
@@ -821,12 +799,12 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
     launch_seed(sb, s);
     HIP_CHECK(hipGetLastError());
     // the overflow reads' slots replace their fixed ones in the per-read offsets
-    launch_scatter_u64(c->d_hoff.p, c->d_list.p, c->d_ovoff.p, (uint32_t)ov.size(), s);
+    launch_scatter_u64(c->d_hoff.p, c->d_list.p, c->d_ovoff.p, n_ov, s);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(s));
     hits_base = arena;
     hflag_base = farena;
-    c->stats.seed_overflow_reads += ov.size();
+    c->stats.seed_overflow_reads += n_ov;
   }
   c->stats.ms_seed += ms_since(t0);
   t0 = std::chrono::steady_clock::now();
@@ -1852,7 +1830,6 @@ void pbw_ctx_destroy(pbw_ctx* c) {
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
-  if (c->h_nh) (void)hipHostFree(c->h_nh);
   if (c->h_pe) (void)hipHostFree(c->h_pe);
   for (uint8_t* p : c->h_up)
     if (p) (void)hipHostFree(p);

@@ -4394,6 +4394,34 @@ __global__ void k_scatter_u64(uint64_t* out, const uint32_t* idx, const uint64_t
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) out[idx[k]] = val[k];
 }
+// The seed pass's overflow reads (more than cap1 hits), found on the device:
+// each gets a list entry, its exact hit count and a disjoint range of slots
+// after the fixed ones (base + running total; the list order is immaterial,
+// since every read's hits stay in its own range).  st: [0] overflow reads,
+// [1] most hits of any read, [2..5] reads with more than HIT_CAP << k hits,
+// [6..7] total overflow hits (64 bits).  st must start at zero.
+__global__ __launch_bounds__(256) void k_seed_overflow(const uint32_t* nhits, uint32_t B, uint32_t cap1, uint64_t base,
+                                                       uint32_t* list, uint64_t* off, uint32_t* cap, uint32_t* st) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t nh = i < B ? nhits[i] : 0u;
+  uint32_t mx = nh;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, o, 64));
+  if (lane_id() == 0 && mx) atomicMax(&st[1], mx);
+  if (nh <= cap1) return;
+  const uint32_t k = atomicAdd(&st[0], 1u);
+  const uint64_t at = atomicAdd((unsigned long long*)(st + 6), (unsigned long long)nh);
+  list[k] = i;
+  off[k] = base + at;
+  cap[k] = nh;
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+    if (nh > ((uint32_t)HIT_CAP << j)) atomicAdd(&st[2 + j], 1u);
+}
+void launch_seed_overflow(const uint32_t* nhits, uint32_t B, uint32_t cap1, uint64_t base, uint32_t* list,
+                          uint64_t* off, uint32_t* cap, uint32_t* st, hipStream_t s) {
+  if (B) hipLaunchKernelGGL(k_seed_overflow, dim3((B + 255) / 256), dim3(256), 0, s, nhits, B, cap1, base, list, off, cap, st);
+}
 void launch_scatter_u64(uint64_t* out, const uint32_t* idx, const uint64_t* val, uint32_t n, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_scatter_u64, dim3((n + 255) / 256), dim3(256), 0, s, out, idx, val, n);
 }
