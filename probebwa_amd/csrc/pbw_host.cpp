@@ -744,10 +744,9 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
   c->d_list.ensure(B);
   c->d_ovoff.ensure(B);
   c->d_hcap.ensure(B);
-  c->d_seed_st.ensure(8);
-  HIP_CHECK(hipMemsetAsync(c->d_seed_st.p, 0, 8 * sizeof(uint32_t), s));
-  launch_seed_overflow(c->d_nhits.p, B, cap1, (uint64_t)B * cap1, c->d_list.p, c->d_ovoff.p, c->d_hcap.p,
-                       c->d_seed_st.p, s);
+  c->d_seed_st.ensure(8 + 8 * (size_t)seed_overflow_blocks(B));
+  launch_seed_overflow(c->d_nhits.p, B, cap1, (uint64_t)B * cap1, c->d_seed_st.p + 8, c->d_list.p, c->d_ovoff.p,
+                       c->d_hcap.p, c->d_seed_st.p, s);
   HIP_CHECK(hipGetLastError());
   uint32_t st[8];
   HIP_CHECK(hipMemcpyAsync(st, c->d_seed_st.p, sizeof(st), hipMemcpyDeviceToHost, s));

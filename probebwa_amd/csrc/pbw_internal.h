@@ -308,8 +308,9 @@ void launch_fill_pres(const uint32_t* koff, uint64_t* pres, uint32_t rot, hipStr
 // out[i] = i * stride for i < n
 void launch_fill_stride(uint64_t* out, uint32_t n, uint64_t stride, hipStream_t s);
 // out[idx[k]] = val[k] for k < n
-void launch_seed_overflow(const uint32_t* nhits, uint32_t B, uint32_t cap1, uint64_t base, uint32_t* list,
-                          uint64_t* off, uint32_t* cap, uint32_t* st, hipStream_t s);
+uint32_t seed_overflow_blocks(uint32_t B);  // 8 words of scratch each
+void launch_seed_overflow(const uint32_t* nhits, uint32_t B, uint32_t cap1, uint64_t base, uint32_t* part,
+                          uint32_t* list, uint64_t* off, uint32_t* cap, uint32_t* st, hipStream_t s);
 void launch_scatter_u64(uint64_t* out, const uint32_t* idx, const uint64_t* val, uint32_t n, hipStream_t s);
 void launch_fill_koff(uint32_t* koff, const uint32_t* keys, const uint32_t* kstart, uint64_t nkeys, uint32_t total,
                       hipStream_t s);

@@ -4394,33 +4394,140 @@ __global__ void k_scatter_u64(uint64_t* out, const uint32_t* idx, const uint64_t
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) out[idx[k]] = val[k];
 }
-// The seed pass's overflow reads (more than cap1 hits), found on the device:
-// each gets a list entry, its exact hit count and a disjoint range of slots
-// after the fixed ones (base + running total; the list order is immaterial,
-// since every read's hits stay in its own range).  st: [0] overflow reads,
-// [1] most hits of any read, [2..5] reads with more than HIT_CAP << k hits,
-// [6..7] total overflow hits (64 bits).  st must start at zero.
-__global__ __launch_bounds__(256) void k_seed_overflow(const uint32_t* nhits, uint32_t B, uint32_t cap1, uint64_t base,
-                                                       uint32_t* list, uint64_t* off, uint32_t* cap, uint32_t* st) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// The seed pass's overflow reads (more than cap1 hits), found on the device in
+// three passes without global atomics: per-block counts, one block's scan of
+// them, and the list itself in read order (entry k: read, exact hit count,
+// and slots base + the hits of the overflow reads before it).  st: [0]
+// overflow reads, [1] most hits of any read, [2..5] reads with more than
+// HIT_CAP << k hits, [6..7] total overflow hits (64 bits).
+constexpr uint32_t OV_BLOCK = 1024;
+// Block-wide exclusive sum of x (OV_BLOCK threads); *total gets the sum.
+__device__ __forceinline__ uint32_t ov_block_scan(uint32_t x, uint32_t* sh, uint32_t& total) {
+  const uint32_t w = threadIdx.x >> 6, l = (uint32_t)lane_id();
+  const uint32_t incl = wave_incl_scan(x);
+  if (l == 63) sh[w] = incl;
+  __syncthreads();
+  uint32_t before = 0, tot = 0;
+  for (uint32_t v = 0; v < OV_BLOCK / 64; v++) {
+    const uint32_t t = sh[v];
+    before += v < w ? t : 0u;
+    tot += t;
+  }
+  __syncthreads();
+  total = tot;
+  return before + incl - x;
+}
+// pass 1: per block [count, max, over0..3, hits]
+__global__ __launch_bounds__(OV_BLOCK) void k_ov_count(const uint32_t* nhits, uint32_t B, uint32_t cap1, uint32_t* part) {
+  __shared__ uint32_t sh[OV_BLOCK / 64];
+  __shared__ uint32_t smax[OV_BLOCK / 64];
+  const uint32_t i = blockIdx.x * OV_BLOCK + threadIdx.x;
   const uint32_t nh = i < B ? nhits[i] : 0u;
+  const bool ov = nh > cap1;
   uint32_t mx = nh;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, o, 64));
-  if (lane_id() == 0 && mx) atomicMax(&st[1], mx);
-  if (nh <= cap1) return;
-  const uint32_t k = atomicAdd(&st[0], 1u);
-  const uint64_t at = atomicAdd((unsigned long long*)(st + 6), (unsigned long long)nh);
-  list[k] = i;
-  off[k] = base + at;
-  cap[k] = nh;
+  if (lane_id() == 0) smax[threadIdx.x >> 6] = mx;
+  uint32_t c[6] = {ov ? 1u : 0u, ov ? nh : 0u, 0, 0, 0, 0};
 #pragma unroll
-  for (int j = 0; j < 4; j++)
-    if (nh > ((uint32_t)HIT_CAP << j)) atomicAdd(&st[2 + j], 1u);
+  for (int j = 0; j < 4; j++) c[2 + j] = (ov && nh > ((uint32_t)HIT_CAP << j)) ? 1u : 0u;
+  uint32_t tot[6];
+#pragma unroll
+  for (int j = 0; j < 6; j++) (void)ov_block_scan(c[j], sh, tot[j]);
+  if (threadIdx.x == 0) {
+    uint32_t m = 0;
+    for (uint32_t v = 0; v < OV_BLOCK / 64; v++) m = max(m, smax[v]);
+    uint32_t* p = part + (uint64_t)blockIdx.x * 8;
+    p[0] = tot[0];
+    p[1] = m;
+    p[2] = tot[2];
+    p[3] = tot[3];
+    p[4] = tot[4];
+    p[5] = tot[5];
+    p[6] = tot[1];
+    p[7] = 0;
+  }
 }
-void launch_seed_overflow(const uint32_t* nhits, uint32_t B, uint32_t cap1, uint64_t base, uint32_t* list,
-                          uint64_t* off, uint32_t* cap, uint32_t* st, hipStream_t s) {
-  if (B) hipLaunchKernelGGL(k_seed_overflow, dim3((B + 255) / 256), dim3(256), 0, s, nhits, B, cap1, base, list, off, cap, st);
+// pass 2 (one block): block bases (count, hits) in place of the counts, and st
+__global__ __launch_bounds__(OV_BLOCK) void k_ov_scan(uint32_t* part, uint32_t nb, uint32_t* st) {
+  __shared__ uint32_t sh[OV_BLOCK / 64];
+  __shared__ unsigned long long carry_h;
+  __shared__ uint32_t carry_c, cmax, cover[4];
+  if (threadIdx.x == 0) {
+    carry_h = 0;
+    carry_c = cmax = 0;
+    for (int j = 0; j < 4; j++) cover[j] = 0;
+  }
+  __syncthreads();
+  for (uint32_t b0 = 0; b0 < nb; b0 += OV_BLOCK) {
+    const uint32_t bi = b0 + threadIdx.x;
+    uint32_t* p = part + (uint64_t)bi * 8;
+    const bool v = bi < nb;
+    const uint32_t cnt = v ? p[0] : 0u, hits = v ? p[6] : 0u;
+    // the hits in 64 bits: their low and high halves scanned apart (each sum < 2^26)
+    uint32_t tc, tl, tu;
+    const uint32_t ec = ov_block_scan(cnt, sh, tc);
+    const uint32_t el = ov_block_scan(hits & 0xFFFFu, sh, tl), eu = ov_block_scan(hits >> 16, sh, tu);
+    const uint64_t eh = (uint64_t)el + ((uint64_t)eu << 16), th = (uint64_t)tl + ((uint64_t)tu << 16);
+    const uint32_t m = v ? p[1] : 0u;
+    uint32_t o[4];
+    for (int j = 0; j < 4; j++) o[j] = v ? p[2 + j] : 0u;
+    const unsigned long long hb = carry_h;
+    const uint32_t cb = carry_c;
+    if (v) {
+      p[0] = cb + ec;
+      p[6] = (uint32_t)(hb + eh);
+      p[7] = (uint32_t)((hb + eh) >> 32);
+    }
+    uint32_t mm = m;
+#pragma unroll
+    for (int q = 32; q > 0; q >>= 1) mm = max(mm, (uint32_t)__shfl_xor((int)mm, q, 64));
+    uint32_t to[4];
+    for (int j = 0; j < 4; j++) (void)ov_block_scan(o[j], sh, to[j]);
+    if (lane_id() == 0) atomicMax(&cmax, mm);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      carry_c = cb + tc;
+      carry_h = hb + th;
+      for (int j = 0; j < 4; j++) cover[j] += to[j];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    st[0] = carry_c;
+    st[1] = cmax;
+    for (int j = 0; j < 4; j++) st[2 + j] = cover[j];
+    st[6] = (uint32_t)carry_h;
+    st[7] = (uint32_t)(carry_h >> 32);
+  }
+}
+// pass 3: the list, in read order
+__global__ __launch_bounds__(OV_BLOCK) void k_ov_write(const uint32_t* nhits, uint32_t B, uint32_t cap1, uint64_t base,
+                                                      const uint32_t* part, uint32_t* list, uint64_t* off,
+                                                      uint32_t* cap) {
+  __shared__ uint32_t sh[OV_BLOCK / 64];
+  const uint32_t i = blockIdx.x * OV_BLOCK + threadIdx.x;
+  const uint32_t nh = i < B ? nhits[i] : 0u;
+  const bool ov = nh > cap1;
+  uint32_t t0, t1;
+  const uint32_t ec = ov_block_scan(ov ? 1u : 0u, sh, t0);
+  const uint32_t eh = ov_block_scan(ov ? nh : 0u, sh, t1);
+  if (!ov) return;
+  const uint32_t* p = part + (uint64_t)blockIdx.x * 8;
+  const uint32_t k = p[0] + ec;
+  const uint64_t hb = (uint64_t)p[6] | ((uint64_t)p[7] << 32);
+  list[k] = i;
+  off[k] = base + hb + eh;
+  cap[k] = nh;
+}
+uint32_t seed_overflow_blocks(uint32_t B) { return (B + OV_BLOCK - 1) / OV_BLOCK; }
+void launch_seed_overflow(const uint32_t* nhits, uint32_t B, uint32_t cap1, uint64_t base, uint32_t* part,
+                          uint32_t* list, uint64_t* off, uint32_t* cap, uint32_t* st, hipStream_t s) {
+  const uint32_t nb = seed_overflow_blocks(B);
+  if (!nb) return;
+  hipLaunchKernelGGL(k_ov_count, dim3(nb), dim3(OV_BLOCK), 0, s, nhits, B, cap1, part);
+  hipLaunchKernelGGL(k_ov_scan, dim3(1), dim3(OV_BLOCK), 0, s, part, nb, st);
+  hipLaunchKernelGGL(k_ov_write, dim3(nb), dim3(OV_BLOCK), 0, s, nhits, B, cap1, base, part, list, off, cap);
 }
 void launch_scatter_u64(uint64_t* out, const uint32_t* idx, const uint64_t* val, uint32_t n, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_scatter_u64, dim3((n + 255) / 256), dim3(256), 0, s, out, idx, val, n);
