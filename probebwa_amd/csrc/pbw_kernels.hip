@@ -3412,11 +3412,11 @@ void launch_dp_tiled(const DpArgs& a, uint32_t nwaves, hipStream_t s) {
 // k_dp_tiled's recurrence with four columns per lane and 256-column tiles, one
 // wave per tile (a block of T waves per job, n <= 256 T), the previous row's
 // M, I and D in registers.
-//  - Wave w runs one row behind wave w - 1 (step s: row s - w) and takes the
-//    previous tile's last column of its row (M, I, D) from LDS: M and D as the
-//    current row's left neighbours, and, one step later, all three as the
-//    diagonal.  Two LDS slots per tile (row parity) and a block barrier per
-//    step keep writer and reader apart.
+//  - A step is two rows; wave w runs one step behind wave w - 1 (step s: rows
+//    2 (s - w) - 1 and 2 (s - w)) and takes the previous tile's last column of
+//    each row (M, I, D) from LDS: M and D as the row's left neighbours, and,
+//    for the next row, all three as the diagonal.  Four LDS slots per tile
+//    (row mod 4) and a block barrier per step keep writer and reader apart.
 //  - Up neighbour: the same register; diagonal: the lane's previous column or,
 //    for its first column, the left lane's last one by a wave_shr DPP move.
 //  - D: lane-local max-plus prefix over four columns, then an inclusive max
@@ -3430,7 +3430,7 @@ void launch_dp_tiled(const DpArgs& a, uint32_t nwaves, hipStream_t s) {
 // ---------------------------------------------------------------------------
 constexpr uint32_t WIDE_TB_ROWS = 16;  // traceback rows staged in LDS at a time
 __host__ __device__ inline uint32_t dp_wide_lds_bytes(uint32_t max_m, uint32_t max_n, uint32_t T) {
-  return dp_tiled_lds_bytes(max_m, max_n) + WIDE_TB_ROWS * T * 128 + 32 * T;
+  return dp_tiled_lds_bytes(max_m, max_n) + WIDE_TB_ROWS * T * 128 + 56 * T;
 }
 template <int T>
 __global__ __launch_bounds__(64 * T) void k_dp_wide(DpArgs a) {
@@ -3446,7 +3446,7 @@ __global__ __launch_bounds__(64 * T) void k_dp_wide(DpArgs a) {
   uint16_t* TB = (uint16_t*)a.tb + (uint64_t)blockIdx.x * (uint64_t)(a.max_m + 1) * T * 64;
   uint32_t* tbs = (uint32_t*)(dyn + dp_tiled_lds_bytes(a.max_m, a.max_n));  // traceback staging
   int64_t* ekey = (int64_t*)(tbs + WIDE_TB_ROWS * T * 32);                  // per tile: endpoint key
-  int32_t* carry = (int32_t*)(ekey + T);                                    // [tile][row parity][M, I, D]
+  int32_t* carry = (int32_t*)(ekey + T);                                    // [tile][row % 4][M, I, D]
   const int32_t ge = a.gap_extend;
   unsigned long long c_cells = 0;
   const uint32_t nj = a.njobs_dev ? *a.njobs_dev : a.njobs;
@@ -3488,9 +3488,8 @@ __global__ __launch_bounds__(64 * T) void k_dp_wide(DpArgs a) {
     }
     // previous row at column 256 w (row 0 to start with)
     int32_t cM = 0, cI = NEG_INF, cD = NEG_INF;
-    // Row i reads the previous row from one register set and writes the other;
-    // the sets alternate by step parity (odd steps read P*, even steps N*), so
-    // no copies.  Both start as row 0: a wave's first row may fall on either.
+    // Row i reads the previous row from one register set and writes the other:
+    // odd rows read P* and write N*, even rows the reverse, so no copies.
     int32_t NM[4], NI[4], ND[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -3498,9 +3497,8 @@ __global__ __launch_bounds__(64 * T) void k_dp_wide(DpArgs a) {
       NI[k] = PI[k];
       ND[k] = PD[k];
     }
-    auto step = [&](uint32_t s, const int32_t (&PM)[4], const int32_t (&PI)[4], const int32_t (&PD)[4],
-                    int32_t (&NM)[4], int32_t (&NI)[4], int32_t (&ND)[4]) {
-      const int32_t i = (int32_t)s - (int32_t)w;
+    auto row = [&](int32_t i, const int32_t (&PM)[4], const int32_t (&PI)[4], const int32_t (&PD)[4],
+                   int32_t (&NM)[4], int32_t (&NI)[4], int32_t (&ND)[4]) {
       if (i >= 1 && i <= (int32_t)m) {
         const uint32_t rb = st.q[i - 1];
         const int32_t mis = rmis[i - 1];
@@ -3510,7 +3508,7 @@ __global__ __launch_bounds__(64 * T) void k_dp_wide(DpArgs a) {
           dgI = dgD = NEG_INF;
           lM = lD = NEG_INF;
         } else {
-          const int32_t* cr = carry + ((w - 1) * 2 + ((uint32_t)i & 1u)) * 3;
+          const int32_t* cr = carry + ((w - 1) * 4 + ((uint32_t)i & 3u)) * 3;
           dgM = cM;
           dgI = cI;
           dgD = cD;
@@ -3571,22 +3569,23 @@ __global__ __launch_bounds__(64 * T) void k_dp_wide(DpArgs a) {
           ND[k] = D[k];
         }
         if (w + 1 < (uint32_t)T && lane == 63) {  // the tile's last column to wave w + 1
-          int32_t* cw = carry + (w * 2 + ((uint32_t)i & 1u)) * 3;
+          int32_t* cw = carry + (w * 4 + ((uint32_t)i & 3u)) * 3;
           cw[0] = M[3];
           cw[1] = I[3];
           cw[2] = D[3];
         }
       }
     };
-    for (uint32_t s = 1; s < m + T; s += 2) {
-      step(s, PM, PI, PD, NM, NI, ND);
+    // step s: rows 2 g - 1 and 2 g of group g = s - w (the first from P* into
+    // N*, the second back), then the block barrier
+    const uint32_t ng = (m + 1) / 2;
+    for (uint32_t s = 1; s < ng + T; s++) {
+      const int32_t g = (int32_t)s - (int32_t)w;
+      row(2 * g - 1, PM, PI, PD, NM, NI, ND);
+      row(2 * g, NM, NI, ND, PM, PI, PD);
       __syncthreads();
-      if (s + 1 < m + T) {
-        step(s + 1, NM, NI, ND, PM, PI, PD);
-        __syncthreads();
-      }
     }
-    if ((m + w) & 1u) {  // row m was written to N* (an odd step)
+    if (m & 1u) {  // row m was the first of its group: in N*
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         PM[k] = NM[k];
