@@ -256,6 +256,38 @@ def test_map_se_first_pass_capacity_adapts():
     assert ov1 > 0 and ov2 <= ov1, (ov1, ov2)
 
 
+def test_map_se_batch_over_a_million_reads_matches_smaller_batches():
+    """1.1 M reads in one batch: the seed pass's overflow reads (those from a
+    400-copy repeat, scattered through the batch) are listed over more than
+    1024 blocks of 1024 reads, so the block scan runs in two chunks.  The
+    records equal those of 65,536-read batches, byte for byte."""
+    rng = np.random.default_rng(7)
+    unit = synth.random_bases(rng, 41)
+    rep = np.concatenate([synth.mutate(rng, unit, 0.03) for _ in range(400)])
+    uniq = synth.random_bases(rng, 3_000_000)
+    g = P.Genome()
+    g.add_contig("rep", rep.tobytes())
+    g.add_contig("uniq", uniq.tobytes())
+    n, L, n_rep = 1_100_000, 36, 3000
+    src = np.where(np.arange(n) < n_rep, 0, 1)
+    rng.shuffle(src)
+    starts = np.where(src == 0, rng.integers(0, len(rep) - L, n), rng.integers(0, len(uniq) - L, n))
+    idx = starts[:, None] + np.arange(L)[None, :]
+    seq = np.where((src == 0)[:, None], rep[np.minimum(idx, len(rep) - 1)], uniq[idx]).astype(np.uint8).ravel()
+    qual = rng.integers(20, 41, n * L).astype(np.uint8)
+    off = np.arange(0, n * L + 1, L, dtype=np.uint64)
+    reads = P.ReadBatch.from_arrays(seq, off, qual, off)
+    ctx = P.Context(g)
+    ctx.tune = dict(batch_reads=1_200_000)
+    big = ctx.map_se(reads)
+    assert ctx.stats()["seed_overflow_reads"] > 0
+    ctx2 = P.Context(g)
+    ctx2.tune = dict(batch_reads=65536)
+    small = ctx2.map_se(reads)
+    assert np.array_equal(np.asarray(big.rec).view(np.uint8), np.asarray(small.rec).view(np.uint8))
+    assert np.array_equal(np.asarray(big.cigar), np.asarray(small.cigar))
+
+
 @pytest.mark.parametrize("n_contigs,lmax", [(60, 30), (150, 60)])
 def test_map_se_many_contig_groups(n_contigs, lmax):
     """A 400 bp unit in every contig, in alternating orientation: short reads
