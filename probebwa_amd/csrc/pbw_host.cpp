@@ -292,8 +292,8 @@ struct pbw_ctx {
   // paired-end workspace
   DevBuf<DpJob> d_wjobs;
   DevBuf<DpOut> d_wout;
-  DevBuf<uint8_t> d_wvalid, d_slcand, d_has_obs;
-  DevBuf<uint32_t> d_plan, d_wcigar, d_name_class, d_wlist, d_pe_cnt;
+  DevBuf<uint8_t> d_wvalid, d_slcand, d_pe_io;
+  DevBuf<uint32_t> d_plan, d_wcigar, d_name_class, d_wlist;
   DevBuf<Counters> d_ctr_w;  // the cross-alignment DP's counters
   uint8_t* h_pe = nullptr;   // pinned: a pair batch's counts and observations
   size_t h_pe_cap = 0;
@@ -301,7 +301,6 @@ struct pbw_ctx {
   size_t h_up_cap[2] = {0, 0};
   DevBuf<uint64_t> d_cig_src, d_rec_src;
   DevBuf<uint32_t> d_rec_bsum;
-  DevBuf<double> d_obs;
   // candidate prefilter network
   DevBuf<float> d_mlp_f;     // theta | m | v | b1 | b2 | b3
   DevBuf<uint16_t> d_mlp_h;  // w1t | w2t | w2 | w3
@@ -1223,8 +1222,6 @@ static void pe_prepare(pbw_ctx* c, const PairPack& pk, Batch& b, HostRecords* pr
   c->d_wout.ensure(nws);
   c->d_rec.ensure((size_t)2 * P * sizeof(pbw_record));
   c->d_cig_src.ensure(2 * (size_t)P);
-  c->d_obs.ensure(P);
-  c->d_has_obs.ensure(P);
 }
 
 // Cross-alignment windows from a device list whose length is on the device
@@ -1298,8 +1295,12 @@ static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, const
                      HostRecords* hr, std::vector<double>& obs_out, std::vector<uint8_t>& has_out) {
   hipStream_t s = c->stream;
   const uint32_t P = q1 - q0, nws = P * 40;
-  c->d_pe_cnt.ensure(4);
-  uint32_t* d_wn = c->d_pe_cnt.p;  // [0] windows listed, [1] flagged by k_dp_wide, [2] beyond the tiled shape
+  // the call's device words for the host, one copy up at the end: 4 counts
+  // ([0] windows listed, [1] flagged by k_dp_wide, [2] beyond the tiled shape,
+  // [3] the records' CIGAR total), then the observations and their flags
+  const size_t io_bytes = 16 + (size_t)P * 9;
+  c->d_pe_io.ensure(io_bytes);
+  uint32_t* d_wn = reinterpret_cast<uint32_t*>(c->d_pe_io.p);
   HIP_CHECK(hipMemsetAsync(d_wn, 0, 4 * sizeof(uint32_t), s));
   c->d_wlist.ensure(nws);
   PairArgs pa{};
@@ -1359,8 +1360,8 @@ static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, const
   pa.wbase = wbase;
   pa.rec = c->d_rec.p;
   pa.cig_src = c->d_cig_src.p;
-  pa.obs = c->d_obs.p;
-  pa.has_obs = c->d_has_obs.p;
+  pa.obs = reinterpret_cast<double*>(c->d_pe_io.p + 16);
+  pa.has_obs = c->d_pe_io.p + 16 + (size_t)P * 8;
   launch_pair_finalize(pa, s);
   HIP_CHECK(hipGetLastError());
   // records [2 q0, 2 q1): CIGAR offsets on the device, then one synchronisation
@@ -1374,14 +1375,14 @@ static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, const
       throw PbwError{PBW_ERR_LIMIT, "more than 2^32 CIGAR words in one call; map fewer reads per call"};
     base = (uint32_t)hr->cigar.size();
     const uint32_t nb = (nrec + REC_BLOCK - 1) / REC_BLOCK;
-    c->d_rec_bsum.ensure(nb + 1);
-    d_total = c->d_rec_bsum.p + nb;
+    c->d_rec_bsum.ensure(nb);
+    d_total = d_wn + 3;
     launch_rec_offsets(d_rec, d_src, nrec, base, c->d_rec_bsum.p, d_total, s);
     HIP_CHECK(hipGetLastError());
     pbw_record* hrec = hr->rec.grow(nrec);
     HIP_CHECK(hipMemcpyAsync(hrec, d_rec, (size_t)nrec * sizeof(pbw_record), hipMemcpyDeviceToHost, s));
   }
-  const size_t stage = 16 + (size_t)P * 9;
+  const size_t stage = io_bytes;
   if (c->h_pe_cap < stage) {
     if (c->h_pe) (void)hipHostFree(c->h_pe);
     c->h_pe = nullptr;
@@ -1390,13 +1391,10 @@ static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, const
       throw PbwError{PBW_ERR_LIMIT, "pinned host allocation of " + std::to_string(stage) + " bytes failed"};
     c->h_pe_cap = stage;
   }
-  uint32_t* h_cnt = reinterpret_cast<uint32_t*>(c->h_pe);  // 3 counts + total
+  uint32_t* h_cnt = reinterpret_cast<uint32_t*>(c->h_pe);
   double* h_obs = reinterpret_cast<double*>(c->h_pe + 16);
   uint8_t* h_has = c->h_pe + 16 + (size_t)P * 8;
-  HIP_CHECK(hipMemcpyAsync(h_cnt, d_wn, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  if (hr) HIP_CHECK(hipMemcpyAsync(h_cnt + 3, d_total, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipMemcpyAsync(h_obs, c->d_obs.p + q0, (size_t)P * 8, hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipMemcpyAsync(h_has, c->d_has_obs.p + q0, P, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(c->h_pe, c->d_pe_io.p, io_bytes, hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
   if (h_cnt[2])
     throw PbwError{PBW_ERR_LIMIT, wmax > 8000 ? "insert-size search window wider than 8000 bases"

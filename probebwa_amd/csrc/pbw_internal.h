@@ -285,7 +285,7 @@ struct PairArgs {
   const uint32_t* name_class;              // contig -> first contig with the same name
   uint8_t* rec;            // 2P records
   uint64_t* cig_src;       // per record CIGAR location (bit 63: cross-alignment pool)
-  double* obs;
+  double* obs;             // per pair of the launch (index p - p_base)
   uint8_t* has_obs;
   uint32_t P;              // pairs handled by this launch: p_base .. p_base + P - 1 of the batch
   uint32_t p_base;
