@@ -3433,7 +3433,7 @@ __host__ __device__ inline uint32_t dp_wide_lds_bytes(uint32_t max_m, uint32_t m
   return dp_tiled_lds_bytes(max_m, max_n) + WIDE_TB_ROWS * T * 128 + 56 * T;
 }
 template <int T>
-__global__ __launch_bounds__(64 * T) void k_dp_wide(DpArgs a) {
+__global__ __launch_bounds__(64 * T) __attribute__((amdgpu_waves_per_eu(4))) void k_dp_wide(DpArgs a) {
   extern __shared__ __align__(16) uint8_t dyn[];
   const int lane = lane_id();
   const uint32_t w = threadIdx.x >> 6;  // this wave's tile
@@ -3474,7 +3474,7 @@ __global__ __launch_bounds__(64 * T) void k_dp_wide(DpArgs a) {
     __syncthreads();
     // per-column constants: reference byte and gap open of column c = 256 w + 4 lane + k + 1
     uint32_t rbyte[4];
-    int32_t gop[4];
+    int32_t gop[4], gk[4];  // gap open, and gap open - position * ge
     int32_t PM[4], PI[4], PD[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -3482,6 +3482,7 @@ __global__ __launch_bounds__(64 * T) void k_dp_wide(DpArgs a) {
       const bool v = c <= n;
       rbyte[k] = v ? st.ref[c - 1] : 0x100u;
       gop[k] = v ? (int32_t)st.go[c - 1] : -1;
+      gk[k] = gop[k] - (int32_t)(4 * lane + k) * ge;
       PM[k] = 0;  // row 0: M = 0 on [0, n], I = D = NEG_INF
       PI[k] = NEG_INF;
       PD[k] = NEG_INF;
@@ -3541,26 +3542,24 @@ __global__ __launch_bounds__(64 * T) void k_dp_wide(DpArgs a) {
           const uint32_t tm = dM >= xid ? 0u : (dI >= dD ? 1u : 2u);
           bits |= (tm | (ie > io ? 4u : 0u)) << (4 * k);
         }
-        // D: A(c) = M(c - 1) + go(c); D(c) = max(A(c), D(c - 1) + ge)
+        // D: A(c) = M(c - 1) + go(c); D(c) = max(A(c), D(c - 1) + ge), in key
+        // space (key = A - position * ge, so key = M(c - 1) + gk(c)): the prefix
+        // maximum there, and D differs from A exactly where it differs from key
         const int32_t Ml0 = dpp_wave_shr1(lM, M[3]);
-        int32_t A[4], key[4];
+        int32_t key[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-          const int32_t pos = 4 * lane + k;
-          A[k] = (k ? M[k - 1] : Ml0) + gop[k];
-          key[k] = A[k] - pos * ge;
-        }
+        for (int k = 0; k < 4; k++) key[k] = (k ? M[k - 1] : Ml0) + gk[k];
         const int32_t l1 = max(key[0], key[1]), l2 = max(l1, key[2]), l3 = max(l2, key[3]);
         const int32_t incl = wave_incl_max(l3);
         const int32_t ex = dpp_wave_shr1(INT32_MIN, incl);  // lanes left of this one
         const int32_t base = max(ex, lD + ge);
+        const int32_t dk[4] = {max(key[0], base), max(l1, base), max(l2, base), max(l3, base)};
         int32_t D[4];
-        D[0] = 4 * lane * ge + max(key[0], base);
-        D[1] = (4 * lane + 1) * ge + max(l1, base);
-        D[2] = (4 * lane + 2) * ge + max(l2, base);
-        D[3] = (4 * lane + 3) * ge + max(l3, base);
 #pragma unroll
-        for (int k = 0; k < 4; k++) bits |= (D[k] != A[k] ? 8u : 0u) << (4 * k);
+        for (int k = 0; k < 4; k++) {
+          D[k] = (4 * lane + k) * ge + dk[k];
+          bits |= (dk[k] != key[k] ? 8u : 0u) << (4 * k);
+        }
         TB[((uint64_t)i * T + w) * 64 + lane] = (uint16_t)bits;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
