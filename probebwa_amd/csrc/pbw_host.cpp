@@ -143,96 +143,6 @@ static bool read_line(FILE* f, std::string& line) {
 }
 
 // ---------------------------------------------------------------------------
-// index build: hashtable.rs:248-322 (k=15, stride 5, cap 300 in scan order).
-// Buckets are keyed by the 30-bit code: mix() (hashtable.rs:117-124) is a
-// bijection, so the code -> bucket map equals the hash -> bucket map.
-// ---------------------------------------------------------------------------
-struct HostIndex {
-  std::vector<uint32_t> keys;    // distinct codes, ascending
-  std::vector<uint32_t> kstart;  // bucket start per key (+ total at the end)
-  std::vector<uint64_t> pos;     // packed positions (hashtable.rs:157-164)
-};
-
-static void build_index(const pbw_genome* g, HostIndex& out) {
-  std::vector<uint32_t> code;
-  std::vector<uint64_t> packed;
-  uint64_t est = 0;
-  for (uint64_t L : g->lengths) est += 2 * (L / STRIDE + 1);
-  code.reserve(est);
-  packed.reserve(est);
-  const uint32_t MASK30 = (1u << 30) - 1;
-  for (size_t cid = 0; cid < g->lengths.size(); cid++) {
-    const uint64_t L = g->lengths[cid], base = g->coff[cid];
-    if (L < (uint64_t)K) continue;
-    const uint64_t nw = L - K + 1;
-    std::vector<uint32_t> fcode(nw), rcode(nw);
-    std::vector<uint8_t> clean(nw);
-    uint32_t f = 0, r = 0;
-    int nrun = 0;  // N count in the current window
-    for (uint64_t p = 0; p < L; p++) {
-      uint32_t b = gbase2(g, base + p);
-      bool isn = gisn(g, base + p);
-      uint32_t fb = isn ? 0 : b;  // base2bits('N') == 0
-      f = ((f << 2) | fb) & MASK30;
-      r = (r >> 2) | ((isn ? 0u : (3u - b)) << 28);  // complement of 'N' is 'N' -> 0
-      nrun += isn ? 1 : 0;
-      if (p >= (uint64_t)K) nrun -= gisn(g, base + p - K) ? 1 : 0;
-      if (p + 1 >= (uint64_t)K) {
-        uint64_t w = p + 1 - K;
-        fcode[w] = f;
-        rcode[w] = r;
-        clean[w] = nrun == 0;
-      }
-    }
-    // forward strand: windows i = 0, 5, 10, ...
-    for (uint64_t i = 0; i < nw; i += STRIDE) {
-      if (!clean[i]) continue;
-      code.push_back(fcode[i]);
-      packed.push_back(((uint64_t)cid << 40) | i);
-    }
-    // reverse-complement strand: rc windows i = 0, 5, ... at position L-i-15
-    for (uint64_t i = 0; i < nw; i += STRIDE) {
-      uint64_t w = L - i - K;
-      if (!clean[w]) continue;
-      code.push_back(rcode[w]);
-      packed.push_back((1ull << 63) | ((uint64_t)cid << 40) | w);
-    }
-  }
-  // stable LSD radix sort by code (2 x 15 bits)
-  size_t n = code.size();
-  std::vector<uint32_t> c2(n);
-  std::vector<uint64_t> p2(n);
-  for (int pass = 0; pass < 2; pass++) {
-    int sh = pass * 15;
-    std::vector<uint64_t> cnt((1u << 15) + 1, 0);
-    for (size_t i = 0; i < n; i++) cnt[((code[i] >> sh) & 0x7FFF) + 1]++;
-    for (size_t i = 0; i < (1u << 15); i++) cnt[i + 1] += cnt[i];
-    for (size_t i = 0; i < n; i++) {
-      uint64_t d = cnt[(code[i] >> sh) & 0x7FFF]++;
-      c2[d] = code[i];
-      p2[d] = packed[i];
-    }
-    code.swap(c2);
-    packed.swap(p2);
-  }
-  out.keys.clear();
-  out.kstart.clear();
-  out.pos.clear();
-  out.pos.reserve(n);
-  size_t i = 0;
-  while (i < n) {
-    uint32_t c = code[i];
-    size_t s = i;
-    while (i < n && code[i] == c) i++;
-    size_t take = std::min<size_t>(i - s, MAX_COUNT);  // first 300 in scan order
-    out.keys.push_back(c);
-    out.kstart.push_back((uint32_t)out.pos.size());
-    out.pos.insert(out.pos.end(), packed.begin() + s, packed.begin() + s + take);
-  }
-  out.kstart.push_back((uint32_t)out.pos.size());
-}
-
-// ---------------------------------------------------------------------------
 // host copies of the reference's scalar math (for LUTs)
 // ---------------------------------------------------------------------------
 static const double H_LN_10 = 2.302585092994045684017991454684364208;
@@ -373,17 +283,19 @@ static void ctx_build(pbw_ctx* c) {
     c->d_name_class.upload(nc.data(), nc.size(), s);
     HIP_CHECK(hipStreamSynchronize(s));
   }
-  // index
-  HostIndex hi;
-  build_index(g, hi);
-  c->total_entries = hi.pos.size();
-  if (hi.pos.size() >= (1ull << 32)) throw PbwError{PBW_ERR_LIMIT, "index larger than 2^32 entries"};
-  c->d_kpos.upload(hi.pos.data(), hi.pos.size(), s);
-  DevBuf<uint32_t> dkeys, dstart;
-  dkeys.upload(hi.keys.data(), hi.keys.size(), s);
-  dstart.upload(hi.kstart.data(), hi.kstart.size(), s);
+  // index: HashTableBuilder::build (hashtable.rs:248-322) on the device
   c->d_koff.ensure(CODE_SPACE + 1);
-  launch_fill_koff(c->d_koff.p, dkeys.p, dstart.p, hi.keys.size(), (uint32_t)hi.pos.size(), s);
+  {
+    uint64_t* kpos = nullptr;
+    uint64_t nent = 0;
+    std::string err;
+    if (gpu_build_index(c->dev_genome(), g->lengths.data(), c->d_koff.p, &kpos, &nent, s, err) != hipSuccess)
+      throw PbwError{err.find("2^32") != std::string::npos ? PBW_ERR_LIMIT : PBW_ERR_HIP, err};
+    c->d_kpos.release();
+    c->d_kpos.p = kpos;
+    c->d_kpos.n = std::max<uint64_t>(nent, 1);
+    c->total_entries = nent;
+  }
   c->d_pres.ensure(3 * CODE_SPACE / 64);  // natural, rotated by 10, rotated by 20
   launch_fill_pres(c->d_koff.p, c->d_pres.p, 0, s);
   launch_fill_pres(c->d_koff.p, c->d_pres.p + CODE_SPACE / 64, 10, s);

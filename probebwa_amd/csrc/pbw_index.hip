@@ -1,0 +1,240 @@
+// Bucket table built on the device: HashTableBuilder::build
+// (src/index/hashtable.rs:248-322) with k = 15, stride 5 and the cap of 300
+// entries per 15-mer applied in scan order.
+//
+// Scan order is the reference's: contigs ascending; within a contig the
+// forward windows i = 0, 5, 10, ... (hashtable.rs:266-275), then the windows
+// i = 0, 5, ... of the reverse-complement string, stored at position
+// len - i - 15 on the reverse strand (hashtable.rs:279-291).  Windows holding
+// a non-ACGT base are not indexed (is_acgt_only, hashtable.rs:147-149).
+//
+// Steps, all on the library's stream:
+//   1. one thread per scan slot j (global scan index) writes (code, j); an
+//      N window writes the sentinel code 2^30 so that it sorts last;
+//   2. a stable LSD radix sort of (code, j) by the low 31 bits of the code:
+//      within a code, entries stay in scan order (the reference's stable
+//      sort_by_key, hashtable.rs:297, over hashes; mix() is a bijection so
+//      code order and hash order group identically);
+//   3. dense per-code run starts and capped run lengths (min(run, 300): the
+//      reference stops pushing a k-mer's positions once 300 are kept,
+//      hashtable.rs:270-273, 286-289 — the same first 300 of the scan);
+//   4. an exclusive scan of the capped lengths over all 4^15 codes gives
+//      koff[c] (bucket c = [koff[c], koff[c+1]));
+//   5. the first 300 entries of each run are decoded from j and placed.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <string>
+#include <vector>
+
+#include "pbw_internal.h"
+
+namespace pbw {
+
+namespace {
+
+constexpr uint32_t IDX_SENTINEL = 1u << 30;
+
+// Contig of global scan slot j: the last c with cbase[c] <= j.
+__device__ __forceinline__ int32_t slot_contig(const uint64_t* cbase, int32_t nctg, uint64_t j) {
+  int32_t lo = 0, hi = nctg - 1;
+  while (lo < hi) {
+    int32_t mid = (lo + hi + 1) >> 1;
+    if (cbase[mid] <= j) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Window position and strand of slot j of contig c (length L >= 15).
+__device__ __forceinline__ void slot_window(uint64_t t, uint64_t L, uint64_t& pos, uint32_t& strand) {
+  const uint64_t nf = (L - K) / STRIDE + 1;  // windows i = 0, 5, ..., <= L - 15 on each strand
+  if (t < nf) {
+    pos = t * STRIDE;
+    strand = 0;
+  } else {
+    pos = L - (t - nf) * STRIDE - K;  // hashtable.rs:284 (len - i - k)
+    strand = 1;
+  }
+}
+
+__global__ void k_idx_windows(const uint64_t* __restrict__ g2b, const uint64_t* __restrict__ nmask,
+                              const uint64_t* __restrict__ coff, const uint64_t* __restrict__ clen,
+                              const uint64_t* __restrict__ cbase, int32_t nctg, uint64_t n_total,
+                              uint32_t* __restrict__ key, uint32_t* __restrict__ val) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n_total; j += stride) {
+    const int32_t c = slot_contig(cbase, nctg, j);
+    uint64_t pos;
+    uint32_t strand;
+    slot_window(j - cbase[c], clen[c], pos, strand);
+    const uint64_t gb = coff[c] + pos;
+    // 15 bases, LSB first: base k of the window at bits 2k..2k+1
+    const uint64_t w = gb >> 5;
+    const uint32_t sh = (uint32_t)(gb & 31) * 2;
+    uint64_t bits = g2b[w] >> sh;
+    if (sh) bits |= g2b[w + 1] << (64 - sh);
+    const uint32_t lsb = (uint32_t)bits & 0x3FFFFFFFu;
+    const uint64_t nw = gb >> 6;
+    const uint32_t nsh = (uint32_t)(gb & 63);
+    uint64_t nb = nmask[nw] >> nsh;
+    if (nsh > 64 - K) nb |= nmask[nw + 1] << (64 - nsh);
+    uint32_t code;
+    if (nb & ((1ull << K) - 1)) {
+      code = IDX_SENTINEL;
+    } else if (strand == 0) {
+      // MSB-first packing (hash_kmer, hashtable.rs:107-113): reverse the order of the 2-bit fields
+      uint32_t r = __builtin_bitreverse32(lsb) >> 2;           // fields reversed, bits swapped within each
+      code = ((r & 0x15555555u) << 1) | ((r >> 1) & 0x15555555u);
+    } else {
+      // reverse complement, MSB first: its base k is the complement of window base 14 - k,
+      // which lands at bits 2(14 - k) -- the LSB-first field of that base, complemented
+      code = ~lsb & 0x3FFFFFFFu;
+    }
+    key[j] = code;
+    val[j] = (uint32_t)j;
+  }
+}
+
+// Run starts of the sorted codes (N windows sort last under the sentinel).
+__global__ void k_idx_starts(const uint32_t* __restrict__ key, uint64_t n, uint32_t* __restrict__ dstart) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t k = key[i];
+    if (k != IDX_SENTINEL && (i == 0 || key[i - 1] != k)) dstart[k] = (uint32_t)i;
+  }
+}
+
+// Capped run length at each run's last entry (dstart complete: previous launch).
+__global__ void k_idx_counts(const uint32_t* __restrict__ key, uint64_t n, const uint32_t* __restrict__ dstart,
+                             uint32_t* __restrict__ cnt) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t k = key[i];
+    if (k == IDX_SENTINEL) continue;
+    if (i + 1 == n || key[i + 1] != k) {
+      const uint64_t run = i + 1 - dstart[k];
+      cnt[k] = (uint32_t)(run < (uint64_t)MAX_COUNT ? run : (uint64_t)MAX_COUNT);
+    }
+  }
+}
+
+// The first MAX_COUNT entries of every run, decoded to packed positions
+// (pack_position, hashtable.rs:157-164: bit 63 strand, 62..40 contig, 39..0 position).
+__global__ void k_idx_place(const uint32_t* __restrict__ key, const uint32_t* __restrict__ val, uint64_t n,
+                            const uint32_t* __restrict__ dstart, const uint32_t* __restrict__ koff,
+                            const uint64_t* __restrict__ clen, const uint64_t* __restrict__ cbase, int32_t nctg,
+                            uint64_t* __restrict__ kpos) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t k = key[i];
+    if (k == IDX_SENTINEL) continue;
+    const uint64_t r = i - dstart[k];
+    if (r >= (uint64_t)MAX_COUNT) continue;
+    const uint64_t j = val[i];
+    const int32_t c = slot_contig(cbase, nctg, j);
+    uint64_t pos;
+    uint32_t strand;
+    slot_window(j - cbase[c], clen[c], pos, strand);
+    kpos[(uint64_t)koff[k] + r] = ((uint64_t)strand << 63) | ((uint64_t)c << 40) | pos;
+  }
+}
+
+struct Tmp {
+  void* p = nullptr;
+  ~Tmp() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+}  // namespace
+
+#define IDX_CHECK(x)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (x);                                                               \
+    if (e_ != hipSuccess) {                                                            \
+      err = std::string(#x " failed: ") + hipGetErrorString(e_);                       \
+      return e_;                                                                       \
+    }                                                                                  \
+  } while (0)
+
+hipError_t gpu_build_index(const DevGenome& G, const uint64_t* h_clen, uint32_t* koff, uint64_t** kpos_out,
+                           uint64_t* n_entries, hipStream_t s, std::string& err) {
+  *kpos_out = nullptr;
+  *n_entries = 0;
+  const int32_t nctg = G.ncontigs;
+  std::vector<uint64_t> cbase(nctg + 1, 0);
+  for (int32_t c = 0; c < nctg; c++) {
+    const uint64_t L = h_clen[c];
+    const uint64_t per = L >= (uint64_t)K ? 2 * ((L - K) / STRIDE + 1) : 0;
+    cbase[c + 1] = cbase[c] + per;
+  }
+  const uint64_t n = cbase[nctg];
+  if (n >= (1ull << 32)) {
+    err = "more than 2^32 indexed windows";
+    return hipErrorInvalidValue;
+  }
+  // the code space and koff are always complete, even for an empty index
+  IDX_CHECK(hipMemsetAsync(koff, 0, (CODE_SPACE + 1) * sizeof(uint32_t), s));
+  if (n == 0) {
+    IDX_CHECK(hipMalloc((void**)kpos_out, sizeof(uint64_t)));
+    return hipSuccess;
+  }
+  Tmp t_cbase, t_k0, t_k1, t_v0, t_v1, t_start, t_cnt, t_sort, t_scan;
+  IDX_CHECK(hipMalloc(&t_cbase.p, (nctg + 1) * sizeof(uint64_t)));
+  IDX_CHECK(hipMemcpyAsync(t_cbase.p, cbase.data(), (nctg + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+  IDX_CHECK(hipMalloc(&t_k0.p, n * sizeof(uint32_t)));
+  IDX_CHECK(hipMalloc(&t_k1.p, n * sizeof(uint32_t)));
+  IDX_CHECK(hipMalloc(&t_v0.p, n * sizeof(uint32_t)));
+  IDX_CHECK(hipMalloc(&t_v1.p, n * sizeof(uint32_t)));
+  const uint64_t* dcb = (const uint64_t*)t_cbase.p;
+  uint32_t *k0 = (uint32_t*)t_k0.p, *k1 = (uint32_t*)t_k1.p, *v0 = (uint32_t*)t_v0.p, *v1 = (uint32_t*)t_v1.p;
+  const dim3 grid(16384), block(256);
+  hipLaunchKernelGGL(k_idx_windows, grid, block, 0, s, G.g2b, G.nmask, G.coff, G.clen, dcb, nctg, n, k0, v0);
+  IDX_CHECK(hipGetLastError());
+
+  size_t sort_bytes = 0;
+  IDX_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, k0, k1, v0, v1, n, 0, 31, s));
+  IDX_CHECK(hipMalloc(&t_sort.p, sort_bytes));
+  IDX_CHECK(hipcub::DeviceRadixSort::SortPairs(t_sort.p, sort_bytes, k0, k1, v0, v1, n, 0, 31, s));
+  IDX_CHECK(hipFree(t_sort.p));
+  t_sort.p = nullptr;
+  IDX_CHECK(hipFree(t_k0.p));
+  t_k0.p = nullptr;
+  IDX_CHECK(hipFree(t_v0.p));
+  t_v0.p = nullptr;
+
+  IDX_CHECK(hipMalloc(&t_start.p, CODE_SPACE * sizeof(uint32_t)));
+  IDX_CHECK(hipMalloc(&t_cnt.p, (CODE_SPACE + 1) * sizeof(uint32_t)));
+  uint32_t *dstart = (uint32_t*)t_start.p, *cnt = (uint32_t*)t_cnt.p;
+  IDX_CHECK(hipMemsetAsync(cnt, 0, (CODE_SPACE + 1) * sizeof(uint32_t), s));
+  hipLaunchKernelGGL(k_idx_starts, grid, block, 0, s, k1, n, dstart);
+  hipLaunchKernelGGL(k_idx_counts, grid, block, 0, s, k1, n, dstart, cnt);
+  IDX_CHECK(hipGetLastError());
+  size_t scan_bytes = 0;
+  IDX_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, cnt, koff, CODE_SPACE + 1, s));
+  IDX_CHECK(hipMalloc(&t_scan.p, scan_bytes));
+  IDX_CHECK(hipcub::DeviceScan::ExclusiveSum(t_scan.p, scan_bytes, cnt, koff, CODE_SPACE + 1, s));
+  uint32_t total = 0;
+  IDX_CHECK(hipMemcpyAsync(&total, koff + CODE_SPACE, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  IDX_CHECK(hipStreamSynchronize(s));
+  IDX_CHECK(hipFree(t_scan.p));
+  t_scan.p = nullptr;
+  IDX_CHECK(hipFree(t_cnt.p));
+  t_cnt.p = nullptr;
+  uint64_t* kpos = nullptr;
+  IDX_CHECK(hipMalloc((void**)&kpos, std::max<uint64_t>(total, 1) * sizeof(uint64_t)));
+  hipLaunchKernelGGL(k_idx_place, grid, block, 0, s, k1, v1, n, dstart, koff, G.clen, dcb, nctg, kpos);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    (void)hipFree(kpos);
+    err = std::string("index placement failed: ") + hipGetErrorString(e);
+    return e;
+  }
+  *kpos_out = kpos;
+  *n_entries = total;
+  return hipSuccess;
+}
+
+}  // namespace pbw

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 namespace pbw {
 
 // ---- constants of the reference algorithm ----
@@ -312,8 +314,10 @@ uint32_t seed_overflow_blocks(uint32_t B);  // 8 words of scratch each
 void launch_seed_overflow(const uint32_t* nhits, uint32_t B, uint32_t cap1, uint64_t base, uint32_t* part,
                           uint32_t* list, uint64_t* off, uint32_t* cap, uint32_t* st, hipStream_t s);
 void launch_scatter_u64(uint64_t* out, const uint32_t* idx, const uint64_t* val, uint32_t n, hipStream_t s);
-void launch_fill_koff(uint32_t* koff, const uint32_t* keys, const uint32_t* kstart, uint64_t nkeys, uint32_t total,
-                      hipStream_t s);
+// Bucket table on the device (pbw_index.hip): fills koff[0 .. 2^30] and allocates
+// *kpos (hipMalloc, owned by the caller) with *n_entries packed positions.
+hipError_t gpu_build_index(const DevGenome& G, const uint64_t* h_clen, uint32_t* koff, uint64_t** kpos,
+                           uint64_t* n_entries, hipStream_t s, std::string& err);
 
 __device__ __forceinline__ uint16_t float_to_bf16(float f) {  // round to nearest even
   uint32_t u = __float_as_uint(f);

@@ -4357,20 +4357,6 @@ void gather_cigar2(const uint32_t* cig0, const uint32_t* cig1, const uint64_t* s
 // ---------------------------------------------------------------------------
 // dense bucket table from the sorted key list
 // ---------------------------------------------------------------------------
-__global__ void k_fill_koff(uint32_t* koff, const uint32_t* keys, const uint32_t* kstart, uint64_t nkeys,
-                            uint32_t total) {
-  uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (; c <= CODE_SPACE; c += stride) {
-    uint64_t lo = 0, hi = nkeys;  // first key >= c
-    while (lo < hi) {
-      uint64_t mid = (lo + hi) >> 1;
-      if ((uint64_t)keys[mid] < c) lo = mid + 1;
-      else hi = mid;
-    }
-    koff[c] = lo < nkeys ? kstart[lo] : total;
-  }
-}
 // Presence bitmap of the bucket table, bit i = bucket of code rotl30(i, rot)
 // non-empty: one wave per 64 consecutive bits.
 __global__ void k_fill_pres(const uint32_t* koff, uint64_t* pres, uint32_t rot) {
@@ -4535,9 +4521,5 @@ void launch_fill_pres(const uint32_t* koff, uint64_t* pres, uint32_t rot, hipStr
   hipLaunchKernelGGL(k_fill_pres, dim3(65536), dim3(256), 0, s, koff, pres, rot);
 }
 
-void launch_fill_koff(uint32_t* koff, const uint32_t* keys, const uint32_t* kstart, uint64_t nkeys, uint32_t total,
-                      hipStream_t s) {
-  hipLaunchKernelGGL(k_fill_koff, dim3(65536), dim3(256), 0, s, koff, keys, kstart, nkeys, total);
-}
 
 }  // namespace pbw

@@ -150,6 +150,53 @@ def test_index_buckets_match_oracle():
     assert checked > 2000
 
 
+def _bucket_code(contigs, packed):
+    """Code of the 15-mer a packed position (hashtable.rs:157-164) indexes."""
+    packed = int(packed)
+    cid, pos = (packed >> 40) & ((1 << 23) - 1), packed & ((1 << 40) - 1)
+    kmer = np.asarray(contigs[cid][1][pos:pos + 15], np.uint8)
+    if packed >> 63:
+        kmer = synth.revcomp(kmer)
+    return P.code_of(kmer.tobytes())
+
+
+def test_index_full_table_matches_oracle():
+    """The device-built bucket table equals HashTableBuilder::build (hashtable.rs:248-322) as a whole:
+    every oracle bucket equals the device bucket of its code, in scan order, and the totals agree,
+    so the device holds no other entry.  The genome hits the build's edge cases: contigs shorter
+    than, equal to and just over k (no window, one window per strand, one window), lengths at every
+    residue mod 5, N runs, and a poly-A run whose 15-mer passes the 300 cap across strands and
+    contigs (the cap counts in scan order: contig, forward windows, then reverse windows)."""
+    rng = np.random.default_rng(np.random.PCG64(11))
+    contigs = []
+    for i, ln in enumerate([0, 7, 14, 15, 16, 19, 20, 21, 22, 23, 24, 25, 40, 1234]):
+        contigs.append(("t%d" % i, synth.random_bases(rng, ln)))
+    big = synth.random_bases(rng, 20000)
+    big[3000:3900] = ord("A")   # 900 A's: 178 forward AAA.. windows, 178 reverse TTT.. windows
+    big[9000:9040] = ord("N")
+    big[12001:12003] = ord("N")
+    contigs.append(("big1", big))
+    big2 = synth.random_bases(rng, 8003)
+    big2[100:900] = ord("T")    # the T-run's reverse windows add AAA.. entries past the cap
+    big2[5000] = ord("N")
+    contigs.append(("big2", big2))
+    contigs += synth.small_genome(5, n_contigs=2, length=15001)
+    go, gp = pair_of_genomes(contigs)
+    ht = O.HashTable(go)
+    ctx = P.Context(gp)
+    assert ctx.total_entries == ht.total_entries
+    n_keys, n_entries, capped = 0, 0, 0
+    for _h, bucket in ht.buckets():
+        code = _bucket_code(contigs, bucket[0])
+        dev = ctx.lookup_code(code)
+        assert np.array_equal(dev, bucket), "bucket of code %d differs" % code
+        n_keys += 1
+        n_entries += len(bucket)
+        capped += len(bucket) == 300
+    assert n_entries == ctx.total_entries
+    assert capped >= 2 and n_keys > 10000
+
+
 def random_align_jobs(rng, n, wide=False):
     jobs = []
     for _ in range(n):
