@@ -131,6 +131,8 @@ typedef struct {
   uint64_t pf_audit_reads;      /* reads re-run plain as the audit sample */
   uint64_t pf_audit_mismatches; /* audited reads whose SAM fields differ from the plain run */
   double ms_pf;                 /* host wall time of scoring and selection */
+  double ms_k_cluster;          /* HIP-event time of the clustering kernels (every tier) */
+  uint64_t n_k_cluster;         /* clustering stages timed */
 } pbw_stats;
 
 /* ---- genome: GenomeIndexBuilder (src/index/genome.rs:101-147) ---- */

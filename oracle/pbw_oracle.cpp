@@ -198,44 +198,34 @@ static inline void unpack_position(uint64_t p, uint64_t& cid, uint64_t& pos, Str
 }
 
 // Open-addressing u32 -> (u32,u32) map standing in for FxHashMap<u32,(u32,u32)>
-// (hashtable.rs:62).  Only lookup results matter on the path, not its layout.
+// (hashtable.rs:62).  Only lookup results matter on the path, not its layout;
+// a slot holds the key and both values (b = 0 marks an empty slot: every
+// stored count is at least 1), so a probe reads one line.
 struct U32Map {
-  std::vector<uint32_t> keys;
-  std::vector<uint32_t> a, b;
-  std::vector<uint8_t> used;
-  uint64_t mask = 0, count = 0;
+  struct Slot {
+    uint32_t key, a, b;
+  };
+  std::vector<Slot> s;
+  uint64_t mask = 0;
   void init(uint64_t expected) {
     uint64_t cap = 16;
-    while (cap < expected * 2 + 16) cap <<= 1;
-    keys.assign(cap, 0);
-    a.assign(cap, 0);
-    b.assign(cap, 0);
-    used.assign(cap, 0);
+    while (cap < expected + expected / 2 + 16) cap <<= 1;
+    s.assign(cap, Slot{0, 0, 0});
     mask = cap - 1;
-    count = 0;
   }
   static inline uint64_t slot_of(uint32_t k) { return (uint64_t)k * 0x9E3779B97F4A7C15ull >> 20; }
-  // returns index of key (inserting it if absent)
-  uint64_t find_or_insert(uint32_t k, bool& inserted) {
+  // slot of a key not yet present (b is set by the caller)
+  uint64_t insert_new(uint32_t k) {
     uint64_t i = slot_of(k) & mask;
-    while (used[i]) {
-      if (keys[i] == k) {
-        inserted = false;
-        return i;
-      }
-      i = (i + 1) & mask;
-    }
-    used[i] = 1;
-    keys[i] = k;
-    count++;
-    inserted = true;
+    while (s[i].b) i = (i + 1) & mask;
+    s[i].key = k;
     return i;
   }
   bool find(uint32_t k, uint64_t& idx) const {
-    if (keys.empty()) return false;
+    if (s.empty()) return false;
     uint64_t i = slot_of(k) & mask;
-    while (used[i]) {
-      if (keys[i] == k) {
+    while (s[i].b) {
+      if (s[i].key == k) {
         idx = i;
         return true;
       }
@@ -245,8 +235,37 @@ struct U32Map {
   }
 };
 
+// The build's per-k-mer counter (`seen`, hashtable.rs:253): u32 -> u32 count.
+struct CountMap {
+  struct Slot {
+    uint32_t key, count;  // count 0: empty
+  };
+  std::vector<Slot> s;
+  uint64_t mask = 0;
+  void init(uint64_t expected) {
+    uint64_t cap = 16;
+    while (cap < expected + expected / 2 + 16) cap <<= 1;
+    s.assign(cap, Slot{0, 0});
+    mask = cap - 1;
+  }
+  // the count of k, inserting k (count 0) if absent
+  uint32_t& at(uint32_t k) {
+    uint64_t i = U32Map::slot_of(k) & mask;
+    while (s[i].count) {
+      if (s[i].key == k) return s[i].count;
+      i = (i + 1) & mask;
+    }
+    s[i].key = k;
+    return s[i].count;  // 0 until the caller counts the window
+  }
+};
+
+// The table is held as 2^pbits sub-maps, one per range of hash values (the top
+// pbits bits); `positions` holds every bucket in ascending hash order, so the
+// whole is one FxHashMap<u32,(u32,u32)> over one positions vector.
 struct HashTable {
-  U32Map table;                     // hash -> (offset, count)
+  std::vector<U32Map> tables;       // hash -> (offset, count), by hash range
+  uint32_t pbits = 0;
   std::vector<uint64_t> positions;  // flat store, buckets in ascending hash order
   std::vector<uint32_t> sorted_keys;
   std::vector<uint32_t> key_off;
@@ -256,10 +275,11 @@ struct HashTable {
   // hashtable.rs:90-102 lookup/get
   inline const uint64_t* lookup(const uint8_t* kmer, size_t len, uint64_t& count) const {
     uint32_t h = hash_kmer(kmer, len);
+    const U32Map& t = tables[pbits ? (h >> (32 - pbits)) : 0];
     uint64_t idx;
-    if (table.find(h, idx)) {
-      count = table.b[idx];
-      return positions.data() + table.a[idx];
+    if (t.find(h, idx)) {
+      count = t.s[idx].b;
+      return positions.data() + t.s[idx].a;
     }
     count = 0;
     return nullptr;
@@ -289,76 +309,159 @@ static void stable_sort_by_hash(std::vector<uint32_t>& h, std::vector<uint64_t>&
 
 // Progress lines of long index builds (genome-scale references), on stderr.
 static int g_progress = 0;
+static unsigned g_index_threads = 1;
 
-// hashtable.rs:248-322 HashTableBuilder::build
-static HashTable* hash_build(const Genome& g, size_t k, size_t stride, size_t max_count) {
-  std::vector<uint32_t> ph;
-  std::vector<uint64_t> pp;
-  U32Map seen;
-  uint64_t est = 0;
-  for (auto& c : g.contigs) est += 2 * (c.length / std::max<size_t>(stride, 1) + 1);
-  seen.init(std::max<uint64_t>(est, 16));
-  std::vector<uint8_t> seq;
+// hashtable.rs:248-322 HashTableBuilder::build.
+//
+// The reference scans contigs in order, forward windows i = 0, stride, ... then
+// the windows of the reverse complement, keeping a k-mer's first max_count
+// windows in that scan order, then stably sorts (hash, packed) by hash and
+// records one (offset, count) per hash run.  Here the scan is cut into units
+// that run on `threads` threads, each unit sorting its windows into 2^pbits
+// lists by hash range (the top pbits bits); range p then concatenates its lists
+// in unit order -- the scan order -- and applies the cap, the stable sort and
+// the run table on its own.  A k-mer's windows all fall into one range and stay
+// in scan order, so every bucket, and the ascending-hash concatenation of the
+// ranges, is the reference's.
+static HashTable* hash_build(const Genome& g, size_t k, size_t stride, size_t max_count, unsigned threads) {
+  threads = std::max(1u, threads);
+  uint64_t windows = 0;
+  for (auto& c : g.contigs) windows += 2 * (c.length / std::max<size_t>(stride, 1) + 1);
+  // at least one range per thread, and ranges of at most ~2^25 windows (fewer large
+  // transients alive at once); the table does not depend on the count
+  uint32_t pbits = 0;
+  while (((1u << pbits) < threads || (windows >> pbits) > (1ull << 25)) && pbits < 8) pbits++;
+  const uint32_t P = 1u << pbits;
+  auto range_of = [&](uint32_t h) { return pbits ? (h >> (32 - pbits)) : 0u; };
+  const size_t UNIT = 1u << 22;  // windows per unit
+  // unit lists: per unit, per range, (hash, packed) in scan order
+  struct Unit {
+    std::vector<std::vector<uint32_t>> h;
+    std::vector<std::vector<uint64_t>> p;
+  };
+  std::vector<Unit> units;
+  std::vector<uint8_t> seq, rc;
+  stride = std::max<size_t>(stride, 1);
   for (size_t cid = 0; cid < g.contigs.size(); cid++) {
     const Contig& c = g.contigs[cid];
     if (g_progress) fprintf(stderr, "[oracle] index: scanning contig %zu/%zu\n", cid + 1, g.contigs.size());
     contig_slice(c, 0, c.length, seq);
     if (seq.size() < k) continue;
-    // forward strand: hashtable.rs:266-275
-    for (size_t i = 0; i <= seq.size() - k; i += stride) {
-      const uint8_t* km = seq.data() + i;
-      if (!is_acgt_only(km, k)) continue;
-      uint32_t h = hash_kmer(km, k);
-      bool ins;
-      uint64_t si = seen.find_or_insert(h, ins);
-      if (seen.a[si] < max_count) {
-        ph.push_back(h);
-        pp.push_back(pack_position(cid, i, Forward));
-        seen.a[si]++;
+    rc = reverse_complement(seq.data(), seq.size());  // hashtable.rs:279
+    const size_t nw = (seq.size() - k) / stride + 1;  // windows i = 0, stride, ... <= len - k, per strand
+    // units of this contig: forward windows [w0, w1), then reverse windows [w0, w1)
+    std::vector<std::pair<int, std::pair<size_t, size_t>>> todo;
+    for (int strand = 0; strand < 2; strand++)
+      for (size_t w0 = 0; w0 < nw; w0 += UNIT) todo.push_back({strand, {w0, std::min(nw, w0 + UNIT)}});
+    const size_t u0 = units.size();
+    units.resize(u0 + todo.size());
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+      for (size_t t; (t = next.fetch_add(1)) < todo.size();) {
+        Unit& U = units[u0 + t];
+        U.h.assign(P, {});
+        U.p.assign(P, {});
+        const bool fwd = todo[t].first == 0;
+        const uint8_t* src = fwd ? seq.data() : rc.data();
+        for (size_t w = todo[t].second.first; w < todo[t].second.second; w++) {
+          const size_t i = w * stride;
+          const uint8_t* km = src + i;
+          if (!is_acgt_only(km, k)) continue;  // hashtable.rs:268, 282
+          const uint32_t h = hash_kmer(km, k);
+          const uint32_t r = range_of(h);
+          U.h[r].push_back(h);
+          // hashtable.rs:272 (forward), :284-288 (reverse: position len - i - k)
+          U.p[r].push_back(fwd ? pack_position(cid, i, Forward) : pack_position(cid, seq.size() - i - k, Reverse));
+        }
       }
-    }
-    // reverse-complement strand: hashtable.rs:279-291
-    std::vector<uint8_t> rc = reverse_complement(seq.data(), seq.size());
-    for (size_t i = 0; i <= rc.size() - k; i += stride) {
-      const uint8_t* km = rc.data() + i;
-      if (!is_acgt_only(km, k)) continue;
-      uint32_t h = hash_kmer(km, k);
-      bool ins;
-      uint64_t si = seen.find_or_insert(h, ins);
-      if (seen.a[si] < max_count) {
-        ph.push_back(h);
-        pp.push_back(pack_position(cid, seq.size() - i - k, Reverse));
-        seen.a[si]++;
-      }
-    }
+    };
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < std::min<size_t>(threads, todo.size()); t++) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
   }
-  seen = U32Map();
-  if (g_progress) fprintf(stderr, "[oracle] index: sorting %zu entries\n", ph.size());
-  stable_sort_by_hash(ph, pp);
-  if (g_progress) fprintf(stderr, "[oracle] index: building buckets\n");
+  if (g_progress) fprintf(stderr, "[oracle] index: capping, sorting and tabling %u hash ranges\n", P);
   HashTable* ht = new HashTable();
   ht->k = k;
   ht->stride = stride;
-  ht->positions = std::move(pp);
-  // hashtable.rs:303-313: one (offset, count) per hash run
-  uint64_t nkeys = 0;
-  for (size_t i = 0; i < ph.size(); i++)
-    if (i == 0 || ph[i] != ph[i - 1]) nkeys++;
-  ht->table.init(nkeys);
-  size_t i = 0;
-  while (i < ph.size()) {
-    uint32_t h = ph[i];
-    size_t start = i;
-    while (i < ph.size() && ph[i] == h) i++;
-    bool ins;
-    uint64_t si = ht->table.find_or_insert(h, ins);
-    ht->table.a[si] = (uint32_t)start;
-    ht->table.b[si] = (uint32_t)(i - start);
-    ht->sorted_keys.push_back(h);
-    ht->key_off.push_back((uint32_t)start);
-    ht->total_entries += i - start;
+  ht->pbits = pbits;
+  ht->tables.resize(P);
+  std::vector<std::vector<uint32_t>> rh(P), rkeys(P), roff(P);
+  std::vector<std::vector<uint64_t>> rp(P);
+  std::atomic<uint32_t> next{0};
+  auto range_work = [&]() {
+    for (uint32_t r; (r = next.fetch_add(1)) < P;) {
+      std::vector<uint32_t>& ph = rh[r];
+      std::vector<uint64_t>& pp = rp[r];
+      uint64_t n = 0;
+      for (auto& U : units) n += U.h[r].size();
+      CountMap seen;
+      seen.init(std::max<uint64_t>(n, 16));
+      ph.reserve(n);
+      pp.reserve(n);
+      for (auto& U : units) {  // scan order
+        for (size_t e = 0; e < U.h[r].size(); e++) {
+          uint32_t& cnt = seen.at(U.h[r][e]);
+          if (cnt < max_count) {  // hashtable.rs:270-273, 286-289
+            ph.push_back(U.h[r][e]);
+            pp.push_back(U.p[r][e]);
+            cnt++;
+          }
+        }
+        std::vector<uint32_t>().swap(U.h[r]);
+        std::vector<uint64_t>().swap(U.p[r]);
+      }
+      seen = CountMap();
+      stable_sort_by_hash(ph, pp);
+      // hashtable.rs:303-313: one (offset, count) per hash run; offsets within the range
+      uint64_t nkeys = 0;
+      for (size_t i = 0; i < ph.size(); i++)
+        if (i == 0 || ph[i] != ph[i - 1]) nkeys++;
+      U32Map& tab = ht->tables[r];
+      tab.init(nkeys);
+      size_t i = 0;
+      while (i < ph.size()) {
+        uint32_t h = ph[i];
+        size_t start = i;
+        while (i < ph.size() && ph[i] == h) i++;
+        uint64_t si = tab.insert_new(h);
+        tab.s[si].a = (uint32_t)start;
+        tab.s[si].b = (uint32_t)(i - start);
+        rkeys[r].push_back(h);
+        roff[r].push_back((uint32_t)start);
+      }
+      std::vector<uint32_t>().swap(ph);
+    }
+  };
+  {
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < std::min<unsigned>(threads, P); t++) th.emplace_back(range_work);
+    range_work();
+    for (auto& t : th) t.join();
   }
-  ht->key_off.push_back((uint32_t)ph.size());
+  units.clear();
+  // ranges in ascending hash order: shift each range's offsets by the entries before it
+  uint64_t total = 0;
+  for (uint32_t r = 0; r < P; r++) total += rp[r].size();
+  if (total >= (1ull << 32)) {
+    delete ht;
+    return nullptr;
+  }
+  ht->positions.resize(total);
+  uint64_t base = 0;
+  for (uint32_t r = 0; r < P; r++) {
+    std::copy(rp[r].begin(), rp[r].end(), ht->positions.begin() + base);
+    for (auto& sl : ht->tables[r].s)
+      if (sl.b) sl.a += (uint32_t)base;
+    for (size_t q = 0; q < rkeys[r].size(); q++) {
+      ht->sorted_keys.push_back(rkeys[r][q]);
+      ht->key_off.push_back((uint32_t)(roff[r][q] + base));
+    }
+    base += rp[r].size();
+    std::vector<uint64_t>().swap(rp[r]);
+  }
+  ht->key_off.push_back((uint32_t)total);
+  ht->total_entries = total;
   return ht;
 }
 
@@ -1754,8 +1857,9 @@ void orc_unpack_position(uint64_t packed, uint64_t* c, uint64_t* p, int* s) {
 }
 int orc_is_acgt_only(const uint8_t* k, uint64_t len) { return is_acgt_only(k, len) ? 1 : 0; }
 void orc_set_progress(int on) { g_progress = on; }
+void orc_set_index_threads(int n) { g_index_threads = n > 0 ? (unsigned)n : 1u; }
 void* orc_hash_build(const void* g, int k, int stride, int max_count) {
-  return hash_build(*(const Genome*)g, (size_t)k, (size_t)stride, (size_t)max_count);
+  return hash_build(*(const Genome*)g, (size_t)k, (size_t)stride, (size_t)max_count, g_index_threads);
 }
 void orc_hash_free(void* h) { delete (HashTable*)h; }
 uint64_t orc_hash_lookup(const void* h, const uint8_t* kmer, uint64_t len, const uint64_t** out) {

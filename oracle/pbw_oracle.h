@@ -95,6 +95,8 @@ uint32_t orc_hash_kmer(const uint8_t* kmer, uint64_t len);
 uint64_t orc_pack_position(uint64_t contig, uint64_t pos, int strand);
 void orc_unpack_position(uint64_t packed, uint64_t* contig, uint64_t* pos, int* strand);
 int orc_is_acgt_only(const uint8_t* kmer, uint64_t len);
+/* threads used by the next orc_hash_build calls (default 1); the table is the same for any count */
+void orc_set_index_threads(int n);
 void* orc_hash_build(const void* g, int k, int stride, int max_count);
 void orc_hash_free(void* h);
 /* returns bucket size; *out points into the table's position store */

@@ -145,6 +145,7 @@ def lib():
             "orc_records_release": (None, [C.POINTER(_Records)]),
             "orc_set_group_order": (None, [i32]),
             "orc_set_progress": (None, [i32]),
+            "orc_set_index_threads": (None, [i32]),
             "orc_read_reads": (i32, [C.c_char_p, i32, C.c_uint8, C.c_char_p, C.POINTER(_Parsed)]),
             "orc_parsed_reads_free": (None, [C.POINTER(_Parsed)]),
             "orc_format_sam": (C.c_void_p, [vp, C.POINTER(_Parsed), C.POINTER(_Records), C.c_char_p, C.c_char_p,
@@ -230,11 +231,14 @@ def is_acgt_only(kmer):
 class HashTable:
     """`HashTableBuilder::new(&genome).with_k(k).with_stride(s).with_max_count(c).build()` (hashtable.rs:248)."""
 
-    def __init__(self, genome, k=15, stride=5, max_count=300, progress=False):
+    def __init__(self, genome, k=15, stride=5, max_count=300, progress=False, threads=1):
         self.genome = genome
         self.k = k
         lib().orc_set_progress(1 if progress else 0)
+        lib().orc_set_index_threads(int(threads))
         self.h = lib().orc_hash_build(genome.h, k, stride, max_count)
+        if not self.h:
+            raise RuntimeError("index larger than 2^32 entries")
 
     def __del__(self):
         if getattr(self, "h", None):

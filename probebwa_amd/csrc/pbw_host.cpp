@@ -170,7 +170,7 @@ struct pbw_ctx {
   pbw_options opt;
   int device;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[8] = {};  // [0,1] seed, [2,3] DP, [4,5] wave-tier and [6,7] block-tier clustering
   std::string error;
   pbw_stats stats{};
   // genome + index
@@ -742,11 +742,15 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
     ca.rcode = c->d_rcode.p;
     ca.feat = c->d_feat.p;
   }
+  HIP_CHECK(hipEventRecord(c->ev[4], s));
   launch_cluster(ca, s);
   HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipEventRecord(c->ev[5], s));
   uint32_t nt2 = 0;
   HIP_CHECK(hipMemcpyAsync(&nt2, c->d_ntier2.p, 4, hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
+  float ms_cl = 0.f;
+  HIP_CHECK(hipEventElapsedTime(&ms_cl, c->ev[4], c->ev[5]));
   if (nt2) {
     // one 256-thread block per read, per-block scratch sized for the largest read; up to
     // 4096 blocks within 16 GiB (repeat-rich genomes send most reads here, with 10^4 hits each)
@@ -755,11 +759,20 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
     const uint32_t blocks = (uint32_t)std::min<uint64_t>({(uint64_t)nt2, 4096, fit});
     c->d_scratch.ensure((size_t)blocks * ca.scratch_per_block);
     ca.scratch = c->d_scratch.p;
+    HIP_CHECK(hipEventRecord(c->ev[6], s));
     launch_cluster_tier2(ca, c->d_tier2.p, nt2, blocks, s);
     HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(c->ev[7], s));
     c->stats.tier2_cluster_reads += nt2;
   }
   HIP_CHECK(hipStreamSynchronize(s));
+  if (nt2) {
+    float ms2 = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms2, c->ev[6], c->ev[7]));
+    ms_cl += ms2;
+  }
+  c->stats.ms_k_cluster += ms_cl;
+  c->stats.n_k_cluster += 1;
   c->stats.ms_cluster += ms_since(t0);
   t0 = std::chrono::steady_clock::now();
   // ---- DP jobs ----

@@ -16,6 +16,10 @@ Workloads (the reference publishes no dataset for its mapping path):
                      clipped to [200,600], FR orientation, 10% of pairs with
                      one mate inside a repeat copy.
 """
+import concurrent.futures
+import multiprocessing
+import os
+
 import numpy as np
 
 BASES = np.frombuffer(b"ACGT", dtype=np.uint8)
@@ -111,45 +115,60 @@ def _mutated_copies(rng, consensus, n, dmin, dmax):
     return out
 
 
-def genome_scale_3gbp(seed=5, n_contigs=24, total=3_100_000_000):
+def _genome_scale_contig(args):
+    """Contig `ci` of genome_scale_3gbp: its own generator (child `ci` of the workload seed)
+    and the two repeat consensi (drawn from the workload seed itself)."""
+    seed, ci, L = args
+    rng0 = np.random.default_rng(np.random.PCG64(seed))
+    fam_s = random_bases(rng0, 300)
+    fam_l = random_bases(rng0, 6000)
+    rng = np.random.default_rng(np.random.PCG64(np.random.SeedSequence(seed).spawn(ci + 1)[ci]))
+    g = BASES[rng.integers(0, 4, L, dtype=np.uint8)]
+    ns = int(0.225 * L / 300)
+    nl = int(0.225 * L / 6000)
+    lens = np.concatenate([np.full(ns, 300, np.int64), np.full(nl, 6000, np.int64)])
+    kind = np.concatenate([np.zeros(ns, np.int8), np.ones(nl, np.int8)])
+    order = rng.permutation(len(lens))
+    lens, kind = lens[order], kind[order]
+    gaps = rng.exponential(1.0, len(lens) + 1)
+    gaps = np.floor(gaps / gaps.sum() * (L - lens.sum())).astype(np.int64)
+    starts = np.cumsum(gaps[:-1]) + np.concatenate([[0], np.cumsum(lens)[:-1]])
+    for k, cons, dlo, dhi in ((0, fam_s, 0.10, 0.20), (1, fam_l, 0.05, 0.15)):
+        sel = np.nonzero(kind == k)[0]
+        cp = _mutated_copies(rng, cons, len(sel), dlo, dhi)
+        idx = starts[sel][:, None] + np.arange(len(cons))[None, :]
+        g[idx] = cp
+    # segmental duplications: 5% of the contig
+    done = 0
+    while done < 0.05 * L:
+        ln = int(rng.integers(1000, 10001))
+        src = int(rng.integers(0, L - ln))
+        dst = int(rng.integers(0, L - ln))
+        seg = mutate(rng, g[src:src + ln].copy(), float(rng.uniform(0.005, 0.03)))
+        g[dst:dst + ln] = seg
+        done += ln
+    return g
+
+
+def genome_scale_3gbp(seed=5, n_contigs=24, total=3_100_000_000, workers=None):
     """Genome-scale synthetic reference: `n_contigs` contigs of equal length summing to
     `total` bases; a uniform background with 45% of the bases in interspersed repeats from
     two families (a 300 bp consensus whose copies diverge 10-20%, a 6 kb consensus at
     5-15%; half of the repeat bases each, both strands, in random order with exponential
     background gaps) and 5% in segmental duplications (copies of 1-10 kb segments of the
-    same contig at 0.5-3% divergence).  Built contig by contig in uint8."""
-    rng = np.random.default_rng(np.random.PCG64(seed))
-    fam_s = random_bases(rng, 300)
-    fam_l = random_bases(rng, 6000)
+    same contig at 0.5-3% divergence).  Contig ci draws from child ci of the seed, so the
+    contigs are built in parallel processes (spawned: safe after a GPU runtime has started)
+    and the genome does not depend on the process count."""
     L = total // n_contigs
-    contigs = []
-    for ci in range(n_contigs):
-        g = BASES[rng.integers(0, 4, L, dtype=np.uint8)]
-        ns = int(0.225 * L / 300)
-        nl = int(0.225 * L / 6000)
-        lens = np.concatenate([np.full(ns, 300, np.int64), np.full(nl, 6000, np.int64)])
-        kind = np.concatenate([np.zeros(ns, np.int8), np.ones(nl, np.int8)])
-        order = rng.permutation(len(lens))
-        lens, kind = lens[order], kind[order]
-        gaps = rng.exponential(1.0, len(lens) + 1)
-        gaps = np.floor(gaps / gaps.sum() * (L - lens.sum())).astype(np.int64)
-        starts = np.cumsum(gaps[:-1]) + np.concatenate([[0], np.cumsum(lens)[:-1]])
-        for k, cons, dlo, dhi in ((0, fam_s, 0.10, 0.20), (1, fam_l, 0.05, 0.15)):
-            sel = np.nonzero(kind == k)[0]
-            cp = _mutated_copies(rng, cons, len(sel), dlo, dhi)
-            idx = starts[sel][:, None] + np.arange(len(cons))[None, :]
-            g[idx] = cp
-        # segmental duplications: 5% of the contig
-        done = 0
-        while done < 0.05 * L:
-            ln = int(rng.integers(1000, 10001))
-            src = int(rng.integers(0, L - ln))
-            dst = int(rng.integers(0, L - ln))
-            seg = mutate(rng, g[src:src + ln].copy(), float(rng.uniform(0.005, 0.03)))
-            g[dst:dst + ln] = seg
-            done += ln
-        contigs.append(("chr%d" % (ci + 1), g))
-    return contigs
+    jobs = [(seed, ci, L) for ci in range(n_contigs)]
+    workers = workers or max(1, min(n_contigs, 16, os.cpu_count() or 1))
+    if workers == 1:
+        seqs = [_genome_scale_contig(j) for j in jobs]
+    else:
+        ctx = multiprocessing.get_context("spawn")
+        with concurrent.futures.ProcessPoolExecutor(workers, mp_context=ctx) as ex:
+            seqs = list(ex.map(_genome_scale_contig, jobs))
+    return [("chr%d" % (ci + 1), g) for ci, g in enumerate(seqs)]
 
 
 def _flat(seqs, quals):
@@ -360,6 +379,46 @@ def reads_pairs_probe(contigs, n=1_000_000, L=150, seed=3, repeat_spans=(), chun
         out[1][c0 * L:c1 * L] = q1.ravel()
         out[2][c0 * L:c1 * L] = m2.ravel()
         out[3][c0 * L:c1 * L] = q2.ravel()
+    return (out[0], so, out[1], so.copy()), (out[2], so.copy(), out[3], so.copy())
+
+
+def reads_pairs_genome(contigs, n=1_000_000, L=150, seed=58, chunk=65536):
+    """FR pairs of 2 x L over a multi-contig genome (the genome-scale pairs): contig by
+    length, fragment ~ N(350, 50) rounded and clipped to [200, 600] with a uniform start;
+    mate 1 = the fragment's first L bases, mate 2 = the reverse complement of its last L;
+    mates swapped for half of the pairs.  Each mate has the decaying quality model of
+    reads_uneven_se (Q(pos) = 38 - 16 pos/(L-1) + N(0,3), clamped to [2,41]) and
+    substitutions at the Phred-implied rate.  Returns ((seq, so, qual, qo) of mate 1, ... of mate 2)."""
+    rng = np.random.default_rng(np.random.PCG64(seed))
+    genome = np.concatenate([c[1] for c in contigs])
+    lens = np.array([len(c[1]) for c in contigs], np.int64)
+    starts = np.concatenate([[0], np.cumsum(lens)[:-1]])
+    frag = np.clip(np.rint(rng.normal(350, 50, n)), 200, 600).astype(np.int64)
+    cid = rng.choice(len(contigs), n, p=lens / lens.sum())
+    st = (rng.random(n) * (lens[cid] - frag)).astype(np.int64) + starts[cid]
+    swap = rng.random(n) < 0.5
+    so = np.arange(n + 1, dtype=np.uint64) * np.uint64(L)
+    out = [np.zeros(n * L + 1, np.uint8) for _ in range(4)]
+    col = np.arange(L)
+
+    def mate(b):
+        m = b.shape[0]
+        q = np.rint(38.0 - 16.0 * col[None, :] / (L - 1) + rng.normal(0.0, 3.0, (m, L)))
+        q = np.clip(q, 2, 41).astype(np.uint8)
+        err = rng.random((m, L)) < np.power(10.0, -q.astype(np.float64) / 10.0)
+        codes = np.searchsorted(BASES, b)
+        return np.where(err, BASES[(codes + rng.integers(1, 4, (m, L))) % 4], b), q
+
+    for c0 in range(0, n, chunk):
+        c1 = min(n, c0 + chunk)
+        s, fr = st[c0:c1], frag[c0:c1]
+        a, qa = mate(genome[s[:, None] + col[None, :]])
+        b, qb = mate(COMP[genome[(s + fr - 1)[:, None] - col[None, :]]])
+        sw = swap[c0:c1][:, None]
+        out[0][c0 * L:c1 * L] = np.where(sw, b, a).ravel()
+        out[1][c0 * L:c1 * L] = np.where(sw, qb, qa).ravel()
+        out[2][c0 * L:c1 * L] = np.where(sw, a, b).ravel()
+        out[3][c0 * L:c1 * L] = np.where(sw, qa, qb).ravel()
     return (out[0], so, out[1], so.copy()), (out[2], so.copy(), out[3], so.copy())
 
 

@@ -214,6 +214,34 @@ def test_max_count_caps_a_hyper_repetitive_kmer():
     assert len(ht.lookup(unit)) == 5
 
 
+# hashtable.rs:248-322: the build split over threads (hash ranges, scan cut into units) is
+# the single-thread scan: same buckets in the same ascending-hash order, same lookups
+# through the per-range maps, the cap still counted in scan order across contigs.
+def test_threaded_index_build_equals_serial_scan():
+    rng = np.random.default_rng(7)
+    parts = []
+    for ln in (0, 14, 15, 16, 23, 40_000, 9_000):
+        s = bytearray(b"ACGT"[i] for i in rng.integers(0, 4, ln))
+        if ln > 5000:
+            s[1000:2600] = b"A" * 1600          # forward AAA.. windows past the cap of 300
+            s[3000:3040] = b"N" * 40
+        parts.append(bytes(s))
+    g = O.Genome()
+    for i, s in enumerate(parts):
+        g.add_contig("c%d" % i, s)
+    serial = O.HashTable(g, threads=1)
+    ref = list(serial.buckets())
+    probes = [parts[5][p:p + 15] for p in range(0, 39_000, 37)] + [b"A" * 15, b"T" * 15]
+    for th in (2, 3, 8):
+        par = O.HashTable(g, threads=th)
+        got = list(par.buckets())
+        assert len(got) == len(ref) and par.total_entries == serial.total_entries
+        assert all(a[0] == b[0] and np.array_equal(a[1], b[1]) for a, b in zip(ref, got))
+        for k in probes:
+            assert par.lookup(k) == serial.lookup(k)
+    assert len(serial.lookup(b"A" * 15)) == 300 and len(serial.lookup(b"T" * 15)) == 300
+
+
 # examples/tune_homopolymer.rs:87-146.  At the current gap-open of -90 the
 # reference states the battery passes "at 100%" (smith_waterman.rs:47-52); the
 # 284/285 figure of smith_waterman.rs:266-271 predates that recalibration.
