@@ -2,28 +2,35 @@
 with the learned candidate prefilter.
 
 Workloads (probebwa_amd/synth.py):
-- uneven_se_100mbp (default): a 100 Mbp synthetic genome over 10 contigs
-  (uniform background, 5% segmental duplications at 0.5-3% divergence, 5-15 bp
-  homopolymer runs every ~2 kb) and 2,000,000 single-end reads with lengths
-  U{80..150}, Phred qualities decaying 38 -> 22 along the read (+ N(0,3) noise),
-  substitutions at the Phred-implied rate.  The prefilter trains on 131,072
-  other reads of the same generator.
-- prefilter_se_2mbp: the 2.1 Mbp probe genome (2 Mbp uniform + 50 copies of a
-  2 kb unit at 1% divergence); the prefilter trains on 500,000 reads and is
-  evaluated on 2,000,000 held-out reads (L ~ U{80..150}, 1% substitutions,
-  0.05% indels, Q ~ U{20..40}).
+- genome_se_3gbp (default): a 3.1 Gbp synthetic genome over 24 contigs (45%
+  interspersed repeats from a 300 bp family at 10-20% divergence and a 6 kb family at
+  5-15%, 5% segmental duplications), single-end reads of 60-250 bp with Phred
+  qualities decaying 38 -> 22 along the read (+ N(0,3) noise) and substitutions at
+  the Phred-implied rate; plus a paired-end leg of 65,536 FR pairs of 2 x 150 on the
+  same genome.  The prefilter trains on 131,072 other reads of the same generator.
+- uneven_se_100mbp: a 100 Mbp genome over 10 contigs (uniform background, 5%
+  segmental duplications at 0.5-3% divergence, 5-15 bp homopolymer runs every
+  ~2 kb), 2,000,000 reads of 80-150 bp with the same read model.
+- prefilter_se_2mbp: the 2.1 Mbp probe genome (2 Mbp uniform + 50 copies of a 2 kb
+  unit at 1% divergence); the prefilter trains on 500,000 reads and is evaluated on
+  2,000,000 held-out reads (L ~ U{80..150}, 1% substitutions, 0.05% indels,
+  Q ~ U{20..40}).
+- probe_pe_2mbp: 1,000,000 FR pairs of 2 x 150 on the probe genome (pairs/s).
 
 A step maps one batch of `--batch` reads that are already resident in HBM
-(`pbw_reads_upload` once, then `pbw_map_se_device` per step; calls return
-after the records are in host memory): seed -> cluster -> banded DP -> MAPQ.
-`value` = reads mapped per second over the K timed steps of the plain path.
-The same K steps are then timed with the trained prefilter on (`learned`),
-and every record of them is compared with the plain path's.
+(`pbw_reads_upload` once, then `pbw_map_se_device` per step; calls return after the
+records are in host memory): seed -> cluster -> banded DP -> MAPQ.  `value` = reads
+mapped per second over the K timed steps of the plain path.  The reads' upload over
+PCIe is timed on its own and reported beside it (`pcie_inclusive_reads_per_s`).
+The same K steps are then timed with the trained prefilter on (`learned`), and every
+record of them is compared with the plain path's.
 
-Also reported: the dominant kernel's roofline position (HIP-event timed inside
-the library on its own stream), and the CPU oracle's reads/s on a bounded
-sample of the same reads on the host cores, with the sample's SAM-field
-concordance between the HIP path and the oracle.
+Also reported: the roofline position of the kernel that takes the most time per step
+(seed gather against HBM, clustering against HBM, or the banded DP against the
+integer VALU), HIP-event timed inside the library on its own stream; and the CPU
+oracle's reads/s on bounded samples of the same reads on the host cores (16 threads,
+and 1 thread), with the sample's SAM-field concordance between the HIP path and the
+oracle.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
@@ -42,45 +49,42 @@ PEAK_HBM_GBPS = 8000.0
 # integer VALU: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz (no packed int32)
 PEAK_INT32_TOPS = 256 * 64 * 2.4e9 / 1e12
 DP_OPS_PER_CELL = 20  # VALU ops of one cell of the reference recurrence (smith_waterman.rs:204-233)
+CAND_BYTES = 20 * 16 + 1  # a read's candidate list written by the clustering kernels
+PROFILE_BATCH = 262144  # reads per launch the committed counter summaries were taken at
+HOST_THREADS = min(16, os.cpu_count() or 1)  # the GPU host's CPU share is 16 threads
 
 
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_traffic_uneven_se_100mbp.txt")
-TRAFFIC_BATCH = 262144  # reads per launch the profile was taken at
-
-
-VALU_PROFILE = os.path.join(ROOT, "profiles", "pmc_valu_uneven_se_100mbp.txt")
-
-
-PROFILED_WORKLOAD = "uneven_se_100mbp"  # the workload the committed counter summaries were taken on
-
-
-def measured_valu(kernel, batch, workload=PROFILED_WORKLOAD):
-    """VALU issue counters of `kernel` from the committed rocprofv3 summary (taken at the
-    default batch): fraction of the VALU issue capacity used, int32 share of it, active lanes."""
-    if workload != PROFILED_WORKLOAD or batch != TRAFFIC_BATCH or not os.path.exists(VALU_PROFILE):
-        return None
-    for line in open(VALU_PROFILE):
+def _profile_rows(kind, workload, batch, ncols):
+    path = os.path.join(ROOT, "profiles", "pmc_%s_%s.txt" % (kind, workload))
+    if batch != PROFILE_BATCH or not os.path.exists(path):
+        return {}
+    rows = {}
+    for line in open(path):
         f = line.strip().split(",")
-        if len(f) == 10 and f[0] == kernel:
-            return {"valu_busy": float(f[7]), "int32_util": float(f[8]), "lanes_active": float(f[9]),
-                    "valu_insts": int(f[1]), "int32_insts": int(f[2])}
-    return None
+        if len(f) == ncols and not f[0].startswith("#") and f[0] != "kernel":
+            rows[f[0]] = f
+    return rows
 
 
-def measured_traffic(kernel, batch, workload=PROFILED_WORKLOAD):
+def measured_traffic(kernel, batch, workload):
     """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, copy-calibrated) from the
-    committed rocprofv3 counter summary, when it was taken on this workload and batch size; else None."""
-    if workload != PROFILED_WORKLOAD or batch != TRAFFIC_BATCH or not os.path.exists(TRAFFIC_PROFILE):
+    committed rocprofv3 counter summary of this workload at the default batch; else None."""
+    f = _profile_rows("traffic", workload, batch, 7).get(kernel)
+    return int(float(f[5])) if f else None
+
+
+def measured_valu(kernel, batch, workload):
+    """VALU issue counters of `kernel` from the committed rocprofv3 summary: fraction of the VALU
+    issue capacity used, int32 share of it, active lanes."""
+    f = _profile_rows("valu", workload, batch, 10).get(kernel)
+    if not f:
         return None
-    for line in open(TRAFFIC_PROFILE):
-        f = line.strip().split(",")
-        if len(f) == 7 and f[0] == kernel:
-            return int(float(f[5]))
-    return None
+    return {"valu_busy": float(f[7]), "int32_util": float(f[8]), "lanes_active": float(f[9]),
+            "valu_insts": int(f[1]), "int32_insts": int(f[2])}
 
 
 def bench_pe(args):
@@ -115,7 +119,7 @@ def bench_pe(args):
         go = O.Genome()
         for name, s in contigs:
             go.add_contig(name, s.tobytes())
-        m = O.Mapper(go, O.HashTable(go), threads=args.cpu_threads)
+        m = O.Mapper(go, O.HashTable(go, threads=args.cpu_threads), threads=args.cpu_threads)
         a = O.ReadBatch.from_arrays(*[x for x in m1[:4]]).subset(0, S)
         b = O.ReadBatch.from_arrays(*[x for x in m2[:4]]).subset(0, S)
         t2 = time.perf_counter()
@@ -141,13 +145,13 @@ def bench_pe(args):
 
 
 WORKLOADS = {
-    "uneven_se_100mbp": {"reads": 2_000_000, "train_reads": 131072,
-                         "desc": "single-end, 100 Mbp / 10 contigs, 2M reads of 80-150 bp"},
-    "prefilter_se_2mbp": {"reads": 2_000_000, "train_reads": 500_000,
-                          "desc": "single-end, 2.1 Mbp probe genome, 2M held-out reads of 80-150 bp"},
-    "genome_se_3gbp": {"reads": 10_000_000, "train_reads": 131072,
+    "genome_se_3gbp": {"reads": 10_000_000, "train_reads": 131072, "pe_pairs": 65536,
                        "desc": "single-end, 3.1 Gbp / 24 contigs (45% interspersed repeats, 5% segmental "
-                               "duplications), 10M reads of 60-250 bp"},
+                               "duplications), reads of 60-250 bp"},
+    "uneven_se_100mbp": {"reads": 2_000_000, "train_reads": 131072, "pe_pairs": 0,
+                         "desc": "single-end, 100 Mbp / 10 contigs, 2M reads of 80-150 bp"},
+    "prefilter_se_2mbp": {"reads": 2_000_000, "train_reads": 500_000, "pe_pairs": 0,
+                          "desc": "single-end, 2.1 Mbp probe genome, 2M held-out reads of 80-150 bp"},
 }
 
 
@@ -166,18 +170,133 @@ def make_workload(name, n, n_train):
             lambda: synth.reads_probe_uneven_se(contigs, n=n_train, seed=41), "seeds 1/42/41")
 
 
+def stage_rooflines(agg, B, workload):
+    """Roofline position of each stage's kernels per launch, from the library's HIP-event times
+    and the oracle-defined algorithmic work units, and the stage that takes the most time."""
+    n_seed, n_cl, n_dp = (max(agg[k], 1) for k in ("n_k_seed", "n_k_cluster", "n_k_dp"))
+    ms_seed, ms_cl, ms_dp = agg["ms_k_seed"] / n_seed, agg["ms_k_cluster"] / n_cl, agg["ms_k_dp"] / n_dp
+    gather_bytes = 8 * (agg["lookups"] + agg["positions"] + agg["filtered_hits"]) / n_seed
+    cl_bytes = (8 * agg["filtered_hits"] + CAND_BYTES * agg["reads"]) / n_cl
+    cells = agg["dp_cells"] / n_dp
+    seed = {"kernel": "k_seed", "bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBPS,
+            "achieved": round(gather_bytes / (ms_seed * 1e-3) / 1e9, 2) if ms_seed > 0 else 0.0,
+            "traffic": measured_traffic("pbw::k_seed", B, workload),
+            "per_launch": {"bytes": gather_bytes, "ms": ms_seed,
+                           "unit": "8 B per lookup + 8 B per position + 8 B per filtered hit"}}
+    cluster = {"kernel": "k_cluster (all tiers)", "bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBPS,
+               "achieved": round(cl_bytes / (ms_cl * 1e-3) / 1e9, 2) if ms_cl > 0 else 0.0,
+               "traffic": measured_traffic("pbw::k_cluster_tier2", B, workload),
+               "per_launch": {"bytes": cl_bytes, "ms": ms_cl,
+                              "unit": "8 B per hit key read + 321 B of candidates per read written"}}
+    ach = cells / (ms_dp * 1e-3) * DP_OPS_PER_CELL / 1e12 if ms_dp > 0 else 0.0
+    dp = {"kernel": "k_dp_quad", "bound": "valu-int32", "unit": "Tops/s", "peak": round(PEAK_INT32_TOPS, 2),
+          "achieved": round(ach, 3), "traffic": measured_traffic("pbw::k_dp_quad", B, workload),
+          "per_launch": {"cells": cells, "ms": ms_dp, "ops_per_cell": DP_OPS_PER_CELL},
+          "counters": measured_valu("pbw::k_dp_quad", B, workload)}
+    for r in (seed, cluster, dp):
+        r["frac"] = round(r["achieved"] / r["peak"], 4)
+    times = {"seed": ms_seed, "cluster": ms_cl, "dp": ms_dp}
+    stages = {"seed": seed, "cluster": cluster, "dp": dp}
+    top = max(times, key=times.get)
+    return stages[top], stages, cells / (ms_dp * 1e-3) / 1e9 if ms_dp > 0 else 0.0, seed["achieved"]
+
+
+def run_steps(ctx, dreads, B, n, k, label):
+    agg, recs_all = {}, []
+    t0 = time.perf_counter()
+    for s in range(k):
+        lo = (s * B) % n
+        hi = min(n, lo + B)
+        recs = ctx.map_se_device(dreads, lo, hi)
+        st = ctx.stats()
+        recs_all.append(recs)
+        for key, v in st.items():
+            agg[key] = agg.get(key, 0) + v
+        log("%s step %d: %d reads in %.1f ms" % (label, s, st["reads"], st["ms_total"]))
+    return agg, recs_all, time.perf_counter() - t0
+
+
+def pe_leg(ctx, contigs, pairs):
+    """Paired-end throughput on the same genome and context: one map_pe call over `pairs` FR
+    pairs (500 serial warm-up pairs, then 2000-pair frozen-snapshot batches)."""
+    import probebwa_amd as P
+    from probebwa_amd import synth
+    a, b = synth.reads_pairs_genome(contigs, n=pairs, seed=58)
+    r1, r2 = P.ReadBatch.from_arrays(*a), P.ReadBatch.from_arrays(*b)
+    ctx.map_pe(r1.subset(0, 2500), r2.subset(0, 2500))  # warm-up call
+    t0 = time.perf_counter()
+    recs = ctx.map_pe(r1, r2)
+    dt = time.perf_counter() - t0
+    st = ctx.stats()
+    proper = float(((recs.rec["flag"] & 2) != 0).mean())
+    log("paired-end: %d pairs in %.2f s" % (pairs, dt))
+    return {"pairs": pairs, "pairs_per_s": round(pairs / dt, 1), "reads_per_s": round(2 * pairs / dt, 1),
+            "seconds": round(dt, 3), "proper_pair_frac": round(proper, 4), "wide_jobs": int(st["wide_jobs"]),
+            "wide_cells": int(st["wide_cells"]), "insert_model": [round(x, 3) for x in ctx.insert_model[:2]],
+            "stage_ms": {k: round(st[k], 1) for k in ("ms_seed", "ms_cluster", "ms_dp", "ms_pair", "ms_total")},
+            "data": "FR pairs of 2x150 (fragment N(350,50) clipped to [200,600]), seed 58"}
+
+
+def cpu_leg(args, contigs, reads, first_recs, sample, sample_1t):
+    """The oracle (C++ restatement of the reference CPU path) on bounded samples of the timed
+    reads: `sample` reads on HOST_THREADS threads (the baseline) and `sample_1t` on one; the
+    sample's SAM concordance with the HIP path; and how many of the sample's records change
+    when the oracle's cluster group order switches from the emulated FxHashMap order to
+    (contig, forward before reverse) -- the reads whose records depend on the one ordering
+    no reference test pins."""
+    import probebwa_amd as P
+    from oracle import pbw_oracle as O
+    seq, so, qual, qo = reads
+    t1 = time.time()
+    go = O.Genome()
+    for name, s in contigs:
+        go.add_contig(name, s.tobytes())
+    ht = O.HashTable(go, progress=True, threads=args.cpu_threads)
+    log("oracle index %.1f s (%d threads)" % (time.time() - t1, args.cpu_threads))
+
+    def sub(S):
+        return O.ReadBatch.from_arrays(seq[:int(so[S])], so[:S + 1], qual[:int(qo[S])], qo[:S + 1])
+
+    m = O.Mapper(go, ht, threads=args.cpu_threads)
+    t1 = time.perf_counter()
+    ro = m.map_se(sub(sample))
+    cdt = time.perf_counter() - t1
+    log("oracle %d reads on %d threads in %.1f s" % (sample, args.cpu_threads, cdt))
+    cpu = {"value": round(sample / cdt, 1), "unit": "reads/s", "cores": args.cpu_threads, "kind": "port",
+           "sample": "first %d reads of the timed reads, oracle (C++ restatement of the reference CPU path) "
+                     "on %d threads, %.1f s" % (sample, args.cpu_threads, cdt)}
+    concord = float(P.primary_identical(ro, P.Records(first_recs.rec[:sample], first_recs.cigar)).mean())
+    O.set_group_order(1)
+    try:
+        rl = O.Mapper(go, ht, threads=args.cpu_threads).map_se(sub(sample))
+    finally:
+        O.set_group_order(0)
+    changed = int((~P.primary_identical(ro, rl)).sum() + ((ro.rec["log_lk"] != rl.rec["log_lk"])
+                                                           & P.primary_identical(ro, rl)).sum())
+    m1 = O.Mapper(go, ht, threads=1)
+    t1 = time.perf_counter()
+    m1.map_se(sub(sample_1t))
+    cdt1 = time.perf_counter() - t1
+    log("oracle %d reads on 1 thread in %.1f s" % (sample_1t, cdt1))
+    cpu1 = {"value": round(sample_1t / cdt1, 1), "unit": "reads/s", "cores": 1, "kind": "port",
+            "sample": "first %d reads of the timed reads, oracle on 1 thread, %.1f s" % (sample_1t, cdt1)}
+    return cpu, cpu1, concord, {"reads": sample, "records_changed": changed}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=262144)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["probe_pe_2mbp"], default="uneven_se_100mbp")
-    ap.add_argument("--reads", type=int, default=None)
-    ap.add_argument("--cpu-sample", type=int, default=131072)
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["probe_pe_2mbp"], default="genome_se_3gbp")
+    ap.add_argument("--reads", type=int, default=None,
+                    help="reads generated (default: enough for the timed steps, at most the workload's count)")
+    ap.add_argument("--cpu-sample", type=int, default=None)
+    ap.add_argument("--cpu-threads", type=int, default=HOST_THREADS)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-pe", action="store_true")
     ap.add_argument("--autotune", action="store_true",
                     help="tune launch shapes on the first 524288 reads before the timed steps")
     ap.add_argument("--train-reads", type=int, default=None)
@@ -205,27 +324,35 @@ def main():
     if args.workload == "probe_pe_2mbp":
         bench_pe(args)
         return
-    import probebwa_amd as P
 
     wl = WORKLOADS[args.workload]
-    n_reads = args.reads or wl["reads"]
+    n_reads = args.reads or min(wl["reads"], args.batch * max(args.steps, 1))
     n_train = args.train_reads or wl["train_reads"]
     t0 = time.time()
     contigs, gen_reads, gen_train, seeds = make_workload(args.workload, n_reads, n_train)
-    gp = P.Genome()
-    for name, s in contigs:
-        gp.add_contig(name, s.tobytes())
     log("genome %.1f s" % (time.time() - t0))
     t0 = time.time()
     seq, so, qual, qo = gen_reads()
     log("reads %.1f s (%d reads, %d bases)" % (time.time() - t0, n_reads, int(so[-1])))
-    t0 = time.time()
-    ctx = P.Context(gp, batch_reads=args.batch, device=0)
-    log("context (index build + upload) %.1f s, %d index entries" % (time.time() - t0, ctx.total_entries))
 
-    dreads = ctx.upload_reads(P.ReadBatch.from_arrays(seq, so, qual, qo))
+    import probebwa_amd as P
+    t0 = time.time()
+    gp = P.Genome()
+    for name, s in contigs:
+        gp.add_contig(name, s.tobytes())
+    ctx = P.Context(gp, batch_reads=args.batch, device=0)
+    log("context (genome packing, index build on the device, upload) %.1f s, %d index entries"
+        % (time.time() - t0, ctx.total_entries))
+
+    batch_all = P.ReadBatch.from_arrays(seq, so, qual, qo)
+    dreads = ctx.upload_reads(batch_all)  # first upload: allocation
+    dreads = None
+    t0 = time.perf_counter()
+    dreads = ctx.upload_reads(batch_all)
+    t_up = time.perf_counter() - t0
     n = n_reads
     B = min(args.batch, n)
+    log("reads upload %.3f s (%.2f GB)" % (t_up, (int(so[-1]) * 2) / 1e9))
 
     tuned = None
     if args.autotune:
@@ -236,48 +363,12 @@ def main():
         B = min(ctx.tune.batch_reads, n) if args.batch == 262144 else B
         log("autotune %.1f s: %s" % (time.time() - t1, rep["best"]))
 
-    def step(s):
-        lo = (s * B) % n
-        hi = min(n, lo + B)
-        return ctx.map_se_device(dreads, lo, hi), ctx.stats()
-
-    for w in range(args.warmup):
-        step(w)
-        log("warmup step %d done" % w)
-    agg = {}
-    plain_recs = []
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        recs, st = step(k)
-        plain_recs.append(recs)
-        for key, v in st.items():
-            agg[key] = agg.get(key, 0) + v
-        log("step %d: %d reads in %.1f ms" % (k, st["reads"], st["ms_total"]))
-    dt = time.perf_counter() - t0
+    run_steps(ctx, dreads, B, n, args.warmup, "warmup")
+    agg, plain_recs, dt = run_steps(ctx, dreads, B, n, args.steps, "plain")
     reads_done = int(agg["reads"])
     value = reads_done / dt
-
-    # dominant kernel + roofline (HIP events on the library's stream)
-    ms_dp = agg["ms_k_dp"] / max(agg["n_k_dp"], 1)
-    ms_seed = agg["ms_k_seed"] / max(agg["n_k_seed"], 1)
-    cells_per_launch = agg["dp_cells"] / max(agg["n_k_dp"], 1)
-    gather_bytes = 8 * (agg["lookups"] + agg["positions"] + agg["filtered_hits"])
-    gbytes_per_launch = gather_bytes / max(agg["n_k_seed"], 1)
-    dp_gcups = cells_per_launch / (ms_dp * 1e-3) / 1e9 if ms_dp > 0 else 0.0
-    gather_gbps = gbytes_per_launch / (ms_seed * 1e-3) / 1e9 if ms_seed > 0 else 0.0
-    if agg["ms_k_dp"] >= agg["ms_k_seed"]:
-        ach = dp_gcups * DP_OPS_PER_CELL / 1000.0
-        roof = {"kernel": "k_dp_quad", "bound": "valu-int32", "achieved": round(ach, 3), "peak": round(PEAK_INT32_TOPS, 2),
-                "unit": "Tops/s", "frac": round(ach / PEAK_INT32_TOPS, 4), "traffic": measured_traffic("pbw::k_dp_quad", B, args.workload),
-                "per_launch": {"cells": cells_per_launch, "ms": ms_dp, "ops_per_cell": DP_OPS_PER_CELL},
-                "counters": measured_valu("pbw::k_dp_quad", B, args.workload)}
-    else:
-        roof = {"kernel": "k_seed", "bound": "hbm", "achieved": round(gather_gbps, 2), "peak": PEAK_HBM_GBPS,
-                "unit": "GB/s", "frac": round(gather_gbps / PEAK_HBM_GBPS, 4), "traffic": measured_traffic("pbw::k_seed", B, args.workload),
-                "per_launch": {"bytes": gbytes_per_launch, "ms": ms_seed}}
-    gather_roof = {"kernel": "k_seed", "bound": "hbm", "achieved": round(gather_gbps, 2), "peak": PEAK_HBM_GBPS,
-                   "unit": "GB/s", "frac": round(gather_gbps / PEAK_HBM_GBPS, 4),
-                   "traffic": measured_traffic("pbw::k_seed", B, args.workload)}
+    up_per_step = t_up * min(1.0, reads_done / n)  # PCIe time of the reads the steps mapped
+    roof, stages, dp_gcups, gather_gbps = stage_rooflines(agg, B, args.workload)
 
     learned = None
     if not args.no_learned:
@@ -291,18 +382,8 @@ def main():
         log("prefilter trained in %.1f s on %d rows (%d positive), held-out skip %.3f recall %.5f"
             % (time.time() - t1, rep["rows"], rep["positives"], rep["val_skip"], rep["val_recall"]))
         ctx.set_prefilter(enabled=1, audit_period=64)
-        for w in range(args.warmup):
-            step(w)
-        agg2 = {}
-        learned_recs = []
-        t1 = time.perf_counter()
-        for k in range(args.steps):
-            recs, st = step(k)
-            learned_recs.append(recs)
-            for key, v in st.items():
-                agg2[key] = agg2.get(key, 0) + v
-            log("learned step %d: %d reads in %.1f ms" % (k, st["reads"], st["ms_total"]))
-        dt2 = time.perf_counter() - t1
+        run_steps(ctx, dreads, B, n, args.warmup, "learned warmup")
+        agg2, learned_recs, dt2 = run_steps(ctx, dreads, B, n, args.steps, "learned")
         ctx.set_prefilter(enabled=0)
         same = np.concatenate([P.primary_identical(a, b) for a, b in zip(plain_recs, learned_recs)])
         del learned_recs
@@ -331,28 +412,17 @@ def main():
                          "loss_last_epoch": round(rep["loss_last"], 6)},
         }
 
-    cpu = None
-    concord = None
-    if not args.no_cpu:
-        from oracle import pbw_oracle as O
+    pe = None
+    if wl["pe_pairs"] and not args.no_pe:
+        pe = pe_leg(ctx, contigs, wl["pe_pairs"])
 
-        S = min(args.cpu_sample, B)
-        t1 = time.time()
-        go = O.Genome()
-        for name, s in contigs:
-            go.add_contig(name, s.tobytes())
-        ht = O.HashTable(go, progress=True)
-        m = O.Mapper(go, ht, threads=args.cpu_threads)
-        log("oracle index %.1f s" % (time.time() - t1))
-        rb = O.ReadBatch.from_arrays(seq[:int(so[S])], so[:S + 1], qual[:int(qo[S])], qo[:S + 1])
-        t1 = time.perf_counter()
-        ro = m.map_se(rb)
-        cdt = time.perf_counter() - t1
-        cpu = {"value": round(S / cdt, 1), "unit": "reads/s", "cores": args.cpu_threads, "kind": "port",
-               "sample": "first %d reads of the workload, oracle (C++ restatement) on %d threads, %.1f s"
-                         % (S, args.cpu_threads, cdt)}
-        first = plain_recs[0]
-        concord = float(P.primary_identical(ro, P.Records(first.rec[:S], first.cigar)).mean())
+    cpu = cpu1 = concord = order = None
+    if not args.no_cpu:
+        big = args.workload == "genome_se_3gbp"
+        S = min(args.cpu_sample or (32768 if big else 131072), B)
+        S1 = min(2048 if big else 16384, S)
+        del ctx, dreads
+        cpu, cpu1, concord, order = cpu_leg(args, contigs, (seq, so, qual, qo), plain_recs[0], S, S1)
 
     out = {
         "metric": "reads/s on 1 GPU, learned vs plain @≥99.9% SAM concordance; DP GCUPS",
@@ -368,17 +438,23 @@ def main():
         "dtype": "int32+f64",
         "data": "synthetic (%s generator, %s)" % (args.workload, seeds),
         "config": {"workload": "%s: plain GPU path, %s" % (args.workload, wl["desc"]), "batch_reads": B,
-                   "reads_per_step": B, "path": "plain (no learned prefilter)"},
+                   "reads_per_step": B, "reads_generated": n, "path": "plain (no learned prefilter)"},
+        "pcie_inclusive_reads_per_s": round(reads_done / (dt + up_per_step), 1),
+        "upload_ms_per_step": round(up_per_step * 1000.0 / args.steps, 3),
         "dp_gcups": round(dp_gcups, 2),
         "gather_gbps": round(gather_gbps, 2),
         "roofline": roof,
-        "roofline_gather": gather_roof,
+        "roofline_stages": stages,
         "autotune": tuned,
         "learned": learned,
         "learned_vs_plain": learned["speedup_vs_plain"] if learned else None,
+        "paired_end": pe,
         "cpu_baseline": cpu,
+        "cpu_baseline_1_thread": cpu1,
         "sample_sam_concordance": concord,
+        "cluster_group_order_sensitivity": order,
         "stage_ms": {k: round(agg[k], 1) for k in ("ms_seed", "ms_cluster", "ms_dp", "ms_mapq", "ms_total")},
+        "kernel_ms_per_step": {k: round(agg["ms_k_" + k] / args.steps, 3) for k in ("seed", "cluster", "dp")},
         "counters": {k: int(agg[k]) for k in ("reads", "lookups", "positions", "filtered_hits", "dp_cells",
                                                "seed_overflow_reads", "tier2_cluster_reads", "band_dp_jobs",
                                                "generic_dp_jobs")},

@@ -15,12 +15,14 @@
 //      within a code, entries stay in scan order (the reference's stable
 //      sort_by_key, hashtable.rs:297, over hashes; mix() is a bijection so
 //      code order and hash order group identically);
-//   3. dense per-code run starts and capped run lengths (min(run, 300): the
-//      reference stops pushing a k-mer's positions once 300 are kept,
-//      hashtable.rs:270-273, 286-289 — the same first 300 of the scan);
-//   4. an exclusive scan of the capped lengths over all 4^15 codes gives
-//      koff[c] (bucket c = [koff[c], koff[c+1]));
+//   3. every sorted entry's run start (a max-scan of run-start marks) and,
+//      at each run's end, its capped length min(run, 300) into koff[code]: the
+//      reference stops pushing a k-mer's positions once 300 are kept
+//      (hashtable.rs:270-273, 286-289), i.e. it keeps the first 300 of the scan;
+//   4. an in-place exclusive scan of koff over all 4^15 codes gives the
+//      bucket starts (bucket c = [koff[c], koff[c+1]));
 //   5. the first 300 entries of each run are decoded from j and placed.
+// Temporaries are sized by the number of windows, not by the code space.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -96,25 +98,24 @@ __global__ void k_idx_windows(const uint64_t* __restrict__ g2b, const uint64_t* 
   }
 }
 
-// Run starts of the sorted codes (N windows sort last under the sentinel).
-__global__ void k_idx_starts(const uint32_t* __restrict__ key, uint64_t n, uint32_t* __restrict__ dstart) {
+// Run start of every sorted entry, step 1: i at a run's first entry, else 0
+// (an inclusive max-scan then carries each run's start along it).
+__global__ void k_idx_startmark(const uint32_t* __restrict__ key, uint64_t n, uint32_t* __restrict__ mark) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const uint32_t k = key[i];
-    if (k != IDX_SENTINEL && (i == 0 || key[i - 1] != k)) dstart[k] = (uint32_t)i;
-  }
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    mark[i] = (i == 0 || key[i - 1] != key[i]) ? (uint32_t)i : 0u;
 }
 
-// Capped run length at each run's last entry (dstart complete: previous launch).
-__global__ void k_idx_counts(const uint32_t* __restrict__ key, uint64_t n, const uint32_t* __restrict__ dstart,
-                             uint32_t* __restrict__ cnt) {
+// Capped run length (min(run, 300)) into koff[code] at each run's last entry.
+__global__ void k_idx_counts(const uint32_t* __restrict__ key, uint64_t n, const uint32_t* __restrict__ rstart,
+                             uint32_t* __restrict__ koff) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const uint32_t k = key[i];
     if (k == IDX_SENTINEL) continue;
     if (i + 1 == n || key[i + 1] != k) {
-      const uint64_t run = i + 1 - dstart[k];
-      cnt[k] = (uint32_t)(run < (uint64_t)MAX_COUNT ? run : (uint64_t)MAX_COUNT);
+      const uint64_t run = i + 1 - rstart[i];
+      koff[k] = (uint32_t)(run < (uint64_t)MAX_COUNT ? run : (uint64_t)MAX_COUNT);
     }
   }
 }
@@ -122,14 +123,14 @@ __global__ void k_idx_counts(const uint32_t* __restrict__ key, uint64_t n, const
 // The first MAX_COUNT entries of every run, decoded to packed positions
 // (pack_position, hashtable.rs:157-164: bit 63 strand, 62..40 contig, 39..0 position).
 __global__ void k_idx_place(const uint32_t* __restrict__ key, const uint32_t* __restrict__ val, uint64_t n,
-                            const uint32_t* __restrict__ dstart, const uint32_t* __restrict__ koff,
+                            const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ koff,
                             const uint64_t* __restrict__ clen, const uint64_t* __restrict__ cbase, int32_t nctg,
                             uint64_t* __restrict__ kpos) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const uint32_t k = key[i];
     if (k == IDX_SENTINEL) continue;
-    const uint64_t r = i - dstart[k];
+    const uint64_t r = i - rstart[i];
     if (r >= (uint64_t)MAX_COUNT) continue;
     const uint64_t j = val[i];
     const int32_t c = slot_contig(cbase, nctg, j);
@@ -138,6 +139,120 @@ __global__ void k_idx_place(const uint32_t* __restrict__ key, const uint32_t* __
     slot_window(j - cbase[c], clen[c], pos, strand);
     kpos[(uint64_t)koff[k] + r] = ((uint64_t)strand << 63) | ((uint64_t)c << 40) | pos;
   }
+}
+
+// In-place exclusive sum of x[0, n) in three passes: tile sums, one block's
+// scan of them, and each tile's scan from its base.  A tile is read and written
+// only by its own block, so the scan is safe in place.
+constexpr uint32_t SCAN_T = 256, SCAN_E = 16, SCAN_TILE = SCAN_T * SCAN_E;
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_tiles(const uint32_t* __restrict__ x, uint64_t n,
+                                                       uint32_t* __restrict__ part) {
+  using BR = hipcub::BlockReduce<uint32_t, SCAN_T>;
+  __shared__ typename BR::TempStorage tmp;
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+  uint32_t s = 0;
+  for (uint32_t e = 0; e < SCAN_E; e++) {
+    const uint64_t i = base + (uint64_t)e * SCAN_T + threadIdx.x;
+    if (i < n) s += x[i];
+  }
+  s = BR(tmp).Sum(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_parts(uint32_t* part, uint32_t np) {
+  using BS = hipcub::BlockScan<uint32_t, 1024>;
+  __shared__ typename BS::TempStorage tmp;
+  __shared__ uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t b0 = 0; b0 < np; b0 += 1024) {
+    const uint32_t i = b0 + threadIdx.x;
+    const uint32_t v = i < np ? part[i] : 0u;
+    uint32_t ex, tot;
+    BS(tmp).ExclusiveSum(v, ex, tot);
+    if (i < np) part[i] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += tot;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_apply(uint32_t* __restrict__ x, uint64_t n,
+                                                       const uint32_t* __restrict__ part) {
+  using BL = hipcub::BlockLoad<uint32_t, SCAN_T, SCAN_E, hipcub::BLOCK_LOAD_TRANSPOSE>;
+  using BSt = hipcub::BlockStore<uint32_t, SCAN_T, SCAN_E, hipcub::BLOCK_STORE_TRANSPOSE>;
+  using BS = hipcub::BlockScan<uint32_t, SCAN_T>;
+  __shared__ union {
+    typename BL::TempStorage load;
+    typename BSt::TempStorage store;
+    typename BS::TempStorage scan;
+  } tmp;
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+  const uint32_t valid = (uint32_t)min<uint64_t>(SCAN_TILE, n - base);
+  uint32_t v[SCAN_E];
+  BL(tmp.load).Load(x + base, v, valid, 0u);
+  __syncthreads();
+  BS(tmp.scan).ExclusiveSum(v, v);
+  __syncthreads();
+  const uint32_t b = part[blockIdx.x];
+  for (uint32_t e = 0; e < SCAN_E; e++) v[e] += b;
+  BSt(tmp.store).Store(x + base, v, valid);
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_max_tiles(const uint32_t* __restrict__ x, uint64_t n,
+                                                      uint32_t* __restrict__ part) {
+  using BR = hipcub::BlockReduce<uint32_t, SCAN_T>;
+  __shared__ typename BR::TempStorage tmp;
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+  uint32_t s = 0;
+  for (uint32_t e = 0; e < SCAN_E; e++) {
+    const uint64_t i = base + (uint64_t)e * SCAN_T + threadIdx.x;
+    if (i < n) s = max(s, x[i]);
+  }
+  s = BR(tmp).Reduce(s, hipcub::Max());
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// Inclusive max-scan of the tile maxima, shifted by one: part[b] = max of tiles < b.
+__global__ __launch_bounds__(1024) void k_max_parts(uint32_t* part, uint32_t np) {
+  using BS = hipcub::BlockScan<uint32_t, 1024>;
+  __shared__ typename BS::TempStorage tmp;
+  __shared__ uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t b0 = 0; b0 < np; b0 += 1024) {
+    const uint32_t i = b0 + threadIdx.x;
+    const uint32_t v = i < np ? part[i] : 0u;
+    uint32_t ex, tot;
+    BS(tmp).ExclusiveScan(v, ex, 0u, hipcub::Max(), tot);
+    if (i < np) part[i] = max(carry, ex);
+    __syncthreads();
+    if (threadIdx.x == 0) carry = max(carry, tot);
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_max_apply(uint32_t* __restrict__ x, uint64_t n,
+                                                      const uint32_t* __restrict__ part) {
+  using BL = hipcub::BlockLoad<uint32_t, SCAN_T, SCAN_E, hipcub::BLOCK_LOAD_TRANSPOSE>;
+  using BSt = hipcub::BlockStore<uint32_t, SCAN_T, SCAN_E, hipcub::BLOCK_STORE_TRANSPOSE>;
+  using BS = hipcub::BlockScan<uint32_t, SCAN_T>;
+  __shared__ union {
+    typename BL::TempStorage load;
+    typename BSt::TempStorage store;
+    typename BS::TempStorage scan;
+  } tmp;
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+  const uint32_t valid = (uint32_t)min<uint64_t>(SCAN_TILE, n - base);
+  uint32_t v[SCAN_E];
+  BL(tmp.load).Load(x + base, v, valid, 0u);
+  __syncthreads();
+  BS(tmp.scan).InclusiveScan(v, v, hipcub::Max());
+  __syncthreads();
+  const uint32_t b = part[blockIdx.x];
+  for (uint32_t e = 0; e < SCAN_E; e++) v[e] = max(v[e], b);
+  BSt(tmp.store).Store(x + base, v, valid);
 }
 
 struct Tmp {
@@ -174,62 +289,57 @@ hipError_t gpu_build_index(const DevGenome& G, const uint64_t* h_clen, uint32_t*
     err = "more than 2^32 indexed windows";
     return hipErrorInvalidValue;
   }
-  // the code space and koff are always complete, even for an empty index
+  // koff first holds the capped run length of every code (0 for absent codes)
   IDX_CHECK(hipMemsetAsync(koff, 0, (CODE_SPACE + 1) * sizeof(uint32_t), s));
-  if (n == 0) {
-    IDX_CHECK(hipMalloc((void**)kpos_out, sizeof(uint64_t)));
-    return hipSuccess;
-  }
-  Tmp t_cbase, t_k0, t_k1, t_v0, t_v1, t_start, t_cnt, t_sort, t_scan;
-  IDX_CHECK(hipMalloc(&t_cbase.p, (nctg + 1) * sizeof(uint64_t)));
-  IDX_CHECK(hipMemcpyAsync(t_cbase.p, cbase.data(), (nctg + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
-  IDX_CHECK(hipMalloc(&t_k0.p, n * sizeof(uint32_t)));
-  IDX_CHECK(hipMalloc(&t_k1.p, n * sizeof(uint32_t)));
-  IDX_CHECK(hipMalloc(&t_v0.p, n * sizeof(uint32_t)));
-  IDX_CHECK(hipMalloc(&t_v1.p, n * sizeof(uint32_t)));
-  const uint64_t* dcb = (const uint64_t*)t_cbase.p;
-  uint32_t *k0 = (uint32_t*)t_k0.p, *k1 = (uint32_t*)t_k1.p, *v0 = (uint32_t*)t_v0.p, *v1 = (uint32_t*)t_v1.p;
-  const dim3 grid(16384), block(256);
-  hipLaunchKernelGGL(k_idx_windows, grid, block, 0, s, G.g2b, G.nmask, G.coff, G.clen, dcb, nctg, n, k0, v0);
-  IDX_CHECK(hipGetLastError());
-
-  size_t sort_bytes = 0;
-  IDX_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, k0, k1, v0, v1, n, 0, 31, s));
-  IDX_CHECK(hipMalloc(&t_sort.p, sort_bytes));
-  IDX_CHECK(hipcub::DeviceRadixSort::SortPairs(t_sort.p, sort_bytes, k0, k1, v0, v1, n, 0, 31, s));
-  IDX_CHECK(hipFree(t_sort.p));
-  t_sort.p = nullptr;
-  IDX_CHECK(hipFree(t_k0.p));
-  t_k0.p = nullptr;
-  IDX_CHECK(hipFree(t_v0.p));
-  t_v0.p = nullptr;
-
-  IDX_CHECK(hipMalloc(&t_start.p, CODE_SPACE * sizeof(uint32_t)));
-  IDX_CHECK(hipMalloc(&t_cnt.p, (CODE_SPACE + 1) * sizeof(uint32_t)));
-  uint32_t *dstart = (uint32_t*)t_start.p, *cnt = (uint32_t*)t_cnt.p;
-  IDX_CHECK(hipMemsetAsync(cnt, 0, (CODE_SPACE + 1) * sizeof(uint32_t), s));
-  hipLaunchKernelGGL(k_idx_starts, grid, block, 0, s, k1, n, dstart);
-  hipLaunchKernelGGL(k_idx_counts, grid, block, 0, s, k1, n, dstart, cnt);
-  IDX_CHECK(hipGetLastError());
-  size_t scan_bytes = 0;
-  IDX_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, cnt, koff, CODE_SPACE + 1, s));
-  IDX_CHECK(hipMalloc(&t_scan.p, scan_bytes));
-  IDX_CHECK(hipcub::DeviceScan::ExclusiveSum(t_scan.p, scan_bytes, cnt, koff, CODE_SPACE + 1, s));
-  uint32_t total = 0;
-  IDX_CHECK(hipMemcpyAsync(&total, koff + CODE_SPACE, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  IDX_CHECK(hipStreamSynchronize(s));
-  IDX_CHECK(hipFree(t_scan.p));
-  t_scan.p = nullptr;
-  IDX_CHECK(hipFree(t_cnt.p));
-  t_cnt.p = nullptr;
+  const uint64_t nk = CODE_SPACE + 1;
+  const uint32_t np_k = (uint32_t)((nk + SCAN_TILE - 1) / SCAN_TILE);
+  const uint32_t np_n = (uint32_t)((n + SCAN_TILE - 1) / SCAN_TILE);
+  Tmp t_cbase, t_k0, t_k1, t_v0, t_v1, t_sort, t_part;
+  IDX_CHECK(hipMalloc(&t_part.p, (size_t)std::max(np_k, np_n) * sizeof(uint32_t)));
+  uint32_t* part = (uint32_t*)t_part.p;
   uint64_t* kpos = nullptr;
-  IDX_CHECK(hipMalloc((void**)&kpos, std::max<uint64_t>(total, 1) * sizeof(uint64_t)));
-  hipLaunchKernelGGL(k_idx_place, grid, block, 0, s, k1, v1, n, dstart, koff, G.clen, dcb, nctg, kpos);
+  uint32_t total = 0;
+  if (n) {
+    IDX_CHECK(hipMalloc(&t_cbase.p, (nctg + 1) * sizeof(uint64_t)));
+    IDX_CHECK(hipMemcpyAsync(t_cbase.p, cbase.data(), (nctg + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    IDX_CHECK(hipMalloc(&t_k0.p, n * sizeof(uint32_t)));
+    IDX_CHECK(hipMalloc(&t_k1.p, n * sizeof(uint32_t)));
+    IDX_CHECK(hipMalloc(&t_v0.p, n * sizeof(uint32_t)));
+    IDX_CHECK(hipMalloc(&t_v1.p, n * sizeof(uint32_t)));
+    const uint64_t* dcb = (const uint64_t*)t_cbase.p;
+    uint32_t *k0 = (uint32_t*)t_k0.p, *k1 = (uint32_t*)t_k1.p, *v0 = (uint32_t*)t_v0.p, *v1 = (uint32_t*)t_v1.p;
+    const dim3 grid(16384), block(256);
+    hipLaunchKernelGGL(k_idx_windows, grid, block, 0, s, G.g2b, G.nmask, G.coff, G.clen, dcb, nctg, n, k0, v0);
+    IDX_CHECK(hipGetLastError());
+    size_t sort_bytes = 0;
+    IDX_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, k0, k1, v0, v1, n, 0, 31, s));
+    IDX_CHECK(hipMalloc(&t_sort.p, sort_bytes));
+    IDX_CHECK(hipcub::DeviceRadixSort::SortPairs(t_sort.p, sort_bytes, k0, k1, v0, v1, n, 0, 31, s));
+    // run start of every sorted entry (in k0, free after the sort): marks, then a max-scan in place
+    uint32_t* rstart = k0;
+    hipLaunchKernelGGL(k_idx_startmark, grid, block, 0, s, k1, n, rstart);
+    hipLaunchKernelGGL(k_max_tiles, dim3(np_n), dim3(SCAN_T), 0, s, rstart, n, part);
+    hipLaunchKernelGGL(k_max_parts, dim3(1), dim3(1024), 0, s, part, np_n);
+    hipLaunchKernelGGL(k_max_apply, dim3(np_n), dim3(SCAN_T), 0, s, rstart, n, part);
+    hipLaunchKernelGGL(k_idx_counts, grid, block, 0, s, k1, n, rstart, koff);
+    IDX_CHECK(hipGetLastError());
+    // koff: exclusive sum of the capped lengths over every code, in place
+    hipLaunchKernelGGL(k_scan_tiles, dim3(np_k), dim3(SCAN_T), 0, s, koff, nk, part);
+    hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(1024), 0, s, part, np_k);
+    hipLaunchKernelGGL(k_scan_apply, dim3(np_k), dim3(SCAN_T), 0, s, koff, nk, part);
+    IDX_CHECK(hipGetLastError());
+    IDX_CHECK(hipMemcpyAsync(&total, koff + CODE_SPACE, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    IDX_CHECK(hipStreamSynchronize(s));
+    IDX_CHECK(hipMalloc((void**)&kpos, std::max<uint64_t>(total, 1) * sizeof(uint64_t)));
+    hipLaunchKernelGGL(k_idx_place, grid, block, 0, s, k1, v1, n, rstart, koff, G.clen, dcb, nctg, kpos);
+  } else {
+    IDX_CHECK(hipMalloc((void**)&kpos, sizeof(uint64_t)));
+  }
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
     (void)hipFree(kpos);
-    err = std::string("index placement failed: ") + hipGetErrorString(e);
+    err = std::string("index build failed: ") + hipGetErrorString(e);
     return e;
   }
   *kpos_out = kpos;
