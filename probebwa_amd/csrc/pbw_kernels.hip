@@ -31,6 +31,7 @@
 #include "pbw_internal.h"
 
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 namespace pbw {
 
@@ -97,6 +98,15 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
   return (uint32_t)v;
 }
 // Inclusive prefix max over the wave (same pattern, identity INT32_MIN).
+// The group of NL lanes (NL a power of two, 8..64, aligned in the wave) that
+// holds one read: every lane of it active, in the same control flow.
+template <int NL>
+__device__ __forceinline__ uint32_t group_sum_u32(uint32_t x) {
+#pragma unroll
+  for (int o = NL / 2; o > 0; o >>= 1) x += (uint32_t)__shfl_xor((int)x, o, 64);
+  return x;
+}
+
 __device__ __forceinline__ int32_t wave_incl_max(int32_t x) {
   const int32_t I = INT32_MIN;
   x = max(x, __builtin_amdgcn_update_dpp(I, x, 0x111, 0xf, 0xf, false));
@@ -828,9 +838,20 @@ struct ClusterMem {
   uint32_t* gt_key;     // block tier: LDS table of the read's distinct groups (CLUSTER_GT slots)
   uint32_t* gt_cnt;     //   hits per group, then the group's scatter cursor
   uint32_t* gt_misc;    //   [0] distinct groups, [1] listed groups
+  unsigned long long* gt_lo;  //   lowest biased diagonal per group slot, then per group rank
+  unsigned long long* gt_hi;  //   highest per slot, then the diagonal range per rank
+  void* rsort;                //   LDS of the block radix sort
 };
 constexpr uint32_t CLUSTER_STAGE = 1024;
 constexpr uint32_t CLUSTER_GT = 1024, CLUSTER_GT_MAX = 512;  // table slots, groups handled
+// Block-tier runs of whole segments sorted by one block radix sort (256 threads).
+constexpr uint32_t RSORT_T = 256, RSORT_BIG = 16, RSORT_SMALL = 4, RSORT_TILE = RSORT_T * RSORT_BIG;
+using RSortBig = hipcub::BlockRadixSort<uint32_t, RSORT_T, RSORT_BIG, uint32_t>;
+using RSortSmall = hipcub::BlockRadixSort<uint32_t, RSORT_T, RSORT_SMALL, uint32_t>;
+union RSortLds {
+  typename RSortBig::TempStorage big;
+  typename RSortSmall::TempStorage small;
+};
 
 // The same network for p2 <= 64 * E keys held by one wave in registers:
 // element e * 64 + lane sits in slot e of lane `lane`; partners at distance
@@ -934,6 +955,37 @@ struct ClusterFeat {
 
 __device__ __forceinline__ uint32_t gt_slot(uint32_t g) { return (g * 0x9E3779B1u) >> 22; }  // 10 bits
 
+// One merge-path step of a block: the sorted runs src[s, s + na) and
+// src[s + na, s + na + nb) into dst[s, s + na + nb), keys with their values.
+// Thread t writes outputs [t * per, (t + 1) * per), its split found by binary
+// search along its diagonal.
+__device__ void block_merge(const uint64_t* sk, const uint32_t* sv, uint64_t* dk, uint32_t* dv, uint32_t s,
+                            uint32_t na, uint32_t nb, uint32_t tid) {
+  const uint64_t* A = sk + s;
+  const uint64_t* B = sk + s + na;
+  const uint32_t total = na + nb, per = (total + RSORT_T - 1) / RSORT_T;
+  const uint32_t d0 = min(total, tid * per), d1 = min(total, d0 + per);
+  uint32_t lo = d0 > nb ? d0 - nb : 0u, hi = min(d0, na);
+  while (lo < hi) {  // i = A elements among the first d0 outputs (A first on ties)
+    const uint32_t mid = (lo + hi) >> 1;
+    if (A[mid] <= B[d0 - 1 - mid]) lo = mid + 1;
+    else hi = mid;
+  }
+  uint32_t i = lo, j = d0 - lo;
+  for (uint32_t d = d0; d < d1; d++) {
+    const bool takeA = j >= nb || (i < na && A[i] <= B[j]);
+    if (takeA) {
+      dk[s + d] = A[i];
+      dv[s + d] = sv[s + i];
+      i++;
+    } else {
+      dk[s + d] = B[j];
+      dv[s + d] = sv[s + na + j];
+      j++;
+    }
+  }
+}
+
 // Block tier: the (group, diagonal) sort as a segmented sort.  The hits are
 // counted per group in an LDS table, the groups ranked in ascending order, the
 // hits scattered into one segment per group (key[] / idx[], emission index as
@@ -945,25 +997,108 @@ __device__ __forceinline__ uint32_t gt_slot(uint32_t g) { return (g * 0x9E3779B1
 // equal keys may land in any order, as in the full sort: the clustering only
 // sums diagonals and takes minimum emission indices.  Returns false (nothing
 // written) when the read has more than CLUSTER_GT_MAX groups.
+// Rank of the run segment holding composite key x: the last q in [r, r1) with pre[q] <= x.
+__device__ __forceinline__ uint32_t run_segment(const uint32_t* pre, uint32_t r, uint32_t r1, uint32_t x) {
+  uint32_t lo = r, hi = r1 - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= x) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Sorts hits [a0, a0 + cnt) = whole segments r..r1-1 with one block radix sort
+// of 32-bit keys: segment q's biased diagonals [lo_q, lo_q + range_q] map to
+// pre_q + (d - lo_q), the segments' ranges laid end to end in group order, so the
+// composite key orders (group, diagonal) as the 64-bit hit key does.  The
+// emission index rides along as the value.
+template <int IPT, typename BRS>
+__device__ void run_radix_sort(const ClusterMem& M, uint32_t a0, uint32_t cnt, uint32_t r, uint32_t r1,
+                               int bits, uint32_t tid) {
+  const uint32_t* pre = M.gt_cnt + CLUSTER_GT_MAX;
+  uint32_t k[IPT], v[IPT];
+  const uint32_t pad = bits >= 32 ? 0xFFFFFFFFu : ((1u << bits) - 1u);
+#pragma unroll
+  for (int j = 0; j < IPT; j++) {  // striped loads: a sort takes its input in any order
+    const uint32_t pos = (uint32_t)j * RSORT_T + tid;
+    k[j] = pad;
+    v[j] = 0;
+    if (pos < cnt) {
+      const uint64_t key = M.key[a0 + pos];
+      const uint32_t q = run_segment(M.gt_key, r, r1, a0 + pos);
+      k[j] = pre[q] + (uint32_t)((key & ((1ull << DIAG_BITS) - 1)) - M.gt_lo[q]);
+      v[j] = M.idx[a0 + pos];
+    }
+  }
+  BRS(*reinterpret_cast<typename BRS::TempStorage*>(M.rsort)).Sort(k, v, 0, bits);
+#pragma unroll
+  for (int j = 0; j < IPT; j++) {  // blocked: thread tid holds ranks tid * IPT + j
+    const uint32_t pos = tid * (uint32_t)IPT + (uint32_t)j;
+    if (pos < cnt) {
+      const uint32_t q = run_segment(pre, r, r1, k[j]);
+      M.key[a0 + pos] = ((uint64_t)M.gt_cnt[q] << DIAG_BITS) | (M.gt_lo[q] + (k[j] - pre[q]));
+      M.idx[a0 + pos] = v[j];
+    }
+  }
+  __syncthreads();
+}
+
+// Block tier: the (group, diagonal) sort as a segmented sort.  The hits are
+// counted per group in an LDS table (with each group's lowest and highest
+// diagonal), the groups ranked in ascending order, the hits scattered into one
+// segment per group (key[] / idx[], emission index as the value), and then
+// runs of consecutive whole segments are sorted by a block radix sort of 32-bit
+// composite keys (run_radix_sort) -- as many segments as fit one 4096-hit tile
+// and a 2^32 span of diagonals.  A segment larger than a tile is sorted tile by
+// tile and the sorted tiles merged pairwise (block_merge) through the scratch
+// that cl_sum / cl_start provide until the clustering needs them; one whose
+// diagonals span 2^32 or more goes through the staged bitonic network there.  Ties between equal keys may land in any
+// order, as in the full sort: the clustering only sums diagonals and takes
+// minimum emission indices.  Returns false (nothing written) when the read has
+// more than CLUSTER_GT_MAX groups.
 template <int T>
 __device__ bool group_segment_sort(const uint64_t* hits, uint32_t n, const ClusterMem& M, uint32_t tid) {
+  static_assert(T == RSORT_T, "the block tier runs 256 threads");
   for (uint32_t i = tid; i < CLUSTER_GT; i += T) {
     M.gt_key[i] = 0xFFFFFFFFu;
     M.gt_cnt[i] = 0;
+    M.gt_lo[i] = ~0ull;
+    M.gt_hi[i] = 0ull;
   }
   if (tid < 2) M.gt_misc[tid] = 0;
   team_sync<T>();
   for (uint32_t i = tid; i < n; i += T) {
-    const uint32_t g = (uint32_t)(hits[i] >> DIAG_BITS);
+    const uint64_t k = hits[i];
+    const uint32_t g = (uint32_t)(k >> DIAG_BITS);
+    const unsigned long long d = k & ((1ull << DIAG_BITS) - 1);
     uint32_t sl = gt_slot(g);
     for (uint32_t probe = 0; probe < CLUSTER_GT; probe++) {
       const uint32_t prev = atomicCAS(&M.gt_key[sl], 0xFFFFFFFFu, g);
       if (prev == 0xFFFFFFFFu) atomicAdd(&M.gt_misc[0], 1u);
-      if (prev == 0xFFFFFFFFu || prev == g) {
-        atomicAdd(&M.gt_cnt[sl], 1u);
-        break;
-      }
+      if (prev == 0xFFFFFFFFu || prev == g) break;
       sl = (sl + 1) & (CLUSTER_GT - 1);
+    }
+    // the wave's hits mostly share one group (a bucket's entries are in group
+    // order): one update per distinct slot in the wave
+    const uint32_t sl0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sl);
+    const uint64_t act = __ballot(true);
+    if (act == ~0ull && __ballot(sl == sl0) == act) {
+      unsigned long long lo = d, hi = d;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, (unsigned long long)shfl_xor_i64((int64_t)lo, o));
+        hi = max(hi, (unsigned long long)shfl_xor_i64((int64_t)hi, o));
+      }
+      if (lane_id() == 0) {
+        atomicAdd(&M.gt_cnt[sl0], 64u);
+        atomicMin(&M.gt_lo[sl0], lo);
+        atomicMax(&M.gt_hi[sl0], hi);
+      }
+    } else {
+      atomicAdd(&M.gt_cnt[sl], 1u);
+      atomicMin(&M.gt_lo[sl], d);
+      atomicMax(&M.gt_hi[sl], d);
     }
   }
   team_sync<T>();
@@ -1007,48 +1142,115 @@ __device__ bool group_segment_sort(const uint64_t* hits, uint32_t n, const Clust
     M.idx[pos] = i;
   }
   team_sync<T>();
-  // segment boundaries (ranks 0..G) move from stage_val to the group table so
-  // stage_* can hold the chunks
+  // per group rank r (r < G <= 512, two per thread at most): segment start in
+  // gt_key[r] (gt_key[G] = n), group key in gt_cnt[r], lowest diagonal in
+  // gt_lo[r] and the diagonal range in gt_hi[r]; slot data read before the
+  // barrier, rank data written after it
+  uint64_t rl[2] = {0, 0}, rh[2] = {0, 0};
+  uint32_t rg[2] = {0, 0};
+  for (uint32_t u = 0; u < 2; u++) {
+    const uint32_t r = tid + u * T;
+    if (r < G) {
+      const uint32_t sl = (uint32_t)M.stage_key[r];
+      rg[u] = (uint32_t)(M.stage_key[r] >> 32);
+      rl[u] = M.gt_lo[sl];
+      rh[u] = M.gt_hi[sl] - M.gt_lo[sl];
+    }
+  }
+  team_sync<T>();
+  for (uint32_t u = 0; u < 2; u++) {
+    const uint32_t r = tid + u * T;
+    if (r < G) {
+      M.gt_cnt[r] = rg[u];
+      M.gt_lo[r] = rl[u];
+      M.gt_hi[r] = rh[u];
+    }
+  }
   for (uint32_t r = tid; r <= G; r += T) M.gt_key[r] = M.stage_val[r];
   team_sync<T>();
+  uint32_t* pre = M.gt_cnt + CLUSTER_GT_MAX;  // composite-key offsets of the current run
   uint32_t r = 0;
   while (r < G) {
-    const uint32_t a0 = M.gt_key[r];
-    uint32_t r1 = r + 1;
-    while (r1 < G && M.gt_key[r1 + 1] - a0 <= CLUSTER_STAGE) r1++;
-    const uint32_t cnt = M.gt_key[r1] - a0;
-    if (cnt <= CLUSTER_STAGE) {  // whole segments r..r1-1 in one LDS sort
-      uint32_t p2 = 1;
-      while (p2 < cnt) p2 <<= 1;
-      for (uint32_t i = tid; i < p2; i += T) {
-        M.stage_key[i] = i < cnt ? M.key[a0 + i] : ~0ull;
-        M.stage_val[i] = i < cnt ? M.idx[a0 + i] : 0u;
+    // the run r..r1-1: thread 0 lays the segments' ranges end to end
+    if (tid == 0) {
+      const uint32_t a0 = M.gt_key[r];
+      uint64_t span = 0;
+      uint32_t r1 = r;
+      while (r1 < G && M.gt_key[r1 + 1] - a0 <= RSORT_TILE && span + M.gt_hi[r1] + 1 < (1ull << 32)) {
+        pre[r1] = (uint32_t)span;
+        span += M.gt_hi[r1] + 1;
+        r1++;
       }
-      team_sync<T>();
-      team_bitonic<T>(M.stage_key, M.stage_val, p2, tid);
-      for (uint32_t i = tid; i < cnt; i += T) {
-        M.key[a0 + i] = M.stage_key[i];
-        M.idx[a0 + i] = M.stage_val[i];
-      }
-      team_sync<T>();
-    } else {  // one segment longer than a chunk: padded copy in cl_sum / cl_start
-      uint32_t p2 = 1;
-      while (p2 < cnt) p2 <<= 1;
-      uint64_t* tk = reinterpret_cast<uint64_t*>(M.cl_sum);
-      uint32_t* tv = M.cl_start;
-      for (uint32_t i = tid; i < p2; i += T) {
-        tk[i] = i < cnt ? M.key[a0 + i] : ~0ull;
-        tv[i] = i < cnt ? M.idx[a0 + i] : 0u;
-      }
-      team_sync<T>();
-      team_bitonic_staged<T, CLUSTER_STAGE>(tk, tv, p2, tid, M.stage_key, M.stage_val);
-      for (uint32_t i = tid; i < cnt; i += T) {
-        M.key[a0 + i] = tk[i];
-        M.idx[a0 + i] = tv[i];
-      }
-      team_sync<T>();
+      M.gt_misc[0] = r1;
+      M.gt_misc[1] = (uint32_t)(span ? 64 - __builtin_clzll(span) : 0);  // key bits
     }
-    r = r1;
+    team_sync<T>();
+    const uint32_t r1 = M.gt_misc[0], bits = M.gt_misc[1];
+    const uint32_t a0 = M.gt_key[r];
+    team_sync<T>();
+    if (r1 > r) {
+      const uint32_t cnt = M.gt_key[r1] - a0;
+      if (cnt > 1) {
+        if (cnt <= RSORT_SMALL * RSORT_T) run_radix_sort<RSORT_SMALL, RSortSmall>(M, a0, cnt, r, r1, (int)bits, tid);
+        else run_radix_sort<RSORT_BIG, RSortBig>(M, a0, cnt, r, r1, (int)bits, tid);
+      }
+      r = r1;
+      continue;
+    }
+    const uint32_t cnt = M.gt_key[r + 1] - a0;
+    uint64_t* tk = reinterpret_cast<uint64_t*>(M.cl_sum);
+    uint32_t* tv = M.cl_start;
+    if (M.gt_hi[r] + 1 < (1ull << 32)) {
+      // segment r alone exceeds a tile: each tile sorted on its own (a one-segment
+      // run), then pairs of sorted runs merged, alternating between key/idx and
+      // the cl_sum / cl_start scratch (both hold n entries)
+      if (tid == 0) {
+        pre[r] = 0;
+        const uint64_t span = M.gt_hi[r] + 1;
+        M.gt_misc[1] = (uint32_t)(64 - __builtin_clzll(span));
+      }
+      team_sync<T>();
+      const int sbits = (int)M.gt_misc[1];
+      for (uint32_t t0 = 0; t0 < cnt; t0 += RSORT_TILE)
+        run_radix_sort<RSORT_BIG, RSortBig>(M, a0 + t0, min(RSORT_TILE, cnt - t0), r, r + 1, sbits, tid);
+      bool in_key = true;
+      for (uint32_t w = RSORT_TILE; w < cnt; w <<= 1) {
+        const uint64_t* sk = in_key ? M.key : tk;
+        const uint32_t* sv = in_key ? M.idx : tv;
+        uint64_t* dk = in_key ? tk : M.key;
+        uint32_t* dv = in_key ? tv : M.idx;
+        for (uint32_t s0 = 0; s0 < cnt; s0 += 2 * w) {
+          const uint32_t na = min(w, cnt - s0), nb = min(w, cnt - s0 - na);
+          block_merge(sk, sv, dk, dv, a0 + s0, na, nb, tid);
+        }
+        team_sync<T>();
+        in_key = !in_key;
+      }
+      if (!in_key) {
+        for (uint32_t i = tid; i < cnt; i += T) {
+          M.key[a0 + i] = tk[a0 + i];
+          M.idx[a0 + i] = tv[a0 + i];
+        }
+        team_sync<T>();
+      }
+      r = r + 1;
+      continue;
+    }
+    // a segment whose diagonals span 2^32 or more: padded copy in cl_sum / cl_start
+    uint32_t p2 = 1;
+    while (p2 < cnt) p2 <<= 1;
+    for (uint32_t i = tid; i < p2; i += T) {
+      tk[i] = i < cnt ? M.key[a0 + i] : ~0ull;
+      tv[i] = i < cnt ? M.idx[a0 + i] : 0u;
+    }
+    team_sync<T>();
+    team_bitonic_staged<T, CLUSTER_STAGE>(tk, tv, p2, tid, M.stage_key, M.stage_val);
+    for (uint32_t i = tid; i < cnt; i += T) {
+      M.key[a0 + i] = tk[i];
+      M.idx[a0 + i] = tv[i];
+    }
+    team_sync<T>();
+    r = r + 1;
   }
   return true;
 }
@@ -1179,11 +1381,27 @@ __device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const
     int64_t psum = 0;
     uint32_t pmin = 0xFFFFFFFFu;
     uint32_t fnb = 0, fms = 0, fmn = 15;  // feature sums of the current run of a cluster
-    auto flush_feat = [&](int32_t c) {
-      if constexpr (FEAT) {
-        if (fnb) atomicAdd(&F.f1[c], fnb);
-        atomicAdd(&F.f2[c], fms);
-        atomicMin(&F.f4[c], fmn);
+    // A cluster that lies wholly inside this thread's chunk is written with plain
+    // stores; only the first (begun in an earlier chunk) and the last (it may run
+    // on into the next chunk) can have other contributors, and they add atomically
+    // to the zeroed sums.
+    bool shared = !(c0 == 0 || (M.key[c0] >> DIAG_BITS) != (kp >> DIAG_BITS) ||
+                    hit_diag(M.key[c0]) - hit_diag(kp) > MERGE_DISTANCE);
+    auto flush = [&](int32_t c, bool atomic) {
+      if (atomic) {
+        atomicAdd(&M.cl_sum[c], (unsigned long long)psum);
+        if constexpr (FEAT) {
+          if (fnb) atomicAdd(&F.f1[c], fnb);
+          atomicAdd(&F.f2[c], fms);
+          atomicMin(&F.f4[c], fmn);
+        }
+      } else {
+        M.cl_sum[c] = (unsigned long long)psum;
+        if constexpr (FEAT) {
+          F.f1[c] = fnb;
+          F.f2[c] = fms;
+          F.f4[c] = fmn;
+        }
       }
     };
     for (uint32_t e = c0; e < c1; e++) {
@@ -1197,10 +1415,8 @@ __device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const
         M.g_key[gid] = (uint32_t)(k >> DIAG_BITS);
       }
       if (cs) {
-        if (cid >= 0 && e > c0) {
-          atomicAdd(&M.cl_sum[cid], (unsigned long long)psum);
-          flush_feat(cid);
-        }
+        if (cid >= 0 && e > c0) flush(cid, shared);
+        shared = false;
         psum = 0;
         fnb = fms = 0;
         fmn = 15;
@@ -1219,8 +1435,10 @@ __device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const
       }
       kp = k;
     }
-    atomicAdd(&M.cl_sum[cid], (unsigned long long)psum);
-    flush_feat(cid);
+    // the last cluster ends at c1 unless the next chunk's first hit continues it
+    const bool ends = c1 == n || (M.key[c1] >> DIAG_BITS) != (kp >> DIAG_BITS) ||
+                      hit_diag(M.key[c1]) - hit_diag(kp) > MERGE_DISTANCE;
+    flush(cid, shared || !ends);
     atomicMin(&M.g_min[gid], pmin);
   }
   team_sync<T>();
@@ -1241,10 +1459,16 @@ __device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const
     }
     const uint32_t nwin = L >= (uint32_t)K ? L - K + 1 : 0;
     for (uint32_t x = tid; x < nwin; x += T) rep += F.rcode[x] != 0;
-    if (qs) atomicAdd(&F.cnt[1], qs);
-    if (q10) atomicAdd(&F.cnt[2], q10);
-    if (q20) atomicAdd(&F.cnt[3], q20);
-    if (rep) atomicAdd(&F.cnt[4], rep);
+    qs = group_sum_u32<64>(qs);
+    q10 = group_sum_u32<64>(q10);
+    q20 = group_sum_u32<64>(q20);
+    rep = group_sum_u32<64>(rep);
+    if (lane_id() == 0) {
+      if (qs) atomicAdd(&F.cnt[1], qs);
+      if (q10) atomicAdd(&F.cnt[2], q10);
+      if (q20) atomicAdd(&F.cnt[3], q20);
+      if (rep) atomicAdd(&F.cnt[4], rep);
+    }
   }
   team_sync<T>();
   // insertion order of the groups = order of first appearance in the hit list
@@ -1296,8 +1520,10 @@ __device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const
   }
   if constexpr (FEAT) {
     const uint64_t top = M.key[0] >> 42;
-    for (uint32_t c = tid; c < C; c += T)
-      if ((M.key[c] >> 42) == top) atomicAdd(&F.cnt[0], 1u);
+    uint32_t ties = 0;
+    for (uint32_t c = tid; c < C; c += T) ties += (M.key[c] >> 42) == top;
+    ties = group_sum_u32<64>(ties);
+    if (lane_id() == 0 && ties) atomicAdd(&F.cnt[0], ties);
     team_sync<T>();
     if (tid < nt) {
       const uint32_t c = M.idx[tid];
@@ -1431,6 +1657,8 @@ __global__ __launch_bounds__(256) void k_cluster_tier2(ClusterArgs a, const uint
   __shared__ uint64_t stage_key[CLUSTER_STAGE];
   __shared__ uint32_t stage_val[CLUSTER_STAGE];
   __shared__ uint32_t gt_key[CLUSTER_GT], gt_cnt[CLUSTER_GT], gt_misc[2 + 256 / 64];
+  __shared__ unsigned long long gt_lo[CLUSTER_GT], gt_hi[CLUSTER_GT];
+  __shared__ RSortLds rsort;
   for (uint32_t li = blockIdx.x; li < nlist; li += gridDim.x) {
     const uint32_t r = list[li];
     const uint32_t n = a.nhits[r];
@@ -1457,6 +1685,9 @@ __global__ __launch_bounds__(256) void k_cluster_tier2(ClusterArgs a, const uint
     M.gt_key = gt_key;
     M.gt_cnt = gt_cnt;
     M.gt_misc = gt_misc;
+    M.gt_lo = gt_lo;
+    M.gt_hi = gt_hi;
+    M.rsort = &rsort;
     ClusterFeat F{};
     if (FEAT) {
       const uint64_t so = a.R.seq_off[r], qo = a.R.qual_off[r];
@@ -3896,14 +4127,6 @@ __device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int l) {
   return ((uint64_t)hi << 32) | lo;
 }
 
-// The group of NL lanes (NL a power of two, 8..64, aligned in the wave) that
-// holds one read: every lane of it active, in the same control flow.
-template <int NL>
-__device__ __forceinline__ uint32_t group_sum_u32(uint32_t x) {
-#pragma unroll
-  for (int o = NL / 2; o > 0; o >>= 1) x += (uint32_t)__shfl_xor((int)x, o, 64);
-  return x;
-}
 
 // log_entropy with the per-base and per-word counting spread over the group:
 // integer counts (exact in any order), then the same FP64 tail.  The operation
