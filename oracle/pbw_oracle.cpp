@@ -1437,6 +1437,20 @@ struct Mapper {
     finalize(se, r, c[bi], mask, mapq, post);
     return to_record(c[bi], mapq, post);
   }
+  // paired.rs:346-351: the raw pairs kept, first occurrence of each pair of loci
+  static std::vector<size_t> dedup_pairs(const std::vector<std::pair<CandAln, CandAln>>& raw) {
+    std::vector<size_t> kept;
+    for (size_t i = 0; i < raw.size(); i++) {
+      bool dup = false;
+      for (size_t k : kept)
+        if (same_locus(raw[k].first, raw[i].first) && same_locus(raw[k].second, raw[i].second)) {
+          dup = true;
+          break;
+        }
+      if (!dup) kept.push_back(i);
+    }
+    return kept;
+  }
   // paired.rs:398-435 mapq_for_winning_pair
   void mapq_for_winning_pair(const Read& r1, const Read& r2, const CandAln& bx, const CandAln& by, double pair_se,
                              const std::vector<double>& lk1, const std::vector<double>& lk2,
@@ -1523,15 +1537,7 @@ struct Mapper {
       return;
     }
     std::vector<std::pair<CandAln, CandAln>> pairs;  // dedup :346-351
-    for (auto& pr : raw) {
-      bool dup = false;
-      for (auto& q : pairs)
-        if (same_locus(q.first, pr.first) && same_locus(q.second, pr.second)) {
-          dup = true;
-          break;
-        }
-      if (!dup) pairs.push_back(pr);
-    }
+    for (size_t i : dedup_pairs(raw)) pairs.push_back(raw[i]);
     std::vector<double> jl;
     for (auto& pr : pairs) jl.push_back(pr.first.log_lk + pr.second.log_lk + pair_distance_log_lk(pr.first, pr.second, im));
     size_t bi;
@@ -1931,6 +1937,49 @@ void orc_finalize_mapq(double sub_rate, uint64_t gsize, double sum_exp, const ui
   Read r{qb, qlen, rq, rql};
   mq.finalize_mapq(sum_exp, r, decode_cigar(cigar, n_ops), qb, qlen, qq, qqlen, rw, rwlen, mask, mlen, *mapq,
                    *posterior);
+}
+double orc_log_entropy(double sub_rate, uint64_t gsize, const uint32_t* cigar, uint32_t n_ops, const uint8_t* qb,
+                       uint64_t qlen, const uint8_t* qq, uint64_t qqlen, const uint8_t* rw, uint64_t rwlen) {
+  Mapq mq(sub_rate, 15, gsize);
+  return mq.log_entropy(decode_cigar(cigar, n_ops), qb, qlen, qq, qqlen, rw, rwlen);
+}
+uint64_t orc_shortlist_indices(const double* post, uint64_t n, uint64_t* out) {
+  std::vector<size_t> idx = Mapper::shortlist_indices(std::vector<double>(post, post + n));
+  for (size_t i = 0; i < idx.size(); i++) out[i] = idx[i];
+  return idx.size();
+}
+uint64_t orc_dedup_pairs(const uint64_t* loci, uint64_t n, uint64_t* out) {
+  std::vector<std::pair<CandAln, CandAln>> raw(n);
+  for (uint64_t i = 0; i < n; i++) {
+    const uint64_t* l = loci + 6 * i;  // contig, position, strand of mate 1, then of mate 2
+    raw[i].first.contig = l[0];
+    raw[i].first.position = l[1];
+    raw[i].first.strand = l[2] ? Reverse : Forward;
+    raw[i].second.contig = l[3];
+    raw[i].second.position = l[4];
+    raw[i].second.strand = l[5] ? Reverse : Forward;
+  }
+  std::vector<size_t> kept = Mapper::dedup_pairs(raw);
+  for (size_t i = 0; i < kept.size(); i++) out[i] = kept[i];
+  return kept.size();
+}
+void orc_mapq_for_winning_pair(const void* mp, double pair_se, int bxo, int byo, const orc_pair_mate* a,
+                               const orc_pair_mate* b, uint8_t* q, double* p) {
+  const Mapper* m = (const Mapper*)mp;
+  const orc_pair_mate* in[2] = {a, b};
+  Read r[2];
+  CandAln c[2];
+  std::vector<double> lk[2], mask[2];
+  for (int k = 0; k < 2; k++) {
+    r[k] = Read{in[k]->seq, in[k]->len, in[k]->qual, in[k]->qlen};
+    c[k].cigar = decode_cigar(in[k]->cigar, in[k]->n_ops);
+    c[k].ref_window.assign(in[k]->ref_window, in[k]->ref_window + in[k]->rwlen);
+    c[k].rc = false;
+    lk[k].assign(in[k]->lk, in[k]->lk + in[k]->nlk);
+    mask[k].assign(in[k]->mask, in[k]->mask + in[k]->mlen);
+  }
+  m->mapq_for_winning_pair(r[0], r[1], c[0], c[1], pair_se, lk[0], lk[1], mask[0], mask[1], bxo != 0, byo != 0, q[0],
+                           p[0], q[1], p[1]);
 }
 void orc_insert_update(double* mean, double* sd, uint64_t* n_obs, double observed) {
   InsertModel im{*mean, *sd, *n_obs};

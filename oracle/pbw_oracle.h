@@ -130,6 +130,32 @@ void orc_finalize_mapq(double sub_rate, uint64_t genome_size, double sum_exp,
                        const uint8_t* ref_window, uint64_t rwlen,
                        const double* repeat_mask, uint64_t mask_len,
                        uint8_t* mapq, double* posterior);
+/* bayesian.rs:306-373 log_entropy of one alignment (cigar words (len << 4) | op) */
+double orc_log_entropy(double sub_rate, uint64_t gsize, const uint32_t* cigar, uint32_t n_ops, const uint8_t* qb,
+                       uint64_t qlen, const uint8_t* qq, uint64_t qqlen, const uint8_t* rw, uint64_t rwlen);
+/* paired.rs:553-566 shortlist_indices; out holds up to n (at most 20) indices */
+uint64_t orc_shortlist_indices(const double* post, uint64_t n, uint64_t* out);
+/* paired.rs:346-351 dedup of raw pairs: loci are n x (contig, position, strand, contig, position, strand);
+ * out receives the kept indices in order */
+uint64_t orc_dedup_pairs(const uint64_t* loci, uint64_t n, uint64_t* out);
+/* paired.rs:398-435 mapq_for_winning_pair: one mate's read, winning alignment (forward orientation),
+ * own candidate log-likelihoods and repeat mask */
+typedef struct {
+  const uint8_t* seq;
+  uint64_t len;
+  const uint8_t* qual;
+  uint64_t qlen;
+  const uint32_t* cigar;
+  uint32_t n_ops;
+  const uint8_t* ref_window;
+  uint64_t rwlen;
+  const double* lk;
+  uint64_t nlk;
+  const double* mask;
+  uint64_t mlen;
+} orc_pair_mate;
+void orc_mapq_for_winning_pair(const void* mapper, double pair_se, int bxo, int byo, const orc_pair_mate* a,
+                               const orc_pair_mate* b, uint8_t* q, double* p);
 void orc_insert_update(double* mean, double* sd, uint64_t* n_obs, double observed);
 double orc_insert_log_likelihood(double mean, double sd, double insert_size);
 

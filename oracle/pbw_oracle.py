@@ -85,6 +85,12 @@ class _Parsed(C.Structure):
 _lib = None
 
 
+class _PairMate(C.Structure):
+    _fields_ = [("seq", C.c_char_p), ("len", C.c_uint64), ("qual", C.c_char_p), ("qlen", C.c_uint64),
+                ("cigar", C.c_void_p), ("n_ops", C.c_uint32), ("ref_window", C.c_char_p), ("rwlen", C.c_uint64),
+                ("lk", C.c_void_p), ("nlk", C.c_uint64), ("mask", C.c_void_p), ("mlen", C.c_uint64)]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -129,6 +135,11 @@ def lib():
                 [dbl, u64, dbl, C.c_char_p, u64, vp, u32, C.c_char_p, u64, C.c_char_p, u64, C.c_char_p, u64, vp, u64,
                  C.POINTER(C.c_uint8), C.POINTER(dbl)],
             ),
+            "orc_log_entropy": (dbl, [dbl, u64, vp, u32, C.c_char_p, u64, C.c_char_p, u64, C.c_char_p, u64]),
+            "orc_shortlist_indices": (u64, [vp, u64, vp]),
+            "orc_dedup_pairs": (u64, [vp, u64, vp]),
+            "orc_mapq_for_winning_pair": (None, [vp, dbl, i32, i32, C.POINTER(_PairMate), C.POINTER(_PairMate),
+                                                 vp, vp]),
             "orc_insert_update": (None, [C.POINTER(dbl), C.POINTER(dbl), C.POINTER(u64), dbl]),
             "orc_insert_log_likelihood": (dbl, [dbl, dbl, dbl]),
             "orc_default_options": (None, [C.POINTER(_Options)]),
@@ -325,6 +336,72 @@ def missing_locus_probability(qual, repeat_mask=(), sub_rate=0.001):
     mask = np.ascontiguousarray(np.asarray(repeat_mask, dtype=np.float64))
     return lib().orc_missing_locus_probability(sub_rate, qual, len(qual), mask.ctypes.data if len(mask) else None,
                                                len(mask))
+
+
+def _u32(words):
+    return np.ascontiguousarray(np.asarray(words, dtype=np.uint32).reshape(-1)) if len(words) else np.zeros(1, np.uint32)
+
+
+def _f64(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(-1)) if len(x) else np.zeros(1, np.float64)
+
+
+def log_entropy(cigar_words, query, qual, ref_window, genome_size, sub_rate=0.001):
+    """`MapqCalculator::log_entropy` (bayesian.rs:306-373); CIGAR words are (len << 4) | op."""
+    cig = _u32(cigar_words)
+    query, qual, ref_window = _b(query), _b(qual), _b(ref_window)
+    return lib().orc_log_entropy(sub_rate, genome_size, cig.ctypes.data, len(cigar_words), query, len(query), qual,
+                                 len(qual), ref_window, len(ref_window))
+
+
+def finalize_mapq(sum_exp, read_qual, cigar_words, query, qual, ref_window, genome_size, repeat_mask=(),
+                  sub_rate=0.001):
+    """`MapqCalculator::finalize_mapq` (bayesian.rs:251-282) -> (mapq, posterior)."""
+    cig, mask = _u32(cigar_words), _f64(repeat_mask)
+    read_qual, query, qual, ref_window = _b(read_qual), _b(query), _b(qual), _b(ref_window)
+    mq, post = C.c_uint8(), C.c_double()
+    lib().orc_finalize_mapq(sub_rate, genome_size, sum_exp, read_qual, len(read_qual), cig.ctypes.data,
+                            len(cigar_words), query, len(query), qual, len(qual), ref_window, len(ref_window),
+                            mask.ctypes.data if len(repeat_mask) else None, len(repeat_mask), C.byref(mq),
+                            C.byref(post))
+    return mq.value, post.value
+
+
+def shortlist_indices(post):
+    """`shortlist_indices` (paired.rs:553-566)."""
+    p = _f64(post)
+    out = np.zeros(max(1, len(post)), np.uint64)
+    n = lib().orc_shortlist_indices(p.ctypes.data, len(post), out.ctypes.data)
+    return [int(x) for x in out[:n]]
+
+
+def dedup_pairs(loci):
+    """The raw-pair dedup of joint_map (paired.rs:346-351): loci = [((contig, pos, strand), (contig, pos, strand))];
+    returns the kept indices."""
+    arr = np.ascontiguousarray(np.asarray([a + b for a, b in loci], dtype=np.uint64).reshape(-1))
+    out = np.zeros(max(1, len(loci)), np.uint64)
+    n = lib().orc_dedup_pairs(arr.ctypes.data if len(loci) else None, len(loci), out.ctypes.data)
+    return [int(x) for x in out[:n]]
+
+
+def mapq_for_winning_pair(mapper, pair_se, own1, own2, mate1, mate2):
+    """`mapq_for_winning_pair` (paired.rs:398-435) -> ((mapq1, posterior1), (mapq2, posterior2)).  Each mate
+    is a dict: seq, qual, cigar (words), ref_window, lk (its candidates' log-likelihoods), mask."""
+    keep = []
+
+    def mk(d):
+        cig, lk, mask = _u32(d["cigar"]), _f64(d["lk"]), _f64(d.get("mask", ()))
+        keep.extend([cig, lk, mask])
+        seq, qual, rw = _b(d["seq"]), _b(d["qual"]), _b(d["ref_window"])
+        return _PairMate(seq, len(seq), qual, len(qual), cig.ctypes.data, len(d["cigar"]), rw, len(rw),
+                         lk.ctypes.data, len(d["lk"]), mask.ctypes.data if len(d.get("mask", ())) else None,
+                         len(d.get("mask", ())))
+    a, b = mk(mate1), mk(mate2)
+    q = np.zeros(2, np.uint8)
+    p = np.zeros(2, np.float64)
+    lib().orc_mapq_for_winning_pair(mapper.h, pair_se, int(own1), int(own2), C.byref(a), C.byref(b), q.ctypes.data,
+                                    p.ctypes.data)
+    return (int(q[0]), float(p[0])), (int(q[1]), float(p[1]))
 
 
 def insert_update(mean, sd, n_obs, observed):

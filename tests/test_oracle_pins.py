@@ -366,3 +366,238 @@ def test_adapter_strip_and_sam_fields():  # tests/cli.rs:113-146
         line = [l for l in O.format_sam(g, rb, recs).decode().split("\n") if l.startswith("r1\t")][0]
         f = line.split("\t")
         assert f[3] == "41" and f[5] == "40M" and len(f[9]) == 40
+
+
+# ---------------------------------------------------------------------------
+# log_entropy (bayesian.rs:306-373): the random-match entropy from the counts
+# (n HQ aligned bases, m HQ mismatches, read length consumed, insertion and
+# deletion runs) with exact integer binomials.  The reference counts a "run"
+# at every change of operation type, a return to M after an indel included
+# (bayesian.rs:323-346), so 20M 2I 20M has two non-deletion runs; the counts
+# of each hand case below are derived from the code by hand and written out.
+# ---------------------------------------------------------------------------
+LN10 = math.log(10.0)
+
+
+def pyround(x):
+    return math.trunc(x + 0.5)
+
+
+def phred_choose_exact(n, m):
+    """py_int_round(10 log10 C(n, m)) from the exact binomial; 0 when the reduced m <= 0 (bayesian.rs:72-86)."""
+    m = n - m if m > n // 2 else m
+    if m <= 0:
+        return 0
+    return pyround(10.0 * math.log10(math.comb(n, m)))
+
+
+def entropy_from_counts(n, m, readlen, ins_runs, del_runs, window, gsize):
+    gc = sum(window.upper().count(c) for c in "GC")
+    total = gc + sum(window.upper().count(c) for c in "AT")
+    f = 0.5 if total == 0 else gc / total
+    f = min(max(f, 1e-6), 1.0 - 1e-6)
+    b = math.exp(-f * math.log(f / 2.0) - (1.0 - f) * math.log((1.0 - f) / 2.0))
+    match = pyround(10.0 * (n * math.log(b) - math.log(2.0 * gsize)) / LN10)
+    mis = pyround(10.0 * m * math.log(max(b - 1.0, 1e-6)) / LN10)
+    dterm = pyround(del_runs * 10.0 * math.log(30.0) / LN10) if del_runs else 0
+    return match - mis - phred_choose_exact(n, m) - phred_choose_exact(readlen, 2 * ins_runs + del_runs) - dterm
+
+
+def words(*ops):
+    code = {"M": 0, "I": 1, "D": 2, "S": 4}
+    return [(n << 4) | code[o] for n, o in ops]
+
+
+def _mut(s, at, to):
+    s = list(s)
+    for i in at:
+        s[i] = to if s[i] != to else ("C" if to == "A" else "A")
+    return "".join(s)
+
+
+def test_log_entropy_hand_cases_closed_form():
+    gsize = 2_000_000
+    w40 = Xorshift64(31).gen_seq(43)
+    ref40 = w40[:40]
+    q40s = bytes([40] * 40)
+    cases = []
+    # 40M perfect: n 40, m 0, readlen 40
+    cases.append((words((40, "M")), ref40, q40s, ref40, (40, 0, 40, 0, 0)))
+    # mismatches: HQ at 3, 10, 17; read N at 20 (not counted); Q5 mismatches at 25, 33 (not HQ: off n and m)
+    q = bytearray(q40s)
+    q[25] = q[33] = 5
+    rd = _mut(ref40, (3, 10, 17, 25, 33), "T")
+    rd = rd[:20] + "N" + rd[21:]
+    cases.append((words((40, "M")), rd, bytes(q), ref40, (38, 3, 40, 0, 0)))
+    # 20M 2I 20M: runs I (entry), M (return): two non-deletion runs; read length 42
+    ins = ref40[:20] + "GG" + ref40[20:]
+    cases.append((words((20, "M"), (2, "I"), (20, "M")), ins, bytes([40] * 42), ref40, (40, 0, 42, 2, 0)))
+    # 20M 3D 20M: D (entry, a deletion run) then M (return): one deletion run, one other run
+    dele = w40[:20] + w40[23:43]
+    cases.append((words((20, "M"), (3, "D"), (20, "M")), dele, q40s, w40, (40, 0, 40, 1, 1)))
+    # 10M 2D 2I 10M: D, I, M changes: one deletion run, two others; read 22, window 22
+    rd5 = ref40[:10] + "AC" + ref40[12:22]
+    cases.append((words((10, "M"), (2, "D"), (2, "I"), (10, "M")), rd5, bytes([40] * 22), ref40[:22], (20, 0, 22, 2, 1)))
+    # 2I 38M: a leading insertion is a change from the initial M, then the return to M
+    cases.append((words((2, "I"), (38, "M")), "TT" + ref40[:38], q40s, ref40[:38], (38, 0, 40, 2, 0)))
+    # window shorter than the alignment: M columns stop at the window's end (n 30, read consumed 30)
+    cases.append((words((40, "M")), ref40, q40s, ref40[:30], (30, 0, 30, 0, 0)))
+    # qualities shorter than the read: missing qualities read 0 (not HQ)
+    cases.append((words((40, "M")), ref40, bytes([40] * 25), ref40, (25, 0, 40, 0, 0)))
+    # GC-rich window (32 of 40 G/C): b < 4 moves the match and mismatch terms
+    gcw = "GCGCCGGCAT" * 4
+    gcr = _mut(gcw, (5, 15), "A")
+    cases.append((words((40, "M")), gcr, q40s, gcw, (40, 2, 40, 0, 0)))
+    # soft clips are skipped entirely
+    cases.append((words((5, "S"), (35, "M")), ref40[:35], q40s, ref40[:35], (35, 0, 35, 0, 0)))
+    seen = set()
+    for cig, rd, qual, win, counts in cases:
+        got = O.log_entropy(cig, rd, qual, win, gsize)
+        assert got == entropy_from_counts(*counts, win, gsize), (cig, counts, got)
+        seen.add(got)
+    assert len(seen) >= 8  # the cases move different terms
+
+
+def _entropy_counts_by_columns(cig, rd, qual, win):
+    """Counts from the expanded operation string: runs are the type changes of the I/D/M sequence
+    (starting from M); M columns count until the first that reaches past the read or the window."""
+    ops = "".join({0: "M", 1: "I", 2: "D"}.get(w & 15, "") * (w >> 4) for w in cig)
+    prev = ["M"] + list(ops[:-1]) if ops else []
+    changes = [o for o, p in zip(ops, prev) if o != p]
+    del_runs = changes.count("D")
+    r = p = n = m = 0
+    stopped = False
+    for o in ops:
+        if o == "I":
+            r += 1
+        elif o == "D":
+            p += 1
+        else:
+            if stopped or r >= len(rd) or p >= len(win):
+                stopped = True
+                continue
+            hq = (qual[r] if r < len(qual) else 0) >= 10
+            n += hq
+            m += hq and rd[r] != win[p] and rd[r] != "N" and win[p] != "N"
+            r += 1
+            p += 1
+    return n, m, r, len(changes) - del_runs, del_runs
+
+
+def test_log_entropy_random_alignments_match_closed_form():
+    rng = np.random.default_rng(17)
+    checked = 0
+    for _ in range(1500):
+        cig, L = [], 0
+        for _k in range(int(rng.integers(1, 6))):
+            op = int(rng.choice([0, 0, 0, 1, 2, 4]))
+            ln = int(rng.integers(1, 30))
+            cig.append((ln << 4) | op)
+            L += ln if op in (0, 1, 4) else 0
+        rd = "".join(rng.choice(list("ACGTN"), L, p=[0.24, 0.24, 0.24, 0.24, 0.04]))
+        win = "".join(rng.choice(list("ACGTN"), int(rng.integers(0, L + 20)), p=[0.3, 0.2, 0.2, 0.28, 0.02]))
+        qual = bytes(rng.integers(0, 42, int(rng.integers(0, L + 3))).tolist())
+        counts = _entropy_counts_by_columns(cig, rd, qual, win)
+        want = entropy_from_counts(*counts, win, 5_000_000)
+        assert O.log_entropy(cig, rd, qual, win, 5_000_000) == want, (cig, counts)
+        checked += 1
+    assert checked == 1500
+
+
+# ---------------------------------------------------------------------------
+# missing_locus_probability with a repeat mask (bayesian.rs:405-439): per
+# window a probabilistic OR of P(>= 2 errors in 15) and the mask at the
+# window's start, product over a phase's windows, mean over 5 phases
+# ---------------------------------------------------------------------------
+def test_missing_locus_with_repeat_mask_closed_form():
+    L = 50
+    e = 0.01  # Q20 with a substitution-rate Phred of 30: logadd(20, 30) = 20
+    p2 = 1.0 - (1.0 - e) ** 15 - 15 * e * (1.0 - e) ** 14
+    mask = np.zeros(L - 14)
+    mask[15], mask[31], mask[4], mask[19], mask[34] = 0.5, 1.0, 0.2, 0.9, 0.05
+    phases = []
+    for ph in range(5):
+        prod = 1.0
+        for start in range(ph, L - 15 + 1, 15):
+            prod *= min(max(1.0 - (1.0 - p2) * (1.0 - mask[start]), 1e-300), 1.0)
+        phases.append(prod)
+    want = min(sum(phases) / 5.0, 0.99)
+    got = O.missing_locus_probability(bytes([20] * L), mask)
+    assert abs(got - want) <= 1e-12 * want
+    assert abs(got - O.missing_locus_probability(bytes([20] * L))) > 1e-3 * want  # the mask matters
+    shifted = np.zeros(L - 14)
+    shifted[[16, 32, 5, 20, 35]] = mask[[15, 31, 4, 19, 34]]  # the mask one base later (window end-ish)
+    assert abs(O.missing_locus_probability(bytes([20] * L), shifted) - want) > 1e-3 * want
+
+
+# ---------------------------------------------------------------------------
+# shortlist_indices (paired.rs:553-566): the shortest prefix of the stable
+# descending order that is 20 long, or at least 3 long with mass >= 0.999
+# ---------------------------------------------------------------------------
+def test_shortlist_indices_brute_force():
+    rng = np.random.default_rng(5)
+    for trial in range(3000):
+        n = int(rng.integers(0, 41))
+        if trial % 3 == 0:  # ties
+            post = rng.choice([0.0, 0.1, 0.25, 0.5], n)
+        else:
+            post = rng.dirichlet(np.full(max(n, 1), float(rng.choice([0.05, 0.5, 5.0]))))[:n]
+        order = sorted(range(n), key=lambda i: (-post[i], i))
+        take = n
+        for t in range(n + 1):
+            if t == 20 or (t >= 3 and sum(post[order[k]] for k in range(t)) >= 0.999):
+                take = t
+                break
+        assert O.shortlist_indices(post) == order[:take], (trial, list(post))
+
+
+# ---------------------------------------------------------------------------
+# joint_map's dedup of raw pairs (paired.rs:346-351): first occurrence of each
+# (mate-1 locus, mate-2 locus), order kept
+# ---------------------------------------------------------------------------
+def test_pair_dedup_keeps_first_occurrence():
+    rng = np.random.default_rng(9)
+    for _ in range(500):
+        pool = [(int(rng.integers(0, 3)), int(rng.integers(0, 3)), int(rng.integers(0, 2))) for _ in range(4)]
+        raw = [(pool[int(rng.integers(0, 4))], pool[int(rng.integers(0, 4))]) for _ in range(int(rng.integers(0, 12)))]
+        first = {}
+        for i, pr in enumerate(raw):
+            first.setdefault(pr, i)
+        assert O.dedup_pairs(raw) == sorted(first.values())
+
+
+# ---------------------------------------------------------------------------
+# mapq_for_winning_pair (paired.rs:398-435): both mates own-best -> each its
+# own finalize; exactly one -> the anchor's (MAPQ, posterior) for both;
+# neither -> each finalized with the pair's sum of exponentials
+# ---------------------------------------------------------------------------
+def test_mapq_for_winning_pair_branches():
+    genome = Xorshift64(77).gen_seq(3000)
+    g = O.Genome().add_contig("c", genome)
+    mapper = O.Mapper(g, O.HashTable(g))
+    gsize = 3000
+
+    def mate(at, lk):
+        s = genome[at:at + 50]
+        return dict(seq=s, qual=bytes([30] * 50), cigar=words((50, "M")), ref_window=s, lk=lk, mask=())
+
+    a = mate(100, [-20.0, -45.0, -60.5])
+    b = mate(900, [-30.0, -31.0])
+
+    def fin(se, d):
+        return O.finalize_mapq(se, d["qual"], d["cigar"], d["seq"], d["qual"], d["ref_window"], gsize)
+
+    def se(lk):
+        mx = max(lk)
+        return sum(math.exp(x - mx) for x in lk)
+
+    pair_se = 3.0
+    want = {
+        (True, True): (fin(se(a["lk"]), a), fin(se(b["lk"]), b)),
+        (True, False): (fin(se(a["lk"]), a),) * 2,
+        (False, True): (fin(se(b["lk"]), b),) * 2,
+        (False, False): (fin(pair_se, a), fin(pair_se, b)),
+    }
+    for (oa, ob), w in want.items():
+        assert O.mapq_for_winning_pair(mapper, pair_se, oa, ob, a, b) == w, (oa, ob)
+    assert len(set(want.values())) == 4  # the branches give four different outcomes here
