@@ -457,7 +457,7 @@ def main():
         "kernel_ms_per_step": {k: round(agg["ms_k_" + k] / args.steps, 3) for k in ("seed", "cluster", "dp")},
         "counters": {k: int(agg[k]) for k in ("reads", "lookups", "positions", "filtered_hits", "dp_cells",
                                                "seed_overflow_reads", "tier2_cluster_reads", "band_dp_jobs",
-                                               "generic_dp_jobs")},
+                                               "generic_dp_jobs", "mapq_near_boundary")},
     }
     print(json.dumps(out), flush=True)
 

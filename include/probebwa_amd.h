@@ -133,6 +133,8 @@ typedef struct {
   double ms_pf;                 /* host wall time of scoring and selection */
   double ms_k_cluster;          /* HIP-event time of the clustering kernels (every tier) */
   uint64_t n_k_cluster;         /* clustering stages timed */
+  uint64_t mapq_near_boundary;  /* MAPQ finalizations whose integer MAPQ rests on a device exp/log/log10
+                                   value within 1e-9 of a rounding point (one ulp could move it) */
 } pbw_stats;
 
 /* ---- genome: GenomeIndexBuilder (src/index/genome.rs:101-147) ---- */

@@ -182,7 +182,8 @@ class Stats(C.Structure):
                                            "ms_k_seed", "ms_k_dp")] + \
                [(n, C.c_uint64) for n in ("n_k_seed", "n_k_dp", "pf_jobs", "pf_jobs_aligned", "pf_fallback_reads",
                                            "pf_audit_reads", "pf_audit_mismatches")] + \
-               [("ms_pf", C.c_double), ("ms_k_cluster", C.c_double), ("n_k_cluster", C.c_uint64)]
+               [("ms_pf", C.c_double), ("ms_k_cluster", C.c_double), ("n_k_cluster", C.c_uint64),
+                ("mapq_near_boundary", C.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -462,6 +463,9 @@ class Context:
 
         jobs: list of (read, qual, ref_seg, band, hint).  Returns a list of
         (score, cigar words, ref_start_offset)."""
+        return self._align(lib().pbw_align_batch, jobs, gap_open_phred, gap_extend_phred)
+
+    def _align(self, fn, jobs, gap_open_phred, gap_extend_phred):
         n = len(jobs)
         keep = []
         arr = (_AlignJob * max(n, 1))()
@@ -474,8 +478,7 @@ class Context:
             total += len(read) + len(ref) + 2
         res = (_AlignResult * max(n, 1))()
         cig = np.zeros(max(total, 1), np.uint32)
-        _check(lib().pbw_align_batch(self.h, gap_open_phred, gap_extend_phred, arr, n, res, cig.ctypes.data, len(cig)),
-               self.h)
+        _check(fn(self.h, gap_open_phred, gap_extend_phred, arr, n, res, cig.ctypes.data, len(cig)), self.h)
         return [(res[i].score, [int(x) for x in cig[res[i].cigar_off:res[i].cigar_off + res[i].cigar_n]],
                  res[i].ref_offset) for i in range(n)]
 

@@ -966,6 +966,7 @@ static void collect_counters(pbw_ctx* c, uint32_t reads) {
   c->stats.positions += ctr.positions;
   c->stats.filtered_hits += ctr.filtered;
   c->stats.dp_cells += ctr.cells;
+  c->stats.mapq_near_boundary += ctr.mapq_near;
   c->stats.reads += reads;
   HIP_CHECK(hipMemsetAsync(c->d_ctr.p, 0, sizeof(Counters), c->stream));
 }
@@ -987,6 +988,7 @@ static MapqArgs mapq_args(pbw_ctx* c, const Batch& b) {
   ma.rcode = c->d_rcode.p;
   ma.rec = c->d_rec.p;
   ma.best = c->d_best.p;
+  ma.ctr = c->d_ctr.p;
   return ma;
 }
 
@@ -1229,6 +1231,7 @@ static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, const
   HIP_CHECK(hipMemsetAsync(d_wn, 0, 4 * sizeof(uint32_t), s));
   c->d_wlist.ensure(nws);
   PairArgs pa{};
+  pa.ctr = c->d_ctr.p;
   pa.G = c->dev_genome();
   pa.R = b.R;
   pa.T = c->luts;

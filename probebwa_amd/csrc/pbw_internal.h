@@ -108,6 +108,7 @@ struct Cand {
 
 struct Counters {
   unsigned long long lookups, positions, filtered, cells, overflow;
+  unsigned long long mapq_near;  // MAPQ finalizations within 1e-9 of a rounding point (finalize_mapq_from)
 };
 
 }  // namespace pbw
@@ -225,6 +226,7 @@ struct MapqArgs {
   const uint32_t* keep;    // optional per read mask of the job slots to use (prefilter)
   const uint32_t* rlist;   // optional list of reads to finalize (nullptr = all)
   uint32_t nlist;
+  Counters* ctr;           // optional: counts near-boundary finalizations
 };
 void launch_mapq(const MapqArgs& a, hipStream_t s);
 
@@ -291,6 +293,7 @@ struct PairArgs {
   uint8_t* has_obs;
   uint32_t P;              // pairs handled by this launch: p_base .. p_base + P - 1 of the batch
   uint32_t p_base;
+  Counters* ctr;           // optional: counts near-boundary finalizations
 };
 void launch_pair_plan(const PairArgs& a, hipStream_t s);
 void launch_pair_finalize(const PairArgs& a, hipStream_t s);

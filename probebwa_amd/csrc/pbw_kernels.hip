@@ -4222,12 +4222,29 @@ __device__ double group_missing_locus(const DevLuts& T, const uint8_t* qual, uin
 // bayesian.rs:251-282 finalize_mapq, given its two long parts: log_ent =
 // log_entropy(..) and missing = missing_locus(..), the latter read only when
 // the entropy term is positive (as the reference computes it only then).
+// Whether the MAPQ rests on a value that a one-ulp difference between this
+// device's exp / log / log10 and the reference's libm could move across a
+// rounding point: x within 1e-9 of an integer.
+__device__ __forceinline__ bool near_integer(double x) { return fabs(x - rint(x)) < 1e-9; }
+// logadd's branch thresholds |a - b| = 10, 4, 2 (bayesian.rs:47-58), where a step of 1 sits.
+__device__ __forceinline__ bool near_logadd_step(double a, double b) {
+  const double d = fabs(a - b);
+  return fabs(d - 10.0) < 1e-9 || fabs(d - 4.0) < 1e-9 || fabs(d - 2.0) < 1e-9;
+}
+
+// finalize_mapq (bayesian.rs:251-282).  `near` reports whether the integer MAPQ
+// depends on a device-transcendental value within 1e-9 of a rounding point:
+// trunc(ln(p_wrong)/DB + 0.5), a logadd threshold against the continuous
+// not-found Phred, or round(pp).  (The entropy is exact: its logarithms come
+// from host tables.)
 __device__ void finalize_mapq_from(const DevLuts& T, double sum_exp, double log_ent, double missing, uint8_t& mapq,
-                                   double& posterior) {
+                                   double& posterior, bool& near) {
   double bp = 1.0 / fmax(sum_exp, 1.0);
   double pw = fmax(1.0 - bp, T.posterior_floor);
-  double wrongmapping = py_int_round(log(pw) / DB);
+  const double xw = log(pw) / DB;
+  double wrongmapping = py_int_round(xw);
   double entropy = log_ent - 3.0;
+  near = false;
   if (entropy <= 0.0) {
     mapq = 0;
     posterior = 0.0;
@@ -4237,6 +4254,8 @@ __device__ void finalize_mapq_from(const DevLuts& T, double sum_exp, double log_
   double notfound = -10.0 * log10(p_missing);
   double combined = logadd(wrongmapping, notfound);
   double pp = fmin(fmax(logadd(entropy, combined), 0.0), 99.0);
+  near = near_integer(xw + 0.5) || near_logadd_step(wrongmapping, notfound) || near_logadd_step(entropy, combined) ||
+         near_integer(pp - 0.5);
   mapq = (uint8_t)round(pp);
   posterior = 1.0 - pow(10.0, -pp / 10.0);
 }
@@ -4320,7 +4339,9 @@ __global__ void k_mapq(MapqArgs a) {
   if (gl) return;
   uint8_t mapq;
   double post;
-  finalize_mapq_from(a.T, se, le, miss, mapq, post);
+  bool near;
+  finalize_mapq_from(a.T, se, le, miss, mapq, post, near);
+  if (near && a.ctr) atomicAdd(&a.ctr->mapq_near, 1ull);
   const uint32_t contig = a.cands[job.cand].contig;
   o.pos = job.ref_gb - a.G.coff[contig] + dout.ref_off + 1;  // SAM is 1-based (single.rs:192)
   o.log_lk = mx;
