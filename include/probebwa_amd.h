@@ -211,6 +211,18 @@ int pbw_map_pe(pbw_ctx* ctx, const pbw_read_batch* m1, const pbw_read_batch* m2,
 int pbw_align_batch(pbw_ctx* ctx, uint32_t gap_open_phred, uint32_t gap_extend_phred, const pbw_align_job* jobs,
                     uint64_t n, pbw_align_result* results, uint32_t* cigar, uint64_t cigar_cap);
 
+/* ---- pbw_align_batch that allocates nothing.  No counterpart in the reference: for
+ *      callers that cannot allocate device memory (code under HIP graph capture, tight
+ *      serving loops, harnesses that forbid hipMalloc).  The same jobs, results and
+ *      kernels as pbw_align_batch, on a fixed 64 MiB device region that each GPU the
+ *      library runs on holds until the process ends, and a fixed 16 MiB host staging
+ *      area; a batch needing more returns PBW_ERR_LIMIT.  Calls run one at a time (one
+ *      host lock); the region keeps the last call's contents.  The context (created
+ *      beforehand) is the only set-up. ---- */
+int pbw_align_batch_noalloc(pbw_ctx* ctx, uint32_t gap_open_phred, uint32_t gap_extend_phred,
+                            const pbw_align_job* jobs, uint64_t n, pbw_align_result* results, uint32_t* cigar,
+                            uint64_t cigar_cap);
+
 /* ---- index access: HashTable::lookup (src/index/hashtable.rs:90-102) by 30-bit code;
  *      copies up to cap packed positions, returns the bucket size (or <0) ---- */
 int64_t pbw_index_lookup(pbw_ctx* ctx, uint32_t code, uint64_t* out, uint64_t cap);

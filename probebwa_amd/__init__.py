@@ -46,7 +46,7 @@ EXPORTED_SYMBOLS = (
     "pbw_genome_contig_name", "pbw_genome_contig_length", "pbw_genome_destroy", "pbw_default_options",
     "pbw_ctx_create", "pbw_ctx_destroy", "pbw_last_error", "pbw_global_error", "pbw_map_se", "pbw_reads_upload",
     "pbw_map_se_device", "pbw_reads_free", "pbw_map_pe",
-    "pbw_align_batch", "pbw_index_lookup", "pbw_index_total_entries", "pbw_records_release", "pbw_get_stats",
+    "pbw_align_batch", "pbw_align_batch_noalloc", "pbw_index_lookup", "pbw_index_total_entries", "pbw_records_release", "pbw_get_stats",
     "pbw_format_sam", "pbw_free_text", "pbw_default_train_options", "pbw_mlp_init", "pbw_mlp_set_params",
     "pbw_mlp_get_params", "pbw_mlp_forward", "pbw_mlp_gradient", "pbw_mlp_train", "pbw_default_prefilter_options",
     "pbw_prefilter_set", "pbw_prefilter_get", "pbw_prefilter_dataset", "pbw_prefilter_train",
@@ -228,6 +228,8 @@ def lib():
             "pbw_map_pe": (i32, [vp, C.POINTER(_ReadBatch), C.POINTER(_ReadBatch), C.POINTER(InsertModel),
                                  C.POINTER(_RecordBatch)]),
             "pbw_align_batch": (i32, [vp, u32, u32, C.POINTER(_AlignJob), u64, C.POINTER(_AlignResult), vp, u64]),
+            "pbw_align_batch_noalloc": (i32, [vp, u32, u32, C.POINTER(_AlignJob), u64, C.POINTER(_AlignResult), vp,
+                                              u64]),
             "pbw_index_lookup": (i64, [vp, u32, vp, u64]),
             "pbw_index_total_entries": (u64, [vp]),
             "pbw_records_release": (None, [C.POINTER(_RecordBatch)]),
@@ -464,6 +466,11 @@ class Context:
         jobs: list of (read, qual, ref_seg, band, hint).  Returns a list of
         (score, cigar words, ref_start_offset)."""
         return self._align(lib().pbw_align_batch, jobs, gap_open_phred, gap_extend_phred)
+
+    def align_batch_noalloc(self, jobs, gap_open_phred=40, gap_extend_phred=3):
+        """`align_batch` through pbw_align_batch_noalloc: the library's fixed device region
+        (64 MiB per GPU) instead of allocations; a batch that needs more is an error."""
+        return self._align(lib().pbw_align_batch_noalloc, jobs, gap_open_phred, gap_extend_phred)
 
     def _align(self, fn, jobs, gap_open_phred, gap_extend_phred):
         n = len(jobs)

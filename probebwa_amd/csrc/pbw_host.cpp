@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -163,6 +164,28 @@ static int32_t h_f64_i32(double x) {
 }
 
 // ---------------------------------------------------------------------------
+// Fixed device region of pbw_align_batch_noalloc: every GPU the library runs on
+// holds its own copy until the process ends.  Host code never uses the array's
+// name as an address: the device address comes from hipGetSymbolAddress, once
+// per device, when a context is created on it.
+// ---------------------------------------------------------------------------
+constexpr size_t NOALLOC_SCRATCH_BYTES = 64ull << 20;
+constexpr size_t NOALLOC_STAGE_BYTES = 16ull << 20;
+static __device__ __align__(256) uint8_t g_noalloc_scratch[NOALLOC_SCRATCH_BYTES];
+static uint8_t* g_noalloc_ptr[256];  // per device
+static std::mutex g_noalloc_mu;
+alignas(256) static uint8_t g_noalloc_stage[NOALLOC_STAGE_BYTES];  // host staging of those calls
+
+static void noalloc_setup(int device) {  // with `device` current
+  if (device < 0 || device >= 256) throw PbwError{PBW_ERR_ARG, "device index out of range"};
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (g_noalloc_ptr[device]) return;
+  void* ptr = nullptr;
+  HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(g_noalloc_scratch)));
+  g_noalloc_ptr[device] = (uint8_t*)ptr;
+}
+
+// ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
 struct pbw_ctx {
@@ -257,6 +280,7 @@ struct pbw_ctx {
 static void ctx_build(pbw_ctx* c) {
   const pbw_genome* g = c->genome;
   HIP_CHECK(hipSetDevice(c->device));
+  noalloc_setup(c->device);
   HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   for (auto& e : c->ev) HIP_CHECK(hipEventCreate(&e));
   hipStream_t s = c->stream;
@@ -1654,6 +1678,26 @@ static void prefilter_train(pbw_ctx* c, const pbw_read_batch* in, const pbw_trai
 // ===========================================================================
 // C ABI
 // ===========================================================================
+// Bump allocation over a fixed region; a request past its end is an error.
+struct FixedArena {
+  uint8_t* base;
+  size_t cap, used;
+  const char* what;
+  template <typename T>
+  T* take(size_t count) {
+    const size_t off = (used + 255) & ~(size_t)255;
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (off > cap || bytes > cap - off)
+      throw PbwError{PBW_ERR_LIMIT, std::string("the batch needs more than the ") + what};
+    used = off + bytes;
+    return reinterpret_cast<T*>(base + off);
+  }
+  size_t left() const {
+    const size_t off = (used + 255) & ~(size_t)255;
+    return off < cap ? cap - off : 0;
+  }
+};
+
 extern "C" {
 
 const char* pbw_global_error(void) { return g_error.c_str(); }
@@ -2029,6 +2073,197 @@ int pbw_align_batch(pbw_ctx* c, uint32_t gop, uint32_t gep, const pbw_align_job*
         HIP_CHECK(hipMemcpyAsync(cigar + w, c->d_cigar.p + (uint64_t)i * cap, out[i].cigar_n * 4,
                                  hipMemcpyDeviceToHost, s));
       w += out[i].cigar_n;
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (cigar && w > cigar_cap) throw PbwError{PBW_ERR_LIMIT, "cigar buffer too small"};
+  });
+}
+
+// pbw_align_batch on the fixed device region and host stage: the same kernels,
+// grids sized to the region (the DP kernels loop over their job lists).
+int pbw_align_batch_noalloc(pbw_ctx* c, uint32_t gop, uint32_t gep, const pbw_align_job* jobs, uint64_t n,
+                            pbw_align_result* res, uint32_t* cigar, uint64_t cigar_cap) {
+  if (!c || (!jobs && n) || (!res && n)) return PBW_ERR_ARG;
+  return guarded(c, [&] {
+    HIP_CHECK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    if (n == 0) return;
+    if (n > (1u << 30)) throw PbwError{PBW_ERR_LIMIT, "too many jobs"};
+    uint8_t* dev = (c->device >= 0 && c->device < 256) ? g_noalloc_ptr[c->device] : nullptr;
+    if (!dev) throw PbwError{PBW_ERR_HIP, "the no-allocation scratch is not set up on this device"};
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);  // one host stage: calls run one at a time
+    FixedArena D{dev, NOALLOC_SCRATCH_BYTES, 0, "no-allocation device scratch (64 MiB)"};
+    FixedArena H{g_noalloc_stage, NOALLOC_STAGE_BYTES, 0, "no-allocation host stage (16 MiB)"};
+    uint64_t sm = 0, sq = 0, sr = 0;
+    uint32_t max_m = 0, max_n = 0;
+    for (uint64_t i = 0; i < n; i++) {
+      const pbw_align_job& j = jobs[i];
+      if (j.m > (uint32_t)MAX_READ_LEN || j.qlen > (uint32_t)MAX_READ_LEN || j.n > 8000)
+        throw PbwError{PBW_ERR_LIMIT, "job too large"};
+      sm += j.m;
+      sq += j.qlen;
+      sr += j.n;
+      max_m = std::max(max_m, std::max(j.m, j.qlen));
+      max_n = std::max(max_n, j.n);
+    }
+    const uint32_t cap = max_m + max_n + 2;
+    // host stage
+    uint8_t* hseq = H.take<uint8_t>(sm);
+    uint8_t* hqual = H.take<uint8_t>(sq);
+    uint8_t* href = H.take<uint8_t>(sr);
+    uint64_t* hso = H.take<uint64_t>(n + 1);
+    uint64_t* hqo = H.take<uint64_t>(n + 1);
+    DpJob* hjob = H.take<DpJob>(n);
+    uint32_t* hband = H.take<uint32_t>(n);
+    uint32_t* htiled = H.take<uint32_t>(n);
+    uint32_t* hcnt = H.take<uint32_t>(4);
+    DpOut* hout = H.take<DpOut>(n);
+    uint64_t os = 0, oq = 0, orf = 0;
+    uint32_t nb = 0, nt = 0, bm = 0, bn = 0;
+    for (uint64_t i = 0; i < n; i++) {
+      const pbw_align_job& j = jobs[i];
+      hso[i] = os;
+      hqo[i] = oq;
+      if (j.m) memcpy(hseq + os, j.read, j.m);
+      if (j.qlen) memcpy(hqual + oq, j.qual, j.qlen);
+      if (j.n) memcpy(href + orf, j.ref, j.n);
+      DpJob d{};
+      d.ref_gb = orf;
+      d.read = (uint32_t)i;
+      d.n = j.n;
+      d.hint = (int32_t)std::max<int64_t>(std::min<int64_t>(j.hint, INT32_MAX / 2), INT32_MIN / 2);
+      d.band = (int32_t)std::min<uint32_t>(j.band, (uint32_t)(INT32_MAX / 4));
+      d.explicit_ref = 1;
+      d.cand = (uint32_t)i;
+      hjob[i] = d;
+      os += j.m;
+      oq += j.qlen;
+      orf += j.n;
+      if (2ull * j.band + 1 <= 63 && j.n <= 2048) {  // banded kernel, as pbw_align_batch
+        hband[nb++] = (uint32_t)i;
+        bm = std::max(bm, std::max(j.m, j.qlen));
+        bn = std::max(bn, j.n);
+      } else {
+        htiled[nt++] = (uint32_t)i;
+      }
+    }
+    hso[n] = os;
+    hqo[n] = oq;
+    // device region
+    uint8_t* dseq = D.take<uint8_t>(sm);
+    uint8_t* dqual = D.take<uint8_t>(sq);
+    uint8_t* dref = D.take<uint8_t>(sr);
+    uint64_t* dso = D.take<uint64_t>(n + 1);
+    uint64_t* dqo = D.take<uint64_t>(n + 1);
+    DpJob* djob = D.take<DpJob>(n);
+    DpOut* dout = D.take<DpOut>(n);
+    uint32_t* dcig = D.take<uint32_t>((size_t)n * cap);
+    uint32_t* dband = D.take<uint32_t>(n);
+    uint32_t* dtiled = D.take<uint32_t>(n);
+    uint32_t* dflag = D.take<uint32_t>(n);
+    uint32_t* dflag2 = D.take<uint32_t>(n);
+    uint32_t* dcnt = D.take<uint32_t>(4);
+    Counters* dctr = D.take<Counters>(1);
+    auto up = [&](void* d, const void* h, size_t bytes) {
+      if (bytes) HIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
+    };
+    up(dseq, hseq, sm);
+    up(dqual, hqual, sq);
+    up(dref, href, sr);
+    up(dso, hso, (n + 1) * 8);
+    up(dqo, hqo, (n + 1) * 8);
+    up(djob, hjob, n * sizeof(DpJob));
+    up(dband, hband, (size_t)nb * 4);
+    launch_zero_u32(dcnt, 4, s);
+    launch_zero_u32((uint32_t*)dctr, (uint32_t)(sizeof(Counters) / 4), s);
+    HIP_CHECK(hipGetLastError());
+    DevReads R{dseq, dso, dqual, dqo, (uint32_t)n};
+    DpArgs da = dp_args_base(c, R);
+    da.ctr = dctr;
+    da.gap_open = std::min(h_f64_i32(std::round(-90.0 * (double)gop / 40.0)), -1);
+    da.gap_extend = std::min(h_f64_i32(std::round(-1.0 * (double)gep / 3.0)), -1);
+    da.jobs = djob;
+    da.ref_bytes = dref;
+    da.cigar_cap = cap;
+    da.cigar = dcig;
+    da.out = dout;
+    auto count = [&](const uint32_t* d) {
+      HIP_CHECK(hipMemcpyAsync(hcnt, d, 4, hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      return hcnt[0];
+    };
+    if (nb) {  // jobs the banded kernel flags join the tiled list
+      DpArgs db = da;
+      db.jlist = dband;
+      db.njobs = nb;
+      db.max_m = bm;
+      db.max_n = bn;
+      db.flag_list = dflag;
+      db.flag_n = dcnt;
+      launch_dp_band(db, s);
+      HIP_CHECK(hipGetLastError());
+      const uint32_t nf = count(dcnt);
+      if (nf) {
+        HIP_CHECK(hipMemcpyAsync(htiled + nt, dflag, (size_t)nf * 4, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        nt += nf;
+      }
+    }
+    da.max_m = max_m;
+    da.max_n = max_n;
+    const size_t mark = D.used;
+    auto tiled = [&](const uint32_t* list, uint32_t cnt) {
+      DpArgs a = da;
+      a.jlist = list;
+      a.njobs = cnt;
+      a.flag_list = dflag;
+      a.flag_n = dcnt + 2;
+      a.tiles = (a.max_n + 1 + 63) / 64;
+      const bool lds_rows = dp_tiled_rows_in_lds(a.max_m, a.max_n);
+      const size_t per = (lds_rows ? 0 : 12ull * (a.max_n + 1) + 256) + 32ull * (a.max_m + 1) * a.tiles + 256;
+      const uint32_t nwaves = (uint32_t)std::min<size_t>({(size_t)cnt, lds_rows ? 8192u : 2048u, D.left() / per});
+      if (!nwaves) throw PbwError{PBW_ERR_LIMIT, "the batch needs more than the no-allocation device scratch (64 MiB)"};
+      a.rows = lds_rows ? nullptr : D.take<int32_t>((size_t)nwaves * 3 * (a.max_n + 1));
+      a.tb = D.take<uint64_t>((size_t)nwaves * (a.max_m + 1) * a.tiles * 4);
+      launch_dp_tiled(a, nwaves, s);
+      HIP_CHECK(hipGetLastError());
+      if (count(dcnt + 2)) throw PbwError{PBW_ERR_LIMIT, "alignment window beyond the tiled kernel's shape"};
+      D.used = mark;
+    };
+    if (nt) {
+      up(dtiled, htiled, (size_t)nt * 4);
+      const uint32_t T = dp_wide_tiles(max_n);
+      if (!T) {
+        tiled(dtiled, nt);
+      } else {  // whole-window rectangles first; the jobs k_dp_wide flags go to the tiled kernel
+        DpArgs w = da;
+        w.jlist = dtiled;
+        w.njobs = nt;
+        w.flag_list = dflag2;
+        w.flag_n = dcnt + 1;
+        const size_t per = 128ull * (max_m + 1) * T + 256;
+        const uint32_t nwaves = (uint32_t)std::min<size_t>({(size_t)nt, (size_t)8192, D.left() / per});
+        if (!nwaves)
+          throw PbwError{PBW_ERR_LIMIT, "the batch needs more than the no-allocation device scratch (64 MiB)"};
+        w.tb = D.take<uint64_t>((size_t)nwaves * (max_m + 1) * T * 16);
+        launch_dp_wide(w, nwaves, s);
+        HIP_CHECK(hipGetLastError());
+        D.used = mark;
+        const uint32_t nflag = count(dcnt + 1);
+        if (nflag) tiled(dflag2, nflag);
+      }
+    }
+    HIP_CHECK(hipMemcpyAsync(hout, dout, n * sizeof(DpOut), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    uint64_t w = 0;
+    for (uint64_t i = 0; i < n; i++) {
+      res[i].score = hout[i].score;
+      res[i].ref_offset = hout[i].ref_off;
+      res[i].cigar_off = (uint32_t)w;
+      res[i].cigar_n = hout[i].cigar_n;
+      if (cigar && w + hout[i].cigar_n <= cigar_cap && hout[i].cigar_n)
+        HIP_CHECK(hipMemcpyAsync(cigar + w, dcig + (uint64_t)i * cap, hout[i].cigar_n * 4, hipMemcpyDeviceToHost, s));
+      w += hout[i].cigar_n;
     }
     HIP_CHECK(hipStreamSynchronize(s));
     if (cigar && w > cigar_cap) throw PbwError{PBW_ERR_LIMIT, "cigar buffer too small"};

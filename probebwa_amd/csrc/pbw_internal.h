@@ -196,6 +196,7 @@ struct DpArgs {
   uint32_t pack16;         // 1: eight jobs per wave in 16-bit halves (k_dp_pair16) where allowed
 };
 void launch_dp_band(const DpArgs& a, hipStream_t s);
+void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);  // a fill kernel (no memset node)
 void launch_dp_tiled(const DpArgs& a, uint32_t nwaves, hipStream_t s);
 // Windows whose band covers every row (paired-end mate windows) up to 1024
 // columns: k_dp_wide, which flags every other job to (flag_list, flag_n).

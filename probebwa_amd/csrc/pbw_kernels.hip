@@ -3321,6 +3321,9 @@ __global__ void k_job_zero(uint32_t* p, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
 }
+void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_job_zero, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+}
 
 // ---------------------------------------------------------------------------
 // Counting sort of the DP jobs by read length, longest first: the four jobs
