@@ -72,9 +72,14 @@ def _profile_rows(kind, workload, batch, ncols):
 
 def measured_traffic(kernel, batch, workload):
     """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, copy-calibrated) from the
-    committed rocprofv3 counter summary of this workload at the default batch; else None."""
-    f = _profile_rows("traffic", workload, batch, 7).get(kernel)
-    return int(float(f[5])) if f else None
+    committed rocprofv3 counter summary of this workload at the default batch (the last
+    dispatch of that name: the one of the timed step); else None.  `kernel` may be a tuple:
+    the sum over those kernels."""
+    rows = _profile_rows("traffic", workload, batch, 7)
+    names = kernel if isinstance(kernel, tuple) else (kernel,)
+    if not all(k in rows for k in names):
+        return None
+    return sum(int(float(rows[k][5])) for k in names)
 
 
 def measured_valu(kernel, batch, workload):
@@ -185,7 +190,8 @@ def stage_rooflines(agg, B, workload):
                            "unit": "8 B per lookup + 8 B per position + 8 B per filtered hit"}}
     cluster = {"kernel": "k_cluster (all tiers)", "bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBPS,
                "achieved": round(cl_bytes / (ms_cl * 1e-3) / 1e9, 2) if ms_cl > 0 else 0.0,
-               "traffic": measured_traffic("pbw::k_cluster_tier2", B, workload),
+               "traffic": measured_traffic(("pbw::k_cluster<false, 128u>", "pbw::k_cluster<false, 512u>",
+                                            "pbw::k_cluster_tier2<false>"), B, workload),
                "per_launch": {"bytes": cl_bytes, "ms": ms_cl,
                               "unit": "8 B per hit key read + 321 B of candidates per read written"}}
     ach = cells / (ms_dp * 1e-3) * DP_OPS_PER_CELL / 1e12 if ms_dp > 0 else 0.0
