@@ -190,7 +190,7 @@ def stage_rooflines(agg, B, workload):
                            "unit": "8 B per lookup + 8 B per position + 8 B per filtered hit"}}
     cluster = {"kernel": "k_cluster (all tiers)", "bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBPS,
                "achieved": round(cl_bytes / (ms_cl * 1e-3) / 1e9, 2) if ms_cl > 0 else 0.0,
-               "traffic": measured_traffic(("pbw::k_cluster<false, 128u>", "pbw::k_cluster<false, 512u>",
+               "traffic": measured_traffic(("pbw::k_cluster<false 128u>", "pbw::k_cluster<false 512u>",
                                             "pbw::k_cluster_tier2<false>"), B, workload),
                "per_launch": {"bytes": cl_bytes, "ms": ms_cl,
                               "unit": "8 B per hit key read + 321 B of candidates per read written"}}
