@@ -201,6 +201,7 @@ struct pbw_ctx {
   DevBuf<uint32_t> d_koff;
   DevBuf<uint64_t> d_pres;
   DevBuf<uint64_t> d_kpos;
+  DevBuf<uint16_t> d_kfp;
   uint64_t total_entries = 0;
   // LUTs
   DevBuf<double> d_lut;  // packed double tables
@@ -311,13 +312,17 @@ static void ctx_build(pbw_ctx* c) {
   c->d_koff.ensure(CODE_SPACE + 1);
   {
     uint64_t* kpos = nullptr;
+    uint16_t* kfp = nullptr;
     uint64_t nent = 0;
     std::string err;
-    if (gpu_build_index(c->dev_genome(), g->lengths.data(), c->d_koff.p, &kpos, &nent, s, err) != hipSuccess)
+    if (gpu_build_index(c->dev_genome(), g->lengths.data(), c->d_koff.p, &kpos, &kfp, &nent, s, err) != hipSuccess)
       throw PbwError{err.find("2^32") != std::string::npos ? PBW_ERR_LIMIT : PBW_ERR_HIP, err};
     c->d_kpos.release();
     c->d_kpos.p = kpos;
     c->d_kpos.n = std::max<uint64_t>(nent, 1);
+    c->d_kfp.release();
+    c->d_kfp.p = kfp;
+    c->d_kfp.n = std::max<uint64_t>(nent, 1);
     c->total_entries = nent;
   }
   c->d_pres.ensure(3 * CODE_SPACE / 64);  // natural, rotated by 10, rotated by 20
@@ -650,7 +655,7 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
   SeedArgs sa{};
   sa.G = c->dev_genome();
   sa.X = DevIndex{c->d_koff.p, c->d_kpos.p, c->d_pres.p, c->d_pres.p + CODE_SPACE / 64,
-                  c->d_pres.p + 2 * CODE_SPACE / 64};
+                  c->d_pres.p + 2 * CODE_SPACE / 64, c->d_kfp.p};
   sa.R = R;
   sa.rlist = nullptr;
   sa.nlist = B;

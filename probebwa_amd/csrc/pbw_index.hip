@@ -21,7 +21,8 @@
 //      (hashtable.rs:270-273, 286-289), i.e. it keeps the first 300 of the scan;
 //   4. an in-place exclusive scan of koff over all 4^15 codes gives the
 //      bucket starts (bucket c = [koff[c], koff[c+1]));
-//   5. the first 300 entries of each run are decoded from j and placed.
+//   5. the first 300 entries of each run are decoded from j and placed, each
+//      with the genome flank fingerprint the seed filter compares against.
 // Temporaries are sized by the number of windows, not by the code space.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -120,12 +121,39 @@ __global__ void k_idx_counts(const uint32_t* __restrict__ key, uint64_t n, const
   }
 }
 
+// Genome side of the seed similarity filter (seeds.rs:162-219) for a window at
+// `pos` of contig c: the A/C/G/T counts (4 bits each, A lowest) of the 4 bases
+// before and the 8 after the 15-mer, N not counted; GENOME_FP_OPEN when those
+// flanks run past the contig (the filter then passes every hit).
+__device__ __forceinline__ uint16_t genome_flank_fp(const uint64_t* g2b, const uint64_t* nmask, uint64_t coff,
+                                                    uint64_t clen, uint64_t pos) {
+  if (pos < 4 || pos + 23 > clen) return GENOME_FP_OPEN;
+  const uint64_t gb = coff + pos - 4;  // bases gb .. gb + 26
+  const uint64_t w0 = gb >> 5;
+  const uint32_t s = (uint32_t)(gb & 31) * 2;
+  const uint64_t bits = s ? ((g2b[w0] >> s) | (g2b[w0 + 1] << (64 - s))) : g2b[w0];
+  const uint64_t n0 = gb >> 6;
+  const uint32_t ns = (uint32_t)(gb & 63);
+  const uint64_t nb = ns ? ((nmask[n0] >> ns) | (nmask[n0 + 1] << (64 - ns))) : nmask[n0];
+  uint32_t fp = 0;
+#pragma unroll
+  for (int t = 0; t < 27; t++) {
+    if (t >= 4 && t < 19) continue;
+    if ((nb >> t) & 1ull) continue;
+    fp += 1u << (4 * ((bits >> (2 * t)) & 3));
+  }
+  return (uint16_t)fp;
+}
+
 // The first MAX_COUNT entries of every run, decoded to packed positions
-// (pack_position, hashtable.rs:157-164: bit 63 strand, 62..40 contig, 39..0 position).
+// (pack_position, hashtable.rs:157-164: bit 63 strand, 62..40 contig, 39..0 position),
+// with each entry's genome flank fingerprint beside it.
 __global__ void k_idx_place(const uint32_t* __restrict__ key, const uint32_t* __restrict__ val, uint64_t n,
                             const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ koff,
-                            const uint64_t* __restrict__ clen, const uint64_t* __restrict__ cbase, int32_t nctg,
-                            uint64_t* __restrict__ kpos) {
+                            const uint64_t* __restrict__ g2b, const uint64_t* __restrict__ nmask,
+                            const uint64_t* __restrict__ coff, const uint64_t* __restrict__ clen,
+                            const uint64_t* __restrict__ cbase, int32_t nctg, uint64_t* __restrict__ kpos,
+                            uint16_t* __restrict__ kfp) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const uint32_t k = key[i];
@@ -137,7 +165,9 @@ __global__ void k_idx_place(const uint32_t* __restrict__ key, const uint32_t* __
     uint64_t pos;
     uint32_t strand;
     slot_window(j - cbase[c], clen[c], pos, strand);
-    kpos[(uint64_t)koff[k] + r] = ((uint64_t)strand << 63) | ((uint64_t)c << 40) | pos;
+    const uint64_t at = (uint64_t)koff[k] + r;
+    kpos[at] = ((uint64_t)strand << 63) | ((uint64_t)c << 40) | pos;
+    kfp[at] = genome_flank_fp(g2b, nmask, coff[c], clen[c], pos);
   }
 }
 
@@ -274,8 +304,9 @@ struct Tmp {
   } while (0)
 
 hipError_t gpu_build_index(const DevGenome& G, const uint64_t* h_clen, uint32_t* koff, uint64_t** kpos_out,
-                           uint64_t* n_entries, hipStream_t s, std::string& err) {
+                           uint16_t** kfp_out, uint64_t* n_entries, hipStream_t s, std::string& err) {
   *kpos_out = nullptr;
+  *kfp_out = nullptr;
   *n_entries = 0;
   const int32_t nctg = G.ncontigs;
   std::vector<uint64_t> cbase(nctg + 1, 0);
@@ -298,6 +329,7 @@ hipError_t gpu_build_index(const DevGenome& G, const uint64_t* h_clen, uint32_t*
   IDX_CHECK(hipMalloc(&t_part.p, (size_t)std::max(np_k, np_n) * sizeof(uint32_t)));
   uint32_t* part = (uint32_t*)t_part.p;
   uint64_t* kpos = nullptr;
+  uint16_t* kfp = nullptr;
   uint32_t total = 0;
   if (n) {
     IDX_CHECK(hipMalloc(&t_cbase.p, (nctg + 1) * sizeof(uint64_t)));
@@ -331,18 +363,23 @@ hipError_t gpu_build_index(const DevGenome& G, const uint64_t* h_clen, uint32_t*
     IDX_CHECK(hipMemcpyAsync(&total, koff + CODE_SPACE, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     IDX_CHECK(hipStreamSynchronize(s));
     IDX_CHECK(hipMalloc((void**)&kpos, std::max<uint64_t>(total, 1) * sizeof(uint64_t)));
-    hipLaunchKernelGGL(k_idx_place, grid, block, 0, s, k1, v1, n, rstart, koff, G.clen, dcb, nctg, kpos);
+    IDX_CHECK(hipMalloc((void**)&kfp, std::max<uint64_t>(total, 1) * sizeof(uint16_t)));
+    hipLaunchKernelGGL(k_idx_place, grid, block, 0, s, k1, v1, n, rstart, koff, G.g2b, G.nmask, G.coff, G.clen, dcb,
+                       nctg, kpos, kfp);
   } else {
     IDX_CHECK(hipMalloc((void**)&kpos, sizeof(uint64_t)));
+    IDX_CHECK(hipMalloc((void**)&kfp, sizeof(uint16_t)));
   }
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
     (void)hipFree(kpos);
+    (void)hipFree(kfp);
     err = std::string("index build failed: ") + hipGetErrorString(e);
     return e;
   }
   *kpos_out = kpos;
+  *kfp_out = kfp;
   *n_entries = total;
   return hipSuccess;
 }

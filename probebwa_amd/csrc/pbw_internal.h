@@ -18,6 +18,7 @@ constexpr int LEFT_PAD = 10;             // single.rs:109
 constexpr int MATCH_SCORE = 20;          // smith_waterman.rs:81
 constexpr int32_t NEG_INF = INT32_MIN / 2;  // smith_waterman.rs:173
 constexpr uint64_t CODE_SPACE = 1ull << 30;  // 4^15 15-mer codes
+constexpr uint16_t GENOME_FP_OPEN = 0xFFFF;  // no complete genome flanks (counts never reach 15)
 constexpr int DIAG_BITS = 41;            // hit key: (group << 41) | (diag + 2^40)
 constexpr int64_t DIAG_BIAS = 1ll << 40;
 
@@ -48,6 +49,7 @@ struct DevIndex {
   // original code's 1024-bit line, as positions 10-14 do in `pres`
   const uint64_t* pres_r10;
   const uint64_t* pres_r20;
+  const uint16_t* kfp;   // per entry of kpos: genome flank fingerprint (GENOME_FP_OPEN: flanks off the contig)
 };
 
 struct DevReads {
@@ -321,7 +323,7 @@ void launch_scatter_u64(uint64_t* out, const uint32_t* idx, const uint64_t* val,
 // Bucket table on the device (pbw_index.hip): fills koff[0 .. 2^30] and allocates
 // *kpos (hipMalloc, owned by the caller) with *n_entries packed positions.
 hipError_t gpu_build_index(const DevGenome& G, const uint64_t* h_clen, uint32_t* koff, uint64_t** kpos,
-                           uint64_t* n_entries, hipStream_t s, std::string& err);
+                           uint16_t** kfp, uint64_t* n_entries, hipStream_t s, std::string& err);
 
 __device__ __forceinline__ uint16_t float_to_bf16(float f) {  // round to nearest even
   uint32_t u = __float_as_uint(f);

@@ -215,33 +215,18 @@ __device__ __forceinline__ uint32_t fp_count(uint8_t b, bool complement) {
 }
 
 // seeds.rs:162-219 for the only case that can reject: both flanks complete on
-// both sides (after_len + before_len == 12); otherwise the hit passes.
-__device__ __forceinline__ bool seed_filter(const DevGenome& G, uint32_t contig, uint64_t pos, bool reverse,
-                                            bool neighbour, uint8_t wflags, uint16_t fpF, uint16_t fpR) {
+// both sides (after_len + before_len == 12); otherwise the hit passes.  The
+// genome side is the entry's stored flank fingerprint (the counts of the 4
+// bases before and the 8 after the 15-mer, N not counted, written by the index
+// build), read with the position itself instead of from the 2-bit genome.
+__device__ __forceinline__ bool seed_filter(uint16_t gfp, bool reverse, bool neighbour, uint8_t wflags, uint16_t fpF,
+                                            uint16_t fpR) {
   bool rfull = reverse ? (wflags & 8) : (wflags & 4);
-  if (!rfull) return true;
-  uint64_t clen = G.clen[contig];
-  if (pos < 4 || pos + 23 > clen) return true;
-  uint64_t gb = G.coff[contig] + pos - 4;  // bases gb .. gb+26
-  uint64_t w0 = gb >> 5;
-  uint32_t s = (uint32_t)(gb & 31) * 2;
-  uint64_t lo = G.g2b[w0], hi = G.g2b[w0 + 1];
-  uint64_t bits = s ? ((lo >> s) | (hi << (64 - s))) : lo;
-  uint64_t n0 = gb >> 6;
-  uint32_t ns = (uint32_t)(gb & 63);
-  uint64_t nlo = G.nmask[n0], nhi = G.nmask[n0 + 1];
-  uint64_t nb = ns ? ((nlo >> ns) | (nhi << (64 - ns))) : nlo;
-  int cnt[4] = {0, 0, 0, 0};
-#pragma unroll
-  for (int t = 0; t < 27; t++) {
-    if (t >= 4 && t < 19) continue;
-    if ((nb >> t) & 1ull) continue;
-    cnt[(bits >> (2 * t)) & 3]++;
-  }
+  if (!rfull || gfp == GENOME_FP_OPEN) return true;
   uint16_t rf = reverse ? fpR : fpF;
   int dist = 0;
 #pragma unroll
-  for (int c = 0; c < 4; c++) dist += abs((int)((rf >> (4 * c)) & 15) - cnt[c]);
+  for (int c = 0; c < 4; c++) dist += abs((int)((rf >> (4 * c)) & 15) - (int)((gfp >> (4 * c)) & 15));
   return dist <= (neighbour ? 8 : 10);
 }
 
@@ -318,12 +303,14 @@ __device__ __forceinline__ void seed_positions(const SeedArgs& a, const SeedLds&
       else lo = mid + 1;
     }
     const uint32_t excl = lo ? S.q_incl[lo - 1] : 0u;
-    const uint64_t packed = a.X.kpos[(uint64_t)S.q_off[lo] + (e - excl)];
+    const uint64_t entry = (uint64_t)S.q_off[lo] + (e - excl);
+    const uint64_t packed = a.X.kpos[entry];
+    const uint16_t gfp = a.X.kfp[entry];
     const uint32_t contig = (uint32_t)((packed >> 40) & ((1ull << 23) - 1));
     const uint64_t pos = packed & ((1ull << 40) - 1);
     const bool rev = (packed >> 63) & 1ull;
     const uint32_t fl = S.q_fl[lo], fp = S.q_fp[lo];
-    pass = seed_filter(a.G, contig, pos, rev, (fl & 1u) != 0, (uint8_t)fl, (uint16_t)fp, (uint16_t)(fp >> 16));
+    pass = seed_filter(gfp, rev, (fl & 1u) != 0, (uint8_t)fl, (uint16_t)fp, (uint16_t)(fp >> 16));
     const int64_t rp = (int64_t)S.q_pos[lo];
     const int64_t diag = rev ? (int64_t)pos + rp : (int64_t)pos - rp;
     key = ((uint64_t)(contig * 2u + (rev ? 1u : 0u)) << DIAG_BITS) | (uint64_t)(diag + DIAG_BIAS);
