@@ -1310,6 +1310,80 @@ __device__ void top_clusters(const ClusterMem& M, uint32_t C, uint32_t nt, uint3
   team_sync<T>();
 }
 
+// Reduction over the team (T = 64: one wave; else the block, through sh[T/64]).
+template <int T, typename Op>
+__device__ __forceinline__ uint32_t team_reduce(uint32_t v, Op op, uint32_t* sh) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = op(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  if constexpr (T == 64) {
+    return v;
+  } else {
+    const uint32_t w = threadIdx.x >> 6;
+    if (lane_id() == 0) sh[w] = v;
+    __syncthreads();
+    uint32_t r = sh[0];
+    for (uint32_t k = 1; k < (uint32_t)(T / 64); k++) r = op(r, sh[k]);
+    __syncthreads();
+    return r;
+  }
+}
+
+// Block tier, many clusters, 32-bit selection keys key[0..C) (unique, ascending in
+// the order of the score sort): the nt smallest by an MSB-first radix select --
+// bytes in which every key agrees (kor ^ kand) are taken without a pass, the bin
+// of the nt-th key found by a scan over the 256 bins -- then those nt sorted in
+// LDS; their cluster ids go to idx[0..nt).
+template <int T>
+__device__ void top_clusters32(const ClusterMem& M, const uint32_t* key, uint32_t C, uint32_t nt, uint32_t kor,
+                               uint32_t kand, uint32_t tid) {
+  static_assert(T == 256, "one histogram bin per thread");
+  uint32_t* hist = M.gt_cnt;
+  uint32_t* sh = M.gt_misc;
+  uint32_t prefix = 0, mask = 0, need = nt;
+  for (int byte = 3; byte >= 0; byte--) {
+    const int sh8 = 8 * byte;
+    const uint32_t bm = 0xFFu << sh8;
+    if (((kor ^ kand) & bm) == 0) {
+      prefix |= kand & bm;
+      mask |= bm;
+      continue;
+    }
+    hist[tid] = 0;
+    team_sync<T>();
+    for (uint32_t c = tid; c < C; c += T) {
+      const uint32_t k = key[c];
+      if ((k & mask) == prefix) atomicAdd(&hist[(k >> sh8) & 255u], 1u);
+    }
+    team_sync<T>();
+    const uint32_t v = hist[tid];
+    uint32_t tot;
+    const uint32_t ex = team_excl_scan<T>(v, tot, sh + 2);
+    if (ex < need && ex + v >= need) {
+      sh[0] = tid;
+      sh[1] = need - ex;
+    }
+    team_sync<T>();
+    prefix |= sh[0] << sh8;
+    mask |= bm;
+    need = sh[1];
+    team_sync<T>();
+  }
+  if (tid == 0) sh[0] = 0;
+  team_sync<T>();
+  for (uint32_t c = tid; c < C; c += T)
+    if (key[c] <= prefix) {
+      const uint32_t at = atomicAdd(&sh[0], 1u);
+      M.stage_key[at] = key[c];
+      M.stage_val[at] = c;
+    }
+  team_sync<T>();
+  for (uint32_t j = nt + tid; j < 32; j += T) M.stage_key[j] = ~0ull;
+  team_sync<T>();
+  team_bitonic<T>(M.stage_key, M.stage_val, 32, tid);
+  for (uint32_t j = tid; j < nt; j += T) M.idx[j] = M.stage_val[j];
+  team_sync<T>();
+}
+
 template <int T, bool FEAT>
 __device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const ClusterMem& M, Cand* out,
                              uint8_t* ncand_out, uint32_t* sh, const ClusterFeat& F) {
@@ -1327,11 +1401,9 @@ __device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const
     team_sync<T>();
     cluster_sort<T>(M, M.key, M.idx, p2, tid);
   }
-  for (uint32_t i = tid; i < n; i += T) {
-    M.cl_sum[i] = 0ull;
-    if constexpr (FEAT) {
+  if constexpr (FEAT) {
+    for (uint32_t i = tid; i < n; i += T)
       if (F.hf != F.hflag) F.hf[i] = F.hflag[i];
-    }
   }
   team_sync<T>();
   // group and cluster starts (sliding merge, :51-71), one contiguous chunk per thread
@@ -1361,6 +1433,9 @@ __device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const
     }
   }
   if (tid == 0) M.cl_start[C] = n;
+  // the only clusters summed atomically are those that can run past a chunk's end:
+  // each chunk's last cluster start, zeroed here; every other sum is a plain store
+  if (nc) M.cl_sum[cbase + nc - 1] = 0ull;
   team_sync<T>();
   if (c1 > c0) {
     int32_t gid = (int32_t)gbase - 1, cid = (int32_t)cbase - 1;
@@ -1487,28 +1562,57 @@ __device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const
   // stable sort by score (:76): key (count desc, group rank, position in the group)
   uint32_t pc = 1;
   while (pc < C) pc <<= 1;
-  for (uint32_t c = tid; c < pc; c += T) {
-    uint64_t sk = ~0ull;
-    if (c < C) {
-      const uint32_t s = M.cl_start[c], cnt = M.cl_start[c + 1] - s;
-      sk = ((uint64_t)(0x3FFFFFu - min(cnt, 0x3FFFFFu)) << 42) | ((uint64_t)min(M.g_rank[M.cl_gid[c]], 0xFFFFFu) << 22) |
-           (uint64_t)s;
-    }
-    M.key[c] = sk;
-    M.idx[c] = c;
-  }
-  team_sync<T>();
   const uint32_t nt = min(C, (uint32_t)TOP_CANDIDATES);
+  bool compact = false;
   if constexpr (T != 64) {
-    if (C > 256) top_clusters<T>(M, C, nt, tid);
-    else cluster_sort<T>(M, M.key, M.idx, pc, tid);
-  } else {
-    cluster_sort<T>(M, M.key, M.idx, pc, tid);
+    if (C > 256) {
+      // 32-bit keys (max count - count, group rank, cluster id) when they fit: the
+      // cluster id orders a group's clusters as their first sorted hits do
+      uint32_t mx = 0;
+      for (uint32_t c = tid; c < C; c += T) mx = max(mx, M.cl_start[c + 1] - M.cl_start[c]);
+      mx = team_reduce<T>(mx, [](uint32_t a, uint32_t b) { return a > b ? a : b; }, sh);
+      const uint32_t cb = 32 - __builtin_clz(mx), gb = G > 1 ? 32 - __builtin_clz(G - 1) : 0,
+                     sb = 32 - __builtin_clz(C - 1);
+      if (cb + gb + sb <= 32) {
+        uint32_t* k32 = reinterpret_cast<uint32_t*>(M.key);
+        uint32_t kor = 0, kand = 0xFFFFFFFFu;
+        for (uint32_t c = tid; c < C; c += T) {
+          const uint32_t k = ((mx - (M.cl_start[c + 1] - M.cl_start[c])) << (gb + sb)) |
+                             (M.g_rank[M.cl_gid[c]] << sb) | c;
+          k32[c] = k;
+          kor |= k;
+          kand &= k;
+        }
+        kor = team_reduce<T>(kor, [](uint32_t a, uint32_t b) { return a | b; }, sh);
+        kand = team_reduce<T>(kand, [](uint32_t a, uint32_t b) { return a & b; }, sh);
+        top_clusters32<T>(M, k32, C, nt, kor, kand, tid);
+        compact = true;
+      }
+    }
+  }
+  if (!compact) {
+    for (uint32_t c = tid; c < pc; c += T) {
+      uint64_t sk = ~0ull;
+      if (c < C) {
+        const uint32_t s = M.cl_start[c], cnt = M.cl_start[c + 1] - s;
+        sk = ((uint64_t)(0x3FFFFFu - min(cnt, 0x3FFFFFu)) << 42) |
+             ((uint64_t)min(M.g_rank[M.cl_gid[c]], 0xFFFFFu) << 22) | (uint64_t)s;
+      }
+      M.key[c] = sk;
+      M.idx[c] = c;
+    }
+    team_sync<T>();
+    if constexpr (T != 64) {
+      if (C > 256) top_clusters<T>(M, C, nt, tid);
+      else cluster_sort<T>(M, M.key, M.idx, pc, tid);
+    } else {
+      cluster_sort<T>(M, M.key, M.idx, pc, tid);
+    }
   }
   if constexpr (FEAT) {
-    const uint64_t top = M.key[0] >> 42;
+    const uint32_t top = M.cl_start[M.idx[0] + 1] - M.cl_start[M.idx[0]];  // the largest count
     uint32_t ties = 0;
-    for (uint32_t c = tid; c < C; c += T) ties += (M.key[c] >> 42) == top;
+    for (uint32_t c = tid; c < C; c += T) ties += M.cl_start[c + 1] - M.cl_start[c] == top;
     ties = group_sum_u32<64>(ties);
     if (lane_id() == 0 && ties) atomicAdd(&F.cnt[0], ties);
     team_sync<T>();
