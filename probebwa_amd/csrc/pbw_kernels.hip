@@ -1018,10 +1018,10 @@ __device__ void run_radix_sort(const ClusterMem& M, uint32_t a0, uint32_t cnt, u
       v[j] = M.idx[a0 + pos];
     }
   }
-  BRS(*reinterpret_cast<typename BRS::TempStorage*>(M.rsort)).Sort(k, v, 0, bits);
+  BRS(*reinterpret_cast<typename BRS::TempStorage*>(M.rsort)).SortBlockedToStriped(k, v, 0, bits);
 #pragma unroll
-  for (int j = 0; j < IPT; j++) {  // blocked: thread tid holds ranks tid * IPT + j
-    const uint32_t pos = tid * (uint32_t)IPT + (uint32_t)j;
+  for (int j = 0; j < IPT; j++) {  // striped: thread tid holds ranks j * RSORT_T + tid (coalesced stores)
+    const uint32_t pos = (uint32_t)j * RSORT_T + tid;
     if (pos < cnt) {
       const uint32_t q = run_segment(pre, r, r1, k[j]);
       M.key[a0 + pos] = ((uint64_t)M.gt_cnt[q] << DIAG_BITS) | (M.gt_lo[q] + (k[j] - pre[q]));
@@ -1059,18 +1059,23 @@ __device__ bool group_segment_sort(const uint64_t* hits, uint32_t n, const Clust
     const uint64_t k = hits[i];
     const uint32_t g = (uint32_t)(k >> DIAG_BITS);
     const unsigned long long d = k & ((1ull << DIAG_BITS) - 1);
-    uint32_t sl = gt_slot(g);
-    for (uint32_t probe = 0; probe < CLUSTER_GT; probe++) {
-      const uint32_t prev = atomicCAS(&M.gt_key[sl], 0xFFFFFFFFu, g);
-      if (prev == 0xFFFFFFFFu) atomicAdd(&M.gt_misc[0], 1u);
-      if (prev == 0xFFFFFFFFu || prev == g) break;
-      sl = (sl + 1) & (CLUSTER_GT - 1);
-    }
     // the wave's hits mostly share one group (a bucket's entries are in group
-    // order): one update per distinct slot in the wave
-    const uint32_t sl0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sl);
+    // order): a full wave of one group probes the table once, from lane 0, and
+    // updates its slot once
+    const uint32_t g0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)g);
     const uint64_t act = __ballot(true);
-    if (act == ~0ull && __ballot(sl == sl0) == act) {
+    const bool uni = act == ~0ull && __ballot(g == g0) == act;
+    uint32_t sl = gt_slot(g);
+    if (!uni || lane_id() == 0) {
+      for (uint32_t probe = 0; probe < CLUSTER_GT; probe++) {
+        const uint32_t prev = atomicCAS(&M.gt_key[sl], 0xFFFFFFFFu, g);
+        if (prev == 0xFFFFFFFFu) atomicAdd(&M.gt_misc[0], 1u);
+        if (prev == 0xFFFFFFFFu || prev == g) break;
+        sl = (sl + 1) & (CLUSTER_GT - 1);
+      }
+    }
+    if (uni) {
+      const uint32_t sl0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sl);
       unsigned long long lo = d, hi = d;
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) {
@@ -1122,9 +1127,22 @@ __device__ bool group_segment_sort(const uint64_t* hits, uint32_t n, const Clust
   for (uint32_t i = tid; i < n; i += T) {
     const uint64_t k = hits[i];
     const uint32_t g = (uint32_t)(k >> DIAG_BITS);
-    uint32_t sl = gt_slot(g);
-    while (M.gt_key[sl] != g) sl = (sl + 1) & (CLUSTER_GT - 1);
-    const uint32_t pos = atomicAdd(&M.gt_cnt[sl], 1u);
+    const uint32_t g0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)g);
+    const uint64_t act = __ballot(true);
+    uint32_t pos;
+    if (__ballot(g == g0) == act) {
+      // one group across the active lanes (a prefix of the wave): one cursor
+      // update, the lanes at consecutive positions
+      uint32_t sl = gt_slot(g0);
+      while (M.gt_key[sl] != g0) sl = (sl + 1) & (CLUSTER_GT - 1);
+      uint32_t base = 0;
+      if (lane_id() == 0) base = atomicAdd(&M.gt_cnt[sl], (uint32_t)__popcll(act));
+      pos = (uint32_t)__builtin_amdgcn_readfirstlane((int)base) + lane_id();
+    } else {
+      uint32_t sl = gt_slot(g);
+      while (M.gt_key[sl] != g) sl = (sl + 1) & (CLUSTER_GT - 1);
+      pos = atomicAdd(&M.gt_cnt[sl], 1u);
+    }
     M.key[pos] = k;
     M.idx[pos] = i;
   }
