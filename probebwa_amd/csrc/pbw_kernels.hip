@@ -828,16 +828,19 @@ struct ClusterMem {
   unsigned long long* gt_lo;  //   lowest biased diagonal per group slot, then per group rank
   unsigned long long* gt_hi;  //   highest per slot, then the diagonal range per rank
   void* rsort;                //   LDS of the block radix sort
+  uint32_t* gt_emin;          //   first emission index per slot, then per group rank
 };
 constexpr uint32_t CLUSTER_STAGE = 1024;
 constexpr uint32_t CLUSTER_GT = 1024, CLUSTER_GT_MAX = 512;  // table slots, groups handled
 // Block-tier runs of whole segments sorted by one block radix sort (256 threads).
 constexpr uint32_t RSORT_T = 256, RSORT_BIG = 16, RSORT_SMALL = 4, RSORT_TILE = RSORT_T * RSORT_BIG;
-using RSortBig = hipcub::BlockRadixSort<uint32_t, RSORT_T, RSORT_BIG, uint32_t>;
-using RSortSmall = hipcub::BlockRadixSort<uint32_t, RSORT_T, RSORT_SMALL, uint32_t>;
+template <bool VALS, int IPT>
+using RSort = hipcub::BlockRadixSort<uint32_t, RSORT_T, IPT, typename std::conditional<VALS, uint32_t, hipcub::NullType>::type>;
 union RSortLds {
-  typename RSortBig::TempStorage big;
-  typename RSortSmall::TempStorage small;
+  typename RSort<true, RSORT_BIG>::TempStorage big;
+  typename RSort<true, RSORT_SMALL>::TempStorage small;
+  typename RSort<false, RSORT_BIG>::TempStorage big_keys;
+  typename RSort<false, RSORT_SMALL>::TempStorage small_keys;
 };
 
 // The same network for p2 <= 64 * E keys held by one wave in registers:
@@ -942,48 +945,66 @@ struct ClusterFeat {
 
 __device__ __forceinline__ uint32_t gt_slot(uint32_t g) { return (g * 0x9E3779B1u) >> 22; }  // 10 bits
 
-// One merge-path step of a block: the sorted runs src[s, s + na) and
-// src[s + na, s + na + nb) into dst[s, s + na + nb), keys with their values.
-// Thread t writes outputs [t * per, (t + 1) * per), its split found by binary
-// search along its diagonal.
-__device__ void block_merge(const uint64_t* sk, const uint32_t* sv, uint64_t* dk, uint32_t* dv, uint32_t s,
-                            uint32_t na, uint32_t nb, uint32_t tid) {
-  const uint64_t* A = sk + s;
-  const uint64_t* B = sk + s + na;
-  const uint32_t total = na + nb, per = (total + RSORT_T - 1) / RSORT_T;
-  const uint32_t d0 = min(total, tid * per), d1 = min(total, d0 + per);
-  uint32_t lo = d0 > nb ? d0 - nb : 0u, hi = min(d0, na);
-  while (lo < hi) {  // i = A elements among the first d0 outputs (A first on ties)
+// Merge path: how many of the first d outputs of merging the sorted runs
+// A[0, na) and B[0, nb) come from A (A first on ties).
+__device__ __forceinline__ uint32_t merge_split(const uint64_t* A, uint32_t na, const uint64_t* B, uint32_t nb,
+                                                uint32_t d) {
+  uint32_t lo = d > nb ? d - nb : 0u, hi = min(d, na);
+  while (lo < hi) {
     const uint32_t mid = (lo + hi) >> 1;
-    if (A[mid] <= B[d0 - 1 - mid]) lo = mid + 1;
+    if (A[mid] <= B[d - 1 - mid]) lo = mid + 1;
     else hi = mid;
   }
-  uint32_t i = lo, j = d0 - lo;
-  for (uint32_t d = d0; d < d1; d++) {
-    const bool takeA = j >= nb || (i < na && A[i] <= B[j]);
-    if (takeA) {
-      dk[s + d] = A[i];
-      dv[s + d] = sv[s + i];
-      i++;
-    } else {
-      dk[s + d] = B[j];
-      dv[s + d] = sv[s + na + j];
-      j++;
+  return lo;
+}
+
+// One merge step of a block: the sorted runs src[s, s + na) and
+// src[s + na, s + na + nb) into dst[s, s + na + nb) (keys, with their values when
+// VALS), MERGE_CHUNK outputs at a time.  Two threads find where the chunk's
+// outputs begin and end along the merge path; the block loads the two windows
+// of the runs that feed them into LDS (the radix sort's storage and stage_key,
+// free while the sort merges) with coalesced loads, and each thread merges its
+// MERGE_CHUNK / 256 outputs from there.
+constexpr uint32_t MERGE_CHUNK = 2048;
+template <bool VALS>
+__device__ void block_merge(const ClusterMem& M, const uint64_t* sk, const uint32_t* sv, uint64_t* dk, uint32_t* dv,
+                            uint32_t s, uint32_t na, uint32_t nb, uint32_t tid) {
+  static_assert(sizeof(RSortLds) >= MERGE_CHUNK * sizeof(uint64_t), "merge window keys fit the sort's LDS");
+  static_assert(CLUSTER_STAGE * sizeof(uint64_t) >= MERGE_CHUNK * sizeof(uint32_t), "merge window values fit");
+  constexpr uint32_t PER = MERGE_CHUNK / RSORT_T;
+  uint64_t* lk = reinterpret_cast<uint64_t*>(M.rsort);
+  uint32_t* lv = reinterpret_cast<uint32_t*>(M.stage_key);
+  uint32_t* sp = M.gt_misc;
+  const uint64_t* A = sk + s;
+  const uint64_t* B = sk + s + na;
+  const uint32_t total = na + nb;
+  for (uint32_t D = 0; D < total; D += MERGE_CHUNK) {
+    const uint32_t E = min(total, D + MERGE_CHUNK);
+    if (tid < 2) sp[tid] = merge_split(A, na, B, nb, tid == 0 ? D : E);
+    __syncthreads();
+    const uint32_t ia = sp[0], ja = D - ia;
+    const uint32_t an = sp[1] - ia, bn = (E - sp[1]) - ja;
+    __syncthreads();
+    for (uint32_t x = tid; x < an + bn; x += RSORT_T) {
+      const uint32_t src = x < an ? s + ia + x : s + na + ja + (x - an);
+      lk[x] = sk[src];
+      if constexpr (VALS) lv[x] = sv[src];
     }
+    __syncthreads();
+    const uint32_t t0 = min(an + bn, tid * PER), t1 = min(an + bn, t0 + PER);
+    uint32_t i = merge_split(lk, an, lk + an, bn, t0), j = t0 - i;
+    for (uint32_t d = t0; d < t1; d++) {
+      const bool takeA = j >= bn || (i < an && lk[i] <= lk[an + j]);
+      const uint32_t x = takeA ? i : an + j;
+      dk[s + D + d] = lk[x];
+      if constexpr (VALS) dv[s + D + d] = lv[x];
+      if (takeA) i++;
+      else j++;
+    }
+    __syncthreads();
   }
 }
 
-// Block tier: the (group, diagonal) sort as a segmented sort.  The hits are
-// counted per group in an LDS table, the groups ranked in ascending order, the
-// hits scattered into one segment per group (key[] / idx[], emission index as
-// the value), and then every run of consecutive whole segments that fits
-// CLUSTER_STAGE keys is sorted in LDS (the group sits in the key's high bits,
-// so sorting several whole segments together keeps them apart); a segment
-// longer than that is sorted through the staged network in the scratch that
-// cl_sum / cl_start provide until the clustering needs them.  Ties between
-// equal keys may land in any order, as in the full sort: the clustering only
-// sums diagonals and takes minimum emission indices.  Returns false (nothing
-// written) when the read has more than CLUSTER_GT_MAX groups.
 // Rank of the run segment holding composite key x: the last q in [r, r1) with pre[q] <= x.
 __device__ __forceinline__ uint32_t run_segment(const uint32_t* pre, uint32_t r, uint32_t r1, uint32_t x) {
   uint32_t lo = r, hi = r1 - 1;
@@ -1000,9 +1021,10 @@ __device__ __forceinline__ uint32_t run_segment(const uint32_t* pre, uint32_t r,
 // pre_q + (d - lo_q), the segments' ranges laid end to end in group order, so the
 // composite key orders (group, diagonal) as the 64-bit hit key does.  The
 // emission index rides along as the value.
-template <int IPT, typename BRS>
+template <int IPT, bool VALS>
 __device__ void run_radix_sort(const ClusterMem& M, uint32_t a0, uint32_t cnt, uint32_t r, uint32_t r1,
                                int bits, uint32_t tid) {
+  using BRS = RSort<VALS, IPT>;
   const uint32_t* pre = M.gt_cnt + CLUSTER_GT_MAX;
   uint32_t k[IPT], v[IPT];
   const uint32_t pad = bits >= 32 ? 0xFFFFFFFFu : ((1u << bits) - 1u);
@@ -1015,17 +1037,18 @@ __device__ void run_radix_sort(const ClusterMem& M, uint32_t a0, uint32_t cnt, u
       const uint64_t key = M.key[a0 + pos];
       const uint32_t q = run_segment(M.gt_key, r, r1, a0 + pos);
       k[j] = pre[q] + (uint32_t)((key & ((1ull << DIAG_BITS) - 1)) - M.gt_lo[q]);
-      v[j] = M.idx[a0 + pos];
+      if constexpr (VALS) v[j] = M.idx[a0 + pos];
     }
   }
-  BRS(*reinterpret_cast<typename BRS::TempStorage*>(M.rsort)).SortBlockedToStriped(k, v, 0, bits);
+  if constexpr (VALS) BRS(*reinterpret_cast<typename BRS::TempStorage*>(M.rsort)).SortBlockedToStriped(k, v, 0, bits);
+  else BRS(*reinterpret_cast<typename BRS::TempStorage*>(M.rsort)).SortBlockedToStriped(k, 0, bits);
 #pragma unroll
   for (int j = 0; j < IPT; j++) {  // striped: thread tid holds ranks j * RSORT_T + tid (coalesced stores)
     const uint32_t pos = (uint32_t)j * RSORT_T + tid;
     if (pos < cnt) {
       const uint32_t q = run_segment(pre, r, r1, k[j]);
       M.key[a0 + pos] = ((uint64_t)M.gt_cnt[q] << DIAG_BITS) | (M.gt_lo[q] + (k[j] - pre[q]));
-      M.idx[a0 + pos] = v[j];
+      if constexpr (VALS) M.idx[a0 + pos] = v[j];
     }
   }
   __syncthreads();
@@ -1033,64 +1056,86 @@ __device__ void run_radix_sort(const ClusterMem& M, uint32_t a0, uint32_t cnt, u
 
 // Block tier: the (group, diagonal) sort as a segmented sort.  The hits are
 // counted per group in an LDS table (with each group's lowest and highest
-// diagonal), the groups ranked in ascending order, the hits scattered into one
-// segment per group (key[] / idx[], emission index as the value), and then
-// runs of consecutive whole segments are sorted by a block radix sort of 32-bit
-// composite keys (run_radix_sort) -- as many segments as fit one 4096-hit tile
-// and a 2^32 span of diagonals.  A segment larger than a tile is sorted tile by
-// tile and the sorted tiles merged pairwise (block_merge) through the scratch
-// that cl_sum / cl_start provide until the clustering needs them; one whose
-// diagonals span 2^32 or more goes through the staged bitonic network there.  Ties between equal keys may land in any
-// order, as in the full sort: the clustering only sums diagonals and takes
-// minimum emission indices.  Returns false (nothing written) when the read has
-// more than CLUSTER_GT_MAX groups.
-template <int T>
+// diagonal and its first emission index), the groups ranked in ascending order,
+// the hits scattered into one segment per group (key[], with the emission index
+// in idx[] when VALS), and then runs of consecutive whole segments are sorted by
+// a block radix sort of 32-bit composite keys (run_radix_sort) -- as many
+// segments as fit one 4096-hit tile and a 2^32 span of diagonals.  A segment
+// larger than a tile is sorted tile by tile and the sorted tiles merged pairwise
+// (block_merge) through the scratch that cl_sum / cl_start provide until the
+// clustering needs them; one whose diagonals span 2^32 or more goes through the
+// staged bitonic network there.  Ties between equal keys may land in any order,
+// as in the full sort: the clustering only sums diagonals (and, with VALS, reads
+// per-hit flags).  On return gt_key[r] is the segment start of group rank r
+// (gt_key[G] = n), gt_cnt[r] its group key and gt_emin[r] its first emission
+// index.  Returns false (nothing written) when the read has more than
+// CLUSTER_GT_MAX groups.
+template <int T, bool VALS>
 __device__ bool group_segment_sort(const uint64_t* hits, uint32_t n, const ClusterMem& M, uint32_t tid) {
   static_assert(T == RSORT_T, "the block tier runs 256 threads");
+  constexpr uint32_t U = 8;  // hits loaded ahead per thread: the loads of a pass overlap
   for (uint32_t i = tid; i < CLUSTER_GT; i += T) {
     M.gt_key[i] = 0xFFFFFFFFu;
     M.gt_cnt[i] = 0;
     M.gt_lo[i] = ~0ull;
     M.gt_hi[i] = 0ull;
+    M.gt_emin[i] = 0xFFFFFFFFu;
   }
   if (tid < 2) M.gt_misc[tid] = 0;
   team_sync<T>();
-  for (uint32_t i = tid; i < n; i += T) {
-    const uint64_t k = hits[i];
-    const uint32_t g = (uint32_t)(k >> DIAG_BITS);
-    const unsigned long long d = k & ((1ull << DIAG_BITS) - 1);
-    // the wave's hits mostly share one group (a bucket's entries are in group
-    // order): a full wave of one group probes the table once, from lane 0, and
-    // updates its slot once
-    const uint32_t g0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)g);
-    const uint64_t act = __ballot(true);
-    const bool uni = act == ~0ull && __ballot(g == g0) == act;
-    uint32_t sl = gt_slot(g);
-    if (!uni || lane_id() == 0) {
-      for (uint32_t probe = 0; probe < CLUSTER_GT; probe++) {
-        const uint32_t prev = atomicCAS(&M.gt_key[sl], 0xFFFFFFFFu, g);
-        if (prev == 0xFFFFFFFFu) atomicAdd(&M.gt_misc[0], 1u);
-        if (prev == 0xFFFFFFFFu || prev == g) break;
-        sl = (sl + 1) & (CLUSTER_GT - 1);
-      }
-    }
-    if (uni) {
-      const uint32_t sl0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sl);
-      unsigned long long lo = d, hi = d;
+  for (uint32_t i0 = tid; i0 < n; i0 += U * T) {
+    uint64_t kb[U];
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        lo = min(lo, (unsigned long long)shfl_xor_i64((int64_t)lo, o));
-        hi = max(hi, (unsigned long long)shfl_xor_i64((int64_t)hi, o));
+    for (uint32_t u = 0; u < U; u++) kb[u] = i0 + u * T < n ? hits[i0 + u * T] : 0ull;
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++) {
+      const uint32_t i = i0 + u * T;
+      if (i >= n) break;
+      const uint64_t k = kb[u];
+      const uint32_t g = (uint32_t)(k >> DIAG_BITS);
+      const unsigned long long d = k & ((1ull << DIAG_BITS) - 1);
+      // the wave's hits mostly share one group (a bucket's entries are in group
+      // order): a full wave of one group probes the table once, from lane 0, and
+      // updates its slot once (lane 0 holds the wave's first emission index)
+      const uint32_t g0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)g);
+      const uint64_t act = __ballot(true);
+      const bool uni = act == ~0ull && __ballot(g == g0) == act;
+      // a slot, once keyed, keeps its key: a plain read finds the group, and only
+      // an empty slot takes a compare-and-swap; the minima and maxima below are
+      // only ever lowered / raised, so a read that shows no change needed is final
+      uint32_t sl = gt_slot(g);
+      if (!uni || lane_id() == 0) {
+        for (uint32_t probe = 0; probe < CLUSTER_GT; probe++) {
+          const uint32_t cur = __atomic_load_n(&M.gt_key[sl], __ATOMIC_RELAXED);
+          if (cur == g) break;
+          if (cur == 0xFFFFFFFFu) {
+            const uint32_t prev = atomicCAS(&M.gt_key[sl], 0xFFFFFFFFu, g);
+            if (prev == 0xFFFFFFFFu) atomicAdd(&M.gt_misc[0], 1u);
+            if (prev == 0xFFFFFFFFu || prev == g) break;
+          }
+          sl = (sl + 1) & (CLUSTER_GT - 1);
+        }
       }
-      if (lane_id() == 0) {
-        atomicAdd(&M.gt_cnt[sl0], 64u);
-        atomicMin(&M.gt_lo[sl0], lo);
-        atomicMax(&M.gt_hi[sl0], hi);
+      if (uni) {
+        const uint32_t sl0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sl);
+        unsigned long long lo = d, hi = d;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+          lo = min(lo, (unsigned long long)shfl_xor_i64((int64_t)lo, o));
+          hi = max(hi, (unsigned long long)shfl_xor_i64((int64_t)hi, o));
+        }
+        if (lane_id() == 0) {
+          atomicAdd(&M.gt_cnt[sl0], 64u);
+          if (lo < __atomic_load_n(&M.gt_lo[sl0], __ATOMIC_RELAXED)) atomicMin(&M.gt_lo[sl0], lo);
+          if (hi > __atomic_load_n(&M.gt_hi[sl0], __ATOMIC_RELAXED)) atomicMax(&M.gt_hi[sl0], hi);
+          if (i < __atomic_load_n(&M.gt_emin[sl0], __ATOMIC_RELAXED)) atomicMin(&M.gt_emin[sl0], i);
+        }
+      } else {
+        atomicAdd(&M.gt_cnt[sl], 1u);
+        if (d < __atomic_load_n(&M.gt_lo[sl], __ATOMIC_RELAXED)) atomicMin(&M.gt_lo[sl], d);
+        if (d > __atomic_load_n(&M.gt_hi[sl], __ATOMIC_RELAXED)) atomicMax(&M.gt_hi[sl], d);
+        if (i < __atomic_load_n(&M.gt_emin[sl], __ATOMIC_RELAXED)) atomicMin(&M.gt_emin[sl], i);
       }
-    } else {
-      atomicAdd(&M.gt_cnt[sl], 1u);
-      atomicMin(&M.gt_lo[sl], d);
-      atomicMax(&M.gt_hi[sl], d);
     }
   }
   team_sync<T>();
@@ -1124,35 +1169,43 @@ __device__ bool group_segment_sort(const uint64_t* hits, uint32_t n, const Clust
   }
   if (tid == 0) M.stage_val[G] = n;
   team_sync<T>();
-  for (uint32_t i = tid; i < n; i += T) {
-    const uint64_t k = hits[i];
-    const uint32_t g = (uint32_t)(k >> DIAG_BITS);
-    const uint32_t g0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)g);
-    const uint64_t act = __ballot(true);
-    uint32_t pos;
-    if (__ballot(g == g0) == act) {
-      // one group across the active lanes (a prefix of the wave): one cursor
-      // update, the lanes at consecutive positions
-      uint32_t sl = gt_slot(g0);
-      while (M.gt_key[sl] != g0) sl = (sl + 1) & (CLUSTER_GT - 1);
-      uint32_t base = 0;
-      if (lane_id() == 0) base = atomicAdd(&M.gt_cnt[sl], (uint32_t)__popcll(act));
-      pos = (uint32_t)__builtin_amdgcn_readfirstlane((int)base) + lane_id();
-    } else {
-      uint32_t sl = gt_slot(g);
-      while (M.gt_key[sl] != g) sl = (sl + 1) & (CLUSTER_GT - 1);
-      pos = atomicAdd(&M.gt_cnt[sl], 1u);
+  for (uint32_t i0 = tid; i0 < n; i0 += U * T) {
+    uint64_t kb[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++) kb[u] = i0 + u * T < n ? hits[i0 + u * T] : 0ull;
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++) {
+      const uint32_t i = i0 + u * T;
+      if (i >= n) break;
+      const uint64_t k = kb[u];
+      const uint32_t g = (uint32_t)(k >> DIAG_BITS);
+      const uint32_t g0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)g);
+      const uint64_t act = __ballot(true);
+      uint32_t pos;
+      if (__ballot(g == g0) == act) {
+        // one group across the active lanes (a prefix of the wave): one cursor
+        // update, the lanes at consecutive positions
+        uint32_t sl = gt_slot(g0);
+        while (M.gt_key[sl] != g0) sl = (sl + 1) & (CLUSTER_GT - 1);
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(&M.gt_cnt[sl], (uint32_t)__popcll(act));
+        pos = (uint32_t)__builtin_amdgcn_readfirstlane((int)base) + lane_id();
+      } else {
+        uint32_t sl = gt_slot(g);
+        while (M.gt_key[sl] != g) sl = (sl + 1) & (CLUSTER_GT - 1);
+        pos = atomicAdd(&M.gt_cnt[sl], 1u);
+      }
+      M.key[pos] = k;
+      if constexpr (VALS) M.idx[pos] = i;
     }
-    M.key[pos] = k;
-    M.idx[pos] = i;
   }
   team_sync<T>();
   // per group rank r (r < G <= 512, two per thread at most): segment start in
   // gt_key[r] (gt_key[G] = n), group key in gt_cnt[r], lowest diagonal in
-  // gt_lo[r] and the diagonal range in gt_hi[r]; slot data read before the
-  // barrier, rank data written after it
+  // gt_lo[r], the diagonal range in gt_hi[r] and the first emission index in
+  // gt_emin[r]; slot data read before the barrier, rank data written after it
   uint64_t rl[2] = {0, 0}, rh[2] = {0, 0};
-  uint32_t rg[2] = {0, 0};
+  uint32_t rg[2] = {0, 0}, re[2] = {0, 0};
   for (uint32_t u = 0; u < 2; u++) {
     const uint32_t r = tid + u * T;
     if (r < G) {
@@ -1160,6 +1213,7 @@ __device__ bool group_segment_sort(const uint64_t* hits, uint32_t n, const Clust
       rg[u] = (uint32_t)(M.stage_key[r] >> 32);
       rl[u] = M.gt_lo[sl];
       rh[u] = M.gt_hi[sl] - M.gt_lo[sl];
+      re[u] = M.gt_emin[sl];
     }
   }
   team_sync<T>();
@@ -1169,6 +1223,7 @@ __device__ bool group_segment_sort(const uint64_t* hits, uint32_t n, const Clust
       M.gt_cnt[r] = rg[u];
       M.gt_lo[r] = rl[u];
       M.gt_hi[r] = rh[u];
+      M.gt_emin[r] = re[u];
     }
   }
   for (uint32_t r = tid; r <= G; r += T) M.gt_key[r] = M.stage_val[r];
@@ -1196,15 +1251,15 @@ __device__ bool group_segment_sort(const uint64_t* hits, uint32_t n, const Clust
     if (r1 > r) {
       const uint32_t cnt = M.gt_key[r1] - a0;
       if (cnt > 1) {
-        if (cnt <= RSORT_SMALL * RSORT_T) run_radix_sort<RSORT_SMALL, RSortSmall>(M, a0, cnt, r, r1, (int)bits, tid);
-        else run_radix_sort<RSORT_BIG, RSortBig>(M, a0, cnt, r, r1, (int)bits, tid);
+        if (cnt <= RSORT_SMALL * RSORT_T) run_radix_sort<RSORT_SMALL, VALS>(M, a0, cnt, r, r1, (int)bits, tid);
+        else run_radix_sort<RSORT_BIG, VALS>(M, a0, cnt, r, r1, (int)bits, tid);
       }
       r = r1;
       continue;
     }
     const uint32_t cnt = M.gt_key[r + 1] - a0;
     uint64_t* tk = reinterpret_cast<uint64_t*>(M.cl_sum);
-    uint32_t* tv = M.cl_start;
+    uint32_t* tv = VALS ? M.cl_start : nullptr;
     if (M.gt_hi[r] + 1 < (1ull << 32)) {
       // segment r alone exceeds a tile: each tile sorted on its own (a one-segment
       // run), then pairs of sorted runs merged, alternating between key/idx and
@@ -1217,7 +1272,7 @@ __device__ bool group_segment_sort(const uint64_t* hits, uint32_t n, const Clust
       team_sync<T>();
       const int sbits = (int)M.gt_misc[1];
       for (uint32_t t0 = 0; t0 < cnt; t0 += RSORT_TILE)
-        run_radix_sort<RSORT_BIG, RSortBig>(M, a0 + t0, min(RSORT_TILE, cnt - t0), r, r + 1, sbits, tid);
+        run_radix_sort<RSORT_BIG, VALS>(M, a0 + t0, min(RSORT_TILE, cnt - t0), r, r + 1, sbits, tid);
       bool in_key = true;
       for (uint32_t w = RSORT_TILE; w < cnt; w <<= 1) {
         const uint64_t* sk = in_key ? M.key : tk;
@@ -1226,7 +1281,7 @@ __device__ bool group_segment_sort(const uint64_t* hits, uint32_t n, const Clust
         uint32_t* dv = in_key ? tv : M.idx;
         for (uint32_t s0 = 0; s0 < cnt; s0 += 2 * w) {
           const uint32_t na = min(w, cnt - s0), nb = min(w, cnt - s0 - na);
-          block_merge(sk, sv, dk, dv, a0 + s0, na, nb, tid);
+          block_merge<VALS>(M, sk, sv, dk, dv, a0 + s0, na, nb, tid);
         }
         team_sync<T>();
         in_key = !in_key;
@@ -1234,7 +1289,7 @@ __device__ bool group_segment_sort(const uint64_t* hits, uint32_t n, const Clust
       if (!in_key) {
         for (uint32_t i = tid; i < cnt; i += T) {
           M.key[a0 + i] = tk[a0 + i];
-          M.idx[a0 + i] = tv[a0 + i];
+          if constexpr (VALS) M.idx[a0 + i] = tv[a0 + i];
         }
         team_sync<T>();
       }
@@ -1246,13 +1301,13 @@ __device__ bool group_segment_sort(const uint64_t* hits, uint32_t n, const Clust
     while (p2 < cnt) p2 <<= 1;
     for (uint32_t i = tid; i < p2; i += T) {
       tk[i] = i < cnt ? M.key[a0 + i] : ~0ull;
-      tv[i] = i < cnt ? M.idx[a0 + i] : 0u;
+      if constexpr (VALS) tv[i] = i < cnt ? M.idx[a0 + i] : 0u;
     }
     team_sync<T>();
     team_bitonic_staged<T, CLUSTER_STAGE>(tk, tv, p2, tid, M.stage_key, M.stage_val);
     for (uint32_t i = tid; i < cnt; i += T) {
       M.key[a0 + i] = tk[i];
-      M.idx[a0 + i] = tv[i];
+      if constexpr (VALS) M.idx[a0 + i] = tv[i];
     }
     team_sync<T>();
     r = r + 1;
@@ -1267,7 +1322,7 @@ __device__ bool group_segment_sort(const uint64_t* hits, uint32_t n, const Clust
 // (the prefilter features count ties over all of them).  idx[c] must be c.
 template <int T>
 __device__ void top_clusters(const ClusterMem& M, uint32_t C, uint32_t nt, uint32_t tid) {
-  uint32_t* hist = M.gt_cnt;
+  uint32_t* hist = M.gt_cnt + CLUSTER_GT_MAX;  // gt_cnt[0, G) may hold the group keys
   uint32_t* sh = M.gt_misc;
   uint64_t prefix = 0, mask = 0;
   uint32_t need = nt;
@@ -1355,7 +1410,7 @@ template <int T>
 __device__ void top_clusters32(const ClusterMem& M, const uint32_t* key, uint32_t C, uint32_t nt, uint32_t kor,
                                uint32_t kand, uint32_t tid) {
   static_assert(T == 256, "one histogram bin per thread");
-  uint32_t* hist = M.gt_cnt;
+  uint32_t* hist = M.gt_cnt + CLUSTER_GT_MAX;  // gt_cnt[0, G) may hold the group keys
   uint32_t* sh = M.gt_misc;
   uint32_t prefix = 0, mask = 0, need = nt;
   for (int byte = 3; byte >= 0; byte--) {
@@ -1403,15 +1458,24 @@ __device__ void top_clusters32(const ClusterMem& M, const uint32_t* key, uint32_
 }
 
 template <int T, bool FEAT>
-__device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const ClusterMem& M, Cand* out,
+__device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const ClusterMem& M0, Cand* out,
                              uint8_t* ncand_out, uint32_t* sh, const ClusterFeat& F) {
   const uint32_t tid = threadIdx.x % T;
   uint32_t p2 = 1;
   while (p2 < n) p2 <<= 1;
+  ClusterMem M = M0;
   // group by (contig, strand), sort each group by diagonal (:36-49): one sort by (group, diagonal)
   bool sorted = false;
-  if constexpr (T != 64) sorted = group_segment_sort<T>(hits, n, M, tid);
-  if (!sorted) {
+  if constexpr (T != 64) sorted = group_segment_sort<T, FEAT>(hits, n, M, tid);
+  if (sorted) {
+    // per-group arrays in LDS (at most CLUSTER_GT_MAX groups): the group keys
+    // and first emission indices by rank as the segmented sort left them, ranks
+    // and insertion order in gt_lo, free once the sort is done
+    M.g_key = M.gt_cnt;
+    M.g_min = M.gt_emin;
+    M.g_rank = reinterpret_cast<uint32_t*>(M.gt_lo);
+    M.g_sort = M.g_rank + CLUSTER_GT_MAX;
+  } else {
     for (uint32_t i = tid; i < p2; i += T) {
       M.key[i] = i < n ? hits[i] : ~0ull;
       M.idx[i] = i;
@@ -1428,21 +1492,31 @@ __device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const
   const uint32_t chunk = (n + T - 1) / T;
   const uint32_t c0 = min(n, tid * chunk), c1 = min(n, c0 + chunk);
   uint32_t ng = 0, nc = 0;
+  constexpr uint32_t U = T == 64 ? 1 : 8;  // keys loaded ahead per thread in the block tier
   {
     uint64_t kp = c0 > 0 ? M.key[c0 - 1] : 0;
-    for (uint32_t e = c0; e < c1; e++) {
-      const uint64_t k = M.key[e];
-      const bool gs = e == 0 || (k >> DIAG_BITS) != (kp >> DIAG_BITS);
-      ng += gs;
-      nc += gs || hit_diag(k) - hit_diag(kp) > MERGE_DISTANCE;
-      kp = k;
+    for (uint32_t e0 = c0; e0 < c1; e0 += U) {
+      uint64_t kb[U];
+#pragma unroll
+      for (uint32_t u = 0; u < U; u++) kb[u] = e0 + u < c1 ? M.key[e0 + u] : 0ull;
+#pragma unroll
+      for (uint32_t u = 0; u < U; u++) {
+        const uint32_t e = e0 + u;
+        if (e >= c1) break;
+        const uint64_t k = kb[u];
+        const bool gs = e == 0 || (k >> DIAG_BITS) != (kp >> DIAG_BITS);
+        ng += gs;
+        nc += gs || hit_diag(k) - hit_diag(kp) > MERGE_DISTANCE;
+        kp = k;
+      }
     }
   }
   uint32_t G, C;
   const uint32_t gbase = team_excl_scan<T>(ng, G, sh);
   const uint32_t cbase = team_excl_scan<T>(nc, C, sh);
   if (G > M.gcap) return false;
-  for (uint32_t g = tid; g < G; g += T) M.g_min[g] = 0xFFFFFFFFu;
+  if (!sorted)
+    for (uint32_t g = tid; g < G; g += T) M.g_min[g] = 0xFFFFFFFFu;
   if constexpr (FEAT) {
     for (uint32_t c = tid; c < C; c += T) {
       F.f1[c] = 0;
@@ -1484,42 +1558,57 @@ __device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const
         }
       }
     };
-    for (uint32_t e = c0; e < c1; e++) {
-      const uint64_t k = M.key[e];
-      const bool gs = e == 0 || (k >> DIAG_BITS) != (kp >> DIAG_BITS);
-      const bool cs = gs || hit_diag(k) - hit_diag(kp) > MERGE_DISTANCE;
-      if (gs) {
-        if (gid >= 0 && pmin != 0xFFFFFFFFu) atomicMin(&M.g_min[gid], pmin);
-        pmin = 0xFFFFFFFFu;
-        gid++;
-        M.g_key[gid] = (uint32_t)(k >> DIAG_BITS);
+    // emission indices: per-hit flags (FEAT) and, unless the segmented sort
+    // found them, each group's first index
+    const bool need_idx = FEAT || !sorted;
+    for (uint32_t e0 = c0; e0 < c1; e0 += U) {
+      uint64_t kb[U];
+      uint32_t ib[U];
+#pragma unroll
+      for (uint32_t u = 0; u < U; u++) {
+        kb[u] = e0 + u < c1 ? M.key[e0 + u] : 0ull;
+        ib[u] = need_idx && e0 + u < c1 ? M.idx[e0 + u] : 0u;
       }
-      if (cs) {
-        if (cid >= 0 && e > c0) flush(cid, shared);
-        shared = false;
-        psum = 0;
-        fnb = fms = 0;
-        fmn = 15;
-        cid++;
-        M.cl_start[cid] = e;
-        M.cl_gid[cid] = (uint32_t)gid;
+#pragma unroll
+      for (uint32_t u = 0; u < U; u++) {
+        const uint32_t e = e0 + u;
+        if (e >= c1) break;
+        const uint64_t k = kb[u];
+        const bool gs = e == 0 || (k >> DIAG_BITS) != (kp >> DIAG_BITS);
+        const bool cs = gs || hit_diag(k) - hit_diag(kp) > MERGE_DISTANCE;
+        if (gs) {
+          if (!sorted && gid >= 0 && pmin != 0xFFFFFFFFu) atomicMin(&M.g_min[gid], pmin);
+          pmin = 0xFFFFFFFFu;
+          gid++;
+          M.g_key[gid] = (uint32_t)(k >> DIAG_BITS);
+        }
+        if (cs) {
+          if (cid >= 0 && e > c0) flush(cid, shared);
+          shared = false;
+          psum = 0;
+          fnb = fms = 0;
+          fmn = 15;
+          cid++;
+          M.cl_start[cid] = e;
+          M.cl_gid[cid] = (uint32_t)gid;
+        }
+        psum += hit_diag(k);
+        const uint32_t ei = ib[u];
+        pmin = min(pmin, ei);
+        if constexpr (FEAT) {
+          const uint32_t f = F.hf[ei];
+          fnb += f >> 7;
+          fms += f & 15u;
+          fmn = min(fmn, f & 15u);
+        }
+        kp = k;
       }
-      psum += hit_diag(k);
-      const uint32_t ei = M.idx[e];
-      pmin = min(pmin, ei);
-      if constexpr (FEAT) {
-        const uint32_t f = F.hf[ei];
-        fnb += f >> 7;
-        fms += f & 15u;
-        fmn = min(fmn, f & 15u);
-      }
-      kp = k;
     }
     // the last cluster ends at c1 unless the next chunk's first hit continues it
     const bool ends = c1 == n || (M.key[c1] >> DIAG_BITS) != (kp >> DIAG_BITS) ||
                       hit_diag(M.key[c1]) - hit_diag(kp) > MERGE_DISTANCE;
     flush(cid, shared || !ends);
-    atomicMin(&M.g_min[gid], pmin);
+    if (!sorted) atomicMin(&M.g_min[gid], pmin);
   }
   team_sync<T>();
   if constexpr (FEAT) {
@@ -1767,6 +1856,7 @@ __global__ __launch_bounds__(256) void k_cluster_tier2(ClusterArgs a, const uint
   __shared__ uint32_t stage_val[CLUSTER_STAGE];
   __shared__ uint32_t gt_key[CLUSTER_GT], gt_cnt[CLUSTER_GT], gt_misc[2 + 256 / 64];
   __shared__ unsigned long long gt_lo[CLUSTER_GT], gt_hi[CLUSTER_GT];
+  __shared__ uint32_t gt_emin[CLUSTER_GT];
   __shared__ RSortLds rsort;
   for (uint32_t li = blockIdx.x; li < nlist; li += gridDim.x) {
     const uint32_t r = list[li];
@@ -1797,6 +1887,7 @@ __global__ __launch_bounds__(256) void k_cluster_tier2(ClusterArgs a, const uint
     M.gt_lo = gt_lo;
     M.gt_hi = gt_hi;
     M.rsort = &rsort;
+    M.gt_emin = gt_emin;
     ClusterFeat F{};
     if (FEAT) {
       const uint64_t so = a.R.seq_off[r], qo = a.R.qual_off[r];
