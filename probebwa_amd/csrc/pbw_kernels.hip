@@ -967,7 +967,7 @@ __device__ __forceinline__ uint32_t merge_split(const uint64_t* A, uint32_t na, 
 // MERGE_CHUNK / 256 outputs from there.
 constexpr uint32_t MERGE_CHUNK = 2048;
 template <bool VALS>
-__device__ void block_merge(const ClusterMem& M, const uint64_t* sk, const uint32_t* sv, uint64_t* dk, uint32_t* dv,
+__device__ __forceinline__ void block_merge(const ClusterMem& M, const uint64_t* sk, const uint32_t* sv, uint64_t* dk, uint32_t* dv,
                             uint32_t s, uint32_t na, uint32_t nb, uint32_t tid) {
   static_assert(sizeof(RSortLds) >= MERGE_CHUNK * sizeof(uint64_t), "merge window keys fit the sort's LDS");
   static_assert(CLUSTER_STAGE * sizeof(uint64_t) >= MERGE_CHUNK * sizeof(uint32_t), "merge window values fit");
@@ -1022,7 +1022,7 @@ __device__ __forceinline__ uint32_t run_segment(const uint32_t* pre, uint32_t r,
 // composite key orders (group, diagonal) as the 64-bit hit key does.  The
 // emission index rides along as the value.
 template <int IPT, bool VALS>
-__device__ void run_radix_sort(const ClusterMem& M, uint32_t a0, uint32_t cnt, uint32_t r, uint32_t r1,
+__device__ __forceinline__ void run_radix_sort(const ClusterMem& M, uint32_t a0, uint32_t cnt, uint32_t r, uint32_t r1,
                                int bits, uint32_t tid) {
   using BRS = RSort<VALS, IPT>;
   const uint32_t* pre = M.gt_cnt + CLUSTER_GT_MAX;
@@ -1071,7 +1071,7 @@ __device__ void run_radix_sort(const ClusterMem& M, uint32_t a0, uint32_t cnt, u
 // index.  Returns false (nothing written) when the read has more than
 // CLUSTER_GT_MAX groups.
 template <int T, bool VALS>
-__device__ bool group_segment_sort(const uint64_t* hits, uint32_t n, const ClusterMem& M, uint32_t tid) {
+__device__ __forceinline__ bool group_segment_sort(const uint64_t* hits, uint32_t n, const ClusterMem& M, uint32_t tid) {
   static_assert(T == RSORT_T, "the block tier runs 256 threads");
   constexpr uint32_t U = 8;  // hits loaded ahead per thread: the loads of a pass overlap
   for (uint32_t i = tid; i < CLUSTER_GT; i += T) {
@@ -1321,7 +1321,7 @@ __device__ bool group_segment_sort(const uint64_t* hits, uint32_t n, const Clust
 // swapped to the front of key[] / idx[], so key[] still holds every cluster's key
 // (the prefilter features count ties over all of them).  idx[c] must be c.
 template <int T>
-__device__ void top_clusters(const ClusterMem& M, uint32_t C, uint32_t nt, uint32_t tid) {
+__device__ __forceinline__ void top_clusters(const ClusterMem& M, uint32_t C, uint32_t nt, uint32_t tid) {
   uint32_t* hist = M.gt_cnt + CLUSTER_GT_MAX;  // gt_cnt[0, G) may hold the group keys
   uint32_t* sh = M.gt_misc;
   uint64_t prefix = 0, mask = 0;
@@ -1407,7 +1407,7 @@ __device__ __forceinline__ uint32_t team_reduce(uint32_t v, Op op, uint32_t* sh)
 // of the nt-th key found by a scan over the 256 bins -- then those nt sorted in
 // LDS; their cluster ids go to idx[0..nt).
 template <int T>
-__device__ void top_clusters32(const ClusterMem& M, const uint32_t* key, uint32_t C, uint32_t nt, uint32_t kor,
+__device__ __forceinline__ void top_clusters32(const ClusterMem& M, const uint32_t* key, uint32_t C, uint32_t nt, uint32_t kor,
                                uint32_t kand, uint32_t tid) {
   static_assert(T == 256, "one histogram bin per thread");
   uint32_t* hist = M.gt_cnt + CLUSTER_GT_MAX;  // gt_cnt[0, G) may hold the group keys
@@ -1458,7 +1458,7 @@ __device__ void top_clusters32(const ClusterMem& M, const uint32_t* key, uint32_
 }
 
 template <int T, bool FEAT>
-__device__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const ClusterMem& M0, Cand* out,
+__device__ __forceinline__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const ClusterMem& M0, Cand* out,
                              uint8_t* ncand_out, uint32_t* sh, const ClusterFeat& F) {
   const uint32_t tid = threadIdx.x % T;
   uint32_t p2 = 1;
