@@ -829,6 +829,7 @@ struct ClusterMem {
   unsigned long long* gt_hi;  //   highest per slot, then the diagonal range per rank
   void* rsort;                //   LDS of the block radix sort
   uint32_t* gt_emin;          //   first emission index per slot, then per group rank
+  uint32_t* g_c0;             // block tier after the segmented sort: first cluster per group ([G] = C)
 };
 constexpr uint32_t CLUSTER_STAGE = 1024;
 constexpr uint32_t CLUSTER_GT = 1024, CLUSTER_GT_MAX = 512;  // table slots, groups handled
@@ -958,50 +959,92 @@ __device__ __forceinline__ uint32_t merge_split(const uint64_t* A, uint32_t na, 
   return lo;
 }
 
+// The same split found by one wave: a 64-way search that evaluates the merge
+// path's predicate (A[i] <= B[d - 1 - i], true below the split) at 64 evenly
+// spaced points per round, so a split in a run of m keys takes about log64(m)
+// rounds of parallel loads instead of log2(m) dependent ones.  All 64 lanes call
+// it; every lane returns the split.
+__device__ __forceinline__ uint32_t merge_split_wave(const uint64_t* A, uint32_t na, const uint64_t* B, uint32_t nb,
+                                                     uint32_t d) {
+  uint32_t lo = d > nb ? d - nb : 0u, hi = min(d, na);
+  const uint32_t lane = (uint32_t)lane_id();
+  while (lo < hi) {
+    const uint32_t step = (hi - lo + 63) / 64;
+    const uint32_t i = lo + lane * step;
+    const bool t = i < hi && A[i] <= B[d - 1 - i];
+    const uint32_t nt = (uint32_t)__popcll(__ballot(t));  // the true points are a prefix
+    if (nt == 0) {
+      hi = lo;
+    } else {
+      const uint32_t last = lo + (nt - 1) * step;
+      hi = min(hi, lo + nt * step);
+      lo = last + 1;
+    }
+  }
+  return lo;
+}
+
 // One merge step of a block: the sorted runs src[s, s + na) and
 // src[s + na, s + na + nb) into dst[s, s + na + nb) (keys, with their values when
-// VALS), MERGE_CHUNK outputs at a time.  Two threads find where the chunk's
-// outputs begin and end along the merge path; the block loads the two windows
-// of the runs that feed them into LDS (the radix sort's storage and stage_key,
-// free while the sort merges) with coalesced loads, and each thread merges its
-// MERGE_CHUNK / 256 outputs from there.
+// VALS), MERGE_CHUNK outputs at a time.  The waves first find where every chunk's
+// outputs begin along the merge path (one split per wave at a time, kept in
+// stage_val); then for each chunk the block loads the two windows of the runs
+// that feed it into LDS (the radix sort's storage and stage_key, free while the
+// sort merges) with coalesced loads, and each thread merges its MERGE_CHUNK / 256
+// outputs from there.
 constexpr uint32_t MERGE_CHUNK = 2048;
 template <bool VALS>
-__device__ __forceinline__ void block_merge(const ClusterMem& M, const uint64_t* sk, const uint32_t* sv, uint64_t* dk, uint32_t* dv,
-                            uint32_t s, uint32_t na, uint32_t nb, uint32_t tid) {
+__device__ __forceinline__ void block_merge(const ClusterMem& M, const uint64_t* sk, const uint32_t* sv,
+                                            uint64_t* dk, uint32_t* dv, uint32_t s, uint32_t na, uint32_t nb,
+                                            uint32_t tid) {
   static_assert(sizeof(RSortLds) >= MERGE_CHUNK * sizeof(uint64_t), "merge window keys fit the sort's LDS");
   static_assert(CLUSTER_STAGE * sizeof(uint64_t) >= MERGE_CHUNK * sizeof(uint32_t), "merge window values fit");
-  constexpr uint32_t PER = MERGE_CHUNK / RSORT_T;
+  constexpr uint32_t PER = MERGE_CHUNK / RSORT_T, NW = RSORT_T / 64, BATCH = CLUSTER_STAGE - 1;
   uint64_t* lk = reinterpret_cast<uint64_t*>(M.rsort);
   uint32_t* lv = reinterpret_cast<uint32_t*>(M.stage_key);
-  uint32_t* sp = M.gt_misc;
+  uint32_t* sp = M.stage_val;  // splits of up to BATCH chunks (BATCH + 1 boundaries)
   const uint64_t* A = sk + s;
   const uint64_t* B = sk + s + na;
   const uint32_t total = na + nb;
-  for (uint32_t D = 0; D < total; D += MERGE_CHUNK) {
-    const uint32_t E = min(total, D + MERGE_CHUNK);
-    if (tid < 2) sp[tid] = merge_split(A, na, B, nb, tid == 0 ? D : E);
+  for (uint32_t D0 = 0; D0 < total; D0 += BATCH * MERGE_CHUNK) {
+    const uint32_t nch = min(BATCH, (total - D0 + MERGE_CHUNK - 1) / MERGE_CHUNK);
+    for (uint32_t k = tid >> 6; k <= nch; k += NW)
+      sp[k] = merge_split_wave(A, na, B, nb, min(total, D0 + k * MERGE_CHUNK));
     __syncthreads();
-    const uint32_t ia = sp[0], ja = D - ia;
-    const uint32_t an = sp[1] - ia, bn = (E - sp[1]) - ja;
-    __syncthreads();
-    for (uint32_t x = tid; x < an + bn; x += RSORT_T) {
-      const uint32_t src = x < an ? s + ia + x : s + na + ja + (x - an);
-      lk[x] = sk[src];
-      if constexpr (VALS) lv[x] = sv[src];
+    for (uint32_t k = 0; k < nch; k++) {
+      const uint32_t D = D0 + k * MERGE_CHUNK, E = min(total, D + MERGE_CHUNK);
+      const uint32_t ia = sp[k], ja = D - ia;
+      const uint32_t an = sp[k + 1] - ia, bn = (E - sp[k + 1]) - ja;
+      uint64_t wk[PER];
+      uint32_t wv[PER];
+#pragma unroll
+      for (uint32_t u = 0; u < PER; u++) {  // all of a thread's window loads in flight at once
+        const uint32_t x = tid + u * RSORT_T;
+        const uint32_t src = x < an ? s + ia + x : s + na + ja + (x - an);
+        wk[u] = x < an + bn ? sk[src] : 0ull;
+        if constexpr (VALS) wv[u] = x < an + bn ? sv[src] : 0u;
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < PER; u++) {
+        const uint32_t x = tid + u * RSORT_T;
+        if (x < an + bn) {
+          lk[x] = wk[u];
+          if constexpr (VALS) lv[x] = wv[u];
+        }
+      }
+      __syncthreads();
+      const uint32_t t0 = min(an + bn, tid * PER), t1 = min(an + bn, t0 + PER);
+      uint32_t i = merge_split(lk, an, lk + an, bn, t0), j = t0 - i;
+      for (uint32_t d = t0; d < t1; d++) {
+        const bool takeA = j >= bn || (i < an && lk[i] <= lk[an + j]);
+        const uint32_t x = takeA ? i : an + j;
+        dk[s + D + d] = lk[x];
+        if constexpr (VALS) dv[s + D + d] = lv[x];
+        if (takeA) i++;
+        else j++;
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    const uint32_t t0 = min(an + bn, tid * PER), t1 = min(an + bn, t0 + PER);
-    uint32_t i = merge_split(lk, an, lk + an, bn, t0), j = t0 - i;
-    for (uint32_t d = t0; d < t1; d++) {
-      const bool takeA = j >= bn || (i < an && lk[i] <= lk[an + j]);
-      const uint32_t x = takeA ? i : an + j;
-      dk[s + D + d] = lk[x];
-      if constexpr (VALS) dv[s + D + d] = lv[x];
-      if (takeA) i++;
-      else j++;
-    }
-    __syncthreads();
   }
 }
 
@@ -1457,6 +1500,138 @@ __device__ __forceinline__ void top_clusters32(const ClusterMem& M, const uint32
   team_sync<T>();
 }
 
+// Adds 1 to hist[v] for every active lane of the wave: one LDS add per distinct
+// value in the wave (cluster counts are mostly 1 or 2, so a per-lane add would
+// queue most of the wave on one address).  All 64 lanes must call it.
+__device__ __forceinline__ void wave_hist_add(uint32_t* hist, uint32_t v, bool act) {
+  uint64_t pend = __ballot(act);
+  while (pend) {
+    const int l0 = __builtin_ctzll(pend);
+    const uint32_t v0 = (uint32_t)readlane_i32((int32_t)v, l0);
+    const uint64_t m = __ballot(act && v == v0);
+    if (lane_id() == l0) atomicAdd(&hist[v0], (uint32_t)__popcll(m));
+    pend &= ~m;
+  }
+}
+
+// Block tier after the segmented sort (G <= CLUSTER_GT_MAX groups, per-group
+// arrays in LDS, g_c0 filled), C > nt clusters: the nt first clusters of the
+// score order (count descending, then group rank, then position in the group)
+// without a sort of all C.  A histogram of the counts (min(count, 255)) gives the
+// count thr of the nt-th cluster: every cluster counted above thr is taken, and
+// of those counted exactly thr the first `need` in (group rank, position) order
+// -- each group's share found from the per-rank tallies of such clusters, and
+// taken by one wave walking the group's clusters in order.  The nt chosen are
+// sorted in LDS and their cluster ids written to idx[0..nt).  Returns false
+// (uniformly, nothing written) when thr falls in the top bin (counts >= 255);
+// the caller then selects by the general path.
+template <int T>
+__device__ __forceinline__ bool top_by_count(const ClusterMem& M, uint32_t C, uint32_t G, uint32_t nt, uint32_t tid) {
+  static_assert(T == 256, "one histogram bin per thread");
+  constexpr uint32_t NW = T / 64;
+  uint32_t* hist = M.gt_cnt + CLUSTER_GT_MAX;  // gt_cnt[0, G) holds the group keys
+  uint32_t* sh = M.gt_misc;
+  uint32_t* g_inv = M.g_c0 + CLUSTER_GT_MAX + 1;  // group of each rank
+  uint32_t* eqr = g_inv + CLUSTER_GT_MAX;          // clusters counted thr, per group rank
+  const uint32_t lane = (uint32_t)lane_id(), wv = tid >> 6;
+  hist[tid] = 0;
+  for (uint32_t q = tid; q < G; q += T) {
+    eqr[q] = 0;
+    g_inv[M.g_rank[q]] = q;
+  }
+  team_sync<T>();
+  constexpr uint32_t U = 4;  // windows of clusters loaded ahead per wave
+  for (uint32_t cb = tid - lane; cb < C; cb += U * T) {
+    uint32_t cnt[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++) {
+      const uint32_t c = cb + u * T + lane;
+      cnt[u] = c < C ? M.cl_start[c + 1] - M.cl_start[c] : 0u;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++) wave_hist_add(hist, min(cnt[u], 255u), cb + u * T + lane < C);
+  }
+  team_sync<T>();
+  // thread t holds bin 255 - t: the inclusive scan is the number counted >= that bin
+  const uint32_t x = hist[255 - tid];
+  uint32_t tot;
+  const uint32_t ex = team_excl_scan<T>(x, tot, sh + 2);
+  if (ex < nt && ex + x >= nt) {
+    sh[0] = 255 - tid;  // thr
+    sh[1] = ex;         // clusters counted above thr
+  }
+  team_sync<T>();
+  const uint32_t thr = sh[0], ngt = sh[1];
+  team_sync<T>();
+  if (thr == 255) return false;
+  if (tid == 0) sh[0] = 0;  // entries appended
+  team_sync<T>();
+  for (uint32_t cb = tid - lane; cb < C; cb += U * T) {
+    uint32_t cnt[U], gi[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++) {
+      const uint32_t c = cb + u * T + lane;
+      cnt[u] = c < C ? M.cl_start[c + 1] - M.cl_start[c] : 0u;
+      gi[u] = c < C ? M.cl_gid[c] : 0u;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++) {
+      const uint32_t c = cb + u * T + lane;
+      const bool act = c < C;
+      const uint32_t q = act ? M.g_rank[gi[u]] : 0u;
+      if (act && cnt[u] > thr) {
+        const uint32_t at = atomicAdd(&sh[0], 1u);
+        M.stage_key[at] = ((uint64_t)(0xFFFFFFFFu - cnt[u]) << 32) | ((uint64_t)q << 23) | c;
+        M.stage_val[at] = c;
+      }
+      wave_hist_add(eqr, q, act && cnt[u] == thr);
+    }
+  }
+  team_sync<T>();
+  // group ranks in order: those whose thr-counted clusters come first take them,
+  // until `need` are taken; the groups with a share are listed in stage_val[32..]
+  const uint32_t need = nt - ngt;
+  if (tid == 0) sh[1] = 0;
+  const uint32_t per = (G + T - 1) / T;
+  uint32_t local = 0;
+  for (uint32_t q = tid * per; q < min(G, tid * per + per); q++) local += eqr[q];
+  uint32_t before = team_excl_scan<T>(local, tot, sh + 2);
+  for (uint32_t q = tid * per; q < min(G, tid * per + per); q++) {
+    const uint32_t e = eqr[q];
+    if (e && before < need) {
+      const uint32_t at = atomicAdd(&sh[1], 1u);
+      M.stage_val[32 + 2 * at] = q;
+      M.stage_val[33 + 2 * at] = min(e, need - before);
+    }
+    before += e;
+  }
+  team_sync<T>();
+  const uint32_t ns = sh[1];
+  for (uint32_t si = wv; si < ns; si += NW) {
+    const uint32_t q = M.stage_val[32 + 2 * si], quota = M.stage_val[33 + 2 * si];
+    const uint32_t g = g_inv[q], cend = M.g_c0[g + 1];
+    uint32_t taken = 0;
+    for (uint32_t cb = M.g_c0[g]; cb < cend && taken < quota; cb += 64) {
+      const uint32_t c = cb + lane;
+      const bool eq = c < cend && M.cl_start[c + 1] - M.cl_start[c] == thr;
+      const uint64_t m = __ballot(eq);
+      if (eq && taken + (uint32_t)__popcll(m & lanemask_lt()) < quota) {
+        const uint32_t at = atomicAdd(&sh[0], 1u);
+        M.stage_key[at] = ((uint64_t)(0xFFFFFFFFu - thr) << 32) | ((uint64_t)q << 23) | c;
+        M.stage_val[at] = c;
+      }
+      taken += (uint32_t)__popcll(m);
+    }
+  }
+  team_sync<T>();
+  for (uint32_t j = nt + tid; j < 32; j += T) M.stage_key[j] = ~0ull;
+  team_sync<T>();
+  team_bitonic<T>(M.stage_key, M.stage_val, 32, tid);
+  for (uint32_t j = tid; j < nt; j += T) M.idx[j] = M.stage_val[j];
+  team_sync<T>();
+  return true;
+}
+
 template <int T, bool FEAT>
 __device__ __forceinline__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const ClusterMem& M0, Cand* out,
                              uint8_t* ncand_out, uint32_t* sh, const ClusterFeat& F) {
@@ -1475,6 +1650,7 @@ __device__ __forceinline__ bool cluster_read(const uint64_t* hits, uint32_t n, u
     M.g_min = M.gt_emin;
     M.g_rank = reinterpret_cast<uint32_t*>(M.gt_lo);
     M.g_sort = M.g_rank + CLUSTER_GT_MAX;
+    M.g_c0 = reinterpret_cast<uint32_t*>(M.gt_hi);  // g_c0, then top_by_count's rank tables
   } else {
     for (uint32_t i = tid; i < p2; i += T) {
       M.key[i] = i < n ? hits[i] : ~0ull;
@@ -1492,7 +1668,7 @@ __device__ __forceinline__ bool cluster_read(const uint64_t* hits, uint32_t n, u
   const uint32_t chunk = (n + T - 1) / T;
   const uint32_t c0 = min(n, tid * chunk), c1 = min(n, c0 + chunk);
   uint32_t ng = 0, nc = 0;
-  constexpr uint32_t U = T == 64 ? 1 : 8;  // keys loaded ahead per thread in the block tier
+  constexpr uint32_t U = T == 64 ? 1 : 16;  // keys loaded ahead per thread in the block tier
   {
     uint64_t kp = c0 > 0 ? M.key[c0 - 1] : 0;
     for (uint32_t e0 = c0; e0 < c1; e0 += U) {
@@ -1524,7 +1700,10 @@ __device__ __forceinline__ bool cluster_read(const uint64_t* hits, uint32_t n, u
       F.f4[c] = 15;
     }
   }
-  if (tid == 0) M.cl_start[C] = n;
+  if (tid == 0) {
+    M.cl_start[C] = n;
+    if (sorted) M.g_c0[G] = C;
+  }
   // the only clusters summed atomically are those that can run past a chunk's end:
   // each chunk's last cluster start, zeroed here; every other sum is a plain store
   if (nc) M.cl_sum[cbase + nc - 1] = 0ull;
@@ -1591,6 +1770,7 @@ __device__ __forceinline__ bool cluster_read(const uint64_t* hits, uint32_t n, u
           cid++;
           M.cl_start[cid] = e;
           M.cl_gid[cid] = (uint32_t)gid;
+          if (sorted && gs) M.g_c0[gid] = (uint32_t)cid;
         }
         psum += hit_diag(k);
         const uint32_t ei = ib[u];
@@ -1672,7 +1852,8 @@ __device__ __forceinline__ bool cluster_read(const uint64_t* hits, uint32_t n, u
   const uint32_t nt = min(C, (uint32_t)TOP_CANDIDATES);
   bool compact = false;
   if constexpr (T != 64) {
-    if (C > 256) {
+    if (sorted && C > 256 && C < (1u << 23)) compact = top_by_count<T>(M, C, G, nt, tid);
+    if (!compact && C > 256) {
       // 32-bit keys (max count - count, group rank, cluster id) when they fit: the
       // cluster id orders a group's clusters as their first sorted hits do
       uint32_t mx = 0;
