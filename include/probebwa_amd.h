@@ -147,12 +147,33 @@ int pbw_genome_n_contigs(const pbw_genome* g);
 const char* pbw_genome_contig_name(const pbw_genome* g, int cid);
 uint64_t pbw_genome_contig_length(const pbw_genome* g, int cid);
 void pbw_genome_destroy(pbw_genome* g);
+/* GenomeIndex::get_sequence (src/index/genome.rs:97-99) -> Contig::slice (:47-68): bases
+ * [start, end) of contig cid, clamped to its length, non-ACGT as 'N'; writes them to out
+ * (when not NULL) and returns their count */
+uint64_t pbw_genome_get_sequence(const pbw_genome* g, int cid, uint64_t start, uint64_t end, uint8_t* out);
+/* GenomeIndex::load / save (src/index/genome.rs:80-94): a bincode-1 `.stidx` (species,
+ * assembly, contigs as name + 2-bit packed bases + N runs + length, total_length) */
+int pbw_genome_load_stidx(const char* path, pbw_genome** out);
+int pbw_genome_save_stidx(const pbw_genome* g, const char* species, const char* assembly, const char* path);
 
 /* ---- context: index build + upload, SingleEndMapper::new (src/mapper/single.rs:52-70),
  *      HashTableBuilder::build (src/index/hashtable.rs:248-322, k=15 stride=5 cap=300) ---- */
 void pbw_default_options(pbw_options* o);
 int pbw_ctx_create(const pbw_genome* g, const pbw_options* o, int device, pbw_ctx** out);
 void pbw_ctx_destroy(pbw_ctx* ctx);
+/* HashTable::load (src/index/hashtable.rs:66-72): the context's buckets read from a bincode-1
+ * `.sthash` (FxHashMap<u32 hash, (offset, count)>, positions: Vec<u64>, k, seed_stride,
+ * total_entries) instead of built from the genome.  k must be 15 and no bucket may hold more
+ * than the builder's cap of 300; `g` must be the genome the table was built from (every
+ * position is checked against its contigs). */
+int pbw_ctx_create_from_sthash(const pbw_genome* g, const char* sthash_path, const pbw_options* o, int device,
+                               pbw_ctx** out);
+/* HashTable::save (src/index/hashtable.rs:74-80): the context's buckets as a `.sthash`,
+ * keyed by hash_kmer (mix of the 30-bit code), in code order */
+int pbw_index_save_sthash(pbw_ctx* ctx, const char* path);
+/* the genome and options a context was created with */
+const pbw_genome* pbw_ctx_genome(const pbw_ctx* ctx);
+int pbw_ctx_get_options(const pbw_ctx* ctx, pbw_options* out);
 const char* pbw_last_error(const pbw_ctx* ctx);
 const char* pbw_global_error(void);
 
@@ -299,10 +320,69 @@ void pbw_records_release(pbw_record_batch* b);
 int pbw_get_stats(const pbw_ctx* ctx, pbw_stats* out);
 
 /* ---- SAM text: format_sam_records (src/io/sam.rs:37-107); ids is n NUL-terminated
- *      strings back to back; free with pbw_free_text ---- */
+ *      strings back to back (NULL: r0, r1, ...); SEQ and QUAL pass through
+ *      String::from_utf8_lossy as the reference's do (sam.rs:50-51); read_group is
+ *      parsed as ReadGroup::parse (sam.rs:17-32) and NULL is returned when it is
+ *      malformed; free with pbw_free_text ---- */
 char* pbw_format_sam(const pbw_genome* g, const pbw_read_batch* reads, const char* ids, const pbw_record_batch* recs,
                      const char* command_line, const char* read_group, uint64_t* len);
 void pbw_free_text(char* t);
+
+/* ---- BAM: write_bam / write_bam_indexed (src/io/bam.rs:19-57) of the records
+ *      pbw_format_sam would write, BGZF-compressed.  With index != 0 the records are
+ *      first coordinate-sorted (sort_for_coordinate_order, src/io/sam.rs:109-124;
+ *      header SO:coordinate) and the BAI is written to `path`.bai.  The reference
+ *      round-trips its SAM text through noodles, which rejects what that text cannot
+ *      carry into BAM: raw_qual = 0 does the same (QUAL bytes -- raw Phred scores in
+ *      this SAM text -- must lie in '!'..'~' and are stored minus 33; QNAME must be
+ *      1..254 bytes of '!'..'~' without '@'; SEQ and QUAL non-empty and equally
+ *      long) and returns PBW_ERR_ARG naming the first record that fails; raw_qual = 1
+ *      stores the Phred scores themselves.  threads: compression threads (0: all). ---- */
+int pbw_write_bam(const pbw_genome* g, const pbw_read_batch* reads, const char* ids, const pbw_record_batch* recs,
+                  const char* command_line, const char* read_group, const char* path, int index, int raw_qual,
+                  uint32_t threads);
+
+/* ---- reads: read_fastq / read_fasta_as_reads (src/io/fastx.rs:128-236) with
+ *      strip_mate_suffix and strip_adapter (fastx.rs:66-118): format 0 FASTQ, 1 FASTA;
+ *      gzip when the path ends in ".gz" (MultiGzDecoder); lines must be UTF-8 (as
+ *      BufRead::read_line into a String requires).  pbw_fastx_next fills `reads` (and
+ *      `ids`, NUL-terminated back to back) with up to max_reads records owned by the
+ *      reader until its next call; reads->n == 0 at the end. ---- */
+typedef struct pbw_fastx pbw_fastx;
+int pbw_fastx_open(const char* path, int format, uint32_t quality_offset, const char* adapter, pbw_fastx** out);
+int pbw_fastx_next(pbw_fastx* f, uint64_t max_reads, pbw_read_batch* reads, const char** ids);
+uint64_t pbw_fastx_bytes_read(const pbw_fastx* f);
+void pbw_fastx_close(pbw_fastx* f);
+
+/* ---- files to SAM/BAM: the map subcommand (src/main.rs:130-195) over
+ *      map_reads_with_format (src/lib.rs:65-143): reads2 == NULL maps reads1
+ *      single-end, otherwise the two files are mates in lockstep (map_paired).
+ *      Single-end input is parsed, mapped and written as overlapping batches
+ *      (records in input order); paired-end input is read whole, as the reference
+ *      reads it.  out_path NULL writes SAM to stdout. ---- */
+typedef struct {
+  int input_format;         /* --inputformat: 0 fastq, 1 fasta */
+  uint32_t quality_offset;  /* 33, or 64 with --phred64 */
+  const char* adapter;      /* --adapter-strip (NULL: none) */
+  const char* label_prefix; /* --labelfilter (NULL: none) */
+  const char* command_line; /* the @PG CL field */
+  const char* read_group;   /* --readgroup (NULL: none) */
+  int output_format;        /* --outputformat: 0 sam, 1 bam */
+  int index;                /* --index: coordinate-sorted BAM plus out_path.bai */
+  int raw_qual;             /* BAM: as pbw_write_bam's raw_qual */
+  uint32_t threads;         /* formatting and compression threads (0: all) */
+  uint64_t batch_reads;     /* single-end reads per mapped batch (262144) */
+} pbw_io_options;
+void pbw_default_io_options(pbw_io_options* o);
+typedef struct {
+  uint64_t reads;      /* reads parsed (both mates) */
+  uint64_t records;    /* records written */
+  uint64_t bytes_in;   /* read-file bytes consumed (compressed for .gz) */
+  uint64_t bytes_out;  /* output bytes */
+  double ms_parse, ms_map, ms_write, ms_total;
+} pbw_io_report;
+int pbw_map_files(pbw_ctx* ctx, const char* reads1, const char* reads2, const char* out_path,
+                  const pbw_io_options* o, pbw_io_report* report);
 
 #ifdef __cplusplus
 }

@@ -1662,30 +1662,102 @@ static Read read_at(const orc_reads* rs, uint64_t i) {
 // ---------------------------------------------------------------------------
 // io/fastx.rs
 // ---------------------------------------------------------------------------
-static inline bool rust_ws(uint8_t c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == 0x0B || c == 0x0C; }
-static std::string trim(const std::string& s) {
-  size_t a = 0, b = s.size();
-  while (a < b && rust_ws((uint8_t)s[a])) a++;
-  while (b > a && rust_ws((uint8_t)s[b - 1])) b--;
-  return s.substr(a, b - a);
+// Rust's str view of a line: UTF-8 (BufRead::read_line into a String rejects
+// anything else), char::is_whitespace for trim / split_whitespace.
+static int utf8_len_at(const uint8_t* p, size_t n, size_t i) {  // > 0 valid length, else -(invalid prefix)
+  uint8_t b0 = p[i];
+  if (b0 < 0x80) return 1;
+  int need;
+  uint8_t lo = 0x80, hi = 0xBF;
+  if (b0 >= 0xC2 && b0 <= 0xDF) need = 1;
+  else if (b0 >= 0xE0 && b0 <= 0xEF) { need = 2; if (b0 == 0xE0) lo = 0xA0; if (b0 == 0xED) hi = 0x9F; }
+  else if (b0 >= 0xF0 && b0 <= 0xF4) { need = 3; if (b0 == 0xF0) lo = 0x90; if (b0 == 0xF4) hi = 0x8F; }
+  else return -1;
+  for (int k = 1; k <= need; k++) {
+    if (i + k >= n) return -k;
+    uint8_t b = p[i + k];
+    if (b < (k == 1 ? lo : 0x80) || b > (k == 1 ? hi : 0xBF)) return -k;
+  }
+  return need + 1;
 }
-static std::string first_token(const std::string& s) {
+static uint32_t utf8_code(const uint8_t* p, int len) {
+  if (len == 1) return p[0];
+  uint32_t c = p[0] & (0x7F >> len);
+  for (int k = 1; k < len; k++) c = (c << 6) | (p[k] & 0x3F);
+  return c;
+}
+static bool unicode_ws(uint32_t c) {
+  return (c >= 9 && c <= 13) || c == 32 || c == 0x85 || c == 0xA0 || c == 0x1680 || (c >= 0x2000 && c <= 0x200A) ||
+         c == 0x2028 || c == 0x2029 || c == 0x202F || c == 0x205F || c == 0x3000;
+}
+static std::vector<std::pair<size_t, int>> utf8_chars(const std::string& s) {  // (offset, length); s valid
+  std::vector<std::pair<size_t, int>> v;
+  const uint8_t* p = (const uint8_t*)s.data();
+  for (size_t i = 0; i < s.size();) {
+    int l = utf8_len_at(p, s.size(), i);
+    v.push_back({i, l});
+    i += l;
+  }
+  return v;
+}
+static bool char_ws(const std::string& s, std::pair<size_t, int> c) {
+  return unicode_ws(utf8_code((const uint8_t*)s.data() + c.first, c.second));
+}
+static std::string trim(const std::string& s) {  // str::trim
+  auto cs = utf8_chars(s);
+  size_t a = 0, b = cs.size();
+  while (a < b && char_ws(s, cs[a])) a++;
+  while (b > a && char_ws(s, cs[b - 1])) b--;
+  if (a == b) return std::string();
+  return s.substr(cs[a].first, cs[b - 1].first + cs[b - 1].second - cs[a].first);
+}
+static std::string first_token(const std::string& s) {  // split_whitespace().next().unwrap_or("")
+  auto cs = utf8_chars(s);
   size_t a = 0;
-  while (a < s.size() && rust_ws((uint8_t)s[a])) a++;
+  while (a < cs.size() && char_ws(s, cs[a])) a++;
   size_t b = a;
-  while (b < s.size() && !rust_ws((uint8_t)s[b])) b++;
-  return s.substr(a, b - a);
+  while (b < cs.size() && !char_ws(s, cs[b])) b++;
+  if (a == b) return std::string();
+  return s.substr(cs[a].first, cs[b - 1].first + cs[b - 1].second - cs[a].first);
+}
+static bool valid_utf8(const std::string& s) {
+  const uint8_t* p = (const uint8_t*)s.data();
+  for (size_t i = 0; i < s.size();) {
+    int l = utf8_len_at(p, s.size(), i);
+    if (l < 0) return false;
+    i += l;
+  }
+  return true;
+}
+// String::from_utf8_lossy: every maximal invalid subpart becomes U+FFFD
+static void append_lossy(std::string& out, const uint8_t* p, size_t n) {
+  for (size_t i = 0; i < n;) {
+    int l = utf8_len_at(p, n, i);
+    if (l > 0) {
+      out.append((const char*)p + i, l);
+      i += l;
+    } else {
+      out += "\xEF\xBF\xBD";
+      i += -l;
+    }
+  }
 }
 struct LineReader {
   FILE* f;
+  bool bad = false;  // a line was not UTF-8: the reference's read_line fails (io::ErrorKind::InvalidData)
   bool read_line(std::string& line) {  // keeps the trailing newline like BufRead::read_line
     line.clear();
+    if (bad) return false;
     int c;
     bool any = false;
     while ((c = fgetc(f)) != EOF) {
       any = true;
       line.push_back((char)c);
       if (c == '\n') break;
+    }
+    if (any && !valid_utf8(line)) {
+      bad = true;
+      return false;
     }
     return any;
   }
@@ -1695,7 +1767,7 @@ struct PReads {
   std::vector<std::vector<uint8_t>> seqs, quals;
 };
 // fastx.rs:255-305 FastqParser::next
-static void parse_fastq(FILE* f, uint8_t offset, PReads& out) {
+static bool parse_fastq(FILE* f, uint8_t offset, PReads& out) {
   LineReader lr{f};
   std::string buf;
   for (;;) {
@@ -1723,9 +1795,10 @@ static void parse_fastq(FILE* f, uint8_t offset, PReads& out) {
     out.seqs.push_back(seq);
     out.quals.push_back(qual);
   }
+  return !lr.bad;
 }
 // fastx.rs:194-236 FastaAsReadsParser::next
-static void parse_fasta_reads(FILE* f, PReads& out) {
+static bool parse_fasta_reads(FILE* f, PReads& out) {
   LineReader lr{f};
   std::string line, pending;
   bool has_pending = false;
@@ -1763,6 +1836,7 @@ static void parse_fasta_reads(FILE* f, PReads& out) {
     out.quals.push_back(std::vector<uint8_t>(seq.size(), 40));  // fastx.rs:157 FASTA_DEFAULT_QUAL
     out.seqs.push_back(seq);
   }
+  return !lr.bad;
 }
 static inline uint8_t ascii_lower(uint8_t c) { return (c >= 'A' && c <= 'Z') ? (uint8_t)(c + 32) : c; }
 // fastx.rs:97-117 strip_adapter
@@ -2165,9 +2239,9 @@ int orc_read_reads(const char* path, int fasta, uint8_t offset, const char* adap
   FILE* f = fopen(path, "rb");
   if (!f) return -1;
   PReads pr;
-  if (fasta) parse_fasta_reads(f, pr);
-  else parse_fastq(f, offset, pr);
+  const bool ok = fasta ? parse_fasta_reads(f, pr) : parse_fastq(f, offset, pr);
   fclose(f);
+  if (!ok) return -2;
   for (size_t i = 0; i < pr.ids.size(); i++) {
     strip_mate_suffix(pr.ids[i]);
     if (adapter) strip_adapter(pr.seqs[i], pr.quals[i], adapter);
@@ -2259,16 +2333,13 @@ char* orc_format_sam(const void* gp, const orc_parsed_reads* rd, const orc_recor
     uint64_t sl = rd->seq_off[i + 1] - rd->seq_off[i];
     const uint8_t* q = rd->qual + rd->qual_off[i];
     uint64_t ql = rd->qual_off[i + 1] - rd->qual_off[i];
-    if (r.seq_rc) {
-      std::vector<uint8_t> rc = reverse_complement(s, sl);
-      out.append((const char*)rc.data(), rc.size());
-      out += "\t";
-      for (uint64_t t = 0; t < ql; t++) out.push_back((char)q[ql - 1 - t]);
-    } else {
-      out.append((const char*)s, sl);
-      out += "\t";
-      out.append((const char*)q, ql);
-    }
+    // sam.rs:50-51: SEQ and QUAL through String::from_utf8_lossy
+    std::vector<uint8_t> sv = r.seq_rc ? reverse_complement(s, sl) : std::vector<uint8_t>(s, s + sl);
+    std::vector<uint8_t> qv(q, q + ql);
+    if (r.seq_rc) std::reverse(qv.begin(), qv.end());
+    append_lossy(out, sv.data(), sv.size());
+    out += "\t";
+    append_lossy(out, qv.data(), qv.size());
     if (rg) out += "\tRG:Z:" + rg_id;
     out += "\n";
   }

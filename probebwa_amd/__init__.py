@@ -51,6 +51,20 @@ EXPORTED_SYMBOLS = (
     "pbw_mlp_get_params", "pbw_mlp_forward", "pbw_mlp_gradient", "pbw_mlp_train", "pbw_default_prefilter_options",
     "pbw_prefilter_set", "pbw_prefilter_get", "pbw_prefilter_dataset", "pbw_prefilter_train",
     "pbw_tune_get", "pbw_tune_set", "pbw_autotune",
+    "pbw_ctx_create_from_sthash",
+    "pbw_ctx_genome",
+    "pbw_ctx_get_options",
+    "pbw_default_io_options",
+    "pbw_fastx_bytes_read",
+    "pbw_fastx_close",
+    "pbw_fastx_next",
+    "pbw_fastx_open",
+    "pbw_genome_get_sequence",
+    "pbw_genome_load_stidx",
+    "pbw_genome_save_stidx",
+    "pbw_index_save_sthash",
+    "pbw_map_files",
+    "pbw_write_bam",
 )
 
 MLP_INPUTS = 32
@@ -193,6 +207,23 @@ class PbwError(RuntimeError):
     pass
 
 
+class IoOptions(C.Structure):
+    """pbw_io_options: the map subcommand's read and output flags (src/main.rs:40-111)."""
+    _fields_ = [("input_format", C.c_int), ("quality_offset", C.c_uint32), ("adapter", C.c_char_p),
+                ("label_prefix", C.c_char_p), ("command_line", C.c_char_p), ("read_group", C.c_char_p),
+                ("output_format", C.c_int), ("index", C.c_int), ("raw_qual", C.c_int), ("threads", C.c_uint32),
+                ("batch_reads", C.c_uint64)]
+
+
+class IoReport(C.Structure):
+    _fields_ = [("reads", C.c_uint64), ("records", C.c_uint64), ("bytes_in", C.c_uint64),
+                ("bytes_out", C.c_uint64), ("ms_parse", C.c_double), ("ms_map", C.c_double),
+                ("ms_write", C.c_double), ("ms_total", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -213,6 +244,22 @@ def lib():
             "pbw_genome_contig_name": (C.c_char_p, [vp, i32]),
             "pbw_genome_contig_length": (u64, [vp, i32]),
             "pbw_genome_destroy": (None, [vp]),
+            "pbw_genome_get_sequence": (u64, [vp, i32, u64, u64, vp]),
+            "pbw_genome_load_stidx": (i32, [C.c_char_p, C.POINTER(vp)]),
+            "pbw_genome_save_stidx": (i32, [vp, C.c_char_p, C.c_char_p, C.c_char_p]),
+            "pbw_ctx_create_from_sthash": (i32, [vp, C.c_char_p, C.POINTER(MapperOptions), i32, C.POINTER(vp)]),
+            "pbw_index_save_sthash": (i32, [vp, C.c_char_p]),
+            "pbw_ctx_genome": (vp, [vp]),
+            "pbw_ctx_get_options": (i32, [vp, C.POINTER(MapperOptions)]),
+            "pbw_write_bam": (i32, [vp, C.POINTER(_ReadBatch), C.c_char_p, C.POINTER(_RecordBatch), C.c_char_p,
+                                    C.c_char_p, C.c_char_p, i32, i32, u32]),
+            "pbw_fastx_open": (i32, [C.c_char_p, i32, u32, C.c_char_p, C.POINTER(vp)]),
+            "pbw_fastx_next": (i32, [vp, u64, C.POINTER(_ReadBatch), C.POINTER(C.c_void_p)]),
+            "pbw_fastx_bytes_read": (u64, [vp]),
+            "pbw_fastx_close": (None, [vp]),
+            "pbw_default_io_options": (None, [C.POINTER(IoOptions)]),
+            "pbw_map_files": (i32, [vp, C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(IoOptions),
+                                    C.POINTER(IoReport)]),
             "pbw_default_options": (None, [C.POINTER(MapperOptions)]),
             "pbw_ctx_create": (i32, [vp, C.POINTER(MapperOptions), i32, C.POINTER(vp)]),
             "pbw_ctx_destroy": (None, [vp]),
@@ -291,6 +338,26 @@ class Genome:
     def add_fasta(self, path):
         _check(lib().pbw_genome_add_fasta(self.h, _b(path)))
         return self
+
+    @classmethod
+    def load_stidx(cls, path):
+        """`GenomeIndex::load` (src/index/genome.rs:80-86): a bincode-1 `.stidx` file."""
+        g = cls.__new__(cls)
+        h = C.c_void_p()
+        _check(lib().pbw_genome_load_stidx(_b(path), C.byref(h)))
+        g.h = h
+        return g
+
+    def save_stidx(self, path, species="", assembly=""):
+        """`GenomeIndex::save` (src/index/genome.rs:88-94)."""
+        _check(lib().pbw_genome_save_stidx(self.h, _b(species), _b(assembly), _b(path)))
+
+    def get_sequence(self, cid, start, end):
+        """`GenomeIndex::get_sequence` (src/index/genome.rs:97-99): bases [start, end), 'N' for non-ACGT."""
+        n = max(0, min(end, self.lengths[cid]) - start)
+        buf = C.create_string_buffer(max(n, 1))
+        got = lib().pbw_genome_get_sequence(self.h, cid, start, end, buf)
+        return buf.raw[:got]
 
     @property
     def names(self):
@@ -398,12 +465,22 @@ class Context:
     (HashTableBuilder, src/index/hashtable.rs:248-322) and owns the HIP stream.
     """
 
-    def __init__(self, genome, options=None, device=0, **kw):
+    def __init__(self, genome, options=None, device=0, sthash=None, **kw):
+        """With `sthash`, the bucket table is read from that `.sthash` file (HashTable::load,
+        src/index/hashtable.rs:66-72) instead of built from the genome."""
         self.genome = genome
         self.options = options if options is not None else MapperOptions(**kw)
         h = C.c_void_p()
-        _check(lib().pbw_ctx_create(genome.h, C.byref(self.options), device, C.byref(h)))
+        if sthash is None:
+            _check(lib().pbw_ctx_create(genome.h, C.byref(self.options), device, C.byref(h)))
+        else:
+            _check(lib().pbw_ctx_create_from_sthash(genome.h, _b(sthash), C.byref(self.options), device,
+                                                    C.byref(h)))
         self.h = h
+
+    def save_sthash(self, path):
+        """`HashTable::save` (src/index/hashtable.rs:74-80) of this context's buckets."""
+        _check(lib().pbw_index_save_sthash(self.h, _b(path)), self.h)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -638,9 +715,77 @@ def format_sam(genome, reads, records, command_line="probebwa", read_group=None)
     ln = C.c_uint64()
     t = lib().pbw_format_sam(genome.h, C.byref(rs), ids, C.byref(rb), _b(command_line),
                              _b(read_group) if read_group else None, C.byref(ln))
+    if not t:
+        _check(-1)
     s = C.string_at(t, ln.value)
     lib().pbw_free_text(t)
     return s
+
+
+def _record_batch(records):
+    rec = np.ascontiguousarray(records.rec)
+    cig = np.ascontiguousarray(records.cigar) if len(records.cigar) else np.zeros(1, np.uint32)
+    return _RecordBatch(len(rec), rec.ctypes.data, cig.ctypes.data, len(records.cigar)), (rec, cig)
+
+
+def write_bam(genome, reads, records, path, command_line="probebwa", read_group=None, index=False,
+              raw_qual=False, threads=0):
+    """`write_bam` / `write_bam_indexed` (src/io/bam.rs:19-57) of the records `format_sam` would
+    write; with `index` the records are coordinate-sorted and `path`.bai is written."""
+    ids = b"".join(_b(i) + b"\0" for i in reads.ids)
+    rs = reads.cstruct()
+    rb, _keep = _record_batch(records)
+    _check(lib().pbw_write_bam(genome.h, C.byref(rs), ids, C.byref(rb), _b(command_line),
+                               _b(read_group) if read_group else None, _b(path), int(bool(index)),
+                               int(bool(raw_qual)), int(threads)))
+
+
+def read_fastx(path, fasta=False, quality_offset=33, adapter=None, batch=1 << 20):
+    """`read_reads_files_with_preprocessing` for one file (src/io/fastx.rs:51-305):
+    a ReadBatch with the record ids, through the library's reader."""
+    h = C.c_void_p()
+    _check(lib().pbw_fastx_open(_b(path), 1 if fasta else 0, int(quality_offset),
+                                _b(adapter) if adapter else None, C.byref(h)))
+    seqs, quals, ids = [], [], []
+    try:
+        while True:
+            rb = _ReadBatch()
+            idp = C.c_void_p()
+            _check(lib().pbw_fastx_next(h, batch, C.byref(rb), C.byref(idp)))
+            if rb.n == 0:
+                break
+            so = np.ctypeslib.as_array(C.cast(rb.seq_off, C.POINTER(C.c_uint64)), (rb.n + 1,))
+            qo = np.ctypeslib.as_array(C.cast(rb.qual_off, C.POINTER(C.c_uint64)), (rb.n + 1,))
+            sq = C.string_at(rb.seq, int(so[-1])) if so[-1] else b""
+            ql = C.string_at(rb.qual, int(qo[-1])) if qo[-1] else b""
+            at = idp.value
+            for i in range(rb.n):
+                seqs.append(sq[so[i]:so[i + 1]])
+                quals.append(ql[qo[i]:qo[i + 1]])
+                t = C.string_at(at)
+                ids.append(t.decode("utf-8", "replace"))
+                at += len(t) + 1
+    finally:
+        lib().pbw_fastx_close(h)
+    return ReadBatch(seqs, quals, ids)
+
+
+def io_options(**kw):
+    o = IoOptions()
+    lib().pbw_default_io_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, _b(v) if isinstance(v, str) else v)
+    return o
+
+
+def map_files(ctx, reads1, reads2=None, out_path=None, **kw):
+    """The map subcommand (src/main.rs:130-195): read file(s) in, SAM or BAM out; returns the
+    pbw_io_report as a dict."""
+    o = io_options(**kw)
+    rep = IoReport()
+    _check(lib().pbw_map_files(ctx.h, _b(reads1), _b(reads2) if reads2 else None,
+                               _b(out_path) if out_path else None, C.byref(o), C.byref(rep)))
+    return rep.as_dict()
 
 
 def code_of(kmer):

@@ -23,6 +23,7 @@ using namespace pbw;
 // errors
 // ---------------------------------------------------------------------------
 static thread_local std::string g_error;
+void pbw::set_global_error(const std::string& e) { g_error = e; }
 
 struct PbwError {
   int code;
@@ -117,32 +118,6 @@ static void genome_add(pbw_genome* g, const std::string& name, const uint8_t* se
 static inline uint32_t gbase2(const pbw_genome* g, uint64_t gb) { return (uint32_t)(g->g2b[gb >> 5] >> ((gb & 31) * 2)) & 3u; }
 static inline bool gisn(const pbw_genome* g, uint64_t gb) { return (g->nmask[gb >> 6] >> (gb & 63)) & 1ull; }
 
-static inline bool rust_ws(uint8_t c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == 0x0B || c == 0x0C; }
-static std::string trim_str(const std::string& s) {
-  size_t a = 0, b = s.size();
-  while (a < b && rust_ws((uint8_t)s[a])) a++;
-  while (b > a && rust_ws((uint8_t)s[b - 1])) b--;
-  return s.substr(a, b - a);
-}
-static std::string first_tok(const std::string& s) {
-  size_t a = 0;
-  while (a < s.size() && rust_ws((uint8_t)s[a])) a++;
-  size_t b = a;
-  while (b < s.size() && !rust_ws((uint8_t)s[b])) b++;
-  return s.substr(a, b - a);
-}
-static bool read_line(FILE* f, std::string& line) {
-  line.clear();
-  int c;
-  bool any = false;
-  while ((c = fgetc(f)) != EOF) {
-    any = true;
-    line.push_back((char)c);
-    if (c == '\n') break;
-  }
-  return any;
-}
-
 // ---------------------------------------------------------------------------
 // host copies of the reference's scalar math (for LUTs)
 // ---------------------------------------------------------------------------
@@ -203,6 +178,7 @@ struct pbw_ctx {
   DevBuf<uint64_t> d_kpos;
   DevBuf<uint16_t> d_kfp;
   uint64_t total_entries = 0;
+  const HostTable* table_src = nullptr;  // set while a context is built from a loaded `.sthash`
   // LUTs
   DevBuf<double> d_lut;  // packed double tables
   DevBuf<int32_t> d_mis;
@@ -315,8 +291,14 @@ static void ctx_build(pbw_ctx* c) {
     uint16_t* kfp = nullptr;
     uint64_t nent = 0;
     std::string err;
-    if (gpu_build_index(c->dev_genome(), g->lengths.data(), c->d_koff.p, &kpos, &kfp, &nent, s, err) != hipSuccess)
+    if (c->table_src) {  // HashTable::load (hashtable.rs:66-72)
+      if (gpu_index_from_table(c->dev_genome(), g->lengths.data(), *c->table_src, c->d_koff.p, &kpos, &kfp, &nent,
+                               s, err) != hipSuccess)
+        throw PbwError{PBW_ERR_IO, "table: " + err};
+    } else if (gpu_build_index(c->dev_genome(), g->lengths.data(), c->d_koff.p, &kpos, &kfp, &nent, s, err) !=
+               hipSuccess) {
       throw PbwError{err.find("2^32") != std::string::npos ? PBW_ERR_LIMIT : PBW_ERR_HIP, err};
+    }
     c->d_kpos.release();
     c->d_kpos.p = kpos;
     c->d_kpos.n = std::max<uint64_t>(nent, 1);
@@ -1718,49 +1700,6 @@ int pbw_genome_add_contig(pbw_genome* g, const char* name, const uint8_t* seq, u
   genome_add(g, name, seq, len);
   return PBW_OK;
 }
-// genome.rs:119-134 + FastaReader (:150-213)
-int pbw_genome_add_fasta(pbw_genome* g, const char* path) {
-  if (!g || !path) return PBW_ERR_ARG;
-  FILE* f = fopen(path, "rb");
-  if (!f) {
-    g_error = std::string("cannot open ") + path;
-    return PBW_ERR_IO;
-  }
-  std::string line, pending;
-  bool has_pending = false;
-  for (;;) {
-    std::string header;
-    if (has_pending) {
-      header = pending;
-      has_pending = false;
-    } else {
-      bool found = false;
-      while (read_line(f, line))
-        if (!line.empty() && line[0] == '>') {
-          found = true;
-          break;
-        }
-      if (!found) break;
-      header = line;
-    }
-    size_t a = 0;
-    while (a < header.size() && header[a] == '>') a++;
-    std::string name = first_tok(header.substr(a));
-    std::vector<uint8_t> seq;
-    while (read_line(f, line)) {
-      std::string t = trim_str(line);
-      if (!t.empty() && t[0] == '>') {
-        pending = line;
-        has_pending = true;
-        break;
-      }
-      seq.insert(seq.end(), t.begin(), t.end());
-    }
-    genome_add(g, name, seq.data(), seq.size());
-  }
-  fclose(f);
-  return PBW_OK;
-}
 int pbw_genome_n_contigs(const pbw_genome* g) { return g ? (int)g->lengths.size() : 0; }
 const char* pbw_genome_contig_name(const pbw_genome* g, int cid) { return g->names[cid].c_str(); }
 uint64_t pbw_genome_contig_length(const pbw_genome* g, int cid) { return g->lengths[cid]; }
@@ -1777,10 +1716,26 @@ void pbw_default_options(pbw_options* o) {
   o->batch_reads = 0;
 }
 
-int pbw_ctx_create(const pbw_genome* g, const pbw_options* o, int device, pbw_ctx** out) {
+// Contig::slice (genome.rs:47-68) through GenomeIndex::get_sequence (:97-99):
+// bases [start, end) of contig cid clamped to its length, non-ACGT as 'N'.
+uint64_t pbw_genome_get_sequence(const pbw_genome* g, int cid, uint64_t start, uint64_t end, uint8_t* out) {
+  if (!g || cid < 0 || (size_t)cid >= g->lengths.size()) return 0;
+  const uint64_t L = g->lengths[cid];
+  end = std::min(end, L);
+  if (start >= end) return 0;
+  static const char BASES[4] = {'A', 'C', 'G', 'T'};
+  const uint64_t base = g->coff[cid];
+  if (out)
+    for (uint64_t p = start; p < end; p++)
+      out[p - start] = gisn(g, base + p) ? 'N' : (uint8_t)BASES[gbase2(g, base + p)];
+  return end - start;
+}
+
+static int ctx_create(const pbw_genome* g, const pbw_options* o, int device, const HostTable* table, pbw_ctx** out) {
   if (!g || !out) return PBW_ERR_ARG;
   pbw_ctx* c = new pbw_ctx();
   c->genome = g;
+  c->table_src = table;
   if (o) c->opt = *o;
   else pbw_default_options(&c->opt);
   c->device = device;
@@ -1793,8 +1748,55 @@ int pbw_ctx_create(const pbw_genome* g, const pbw_options* o, int device, pbw_ct
     *out = nullptr;
     return rc;
   }
+  c->table_src = nullptr;
   *out = c;
   return PBW_OK;
+}
+
+int pbw_ctx_create(const pbw_genome* g, const pbw_options* o, int device, pbw_ctx** out) {
+  return ctx_create(g, o, device, nullptr, out);
+}
+
+int pbw_ctx_create_from_sthash(const pbw_genome* g, const char* sthash_path, const pbw_options* o, int device,
+                               pbw_ctx** out) {
+  if (!g || !sthash_path || !out) return PBW_ERR_ARG;
+  *out = nullptr;
+  HostTable t;
+  std::string err;
+  if (!read_sthash(sthash_path, t, err)) {
+    g_error = err;
+    return PBW_ERR_IO;
+  }
+  return ctx_create(g, o, device, &t, out);
+}
+
+// HashTable::save (hashtable.rs:74-80): the context's buckets in code order, each
+// keyed by mix(code) (hash_kmer, hashtable.rs:107-113), positions in bucket order.
+int pbw_index_save_sthash(pbw_ctx* c, const char* path) {
+  if (!c || !path) return PBW_ERR_ARG;
+  return guarded(c, [&] {
+    HIP_CHECK(hipSetDevice(c->device));
+    HostTable t;
+    t.k = K;
+    t.stride = STRIDE;
+    t.total_entries = c->total_entries;
+    t.pos.resize(c->total_entries);
+    if (c->total_entries)
+      HIP_CHECK(hipMemcpy(t.pos.data(), c->d_kpos.p, c->total_entries * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    const uint64_t CH = 1ull << 24;
+    std::vector<uint32_t> ko(CH + 1);
+    for (uint64_t c0 = 0; c0 < CODE_SPACE; c0 += CH) {
+      HIP_CHECK(hipMemcpy(ko.data(), c->d_koff.p + c0, (CH + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+      for (uint64_t j = 0; j < CH; j++)
+        if (ko[j + 1] > ko[j]) {
+          t.key.push_back(kmer_mix((uint32_t)(c0 + j)));
+          t.off.push_back(ko[j]);
+          t.cnt.push_back(ko[j + 1] - ko[j]);
+        }
+    }
+    std::string err;
+    if (!write_sthash(path, t, err)) throw PbwError{PBW_ERR_IO, err};
+  });
 }
 
 void pbw_ctx_destroy(pbw_ctx* c) {
@@ -2289,6 +2291,12 @@ int64_t pbw_index_lookup(pbw_ctx* c, uint32_t code, uint64_t* out, uint64_t cap)
   return rc == PBW_OK ? result : rc;
 }
 uint64_t pbw_index_total_entries(const pbw_ctx* c) { return c ? c->total_entries : 0; }
+const pbw_genome* pbw_ctx_genome(const pbw_ctx* c) { return c ? c->genome : nullptr; }
+int pbw_ctx_get_options(const pbw_ctx* c, pbw_options* out) {
+  if (!c || !out) return PBW_ERR_ARG;
+  *out = c->opt;
+  return PBW_OK;
+}
 
 void pbw_records_release(pbw_record_batch* b) {
   if (!b) return;
@@ -2304,77 +2312,6 @@ int pbw_get_stats(const pbw_ctx* c, pbw_stats* out) {
   return PBW_OK;
 }
 
-// sam.rs:37-58 format_sam, :83-107 header (QUAL: raw Phred bytes, sam.rs:52)
-char* pbw_format_sam(const pbw_genome* g, const pbw_read_batch* rd, const char* ids, const pbw_record_batch* recs,
-                     const char* cmd, const char* rg, uint64_t* len) {
-  std::string out = "@HD\tVN:1.6\tSO:unsorted\n";
-  for (size_t i = 0; i < g->names.size(); i++)
-    out += "@SQ\tSN:" + g->names[i] + "\tLN:" + std::to_string(g->lengths[i]) + "\n";
-  out += std::string("@PG\tID:probebwa\tPN:probebwa\tVN:0.1.0\tCL:") + (cmd ? cmd : "") + "\n";
-  std::string rg_id;
-  if (rg) {  // sam.rs:11-32
-    std::string spec(rg), hdr = "@RG";
-    size_t st = 0;
-    for (;;) {
-      size_t e = spec.find(',', st);
-      std::string pair = spec.substr(st, e == std::string::npos ? std::string::npos : e - st);
-      size_t colon = pair.find(':');
-      std::string tag = pair.substr(0, colon), val = colon == std::string::npos ? "" : pair.substr(colon + 1);
-      if (tag == "ID") rg_id = val;
-      hdr += "\t" + tag + ":" + val;
-      if (e == std::string::npos) break;
-      st = e + 1;
-    }
-    out += hdr + "\n";
-  }
-  static const char OPS[] = "MIDNSHP=X";
-  const char* idp = ids;
-  for (uint64_t i = 0; i < recs->n; i++) {
-    const pbw_record& r = recs->rec[i];
-    std::string id = idp ? std::string(idp) : ("r" + std::to_string(i));
-    if (idp) idp += strlen(idp) + 1;
-    out += id;
-    out += "\t" + std::to_string(r.flag);
-    out += "\t" + (r.contig < 0 ? std::string("*") : g->names[r.contig]);
-    out += "\t" + std::to_string(r.pos);
-    out += "\t" + std::to_string(r.mapq);
-    out += "\t";
-    if (r.cigar_n == 0) out += "*";
-    for (uint32_t k = 0; k < r.cigar_n; k++) {
-      uint32_t w = recs->cigar[r.cigar_off + k];
-      out += std::to_string(w >> 4);
-      out += OPS[w & 15];
-    }
-    out += "\t" + (r.mate_contig < 0 ? std::string("*") : g->names[r.mate_contig]);
-    out += "\t" + std::to_string(r.mate_pos);
-    out += "\t" + std::to_string(r.tlen);
-    out += "\t";
-    const uint8_t* sq = rd->seq + rd->seq_off[i];
-    uint64_t sl = rd->seq_off[i + 1] - rd->seq_off[i];
-    const uint8_t* q = rd->qual + rd->qual_off[i];
-    uint64_t ql = rd->qual_off[i + 1] - rd->qual_off[i];
-    if (r.seq_rc) {
-      for (uint64_t t = 0; t < sl; t++) {
-        uint8_t b = sq[sl - 1 - t];
-        out.push_back(b == 'A' ? 'T' : b == 'T' ? 'A' : b == 'C' ? 'G' : b == 'G' ? 'C' : 'N');
-      }
-      out += "\t";
-      for (uint64_t t = 0; t < ql; t++) out.push_back((char)q[ql - 1 - t]);
-    } else {
-      out.append((const char*)sq, sl);
-      out += "\t";
-      out.append((const char*)q, ql);
-    }
-    if (rg) out += "\tRG:Z:" + rg_id;
-    out += "\n";
-  }
-  char* t = (char*)malloc(out.size() + 1);
-  memcpy(t, out.data(), out.size());
-  t[out.size()] = 0;
-  if (len) *len = out.size();
-  return t;
-}
-void pbw_free_text(char* t) { free(t); }
 
 void pbw_default_train_options(pbw_train_options* o) {
   if (!o) return;

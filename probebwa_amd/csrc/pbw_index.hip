@@ -285,6 +285,58 @@ __global__ __launch_bounds__(SCAN_T) void k_max_apply(uint32_t* __restrict__ x, 
   BSt(tmp.store).Store(x + base, v, valid);
 }
 
+// Buckets of a loaded table: the capped count of each key's code into koff
+// (counts first, scanned into starts afterwards).  bad[0] counts keys that are
+// not the mix of a 15-mer code and bucket ranges outside the position store.
+__global__ void k_tab_counts(const uint32_t* __restrict__ key, const uint32_t* __restrict__ off,
+                             const uint32_t* __restrict__ cnt, uint64_t n, uint64_t npos, uint32_t* __restrict__ koff,
+                             uint32_t* __restrict__ bad) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t code = kmer_unmix(key[i]);
+    if (code >= CODE_SPACE || (uint64_t)off[i] + cnt[i] > npos) {
+      atomicAdd(bad, 1u);
+      continue;
+    }
+    // a second bucket of the same code (a malformed map) would overrun the first
+    if (atomicExch(&koff[code], cnt[i] | 0x80000000u) != 0u) atomicAdd(bad, 1u);
+  }
+}
+
+__global__ void k_tab_strip(uint32_t* __restrict__ koff, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) koff[i] &= 0x7FFFFFFFu;
+}
+
+// Each bucket's positions copied to its place in code order, with the genome
+// flank fingerprint of every entry; bad[1] counts positions whose contig or
+// window does not exist in the genome (nothing is written for them).
+__global__ void k_tab_place(const uint32_t* __restrict__ key, const uint32_t* __restrict__ off,
+                            const uint32_t* __restrict__ cnt, uint64_t n, const uint64_t* __restrict__ src,
+                            const uint32_t* __restrict__ koff, const uint64_t* __restrict__ g2b,
+                            const uint64_t* __restrict__ nmask, const uint64_t* __restrict__ coff,
+                            const uint64_t* __restrict__ clen, int32_t nctg, uint64_t* __restrict__ kpos,
+                            uint16_t* __restrict__ kfp, uint32_t* __restrict__ bad) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t code = kmer_unmix(key[i]);
+    if (code >= CODE_SPACE) continue;
+    const uint64_t at = koff[code];
+    for (uint32_t j = 0; j < cnt[i]; j++) {
+      const uint64_t p = src[(uint64_t)off[i] + j];
+      const uint64_t c = (p >> 40) & ((1ull << 23) - 1), pos = p & ((1ull << 40) - 1);
+      if (c >= (uint64_t)nctg || pos + K > clen[c]) {
+        atomicAdd(bad + 1, 1u);
+        kpos[at + j] = 0;
+        kfp[at + j] = GENOME_FP_OPEN;
+        continue;
+      }
+      kpos[at + j] = p;
+      kfp[at + j] = genome_flank_fp(g2b, nmask, coff[c], clen[c], pos);
+    }
+  }
+}
+
 struct Tmp {
   void* p = nullptr;
   ~Tmp() {
@@ -381,6 +433,96 @@ hipError_t gpu_build_index(const DevGenome& G, const uint64_t* h_clen, uint32_t*
   *kpos_out = kpos;
   *kfp_out = kfp;
   *n_entries = total;
+  return hipSuccess;
+}
+
+hipError_t gpu_index_from_table(const DevGenome& G, const uint64_t* h_clen, const HostTable& t, uint32_t* koff,
+                                uint64_t** kpos_out, uint16_t** kfp_out, uint64_t* n_entries, hipStream_t s,
+                                std::string& err) {
+  (void)h_clen;
+  *kpos_out = nullptr;
+  *kfp_out = nullptr;
+  *n_entries = 0;
+  const uint64_t n = t.key.size(), npos = t.pos.size();
+  if (t.off.size() != n || t.cnt.size() != n) {
+    err = "malformed table";
+    return hipErrorInvalidValue;
+  }
+  uint64_t total = 0;
+  for (uint64_t i = 0; i < n; i++) {
+    if (t.cnt[i] > (uint32_t)MAX_COUNT) {
+      err = "a table bucket holds more than the 300 entries of the reference's cap";
+      return hipErrorInvalidValue;
+    }
+    total += t.cnt[i];
+  }
+  if (total >= (1ull << 32)) {
+    err = "more than 2^32 - 1 indexed positions";
+    return hipErrorInvalidValue;
+  }
+  IDX_CHECK(hipMemsetAsync(koff, 0, (CODE_SPACE + 1) * sizeof(uint32_t), s));
+  const uint64_t nk = CODE_SPACE + 1;
+  const uint32_t np_k = (uint32_t)((nk + SCAN_TILE - 1) / SCAN_TILE);
+  Tmp t_part, t_key, t_off, t_cnt, t_src, t_bad;
+  IDX_CHECK(hipMalloc(&t_part.p, (size_t)np_k * sizeof(uint32_t)));
+  IDX_CHECK(hipMalloc(&t_bad.p, 2 * sizeof(uint32_t)));
+  IDX_CHECK(hipMemsetAsync(t_bad.p, 0, 2 * sizeof(uint32_t), s));
+  uint32_t* part = (uint32_t*)t_part.p;
+  uint32_t* bad = (uint32_t*)t_bad.p;
+  const dim3 grid(16384), block(256);
+  uint32_t *dkey = nullptr, *doff = nullptr, *dcnt = nullptr;
+  uint64_t* dsrc = nullptr;
+  if (n) {
+    IDX_CHECK(hipMalloc(&t_key.p, n * sizeof(uint32_t)));
+    IDX_CHECK(hipMalloc(&t_off.p, n * sizeof(uint32_t)));
+    IDX_CHECK(hipMalloc(&t_cnt.p, n * sizeof(uint32_t)));
+    IDX_CHECK(hipMalloc(&t_src.p, std::max<uint64_t>(npos, 1) * sizeof(uint64_t)));
+    dkey = (uint32_t*)t_key.p;
+    doff = (uint32_t*)t_off.p;
+    dcnt = (uint32_t*)t_cnt.p;
+    dsrc = (uint64_t*)t_src.p;
+    IDX_CHECK(hipMemcpyAsync(dkey, t.key.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    IDX_CHECK(hipMemcpyAsync(doff, t.off.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    IDX_CHECK(hipMemcpyAsync(dcnt, t.cnt.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    if (npos) IDX_CHECK(hipMemcpyAsync(dsrc, t.pos.data(), npos * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_tab_counts, grid, block, 0, s, dkey, doff, dcnt, n, npos, koff, bad);
+    hipLaunchKernelGGL(k_tab_strip, grid, block, 0, s, koff, nk);
+    IDX_CHECK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_scan_tiles, dim3(np_k), dim3(SCAN_T), 0, s, koff, nk, part);
+  hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(1024), 0, s, part, np_k);
+  hipLaunchKernelGGL(k_scan_apply, dim3(np_k), dim3(SCAN_T), 0, s, koff, nk, part);
+  IDX_CHECK(hipGetLastError());
+  uint32_t h_bad[2] = {0, 0}, placed = 0;
+  IDX_CHECK(hipMemcpyAsync(h_bad, bad, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  IDX_CHECK(hipMemcpyAsync(&placed, koff + CODE_SPACE, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  IDX_CHECK(hipStreamSynchronize(s));
+  if (h_bad[0]) {
+    err = std::to_string(h_bad[0]) +
+          " table buckets whose key is no 15-mer's hash, repeats another's, or points past the positions";
+    return hipErrorInvalidValue;
+  }
+  uint64_t* kpos = nullptr;
+  uint16_t* kfp = nullptr;
+  IDX_CHECK(hipMalloc((void**)&kpos, std::max<uint64_t>(placed, 1) * sizeof(uint64_t)));
+  IDX_CHECK(hipMalloc((void**)&kfp, std::max<uint64_t>(placed, 1) * sizeof(uint16_t)));
+  if (n) {
+    hipLaunchKernelGGL(k_tab_place, grid, block, 0, s, dkey, doff, dcnt, n, dsrc, koff, G.g2b, G.nmask, G.coff,
+                       G.clen, G.ncontigs, kpos, kfp, bad);
+  }
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(h_bad, bad, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess || h_bad[1]) {
+    (void)hipFree(kpos);
+    (void)hipFree(kfp);
+    err = e != hipSuccess ? std::string("index from table failed: ") + hipGetErrorString(e)
+                          : std::to_string(h_bad[1]) + " table positions outside the genome's contigs";
+    return e != hipSuccess ? e : hipErrorInvalidValue;
+  }
+  *kpos_out = kpos;
+  *kfp_out = kfp;
+  *n_entries = placed;
   return hipSuccess;
 }
 

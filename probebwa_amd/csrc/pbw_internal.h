@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 namespace pbw {
 
@@ -324,6 +325,45 @@ void launch_scatter_u64(uint64_t* out, const uint32_t* idx, const uint64_t* val,
 // *kpos (hipMalloc, owned by the caller) with *n_entries packed positions.
 hipError_t gpu_build_index(const DevGenome& G, const uint64_t* h_clen, uint32_t* koff, uint64_t** kpos,
                            uint16_t** kfp, uint64_t* n_entries, hipStream_t s, std::string& err);
+
+// A persisted `.sthash` (HashTable, src/index/hashtable.rs:53-72) read into host
+// arrays: per bucket its key (mix(code)), offset and count into `pos`.
+struct HostTable {
+  std::vector<uint32_t> key, off, cnt;
+  std::vector<uint64_t> pos;
+  uint64_t k = 0, stride = 0, total_entries = 0;
+};
+// Reads a bincode-1 `.sthash` file (pbw_io.cpp); false with `err` set on a
+// malformed or unreadable file.
+bool read_sthash(const char* path, HostTable& t, std::string& err);
+bool write_sthash(const char* path, const HostTable& t, std::string& err);
+// Sets the message pbw_global_error() returns on this thread (pbw_host.cpp).
+void set_global_error(const std::string& e);
+// The same bucket table built from a loaded HostTable instead of the genome:
+// koff by code (code = the inverse of mix of each key), kpos in bucket order,
+// kfp from the genome; every position is checked against the contigs.
+hipError_t gpu_index_from_table(const DevGenome& G, const uint64_t* h_clen, const HostTable& t, uint32_t* koff,
+                                uint64_t** kpos, uint16_t** kfp, uint64_t* n_entries, hipStream_t s,
+                                std::string& err);
+// hashtable.rs:117-124 mix and its inverse (each step is a bijection of u32)
+__host__ __device__ inline uint32_t kmer_mix(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x7feb352du;
+  h ^= h >> 15;
+  h *= 0x846ca68bu;
+  h ^= h >> 16;
+  return h;
+}
+__host__ __device__ inline uint32_t kmer_unmix(uint32_t h) {
+  constexpr uint32_t INV_B = 0x43021123u;  // 0x846ca68b^-1 mod 2^32
+  constexpr uint32_t INV_A = 0x1d69e2a5u;  // 0x7feb352d^-1 mod 2^32
+  h ^= h >> 16;
+  h *= INV_B;
+  h ^= (h >> 15) ^ (h >> 30);
+  h *= INV_A;
+  h ^= h >> 16;
+  return h;
+}
 
 __device__ __forceinline__ uint16_t float_to_bf16(float f) {  // round to nearest even
   uint32_t u = __float_as_uint(f);
