@@ -39,3 +39,21 @@ def test_code_of_matches_hash_packing():
     assert P.code_of("AAAAAAAAAAAAAAA") == 0
     assert P.code_of("TTTTTTTTTTTTTTT") == (1 << 30) - 1
     assert P.code_of("CAAAAAAAAAAAAAA") == 1 << 28
+
+
+def test_rust_crate_declares_the_abi():
+    """probebwa_amd/rust (the probebwa-amd-sys crate; cargo is absent here, so it is not
+    built): every function of the header is declared in its extern block, and every
+    #[repr(C)] struct has the C struct's field count."""
+    src = open(os.path.join(ROOT, "probebwa_amd", "rust", "src", "lib.rs")).read()
+    ext = src[src.index('extern "C" {'):]
+    ext = ext[:ext.index("\n}\n")]
+    declared = sorted(set(re.findall(r"pub fn (pbw_[a-z_0-9]+)\(", ext)))
+    assert declared == header_symbols()
+    text = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "probebwa_amd.h")).read(), flags=re.S)
+    for body, name in re.findall(r"typedef struct \{(.*?)\} (pbw_[a-z_]+);", text, flags=re.S):
+        n_c = sum(len(decl.split(",")) for decl in body.split(";") if decl.strip())
+        m = re.search(r"pub struct %s \{(.*?)\n\}" % name, src, flags=re.S)
+        assert m, name
+        n_r = len(re.findall(r"pub [a-z_0-9]+:", m.group(1)))
+        assert n_r == n_c, (name, n_r, n_c)
