@@ -243,6 +243,35 @@ def pe_leg(ctx, contigs, pairs):
             "data": "FR pairs of 2x150 (fragment N(350,50) clipped to [200,600]), seed 58"}
 
 
+def e2e_leg(ctx, reads, n, batch):
+    """End to end, FASTQ in -> SAM out (the map subcommand, src/main.rs:130-195): the first n
+    reads written as a FASTQ file, then parsed, mapped and written as SAM by pbw_map_files in
+    batches of `batch` reads (parsing the next batch and formatting the last overlap the
+    mapping of the current one); one untimed call, then one timed."""
+    import shutil
+    import tempfile
+    import probebwa_amd as P
+    seq, so, qual, qo = reads
+    d = tempfile.mkdtemp(prefix="pbw_e2e_")
+    try:
+        fq, sam = os.path.join(d, "reads.fq"), os.path.join(d, "out.sam")
+        q33 = (np.asarray(qual[:qo[n]], np.uint16) + 33).astype(np.uint8).tobytes()
+        sb = np.asarray(seq[:so[n]], np.uint8).tobytes()
+        with open(fq, "wb") as f:
+            for i in range(n):
+                f.write(b"@r%d\n%s\n+\n%s\n" % (i, sb[so[i]:so[i + 1]], q33[qo[i]:qo[i + 1]]))
+        P.map_files(ctx, fq, out_path=sam, command_line="bench", batch_reads=batch)
+        rep = P.map_files(ctx, fq, out_path=sam, command_line="bench", batch_reads=batch)
+        log("end to end: %d reads in %.2f s" % (n, rep["ms_total"] / 1000.0))
+        return {"reads": n, "reads_per_s": round(n / (rep["ms_total"] / 1000.0), 1),
+                "ms_total": round(rep["ms_total"], 1), "ms_parse": round(rep["ms_parse"], 1),
+                "ms_map": round(rep["ms_map"], 1), "ms_format_write": round(rep["ms_write"], 1),
+                "fastq_bytes": int(rep["bytes_in"]), "sam_bytes": int(rep["bytes_out"]), "batch_reads": batch,
+                "data": "the workload's first %d reads as FASTQ (Phred+33), SAM to a local file" % n}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def cpu_leg(args, contigs, reads, first_recs, sample, sample_1t):
     """The oracle (C++ restatement of the reference CPU path) on bounded samples of the timed
     reads: `sample` reads on HOST_THREADS threads (the baseline) and `sample_1t` on one; the
@@ -303,6 +332,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-pe", action="store_true")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the FASTQ -> SAM end-to-end leg")
     ap.add_argument("--autotune", action="store_true",
                     help="tune launch shapes on the first 524288 reads before the timed steps")
     ap.add_argument("--train-reads", type=int, default=None)
@@ -422,6 +452,10 @@ def main():
     if wl["pe_pairs"] and not args.no_pe:
         pe = pe_leg(ctx, contigs, wl["pe_pairs"])
 
+    e2e = None
+    if not args.no_e2e:
+        e2e = e2e_leg(ctx, (seq, so, qual, qo), min(3 * B, n), B)
+
     cpu = cpu1 = concord = order = None
     if not args.no_cpu:
         big = args.workload == "genome_se_3gbp"
@@ -455,6 +489,7 @@ def main():
         "learned": learned,
         "learned_vs_plain": learned["speedup_vs_plain"] if learned else None,
         "paired_end": pe,
+        "end_to_end_fastq_to_sam": e2e,
         "cpu_baseline": cpu,
         "cpu_baseline_1_thread": cpu1,
         "sample_sam_concordance": concord,
