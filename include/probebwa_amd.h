@@ -237,7 +237,8 @@ int pbw_align_batch(pbw_ctx* ctx, uint32_t gap_open_phred, uint32_t gap_extend_p
  *      serving loops, harnesses that forbid hipMalloc).  The same jobs, results and
  *      kernels as pbw_align_batch, on a fixed 64 MiB device region that each GPU the
  *      library runs on holds until the process ends, and a fixed 16 MiB host staging
- *      area; a batch needing more returns PBW_ERR_LIMIT.  Calls run one at a time (one
+ *      area; a batch needing more, a read over 1024 bases or a window over 8000
+ *      returns PBW_ERR_LIMIT.  Calls run one at a time (one
  *      host lock); the region keeps the last call's contents.  The context (created
  *      beforehand) is the only set-up. ---- */
 int pbw_align_batch_noalloc(pbw_ctx* ctx, uint32_t gap_open_phred, uint32_t gap_extend_phred,

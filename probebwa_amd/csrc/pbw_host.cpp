@@ -198,6 +198,7 @@ struct pbw_ctx {
   DevBuf<DpOut> d_out;
   DevBuf<Counters> d_ctr;
   DevBuf<int32_t> d_rows;
+  DevBuf<uint8_t> d_gstage;  // DP staging in device memory for tiles beyond LDS
   DevBuf<uint64_t> d_tb;
   // paired-end workspace
   DevBuf<DpJob> d_wjobs;
@@ -450,6 +451,55 @@ static uint32_t download_u32(pbw_ctx* c, const uint32_t* d) {
   return v;
 }
 
+// Device-memory staging of `bytes` per wave for up to `want` waves (within
+// 2 GiB) when a DP launch's tile does not fit LDS; bytes = 0 clears it.
+static void dp_gstage(pbw_ctx* c, DpArgs& a, uint64_t bytes, uint32_t want) {
+  a.gstage = nullptr;
+  a.gstage_stride = 0;
+  a.gstage_waves = 0;
+  if (!bytes) return;
+  const uint64_t fit = std::max<uint64_t>(1, (2ull << 30) / bytes);
+  const uint32_t w = (uint32_t)std::min<uint64_t>(std::max<uint32_t>(want, 1), fit);
+  c->d_gstage.ensure((size_t)w * bytes);
+  a.gstage = c->d_gstage.p;
+  a.gstage_stride = bytes;
+  a.gstage_waves = w;
+}
+
+// k_dp_band over a's job list, its staging in device memory when the tile for
+// (max_m, max_n) does not fit LDS (long reads).
+static void run_dp_band(pbw_ctx* c, DpArgs a) {
+  if (!a.njobs) return;
+  dp_gstage(c, a, dp_band_gstage_bytes(a.max_m, a.max_n), std::min<uint32_t>(a.njobs, 16384));
+  launch_dp_band(a, c->stream);
+}
+
+// Scratch of a k_dp_tiled launch of up to `want` waves: the previous row in
+// LDS when it fits (windows up to ~2000 columns), else per wave in device
+// memory; staging in device memory when even that does not fit LDS (windows
+// of many thousand columns, long reads).  Waves are limited so that the
+// traceback bits stay within 4 GiB.  Returns the waves to launch.
+static uint32_t tiled_scratch(pbw_ctx* c, DpArgs& a, uint32_t want) {
+  a.tiles = (a.max_n + 1 + 63) / 64;
+  const uint64_t gs = dp_tiled_gstage_bytes(a.max_m, a.max_n);
+  const bool lds_rows = !gs && dp_tiled_rows_in_lds(a.max_m, a.max_n);
+  const uint64_t tb_words = (uint64_t)(a.max_m + 1) * a.tiles * 4;
+  const uint64_t row_words = (gs || lds_rows) ? 0 : 3ull * (a.max_n + 1);
+  const uint64_t per = tb_words * 8 + row_words * 4 + gs;
+  const uint32_t nw =
+      (uint32_t)std::min<uint64_t>(std::max<uint32_t>(want, 1), std::max<uint64_t>(1, (4ull << 30) / per));
+  dp_gstage(c, a, gs, nw);
+  if (row_words) {
+    c->d_rows.ensure((size_t)nw * row_words);
+    a.rows = c->d_rows.p;
+  } else {
+    a.rows = nullptr;
+  }
+  c->d_tb.ensure((size_t)nw * tb_words);
+  a.tb = c->d_tb.p;
+  return nw;
+}
+
 // Jobs of a device list through the tiled kernel.  It flags only jobs beyond the
 // batch's max_m / max_n, which the callers never pass; any flag is an error.
 static void run_dp_tiled_dev(pbw_ctx* c, DpArgs a, const uint32_t* dlist, uint32_t n) {
@@ -463,19 +513,8 @@ static void run_dp_tiled_dev(pbw_ctx* c, DpArgs a, const uint32_t* dlist, uint32
   a.flag_list = c->d_flag_c.p;
   a.flag_n = c->d_flag_n.p + 2;
   HIP_CHECK(hipMemsetAsync(a.flag_n, 0, 4, s));
-  a.tiles = (a.max_n + 1 + 63) / 64;
-  // previous row in LDS when it fits (windows up to ~2000 columns): no global
-  // round trip per row, and no per-wave row scratch, so more waves
-  const bool lds_rows = dp_tiled_rows_in_lds(a.max_m, a.max_n);
-  const uint32_t nwaves = std::min<uint32_t>(n, lds_rows ? 8192 : 2048);
-  if (lds_rows) {
-    a.rows = nullptr;
-  } else {
-    c->d_rows.ensure((size_t)nwaves * 3 * (a.max_n + 1));
-    a.rows = c->d_rows.p;
-  }
-  c->d_tb.ensure((size_t)nwaves * (a.max_m + 1) * a.tiles * 4);
-  a.tb = c->d_tb.p;
+  const bool lds_rows = !dp_tiled_gstage_bytes(a.max_m, a.max_n) && dp_tiled_rows_in_lds(a.max_m, a.max_n);
+  const uint32_t nwaves = tiled_scratch(c, a, std::min<uint32_t>(n, lds_rows ? 8192 : 2048));
   launch_dp_tiled(a, nwaves, s);
   HIP_CHECK(hipGetLastError());
   if (download_u32(c, a.flag_n)) throw PbwError{PBW_ERR_LIMIT, "alignment window beyond the tiled kernel's shape"};
@@ -485,7 +524,7 @@ static void run_dp_tiled(pbw_ctx* c, DpArgs a, const std::vector<uint32_t>& list
   hipStream_t s = c->stream;
   const uint32_t n = (uint32_t)list.size();
   c->d_list.upload(list.data(), n, s);
-  const uint32_t T = dp_wide_tiles(a.max_n);
+  const uint32_t T = dp_wide_tiles(a.max_m, a.max_n);
   if (!T) {
     run_dp_tiled_dev(c, a, c->d_list.p, n);
     return;
@@ -538,8 +577,7 @@ static void pair_pack(pbw_ctx* c, int slot, const pbw_read_batch* m1, const pbw_
     const pbw_read_batch* x = mm[i & 1];
     const uint64_t r = p0 + i / 2;
     const uint64_t sl = x->seq_off[r + 1] - x->seq_off[r], ql = x->qual_off[r + 1] - x->qual_off[r];
-    if (sl > (uint64_t)MAX_READ_LEN || ql > (uint64_t)MAX_READ_LEN)
-      throw PbwError{PBW_ERR_LIMIT, "read longer than " + std::to_string(MAX_READ_LEN) + " bases"};
+    if (sl > 0x7FFFFFFFull || ql > 0x7FFFFFFFull) throw PbwError{PBW_ERR_LIMIT, "read longer than 2^31 bases"};
     pk.so[i + 1] = pk.so[i] + sl;
     pk.qo[i + 1] = pk.qo[i] + ql;
     pk.max_m = std::max<uint32_t>(pk.max_m, (uint32_t)std::max(sl, ql));
@@ -601,8 +639,7 @@ static void batch_upload_slice(pbw_ctx* c, Batch& b, const pbw_read_batch* in, u
   }
   for (uint32_t i = 0; i < B; i++) {
     uint64_t L = b.so[i + 1] - b.so[i], ql = b.qo[i + 1] - b.qo[i];
-    if (L > (uint64_t)MAX_READ_LEN || ql > (uint64_t)MAX_READ_LEN)
-      throw PbwError{PBW_ERR_LIMIT, "read longer than " + std::to_string(MAX_READ_LEN) + " bases"};
+    if (L > 0x7FFFFFFFull || ql > 0x7FFFFFFFull) throw PbwError{PBW_ERR_LIMIT, "read longer than 2^31 bases"};
     b.max_m = std::max<uint32_t>(b.max_m, (uint32_t)std::max(L, ql));
   }
   c->d_seq_off.upload(b.so.data(), B + 1, s);
@@ -648,6 +685,7 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
   sa.nhits = c->d_nhits.p;
   sa.rcode = c->d_rcode.p;
   sa.ctr = c->d_ctr.p;
+  sa.has_long = b.max_m > (uint32_t)MAX_READ_LEN;
   if (feat) {
     c->d_hflag.ensure(base_cap + slack);
     sa.hflag = c->d_hflag.p;
@@ -888,7 +926,7 @@ static void batch_dp(pbw_ctx* c, Batch& b, const uint32_t* jlist, uint32_t nlist
       // a launch whose tile does not fit LDS goes to the one-job-per-wave kernel
       if (!launch_dp_quad(q, s)) bands.push_back(band_args((uint32_t)(q.jlist - c->d_sorted.p), q.njobs));
     }
-    for (const DpArgs& q : bands) launch_dp_band(q, s);
+    for (const DpArgs& q : bands) run_dp_band(c, q);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(c->ev[3], s));
     const uint32_t nflag = download_u32(c, c->d_flag_n.p);
@@ -905,7 +943,7 @@ static void batch_dp(pbw_ctx* c, Batch& b, const uint32_t* jlist, uint32_t nlist
       ba.jlist = c->d_flag_a.p;
       ba.njobs = nflag;
       ba.job_valid = nullptr;
-      launch_dp_band(ba, s);
+      run_dp_band(c, ba);
       HIP_CHECK(hipGetLastError());
     }
     const uint32_t ntiled = download_u32(c, ba.flag_n);
@@ -1093,6 +1131,27 @@ static void prefilter_dp_mapq(pbw_ctx* c, Batch& b, uint64_t read_base) {
 }
 
 // SingleEndMapper::map (single.rs:164-207) for every read of a batch.
+// Device words a batch reserves per read for its candidates' CIGAR slots:
+// 20 slots of max_m + max_n + 2 runs, max_n = max_m + right pad + LEFT_PAD.
+static uint64_t cigar_words_per_read(const pbw_ctx* c, uint64_t max_m) {
+  const uint64_t pad = std::max<uint32_t>(c->opt.max_indel_len, 10) + LEFT_PAD;
+  return (uint64_t)TOP_CANDIDATES * (2 * max_m + pad + 2);
+}
+constexpr uint64_t CIGAR_BUDGET_WORDS = (16ull << 30) / 4;  // per batch
+
+// End of the single-end batch that starts at read b0: at most bs reads before
+// `end`, cut where the batch's CIGAR slots would pass 16 GiB (only long reads
+// reach it; the batch boundaries never change a record).
+static uint64_t se_batch_end(const pbw_ctx* c, const pbw_read_batch* in, uint64_t b0, uint64_t end, uint64_t bs) {
+  const uint64_t e = std::min(end, b0 + bs);
+  uint64_t mx = 0;
+  for (uint64_t r = b0; r < e; r++) {
+    mx = std::max({mx, in->seq_off[r + 1] - in->seq_off[r], in->qual_off[r + 1] - in->qual_off[r]});
+    if (r > b0 && (r + 1 - b0) * cigar_words_per_read(c, mx) > CIGAR_BUDGET_WORDS) return r;
+  }
+  return e;
+}
+
 static void map_se_batch(pbw_ctx* c, const pbw_read_batch* in, uint64_t b0, uint64_t b1, HostRecords& hr,
                          bool resident) {
   hipStream_t s = c->stream;
@@ -1172,7 +1231,7 @@ static void run_dp_wide_dev(pbw_ctx* c, DpArgs a, const uint32_t* d_list, const 
   if (!cap) return;
   hipStream_t s = c->stream;
   a.job_valid = nullptr;
-  const uint32_t T = dp_wide_tiles(a.max_n);
+  const uint32_t T = dp_wide_tiles(a.max_m, a.max_n);
   const uint32_t* tl = d_list;
   const uint32_t* tn = d_n;
   uint32_t tw = std::min<uint32_t>(cap, 2048);
@@ -1199,16 +1258,7 @@ static void run_dp_wide_dev(pbw_ctx* c, DpArgs a, const uint32_t* d_list, const 
   c->d_flag_c.ensure(cap);
   a.flag_list = c->d_flag_c.p;
   a.flag_n = d_bad;
-  a.tiles = (a.max_n + 1 + 63) / 64;
-  const bool lds_rows = dp_tiled_rows_in_lds(a.max_m, a.max_n);
-  if (lds_rows) {
-    a.rows = nullptr;
-  } else {
-    c->d_rows.ensure((size_t)tw * 3 * (a.max_n + 1));
-    a.rows = c->d_rows.p;
-  }
-  c->d_tb.ensure((size_t)tw * (a.max_m + 1) * a.tiles * 4);
-  a.tb = c->d_tb.p;
+  tw = tiled_scratch(c, a, tw);
   launch_dp_tiled(a, tw, s);
   HIP_CHECK(hipGetLastError());
 }
@@ -1270,11 +1320,14 @@ static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, const
   pa.p_base = q0;
   launch_pair_plan(pa, s);
   HIP_CHECK(hipGetLastError());
-  // wide DP (align_at_window, single.rs:220-275): window <= 2 radius + L; a
-  // window beyond 8000 bases is counted as beyond the tiled shape
+  // wide DP (align_at_window, single.rs:220-275): window <= 2 radius + L.  A
+  // CIGAR has at most 2 m + 1 runs (every run but deletions takes a read base,
+  // and two deletion runs are never adjacent), so slots of 2 max_m + 2 words
+  // hold any window's alignment.
   const uint64_t wmax = 2 * (pa.radius_base + b.max_m) + b.max_m;
-  const uint32_t max_n = (uint32_t)std::min<uint64_t>(wmax, 8000);
-  const uint32_t wcap = b.max_m + max_n + 2;
+  if (wmax > 0x7FFFFFFFull) throw PbwError{PBW_ERR_LIMIT, "insert-size search window beyond 2^31 bases"};
+  const uint32_t max_n = (uint32_t)wmax;
+  const uint32_t wcap = 2 * b.max_m + 2;
   if (wcap > pool.stride) {
     if (pool.stride) c->d_wcigar.grow_keep(pool.nslots * (size_t)wcap, s);
     else c->d_wcigar.ensure(pool.nslots * (size_t)wcap);
@@ -1335,9 +1388,7 @@ static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, const
   uint8_t* h_has = c->h_pe + 16 + (size_t)P * 8;
   HIP_CHECK(hipMemcpyAsync(c->h_pe, c->d_pe_io.p, io_bytes, hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
-  if (h_cnt[2])
-    throw PbwError{PBW_ERR_LIMIT, wmax > 8000 ? "insert-size search window wider than 8000 bases"
-                                              : "alignment window beyond the tiled kernel's shape"};
+  if (h_cnt[2]) throw PbwError{PBW_ERR_LIMIT, "alignment window beyond the tiled kernel's shape"};
   c->stats.wide_jobs += h_cnt[0];
   obs_out.assign(h_obs, h_obs + P);
   has_out.assign(h_has, h_has + P);
@@ -1398,7 +1449,14 @@ static void map_pe_all(pbw_ctx* c, const pbw_read_batch* m1, const pbw_read_batc
   std::vector<uint8_t> has;
   const uint64_t warm = std::min<uint64_t>(n, 500);
   const uint64_t bs = c->opt.batch_reads ? c->opt.batch_reads : 131072;
-  const uint64_t per_prep = std::max<uint64_t>(1, bs / 2 / 2000) * 2000;
+  // pairs prepared together: whole 2000-pair batches, fewer when long mates
+  // would take the CIGAR slots past their budget
+  uint64_t mx = 0;
+  for (const pbw_read_batch* x : {m1, m2})
+    for (uint64_t r = 0; r < n; r++)
+      mx = std::max({mx, x->seq_off[r + 1] - x->seq_off[r], x->qual_off[r + 1] - x->qual_off[r]});
+  const uint64_t fit = CIGAR_BUDGET_WORDS / (2 * cigar_words_per_read(c, mx) * 2000);
+  const uint64_t per_prep = std::max<uint64_t>(1, std::min<uint64_t>(bs / 2 / 2000, fit)) * 2000;
   PairPack packs[2];
   PackAhead ahead;
   int cur = 0;
@@ -1821,7 +1879,10 @@ int pbw_map_se(pbw_ctx* c, const pbw_read_batch* reads, pbw_record_batch* out) {
     HostRecords hr;  // reserved while the first batch's seed kernel runs
     hr.expect = reads->n;
     uint64_t bs = c->opt.batch_reads ? c->opt.batch_reads : 131072;
-    for (uint64_t b = 0; b < reads->n; b += bs) map_se_batch(c, reads, b, std::min(reads->n, b + bs), hr, false);
+    for (uint64_t b = 0, e; b < reads->n; b = e) {
+      e = se_batch_end(c, reads, b, reads->n, bs);
+      map_se_batch(c, reads, b, e, hr, false);
+    }
     emit_batch(hr, out);
     c->stats.ms_total = ms_since(t0);
   });
@@ -1873,7 +1934,10 @@ int pbw_map_se_device(pbw_ctx* c, const pbw_device_reads* d, uint64_t begin, uin
     hr.expect = end - begin;
     pbw_read_batch view{d->n, d->seq.p, d->seq_off.data(), d->qual.p, d->qual_off.data()};
     uint64_t bs = c->opt.batch_reads ? c->opt.batch_reads : 131072;
-    for (uint64_t b = begin; b < end; b += bs) map_se_batch(c, &view, b, std::min(end, b + bs), hr, true);
+    for (uint64_t b = begin, e; b < end; b = e) {
+      e = se_batch_end(c, &view, b, end, bs);
+      map_se_batch(c, &view, b, e, hr, true);
+    }
     emit_batch(hr, out);
     c->stats.ms_total = ms_since(t0);
   });
@@ -1990,8 +2054,8 @@ int pbw_align_batch(pbw_ctx* c, uint32_t gop, uint32_t gep, const pbw_align_job*
     uint32_t max_m = 0, max_n = 0;
     for (uint64_t i = 0; i < n; i++) {
       const pbw_align_job& j = jobs[i];
-      if (j.m > (uint32_t)MAX_READ_LEN || j.qlen > (uint32_t)MAX_READ_LEN || j.n > 8000)
-        throw PbwError{PBW_ERR_LIMIT, "job too large"};
+      if (j.m > 0x7FFFFFFFu || j.qlen > 0x7FFFFFFFu || j.n > 0x7FFFFFFFu)
+        throw PbwError{PBW_ERR_LIMIT, "job beyond 2^31 bases"};
       so[i] = seq.size();
       qo[i] = qual.size();
       seq.insert(seq.end(), j.read, j.read + j.m);
@@ -2019,7 +2083,9 @@ int pbw_align_batch(pbw_ctx* c, uint32_t gop, uint32_t gep, const pbw_align_job*
     c->d_refbytes.upload(ref.data(), ref.size(), s);
     c->d_jobs.upload(dj.data(), n, s);
     DevReads R{c->d_seq.p, c->d_seq_off.p, c->d_qual.p, c->d_qual_off.p, (uint32_t)n};
-    const uint32_t cap = max_m + max_n + 2;
+    // at most 2 m + 1 runs per alignment (every run but deletions takes a read
+    // base, and two deletion runs are never adjacent)
+    const uint32_t cap = 2 * max_m + 2;
     c->d_out.ensure(n);
     c->d_cigar.ensure(n * cap);
     c->d_flag_a.ensure(n);
@@ -2056,7 +2122,7 @@ int pbw_align_batch(pbw_ctx* c, uint32_t gop, uint32_t gep, const pbw_align_job*
       db.njobs = (uint32_t)band_list.size();
       db.max_m = bm;
       db.max_n = bn;
-      launch_dp_band(db, s);
+      run_dp_band(c, db);
       HIP_CHECK(hipGetLastError());
       const uint32_t nf = download_u32(c, c->d_flag_n.p);
       std::vector<uint32_t> fl(nf);
@@ -2105,15 +2171,17 @@ int pbw_align_batch_noalloc(pbw_ctx* c, uint32_t gop, uint32_t gep, const pbw_al
     uint32_t max_m = 0, max_n = 0;
     for (uint64_t i = 0; i < n; i++) {
       const pbw_align_job& j = jobs[i];
+      // staging in LDS only (no device-memory staging here): reads up to
+      // MAX_READ_LEN bases, windows up to 8000
       if (j.m > (uint32_t)MAX_READ_LEN || j.qlen > (uint32_t)MAX_READ_LEN || j.n > 8000)
-        throw PbwError{PBW_ERR_LIMIT, "job too large"};
+        throw PbwError{PBW_ERR_LIMIT, "job beyond the no-allocation path's shape (reads of 1024, windows of 8000 bases)"};
       sm += j.m;
       sq += j.qlen;
       sr += j.n;
       max_m = std::max(max_m, std::max(j.m, j.qlen));
       max_n = std::max(max_n, j.n);
     }
-    const uint32_t cap = max_m + max_n + 2;
+    const uint32_t cap = 2 * max_m + 2;  // at most 2 m + 1 runs per alignment
     // host stage
     uint8_t* hseq = H.take<uint8_t>(sm);
     uint8_t* hqual = H.take<uint8_t>(sq);
@@ -2239,7 +2307,7 @@ int pbw_align_batch_noalloc(pbw_ctx* c, uint32_t gop, uint32_t gep, const pbw_al
     };
     if (nt) {
       up(dtiled, htiled, (size_t)nt * 4);
-      const uint32_t T = dp_wide_tiles(max_n);
+      const uint32_t T = dp_wide_tiles(max_m, max_n);
       if (!T) {
         tiled(dtiled, nt);
       } else {  // whole-window rectangles first; the jobs k_dp_wide flags go to the tiled kernel

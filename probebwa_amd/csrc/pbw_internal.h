@@ -24,7 +24,7 @@ constexpr int DIAG_BITS = 41;            // hit key: (group << 41) | (diag + 2^4
 constexpr int64_t DIAG_BIAS = 1ll << 40;
 
 // ---- limits of this implementation ----
-constexpr int MAX_READ_LEN = 1024;       // longer reads are rejected with PBW_ERR_LIMIT
+constexpr int MAX_READ_LEN = 1024;       // reads up to this are staged in LDS by the seed kernel; longer ones are read from device memory
 constexpr int HIT_CAP = 1024;            // least hits per read in the first seed pass (overflow -> exact rerun)
 constexpr int HIT_CAP_MAX = 16384;       // most: the first-pass capacity adapts between the two
 constexpr int CLUSTER_LDS_CAP = 512;     // hits per read clustered in LDS by one wave (else tier 2)
@@ -132,6 +132,7 @@ struct SeedArgs {
   uint8_t* rcode;          // per window repeat code, indexed by seq_off[read] + p
   Counters* ctr;
   uint8_t* hflag;          // optional, parallel to hits: bit 7 neighbour probe, bits 0..3 bucket-size class
+  uint32_t has_long;       // some read is longer than MAX_READ_LEN (seeded from device memory by a second kernel)
 };
 void launch_seed(const SeedArgs& a, hipStream_t s);
 
@@ -193,17 +194,29 @@ struct DpArgs {
   int32_t* rows;           // per wave: 3 * (max_n + 1)
   uint64_t* tb;            // per wave: (max_m + 1) * tiles * 4
   uint32_t tiles;          // ceil((max_n + 1) / 64)
+  // k_dp_band / k_dp_tiled staging in device memory instead of LDS (for tiles
+  // beyond LDS: long reads, windows of many thousand columns): wave w uses
+  // gstage + w * gstage_stride; the launch runs at most gstage_waves waves
+  uint8_t* gstage;
+  uint64_t gstage_stride;
+  uint32_t gstage_waves;
   uint32_t wpb;            // waves per block (banded kernel; set by the launcher)
   uint32_t* tbq;           // k_dp_quad traceback scratch: quad_scratch_words() words
   uint32_t quad_grid;      // k_dp_quad grid cap (0: QUAD_MAX_WAVES)
   uint32_t pack16;         // 1: eight jobs per wave in 16-bit halves (k_dp_pair16) where allowed
 };
 void launch_dp_band(const DpArgs& a, hipStream_t s);
+// Bytes per wave of k_dp_band's staging in device memory when its tile for
+// (max_m, max_n) does not fit LDS; 0 when it does (gstage unused).
+uint64_t dp_band_gstage_bytes(uint32_t max_m, uint32_t max_n);
 void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);  // a fill kernel (no memset node)
 void launch_dp_tiled(const DpArgs& a, uint32_t nwaves, hipStream_t s);
-// Windows whose band covers every row (paired-end mate windows) up to 1024
+// Bytes per wave of k_dp_tiled's staging and previous row in device memory
+// when its staging for (max_m, max_n) does not fit LDS; 0 when it does.
+uint64_t dp_tiled_gstage_bytes(uint32_t max_m, uint32_t max_n);
+// Windows whose band covers every row (paired-end mate windows) up to 4096
 // columns: k_dp_wide, which flags every other job to (flag_list, flag_n).
-uint32_t dp_wide_tiles(uint32_t max_n);  // 256-column tiles for max_n, 0: too wide
+uint32_t dp_wide_tiles(uint32_t max_m, uint32_t max_n);  // 256-column tiles per job, 0: too wide
 void launch_dp_wide(const DpArgs& a, uint32_t nwaves, hipStream_t s);
 // true: the tiled kernel keeps its previous row in LDS (pass rows = nullptr)
 bool dp_tiled_rows_in_lds(uint32_t max_m, uint32_t max_n);

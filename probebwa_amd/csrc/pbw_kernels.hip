@@ -188,7 +188,7 @@ struct SeedLds {
   // looked-up buckets holding at least one position, first in first out in
   // emission order; q_incl is the inclusive prefix of their sizes
   uint32_t q_incl[SEED_QCAP], q_off[SEED_QCAP], q_fp[SEED_QCAP];  // q_fp: fpF | fpR << 16
-  uint16_t q_pos[SEED_QCAP];                                        // window start in the read
+  uint32_t q_pos[SEED_QCAP];                                        // window start in the read
   uint8_t q_fl[SEED_QCAP];   // 1 neighbour probe, 4 fullF, 8 fullR
   uint8_t q_hfl[SEED_QCAP];  // prefilter flags of the hits from this bucket
 };
@@ -326,6 +326,10 @@ __device__ __forceinline__ void seed_positions(const SeedArgs& a, const SeedLds&
   c_filt += pass ? 1 : 0;
 }
 
+// LONG = false: reads of at most MAX_READ_LEN bases, staged in LDS; LONG =
+// true: the longer ones, read from device memory (the two launches split a
+// list between them).
+template <bool LONG>
 __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
   __shared__ SeedLds lds_all[SEED_WPB];
   const int wib = threadIdx.x >> 6, lane = lane_id();
@@ -343,12 +347,17 @@ __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
     const uint32_t cap = a.hcap ? a.hcap[li] : a.hit_cap;
     uint32_t hcount = 0;
     uint32_t qn = 0, qtot = 0, qc = 0;  // queued buckets, their positions, positions consumed
+    if (LONG != (L > (uint32_t)MAX_READ_LEN)) continue;  // the other launch's read
     if (L < (uint32_t)K) {
       if (lane == 0) a.nhits[r] = 0;
       continue;
     }
-    for (uint32_t x = lane; x < L; x += 64) S.seq[x] = a.R.seq[so + x];
-    wave_sync();
+    const uint8_t* rs = a.R.seq + so;
+    auto base_at = [&](uint32_t x) -> uint8_t { return LONG ? rs[x] : S.seq[x]; };
+    if (!LONG) {
+      for (uint32_t x = lane; x < L; x += 64) S.seq[x] = rs[x];
+      wave_sync();
+    }
     const uint32_t nwin = L - K + 1;
     for (uint32_t cb = 0; cb < nwin; cb += 64) {
       const uint32_t p = cb + lane;
@@ -360,7 +369,7 @@ __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
       if (act) {
         bool valid = true;
         for (int t = 0; t < K; t++) {
-          uint8_t b = S.seq[p + t];
+          uint8_t b = base_at(p + t);
           code = (code << 2) | base2bits(b);
           valid &= is_acgt(b);
         }
@@ -380,15 +389,15 @@ __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
         if (p >= 4 && p + 23 <= L) {
           fl |= 4;
           uint32_t f = 0;
-          for (uint32_t t = p + 15; t < p + 23; t++) f += fp_count(S.seq[t], false);
-          for (uint32_t t = p - 4; t < p; t++) f += fp_count(S.seq[t], false);
+          for (uint32_t t = p + 15; t < p + 23; t++) f += fp_count(base_at(t), false);
+          for (uint32_t t = p - 4; t < p; t++) f += fp_count(base_at(t), false);
           fpF = (uint16_t)f;
         }
         if (p >= 8 && p + 19 <= L) {  // rc(read): window start L-p-15
           fl |= 8;
           uint32_t f = 0;
-          for (uint32_t t = p - 8; t < p; t++) f += fp_count(S.seq[t], true);
-          for (uint32_t t = p + 15; t < p + 19; t++) f += fp_count(S.seq[t], true);
+          for (uint32_t t = p - 8; t < p; t++) f += fp_count(base_at(t), true);
+          for (uint32_t t = p + 15; t < p + 19; t++) f += fp_count(base_at(t), true);
           fpR = (uint16_t)f;
         }
       }
@@ -446,7 +455,7 @@ __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
           S.q_incl[qi] = qtot + pincl;
           S.q_off[qi] = loff;
           S.q_fp[qi] = (uint32_t)S.fpF[w] | ((uint32_t)S.fpR[w] << 16);
-          S.q_pos[qi] = (uint16_t)(cb + w);
+          S.q_pos[qi] = cb + w;
           S.q_fl[qi] = (uint8_t)((S.wflags[w] & 12u) | (j != 0 ? 1u : 0u));
           // prefilter flags: neighbour probe, bit length of the bucket size
           S.q_hfl[qi] = (uint8_t)((j != 0 ? 0x80u : 0u) | (uint32_t)(32 - __builtin_clz(lcnt)));
@@ -467,7 +476,7 @@ __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
           const uint32_t base = l0 ? S.q_incl[l0 - 1] : 0u;
           const bool mv = l0 + lane < qn;
           uint32_t vi = 0, vo = 0, vf = 0;
-          uint16_t vp = 0;
+          uint32_t vp = 0;
           uint8_t vfl = 0, vh = 0;
           if (mv) {
             const uint32_t sidx = l0 + lane;
@@ -523,7 +532,8 @@ void launch_seed(const SeedArgs& a, hipStream_t s) {
   if (a.nlist == 0) return;
   uint32_t blocks = (a.nlist + SEED_WPB - 1) / SEED_WPB;
   if (blocks > 16384) blocks = 16384;
-  hipLaunchKernelGGL(k_seed, dim3(blocks), dim3(64 * SEED_WPB), 0, s, a);
+  hipLaunchKernelGGL(k_seed<false>, dim3(blocks), dim3(64 * SEED_WPB), 0, s, a);
+  if (a.has_long) hipLaunchKernelGGL(k_seed<true>, dim3(blocks), dim3(64 * SEED_WPB), 0, s, a);
 }
 
 // ---------------------------------------------------------------------------
@@ -2176,7 +2186,7 @@ __device__ void dp_stage_job(const DpArgs& a, const DpJob& job, uint32_t m, uint
   const uint32_t n = job.n;
   for (uint32_t x = lane; x < n; x += 64)
     st.ref[x] = job.explicit_ref ? a.ref_bytes[job.ref_gb + x] : genome_base(a.G, job.ref_gb + x);
-  wave_sync();
+  mem_sync();
   // homopolymer run of every position: start = prefix max of run starts,
   // end = suffix min of run ends (64-column chunks with carries)
   int32_t carry = 0;
@@ -2189,7 +2199,7 @@ __device__ void dp_stage_job(const DpArgs& a, const DpJob& job, uint32_t m, uint
     carry = __shfl(s, 63, 64);
     if (x < n) ((int32_t*)st.runs)[x] = s;  // run start (runs[] is free scratch here)
   }
-  wave_sync();
+  mem_sync();
   int32_t ecarry = INT32_MAX;
   uint32_t nchunks = (n + 63) / 64;
   for (int c = (int)nchunks - 1; c >= 0; c--) {
@@ -2205,7 +2215,7 @@ __device__ void dp_stage_job(const DpArgs& a, const DpJob& job, uint32_t m, uint
       st.go[x] = (int16_t)max(min(g, -1), -32768);
     }
   }
-  wave_sync();
+  mem_sync();
 }
 
 // Lane 0: traceback (smith_waterman.rs:299-345) from (m, best_j, state) with
@@ -2323,7 +2333,10 @@ __host__ __device__ inline uint32_t dp_band_lds_bytes(uint32_t max_m, uint32_t m
 __global__ __launch_bounds__(64 * DP_WPB) void k_dp_band(DpArgs a) {
   extern __shared__ __align__(16) uint8_t dyn[];
   const int wib = threadIdx.x >> 6, lane = lane_id();
-  uint8_t* base = dyn + (size_t)wib * dp_band_lds_bytes(a.max_m, a.max_n);
+  // the wave's staging: its LDS slice, or its slice of device memory when the
+  // tile does not fit LDS
+  uint8_t* base = a.gstage ? a.gstage + (uint64_t)(blockIdx.x * a.wpb + wib) * a.gstage_stride
+                           : dyn + (size_t)wib * dp_band_lds_bytes(a.max_m, a.max_n);
   DpStage st;
   st.q = base;
   st.qq = st.q + ((a.max_m + 15) & ~15u);
@@ -2368,7 +2381,7 @@ __global__ __launch_bounds__(64 * DP_WPB) void k_dp_band(DpArgs a) {
       Ip = NEG_INF;
       Dp = NEG_INF;
     }
-    wave_sync();
+    mem_sync();  // the cleared rows before lane 0 fills them
     uint64_t cells = 0;
     const int32_t ge = a.gap_extend;
     for (int32_t i = i0; i <= i1; i++) {
@@ -2464,7 +2477,7 @@ __global__ __launch_bounds__(64 * DP_WPB) void k_dp_band(DpArgs a) {
     nruns = shfl_u32(nruns, 0);
     off = shfl_u32(off, 0);
     span = shfl_u32(span, 0);
-    wave_sync();
+    mem_sync();  // lane 0's runs to every lane (LDS or device-memory staging)
     dp_emit(a, jid, st, nruns, off, span, best_score, lk, cells);
     c_cells += cells;
     wave_sync();
@@ -2472,16 +2485,29 @@ __global__ __launch_bounds__(64 * DP_WPB) void k_dp_band(DpArgs a) {
   if (lane == 0 && a.ctr) atomicAdd(&a.ctr->cells, c_cells);
 }
 
+uint64_t dp_band_gstage_bytes(uint32_t max_m, uint32_t max_n) {
+  const uint64_t per_wave = dp_band_lds_bytes(max_m, max_n);
+  return per_wave <= 65536 ? 0 : (per_wave + 255) & ~255ull;
+}
 void launch_dp_band(const DpArgs& a0, hipStream_t s) {
   if (a0.njobs == 0) return;
   DpArgs a = a0;
-  uint32_t per_wave = dp_band_lds_bytes(a.max_m, a.max_n);
-  a.wpb = 65536 / per_wave;
-  if (a.wpb > DP_WPB) a.wpb = DP_WPB;
-  if (a.wpb < 1) a.wpb = 1;
-  uint32_t blocks = (a.njobs + a.wpb - 1) / a.wpb;
+  uint32_t blocks;
+  size_t lds = 0;
+  if (a.gstage) {  // staging in device memory: gstage_waves slices of gstage_stride bytes
+    a.wpb = a.gstage_waves < (uint32_t)DP_WPB ? (a.gstage_waves ? a.gstage_waves : 1u) : (uint32_t)DP_WPB;
+    blocks = (a.njobs + a.wpb - 1) / a.wpb;
+    const uint32_t cap = a.gstage_waves / a.wpb;
+    if (blocks > cap) blocks = cap ? cap : 1u;
+  } else {
+    const uint32_t per_wave = dp_band_lds_bytes(a.max_m, a.max_n);
+    a.wpb = 65536 / per_wave;
+    if (a.wpb > DP_WPB) a.wpb = DP_WPB;
+    if (a.wpb < 1) a.wpb = 1;
+    blocks = (a.njobs + a.wpb - 1) / a.wpb;
+    lds = (size_t)a.wpb * per_wave;
+  }
   if (blocks > 65536) blocks = 65536;
-  size_t lds = (size_t)a.wpb * per_wave;
   hipLaunchKernelGGL(k_dp_band, dim3(blocks), dim3(64 * a.wpb), lds, s, a);
 }
 
@@ -3828,16 +3854,18 @@ __host__ __device__ inline uint32_t dp_tiled_lds_bytes(uint32_t max_m, uint32_t 
 __global__ __launch_bounds__(64) void k_dp_tiled(DpArgs a) {
   extern __shared__ __align__(16) uint8_t dyn[];
   const int lane = lane_id();
+  // staging in LDS, or in this wave's slice of device memory when it does not fit
+  uint8_t* sbase = a.gstage ? a.gstage + (uint64_t)blockIdx.x * a.gstage_stride : dyn;
   DpStage st;
-  st.q = dyn;
+  st.q = sbase;
   st.qq = st.q + ((a.max_m + 15) & ~15u);
   st.ref = st.qq + ((a.max_m + 15) & ~15u);
   st.go = (int16_t*)(st.ref + ((a.max_n + 15) & ~15u));
   st.runs = (uint32_t*)((uint8_t*)st.go + (((a.max_n * 2) + 15) & ~15u));
-  // the previous row's M, I, D: in LDS after the staging arrays when the launch
-  // gave it room (a.rows null), else in this wave's global scratch
+  // the previous row's M, I, D: after the staging arrays when the launch gave
+  // it room there (a.rows null), else in this wave's global scratch
   int32_t* RM = a.rows ? a.rows + (uint64_t)blockIdx.x * 3 * (a.max_n + 1)
-                       : (int32_t*)(dyn + dp_tiled_lds_bytes(a.max_m, a.max_n));
+                       : (int32_t*)(sbase + dp_tiled_lds_bytes(a.max_m, a.max_n));
   int32_t* RI = RM + (a.max_n + 1);
   int32_t* RD = RI + (a.max_n + 1);
   uint64_t* TB = a.tb + (uint64_t)blockIdx.x * (uint64_t)(a.max_m + 1) * a.tiles * 4;
@@ -4003,7 +4031,7 @@ __global__ __launch_bounds__(64) void k_dp_tiled(DpArgs a) {
     nruns = shfl_u32(nruns, 0);
     off = shfl_u32(off, 0);
     span = shfl_u32(span, 0);
-    wave_sync();
+    mem_sync();  // lane 0's runs to every lane (LDS or device-memory staging)
     dp_emit(a, jid, st, nruns, off, span, best_score, lk, cells);
     c_cells += cells;
     wave_sync();
@@ -4014,10 +4042,19 @@ __global__ __launch_bounds__(64) void k_dp_tiled(DpArgs a) {
 bool dp_tiled_rows_in_lds(uint32_t max_m, uint32_t max_n) {
   return dp_tiled_lds_bytes(max_m, max_n) + 12ull * (max_n + 1) <= 48 * 1024;
 }
+uint64_t dp_tiled_gstage_bytes(uint32_t max_m, uint32_t max_n) {
+  const uint64_t st = dp_tiled_lds_bytes(max_m, max_n);
+  return st <= 64 * 1024 ? 0 : (st + 12ull * (max_n + 1) + 255) & ~255ull;
+}
 void launch_dp_tiled(const DpArgs& a, uint32_t nwaves, hipStream_t s) {
   if (a.njobs == 0) return;
   uint32_t blocks = nwaves < a.njobs ? nwaves : a.njobs;
-  size_t lds = dp_tiled_lds_bytes(a.max_m, a.max_n) + (a.rows ? 0 : 12ull * (a.max_n + 1));
+  size_t lds = 0;
+  if (a.gstage) {  // staging and the previous row in device memory
+    if (blocks > a.gstage_waves) blocks = a.gstage_waves;
+  } else {
+    lds = dp_tiled_lds_bytes(a.max_m, a.max_n) + (a.rows ? 0 : 12ull * (a.max_n + 1));
+  }
   hipLaunchKernelGGL(k_dp_tiled, dim3(blocks), dim3(64), lds, s, a);
 }
 
@@ -4043,9 +4080,10 @@ void launch_dp_tiled(const DpArgs& a, uint32_t nwaves, hipStream_t s) {
 // Jobs of another shape, or wider than 256 T columns, are flagged to
 // k_dp_tiled.
 // ---------------------------------------------------------------------------
-constexpr uint32_t WIDE_TB_ROWS = 16;  // traceback rows staged in LDS at a time
+// traceback rows staged in LDS at a time: 16, or 8 for blocks of more than four tiles
+__host__ __device__ constexpr uint32_t wide_tb_rows(uint32_t T) { return T <= 4 ? 16u : 8u; }
 __host__ __device__ inline uint32_t dp_wide_lds_bytes(uint32_t max_m, uint32_t max_n, uint32_t T) {
-  return dp_tiled_lds_bytes(max_m, max_n) + WIDE_TB_ROWS * T * 128 + 56 * T;
+  return dp_tiled_lds_bytes(max_m, max_n) + wide_tb_rows(T) * T * 128 + 56 * T;
 }
 template <int T>
 __global__ __launch_bounds__(64 * T) __attribute__((amdgpu_waves_per_eu(4))) void k_dp_wide(DpArgs a) {
@@ -4059,6 +4097,7 @@ __global__ __launch_bounds__(64 * T) __attribute__((amdgpu_waves_per_eu(4))) voi
   st.go = (int16_t*)(st.ref + ((a.max_n + 15) & ~15u));
   st.runs = (uint32_t*)((uint8_t*)st.go + (((a.max_n * 2) + 15) & ~15u));
   uint16_t* TB = (uint16_t*)a.tb + (uint64_t)blockIdx.x * (uint64_t)(a.max_m + 1) * T * 64;
+  constexpr uint32_t WIDE_TB_ROWS = wide_tb_rows(T);
   uint32_t* tbs = (uint32_t*)(dyn + dp_tiled_lds_bytes(a.max_m, a.max_n));  // traceback staging
   int64_t* ekey = (int64_t*)(tbs + WIDE_TB_ROWS * T * 32);                  // per tile: endpoint key
   int32_t* carry = (int32_t*)(ekey + T);                                    // [tile][row % 4][M, I, D]
@@ -4260,21 +4299,30 @@ __global__ __launch_bounds__(64 * T) __attribute__((amdgpu_waves_per_eu(4))) voi
   if (threadIdx.x == 0 && a.ctr) atomicAdd(&a.ctr->cells, c_cells);
 }
 
-// Tiles of k_dp_wide for windows of up to max_n columns (0: too wide for it).
-uint32_t dp_wide_tiles(uint32_t max_n) {
-  const uint32_t t = (max_n + 255) / 256;
-  return t <= 4 ? (t ? t : 1) : 0;
+// Tiles of k_dp_wide per job for windows of up to max_n columns: the least
+// instantiated count that covers them (0: too wide for it, or its staging for
+// (max_m, max_n) beyond 64 KiB of LDS).
+uint32_t dp_wide_tiles(uint32_t max_m, uint32_t max_n) {
+  const uint32_t need = (max_n + 255) / 256;
+  static const uint32_t inst[] = {1, 2, 3, 4, 6, 8, 12, 16};
+  for (uint32_t T : inst)
+    if (T >= need) return dp_wide_lds_bytes(max_m, max_n, T) <= 64 * 1024 ? T : 0u;
+  return 0;
 }
 void launch_dp_wide(const DpArgs& a, uint32_t nwaves, hipStream_t s) {
   if (a.njobs == 0) return;
   const uint32_t blocks = nwaves < a.njobs ? nwaves : a.njobs;
-  const uint32_t T = dp_wide_tiles(a.max_n);
+  const uint32_t T = dp_wide_tiles(a.max_m, a.max_n);
   const size_t lds = dp_wide_lds_bytes(a.max_m, a.max_n, T);
   switch (T) {
     case 1: hipLaunchKernelGGL(k_dp_wide<1>, dim3(blocks), dim3(64), lds, s, a); break;
     case 2: hipLaunchKernelGGL(k_dp_wide<2>, dim3(blocks), dim3(128), lds, s, a); break;
     case 3: hipLaunchKernelGGL(k_dp_wide<3>, dim3(blocks), dim3(192), lds, s, a); break;
     case 4: hipLaunchKernelGGL(k_dp_wide<4>, dim3(blocks), dim3(256), lds, s, a); break;
+    case 6: hipLaunchKernelGGL(k_dp_wide<6>, dim3(blocks), dim3(384), lds, s, a); break;
+    case 8: hipLaunchKernelGGL(k_dp_wide<8>, dim3(blocks), dim3(512), lds, s, a); break;
+    case 12: hipLaunchKernelGGL(k_dp_wide<12>, dim3(blocks), dim3(768), lds, s, a); break;
+    case 16: hipLaunchKernelGGL(k_dp_wide<16>, dim3(blocks), dim3(1024), lds, s, a); break;
     default: break;
   }
 }
@@ -5029,7 +5077,9 @@ __device__ __forceinline__ uint32_t ov_block_scan(uint32_t x, uint32_t* sh, uint
   total = tot;
   return before + incl - x;
 }
-// pass 1: per block [count, max, over0..3, hits]
+// pass 1: per block [count, max, over0..3, hits lo, hits hi] (the hits of a
+// block's overflow reads in 64 bits: the reads' low and high 16-bit halves
+// summed apart, each sum < 2^26)
 __global__ __launch_bounds__(OV_BLOCK) void k_ov_count(const uint32_t* nhits, uint32_t B, uint32_t cap1, uint32_t* part) {
   __shared__ uint32_t sh[OV_BLOCK / 64];
   __shared__ uint32_t smax[OV_BLOCK / 64];
@@ -5040,24 +5090,25 @@ __global__ __launch_bounds__(OV_BLOCK) void k_ov_count(const uint32_t* nhits, ui
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, o, 64));
   if (lane_id() == 0) smax[threadIdx.x >> 6] = mx;
-  uint32_t c[6] = {ov ? 1u : 0u, ov ? nh : 0u, 0, 0, 0, 0};
+  uint32_t c[7] = {ov ? 1u : 0u, ov ? (nh & 0xFFFFu) : 0u, ov ? (nh >> 16) : 0u, 0, 0, 0, 0};
 #pragma unroll
-  for (int j = 0; j < 4; j++) c[2 + j] = (ov && nh > ((uint32_t)HIT_CAP << j)) ? 1u : 0u;
-  uint32_t tot[6];
+  for (int j = 0; j < 4; j++) c[3 + j] = (ov && nh > ((uint32_t)HIT_CAP << j)) ? 1u : 0u;
+  uint32_t tot[7];
 #pragma unroll
-  for (int j = 0; j < 6; j++) (void)ov_block_scan(c[j], sh, tot[j]);
+  for (int j = 0; j < 7; j++) (void)ov_block_scan(c[j], sh, tot[j]);
   if (threadIdx.x == 0) {
     uint32_t m = 0;
     for (uint32_t v = 0; v < OV_BLOCK / 64; v++) m = max(m, smax[v]);
     uint32_t* p = part + (uint64_t)blockIdx.x * 8;
+    const uint64_t hits = (uint64_t)tot[1] + ((uint64_t)tot[2] << 16);
     p[0] = tot[0];
     p[1] = m;
-    p[2] = tot[2];
-    p[3] = tot[3];
-    p[4] = tot[4];
-    p[5] = tot[5];
-    p[6] = tot[1];
-    p[7] = 0;
+    p[2] = tot[3];
+    p[3] = tot[4];
+    p[4] = tot[5];
+    p[5] = tot[6];
+    p[6] = (uint32_t)hits;
+    p[7] = (uint32_t)(hits >> 32);
   }
 }
 // pass 2 (one block): block bases (count, hits) in place of the counts, and st
@@ -5075,12 +5126,16 @@ __global__ __launch_bounds__(OV_BLOCK) void k_ov_scan(uint32_t* part, uint32_t n
     const uint32_t bi = b0 + threadIdx.x;
     uint32_t* p = part + (uint64_t)bi * 8;
     const bool v = bi < nb;
-    const uint32_t cnt = v ? p[0] : 0u, hits = v ? p[6] : 0u;
-    // the hits in 64 bits: their low and high halves scanned apart (each sum < 2^26)
-    uint32_t tc, tl, tu;
+    const uint32_t cnt = v ? p[0] : 0u;
+    const uint64_t hits = v ? ((uint64_t)p[6] | ((uint64_t)p[7] << 32)) : 0ull;  // < 2^42
+    // the hits in 64 bits: three 16-bit parts scanned apart (each sum < 2^26)
+    uint32_t tc, tl, tu, tv;
     const uint32_t ec = ov_block_scan(cnt, sh, tc);
-    const uint32_t el = ov_block_scan(hits & 0xFFFFu, sh, tl), eu = ov_block_scan(hits >> 16, sh, tu);
-    const uint64_t eh = (uint64_t)el + ((uint64_t)eu << 16), th = (uint64_t)tl + ((uint64_t)tu << 16);
+    const uint32_t el = ov_block_scan((uint32_t)hits & 0xFFFFu, sh, tl);
+    const uint32_t eu = ov_block_scan((uint32_t)(hits >> 16) & 0xFFFFu, sh, tu);
+    const uint32_t ev = ov_block_scan((uint32_t)(hits >> 32), sh, tv);
+    const uint64_t eh = (uint64_t)el + ((uint64_t)eu << 16) + ((uint64_t)ev << 32);
+    const uint64_t th = (uint64_t)tl + ((uint64_t)tu << 16) + ((uint64_t)tv << 32);
     const uint32_t m = v ? p[1] : 0u;
     uint32_t o[4];
     for (int j = 0; j < 4; j++) o[j] = v ? p[2 + j] : 0u;
@@ -5121,9 +5176,12 @@ __global__ __launch_bounds__(OV_BLOCK) void k_ov_write(const uint32_t* nhits, ui
   const uint32_t i = blockIdx.x * OV_BLOCK + threadIdx.x;
   const uint32_t nh = i < B ? nhits[i] : 0u;
   const bool ov = nh > cap1;
-  uint32_t t0, t1;
+  uint32_t t0, t1, t2;
   const uint32_t ec = ov_block_scan(ov ? 1u : 0u, sh, t0);
-  const uint32_t eh = ov_block_scan(ov ? nh : 0u, sh, t1);
+  // the hits before this read in 64 bits: low and high 16-bit halves apart
+  const uint32_t el = ov_block_scan(ov ? (nh & 0xFFFFu) : 0u, sh, t1);
+  const uint32_t eu = ov_block_scan(ov ? (nh >> 16) : 0u, sh, t2);
+  const uint64_t eh = (uint64_t)el + ((uint64_t)eu << 16);
   if (!ov) return;
   const uint32_t* p = part + (uint64_t)blockIdx.x * 8;
   const uint32_t k = p[0] + ec;

@@ -422,20 +422,22 @@ def reads_pairs_genome(contigs, n=1_000_000, L=150, seed=58, chunk=65536):
     return (out[0], so, out[1], so.copy()), (out[2], so.copy(), out[3], so.copy())
 
 
-def reads_pairs(contigs, n=10_000, L=150, seed=13, repeat_spans=()):
-    """FR pairs: fragment ~ N(350, 50) clipped to [200, 600]; 10% anchored in a repeat span."""
+def reads_pairs(contigs, n=10_000, L=150, seed=13, repeat_spans=(), frag=(350, 50, 200, 600)):
+    """FR pairs: fragment ~ N(mean, sd) clipped to [lo, hi] (default N(350, 50) in
+    [200, 600]); 10% anchored in a repeat span."""
     rng = np.random.default_rng(np.random.PCG64(seed))
     g = contigs[0][1]
+    fmean, fsd, flo, fhi = frag
     s1, q1, s2, q2 = [], [], [], []
     for i in range(n):
-        frag = int(np.clip(np.rint(rng.normal(350, 50)), 200, 600))
+        flen = int(np.clip(np.rint(rng.normal(fmean, fsd)), flo, fhi))
         if repeat_spans and rng.random() < 0.1:
             a, b = repeat_spans[int(rng.integers(0, len(repeat_spans)))]
-            start = int(np.clip(a + rng.integers(0, max(1, b - a - L)) - frag + L, 0, len(g) - frag - 1))
+            start = int(np.clip(a + rng.integers(0, max(1, b - a - L)) - flen + L, 0, len(g) - flen - 1))
         else:
-            start = int(rng.integers(0, len(g) - frag - 1))
+            start = int(rng.integers(0, len(g) - flen - 1))
         m1 = mutate(rng, g[start:start + L], 0.01)
-        m2 = revcomp(mutate(rng, g[start + frag - L:start + frag], 0.01))
+        m2 = revcomp(mutate(rng, g[start + flen - L:start + flen], 0.01))
         if rng.random() < 0.5:
             m1, m2 = m2, m1
         s1.append(m1)

@@ -451,3 +451,71 @@ def test_map_pe_many_batches_matches_oracle():
         rp = ctx.map_pe(p1, p2)
         compare_records(ro, rp, label="4700 pairs, tune %s" % shape)
         assert tuple(m.insert_model) == tuple(ctx.insert_model)
+
+
+# ---------------------------------------------------------------------------
+# shapes beyond LDS: long reads, mate windows of thousands of columns
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("frag", [(1200, 200, 400, 2500), (2000, 350, 800, 4000), (3000, 1000, 500, 8000)])
+def test_map_pe_wide_insert_windows_match_oracle(frag):
+    """Mate windows of 2 (ceil(5 sd) + L) + L columns (align_mate_near,
+    paired.rs:450-468): about 2,450 and 3,950 columns (k_dp_wide with 12 and 16
+    tiles per job) and about 10,450 (the tiled kernel, staged in device memory).
+    The model starts at the library's own (mean, sd), so the windows stay that wide."""
+    contigs = synth.small_genome(71, n_contigs=2, length=150000, repeat_copies=8)
+    go, gp = pair_of_genomes(contigs)
+    a, b = synth.reads_pairs(contigs, n=700, L=150, seed=72, frag=frag)
+    o1, p1 = both_reads(*a)
+    o2, p2 = both_reads(*b)
+    m = O.Mapper(go, O.HashTable(go), threads=16)
+    ro = m.map_pe(o1, o2, mean=float(frag[0]), sd=float(frag[1]))
+    ctx = P.Context(gp)
+    rp = ctx.map_pe(p1, p2, mean=float(frag[0]), sd=float(frag[1]))
+    compare_records(ro, rp, label="pairs, fragments N(%d, %d)" % frag[:2])
+    assert tuple(m.insert_model) == tuple(ctx.insert_model)
+    assert ctx.stats()["wide_jobs"] > 0
+    assert ((rp.rec["flag"] & 2) != 0).mean() > 0.8
+
+
+def test_map_se_long_reads_match_oracle():
+    """Reads of 30-6000 bases in one call: the longer than 1,024 are seeded from
+    device memory and their banded DP is staged in device memory (the reference
+    has no read-length limit)."""
+    contigs = synth.small_genome(81, n_contigs=2, length=120000)
+    go, gp = pair_of_genomes(contigs)
+    seq, so, qual, qo = synth.sample_reads(contigs, 300, 82, lmin=30, lmax=6000)
+    assert (np.diff(so) > 1024).sum() > 200
+    ro_b, rp_b = both_reads(seq, so, qual, qo)
+    ro = O.Mapper(go, O.HashTable(go), threads=16).map_se(ro_b)
+    for shape in (None, dict(batch_reads=64)):
+        ctx = P.Context(gp)
+        if shape:
+            ctx.tune = shape
+        rp = ctx.map_se(rp_b)
+        compare_records(ro, rp, rp_b, go, gp, "long reads, tune %s" % shape)
+    assert (rp.rec["pos"] > 0).mean() > 0.8
+
+
+def test_align_batch_long_reads_and_wide_windows_match_oracle():
+    """`align` on reads over 1,024 bases and windows over 8,000 columns, banded and
+    whole-window, through the banded, wide and tiled kernels (staging in device
+    memory where LDS is too small)."""
+    rng = np.random.default_rng(91)
+    jobs = []
+    for m, nref, band in [(1500, 1560, 30), (3000, 3100, 20), (2000, 9000, None), (300, 12000, None),
+                          (1200, 5000, 100), (150, 3900, None), (700, 700, 31), (40, 20000, None)]:
+        ref = bytearray(synth.random_bases(rng, nref))
+        p = int(rng.integers(0, max(1, nref - m)))
+        read = synth.mutate(rng, np.frombuffer(bytes(ref[p:p + m]), np.uint8), 0.03).tobytes()
+        if len(read) < m:
+            read = read + bytes(synth.random_bases(rng, m - len(read)))
+        qual = bytes(rng.integers(2, 42, m).astype(np.uint8))
+        b = nref if band is None else band
+        hint = p + int(rng.integers(-5, 6)) if band is not None else p
+        jobs.append((read, qual, bytes(ref), b, hint))
+    g = P.Genome().add_contig("x", "ACGT" * 10)
+    ctx = P.Context(g)
+    got = ctx.align_batch(jobs)
+    for i, (read, qual, ref, band, hint) in enumerate(jobs):
+        s, c, o, _ = O.SmithWatermanAligner(band, 40, 3).align(read, qual, ref, hint)
+        assert got[i] == (s, c, o), "job %d (m=%d n=%d band=%d hint=%d)" % (i, len(read), len(ref), band, hint)
