@@ -1124,7 +1124,8 @@ __device__ __forceinline__ void run_radix_sort(const ClusterMem& M, uint32_t a0,
 // index.  Returns false (nothing written) when the read has more than
 // CLUSTER_GT_MAX groups.
 template <int T, bool VALS>
-__device__ __forceinline__ bool group_segment_sort(const uint64_t* hits, uint32_t n, const ClusterMem& M, uint32_t tid) {
+__device__ __forceinline__ bool group_segment_sort(const uint64_t* hits, uint32_t n, const ClusterMem& M, uint32_t tid,
+                                                   bool& by_rank) {
   static_assert(T == RSORT_T, "the block tier runs 256 threads");
   constexpr uint32_t U = 8;  // hits loaded ahead per thread: the loads of a pass overlap
   for (uint32_t i = tid; i < CLUSTER_GT; i += T) {
@@ -1194,18 +1195,39 @@ __device__ __forceinline__ bool group_segment_sort(const uint64_t* hits, uint32_
   team_sync<T>();
   const uint32_t G = M.gt_misc[0];
   if (G > CLUSTER_GT_MAX) return false;
-  // groups ascending: (group << 32 | slot) keys sorted in LDS
+  // Segment order.  Without per-hit values and for at most 56 groups: the
+  // FxHashMap iteration order (rank) of the groups, found here from their first
+  // emission indices, so the sorted keys run in the order the candidates are
+  // enumerated (candidates.rs:45-72).  Otherwise ascending group keys.
+  by_rank = !VALS && G <= 56;
   uint32_t g2 = 1;
   while (g2 < G) g2 <<= 1;
   for (uint32_t i = tid; i < CLUSTER_GT; i += T)
     if (M.gt_key[i] != 0xFFFFFFFFu) {
       const uint32_t at = atomicAdd(&M.gt_misc[1], 1u);
-      M.stage_key[at] = ((uint64_t)M.gt_key[i] << 32) | i;
+      M.stage_key[at] = ((uint64_t)(by_rank ? M.gt_emin[i] : M.gt_key[i]) << 32) | i;
     }
   team_sync<T>();
   for (uint32_t i = G + tid; i < g2; i += T) M.stage_key[i] = ~0ull;
   team_sync<T>();
   team_bitonic<T>(M.stage_key, nullptr, g2, tid);
+  if (by_rank) {
+    // stage_key[t]: the group inserted t-th (first appearance order); its key in
+    // stage_val[t], t itself as the insertion list, its rank to stage_val[128 + t]
+    static_assert(T >= 64, "one entry per thread for G <= 56");
+    if (tid < G) {
+      M.stage_val[tid] = M.gt_key[(uint32_t)M.stage_key[tid]];
+      M.stage_val[64 + tid] = tid;
+    }
+    team_sync<T>();
+    if (tid < 64) group_ranks_wave(G, M.stage_val, M.stage_val + 64, M.stage_val + 128);
+    team_sync<T>();
+    const uint32_t sl = tid < G ? (uint32_t)M.stage_key[tid] : 0u;
+    const uint32_t rk = tid < G ? M.stage_val[128 + tid] : 0u;
+    team_sync<T>();
+    if (tid < G) M.stage_key[rk] = ((uint64_t)M.gt_key[sl] << 32) | sl;  // (group << 32 | slot) by rank
+    team_sync<T>();
+  }
   // segment starts in group order: the scatter cursor of every slot; the start of
   // group rank r stays in stage_val[r] (stage_val[G] = n)
   uint32_t tot;
@@ -1642,6 +1664,168 @@ __device__ __forceinline__ bool top_by_count(const ClusterMem& M, uint32_t C, ui
   return true;
 }
 
+// Block tier with the segments in group iteration order (group_segment_sort's
+// by_rank): the sorted keys M.key[0, n) run in the order find_candidates
+// enumerates its clusters (groups in FxHashMap order, diagonals ascending within
+// a group, candidates.rs:45-72), so the stable sort by score (:76) keeps, among
+// equal counts, the order of the clusters' first keys.  Two walks over the keys
+// and no per-cluster arrays:
+//  1. every cluster's count: counts 1..CNT_REG tallied in registers per thread,
+//     larger ones in an LDS histogram whose last bin holds every count of
+//     CNT_BINS - 1 or more; from them T, the count of the nt-th cluster in score
+//     order, and gt, the number of clusters counted above T;
+//  2. the clusters counted above T, and of those counted exactly T the first
+//     nt - gt in key order (a thread's ties offset by the ties of the threads
+//     before it), each with its diagonal sum; the nt chosen are then ordered by
+//     (count desc, first key) and written as candidates.
+// A thread owns the clusters whose first key lies in its chunk and follows each
+// to its end, past the chunk when it runs on.  Returns false (uniformly, nothing
+// written) when T falls in the saturated bin; the caller's general path then
+// handles the read.
+constexpr uint32_t CNT_BINS = 1024, CNT_REG = 8;
+template <int T>
+__device__ __forceinline__ bool top20_ordered(const ClusterMem& M, uint32_t n, uint32_t L, Cand* out,
+                                              uint8_t* ncand_out, uint32_t tid) {
+  uint32_t* hist = reinterpret_cast<uint32_t*>(M.rsort);  // CNT_BINS counters
+  uint32_t* misc = hist + CNT_BINS;  // [0] chosen so far, [1] T, [2] ties wanted, [3] nt, [4] T saturated, [6..] scan
+  uint32_t* ocnt = misc + 16;        // chosen clusters: count, first key, diagonal sum
+  uint32_t* ostart = ocnt + TOP_CANDIDATES;
+  long long* osum = reinterpret_cast<long long*>(ostart + TOP_CANDIDATES);
+  static_assert(((CNT_BINS + 16 + 2 * TOP_CANDIDATES) * 4) % 8 == 0, "sums 8-byte aligned");
+  static_assert(sizeof(RSortLds) >= (CNT_BINS + 16 + 2 * TOP_CANDIDATES) * 4 + TOP_CANDIDATES * 8, "fits");
+  for (uint32_t i = tid; i < CNT_BINS; i += T) hist[i] = 0;
+  if (tid < 16) misc[tid] = 0;
+  team_sync<T>();
+  const uint32_t chunk = (n + T - 1) / T;
+  const uint32_t c0 = min(n, tid * chunk), c1 = min(n, c0 + chunk);
+  // f(count, first key, diagonal sum) for each cluster this thread owns, in order
+  auto walk = [&](auto&& f) {
+    if (c0 >= c1) return;
+    uint64_t kp = c0 > 0 ? M.key[c0 - 1] : 0ull;
+    bool active = false, done = false;
+    uint32_t cs = 0, cnt = 0;
+    int64_t sum = 0;
+    constexpr uint32_t U = 16;  // keys loaded ahead
+    for (uint32_t e0 = c0; e0 < n && !done; e0 += U) {
+      uint64_t kb[U];
+#pragma unroll
+      for (uint32_t u = 0; u < U; u++) kb[u] = e0 + u < n ? M.key[e0 + u] : 0ull;
+#pragma unroll
+      for (uint32_t u = 0; u < U; u++) {
+        const uint32_t e = e0 + u;
+        if (done || e >= n) break;
+        const uint64_t k = kb[u];
+        const bool st = e == 0 || (k >> DIAG_BITS) != (kp >> DIAG_BITS) || hit_diag(k) - hit_diag(kp) > MERGE_DISTANCE;
+        if (st) {
+          if (active) f(cnt, cs, sum);
+          active = false;
+          if (e >= c1) {
+            done = true;
+            break;
+          }
+          active = true;
+          cs = e;
+          cnt = 0;
+          sum = 0;
+        }
+        cnt++;
+        sum += hit_diag(k);
+        kp = k;
+      }
+    }
+    if (active) f(cnt, cs, sum);
+  };
+  // 1. counts
+  uint32_t tally[CNT_REG];
+#pragma unroll
+  for (uint32_t j = 0; j < CNT_REG; j++) tally[j] = 0;
+  walk([&](uint32_t c, uint32_t, int64_t) {
+    if (c <= CNT_REG) {
+#pragma unroll
+      for (uint32_t j = 0; j < CNT_REG; j++) tally[j] += c == j + 1 ? 1u : 0u;
+    } else {
+      atomicAdd(&hist[min(c, CNT_BINS - 1)], 1u);
+    }
+  });
+#pragma unroll
+  for (uint32_t j = 0; j < CNT_REG; j++) {
+    const uint32_t v = group_sum_u32<64>(tally[j]);
+    if (lane_id() == 0 && v) atomicAdd(&hist[j + 1], v);
+  }
+  team_sync<T>();
+  {
+    // bins in descending count order, CNT_BINS / T per thread: the clusters counted
+    // above a thread's bins are the scan of the sums of the threads before it
+    constexpr uint32_t PB = CNT_BINS / T;
+    static_assert(CNT_BINS % T == 0, "whole bins per thread");
+    const uint32_t hi = CNT_BINS - 1 - tid * PB;
+    uint32_t local = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < PB; j++) local += hist[hi - j];
+    uint32_t C;
+    uint32_t above = team_excl_scan<T>(local, C, misc + 6);
+    const uint32_t nt = min(C, (uint32_t)TOP_CANDIDATES);
+#pragma unroll
+    for (uint32_t j = 0; j < PB; j++) {
+      const uint32_t h = hist[hi - j];
+      if (above < nt && above + h >= nt) {  // the nt-th cluster is counted hi - j
+        misc[1] = hi - j;
+        misc[2] = nt - above;
+        misc[4] = hi - j == CNT_BINS - 1 ? 1u : 0u;
+      }
+      above += h;
+    }
+    if (tid == 0) misc[3] = nt;
+  }
+  team_sync<T>();
+  const uint32_t Tc = misc[1], want = misc[2], nt = misc[3];
+  if (misc[4]) return false;
+  // 2. the chosen clusters
+  uint32_t mine = 0;
+  if (Tc <= CNT_REG) {
+#pragma unroll
+    for (uint32_t j = 0; j < CNT_REG; j++) mine += Tc == j + 1 ? tally[j] : 0u;
+  } else {
+    walk([&](uint32_t c, uint32_t, int64_t) { mine += c == Tc ? 1u : 0u; });
+  }
+  uint32_t tot;
+  uint32_t tb = team_excl_scan<T>(mine, tot, misc + 6);
+  walk([&](uint32_t c, uint32_t cs, int64_t sum) {
+    bool take = c > Tc;
+    if (c == Tc) {
+      take = tb < want;
+      tb++;
+    }
+    if (take) {
+      const uint32_t at = atomicAdd(&misc[0], 1u);
+      if (at < (uint32_t)TOP_CANDIDATES) {
+        ocnt[at] = c;
+        ostart[at] = cs;
+        osum[at] = (long long)sum;
+      }
+    }
+  });
+  team_sync<T>();
+  if (tid < nt) {
+    const uint32_t c = ocnt[tid], cs = ostart[tid];
+    uint32_t rk = 0;
+    for (uint32_t j = 0; j < nt; j++) rk += (ocnt[j] > c || (ocnt[j] == c && ostart[j] < cs)) ? 1u : 0u;
+    const uint32_t gk = (uint32_t)(M.key[cs] >> DIAG_BITS);
+    const int64_t avg = (int64_t)osum[tid] / (int64_t)c;
+    const bool rev = gk & 1u;
+    int64_t position = rev ? avg - (int64_t)L + K : avg;
+    if (position < 0) position = 0;
+    Cand cd;
+    cd.pos = (uint64_t)position;
+    cd.contig = gk >> 1;
+    cd.strand_score = (rev ? 0x80000000u : 0u) | min(c, 0x7FFFFFFFu);
+    out[rk] = cd;
+  }
+  if (tid == 0) *ncand_out = (uint8_t)nt;
+  team_sync<T>();
+  return true;
+}
+
 template <int T, bool FEAT>
 __device__ __forceinline__ bool cluster_read(const uint64_t* hits, uint32_t n, uint32_t L, const ClusterMem& M0, Cand* out,
                              uint8_t* ncand_out, uint32_t* sh, const ClusterFeat& F) {
@@ -1650,17 +1834,20 @@ __device__ __forceinline__ bool cluster_read(const uint64_t* hits, uint32_t n, u
   while (p2 < n) p2 <<= 1;
   ClusterMem M = M0;
   // group by (contig, strand), sort each group by diagonal (:36-49): one sort by (group, diagonal)
-  bool sorted = false;
-  if constexpr (T != 64) sorted = group_segment_sort<T, FEAT>(hits, n, M, tid);
+  bool sorted = false, by_rank = false;
+  if constexpr (T != 64) sorted = group_segment_sort<T, FEAT>(hits, n, M, tid, by_rank);
   if (sorted) {
     // per-group arrays in LDS (at most CLUSTER_GT_MAX groups): the group keys
-    // and first emission indices by rank as the segmented sort left them, ranks
+    // and first emission indices by segment as the segmented sort left them, ranks
     // and insertion order in gt_lo, free once the sort is done
     M.g_key = M.gt_cnt;
     M.g_min = M.gt_emin;
     M.g_rank = reinterpret_cast<uint32_t*>(M.gt_lo);
     M.g_sort = M.g_rank + CLUSTER_GT_MAX;
     M.g_c0 = reinterpret_cast<uint32_t*>(M.gt_hi);  // g_c0, then top_by_count's rank tables
+    if constexpr (T != 64 && !FEAT) {
+      if (by_rank && top20_ordered<T>(M, n, L, out, ncand_out, tid)) return true;
+    }
   } else {
     for (uint32_t i = tid; i < p2; i += T) {
       M.key[i] = i < n ? hits[i] : ~0ull;
