@@ -199,6 +199,9 @@ struct pbw_ctx {
   DevBuf<Counters> d_ctr;
   DevBuf<int32_t> d_rows;
   DevBuf<uint8_t> d_gstage;  // DP staging in device memory for tiles beyond LDS
+  DevBuf<uint64_t> d_ck, d_cktmp;  // block-tier composite keys and the merges' second buffer
+  DevBuf<CgGroup> d_cg;
+  DevBuf<uint32_t> d_cg_n, d_fb;
   DevBuf<uint64_t> d_tb;
   // paired-end workspace
   DevBuf<DpJob> d_wjobs;
@@ -732,17 +735,19 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
   }
   const uint64_t* hits_base = c->d_hits.p;
   const uint8_t* hflag_base = feat ? c->d_hflag.p : nullptr;
+  // hit slots a read's offset can reach: the fixed ones and the overflow slots
+  const size_t arena_n = base_cap + (n_ov && ov_total > slack ? (size_t)ov_total : slack);
   if (n_ov) {
     // overflow slots after the fixed ones: in place when the slack holds them,
     // else in a second arena that starts with a copy of the fixed slots
     uint64_t* arena = c->d_hits.p;
     uint8_t* farena = feat ? c->d_hflag.p : nullptr;
     if (ov_total > slack) {
-      c->d_hits_ov.ensure(base_cap + ov_total);
+      c->d_hits_ov.ensure(base_cap + ov_total + (c->d_hits_ov.n < base_cap + ov_total ? ov_total / 4 : 0));
       HIP_CHECK(hipMemcpyAsync(c->d_hits_ov.p, c->d_hits.p, base_cap * 8, hipMemcpyDeviceToDevice, s));
       arena = c->d_hits_ov.p;
       if (feat) {
-        c->d_hflag_ov.ensure(base_cap + ov_total);
+        c->d_hflag_ov.ensure(base_cap + ov_total + (c->d_hflag_ov.n < base_cap + ov_total ? ov_total / 4 : 0));
         HIP_CHECK(hipMemcpyAsync(c->d_hflag_ov.p, c->d_hflag.p, base_cap, hipMemcpyDeviceToDevice, s));
         farena = c->d_hflag_ov.p;
       }
@@ -772,8 +777,8 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
   c->d_ncand.ensure(B);
   c->d_tier2.ensure(B);
   c->d_mid.ensure(B);
-  c->d_ntier2.ensure(2);  // [0] block tier, [1] second wave tier
-  HIP_CHECK(hipMemsetAsync(c->d_ntier2.p, 0, 8, s));
+  c->d_ntier2.ensure(4);  // [0] block tier, [1] second wave tier, [2] block tier handed to k_cluster_tier2
+  HIP_CHECK(hipMemsetAsync(c->d_ntier2.p, 0, 16, s));
   ClusterArgs ca{};
   ca.R = R;
   ca.hits = hits_base;
@@ -801,16 +806,36 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
   float ms_cl = 0.f;
   HIP_CHECK(hipEventElapsedTime(&ms_cl, c->ev[4], c->ev[5]));
   if (nt2) {
-    // one 256-thread block per read, per-block scratch sized for the largest read; up to
-    // 4096 blocks within 16 GiB (repeat-rich genomes send most reads here, with 10^4 hits each)
-    ca.scratch_per_block = cluster_scratch_words(max_hits);
-    const uint64_t fit = std::max<uint64_t>(1, (1ull << 31) / ca.scratch_per_block);
-    const uint32_t blocks = (uint32_t)std::min<uint64_t>({(uint64_t)nt2, 4096, fit});
-    c->d_scratch.ensure((size_t)blocks * ca.scratch_per_block);
-    ca.scratch = c->d_scratch.p;
+    const uint32_t* t2list = c->d_tier2.p;
+    uint32_t nt2b = nt2;  // reads for k_cluster_tier2
     HIP_CHECK(hipEventRecord(c->ev[6], s));
-    launch_cluster_tier2(ca, c->d_tier2.p, nt2, blocks, s);
-    HIP_CHECK(hipGetLastError());
+    if (!feat) {
+      // three launches (group / sort / select) for the reads they take; the rest,
+      // listed on the device, to k_cluster_tier2 below
+      if (c->d_ck.n < arena_n) {  // with headroom: the arena's size varies from batch to batch
+        c->d_ck.ensure(arena_n + arena_n / 4);
+        c->d_cktmp.ensure(arena_n + arena_n / 4);
+      }
+      c->d_cg.ensure((size_t)nt2 * (CG_GMAX + 1));
+      c->d_cg_n.ensure(nt2);
+      c->d_fb.ensure(nt2);
+      CgArgs g{c->d_ck.p, c->d_cktmp.p, c->d_cg.p, c->d_cg_n.p, c->d_fb.p, c->d_ntier2.p + 2};
+      launch_cluster_grid(ca, g, c->d_tier2.p, nt2, s);
+      HIP_CHECK(hipGetLastError());
+      nt2b = download_u32(c, c->d_ntier2.p + 2);
+      t2list = c->d_fb.p;
+    }
+    if (nt2b) {
+      // one 256-thread block per read, per-block scratch sized for the largest read; up to
+      // 4096 blocks within 16 GiB (repeat-rich genomes send most reads here, with 10^4 hits each)
+      ca.scratch_per_block = cluster_scratch_words(max_hits);
+      const uint64_t fit = std::max<uint64_t>(1, (1ull << 31) / ca.scratch_per_block);
+      const uint32_t blocks = (uint32_t)std::min<uint64_t>({(uint64_t)nt2b, 4096, fit});
+      c->d_scratch.ensure((size_t)blocks * ca.scratch_per_block);
+      ca.scratch = c->d_scratch.p;
+      launch_cluster_tier2(ca, t2list, nt2b, blocks, s);
+      HIP_CHECK(hipGetLastError());
+    }
     HIP_CHECK(hipEventRecord(c->ev[7], s));
     c->stats.tier2_cluster_reads += nt2;
   }

@@ -156,6 +156,29 @@ struct ClusterArgs {
   uint16_t* feat;
 };
 void launch_cluster(const ClusterArgs& a, hipStream_t s);
+// Block tier in three launches (k_cg_group, k_cg_sort, k_cg_select) for reads of
+// at most CG_GMAX groups whose diagonals each span less than 2^32 - 1: the hits as
+// composite keys in `ck` (group rank << 32 | biased diagonal - the group's lowest,
+// at the read's hit offset), the groups in `cg` (CG_GMAX + 1 per list entry), the
+// group count in cg_n (0: the read is appended to fb_list / fb_n for
+// k_cluster_tier2, as is a read whose 20th cluster count the selection cannot
+// resolve); `tmp` (as large as `ck`) is the merges' second buffer.  The hits
+// stay intact.
+constexpr uint32_t CG_GMAX = 56;
+struct CgGroup {
+  unsigned long long lo;  // lowest biased diagonal
+  uint32_t key;           // contig * 2 + strand
+  uint32_t start;         // segment start within the read's keys ([G]: the hit count)
+};
+struct CgArgs {
+  uint64_t* ck;
+  uint64_t* tmp;
+  CgGroup* cg;
+  uint32_t* cg_n;
+  uint32_t* fb_list;
+  uint32_t* fb_n;
+};
+void launch_cluster_grid(const ClusterArgs& a, const CgArgs& g, const uint32_t* list, uint32_t n, hipStream_t s);
 uint64_t cluster_scratch_words(uint32_t max_hits);  // per-block scratch of the block tier (u64 words)
 void launch_cluster_tier2(const ClusterArgs& a, const uint32_t* list, uint32_t n, uint32_t blocks, hipStream_t s);
 

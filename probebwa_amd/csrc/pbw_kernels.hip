@@ -32,6 +32,7 @@
 
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
+#include <rocprim/block/block_radix_sort.hpp>
 
 namespace pbw {
 
@@ -845,13 +846,17 @@ constexpr uint32_t CLUSTER_STAGE = 1024;
 constexpr uint32_t CLUSTER_GT = 1024, CLUSTER_GT_MAX = 512;  // table slots, groups handled
 // Block-tier runs of whole segments sorted by one block radix sort (256 threads).
 constexpr uint32_t RSORT_T = 256, RSORT_BIG = 16, RSORT_SMALL = 4, RSORT_TILE = RSORT_T * RSORT_BIG;
+// rocPRIM's block radix sort with its wave-match ranking: 8-bit digits, so a
+// run of keys of up to 32 bits takes at most four passes.
 template <bool VALS, int IPT>
-using RSort = hipcub::BlockRadixSort<uint32_t, RSORT_T, IPT, typename std::conditional<VALS, uint32_t, hipcub::NullType>::type>;
+using RSort = rocprim::block_radix_sort<uint32_t, RSORT_T, IPT, typename std::conditional<VALS, uint32_t, rocprim::empty_type>::type,
+                                        1, 1, 8, rocprim::block_radix_rank_algorithm::match>;
 union RSortLds {
-  typename RSort<true, RSORT_BIG>::TempStorage big;
-  typename RSort<true, RSORT_SMALL>::TempStorage small;
-  typename RSort<false, RSORT_BIG>::TempStorage big_keys;
-  typename RSort<false, RSORT_SMALL>::TempStorage small_keys;
+  typename RSort<true, RSORT_BIG>::storage_type big;
+  typename RSort<true, RSORT_SMALL>::storage_type small;
+  typename RSort<false, RSORT_BIG>::storage_type big_keys;
+  typename RSort<false, RSORT_SMALL>::storage_type small_keys;
+  uint64_t raw[2048];  // also the merge windows and the top-20 selection's tables
 };
 
 // The same network for p2 <= 64 * E keys held by one wave in registers:
@@ -1093,8 +1098,9 @@ __device__ __forceinline__ void run_radix_sort(const ClusterMem& M, uint32_t a0,
       if constexpr (VALS) v[j] = M.idx[a0 + pos];
     }
   }
-  if constexpr (VALS) BRS(*reinterpret_cast<typename BRS::TempStorage*>(M.rsort)).SortBlockedToStriped(k, v, 0, bits);
-  else BRS(*reinterpret_cast<typename BRS::TempStorage*>(M.rsort)).SortBlockedToStriped(k, 0, bits);
+  auto& tmp = *reinterpret_cast<typename BRS::storage_type*>(M.rsort);
+  if constexpr (VALS) BRS().sort_to_striped(k, v, tmp, 0u, (unsigned)bits);
+  else BRS().sort_to_striped(k, tmp, 0u, (unsigned)bits);
 #pragma unroll
   for (int j = 0; j < IPT; j++) {  // striped: thread tid holds ranks j * RSORT_T + tid (coalesced stores)
     const uint32_t pos = (uint32_t)j * RSORT_T + tid;
@@ -1107,26 +1113,12 @@ __device__ __forceinline__ void run_radix_sort(const ClusterMem& M, uint32_t a0,
   __syncthreads();
 }
 
-// Block tier: the (group, diagonal) sort as a segmented sort.  The hits are
-// counted per group in an LDS table (with each group's lowest and highest
-// diagonal and its first emission index), the groups ranked in ascending order,
-// the hits scattered into one segment per group (key[], with the emission index
-// in idx[] when VALS), and then runs of consecutive whole segments are sorted by
-// a block radix sort of 32-bit composite keys (run_radix_sort) -- as many
-// segments as fit one 4096-hit tile and a 2^32 span of diagonals.  A segment
-// larger than a tile is sorted tile by tile and the sorted tiles merged pairwise
-// (block_merge) through the scratch that cl_sum / cl_start provide until the
-// clustering needs them; one whose diagonals span 2^32 or more goes through the
-// staged bitonic network there.  Ties between equal keys may land in any order,
-// as in the full sort: the clustering only sums diagonals (and, with VALS, reads
-// per-hit flags).  On return gt_key[r] is the segment start of group rank r
-// (gt_key[G] = n), gt_cnt[r] its group key and gt_emin[r] its first emission
-// index.  Returns false (nothing written) when the read has more than
-// CLUSTER_GT_MAX groups.
-template <int T, bool VALS>
-__device__ __forceinline__ bool group_segment_sort(const uint64_t* hits, uint32_t n, const ClusterMem& M, uint32_t tid,
-                                                   bool& by_rank) {
-  static_assert(T == RSORT_T, "the block tier runs 256 threads");
+// The block tier's group table: every hit's (contig, strand) group counted in
+// the LDS table gt_* (CLUSTER_GT slots; per slot the group key, its hits, its
+// lowest and highest biased diagonal and its first emission index); gt_misc[0]
+// = distinct groups (may exceed CLUSTER_GT_MAX: the table then holds only some).
+template <int T>
+__device__ __forceinline__ void group_table_count(const uint64_t* hits, uint32_t n, const ClusterMem& M, uint32_t tid) {
   constexpr uint32_t U = 8;  // hits loaded ahead per thread: the loads of a pass overlap
   for (uint32_t i = tid; i < CLUSTER_GT; i += T) {
     M.gt_key[i] = 0xFFFFFFFFu;
@@ -1193,6 +1185,30 @@ __device__ __forceinline__ bool group_segment_sort(const uint64_t* hits, uint32_
     }
   }
   team_sync<T>();
+}
+
+// Block tier: the (group, diagonal) sort as a segmented sort.  The hits are
+// counted per group in an LDS table (with each group's lowest and highest
+// diagonal and its first emission index), the groups ranked in ascending order,
+// the hits scattered into one segment per group (key[], with the emission index
+// in idx[] when VALS), and then runs of consecutive whole segments are sorted by
+// a block radix sort of 32-bit composite keys (run_radix_sort) -- as many
+// segments as fit one 4096-hit tile and a 2^32 span of diagonals.  A segment
+// larger than a tile is sorted tile by tile and the sorted tiles merged pairwise
+// (block_merge) through the scratch that cl_sum / cl_start provide until the
+// clustering needs them; one whose diagonals span 2^32 or more goes through the
+// staged bitonic network there.  Ties between equal keys may land in any order,
+// as in the full sort: the clustering only sums diagonals (and, with VALS, reads
+// per-hit flags).  On return gt_key[r] is the segment start of group rank r
+// (gt_key[G] = n), gt_cnt[r] its group key and gt_emin[r] its first emission
+// index.  Returns false (nothing written) when the read has more than
+// CLUSTER_GT_MAX groups.
+template <int T, bool VALS>
+__device__ __forceinline__ bool group_segment_sort(const uint64_t* hits, uint32_t n, const ClusterMem& M, uint32_t tid,
+                                                   bool& by_rank) {
+  static_assert(T == RSORT_T, "the block tier runs 256 threads");
+  constexpr uint32_t U = 8;  // hits loaded ahead per thread in the scatter
+  group_table_count<T>(hits, n, M, tid);
   const uint32_t G = M.gt_misc[0];
   if (G > CLUSTER_GT_MAX) return false;
   // Segment order.  Without per-hit values and for at most 56 groups: the
@@ -1681,18 +1697,23 @@ __device__ __forceinline__ bool top_by_count(const ClusterMem& M, uint32_t C, ui
 // A thread owns the clusters whose first key lies in its chunk and follows each
 // to its end, past the chunk when it runs on.  Returns false (uniformly, nothing
 // written) when T falls in the saturated bin; the caller's general path then
-// handles the read.
+// handles the read.  Keys: hit keys (CK false), or composite keys (group rank <<
+// 32 | biased diagonal - the group's lowest, CK true) with the groups in grp[].
+// lds: CNT_BINS + 16 + 2 * TOP_CANDIDATES words and TOP_CANDIDATES doubles.
 constexpr uint32_t CNT_BINS = 1024, CNT_REG = 8;
-template <int T>
-__device__ __forceinline__ bool top20_ordered(const ClusterMem& M, uint32_t n, uint32_t L, Cand* out,
-                                              uint8_t* ncand_out, uint32_t tid) {
-  uint32_t* hist = reinterpret_cast<uint32_t*>(M.rsort);  // CNT_BINS counters
+constexpr size_t TOP20_LDS_BYTES = (CNT_BINS + 16 + 2 * TOP_CANDIDATES) * 4 + TOP_CANDIDATES * 8;
+template <int T, bool CK>
+__device__ __forceinline__ bool top20_ordered(const uint64_t* key, void* lds, const CgGroup* grp, uint32_t n,
+                                              uint32_t L, Cand* out, uint8_t* ncand_out, uint32_t tid) {
+  // group of a key, and the part of its diagonal that differs within a group
+  auto kgroup = [](uint64_t k) -> uint64_t { return CK ? (k >> 32) : (k >> DIAG_BITS); };
+  auto kdiag = [](uint64_t k) -> int64_t { return CK ? (int64_t)(uint32_t)k : hit_diag(k); };
+  uint32_t* hist = reinterpret_cast<uint32_t*>(lds);  // CNT_BINS counters
   uint32_t* misc = hist + CNT_BINS;  // [0] chosen so far, [1] T, [2] ties wanted, [3] nt, [4] T saturated, [6..] scan
   uint32_t* ocnt = misc + 16;        // chosen clusters: count, first key, diagonal sum
   uint32_t* ostart = ocnt + TOP_CANDIDATES;
   long long* osum = reinterpret_cast<long long*>(ostart + TOP_CANDIDATES);
   static_assert(((CNT_BINS + 16 + 2 * TOP_CANDIDATES) * 4) % 8 == 0, "sums 8-byte aligned");
-  static_assert(sizeof(RSortLds) >= (CNT_BINS + 16 + 2 * TOP_CANDIDATES) * 4 + TOP_CANDIDATES * 8, "fits");
   for (uint32_t i = tid; i < CNT_BINS; i += T) hist[i] = 0;
   if (tid < 16) misc[tid] = 0;
   team_sync<T>();
@@ -1701,7 +1722,7 @@ __device__ __forceinline__ bool top20_ordered(const ClusterMem& M, uint32_t n, u
   // f(count, first key, diagonal sum) for each cluster this thread owns, in order
   auto walk = [&](auto&& f) {
     if (c0 >= c1) return;
-    uint64_t kp = c0 > 0 ? M.key[c0 - 1] : 0ull;
+    uint64_t kp = c0 > 0 ? key[c0 - 1] : 0ull;
     bool active = false, done = false;
     uint32_t cs = 0, cnt = 0;
     int64_t sum = 0;
@@ -1709,13 +1730,13 @@ __device__ __forceinline__ bool top20_ordered(const ClusterMem& M, uint32_t n, u
     for (uint32_t e0 = c0; e0 < n && !done; e0 += U) {
       uint64_t kb[U];
 #pragma unroll
-      for (uint32_t u = 0; u < U; u++) kb[u] = e0 + u < n ? M.key[e0 + u] : 0ull;
+      for (uint32_t u = 0; u < U; u++) kb[u] = e0 + u < n ? key[e0 + u] : 0ull;
 #pragma unroll
       for (uint32_t u = 0; u < U; u++) {
         const uint32_t e = e0 + u;
         if (done || e >= n) break;
         const uint64_t k = kb[u];
-        const bool st = e == 0 || (k >> DIAG_BITS) != (kp >> DIAG_BITS) || hit_diag(k) - hit_diag(kp) > MERGE_DISTANCE;
+        const bool st = e == 0 || kgroup(k) != kgroup(kp) || kdiag(k) - kdiag(kp) > MERGE_DISTANCE;
         if (st) {
           if (active) f(cnt, cs, sum);
           active = false;
@@ -1729,7 +1750,7 @@ __device__ __forceinline__ bool top20_ordered(const ClusterMem& M, uint32_t n, u
           sum = 0;
         }
         cnt++;
-        sum += hit_diag(k);
+        sum += kdiag(k);
         kp = k;
       }
     }
@@ -1810,8 +1831,16 @@ __device__ __forceinline__ bool top20_ordered(const ClusterMem& M, uint32_t n, u
     const uint32_t c = ocnt[tid], cs = ostart[tid];
     uint32_t rk = 0;
     for (uint32_t j = 0; j < nt; j++) rk += (ocnt[j] > c || (ocnt[j] == c && ostart[j] < cs)) ? 1u : 0u;
-    const uint32_t gk = (uint32_t)(M.key[cs] >> DIAG_BITS);
-    const int64_t avg = (int64_t)osum[tid] / (int64_t)c;
+    uint32_t gk;
+    int64_t dsum = (int64_t)osum[tid];
+    if constexpr (CK) {  // the diagonals' sum from the offsets' sum: c (lowest - bias) + sum
+      const CgGroup& g = grp[(uint32_t)(key[cs] >> 32)];
+      gk = g.key;
+      dsum += (int64_t)c * ((int64_t)g.lo - DIAG_BIAS);
+    } else {
+      gk = (uint32_t)(key[cs] >> DIAG_BITS);
+    }
+    const int64_t avg = dsum / (int64_t)c;
     const bool rev = gk & 1u;
     int64_t position = rev ? avg - (int64_t)L + K : avg;
     if (position < 0) position = 0;
@@ -1846,7 +1875,8 @@ __device__ __forceinline__ bool cluster_read(const uint64_t* hits, uint32_t n, u
     M.g_sort = M.g_rank + CLUSTER_GT_MAX;
     M.g_c0 = reinterpret_cast<uint32_t*>(M.gt_hi);  // g_c0, then top_by_count's rank tables
     if constexpr (T != 64 && !FEAT) {
-      if (by_rank && top20_ordered<T>(M, n, L, out, ncand_out, tid)) return true;
+      static_assert(sizeof(RSortLds) >= TOP20_LDS_BYTES, "the selection's tables fit the sort's LDS");
+      if (by_rank && top20_ordered<T, false>(M.key, M.rsort, nullptr, n, L, out, ncand_out, tid)) return true;
     }
   } else {
     for (uint32_t i = tid; i < p2; i += T) {
@@ -2312,6 +2342,254 @@ void launch_cluster_tier2(const ClusterArgs& a, const uint32_t* list, uint32_t n
   if (n == 0) return;
   if (a.feat) hipLaunchKernelGGL(k_cluster_tier2<true>, dim3(blocks), dim3(256), 0, s, a, list, n);
   else hipLaunchKernelGGL(k_cluster_tier2<false>, dim3(blocks), dim3(256), 0, s, a, list, n);
+}
+
+// ---------------------------------------------------------------------------
+// Block tier in three launches, for reads of at most CG_GMAX groups whose
+// diagonals each span less than 2^32 - 1 (features off; at genome scale every
+// read): each phase runs at the occupancy its own registers and LDS allow,
+// where one kernel for all of them would run every phase at the lowest.
+//  k_cg_group:  the group table (group_table_count), the groups ranked in
+//               FxHashMap iteration order from their first emission indices,
+//               segments laid out by rank, every hit scattered into its
+//               segment as a composite key (rank << 32 | biased diagonal - the
+//               group's lowest);
+//  k_cg_sort:   runs of whole segments of up to RSORT_TILE keys sorted by one
+//               block radix sort of the 64-bit composite keys (8-bit digits,
+//               five passes over the 38 bits that vary); a larger segment tile
+//               by tile on its low 32 bits, the tiles then merged pairwise
+//               (block_merge) through a second buffer;
+//  k_cg_select: top20_ordered over the sorted keys.
+// A read that fails the conditions (or whose 20th count lies in the selection's
+// saturated bin) is listed for k_cluster_tier2, its hits left intact.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_cg_group(ClusterArgs a, CgArgs g, const uint32_t* list, uint32_t nlist) {
+  __shared__ uint32_t gt_key[CLUSTER_GT], gt_cnt[CLUSTER_GT], gt_misc[2 + 256 / 64];
+  __shared__ unsigned long long gt_lo[CLUSTER_GT], gt_hi[CLUSTER_GT];
+  __shared__ uint32_t gt_emin[CLUSTER_GT];  // first emission index per slot, then the slot's rank
+  __shared__ uint64_t sk[64];               // (first emission index << 32 | slot), insertion order
+  __shared__ uint32_t sv[192];              // group key and insertion list for the ranks, then ranks
+  __shared__ uint32_t rslot[64];            // slot of each rank
+  __shared__ uint32_t fail;
+  ClusterMem M{};
+  M.gt_key = gt_key;
+  M.gt_cnt = gt_cnt;
+  M.gt_misc = gt_misc;
+  M.gt_lo = gt_lo;
+  M.gt_hi = gt_hi;
+  M.gt_emin = gt_emin;
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t li = blockIdx.x; li < nlist; li += gridDim.x) {
+    const uint32_t r = list[li];
+    const uint32_t n = a.nhits[r];
+    const uint64_t* hits = a.hits + a.hoff[r];
+    if (tid < 64) sk[tid] = ~0ull;
+    if (tid == 0) fail = 0;
+    group_table_count<256>(hits, n, M, tid);
+    const uint32_t G = gt_misc[0];
+    if (G <= CG_GMAX) {
+      for (uint32_t i = tid; i < CLUSTER_GT; i += 256)
+        if (gt_key[i] != 0xFFFFFFFFu) {
+          const uint32_t at = atomicAdd(&gt_misc[1], 1u);
+          sk[at] = ((uint64_t)gt_emin[i] << 32) | i;
+        }
+      __syncthreads();
+      if (tid < 64) {  // wave 0: insertion order, ranks, segment starts
+        const uint32_t lane = tid;
+        wave_bitonic_reg<1>(sk, nullptr, 64);
+        const uint32_t slt = (uint32_t)sk[lane];
+        sv[lane] = lane < G ? gt_key[slt] : 0u;
+        sv[64 + lane] = lane;
+        wave_sync();
+        group_ranks_wave(G, sv, sv + 64, sv + 128);
+        wave_sync();
+        if (lane < G) rslot[sv[128 + lane]] = slt;
+        wave_sync();
+        uint32_t cnt = 0, sl = 0, gkey = 0;
+        unsigned long long lo = 0, hi = 0;
+        if (lane < G) {
+          sl = rslot[lane];
+          cnt = gt_cnt[sl];
+          lo = gt_lo[sl];
+          hi = gt_hi[sl];
+          gkey = gt_key[sl];
+        }
+        const uint32_t start = wave_incl_scan(cnt) - cnt;
+        const bool wide = lane < G && hi - lo >= 0xFFFFFFFFull;
+        CgGroup* cg = g.cg + (uint64_t)li * (CG_GMAX + 1);
+        if (lane < G) {
+          cg[lane].lo = lo;
+          cg[lane].key = gkey;
+          cg[lane].start = start;
+          gt_cnt[sl] = start;  // the slot's scatter cursor
+          gt_emin[sl] = lane;  // the slot's rank
+        }
+        if (lane == 0) {
+          cg[G].lo = 0;
+          cg[G].key = 0;
+          cg[G].start = n;
+        }
+        if (__ballot(wide) && lane == 0) fail = 1;
+      }
+    } else if (tid == 0) {
+      fail = 1;
+    }
+    __syncthreads();
+    if (fail) {
+      if (tid == 0) {
+        g.cg_n[li] = 0;
+        g.fb_list[atomicAdd(g.fb_n, 1u)] = r;
+      }
+      __syncthreads();
+      continue;
+    }
+    if (tid == 0) g.cg_n[li] = G;
+    uint64_t* ck = g.ck + a.hoff[r];
+    constexpr uint32_t U = 8;
+    for (uint32_t i0 = tid; i0 < n; i0 += U * 256) {
+      uint64_t kb[U];
+#pragma unroll
+      for (uint32_t u = 0; u < U; u++) kb[u] = i0 + u * 256 < n ? hits[i0 + u * 256] : 0ull;
+#pragma unroll
+      for (uint32_t u = 0; u < U; u++) {
+        const uint32_t i = i0 + u * 256;
+        if (i >= n) break;
+        const uint64_t k = kb[u];
+        const uint32_t gg = (uint32_t)(k >> DIAG_BITS);
+        const unsigned long long d = k & ((1ull << DIAG_BITS) - 1);
+        const uint32_t g0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)gg);
+        const uint64_t act = __ballot(true);
+        uint32_t pos, sl;
+        if (__ballot(gg == g0) == act) {
+          // one group across the active lanes (a prefix of the wave): one cursor update
+          sl = gt_slot(g0);
+          while (gt_key[sl] != g0) sl = (sl + 1) & (CLUSTER_GT - 1);
+          uint32_t base = 0;
+          if (lane_id() == 0) base = atomicAdd(&gt_cnt[sl], (uint32_t)__popcll(act));
+          pos = (uint32_t)__builtin_amdgcn_readfirstlane((int)base) + lane_id();
+        } else {
+          sl = gt_slot(gg);
+          while (gt_key[sl] != gg) sl = (sl + 1) & (CLUSTER_GT - 1);
+          pos = atomicAdd(&gt_cnt[sl], 1u);
+        }
+        ck[pos] = ((uint64_t)gt_emin[sl] << 32) | (uint64_t)(uint32_t)(d - gt_lo[sl]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// One block radix sort of cnt <= 256 IPT composite keys in place (the padding
+// keys sort last: every real key is below 2^end_bit - 1).
+template <int IPT>
+using CgSort = rocprim::block_radix_sort<uint64_t, 256, IPT, rocprim::empty_type, 1, 1, 8,
+                                         rocprim::block_radix_rank_algorithm::match>;
+union CgSortLds {
+  typename CgSort<16>::storage_type big;
+  typename CgSort<4>::storage_type small;
+  uint64_t raw[MERGE_CHUNK];  // block_merge's windows
+};
+template <int IPT>
+__device__ __forceinline__ void cg_run_sort(uint64_t* key, uint32_t cnt, uint32_t end_bit, CgSortLds& lds,
+                                            uint32_t tid) {
+  uint64_t k[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; j++) {
+    const uint32_t pos = (uint32_t)j * 256 + tid;
+    k[j] = pos < cnt ? key[pos] : ~0ull;
+  }
+  CgSort<IPT>().sort_to_striped(k, *reinterpret_cast<typename CgSort<IPT>::storage_type*>(&lds), 0u, end_bit);
+#pragma unroll
+  for (int j = 0; j < IPT; j++) {  // striped: thread tid holds ranks j * 256 + tid
+    const uint32_t pos = (uint32_t)j * 256 + tid;
+    if (pos < cnt) key[pos] = k[j];
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_cg_sort(ClusterArgs a, CgArgs g, const uint32_t* list, uint32_t nlist) {
+  __shared__ CgSortLds rs;
+  __shared__ uint32_t stage_val[CLUSTER_STAGE];
+  __shared__ uint32_t st[CG_GMAX + 1];
+  ClusterMem M{};
+  M.rsort = &rs;
+  M.stage_val = stage_val;
+  const uint32_t tid = threadIdx.x;
+  constexpr uint32_t RUN_BITS = 38;  // rank < 64 above 32 bits of diagonal offset
+  static_assert(CG_GMAX < 64, "ranks in 6 bits");
+  for (uint32_t li = blockIdx.x; li < nlist; li += gridDim.x) {
+    const uint32_t G = g.cg_n[li];
+    if (G == 0) continue;
+    const uint32_t r = list[li];
+    const CgGroup* cg = g.cg + (uint64_t)li * (CG_GMAX + 1);
+    if (tid <= G) st[tid] = cg[tid].start;
+    __syncthreads();
+    uint64_t* key = g.ck + a.hoff[r];
+    uint64_t* tmp = g.tmp + a.hoff[r];
+    uint32_t q = 0;
+    while (q < G) {
+      const uint32_t a0 = st[q];
+      uint32_t q1 = q + 1;
+      while (q1 < G && st[q1 + 1] - a0 <= RSORT_TILE) q1++;
+      const uint32_t cnt = st[q1] - a0;
+      if (cnt <= RSORT_TILE) {
+        if (cnt > 1) {
+          if (cnt <= 4 * 256) cg_run_sort<4>(key + a0, cnt, RUN_BITS, rs, tid);
+          else cg_run_sort<16>(key + a0, cnt, RUN_BITS, rs, tid);
+        }
+      } else {  // one segment over a tile: its keys share the rank bits
+        for (uint32_t t0 = 0; t0 < cnt; t0 += RSORT_TILE)
+          cg_run_sort<16>(key + a0 + t0, min(RSORT_TILE, cnt - t0), 32u, rs, tid);
+        bool in_key = true;
+        for (uint32_t w = RSORT_TILE; w < cnt; w <<= 1) {
+          const uint64_t* sk = in_key ? key : tmp;
+          uint64_t* dk = in_key ? tmp : key;
+          for (uint32_t s0 = 0; s0 < cnt; s0 += 2 * w) {
+            const uint32_t na = min(w, cnt - s0), nb = min(w, cnt - s0 - na);
+            block_merge<false>(M, sk, nullptr, dk, nullptr, a0 + s0, na, nb, tid);
+          }
+          __syncthreads();
+          in_key = !in_key;
+        }
+        if (!in_key) {
+          for (uint32_t i = tid; i < cnt; i += 256) key[a0 + i] = tmp[a0 + i];
+          __syncthreads();
+        }
+      }
+      q = q1;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_cg_select(ClusterArgs a, CgArgs g, const uint32_t* list, uint32_t nlist) {
+  __shared__ uint64_t lds[(TOP20_LDS_BYTES + 7) / 8];
+  __shared__ CgGroup gs[CG_GMAX + 1];
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t li = blockIdx.x; li < nlist; li += gridDim.x) {
+    const uint32_t G = g.cg_n[li];
+    if (G == 0) continue;
+    const uint32_t r = list[li];
+    const CgGroup* cg = g.cg + (uint64_t)li * (CG_GMAX + 1);
+    if (tid <= G) gs[tid] = cg[tid];
+    __syncthreads();
+    const uint32_t L = (uint32_t)(a.R.seq_off[r + 1] - a.R.seq_off[r]);
+    if (!top20_ordered<256, true>(g.ck + a.hoff[r], lds, gs, a.nhits[r], L, a.cands + (uint64_t)r * TOP_CANDIDATES,
+                                  a.ncand + r, tid)) {
+      // the 20th count lies in the saturated bin (20 clusters of 1023 hits or
+      // more): the general tier takes the read from its intact hits
+      if (tid == 0) g.fb_list[atomicAdd(g.fb_n, 1u)] = r;
+    }
+    __syncthreads();
+  }
+}
+
+void launch_cluster_grid(const ClusterArgs& a, const CgArgs& g, const uint32_t* list, uint32_t n, hipStream_t s) {
+  if (n == 0) return;
+  const uint32_t blocks = n < 65536 ? n : 65536;
+  hipLaunchKernelGGL(k_cg_group, dim3(blocks), dim3(256), 0, s, a, g, list, n);
+  hipLaunchKernelGGL(k_cg_sort, dim3(blocks), dim3(256), 0, s, a, g, list, n);
+  hipLaunchKernelGGL(k_cg_select, dim3(blocks), dim3(256), 0, s, a, g, list, n);
 }
 
 // ---------------------------------------------------------------------------
