@@ -70,6 +70,11 @@ __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
   uint32_t lo = shfl_u32((uint32_t)v, src), hi = shfl_u32((uint32_t)(v >> 32), src);
   return ((uint64_t)hi << 32) | lo;
 }
+__device__ __forceinline__ uint64_t shfl_up_u64(uint64_t v, int d) {
+  const uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)v, (unsigned)d, 64);
+  const uint32_t hi = (uint32_t)__shfl_up((int)(uint32_t)(v >> 32), (unsigned)d, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
 __device__ __forceinline__ int64_t shfl_xor_i64(int64_t v, int m) {
   int lo = __shfl_xor((int)(uint32_t)v, m, 64), hi = __shfl_xor((int)(uint32_t)((uint64_t)v >> 32), m, 64);
   return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
@@ -2562,8 +2567,198 @@ __global__ __launch_bounds__(256) void k_cg_sort(ClusterArgs a, CgArgs g, const 
   }
 }
 
+// First cluster start after position q in the start bitmap (the sentinel bit at
+// the key count bounds the search).
+__device__ __forceinline__ uint32_t bm_next_start(const uint64_t* bm, uint32_t q) {
+  uint32_t w = q >> 6;
+  const uint32_t o = q & 63u;
+  uint64_t b = o == 63u ? 0ull : (bm[w] >> (o + 1)) << (o + 1);
+  while (!b) b = bm[++w];
+  return (w << 6) + (uint32_t)__builtin_ctzll(b);
+}
+
+// top20_ordered's selection over composite keys, bit-parallel: a bitmap of the
+// cluster starts (bit i set when key i begins a cluster, bit n set as a
+// sentinel) built from coalesced loads, one ballot per 64 keys; a cluster of s
+// keys starting at p is a set bit p whose next set bit is p + s, so the clusters
+// of 1..CNT_REG keys are counted per 64-bit word with shifted masks, and only the
+// starts of larger ones are followed to their ends.  The same masks give, once T
+// is known, the clusters counted above it and the ties, whose order is the word
+// order (a scan of the words' tie counts) and the bit order within a word.  The
+// chosen clusters' offset sums are then added up by the waves.  n < 64 BM_WORDS.
+constexpr uint32_t BM_WORDS = 2048;
+template <int T>
+__device__ __forceinline__ bool top20_bitmap(const uint64_t* key, uint64_t* bm, void* tabs, const CgGroup* grp,
+                                             uint32_t n, uint32_t L, Cand* out, uint8_t* ncand_out, uint32_t tid) {
+  uint32_t* hist = reinterpret_cast<uint32_t*>(tabs);  // CNT_BINS counters
+  uint32_t* misc = hist + CNT_BINS;  // [0] chosen, [1] T, [2] ties wanted, [3] nt, [4] T saturated, [5] ties so far, [6..] scans
+  uint32_t* ocnt = misc + 16;        // chosen clusters: count, first key, offset sum
+  uint32_t* ostart = ocnt + TOP_CANDIDATES;
+  long long* osum = reinterpret_cast<long long*>(ostart + TOP_CANDIDATES);
+  const uint32_t lane = (uint32_t)lane_id(), wave = tid >> 6;
+  const uint32_t nw = n / 64 + 1;  // words holding bits 0..n
+  for (uint32_t i = tid; i < CNT_BINS; i += T) hist[i] = 0;
+  if (tid < 16) misc[tid] = 0;
+  // A. the start bitmap: wave w takes positions [base, base + 64) of each round
+  for (uint32_t base = wave * 64; base < nw * 64; base += T) {
+    const uint32_t i = base + lane;
+    const uint64_t k = i < n ? key[i] : 0ull;
+    uint64_t kp = shfl_up_u64(k, 1);
+    if (lane == 0) kp = (i > 0 && i - 1 < n) ? key[i - 1] : 0ull;
+    const bool st = i < n ? (i == 0 || (k >> 32) != (kp >> 32) || (uint32_t)k - (uint32_t)kp > (uint32_t)MERGE_DISTANCE)
+                          : i == n;
+    const uint64_t word = __ballot(st);
+    if (lane == 0) bm[base >> 6] = word;
+  }
+  team_sync<T>();
+  const uint32_t nwd = n >> 6;
+  const uint64_t sent = 1ull << (n & 63u);  // the sentinel bit, in word nwd
+  // starts of clusters of exactly s keys (s <= CNT_REG) in word w, from the starts
+  // m whose size is not known yet; m keeps those of more than s keys
+  auto size_masks = [&](uint32_t w, uint64_t& m, uint64_t* eq) {
+    const uint64_t b = bm[w], nx = w + 1 < nw ? bm[w + 1] : 0ull;
+    m = w == nwd ? b & ~sent : b;
+#pragma unroll
+    for (uint32_t s1 = 1; s1 <= CNT_REG; s1++) {
+      const uint64_t sh = (b >> s1) | (nx << (64 - s1));  // bit p: a start at p + s1
+      eq[s1 - 1] = m & sh;
+      m &= ~sh;
+    }
+  };
+  // B. counts
+  uint32_t tally[CNT_REG];
+#pragma unroll
+  for (uint32_t j = 0; j < CNT_REG; j++) tally[j] = 0;
+  for (uint32_t w = tid; w < nw; w += T) {
+    uint64_t m, eq[CNT_REG];
+    size_masks(w, m, eq);
+#pragma unroll
+    for (uint32_t j = 0; j < CNT_REG; j++) tally[j] += (uint32_t)__popcll(eq[j]);
+    while (m) {
+      const uint32_t p = (w << 6) + (uint32_t)__builtin_ctzll(m);
+      m &= m - 1;
+      atomicAdd(&hist[min(bm_next_start(bm, p) - p, CNT_BINS - 1)], 1u);
+    }
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < CNT_REG; j++) {
+    const uint32_t v = group_sum_u32<64>(tally[j]);
+    if (lane == 0 && v) atomicAdd(&hist[j + 1], v);
+  }
+  team_sync<T>();
+  {
+    constexpr uint32_t PB = CNT_BINS / T;
+    static_assert(CNT_BINS % T == 0, "whole bins per thread");
+    const uint32_t hi = CNT_BINS - 1 - tid * PB;
+    uint32_t local = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < PB; j++) local += hist[hi - j];
+    uint32_t C;
+    uint32_t above = team_excl_scan<T>(local, C, misc + 6);
+    const uint32_t nt = min(C, (uint32_t)TOP_CANDIDATES);
+#pragma unroll
+    for (uint32_t j = 0; j < PB; j++) {
+      const uint32_t h = hist[hi - j];
+      if (above < nt && above + h >= nt) {
+        misc[1] = hi - j;
+        misc[2] = nt - above;
+        misc[4] = hi - j == CNT_BINS - 1 ? 1u : 0u;
+      }
+      above += h;
+    }
+    if (tid == 0) misc[3] = nt;
+  }
+  team_sync<T>();
+  const uint32_t Tc = misc[1], want = misc[2], nt = misc[3];
+  if (misc[4]) return false;
+  // C. the chosen clusters, word by word in order (one word per thread per round)
+  for (uint32_t r0 = 0; r0 < nw; r0 += T) {
+    const uint32_t w = r0 + tid;
+    uint64_t eqT = 0, gtT = 0;
+    if (w < nw) {
+      uint64_t m, eq[CNT_REG];
+      size_masks(w, m, eq);
+      if (Tc <= CNT_REG) {
+        // sizes above T: the starts of sizes T + 1 .. CNT_REG, and the rest in m
+#pragma unroll
+        for (uint32_t j = 0; j < CNT_REG; j++) {
+          if (j + 1 == Tc) eqT = eq[j];
+          if (j + 1 > Tc) gtT |= eq[j];
+        }
+        gtT |= m;
+      } else {
+        uint64_t mm = m;
+        while (mm) {
+          const uint32_t b = (uint32_t)__builtin_ctzll(mm);
+          mm &= mm - 1;
+          const uint32_t p = (w << 6) + b, sz = bm_next_start(bm, p) - p;
+          if (sz == Tc) eqT |= 1ull << b;
+          else if (sz > Tc) gtT |= 1ull << b;
+        }
+      }
+    }
+    uint32_t tot;
+    const uint32_t ex = team_excl_scan<T>((uint32_t)__popcll(eqT), tot, misc + 6) + misc[5];
+    while (gtT) {
+      const uint32_t b = (uint32_t)__builtin_ctzll(gtT);
+      gtT &= gtT - 1;
+      const uint32_t p = (w << 6) + b;
+      const uint32_t at = atomicAdd(&misc[0], 1u);
+      if (at < (uint32_t)TOP_CANDIDATES) {
+        ostart[at] = p;
+        ocnt[at] = bm_next_start(bm, p) - p;
+      }
+    }
+    uint32_t ti = ex;
+    while (eqT && ti < want) {
+      const uint32_t b = (uint32_t)__builtin_ctzll(eqT);
+      eqT &= eqT - 1;
+      const uint32_t at = atomicAdd(&misc[0], 1u);
+      if (at < (uint32_t)TOP_CANDIDATES) {
+        ostart[at] = (w << 6) + b;
+        ocnt[at] = Tc;
+      }
+      ti++;
+    }
+    team_sync<T>();
+    if (tid == 0) misc[5] += tot;
+    team_sync<T>();
+  }
+  // D. offset sums of the chosen clusters, one wave each
+  for (uint32_t e = wave; e < nt; e += T / 64) {
+    const uint32_t p = ostart[e], c = ocnt[e];
+    int64_t sum = 0;
+    for (uint32_t x = lane; x < c; x += 64) sum += (int64_t)(uint32_t)key[p + x];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sum += shfl_xor_i64(sum, o);
+    if (lane == 0) osum[e] = (long long)sum;
+  }
+  team_sync<T>();
+  // E. the candidates, ordered by (count desc, first key)
+  if (tid < nt) {
+    const uint32_t c = ocnt[tid], cs = ostart[tid];
+    uint32_t rk = 0;
+    for (uint32_t j = 0; j < nt; j++) rk += (ocnt[j] > c || (ocnt[j] == c && ostart[j] < cs)) ? 1u : 0u;
+    const CgGroup& g = grp[(uint32_t)(key[cs] >> 32)];
+    const int64_t dsum = (int64_t)osum[tid] + (int64_t)c * ((int64_t)g.lo - DIAG_BIAS);
+    const int64_t avg = dsum / (int64_t)c;
+    const bool rev = g.key & 1u;
+    int64_t position = rev ? avg - (int64_t)L + K : avg;
+    if (position < 0) position = 0;
+    Cand cd;
+    cd.pos = (uint64_t)position;
+    cd.contig = g.key >> 1;
+    cd.strand_score = (rev ? 0x80000000u : 0u) | min(c, 0x7FFFFFFFu);
+    out[rk] = cd;
+  }
+  if (tid == 0) *ncand_out = (uint8_t)nt;
+  team_sync<T>();
+  return true;
+}
+
 __global__ __launch_bounds__(256) void k_cg_select(ClusterArgs a, CgArgs g, const uint32_t* list, uint32_t nlist) {
   __shared__ uint64_t lds[(TOP20_LDS_BYTES + 7) / 8];
+  __shared__ uint64_t bm[BM_WORDS + 1];
   __shared__ CgGroup gs[CG_GMAX + 1];
   const uint32_t tid = threadIdx.x;
   for (uint32_t li = blockIdx.x; li < nlist; li += gridDim.x) {
@@ -2574,8 +2769,11 @@ __global__ __launch_bounds__(256) void k_cg_select(ClusterArgs a, CgArgs g, cons
     if (tid <= G) gs[tid] = cg[tid];
     __syncthreads();
     const uint32_t L = (uint32_t)(a.R.seq_off[r + 1] - a.R.seq_off[r]);
-    if (!top20_ordered<256, true>(g.ck + a.hoff[r], lds, gs, a.nhits[r], L, a.cands + (uint64_t)r * TOP_CANDIDATES,
-                                  a.ncand + r, tid)) {
+    const uint32_t n = a.nhits[r];
+    Cand* out = a.cands + (uint64_t)r * TOP_CANDIDATES;
+    const bool ok = n < 64 * BM_WORDS ? top20_bitmap<256>(g.ck + a.hoff[r], bm, lds, gs, n, L, out, a.ncand + r, tid)
+                                      : top20_ordered<256, true>(g.ck + a.hoff[r], lds, gs, n, L, out, a.ncand + r, tid);
+    if (!ok) {
       // the 20th count lies in the saturated bin (20 clusters of 1023 hits or
       // more): the general tier takes the read from its intact hits
       if (tid == 0) g.fb_list[atomicAdd(g.fb_n, 1u)] = r;
