@@ -199,7 +199,7 @@ struct pbw_ctx {
   DevBuf<Counters> d_ctr;
   DevBuf<int32_t> d_rows;
   DevBuf<uint8_t> d_gstage;  // DP staging in device memory for tiles beyond LDS
-  DevBuf<uint64_t> d_ck, d_cktmp;  // block-tier composite keys and the merges' second buffer
+  DevBuf<uint64_t> d_ck, d_cktmp, d_ck_off;  // block-tier composite keys, the merges' second buffer, regions
   DevBuf<CgGroup> d_cg;
   DevBuf<uint32_t> d_cg_n, d_fb;
   DevBuf<uint64_t> d_tb;
@@ -707,11 +707,11 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
   c->d_list.ensure(B);
   c->d_ovoff.ensure(B);
   c->d_hcap.ensure(B);
-  c->d_seed_st.ensure(8 + 8 * (size_t)seed_overflow_blocks(B));
-  launch_seed_overflow(c->d_nhits.p, B, cap1, (uint64_t)B * cap1, c->d_seed_st.p + 8, c->d_list.p, c->d_ovoff.p,
+  c->d_seed_st.ensure(16 + 12 * (size_t)seed_overflow_blocks(B));
+  launch_seed_overflow(c->d_nhits.p, B, cap1, (uint64_t)B * cap1, c->d_seed_st.p + 16, c->d_list.p, c->d_ovoff.p,
                        c->d_hcap.p, c->d_seed_st.p, s);
   HIP_CHECK(hipGetLastError());
-  uint32_t st[8];
+  uint32_t st[11];
   HIP_CHECK(hipMemcpyAsync(st, c->d_seed_st.p, sizeof(st), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
   {
@@ -722,21 +722,21 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
   }
   const uint32_t n_ov = st[0], max_hits = st[1];
   const uint64_t ov_total = (uint64_t)st[6] | ((uint64_t)st[7] << 32);
+  const uint64_t all_hits = (uint64_t)st[8] | ((uint64_t)st[9] << 32);
   if ((uint64_t)n_ov * 20 > B && cap1 < (uint32_t)HIT_CAP_MAX) {
     // more than 5 % of the reads overflowed: the next batches' first pass takes
     // the least power of two that would have left at most 5 % (within 24 GiB of
-    // hit slots and flags)
-    const uint32_t* over = st + 2;  // reads with more than HIT_CAP << k hits
+    // hit slots and flags: slots spread wider cost the seed kernel more in address
+    // translation than the re-seeding they save)
+    const uint32_t over[5] = {st[2], st[3], st[4], st[5], st[10]};  // reads with more than HIT_CAP << k hits
     uint32_t want = cap1;
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < 5; k++)
       if ((uint64_t)over[k] * 20 > B) want = std::max<uint32_t>(want, (uint32_t)HIT_CAP << (k + 1));
     while (want > cap1 && (uint64_t)B * want * 9 > (24ull << 30)) want >>= 1;
     c->hit_cap = std::min<uint32_t>(want, HIT_CAP_MAX);
   }
   const uint64_t* hits_base = c->d_hits.p;
   const uint8_t* hflag_base = feat ? c->d_hflag.p : nullptr;
-  // hit slots a read's offset can reach: the fixed ones and the overflow slots
-  const size_t arena_n = base_cap + (n_ov && ov_total > slack ? (size_t)ov_total : slack);
   if (n_ov) {
     // overflow slots after the fixed ones: in place when the slack holds them,
     // else in a second arena that starts with a copy of the fixed slots
@@ -812,14 +812,17 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
     if (!feat) {
       // three launches (group / sort / select) for the reads they take; the rest,
       // listed on the device, to k_cluster_tier2 below
-      if (c->d_ck.n < arena_n) {  // with headroom: the arena's size varies from batch to batch
-        c->d_ck.ensure(arena_n + arena_n / 4);
-        c->d_cktmp.ensure(arena_n + arena_n / 4);
+      if (c->d_ck.n < all_hits) {  // with headroom: the count varies from batch to batch
+        c->d_ck.ensure(all_hits + all_hits / 4);
+        c->d_cktmp.ensure(all_hits + all_hits / 4);
       }
       c->d_cg.ensure((size_t)nt2 * (CG_GMAX + 1));
       c->d_cg_n.ensure(nt2);
       c->d_fb.ensure(nt2);
-      CgArgs g{c->d_ck.p, c->d_cktmp.p, c->d_cg.p, c->d_cg_n.p, c->d_fb.p, c->d_ntier2.p + 2};
+      c->d_ck_off.ensure(nt2 + 1);  // [nt2]: the region counter
+      HIP_CHECK(hipMemsetAsync(c->d_ck_off.p + nt2, 0, 8, s));
+      CgArgs g{c->d_ck.p, c->d_cktmp.p, reinterpret_cast<unsigned long long*>(c->d_ck_off.p + nt2), c->d_ck_off.p,
+               c->d_cg.p, c->d_cg_n.p, c->d_fb.p, c->d_ntier2.p + 2};
       launch_cluster_grid(ca, g, c->d_tier2.p, nt2, s);
       HIP_CHECK(hipGetLastError());
       nt2b = download_u32(c, c->d_ntier2.p + 2);

@@ -26,7 +26,7 @@ constexpr int64_t DIAG_BIAS = 1ll << 40;
 // ---- limits of this implementation ----
 constexpr int MAX_READ_LEN = 1024;       // reads up to this are staged in LDS by the seed kernel; longer ones are read from device memory
 constexpr int HIT_CAP = 1024;            // least hits per read in the first seed pass (overflow -> exact rerun)
-constexpr int HIT_CAP_MAX = 16384;       // most: the first-pass capacity adapts between the two
+constexpr int HIT_CAP_MAX = 32768;       // most: the first-pass capacity adapts between the two
 constexpr int CLUSTER_LDS_CAP = 512;     // hits per read clustered in LDS by one wave (else tier 2)
 constexpr int LN_INT_MAX = 4096;         // ln(k) table for phred_n_choose_m
 constexpr int GC_LUT_MAX = 1024;         // (gc, total) table for the GC-adjusted alphabet size
@@ -159,7 +159,7 @@ void launch_cluster(const ClusterArgs& a, hipStream_t s);
 // Block tier in three launches (k_cg_group, k_cg_sort, k_cg_select) for reads of
 // at most CG_GMAX groups whose diagonals each span less than 2^32 - 1: the hits as
 // composite keys in `ck` (group rank << 32 | biased diagonal - the group's lowest,
-// at the read's hit offset), the groups in `cg` (CG_GMAX + 1 per list entry), the
+// in a region of the read's hit count), the groups in `cg` (CG_GMAX + 1 per list entry), the
 // group count in cg_n (0: the read is appended to fb_list / fb_n for
 // k_cluster_tier2, as is a read whose 20th cluster count the selection cannot
 // resolve); `tmp` (as large as `ck`) is the merges' second buffer.  The hits
@@ -171,8 +171,10 @@ struct CgGroup {
   uint32_t start;         // segment start within the read's keys ([G]: the hit count)
 };
 struct CgArgs {
-  uint64_t* ck;
+  uint64_t* ck;            // composite keys: each read's region taken from *ck_next by k_cg_group
   uint64_t* tmp;
+  unsigned long long* ck_next;
+  uint64_t* ck_off;        // per list entry: the read's region in ck / tmp
   CgGroup* cg;
   uint32_t* cg_n;
   uint32_t* fb_list;
@@ -352,10 +354,15 @@ void gather_cigar(const uint32_t* cig, const uint64_t* src, const uint32_t* off,
 void launch_fill_pres(const uint32_t* koff, uint64_t* pres, uint32_t rot, hipStream_t s);
 // out[i] = i * stride for i < n
 void launch_fill_stride(uint64_t* out, uint32_t n, uint64_t stride, hipStream_t s);
-// out[idx[k]] = val[k] for k < n
-uint32_t seed_overflow_blocks(uint32_t B);  // 8 words of scratch each
+// Reads whose hits passed their first-pass slot (cap1), listed in read order
+// with exact slots from `base` on.  part: seed_overflow_blocks(B) * 12 words of
+// scratch; st (11 words): [0] listed reads, [1] most hits of a read, [2..5] reads
+// over HIT_CAP << k hits (k = 0..3), [6..7] the listed reads' hits, [8..9] every
+// read's hits (64-bit values low word first), [10] reads over HIT_CAP << 4.
+uint32_t seed_overflow_blocks(uint32_t B);
 void launch_seed_overflow(const uint32_t* nhits, uint32_t B, uint32_t cap1, uint64_t base, uint32_t* part,
                           uint32_t* list, uint64_t* off, uint32_t* cap, uint32_t* st, hipStream_t s);
+// out[idx[k]] = val[k] for k < n
 void launch_scatter_u64(uint64_t* out, const uint32_t* idx, const uint64_t* val, uint32_t n, hipStream_t s);
 // Bucket table on the device (pbw_index.hip): fills koff[0 .. 2^30] and allocates
 // *kpos (hipMalloc, owned by the caller) with *n_entries packed positions.

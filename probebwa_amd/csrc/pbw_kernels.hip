@@ -2448,8 +2448,13 @@ __global__ __launch_bounds__(256) void k_cg_group(ClusterArgs a, CgArgs g, const
       __syncthreads();
       continue;
     }
-    if (tid == 0) g.cg_n[li] = G;
-    uint64_t* ck = g.ck + a.hoff[r];
+    if (tid == 0) {
+      g.cg_n[li] = G;
+      sk[0] = atomicAdd(g.ck_next, (unsigned long long)n);  // this read's region in ck and tmp
+      g.ck_off[li] = sk[0];
+    }
+    __syncthreads();
+    uint64_t* ck = g.ck + sk[0];
     constexpr uint32_t U = 8;
     for (uint32_t i0 = tid; i0 < n; i0 += U * 256) {
       uint64_t kb[U];
@@ -2529,8 +2534,8 @@ __global__ __launch_bounds__(256) void k_cg_sort(ClusterArgs a, CgArgs g, const 
     const CgGroup* cg = g.cg + (uint64_t)li * (CG_GMAX + 1);
     if (tid <= G) st[tid] = cg[tid].start;
     __syncthreads();
-    uint64_t* key = g.ck + a.hoff[r];
-    uint64_t* tmp = g.tmp + a.hoff[r];
+    uint64_t* key = g.ck + g.ck_off[li];
+    uint64_t* tmp = g.tmp + g.ck_off[li];
     uint32_t q = 0;
     while (q < G) {
       const uint32_t a0 = st[q];
@@ -2771,8 +2776,9 @@ __global__ __launch_bounds__(256) void k_cg_select(ClusterArgs a, CgArgs g, cons
     const uint32_t L = (uint32_t)(a.R.seq_off[r + 1] - a.R.seq_off[r]);
     const uint32_t n = a.nhits[r];
     Cand* out = a.cands + (uint64_t)r * TOP_CANDIDATES;
-    const bool ok = n < 64 * BM_WORDS ? top20_bitmap<256>(g.ck + a.hoff[r], bm, lds, gs, n, L, out, a.ncand + r, tid)
-                                      : top20_ordered<256, true>(g.ck + a.hoff[r], lds, gs, n, L, out, a.ncand + r, tid);
+    const uint64_t* key = g.ck + g.ck_off[li];
+    const bool ok = n < 64 * BM_WORDS ? top20_bitmap<256>(key, bm, lds, gs, n, L, out, a.ncand + r, tid)
+                                      : top20_ordered<256, true>(key, lds, gs, n, L, out, a.ncand + r, tid);
     if (!ok) {
       // the 20th count lies in the saturated bin (20 clusters of 1023 hits or
       // more): the general tier takes the read from its intact hits
@@ -5722,8 +5728,9 @@ __global__ void k_scatter_u64(uint64_t* out, const uint32_t* idx, const uint64_t
 // them, and the list itself in read order (entry k: read, exact hit count,
 // and slots base + the hits of the overflow reads before it).  st: [0]
 // overflow reads, [1] most hits of any read, [2..5] reads with more than
-// HIT_CAP << k hits, [6..7] total overflow hits (64 bits).
-constexpr uint32_t OV_BLOCK = 1024;
+// HIT_CAP << k hits (k = 0..3), [6..7] total overflow hits, [8..9] every read's
+// hits (64 bits), [10] reads with more than HIT_CAP << 4 hits.
+constexpr uint32_t OV_BLOCK = 1024, OV_PART = 12;
 // Block-wide exclusive sum of x (OV_BLOCK threads); *total gets the sum.
 __device__ __forceinline__ uint32_t ov_block_scan(uint32_t x, uint32_t* sh, uint32_t& total) {
   const uint32_t w = threadIdx.x >> 6, l = (uint32_t)lane_id();
@@ -5740,9 +5747,9 @@ __device__ __forceinline__ uint32_t ov_block_scan(uint32_t x, uint32_t* sh, uint
   total = tot;
   return before + incl - x;
 }
-// pass 1: per block [count, max, over0..3, hits lo, hits hi] (the hits of a
-// block's overflow reads in 64 bits: the reads' low and high 16-bit halves
-// summed apart, each sum < 2^26)
+// pass 1: per block (OV_PART words) [count, max, over0..3, overflow hits lo, hi,
+// all hits lo, hi, over4] (hit sums in 64 bits: the reads' low and high 16-bit
+// halves summed apart, each sum < 2^26)
 __global__ __launch_bounds__(OV_BLOCK) void k_ov_count(const uint32_t* nhits, uint32_t B, uint32_t cap1, uint32_t* part) {
   __shared__ uint32_t sh[OV_BLOCK / 64];
   __shared__ uint32_t smax[OV_BLOCK / 64];
@@ -5753,17 +5760,19 @@ __global__ __launch_bounds__(OV_BLOCK) void k_ov_count(const uint32_t* nhits, ui
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, o, 64));
   if (lane_id() == 0) smax[threadIdx.x >> 6] = mx;
-  uint32_t c[7] = {ov ? 1u : 0u, ov ? (nh & 0xFFFFu) : 0u, ov ? (nh >> 16) : 0u, 0, 0, 0, 0};
+  uint32_t c[10] = {ov ? 1u : 0u, ov ? (nh & 0xFFFFu) : 0u, ov ? (nh >> 16) : 0u, 0, 0, 0, 0, nh & 0xFFFFu, nh >> 16,
+                    (ov && nh > ((uint32_t)HIT_CAP << 4)) ? 1u : 0u};
 #pragma unroll
   for (int j = 0; j < 4; j++) c[3 + j] = (ov && nh > ((uint32_t)HIT_CAP << j)) ? 1u : 0u;
-  uint32_t tot[7];
+  uint32_t tot[10];
 #pragma unroll
-  for (int j = 0; j < 7; j++) (void)ov_block_scan(c[j], sh, tot[j]);
+  for (int j = 0; j < 10; j++) (void)ov_block_scan(c[j], sh, tot[j]);
   if (threadIdx.x == 0) {
     uint32_t m = 0;
     for (uint32_t v = 0; v < OV_BLOCK / 64; v++) m = max(m, smax[v]);
-    uint32_t* p = part + (uint64_t)blockIdx.x * 8;
+    uint32_t* p = part + (uint64_t)blockIdx.x * OV_PART;
     const uint64_t hits = (uint64_t)tot[1] + ((uint64_t)tot[2] << 16);
+    const uint64_t all = (uint64_t)tot[7] + ((uint64_t)tot[8] << 16);
     p[0] = tot[0];
     p[1] = m;
     p[2] = tot[3];
@@ -5772,22 +5781,25 @@ __global__ __launch_bounds__(OV_BLOCK) void k_ov_count(const uint32_t* nhits, ui
     p[5] = tot[6];
     p[6] = (uint32_t)hits;
     p[7] = (uint32_t)(hits >> 32);
+    p[8] = (uint32_t)all;
+    p[9] = (uint32_t)(all >> 32);
+    p[10] = tot[9];
   }
 }
 // pass 2 (one block): block bases (count, hits) in place of the counts, and st
 __global__ __launch_bounds__(OV_BLOCK) void k_ov_scan(uint32_t* part, uint32_t nb, uint32_t* st) {
   __shared__ uint32_t sh[OV_BLOCK / 64];
-  __shared__ unsigned long long carry_h;
-  __shared__ uint32_t carry_c, cmax, cover[4];
+  __shared__ unsigned long long carry_h, carry_all;
+  __shared__ uint32_t carry_c, cmax, cover[5];
   if (threadIdx.x == 0) {
-    carry_h = 0;
+    carry_h = carry_all = 0;
     carry_c = cmax = 0;
-    for (int j = 0; j < 4; j++) cover[j] = 0;
+    for (int j = 0; j < 5; j++) cover[j] = 0;
   }
   __syncthreads();
   for (uint32_t b0 = 0; b0 < nb; b0 += OV_BLOCK) {
     const uint32_t bi = b0 + threadIdx.x;
-    uint32_t* p = part + (uint64_t)bi * 8;
+    uint32_t* p = part + (uint64_t)bi * OV_PART;
     const bool v = bi < nb;
     const uint32_t cnt = v ? p[0] : 0u;
     const uint64_t hits = v ? ((uint64_t)p[6] | ((uint64_t)p[7] << 32)) : 0ull;  // < 2^42
@@ -5799,9 +5811,17 @@ __global__ __launch_bounds__(OV_BLOCK) void k_ov_scan(uint32_t* part, uint32_t n
     const uint32_t ev = ov_block_scan((uint32_t)(hits >> 32), sh, tv);
     const uint64_t eh = (uint64_t)el + ((uint64_t)eu << 16) + ((uint64_t)ev << 32);
     const uint64_t th = (uint64_t)tl + ((uint64_t)tu << 16) + ((uint64_t)tv << 32);
+    // every read's hits: only the total
+    const uint64_t all = v ? ((uint64_t)p[8] | ((uint64_t)p[9] << 32)) : 0ull;
+    uint32_t al, au, av;
+    (void)ov_block_scan((uint32_t)all & 0xFFFFu, sh, al);
+    (void)ov_block_scan((uint32_t)(all >> 16) & 0xFFFFu, sh, au);
+    (void)ov_block_scan((uint32_t)(all >> 32), sh, av);
+    const uint64_t ta = (uint64_t)al + ((uint64_t)au << 16) + ((uint64_t)av << 32);
     const uint32_t m = v ? p[1] : 0u;
-    uint32_t o[4];
+    uint32_t o[5];
     for (int j = 0; j < 4; j++) o[j] = v ? p[2 + j] : 0u;
+    o[4] = v ? p[10] : 0u;
     const unsigned long long hb = carry_h;
     const uint32_t cb = carry_c;
     if (v) {
@@ -5812,14 +5832,15 @@ __global__ __launch_bounds__(OV_BLOCK) void k_ov_scan(uint32_t* part, uint32_t n
     uint32_t mm = m;
 #pragma unroll
     for (int q = 32; q > 0; q >>= 1) mm = max(mm, (uint32_t)__shfl_xor((int)mm, q, 64));
-    uint32_t to[4];
-    for (int j = 0; j < 4; j++) (void)ov_block_scan(o[j], sh, to[j]);
+    uint32_t to[5];
+    for (int j = 0; j < 5; j++) (void)ov_block_scan(o[j], sh, to[j]);
     if (lane_id() == 0) atomicMax(&cmax, mm);
     __syncthreads();
     if (threadIdx.x == 0) {
       carry_c = cb + tc;
       carry_h = hb + th;
-      for (int j = 0; j < 4; j++) cover[j] += to[j];
+      carry_all += ta;
+      for (int j = 0; j < 5; j++) cover[j] += to[j];
     }
     __syncthreads();
   }
@@ -5829,6 +5850,9 @@ __global__ __launch_bounds__(OV_BLOCK) void k_ov_scan(uint32_t* part, uint32_t n
     for (int j = 0; j < 4; j++) st[2 + j] = cover[j];
     st[6] = (uint32_t)carry_h;
     st[7] = (uint32_t)(carry_h >> 32);
+    st[8] = (uint32_t)carry_all;
+    st[9] = (uint32_t)(carry_all >> 32);
+    st[10] = cover[4];
   }
 }
 // pass 3: the list, in read order
@@ -5846,7 +5870,7 @@ __global__ __launch_bounds__(OV_BLOCK) void k_ov_write(const uint32_t* nhits, ui
   const uint32_t eu = ov_block_scan(ov ? (nh >> 16) : 0u, sh, t2);
   const uint64_t eh = (uint64_t)el + ((uint64_t)eu << 16);
   if (!ov) return;
-  const uint32_t* p = part + (uint64_t)blockIdx.x * 8;
+  const uint32_t* p = part + (uint64_t)blockIdx.x * OV_PART;
   const uint32_t k = p[0] + ec;
   const uint64_t hb = (uint64_t)p[6] | ((uint64_t)p[7] << 32);
   list[k] = i;
