@@ -205,6 +205,9 @@ typedef struct {
                            register in saturating 16-bit halves, when the gap penalties allow
                            (gap extend >= -4, gap open >= -400); jobs whose best score is below
                            -18000 are re-run in 32 bits */
+  uint32_t seed_pool;   /* the seed pass's pool of 1024-hit chunks per batch; 0: two per read to
+                           start with, then grown to what the batches' hits take.  Reads whose
+                           hits find no chunk are seeded again into exact slots */
 } pbw_tune;
 int pbw_tune_get(const pbw_ctx* ctx, pbw_tune* out);
 int pbw_tune_set(pbw_ctx* ctx, const pbw_tune* t);

@@ -101,7 +101,7 @@ class Tune(C.Structure):
     """Launch shapes of a mapping call (`pbw_tune`): they change its speed, never its records."""
 
     _fields_ = [("batch_reads", C.c_uint32), ("dp_grid", C.c_uint32), ("split_len", C.c_uint32),
-                ("long_waves", C.c_uint32), ("dp_pack16", C.c_uint32)]
+                ("long_waves", C.c_uint32), ("dp_pack16", C.c_uint32), ("seed_pool", C.c_uint32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -187,7 +187,11 @@ struct pbw_ctx {
   // batch workspace
   DevBuf<uint8_t> d_seq, d_qual, d_rcode;
   DevBuf<uint64_t> d_seq_off, d_qual_off, d_hoff, d_hits, d_hits_ov, d_scratch, d_ovoff;
-  uint32_t hit_cap = HIT_CAP;  // first-pass hits per read, adapted batch to batch
+  uint32_t pool_chunks = 0;  // the seed pass's 1024-hit chunks, grown batch to batch to what the hits took
+  DevBuf<uint32_t> d_chunk_tab, d_pool_next;
+  DevBuf<uint8_t> d_dropped;
+  DevBuf<uint64_t> d_hits_g, d_hoff_g;  // contiguous copies of reads' hits for k_cluster_tier2
+  DevBuf<uint8_t> d_hflag_g;
   DevBuf<uint32_t> d_seed_st;  // the seed pass's overflow counts
   DevBuf<uint32_t> d_nhits, d_list, d_hcap, d_tier2, d_mid, d_ntier2, d_best, d_cigar, d_cigar_out, d_cigar_off;
   DevBuf<Cand> d_cands;
@@ -667,13 +671,18 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
   const DevReads R = b.R;
   auto t0 = std::chrono::steady_clock::now();
   // ---- seed ----
-  // fixed slots of cap1 hits per read, then slack for the overflow pass's exact-size slots
-  const uint32_t cap1 = c->hit_cap;
+  // hits in 1024-hit chunks taken from a pool as they come (a read's chunk row in
+  // d_chunk_tab), then slack for the exact slots of the reads a full pool left out
+  const uint32_t pool0 = c->tune.seed_pool ? c->tune.seed_pool : (c->pool_chunks ? c->pool_chunks : 2 * B);
+  const uint32_t pool = (pool0 + SEED_SUBPOOLS - 1) / SEED_SUBPOOLS * SEED_SUBPOOLS;  // whole sub-pools
   c->d_hoff.ensure(B);
-  launch_fill_stride(c->d_hoff.p, B, (uint64_t)cap1, s);
-  const size_t base_cap = (size_t)B * cap1, slack = (size_t)B * 64;
+  const size_t base_cap = (size_t)pool << SEED_CH_BITS, slack = (size_t)B * 64;
   c->d_hits.ensure(base_cap + slack);
   c->d_nhits.ensure(B);
+  c->d_chunk_tab.ensure((size_t)B * SEED_MAXCH);
+  c->d_dropped.ensure(B);
+  c->d_pool_next.ensure(16 * SEED_SUBPOOLS);
+  HIP_CHECK(hipMemsetAsync(c->d_pool_next.p, 0, 64 * SEED_SUBPOOLS, s));
   SeedArgs sa{};
   sa.G = c->dev_genome();
   sa.X = DevIndex{c->d_koff.p, c->d_kpos.p, c->d_pres.p, c->d_pres.p + CODE_SPACE / 64,
@@ -683,7 +692,11 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
   sa.nlist = B;
   sa.hoff = c->d_hoff.p;
   sa.hcap = nullptr;
-  sa.hit_cap = cap1;
+  sa.hit_cap = 0;
+  sa.chunk_tab = c->d_chunk_tab.p;
+  sa.pool_next = c->d_pool_next.p;
+  sa.pool_chunks = pool;
+  sa.dropped = c->d_dropped.p;
   sa.hits = c->d_hits.p;
   sa.nhits = c->d_nhits.p;
   sa.rcode = c->d_rcode.p;
@@ -708,8 +721,8 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
   c->d_ovoff.ensure(B);
   c->d_hcap.ensure(B);
   c->d_seed_st.ensure(16 + 12 * (size_t)seed_overflow_blocks(B));
-  launch_seed_overflow(c->d_nhits.p, B, cap1, (uint64_t)B * cap1, c->d_seed_st.p + 16, c->d_list.p, c->d_ovoff.p,
-                       c->d_hcap.p, c->d_seed_st.p, s);
+  launch_seed_overflow(c->d_nhits.p, c->d_dropped.p, B, 0, (uint64_t)base_cap, c->d_seed_st.p + 16, c->d_list.p,
+                       c->d_ovoff.p, c->d_hcap.p, c->d_seed_st.p, s);
   HIP_CHECK(hipGetLastError());
   uint32_t st[11];
   HIP_CHECK(hipMemcpyAsync(st, c->d_seed_st.p, sizeof(st), hipMemcpyDeviceToHost, s));
@@ -723,17 +736,14 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
   const uint32_t n_ov = st[0], max_hits = st[1];
   const uint64_t ov_total = (uint64_t)st[6] | ((uint64_t)st[7] << 32);
   const uint64_t all_hits = (uint64_t)st[8] | ((uint64_t)st[9] << 32);
-  if ((uint64_t)n_ov * 20 > B && cap1 < (uint32_t)HIT_CAP_MAX) {
-    // more than 5 % of the reads overflowed: the next batches' first pass takes
-    // the least power of two that would have left at most 5 % (within 24 GiB of
-    // hit slots and flags: slots spread wider cost the seed kernel more in address
-    // translation than the re-seeding they save)
-    const uint32_t over[5] = {st[2], st[3], st[4], st[5], st[10]};  // reads with more than HIT_CAP << k hits
-    uint32_t want = cap1;
-    for (int k = 0; k < 5; k++)
-      if ((uint64_t)over[k] * 20 > B) want = std::max<uint32_t>(want, (uint32_t)HIT_CAP << (k + 1));
-    while (want > cap1 && (uint64_t)B * want * 9 > (24ull << 30)) want >>= 1;
-    c->hit_cap = std::min<uint32_t>(want, HIT_CAP_MAX);
+  {
+    // the next batches' pool: the chunks this batch's hits needed (at most one
+    // partial chunk per read) and an eighth more, within 64 GiB of hits and flags
+    const uint64_t need = (all_hits >> SEED_CH_BITS) + B;
+    if (need > pool && !c->tune.seed_pool) {
+      const uint64_t want = std::min<uint64_t>(need + need / 8, (64ull << 30) / (9ull << SEED_CH_BITS));
+      c->pool_chunks = (uint32_t)std::max<uint64_t>(pool, want);
+    }
   }
   const uint64_t* hits_base = c->d_hits.p;
   const uint8_t* hflag_base = feat ? c->d_hflag.p : nullptr;
@@ -757,6 +767,9 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
     sb.nlist = n_ov;
     sb.hoff = c->d_ovoff.p;
     sb.hcap = c->d_hcap.p;
+    sb.chunk_tab = nullptr;  // exact slots; the reads' chunk rows marked contiguous
+    sb.dropped = nullptr;
+    sb.mark_tab = c->d_chunk_tab.p;
     sb.hits = arena;
     sb.ctr = nullptr;  // counted once, in the first pass
     sb.hflag = farena;
@@ -783,6 +796,7 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
   ca.R = R;
   ca.hits = hits_base;
   ca.hoff = c->d_hoff.p;
+  ca.chunk_tab = c->d_chunk_tab.p;
   ca.nhits = c->d_nhits.p;
   ca.cands = c->d_cands.p;
   ca.ncand = c->d_ncand.p;
@@ -829,14 +843,33 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
       t2list = c->d_fb.p;
     }
     if (nt2b) {
+      // the general block tier takes each read's hits contiguously: copies of the
+      // listed reads' (chunked) hits first
+      ClusterArgs cb = ca;
+      if (c->d_hits_g.n < all_hits) {
+        c->d_hits_g.ensure(all_hits + all_hits / 4);
+        if (feat) c->d_hflag_g.ensure(all_hits + all_hits / 4);
+      } else if (feat && c->d_hflag_g.n < all_hits) {
+        c->d_hflag_g.ensure(all_hits + all_hits / 4);
+      }
+      c->d_hoff_g.ensure((size_t)B + 1);  // [B]: the region counter
+      HIP_CHECK(hipMemsetAsync(c->d_hoff_g.p + B, 0, 8, s));
+      launch_hits_gather(hits_base, hflag_base, c->d_hoff.p, c->d_chunk_tab.p, c->d_nhits.p, t2list, nt2b,
+                         c->d_hits_g.p, feat ? c->d_hflag_g.p : nullptr, c->d_hoff_g.p,
+                         reinterpret_cast<unsigned long long*>(c->d_hoff_g.p + B), s);
+      HIP_CHECK(hipGetLastError());
+      cb.hits = c->d_hits_g.p;
+      cb.hoff = c->d_hoff_g.p;
+      cb.chunk_tab = nullptr;
+      if (feat) cb.hflag = c->d_hflag_g.p;
       // one 256-thread block per read, per-block scratch sized for the largest read; up to
       // 4096 blocks within 16 GiB (repeat-rich genomes send most reads here, with 10^4 hits each)
-      ca.scratch_per_block = cluster_scratch_words(max_hits);
-      const uint64_t fit = std::max<uint64_t>(1, (1ull << 31) / ca.scratch_per_block);
+      cb.scratch_per_block = cluster_scratch_words(max_hits);
+      const uint64_t fit = std::max<uint64_t>(1, (1ull << 31) / cb.scratch_per_block);
       const uint32_t blocks = (uint32_t)std::min<uint64_t>({(uint64_t)nt2b, 4096, fit});
-      c->d_scratch.ensure((size_t)blocks * ca.scratch_per_block);
-      ca.scratch = c->d_scratch.p;
-      launch_cluster_tier2(ca, t2list, nt2b, blocks, s);
+      c->d_scratch.ensure((size_t)blocks * cb.scratch_per_block);
+      cb.scratch = c->d_scratch.p;
+      launch_cluster_tier2(cb, t2list, nt2b, blocks, s);
       HIP_CHECK(hipGetLastError());
     }
     HIP_CHECK(hipEventRecord(c->ev[7], s));

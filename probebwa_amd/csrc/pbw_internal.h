@@ -124,7 +124,7 @@ struct SeedArgs {
   DevReads R;
   const uint32_t* rlist;   // reads to process (nullptr = 0..n-1)
   uint32_t nlist;
-  const uint64_t* hoff;    // per list entry: hit buffer offset
+  uint64_t* hoff;          // per list entry: hit buffer offset (written per read in the chunked first pass)
   const uint32_t* hcap;    // per list entry: capacity (nullptr = hit_cap)
   uint32_t hit_cap;        // capacity of every read when hcap is null
   uint64_t* hits;
@@ -133,13 +133,32 @@ struct SeedArgs {
   Counters* ctr;
   uint8_t* hflag;          // optional, parallel to hits: bit 7 neighbour probe, bits 0..3 bucket-size class
   uint32_t has_long;       // some read is longer than MAX_READ_LEN (seeded from device memory by a second kernel)
+  // first pass in chunks (chunk_tab set): read r's hit i at chunk_tab[r * SEED_MAXCH + i / 1024] * 1024 + i % 1024,
+  // chunks taken from sub-pool r % SEED_SUBPOOLS (pool_chunks / SEED_SUBPOOLS each, counter pool_next[16 k]);
+  // hoff[r] = chunk 0's first slot; dropped[r] = 1 when a chunk could not be had (re-seeded into an exact slot)
+  uint32_t* chunk_tab;
+  uint32_t* pool_next;
+  uint32_t pool_chunks;
+  uint8_t* dropped;
+  uint32_t* mark_tab;      // exact-slot passes: marks the read's chunk row SEED_CONTIGUOUS (hits contiguous at hoff)
 };
+constexpr uint32_t SEED_CH_BITS = 10, SEED_MAXCH = 128;  // 1024-hit chunks, at most 131072 hits a read
+constexpr uint32_t SEED_SUBPOOLS = 64;  // chunk counters: one atomic address per 64th of the reads
+constexpr uint32_t SEED_CONTIGUOUS = 0xFFFFFFFFu;
+// Read r's hit i in the hits array under a chunk table (or contiguous from hoff[r]).
+__host__ __device__ inline uint64_t hit_at(const uint32_t* tab, const uint64_t* hoff, uint32_t r, uint32_t i) {
+  if (!tab) return hoff[r] + i;
+  const uint32_t* row = tab + (uint64_t)r * SEED_MAXCH;
+  if (row[0] == SEED_CONTIGUOUS) return hoff[r] + i;
+  return ((uint64_t)row[i >> SEED_CH_BITS] << SEED_CH_BITS) + (i & ((1u << SEED_CH_BITS) - 1));
+}
 void launch_seed(const SeedArgs& a, hipStream_t s);
 
 struct ClusterArgs {
   DevReads R;
   const uint64_t* hits;
   const uint64_t* hoff;    // per read hit offset
+  const uint32_t* chunk_tab;  // when set: the seed pass's chunk rows (hit_at); reads over 512 hits may be chunked
   const uint32_t* nhits;
   Cand* cands;             // read * 20
   uint8_t* ncand;
@@ -354,14 +373,20 @@ void gather_cigar(const uint32_t* cig, const uint64_t* src, const uint32_t* off,
 void launch_fill_pres(const uint32_t* koff, uint64_t* pres, uint32_t rot, hipStream_t s);
 // out[i] = i * stride for i < n
 void launch_fill_stride(uint64_t* out, uint32_t n, uint64_t stride, hipStream_t s);
-// Reads whose hits passed their first-pass slot (cap1), listed in read order
-// with exact slots from `base` on.  part: seed_overflow_blocks(B) * 12 words of
+// Reads whose hits passed their first-pass slot (cap1), or with `dropped` the
+// reads flagged there, listed in read order with exact slots from `base` on.
+// part: seed_overflow_blocks(B) * 12 words of
 // scratch; st (11 words): [0] listed reads, [1] most hits of a read, [2..5] reads
 // over HIT_CAP << k hits (k = 0..3), [6..7] the listed reads' hits, [8..9] every
 // read's hits (64-bit values low word first), [10] reads over HIT_CAP << 4.
 uint32_t seed_overflow_blocks(uint32_t B);
-void launch_seed_overflow(const uint32_t* nhits, uint32_t B, uint32_t cap1, uint64_t base, uint32_t* part,
-                          uint32_t* list, uint64_t* off, uint32_t* cap, uint32_t* st, hipStream_t s);
+void launch_seed_overflow(const uint32_t* nhits, const uint8_t* dropped, uint32_t B, uint32_t cap1, uint64_t base,
+                          uint32_t* part, uint32_t* list, uint64_t* off, uint32_t* cap, uint32_t* st, hipStream_t s);
+// Hits (and flags when hflag is set) of the listed reads copied contiguously:
+// read r's region taken from *next, its offset to off[r].
+void launch_hits_gather(const uint64_t* hits, const uint8_t* hflag, const uint64_t* hoff, const uint32_t* tab,
+                        const uint32_t* nhits, const uint32_t* list, uint32_t nlist, uint64_t* dst, uint8_t* dflag,
+                        uint64_t* off, unsigned long long* next, hipStream_t s);
 // out[idx[k]] = val[k] for k < n
 void launch_scatter_u64(uint64_t* out, const uint32_t* idx, const uint64_t* val, uint32_t n, hipStream_t s);
 // Bucket table on the device (pbw_index.hip): fills koff[0 .. 2^30] and allocates

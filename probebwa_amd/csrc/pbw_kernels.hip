@@ -291,11 +291,22 @@ __device__ __forceinline__ uint32_t select_set_bit(uint64_t m, uint32_t k) {
   return pos;
 }
 
+// Where a read's hits go: a slot of `cap` hits from hbase on (fixed or exact
+// slots), or, with a chunk table, 1024-hit chunks taken from a pool as the hits
+// come (nch chunks held so far, `last` the id of the newest; `dropped` once a
+// chunk could not be had: the hits are then only counted).
+struct HitSlots {
+  uint64_t hbase;
+  uint32_t cap;
+  uint32_t nch, last, first;
+  bool dropped;
+};
+
 // One round of the 64 queued positions from queue position qc on: flank
 // filter, candidate key and compaction of the passing hits in emission order
 // (seeds.rs:120-147, candidates.rs:37-40).
 __device__ __forceinline__ void seed_positions(const SeedArgs& a, const SeedLds& S, uint32_t qn, uint32_t qc,
-                                               uint32_t qtot, uint64_t hbase, uint32_t cap, uint32_t& hcount,
+                                               uint32_t qtot, uint32_t r, HitSlots& hs, uint32_t& hcount,
                                                unsigned long long& c_filt) {
   const uint32_t e = qc + lane_id();
   bool pass = false;
@@ -324,11 +335,40 @@ __device__ __forceinline__ void seed_positions(const SeedArgs& a, const SeedLds&
   }
   const uint64_t m = __ballot(pass);
   const uint32_t pre = (uint32_t)__popcll(m & lanemask_lt());
-  if (pass && hcount + pre < cap) {
-    a.hits[hbase + hcount + pre] = key;
-    if (a.hflag) a.hflag[hbase + hcount + pre] = (uint8_t)hfl;
+  const uint32_t np = (uint32_t)__popcll(m);
+  if (a.chunk_tab) {
+    if (np && !hs.dropped) {
+      // chunks up to the one holding hit hcount + np - 1 (one or two new at most)
+      const uint32_t j1 = (hcount + np - 1) >> SEED_CH_BITS, nch0 = hs.nch, last0 = hs.last;
+      uint32_t got = 0;
+      if (j1 >= nch0) {
+        // from the read's sub-pool (SEED_SUBPOOLS counters on lines of their own)
+        const uint32_t need = j1 + 1 - nch0, sub = r % SEED_SUBPOOLS, per = a.pool_chunks / SEED_SUBPOOLS;
+        if (lane_id() == 0 && j1 < (uint32_t)SEED_MAXCH) got = atomicAdd(a.pool_next + sub * 16, need);
+        got = (uint32_t)__builtin_amdgcn_readfirstlane((int)got);
+        if (j1 >= (uint32_t)SEED_MAXCH || got + need > per) {
+          hs.dropped = true;
+        } else {
+          got += sub * per;
+          if ((uint32_t)lane_id() < need) a.chunk_tab[(uint64_t)r * SEED_MAXCH + nch0 + lane_id()] = got + lane_id();
+          if (nch0 == 0) hs.first = got;
+          hs.nch = j1 + 1;
+          hs.last = got + need - 1;
+        }
+      }
+      if (pass && !hs.dropped) {
+        const uint32_t h = hcount + pre, j = h >> SEED_CH_BITS;
+        const uint32_t id = j < nch0 ? last0 : got + (j - nch0);
+        const uint64_t at = ((uint64_t)id << SEED_CH_BITS) + (h & ((1u << SEED_CH_BITS) - 1));
+        a.hits[at] = key;
+        if (a.hflag) a.hflag[at] = (uint8_t)hfl;
+      }
+    }
+  } else if (pass && hcount + pre < hs.cap) {
+    a.hits[hs.hbase + hcount + pre] = key;
+    if (a.hflag) a.hflag[hs.hbase + hcount + pre] = (uint8_t)hfl;
   }
-  hcount += (uint32_t)__popcll(m);
+  hcount += np;
   c_filt += pass ? 1 : 0;
 }
 
@@ -336,7 +376,7 @@ __device__ __forceinline__ void seed_positions(const SeedArgs& a, const SeedLds&
 // true: the longer ones, read from device memory (the two launches split a
 // list between them).
 template <bool LONG>
-__global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_seed(SeedArgs a) {
   __shared__ SeedLds lds_all[SEED_WPB];
   const int wib = threadIdx.x >> 6, lane = lane_id();
   SeedLds& S = lds_all[wib];
@@ -349,13 +389,24 @@ __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
     const uint64_t qo = a.R.qual_off[r];
     const uint32_t qlen = (uint32_t)(a.R.qual_off[r + 1] - qo);
     (void)qlen;
-    const uint64_t hbase = a.hoff[li];
-    const uint32_t cap = a.hcap ? a.hcap[li] : a.hit_cap;
+    HitSlots hs{};
+    if (!a.chunk_tab) {
+      hs.hbase = a.hoff[li];
+      hs.cap = a.hcap ? a.hcap[li] : a.hit_cap;
+    }
     uint32_t hcount = 0;
     uint32_t qn = 0, qtot = 0, qc = 0;  // queued buckets, their positions, positions consumed
     if (LONG != (L > (uint32_t)MAX_READ_LEN)) continue;  // the other launch's read
     if (L < (uint32_t)K) {
-      if (lane == 0) a.nhits[r] = 0;
+      if (lane == 0) {
+        a.nhits[r] = 0;
+        if (a.chunk_tab) {
+          a.hoff[r] = 0;
+          a.dropped[r] = 0;
+        } else if (a.mark_tab) {
+          a.mark_tab[(uint64_t)r * SEED_MAXCH] = SEED_CONTIGUOUS;
+        }
+      }
       continue;
     }
     const uint8_t* rs = a.R.seq + so;
@@ -471,7 +522,7 @@ __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
         wave_sync();
         if (qtot - qc >= 64) {
           do {
-            seed_positions(a, S, qn, qc, qtot, hbase, cap, hcount, c_filt);
+            seed_positions(a, S, qn, qc, qtot, r, hs, hcount, c_filt);
             qc += 64;
           } while (qtot - qc >= 64);
           // drop the buckets consumed whole and rebase the prefix sums; fewer
@@ -517,8 +568,16 @@ __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
       wave_sync();
     }
     // the positions still queued at the end of the read
-    for (; qc < qtot; qc += 64) seed_positions(a, S, qn, qc, qtot, hbase, cap, hcount, c_filt);
-    if (lane == 0) a.nhits[r] = hcount;
+    for (; qc < qtot; qc += 64) seed_positions(a, S, qn, qc, qtot, r, hs, hcount, c_filt);
+    if (lane == 0) {
+      a.nhits[r] = hcount;
+      if (a.chunk_tab) {  // chunk 0's first slot: reads of up to 1024 hits are contiguous there
+        a.hoff[r] = hs.nch ? ((uint64_t)hs.first << SEED_CH_BITS) : 0ull;
+        a.dropped[r] = hs.dropped ? 1 : 0;
+      } else if (a.mark_tab) {  // an exact slot: contiguous at hoff
+        a.mark_tab[(uint64_t)r * SEED_MAXCH] = SEED_CONTIGUOUS;
+      }
+    }
   }
   // per-wave counter reduction
 #pragma unroll
@@ -1122,8 +1181,9 @@ __device__ __forceinline__ void run_radix_sort(const ClusterMem& M, uint32_t a0,
 // the LDS table gt_* (CLUSTER_GT slots; per slot the group key, its hits, its
 // lowest and highest biased diagonal and its first emission index); gt_misc[0]
 // = distinct groups (may exceed CLUSTER_GT_MAX: the table then holds only some).
-template <int T>
-__device__ __forceinline__ void group_table_count(const uint64_t* hits, uint32_t n, const ClusterMem& M, uint32_t tid) {
+// hits(i): the read's hit i.
+template <int T, typename HitAt>
+__device__ __forceinline__ void group_table_count(HitAt hits, uint32_t n, const ClusterMem& M, uint32_t tid) {
   constexpr uint32_t U = 8;  // hits loaded ahead per thread: the loads of a pass overlap
   for (uint32_t i = tid; i < CLUSTER_GT; i += T) {
     M.gt_key[i] = 0xFFFFFFFFu;
@@ -1137,7 +1197,7 @@ __device__ __forceinline__ void group_table_count(const uint64_t* hits, uint32_t
   for (uint32_t i0 = tid; i0 < n; i0 += U * T) {
     uint64_t kb[U];
 #pragma unroll
-    for (uint32_t u = 0; u < U; u++) kb[u] = i0 + u * T < n ? hits[i0 + u * T] : 0ull;
+    for (uint32_t u = 0; u < U; u++) kb[u] = i0 + u * T < n ? hits(i0 + u * T) : 0ull;
 #pragma unroll
     for (uint32_t u = 0; u < U; u++) {
       const uint32_t i = i0 + u * T;
@@ -1213,7 +1273,7 @@ __device__ __forceinline__ bool group_segment_sort(const uint64_t* hits, uint32_
                                                    bool& by_rank) {
   static_assert(T == RSORT_T, "the block tier runs 256 threads");
   constexpr uint32_t U = 8;  // hits loaded ahead per thread in the scatter
-  group_table_count<T>(hits, n, M, tid);
+  group_table_count<T>([&](uint32_t i) { return hits[i]; }, n, M, tid);
   const uint32_t G = M.gt_misc[0];
   if (G > CLUSTER_GT_MAX) return false;
   // Segment order.  Without per-hit values and for at most 56 groups: the
@@ -2375,6 +2435,7 @@ __global__ __launch_bounds__(256) void k_cg_group(ClusterArgs a, CgArgs g, const
   __shared__ uint64_t sk[64];               // (first emission index << 32 | slot), insertion order
   __shared__ uint32_t sv[192];              // group key and insertion list for the ranks, then ranks
   __shared__ uint32_t rslot[64];            // slot of each rank
+  __shared__ uint32_t ctab[SEED_MAXCH];     // the read's hit chunks
   __shared__ uint32_t fail;
   ClusterMem M{};
   M.gt_key = gt_key;
@@ -2387,10 +2448,18 @@ __global__ __launch_bounds__(256) void k_cg_group(ClusterArgs a, CgArgs g, const
   for (uint32_t li = blockIdx.x; li < nlist; li += gridDim.x) {
     const uint32_t r = list[li];
     const uint32_t n = a.nhits[r];
-    const uint64_t* hits = a.hits + a.hoff[r];
+    const uint64_t hb = a.hoff[r];
+    const bool chunked = a.chunk_tab && a.chunk_tab[(uint64_t)r * SEED_MAXCH] != SEED_CONTIGUOUS;
+    if (chunked)
+      for (uint32_t j = tid; j < SEED_MAXCH; j += 256) ctab[j] = a.chunk_tab[(uint64_t)r * SEED_MAXCH + j];
     if (tid < 64) sk[tid] = ~0ull;
     if (tid == 0) fail = 0;
-    group_table_count<256>(hits, n, M, tid);
+    __syncthreads();
+    auto hit = [&](uint32_t i) -> uint64_t {
+      return a.hits[chunked ? ((uint64_t)ctab[i >> SEED_CH_BITS] << SEED_CH_BITS) + (i & ((1u << SEED_CH_BITS) - 1))
+                            : hb + i];
+    };
+    group_table_count<256>(hit, n, M, tid);
     const uint32_t G = gt_misc[0];
     if (G <= CG_GMAX) {
       for (uint32_t i = tid; i < CLUSTER_GT; i += 256)
@@ -2459,7 +2528,7 @@ __global__ __launch_bounds__(256) void k_cg_group(ClusterArgs a, CgArgs g, const
     for (uint32_t i0 = tid; i0 < n; i0 += U * 256) {
       uint64_t kb[U];
 #pragma unroll
-      for (uint32_t u = 0; u < U; u++) kb[u] = i0 + u * 256 < n ? hits[i0 + u * 256] : 0ull;
+      for (uint32_t u = 0; u < U; u++) kb[u] = i0 + u * 256 < n ? hit(i0 + u * 256) : 0ull;
 #pragma unroll
       for (uint32_t u = 0; u < U; u++) {
         const uint32_t i = i0 + u * 256;
@@ -5750,12 +5819,13 @@ __device__ __forceinline__ uint32_t ov_block_scan(uint32_t x, uint32_t* sh, uint
 // pass 1: per block (OV_PART words) [count, max, over0..3, overflow hits lo, hi,
 // all hits lo, hi, over4] (hit sums in 64 bits: the reads' low and high 16-bit
 // halves summed apart, each sum < 2^26)
-__global__ __launch_bounds__(OV_BLOCK) void k_ov_count(const uint32_t* nhits, uint32_t B, uint32_t cap1, uint32_t* part) {
+__global__ __launch_bounds__(OV_BLOCK) void k_ov_count(const uint32_t* nhits, const uint8_t* dropped, uint32_t B,
+                                                      uint32_t cap1, uint32_t* part) {
   __shared__ uint32_t sh[OV_BLOCK / 64];
   __shared__ uint32_t smax[OV_BLOCK / 64];
   const uint32_t i = blockIdx.x * OV_BLOCK + threadIdx.x;
   const uint32_t nh = i < B ? nhits[i] : 0u;
-  const bool ov = nh > cap1;
+  const bool ov = dropped ? (i < B && dropped[i] != 0) : nh > cap1;
   uint32_t mx = nh;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, o, 64));
@@ -5856,13 +5926,13 @@ __global__ __launch_bounds__(OV_BLOCK) void k_ov_scan(uint32_t* part, uint32_t n
   }
 }
 // pass 3: the list, in read order
-__global__ __launch_bounds__(OV_BLOCK) void k_ov_write(const uint32_t* nhits, uint32_t B, uint32_t cap1, uint64_t base,
-                                                      const uint32_t* part, uint32_t* list, uint64_t* off,
-                                                      uint32_t* cap) {
+__global__ __launch_bounds__(OV_BLOCK) void k_ov_write(const uint32_t* nhits, const uint8_t* dropped, uint32_t B,
+                                                      uint32_t cap1, uint64_t base, const uint32_t* part,
+                                                      uint32_t* list, uint64_t* off, uint32_t* cap) {
   __shared__ uint32_t sh[OV_BLOCK / 64];
   const uint32_t i = blockIdx.x * OV_BLOCK + threadIdx.x;
   const uint32_t nh = i < B ? nhits[i] : 0u;
-  const bool ov = nh > cap1;
+  const bool ov = dropped ? (i < B && dropped[i] != 0) : nh > cap1;
   uint32_t t0, t1, t2;
   const uint32_t ec = ov_block_scan(ov ? 1u : 0u, sh, t0);
   // the hits before this read in 64 bits: low and high 16-bit halves apart
@@ -5878,13 +5948,45 @@ __global__ __launch_bounds__(OV_BLOCK) void k_ov_write(const uint32_t* nhits, ui
   cap[k] = nh;
 }
 uint32_t seed_overflow_blocks(uint32_t B) { return (B + OV_BLOCK - 1) / OV_BLOCK; }
-void launch_seed_overflow(const uint32_t* nhits, uint32_t B, uint32_t cap1, uint64_t base, uint32_t* part,
-                          uint32_t* list, uint64_t* off, uint32_t* cap, uint32_t* st, hipStream_t s) {
+void launch_seed_overflow(const uint32_t* nhits, const uint8_t* dropped, uint32_t B, uint32_t cap1, uint64_t base,
+                          uint32_t* part, uint32_t* list, uint64_t* off, uint32_t* cap, uint32_t* st, hipStream_t s) {
   const uint32_t nb = seed_overflow_blocks(B);
   if (!nb) return;
-  hipLaunchKernelGGL(k_ov_count, dim3(nb), dim3(OV_BLOCK), 0, s, nhits, B, cap1, part);
+  hipLaunchKernelGGL(k_ov_count, dim3(nb), dim3(OV_BLOCK), 0, s, nhits, dropped, B, cap1, part);
   hipLaunchKernelGGL(k_ov_scan, dim3(1), dim3(OV_BLOCK), 0, s, part, nb, st);
-  hipLaunchKernelGGL(k_ov_write, dim3(nb), dim3(OV_BLOCK), 0, s, nhits, B, cap1, base, part, list, off, cap);
+  hipLaunchKernelGGL(k_ov_write, dim3(nb), dim3(OV_BLOCK), 0, s, nhits, dropped, B, cap1, base, part, list, off, cap);
+}
+
+// The listed reads' hits (and flags), chunked or not, copied contiguously for the
+// general block tier: read r's region taken from *next, its offset to off[r].
+__global__ __launch_bounds__(256) void k_hits_gather(const uint64_t* hits, const uint8_t* hflag, const uint64_t* hoff,
+                                                    const uint32_t* tab, const uint32_t* nhits, const uint32_t* list,
+                                                    uint32_t nlist, uint64_t* dst, uint8_t* dflag, uint64_t* off,
+                                                    unsigned long long* next) {
+  __shared__ unsigned long long base;
+  for (uint32_t li = blockIdx.x; li < nlist; li += gridDim.x) {
+    const uint32_t r = list[li];
+    const uint32_t n = nhits[r];
+    if (threadIdx.x == 0) {
+      base = atomicAdd(next, (unsigned long long)n);
+      off[r] = base;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += 256) {
+      const uint64_t src = hit_at(tab, hoff, r, i);
+      dst[base + i] = hits[src];
+      if (hflag) dflag[base + i] = hflag[src];
+    }
+    __syncthreads();
+  }
+}
+void launch_hits_gather(const uint64_t* hits, const uint8_t* hflag, const uint64_t* hoff, const uint32_t* tab,
+                        const uint32_t* nhits, const uint32_t* list, uint32_t nlist, uint64_t* dst, uint8_t* dflag,
+                        uint64_t* off, unsigned long long* next, hipStream_t s) {
+  if (!nlist) return;
+  const uint32_t blocks = nlist < 65536 ? nlist : 65536;
+  hipLaunchKernelGGL(k_hits_gather, dim3(blocks), dim3(256), 0, s, hits, hflag, hoff, tab, nhits, list, nlist, dst,
+                     dflag, off, next);
 }
 void launch_scatter_u64(uint64_t* out, const uint32_t* idx, const uint64_t* val, uint32_t n, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_scatter_u64, dim3((n + 255) / 256), dim3(256), 0, s, out, idx, val, n);

@@ -168,6 +168,7 @@ pub struct pbw_tune {
     pub split_len: u32,
     pub long_waves: u32,
     pub dp_pack16: u32,
+    pub seed_pool: u32,
 }
 
 #[repr(C)]

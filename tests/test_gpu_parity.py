@@ -283,8 +283,9 @@ def test_map_se_repeat_heavy_reads():
 
 
 def test_map_se_first_pass_capacity_adapts():
-    """When more than 5 % of a batch's reads overflow the first seed pass's hit slots, the
-    later batches take larger slots; the records stay the oracle's in every batch and call."""
+    """When a batch's hits outgrow the seed pass's chunk pool, the reads left without a chunk
+    are seeded again into exact slots and the later batches take a larger pool; the records
+    stay the oracle's in every batch and call."""
     rng = np.random.default_rng(43)
     unit = synth.random_bases(rng, 41)
     seqs = [("rep", np.concatenate([synth.mutate(rng, unit, 0.03) for _ in range(1200)])),
@@ -304,10 +305,10 @@ def test_map_se_first_pass_capacity_adapts():
 
 
 def test_map_se_batch_over_a_million_reads_matches_smaller_batches():
-    """1.1 M reads in one batch: the seed pass's overflow reads (those from a
-    400-copy repeat, scattered through the batch) are listed over more than
-    1024 blocks of 1024 reads, so the block scan runs in two chunks.  The
-    records equal those of 65,536-read batches, byte for byte."""
+    """1.1 M reads in one batch with a seed pool of 1000 chunks: nearly every read's
+    hits find no chunk, so the reads seeded again into exact slots are listed over
+    more than 1024 blocks of 1024 reads and the block scan runs in two chunks.  The
+    records equal those of 65,536-read batches with the grown pool, byte for byte."""
     rng = np.random.default_rng(7)
     unit = synth.random_bases(rng, 41)
     rep = np.concatenate([synth.mutate(rng, unit, 0.03) for _ in range(400)])
@@ -325,9 +326,9 @@ def test_map_se_batch_over_a_million_reads_matches_smaller_batches():
     off = np.arange(0, n * L + 1, L, dtype=np.uint64)
     reads = P.ReadBatch.from_arrays(seq, off, qual, off)
     ctx = P.Context(g)
-    ctx.tune = dict(batch_reads=1_200_000)
+    ctx.tune = dict(batch_reads=1_200_000, seed_pool=1000)
     big = ctx.map_se(reads)
-    assert ctx.stats()["seed_overflow_reads"] > 0
+    assert ctx.stats()["seed_overflow_reads"] > 1_000_000
     ctx2 = P.Context(g)
     ctx2.tune = dict(batch_reads=65536)
     small = ctx2.map_se(reads)

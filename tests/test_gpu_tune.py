@@ -46,6 +46,7 @@ def test_default_shape_matches_oracle(setup):
     dict(batch_reads=777, dp_grid=4),
     dict(dp_pack16=1),
     dict(dp_pack16=1, split_len=96, long_waves=0, dp_grid=64),
+    dict(seed_pool=16),
 ])
 def test_launch_shapes_give_identical_records(setup, shape):
     _, gp, _, flat = setup
