@@ -188,6 +188,8 @@ struct CgGroup {
   unsigned long long lo;  // lowest biased diagonal
   uint32_t key;           // contig * 2 + strand
   uint32_t start;         // segment start within the read's keys ([G]: the hit count)
+  uint32_t rbits;         // bits of the highest diagonal offset (hi - lo)
+  uint32_t pad;
 };
 struct CgArgs {
   uint64_t* ck;            // composite keys: each read's region taken from *ck_next by k_cg_group

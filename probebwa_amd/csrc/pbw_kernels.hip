@@ -2495,6 +2495,7 @@ __global__ __launch_bounds__(256) void k_cg_group(ClusterArgs a, CgArgs g, const
           cg[lane].lo = lo;
           cg[lane].key = gkey;
           cg[lane].start = start;
+          cg[lane].rbits = hi - lo ? 64u - (uint32_t)__builtin_clzll(hi - lo) : 0u;
           gt_cnt[sl] = start;  // the slot's scatter cursor
           gt_emin[sl] = lane;  // the slot's rank
         }
@@ -2502,6 +2503,7 @@ __global__ __launch_bounds__(256) void k_cg_group(ClusterArgs a, CgArgs g, const
           cg[G].lo = 0;
           cg[G].key = 0;
           cg[G].start = n;
+          cg[G].rbits = 0;
         }
         if (__ballot(wide) && lane == 0) fail = 1;
       }
@@ -2564,8 +2566,10 @@ template <int IPT>
 using CgSort = rocprim::block_radix_sort<uint64_t, 256, IPT, rocprim::empty_type, 1, 1, 8,
                                          rocprim::block_radix_rank_algorithm::match>;
 union CgSortLds {
-  typename CgSort<16>::storage_type big;
-  typename CgSort<4>::storage_type small;
+  typename CgSort<16>::storage_type s16;
+  typename CgSort<8>::storage_type s8;
+  typename CgSort<4>::storage_type s4;
+  typename CgSort<2>::storage_type s2;
   uint64_t raw[MERGE_CHUNK];  // block_merge's windows
 };
 template <int IPT>
@@ -2586,13 +2590,186 @@ __device__ __forceinline__ void cg_run_sort(uint64_t* key, uint32_t cnt, uint32_
   __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void k_cg_sort(ClusterArgs a, CgArgs g, const uint32_t* list, uint32_t nlist) {
-  __shared__ CgSortLds rs;
-  __shared__ uint32_t stage_val[CLUSTER_STAGE];
-  __shared__ uint32_t st[CG_GMAX + 1];
+// Bitonic sort of the s <= 64 E keys at key[0, s) by one wave, in registers
+// (element e * 64 + lane in slot e of a lane), padded with ~0.
+template <int E>
+__device__ __forceinline__ void wave_bitonic_n(uint64_t* key, uint32_t s) {
+  const uint32_t lane = (uint32_t)lane_id();
+  uint32_t p2 = 1;
+  while (p2 < s) p2 <<= 1;
+  uint64_t kr[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const uint32_t i = (uint32_t)e * 64u + lane;
+    kr[e] = i < s ? key[i] : ~0ull;
+  }
+  for (uint32_t k = 2; k <= p2; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      if (j >= 64) {
+        const uint32_t js = j >> 6;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          const uint32_t ep = (uint32_t)e ^ js;
+          if ((uint32_t)e < ep && ep < (uint32_t)E) {
+            const bool up = (((uint32_t)e * 64u) & k) == 0;
+            const uint64_t x = kr[e], y = kr[ep];
+            if ((x > y) == up) {
+              kr[e] = y;
+              kr[ep] = x;
+            }
+          }
+        }
+      } else {
+        const bool lower = (lane & j) == 0;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          const uint32_t i = (uint32_t)e * 64u + lane;
+          const uint64_t pk = (uint64_t)__shfl_xor((long long)kr[e], (int)j, 64);
+          const bool up = (i & k) == 0;
+          const bool sw = lower ? ((kr[e] > pk) == up) : ((pk > kr[e]) == up);
+          if (sw) kr[e] = pk;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const uint32_t i = (uint32_t)e * 64u + lane;
+    if (i < s) key[i] = kr[e];
+  }
+}
+
+// A run of whole segments (ranks q0..q1-1, cnt <= CG_BTILE keys at key[0, cnt))
+// sorted by one counting pass over buckets: segment q gets the largest power of two
+// of buckets not above its key count, each a 2^sh_q-wide slice of its offsets, so
+// bucket order is key order and a bucket holds one or two keys; every key is counted
+// into its bucket, the buckets' ends come from a block scan, every key is placed
+// in LDS, and then each bucket of two to CG_BSMALL keys is put in order by one
+// thread (insertion), each larger one of up to 512 by a wave (bitonic in
+// registers).  Returns false, with nothing written, when a bucket holds more than
+// 512 keys (the caller then sorts the run by radix).
+constexpr uint32_t CG_BTILE = 4096, CG_BSMALL = 16, CG_NBMAX = CG_BTILE;
+struct CgBucketLds {
+  uint64_t keys[CG_BTILE];
+  uint32_t end[CG_NBMAX];   // bucket counts, then (scan) each bucket's end
+  uint32_t base[CG_GMAX + 1], sh[CG_GMAX + 1];
+  uint32_t big[CG_BTILE / CG_BSMALL];  // buckets of more than CG_BSMALL keys
+  uint32_t nbig, fail, scan[8];
+};
+__device__ __forceinline__ bool cg_bucket_sort(uint64_t* key, uint32_t cnt, uint32_t q0, uint32_t q1, const uint32_t* st,
+                                               const uint32_t* rng_bits, CgBucketLds& B, uint32_t tid) {
+  constexpr uint32_t IPT = CG_BTILE / 256;
+  // buckets per segment
+  if (tid == 0) {
+    uint32_t nb = 0;
+    for (uint32_t q = q0; q < q1; q++) {
+      const uint32_t c = st[q + 1] - st[q];
+      uint32_t lg = 0;  // 2^lg <= c buckets: about one or two keys each
+      while ((2u << lg) <= c) lg++;
+      B.base[q - q0] = nb;
+      B.sh[q - q0] = rng_bits[q] > lg ? rng_bits[q] - lg : 0u;
+      nb += 1u << (rng_bits[q] > lg ? lg : rng_bits[q]);
+    }
+    B.base[q1 - q0] = nb;
+    B.nbig = 0;
+    B.fail = 0;
+  }
+  __syncthreads();
+  const uint32_t NB = B.base[q1 - q0];
+  for (uint32_t b = tid; b < NB; b += 256) B.end[b] = 0;
+  __syncthreads();
+  uint64_t kk[IPT];
+  uint32_t bk[IPT];
+#pragma unroll
+  for (uint32_t j = 0; j < IPT; j++) {
+    const uint32_t pos = j * 256 + tid;
+    kk[j] = pos < cnt ? key[pos] : 0ull;
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < IPT; j++) {
+    const uint32_t pos = j * 256 + tid;
+    if (pos < cnt) {
+      const uint32_t ql = (uint32_t)(kk[j] >> 32) - q0;
+      bk[j] = B.base[ql] + (uint32_t)((uint64_t)(uint32_t)kk[j] >> B.sh[ql]);  // sh <= 32
+      atomicAdd(&B.end[bk[j]], 1u);
+    }
+  }
+  __syncthreads();
+  {  // inclusive scan of the counts: bucket ends
+    const uint32_t per = (NB + 255) / 256, b0 = min(NB, tid * per), b1 = min(NB, b0 + per);
+    uint32_t local = 0;
+    for (uint32_t b = b0; b < b1; b++) local += B.end[b];
+    uint32_t tot;
+    uint32_t run = team_excl_scan<256>(local, tot, B.scan);
+    for (uint32_t b = b0; b < b1; b++) {
+      const uint32_t c = B.end[b];
+      if (c > CG_BSMALL) {
+        if (c > 512) B.fail = 1;
+        else B.big[atomicAdd(&B.nbig, 1u)] = b;
+      }
+      run += c;
+      B.end[b] = run;
+    }
+  }
+  __syncthreads();
+  if (B.fail) return false;
+  // place: a bucket's keys fill it from its end backwards
+#pragma unroll
+  for (uint32_t j = 0; j < IPT; j++) {
+    const uint32_t pos = j * 256 + tid;
+    if (pos < cnt) B.keys[atomicSub(&B.end[bk[j]], 1u) - 1u] = kk[j];
+  }
+  __syncthreads();
+  // B.end[b] is now bucket b's start; its end is the next bucket's start (cnt for the last)
+  for (uint32_t b = tid; b < NB; b += 256) {
+    const uint32_t s0 = B.end[b], s1 = b + 1 < NB ? B.end[b + 1] : cnt;
+    const uint32_t c = s1 - s0;
+    if (c >= 2 && c <= CG_BSMALL) {
+      for (uint32_t x = s0 + 1; x < s1; x++) {
+        const uint64_t v = B.keys[x];
+        uint32_t y = x;
+        while (y > s0 && B.keys[y - 1] > v) {
+          B.keys[y] = B.keys[y - 1];
+          y--;
+        }
+        B.keys[y] = v;
+      }
+    }
+  }
+  for (uint32_t i = tid >> 6; i < B.nbig; i += 4) {
+    const uint32_t b = B.big[i];
+    const uint32_t s0 = B.end[b], s1 = b + 1 < NB ? B.end[b + 1] : cnt;
+    const uint32_t c = s1 - s0;
+    if (c <= 128) wave_bitonic_n<2>(B.keys + s0, c);
+    else if (c <= 256) wave_bitonic_n<4>(B.keys + s0, c);
+    else wave_bitonic_n<8>(B.keys + s0, c);
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t j = 0; j < IPT; j++) {
+    const uint32_t pos = j * 256 + tid;
+    if (pos < cnt) key[pos] = B.keys[pos];
+  }
+  __syncthreads();
+  return true;
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_cg_sort(ClusterArgs a, CgArgs g,
+                                                                                      const uint32_t* list,
+                                                                                      uint32_t nlist) {
+  // the bucket sort's tables, or the radix sort's and the merges' (splits in stage_val)
+  __shared__ union {
+    struct {
+      CgSortLds rs;
+      uint32_t stage_val[CLUSTER_STAGE];
+    } r;
+    CgBucketLds bs;
+  } u;
+  __shared__ uint32_t st[CG_GMAX + 1], rb[CG_GMAX + 1];
+  CgSortLds& rs = u.r.rs;
   ClusterMem M{};
   M.rsort = &rs;
-  M.stage_val = stage_val;
+  M.stage_val = u.r.stage_val;
   const uint32_t tid = threadIdx.x;
   constexpr uint32_t RUN_BITS = 38;  // rank < 64 above 32 bits of diagonal offset
   static_assert(CG_GMAX < 64, "ranks in 6 bits");
@@ -2601,7 +2778,10 @@ __global__ __launch_bounds__(256) void k_cg_sort(ClusterArgs a, CgArgs g, const 
     if (G == 0) continue;
     const uint32_t r = list[li];
     const CgGroup* cg = g.cg + (uint64_t)li * (CG_GMAX + 1);
-    if (tid <= G) st[tid] = cg[tid].start;
+    if (tid <= G) {
+      st[tid] = cg[tid].start;
+      rb[tid] = cg[tid].rbits;
+    }
     __syncthreads();
     uint64_t* key = g.ck + g.ck_off[li];
     uint64_t* tmp = g.tmp + g.ck_off[li];
@@ -2612,8 +2792,11 @@ __global__ __launch_bounds__(256) void k_cg_sort(ClusterArgs a, CgArgs g, const 
       while (q1 < G && st[q1 + 1] - a0 <= RSORT_TILE) q1++;
       const uint32_t cnt = st[q1] - a0;
       if (cnt <= RSORT_TILE) {
-        if (cnt > 1) {
-          if (cnt <= 4 * 256) cg_run_sort<4>(key + a0, cnt, RUN_BITS, rs, tid);
+        if (cnt > 1 && !cg_bucket_sort(key + a0, cnt, q, q1, st, rb, u.bs, tid)) {
+          // a bucket of over 1024 keys: the smallest radix-sort tile that holds the run
+          if (cnt <= 2 * 256) cg_run_sort<2>(key + a0, cnt, RUN_BITS, rs, tid);
+          else if (cnt <= 4 * 256) cg_run_sort<4>(key + a0, cnt, RUN_BITS, rs, tid);
+          else if (cnt <= 8 * 256) cg_run_sort<8>(key + a0, cnt, RUN_BITS, rs, tid);
           else cg_run_sort<16>(key + a0, cnt, RUN_BITS, rs, tid);
         }
       } else {  // one segment over a tile: its keys share the rank bits
