@@ -823,9 +823,9 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
     const uint32_t* t2list = c->d_tier2.p;
     uint32_t nt2b = nt2;  // reads for k_cluster_tier2
     HIP_CHECK(hipEventRecord(c->ev[6], s));
-    if (!feat) {
-      // three launches (group / sort / select) for the reads they take; the rest,
-      // listed on the device, to k_cluster_tier2 below
+    {
+      // three launches (group / sort / select) for the reads they take (features
+      // too); the rest, listed on the device, to k_cluster_tier2 below
       if (c->d_ck.n < all_hits) {  // with headroom: the count varies from batch to batch
         c->d_ck.ensure(all_hits + all_hits / 4);
         c->d_cktmp.ensure(all_hits + all_hits / 4);

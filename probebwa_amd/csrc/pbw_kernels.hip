@@ -1005,6 +1005,60 @@ __device__ __forceinline__ int64_t hit_diag(uint64_t k) {
   return (int64_t)(k & ((1ull << DIAG_BITS) - 1)) - DIAG_BIAS;
 }
 
+// The prefilter's 32 BF16 features of the candidate at rank `rank` (its cluster
+// pc) of a read (pr): the table of the network's inputs, slots 27-31 zero.
+struct PfCluster {
+  uint32_t cnt;   // hits
+  uint32_t nb;    // of them from 1-mismatch neighbours
+  uint32_t ms;    // sum of the hits' bucket-size classes
+  uint32_t mn;    // least bucket-size class
+  uint32_t span;  // diagonal span
+  bool rev;
+};
+struct PfRead {
+  uint32_t c0, c1;               // the top and second candidates' hits
+  uint32_t ties;                 // clusters with the top count
+  uint32_t qsum, q10, q20, qlen; // quality sum, bases below Q10 and Q20, quality length
+  uint32_t rep;                  // repeat-skipped windows
+  uint32_t C, n, L, G;           // clusters, hits, read length, groups
+};
+__device__ __forceinline__ void pf_feature_row(uint16_t* out, uint32_t rank, const PfCluster& pc, const PfRead& pr) {
+  const float cnt = (float)pc.cnt, c0 = (float)pr.c0, c1 = (float)pr.c1;
+  const float nb = (float)pc.nb, mn = (float)pc.mn, ms = (float)pc.ms, span = (float)pc.span;
+  const float nwin = (float)(pr.L >= (uint32_t)K ? pr.L - K + 1 : 1);
+  const float ql = (float)max(pr.qlen, 1u);
+  float f[32];
+  f[0] = log2f(1.f + cnt);
+  f[1] = cnt / c0;
+  f[2] = log2f(1.f + nb);
+  f[3] = nb / cnt;
+  f[4] = (cnt - nb) / cnt;
+  f[5] = log2f(1.f + span);
+  f[6] = span / fmaxf(1.f, cnt - 1.f);
+  f[7] = ms / cnt / 9.f;
+  f[8] = mn / 9.f;
+  f[9] = (float)rank / 20.f;
+  f[10] = rank == 0 ? 1.f : 0.f;
+  f[11] = pc.rev ? 1.f : 0.f;
+  f[12] = log2f(1.f + c0);
+  f[13] = log2f(1.f + c1);
+  f[14] = cnt == c0 ? 1.f : 0.f;
+  f[15] = (float)pr.ties / 20.f;
+  f[16] = log2f(1.f + (float)pr.C) / 10.f;
+  f[17] = log2f(1.f + (float)pr.n) / 10.f;
+  f[18] = (float)pr.L / 150.f;
+  f[19] = (float)pr.qsum / ql / 40.f;
+  f[20] = (float)pr.q10 / ql;
+  f[21] = (float)pr.q20 / ql;
+  f[22] = cnt / nwin;
+  f[23] = (float)pr.rep / nwin;
+  f[24] = (float)pr.G / 20.f;
+  f[25] = f[0] - f[12];
+  f[26] = (c0 - c1) / c0;
+  for (int u = 27; u < 32; u++) f[u] = 0.f;
+  for (int u = 0; u < 32; u++) out[u] = float_to_bf16(f[u]);
+}
+
 // find_candidates (candidates.rs:30-78) of one read by a team of T threads.
 // Returns false (uniformly) when the read has more groups than M.gcap; the
 // caller then hands it to the block tier.
@@ -1028,11 +1082,11 @@ __device__ __forceinline__ uint32_t gt_slot(uint32_t g) { return (g * 0x9E3779B1
 // Merge path: how many of the first d outputs of merging the sorted runs
 // A[0, na) and B[0, nb) come from A (A first on ties).
 __device__ __forceinline__ uint32_t merge_split(const uint64_t* A, uint32_t na, const uint64_t* B, uint32_t nb,
-                                                uint32_t d) {
+                                                uint32_t d, uint64_t cm = ~0ull) {
   uint32_t lo = d > nb ? d - nb : 0u, hi = min(d, na);
   while (lo < hi) {
     const uint32_t mid = (lo + hi) >> 1;
-    if (A[mid] <= B[d - 1 - mid]) lo = mid + 1;
+    if ((A[mid] & cm) <= (B[d - 1 - mid] & cm)) lo = mid + 1;
     else hi = mid;
   }
   return lo;
@@ -1044,13 +1098,13 @@ __device__ __forceinline__ uint32_t merge_split(const uint64_t* A, uint32_t na, 
 // rounds of parallel loads instead of log2(m) dependent ones.  All 64 lanes call
 // it; every lane returns the split.
 __device__ __forceinline__ uint32_t merge_split_wave(const uint64_t* A, uint32_t na, const uint64_t* B, uint32_t nb,
-                                                     uint32_t d) {
+                                                     uint32_t d, uint64_t cm = ~0ull) {
   uint32_t lo = d > nb ? d - nb : 0u, hi = min(d, na);
   const uint32_t lane = (uint32_t)lane_id();
   while (lo < hi) {
     const uint32_t step = (hi - lo + 63) / 64;
     const uint32_t i = lo + lane * step;
-    const bool t = i < hi && A[i] <= B[d - 1 - i];
+    const bool t = i < hi && (A[i] & cm) <= (B[d - 1 - i] & cm);
     const uint32_t nt = (uint32_t)__popcll(__ballot(t));  // the true points are a prefix
     if (nt == 0) {
       hi = lo;
@@ -1075,7 +1129,7 @@ constexpr uint32_t MERGE_CHUNK = 2048;
 template <bool VALS>
 __device__ __forceinline__ void block_merge(const ClusterMem& M, const uint64_t* sk, const uint32_t* sv,
                                             uint64_t* dk, uint32_t* dv, uint32_t s, uint32_t na, uint32_t nb,
-                                            uint32_t tid) {
+                                            uint32_t tid, uint64_t cm = ~0ull) {
   static_assert(sizeof(RSortLds) >= MERGE_CHUNK * sizeof(uint64_t), "merge window keys fit the sort's LDS");
   static_assert(CLUSTER_STAGE * sizeof(uint64_t) >= MERGE_CHUNK * sizeof(uint32_t), "merge window values fit");
   constexpr uint32_t PER = MERGE_CHUNK / RSORT_T, NW = RSORT_T / 64, BATCH = CLUSTER_STAGE - 1;
@@ -1088,7 +1142,7 @@ __device__ __forceinline__ void block_merge(const ClusterMem& M, const uint64_t*
   for (uint32_t D0 = 0; D0 < total; D0 += BATCH * MERGE_CHUNK) {
     const uint32_t nch = min(BATCH, (total - D0 + MERGE_CHUNK - 1) / MERGE_CHUNK);
     for (uint32_t k = tid >> 6; k <= nch; k += NW)
-      sp[k] = merge_split_wave(A, na, B, nb, min(total, D0 + k * MERGE_CHUNK));
+      sp[k] = merge_split_wave(A, na, B, nb, min(total, D0 + k * MERGE_CHUNK), cm);
     __syncthreads();
     for (uint32_t k = 0; k < nch; k++) {
       const uint32_t D = D0 + k * MERGE_CHUNK, E = min(total, D + MERGE_CHUNK);
@@ -1113,9 +1167,9 @@ __device__ __forceinline__ void block_merge(const ClusterMem& M, const uint64_t*
       }
       __syncthreads();
       const uint32_t t0 = min(an + bn, tid * PER), t1 = min(an + bn, t0 + PER);
-      uint32_t i = merge_split(lk, an, lk + an, bn, t0), j = t0 - i;
+      uint32_t i = merge_split(lk, an, lk + an, bn, t0, cm), j = t0 - i;
       for (uint32_t d = t0; d < t1; d++) {
-        const bool takeA = j >= bn || (i < an && lk[i] <= lk[an + j]);
+        const bool takeA = j >= bn || (i < an && (lk[i] & cm) <= (lk[an + j] & cm));
         const uint32_t x = takeA ? i : an + j;
         dk[s + D + d] = lk[x];
         if constexpr (VALS) dv[s + D + d] = lv[x];
@@ -1765,13 +1819,20 @@ __device__ __forceinline__ bool top_by_count(const ClusterMem& M, uint32_t C, ui
 // handles the read.  Keys: hit keys (CK false), or composite keys (group rank <<
 // 32 | biased diagonal - the group's lowest, CK true) with the groups in grp[].
 // lds: CNT_BINS + 16 + 2 * TOP_CANDIDATES words and TOP_CANDIDATES doubles.
+// Composite keys of the three-launch block tier: bits 0..31 the diagonal offset,
+// 32..37 the group rank (together the sort key, CG_KEY_MASK), 40..47 the hit's
+// prefilter flags when features are on (carried, never compared).
+constexpr uint64_t CG_KEY_MASK = (1ull << 38) - 1;
+constexpr uint32_t CG_FLAG_SHIFT = 40;
+__device__ __forceinline__ uint32_t cg_rank(uint64_t k) { return (uint32_t)(k >> 32) & 63u; }
+
 constexpr uint32_t CNT_BINS = 1024, CNT_REG = 8;
 constexpr size_t TOP20_LDS_BYTES = (CNT_BINS + 16 + 2 * TOP_CANDIDATES) * 4 + TOP_CANDIDATES * 8;
 template <int T, bool CK>
 __device__ __forceinline__ bool top20_ordered(const uint64_t* key, void* lds, const CgGroup* grp, uint32_t n,
                                               uint32_t L, Cand* out, uint8_t* ncand_out, uint32_t tid) {
   // group of a key, and the part of its diagonal that differs within a group
-  auto kgroup = [](uint64_t k) -> uint64_t { return CK ? (k >> 32) : (k >> DIAG_BITS); };
+  auto kgroup = [](uint64_t k) -> uint64_t { return CK ? (uint64_t)cg_rank(k) : (k >> DIAG_BITS); };
   auto kdiag = [](uint64_t k) -> int64_t { return CK ? (int64_t)(uint32_t)k : hit_diag(k); };
   uint32_t* hist = reinterpret_cast<uint32_t*>(lds);  // CNT_BINS counters
   uint32_t* misc = hist + CNT_BINS;  // [0] chosen so far, [1] T, [2] ties wanted, [3] nt, [4] T saturated, [6..] scan
@@ -1899,7 +1960,7 @@ __device__ __forceinline__ bool top20_ordered(const uint64_t* key, void* lds, co
     uint32_t gk;
     int64_t dsum = (int64_t)osum[tid];
     if constexpr (CK) {  // the diagonals' sum from the offsets' sum: c (lowest - bias) + sum
-      const CgGroup& g = grp[(uint32_t)(key[cs] >> 32)];
+      const CgGroup& g = grp[cg_rank(key[cs])];
       gk = g.key;
       dsum += (int64_t)c * ((int64_t)g.lo - DIAG_BIAS);
     } else {
@@ -2198,43 +2259,27 @@ __device__ __forceinline__ bool cluster_read(const uint64_t* hits, uint32_t n, u
     team_sync<T>();
     if (tid < nt) {
       const uint32_t c = M.idx[tid];
-      const float cnt = (float)(M.cl_start[c + 1] - M.cl_start[c]);
-      const float c0 = (float)(M.cl_start[M.idx[0] + 1] - M.cl_start[M.idx[0]]);
-      const float c1 = C > 1 ? (float)(M.cl_start[M.idx[1] + 1] - M.cl_start[M.idx[1]]) : 0.f;
-      const float nb = (float)F.f1[c], mn = (float)F.f4[c];
-      const float ms = (float)F.f2[c], span = (float)F.f3[c];
-      const float nwin = (float)(L >= (uint32_t)K ? L - K + 1 : 1);
-      const float ql = (float)max(F.qlen, 1u);
-      float f[32];
-      f[0] = log2f(1.f + cnt);
-      f[1] = cnt / c0;
-      f[2] = log2f(1.f + nb);
-      f[3] = nb / cnt;
-      f[4] = (cnt - nb) / cnt;
-      f[5] = log2f(1.f + span);
-      f[6] = span / fmaxf(1.f, cnt - 1.f);
-      f[7] = ms / cnt / 9.f;
-      f[8] = mn / 9.f;
-      f[9] = (float)tid / 20.f;
-      f[10] = tid == 0 ? 1.f : 0.f;
-      f[11] = (M.g_key[M.cl_gid[c]] & 1u) ? 1.f : 0.f;
-      f[12] = log2f(1.f + c0);
-      f[13] = log2f(1.f + c1);
-      f[14] = cnt == c0 ? 1.f : 0.f;
-      f[15] = (float)F.cnt[0] / 20.f;
-      f[16] = log2f(1.f + (float)C) / 10.f;
-      f[17] = log2f(1.f + (float)n) / 10.f;
-      f[18] = (float)L / 150.f;
-      f[19] = (float)F.cnt[1] / ql / 40.f;
-      f[20] = (float)F.cnt[2] / ql;
-      f[21] = (float)F.cnt[3] / ql;
-      f[22] = cnt / nwin;
-      f[23] = (float)F.cnt[4] / nwin;
-      f[24] = (float)G / 20.f;
-      f[25] = f[0] - f[12];
-      f[26] = (c0 - c1) / c0;
-      for (int u = 27; u < 32; u++) f[u] = 0.f;
-      for (int u = 0; u < 32; u++) F.out[tid * 32 + u] = float_to_bf16(f[u]);
+      PfCluster pc;
+      pc.cnt = M.cl_start[c + 1] - M.cl_start[c];
+      pc.nb = F.f1[c];
+      pc.ms = F.f2[c];
+      pc.mn = F.f4[c];
+      pc.span = F.f3[c];
+      pc.rev = (M.g_key[M.cl_gid[c]] & 1u) != 0;
+      PfRead pr;
+      pr.c0 = M.cl_start[M.idx[0] + 1] - M.cl_start[M.idx[0]];
+      pr.c1 = C > 1 ? M.cl_start[M.idx[1] + 1] - M.cl_start[M.idx[1]] : 0u;
+      pr.ties = F.cnt[0];
+      pr.qsum = F.cnt[1];
+      pr.q10 = F.cnt[2];
+      pr.q20 = F.cnt[3];
+      pr.rep = F.cnt[4];
+      pr.qlen = F.qlen;
+      pr.C = C;
+      pr.n = n;
+      pr.L = L;
+      pr.G = G;
+      pf_feature_row(F.out + tid * 32, tid, pc, pr);
     }
   }
   if (tid < nt) {
@@ -2455,10 +2500,10 @@ __global__ __launch_bounds__(256) void k_cg_group(ClusterArgs a, CgArgs g, const
     if (tid < 64) sk[tid] = ~0ull;
     if (tid == 0) fail = 0;
     __syncthreads();
-    auto hit = [&](uint32_t i) -> uint64_t {
-      return a.hits[chunked ? ((uint64_t)ctab[i >> SEED_CH_BITS] << SEED_CH_BITS) + (i & ((1u << SEED_CH_BITS) - 1))
-                            : hb + i];
+    auto hidx = [&](uint32_t i) -> uint64_t {
+      return chunked ? ((uint64_t)ctab[i >> SEED_CH_BITS] << SEED_CH_BITS) + (i & ((1u << SEED_CH_BITS) - 1)) : hb + i;
     };
+    auto hit = [&](uint32_t i) -> uint64_t { return a.hits[hidx(i)]; };
     group_table_count<256>(hit, n, M, tid);
     const uint32_t G = gt_misc[0];
     if (G <= CG_GMAX) {
@@ -2529,8 +2574,14 @@ __global__ __launch_bounds__(256) void k_cg_group(ClusterArgs a, CgArgs g, const
     constexpr uint32_t U = 8;
     for (uint32_t i0 = tid; i0 < n; i0 += U * 256) {
       uint64_t kb[U];
+      uint32_t fb[U];  // prefilter flags (features on), carried in the keys' bits 40..47
 #pragma unroll
-      for (uint32_t u = 0; u < U; u++) kb[u] = i0 + u * 256 < n ? hit(i0 + u * 256) : 0ull;
+      for (uint32_t u = 0; u < U; u++) {
+        const bool in = i0 + u * 256 < n;
+        const uint64_t x = in ? hidx(i0 + u * 256) : 0ull;
+        kb[u] = in ? a.hits[x] : 0ull;
+        fb[u] = in && a.feat ? (uint32_t)a.hflag[x] : 0u;
+      }
 #pragma unroll
       for (uint32_t u = 0; u < U; u++) {
         const uint32_t i = i0 + u * 256;
@@ -2553,7 +2604,7 @@ __global__ __launch_bounds__(256) void k_cg_group(ClusterArgs a, CgArgs g, const
           while (gt_key[sl] != gg) sl = (sl + 1) & (CLUSTER_GT - 1);
           pos = atomicAdd(&gt_cnt[sl], 1u);
         }
-        ck[pos] = ((uint64_t)gt_emin[sl] << 32) | (uint64_t)(uint32_t)(d - gt_lo[sl]);
+        ck[pos] = ((uint64_t)fb[u] << CG_FLAG_SHIFT) | ((uint64_t)gt_emin[sl] << 32) | (uint64_t)(uint32_t)(d - gt_lo[sl]);
       }
     }
     __syncthreads();
@@ -2591,9 +2642,10 @@ __device__ __forceinline__ void cg_run_sort(uint64_t* key, uint32_t cnt, uint32_
 }
 
 // Bitonic sort of the s <= 64 E keys at key[0, s) by one wave, in registers
-// (element e * 64 + lane in slot e of a lane), padded with ~0.
+// (element e * 64 + lane in slot e of a lane), padded with ~0; keys compare
+// under the mask cm.
 template <int E>
-__device__ __forceinline__ void wave_bitonic_n(uint64_t* key, uint32_t s) {
+__device__ __forceinline__ void wave_bitonic_n(uint64_t* key, uint32_t s, uint64_t cm) {
   const uint32_t lane = (uint32_t)lane_id();
   uint32_t p2 = 1;
   while (p2 < s) p2 <<= 1;
@@ -2613,7 +2665,7 @@ __device__ __forceinline__ void wave_bitonic_n(uint64_t* key, uint32_t s) {
           if ((uint32_t)e < ep && ep < (uint32_t)E) {
             const bool up = (((uint32_t)e * 64u) & k) == 0;
             const uint64_t x = kr[e], y = kr[ep];
-            if ((x > y) == up) {
+            if (((x & cm) > (y & cm)) == up) {
               kr[e] = y;
               kr[ep] = x;
             }
@@ -2626,7 +2678,7 @@ __device__ __forceinline__ void wave_bitonic_n(uint64_t* key, uint32_t s) {
           const uint32_t i = (uint32_t)e * 64u + lane;
           const uint64_t pk = (uint64_t)__shfl_xor((long long)kr[e], (int)j, 64);
           const bool up = (i & k) == 0;
-          const bool sw = lower ? ((kr[e] > pk) == up) : ((pk > kr[e]) == up);
+          const bool sw = lower ? (((kr[e] & cm) > (pk & cm)) == up) : (((pk & cm) > (kr[e] & cm)) == up);
           if (sw) kr[e] = pk;
         }
       }
@@ -2689,7 +2741,7 @@ __device__ __forceinline__ bool cg_bucket_sort(uint64_t* key, uint32_t cnt, uint
   for (uint32_t j = 0; j < IPT; j++) {
     const uint32_t pos = j * 256 + tid;
     if (pos < cnt) {
-      const uint32_t ql = (uint32_t)(kk[j] >> 32) - q0;
+      const uint32_t ql = cg_rank(kk[j]) - q0;
       bk[j] = B.base[ql] + (uint32_t)((uint64_t)(uint32_t)kk[j] >> B.sh[ql]);  // sh <= 32
       atomicAdd(&B.end[bk[j]], 1u);
     }
@@ -2728,7 +2780,7 @@ __device__ __forceinline__ bool cg_bucket_sort(uint64_t* key, uint32_t cnt, uint
       for (uint32_t x = s0 + 1; x < s1; x++) {
         const uint64_t v = B.keys[x];
         uint32_t y = x;
-        while (y > s0 && B.keys[y - 1] > v) {
+        while (y > s0 && (B.keys[y - 1] & CG_KEY_MASK) > (v & CG_KEY_MASK)) {
           B.keys[y] = B.keys[y - 1];
           y--;
         }
@@ -2740,9 +2792,9 @@ __device__ __forceinline__ bool cg_bucket_sort(uint64_t* key, uint32_t cnt, uint
     const uint32_t b = B.big[i];
     const uint32_t s0 = B.end[b], s1 = b + 1 < NB ? B.end[b + 1] : cnt;
     const uint32_t c = s1 - s0;
-    if (c <= 128) wave_bitonic_n<2>(B.keys + s0, c);
-    else if (c <= 256) wave_bitonic_n<4>(B.keys + s0, c);
-    else wave_bitonic_n<8>(B.keys + s0, c);
+    if (c <= 128) wave_bitonic_n<2>(B.keys + s0, c, CG_KEY_MASK);
+    else if (c <= 256) wave_bitonic_n<4>(B.keys + s0, c, CG_KEY_MASK);
+    else wave_bitonic_n<8>(B.keys + s0, c, CG_KEY_MASK);
   }
   __syncthreads();
 #pragma unroll
@@ -2776,7 +2828,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   for (uint32_t li = blockIdx.x; li < nlist; li += gridDim.x) {
     const uint32_t G = g.cg_n[li];
     if (G == 0) continue;
-    const uint32_t r = list[li];
     const CgGroup* cg = g.cg + (uint64_t)li * (CG_GMAX + 1);
     if (tid <= G) {
       st[tid] = cg[tid].start;
@@ -2808,7 +2859,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
           uint64_t* dk = in_key ? tmp : key;
           for (uint32_t s0 = 0; s0 < cnt; s0 += 2 * w) {
             const uint32_t na = min(w, cnt - s0), nb = min(w, cnt - s0 - na);
-            block_merge<false>(M, sk, nullptr, dk, nullptr, a0 + s0, na, nb, tid);
+            block_merge<false>(M, sk, nullptr, dk, nullptr, a0 + s0, na, nb, tid, CG_KEY_MASK);
           }
           __syncthreads();
           in_key = !in_key;
@@ -2843,10 +2894,17 @@ __device__ __forceinline__ uint32_t bm_next_start(const uint64_t* bm, uint32_t q
 // is known, the clusters counted above it and the ties, whose order is the word
 // order (a scan of the words' tie counts) and the bit order within a word.  The
 // chosen clusters' offset sums are then added up by the waves.  n < 64 BM_WORDS.
+// With features (fout set), the chosen clusters' prefilter statistics come from
+// the flags in their keys' bits 40..47 and each candidate's feature row is written
+// (pr: the read's quality and repeat figures; the rest is filled in here).
 constexpr uint32_t BM_WORDS = 2048;
+struct Top20Feat {
+  uint32_t nb[TOP_CANDIDATES], ms[TOP_CANDIDATES], mn[TOP_CANDIDATES], span[TOP_CANDIDATES], rcnt[TOP_CANDIDATES];
+};
 template <int T>
 __device__ __forceinline__ bool top20_bitmap(const uint64_t* key, uint64_t* bm, void* tabs, const CgGroup* grp,
-                                             uint32_t n, uint32_t L, Cand* out, uint8_t* ncand_out, uint32_t tid) {
+                                             uint32_t n, uint32_t L, Cand* out, uint8_t* ncand_out, uint32_t tid,
+                                             uint16_t* fout = nullptr, Top20Feat* tf = nullptr, PfRead pr = PfRead{}) {
   uint32_t* hist = reinterpret_cast<uint32_t*>(tabs);  // CNT_BINS counters
   uint32_t* misc = hist + CNT_BINS;  // [0] chosen, [1] T, [2] ties wanted, [3] nt, [4] T saturated, [5] ties so far, [6..] scans
   uint32_t* ocnt = misc + 16;        // chosen clusters: count, first key, offset sum
@@ -2862,7 +2920,7 @@ __device__ __forceinline__ bool top20_bitmap(const uint64_t* key, uint64_t* bm, 
     const uint64_t k = i < n ? key[i] : 0ull;
     uint64_t kp = shfl_up_u64(k, 1);
     if (lane == 0) kp = (i > 0 && i - 1 < n) ? key[i - 1] : 0ull;
-    const bool st = i < n ? (i == 0 || (k >> 32) != (kp >> 32) || (uint32_t)k - (uint32_t)kp > (uint32_t)MERGE_DISTANCE)
+    const bool st = i < n ? (i == 0 || cg_rank(k) != cg_rank(kp) || (uint32_t)k - (uint32_t)kp > (uint32_t)MERGE_DISTANCE)
                           : i == n;
     const uint64_t word = __ballot(st);
     if (lane == 0) bm[base >> 6] = word;
@@ -2981,22 +3039,46 @@ __device__ __forceinline__ bool top20_bitmap(const uint64_t* key, uint64_t* bm, 
     if (tid == 0) misc[5] += tot;
     team_sync<T>();
   }
-  // D. offset sums of the chosen clusters, one wave each
+  // D. offset sums of the chosen clusters, one wave each (with features: their
+  // flags' neighbour hits, class sum and least class, and the offsets' span)
   for (uint32_t e = wave; e < nt; e += T / 64) {
     const uint32_t p = ostart[e], c = ocnt[e];
     int64_t sum = 0;
-    for (uint32_t x = lane; x < c; x += 64) sum += (int64_t)(uint32_t)key[p + x];
+    uint32_t fnb = 0, fms = 0, fmn = 15;
+    for (uint32_t x = lane; x < c; x += 64) {
+      const uint64_t k = key[p + x];
+      sum += (int64_t)(uint32_t)k;
+      const uint32_t f = (uint32_t)(k >> CG_FLAG_SHIFT) & 0xFFu;
+      fnb += f >> 7;
+      fms += f & 15u;
+      fmn = min(fmn, f & 15u);
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) sum += shfl_xor_i64(sum, o);
+    if (fout) {
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        fnb += (uint32_t)__shfl_xor((int)fnb, o, 64);
+        fms += (uint32_t)__shfl_xor((int)fms, o, 64);
+        fmn = min(fmn, (uint32_t)__shfl_xor((int)fmn, o, 64));
+      }
+      if (lane == 0) {
+        tf->nb[e] = fnb;
+        tf->ms[e] = fms;
+        tf->mn[e] = fmn;
+        tf->span[e] = min((uint32_t)key[p + c - 1] - (uint32_t)key[p], 0xFFFFFFFu);
+      }
+    }
     if (lane == 0) osum[e] = (long long)sum;
   }
   team_sync<T>();
   // E. the candidates, ordered by (count desc, first key)
+  uint32_t rk = 0;
   if (tid < nt) {
     const uint32_t c = ocnt[tid], cs = ostart[tid];
-    uint32_t rk = 0;
     for (uint32_t j = 0; j < nt; j++) rk += (ocnt[j] > c || (ocnt[j] == c && ostart[j] < cs)) ? 1u : 0u;
-    const CgGroup& g = grp[(uint32_t)(key[cs] >> 32)];
+    if (fout) tf->rcnt[rk] = c;
+    const CgGroup& g = grp[cg_rank(key[cs])];
     const int64_t dsum = (int64_t)osum[tid] + (int64_t)c * ((int64_t)g.lo - DIAG_BIAS);
     const int64_t avg = dsum / (int64_t)c;
     const bool rev = g.key & 1u;
@@ -3009,6 +3091,36 @@ __device__ __forceinline__ bool top20_bitmap(const uint64_t* key, uint64_t* bm, 
     out[rk] = cd;
   }
   if (tid == 0) *ncand_out = (uint8_t)nt;
+  if (fout) {
+    team_sync<T>();
+    if (tid < nt) {
+      PfCluster pc;
+      pc.cnt = ocnt[tid];
+      pc.nb = tf->nb[tid];
+      pc.ms = tf->ms[tid];
+      pc.mn = tf->mn[tid];
+      pc.span = tf->span[tid];
+      pc.rev = (grp[cg_rank(key[ostart[tid]])].key & 1u) != 0;
+      PfRead q = pr;
+      q.c0 = tf->rcnt[0];
+      q.c1 = nt > 1 ? tf->rcnt[1] : 0u;
+      // clusters counted as the top one: every one above T is chosen; when the top
+      // count is T itself, all the ties are (hist holds them: T is below the last bin)
+      uint32_t ties = 0;
+      if (q.c0 == Tc) {
+        ties = hist[Tc];
+      } else {
+        for (uint32_t j = 0; j < nt; j++) ties += ocnt[j] == q.c0 ? 1u : 0u;
+      }
+      q.ties = ties;
+      uint32_t C = 0;
+      for (uint32_t b = 0; b < CNT_BINS; b++) C += hist[b];
+      q.C = C;
+      q.n = n;
+      q.L = L;
+      pf_feature_row(fout + rk * 32, rk, pc, q);
+    }
+  }
   team_sync<T>();
   return true;
 }
@@ -3017,6 +3129,8 @@ __global__ __launch_bounds__(256) void k_cg_select(ClusterArgs a, CgArgs g, cons
   __shared__ uint64_t lds[(TOP20_LDS_BYTES + 7) / 8];
   __shared__ uint64_t bm[BM_WORDS + 1];
   __shared__ CgGroup gs[CG_GMAX + 1];
+  __shared__ Top20Feat tf;
+  __shared__ uint32_t qs[4];
   const uint32_t tid = threadIdx.x;
   for (uint32_t li = blockIdx.x; li < nlist; li += gridDim.x) {
     const uint32_t G = g.cg_n[li];
@@ -3024,13 +3138,49 @@ __global__ __launch_bounds__(256) void k_cg_select(ClusterArgs a, CgArgs g, cons
     const uint32_t r = list[li];
     const CgGroup* cg = g.cg + (uint64_t)li * (CG_GMAX + 1);
     if (tid <= G) gs[tid] = cg[tid];
+    if (tid < 4) qs[tid] = 0;
     __syncthreads();
     const uint32_t L = (uint32_t)(a.R.seq_off[r + 1] - a.R.seq_off[r]);
     const uint32_t n = a.nhits[r];
     Cand* out = a.cands + (uint64_t)r * TOP_CANDIDATES;
     const uint64_t* key = g.ck + g.ck_off[li];
-    const bool ok = n < 64 * BM_WORDS ? top20_bitmap<256>(key, bm, lds, gs, n, L, out, a.ncand + r, tid)
-                                      : top20_ordered<256, true>(key, lds, gs, n, L, out, a.ncand + r, tid);
+    PfRead pr{};
+    if (a.feat) {  // the read's quality sum, bases below Q10 and Q20, and repeat-skipped windows
+      const uint64_t qo = a.R.qual_off[r];
+      pr.qlen = (uint32_t)(a.R.qual_off[r + 1] - qo);
+      uint32_t q0 = 0, q10 = 0, q20 = 0, rep = 0;
+      for (uint32_t x = tid; x < pr.qlen; x += 256) {
+        const uint32_t q = a.R.qual[qo + x];
+        q0 += q;
+        q10 += q < 10;
+        q20 += q < 20;
+      }
+      const uint32_t nwin = L >= (uint32_t)K ? L - K + 1 : 0;
+      const uint8_t* rc = a.rcode + a.R.seq_off[r];
+      for (uint32_t x = tid; x < nwin; x += 256) rep += rc[x] != 0;
+      q0 = group_sum_u32<64>(q0);
+      q10 = group_sum_u32<64>(q10);
+      q20 = group_sum_u32<64>(q20);
+      rep = group_sum_u32<64>(rep);
+      if (lane_id() == 0) {
+        atomicAdd(&qs[0], q0);
+        atomicAdd(&qs[1], q10);
+        atomicAdd(&qs[2], q20);
+        atomicAdd(&qs[3], rep);
+      }
+      __syncthreads();
+      pr.qsum = qs[0];
+      pr.q10 = qs[1];
+      pr.q20 = qs[2];
+      pr.rep = qs[3];
+      pr.G = G;
+    }
+    bool ok;
+    if (n < 64 * BM_WORDS)
+      ok = top20_bitmap<256>(key, bm, lds, gs, n, L, out, a.ncand + r, tid,
+                             a.feat ? a.feat + (uint64_t)r * TOP_CANDIDATES * MLP_IN : nullptr, &tf, pr);
+    else  // the walk over the keys (no features: such a read goes to the general tier with them)
+      ok = !a.feat && top20_ordered<256, true>(key, lds, gs, n, L, out, a.ncand + r, tid);
     if (!ok) {
       // the 20th count lies in the saturated bin (20 clusters of 1023 hits or
       // more): the general tier takes the read from its intact hits
