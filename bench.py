@@ -185,13 +185,14 @@ def stage_rooflines(agg, B, workload):
     cells = agg["dp_cells"] / n_dp
     seed = {"kernel": "k_seed", "bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBPS,
             "achieved": round(gather_bytes / (ms_seed * 1e-3) / 1e9, 2) if ms_seed > 0 else 0.0,
-            "traffic": measured_traffic("pbw::k_seed", B, workload),
+            "traffic": measured_traffic("pbw::k_seed<false>", B, workload),
             "per_launch": {"bytes": gather_bytes, "ms": ms_seed,
                            "unit": "8 B per lookup + 8 B per position + 8 B per filtered hit"}}
     cluster = {"kernel": "k_cluster (all tiers)", "bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBPS,
                "achieved": round(cl_bytes / (ms_cl * 1e-3) / 1e9, 2) if ms_cl > 0 else 0.0,
                "traffic": measured_traffic(("pbw::k_cluster<false 128u>", "pbw::k_cluster<false 512u>",
-                                            "pbw::k_cluster_tier2<false>"), B, workload),
+                                            "pbw::k_cg_group", "pbw::k_cg_sort", "pbw::k_cg_select"),
+                                           B, workload),
                "per_launch": {"bytes": cl_bytes, "ms": ms_cl,
                               "unit": "8 B per hit key read + 321 B of candidates per read written"}}
     ach = cells / (ms_dp * 1e-3) * DP_OPS_PER_CELL / 1e12 if ms_dp > 0 else 0.0
