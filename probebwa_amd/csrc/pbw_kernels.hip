@@ -2456,20 +2456,23 @@ void launch_cluster_tier2(const ClusterArgs& a, const uint32_t* list, uint32_t n
 
 // ---------------------------------------------------------------------------
 // Block tier in three launches, for reads of at most CG_GMAX groups whose
-// diagonals each span less than 2^32 - 1 (features off; at genome scale every
-// read): each phase runs at the occupancy its own registers and LDS allow,
-// where one kernel for all of them would run every phase at the lowest.
+// diagonals each span less than 2^32 - 1 (at genome scale every read): each phase
+// runs at the occupancy its own registers and LDS allow, where one kernel for all
+// of them would run every phase at the lowest.
 //  k_cg_group:  the group table (group_table_count), the groups ranked in
 //               FxHashMap iteration order from their first emission indices,
 //               segments laid out by rank, every hit scattered into its
 //               segment as a composite key (rank << 32 | biased diagonal - the
-//               group's lowest);
+//               group's lowest; with features, the hit's flags in bits 40..47);
 //  k_cg_sort:   runs of whole segments of up to RSORT_TILE keys sorted by one
-//               block radix sort of the 64-bit composite keys (8-bit digits,
-//               five passes over the 38 bits that vary); a larger segment tile
-//               by tile on its low 32 bits, the tiles then merged pairwise
-//               (block_merge) through a second buffer;
-//  k_cg_select: top20_ordered over the sorted keys.
+//               bucket pass over their offsets (cg_bucket_sort), or, when a
+//               bucket holds over 512 keys, by one block radix sort of the
+//               composite keys (8-bit digits over the 38 bits that vary); a
+//               larger segment tile by tile on its low 32 bits, the tiles then
+//               merged pairwise (block_merge) through a second buffer;
+//  k_cg_select: the top 20 from a bitmap of cluster starts (top20_bitmap; the
+//               key walk top20_ordered for reads of 2^17 hits or more), and
+//               with features each candidate's feature row.
 // A read that fails the conditions (or whose 20th count lies in the selection's
 // saturated bin) is listed for k_cluster_tier2, its hits left intact.
 // ---------------------------------------------------------------------------
@@ -2844,7 +2847,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       const uint32_t cnt = st[q1] - a0;
       if (cnt <= RSORT_TILE) {
         if (cnt > 1 && !cg_bucket_sort(key + a0, cnt, q, q1, st, rb, u.bs, tid)) {
-          // a bucket of over 1024 keys: the smallest radix-sort tile that holds the run
+          // a bucket of over 512 keys: the smallest radix-sort tile that holds the run
           if (cnt <= 2 * 256) cg_run_sort<2>(key + a0, cnt, RUN_BITS, rs, tid);
           else if (cnt <= 4 * 256) cg_run_sort<4>(key + a0, cnt, RUN_BITS, rs, tid);
           else if (cnt <= 8 * 256) cg_run_sort<8>(key + a0, cnt, RUN_BITS, rs, tid);
