@@ -2700,10 +2700,44 @@ __device__ __forceinline__ void wave_bitonic_n(uint64_t* key, uint32_t s, uint64
 // bucket order is key order and a bucket holds one or two keys; every key is counted
 // into its bucket, the buckets' ends come from a block scan, every key is placed
 // in LDS, and then each bucket of two to CG_BSMALL keys is put in order by one
-// thread (insertion), each larger one of up to 512 by a wave (bitonic in
-// registers).  Returns false, with nothing written, when a bucket holds more than
-// 512 keys (the caller then sorts the run by radix).
+// lane (a sorting network in registers, bucket_net), each larger one of up to 512
+// by a wave (bitonic in registers).  One segment's tile of a larger segment is
+// sorted the same way (one_cnt: its key count, q1 = q0 + 1).  Returns false, with
+// nothing written, when a bucket holds more than 512 keys (the caller then sorts
+// the run by radix).  The keys of one bucket share their rank, so within a bucket
+// they compare by their offsets (bits 0..31) alone.
 constexpr uint32_t CG_BTILE = 4096, CG_BSMALL = 16, CG_NBMAX = CG_BTILE;
+static_assert(RSORT_TILE <= CG_BTILE, "a radix tile fits the bucket sort");
+// The c <= N keys at p[0, c) put in order by one lane: loaded together, a bitonic
+// network over N registers (positions from c on padded with offset 0xFFFFFFFF,
+// above every offset of this tier), the first c stored back.  c == 0: no access.
+template <int N>
+__device__ __forceinline__ void bucket_net(uint64_t* p, uint32_t c) {
+  uint64_t r[N];
+#pragma unroll
+  for (int i = 0; i < N; i++) r[i] = (uint32_t)i < c ? p[i] : 0xFFFFFFFFull;
+#pragma unroll
+  for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+      for (int i = 0; i < N; i++) {
+        const int l = i ^ j;
+        if (l > i) {
+          const bool up = (i & k) == 0;
+          const uint64_t x = r[i], y = r[l];
+          if (((uint32_t)x > (uint32_t)y) == up) {
+            r[i] = y;
+            r[l] = x;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < N; i++)
+    if ((uint32_t)i < c) p[i] = r[i];
+}
 struct CgBucketLds {
   uint64_t keys[CG_BTILE];
   uint32_t end[CG_NBMAX];   // bucket counts, then (scan) each bucket's end
@@ -2712,13 +2746,14 @@ struct CgBucketLds {
   uint32_t nbig, fail, scan[8];
 };
 __device__ __forceinline__ bool cg_bucket_sort(uint64_t* key, uint32_t cnt, uint32_t q0, uint32_t q1, const uint32_t* st,
-                                               const uint32_t* rng_bits, CgBucketLds& B, uint32_t tid) {
+                                               const uint32_t* rng_bits, CgBucketLds& B, uint32_t tid,
+                                               uint32_t one_cnt = 0) {
   constexpr uint32_t IPT = CG_BTILE / 256;
   // buckets per segment
   if (tid == 0) {
     uint32_t nb = 0;
     for (uint32_t q = q0; q < q1; q++) {
-      const uint32_t c = st[q + 1] - st[q];
+      const uint32_t c = one_cnt ? one_cnt : st[q + 1] - st[q];
       uint32_t lg = 0;  // 2^lg <= c buckets: about one or two keys each
       while ((2u << lg) <= c) lg++;
       B.base[q - q0] = nb;
@@ -2767,7 +2802,10 @@ __device__ __forceinline__ bool cg_bucket_sort(uint64_t* key, uint32_t cnt, uint
     }
   }
   __syncthreads();
-  if (B.fail) return false;
+  if (B.fail) {  // every thread has read the flag before the caller reuses the LDS
+    __syncthreads();
+    return false;
+  }
   // place: a bucket's keys fill it from its end backwards
 #pragma unroll
   for (uint32_t j = 0; j < IPT; j++) {
@@ -2775,21 +2813,20 @@ __device__ __forceinline__ bool cg_bucket_sort(uint64_t* key, uint32_t cnt, uint
     if (pos < cnt) B.keys[atomicSub(&B.end[bk[j]], 1u) - 1u] = kk[j];
   }
   __syncthreads();
-  // B.end[b] is now bucket b's start; its end is the next bucket's start (cnt for the last)
-  for (uint32_t b = tid; b < NB; b += 256) {
-    const uint32_t s0 = B.end[b], s1 = b + 1 < NB ? B.end[b + 1] : cnt;
-    const uint32_t c = s1 - s0;
-    if (c >= 2 && c <= CG_BSMALL) {
-      for (uint32_t x = s0 + 1; x < s1; x++) {
-        const uint64_t v = B.keys[x];
-        uint32_t y = x;
-        while (y > s0 && (B.keys[y - 1] & CG_KEY_MASK) > (v & CG_KEY_MASK)) {
-          B.keys[y] = B.keys[y - 1];
-          y--;
-        }
-        B.keys[y] = v;
-      }
+  // B.end[b] is now bucket b's start; its end is the next bucket's start (cnt for
+  // the last).  Each wave sizes its network for the largest small bucket it holds.
+  for (uint32_t b0 = 0; b0 < NB; b0 += 256) {
+    const uint32_t b = b0 + tid;
+    uint32_t s0 = 0, c = 0;
+    if (b < NB) {
+      s0 = B.end[b];
+      c = (b + 1 < NB ? B.end[b + 1] : cnt) - s0;
+      if (c < 2 || c > CG_BSMALL) c = 0;
     }
+    if (__ballot(c > 8)) bucket_net<16>(B.keys + s0, c);
+    else if (__ballot(c > 4)) bucket_net<8>(B.keys + s0, c);
+    else if (__ballot(c > 2)) bucket_net<4>(B.keys + s0, c);
+    else if (__ballot(c != 0)) bucket_net<2>(B.keys + s0, c);
   }
   for (uint32_t i = tid >> 6; i < B.nbig; i += 4) {
     const uint32_t b = B.big[i];
@@ -2854,8 +2891,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
           else cg_run_sort<16>(key + a0, cnt, RUN_BITS, rs, tid);
         }
       } else {  // one segment over a tile: its keys share the rank bits
-        for (uint32_t t0 = 0; t0 < cnt; t0 += RSORT_TILE)
-          cg_run_sort<16>(key + a0 + t0, min(RSORT_TILE, cnt - t0), 32u, rs, tid);
+        for (uint32_t t0 = 0; t0 < cnt; t0 += RSORT_TILE) {
+          const uint32_t tc = min(RSORT_TILE, cnt - t0);
+          if (!cg_bucket_sort(key + a0 + t0, tc, q, q + 1, st, rb, u.bs, tid, tc))
+            cg_run_sort<16>(key + a0 + t0, tc, 32u, rs, tid);
+        }
         bool in_key = true;
         for (uint32_t w = RSORT_TILE; w < cnt; w <<= 1) {
           const uint64_t* sk = in_key ? key : tmp;
