@@ -204,6 +204,7 @@ struct pbw_ctx {
   DevBuf<int32_t> d_rows;
   DevBuf<uint8_t> d_gstage;  // DP staging in device memory for tiles beyond LDS
   DevBuf<uint64_t> d_ck, d_cktmp, d_ck_off;  // block-tier composite keys, the merges' second buffer, regions
+  DevBuf<uint64_t> d_cbm, d_cbm_off;        // block-tier cluster-start bitmaps and each read's first word
   DevBuf<CgGroup> d_cg;
   DevBuf<uint32_t> d_cg_n, d_fb;
   DevBuf<uint64_t> d_tb;
@@ -833,10 +834,16 @@ static void batch_candidates(pbw_ctx* c, Batch& b, bool feat = false, HostRecord
       c->d_cg.ensure((size_t)nt2 * (CG_GMAX + 1));
       c->d_cg_n.ensure(nt2);
       c->d_fb.ensure(nt2);
-      c->d_ck_off.ensure(nt2 + 1);  // [nt2]: the region counter
-      HIP_CHECK(hipMemsetAsync(c->d_ck_off.p + nt2, 0, 8, s));
+      c->d_ck_off.ensure(nt2 + 2);  // [nt2]: the key region counter, [nt2 + 1]: the bitmap's
+      HIP_CHECK(hipMemsetAsync(c->d_ck_off.p + nt2, 0, 16, s));
+      {  // a read's bitmap: (n >> 6) + 1 words
+        const uint64_t bw = (all_hits >> 6) + nt2;
+        if (c->d_cbm.n < bw) c->d_cbm.ensure(bw + bw / 4);
+      }
+      c->d_cbm_off.ensure(nt2);
       CgArgs g{c->d_ck.p, c->d_cktmp.p, reinterpret_cast<unsigned long long*>(c->d_ck_off.p + nt2), c->d_ck_off.p,
-               c->d_cg.p, c->d_cg_n.p, c->d_fb.p, c->d_ntier2.p + 2};
+               c->d_cg.p, c->d_cg_n.p, c->d_fb.p, c->d_ntier2.p + 2, c->d_cbm.p,
+               reinterpret_cast<unsigned long long*>(c->d_ck_off.p + nt2 + 1), c->d_cbm_off.p};
       launch_cluster_grid(ca, g, c->d_tier2.p, nt2, s);
       HIP_CHECK(hipGetLastError());
       nt2b = download_u32(c, c->d_ntier2.p + 2);

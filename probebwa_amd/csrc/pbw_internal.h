@@ -200,6 +200,9 @@ struct CgArgs {
   uint32_t* cg_n;
   uint32_t* fb_list;
   uint32_t* fb_n;
+  uint64_t* bm;            // cluster-start bitmaps: each read's (n >> 6) + 1 words taken from *bm_next
+  unsigned long long* bm_next;
+  uint64_t* bm_off;        // per list entry: the read's first bitmap word
 };
 void launch_cluster_grid(const ClusterArgs& a, const CgArgs& g, const uint32_t* list, uint32_t n, hipStream_t s);
 uint64_t cluster_scratch_words(uint32_t max_hits);  // per-block scratch of the block tier (u64 words)

@@ -2571,8 +2571,11 @@ __global__ __launch_bounds__(256) void k_cg_group(ClusterArgs a, CgArgs g, const
       g.cg_n[li] = G;
       sk[0] = atomicAdd(g.ck_next, (unsigned long long)n);  // this read's region in ck and tmp
       g.ck_off[li] = sk[0];
+      sk[1] = atomicAdd(g.bm_next, (unsigned long long)((n >> 6) + 1));  // and its start bitmap
+      g.bm_off[li] = sk[1];
     }
     __syncthreads();
+    for (uint32_t w = tid; w <= (n >> 6); w += 256) g.bm[sk[1] + w] = 0;
     uint64_t* ck = g.ck + sk[0];
     constexpr uint32_t U = 8;
     for (uint32_t i0 = tid; i0 < n; i0 += U * 256) {
@@ -2694,6 +2697,35 @@ __device__ __forceinline__ void wave_bitonic_n(uint64_t* key, uint32_t s, uint64
   }
 }
 
+// The cluster-start bits of the sorted keys at read positions [a0, a0 + cnt), a run
+// of whole segments (get(i): the key at a0 + i): bit p set when key p starts a
+// group or lies more than MERGE_DISTANCE above key p - 1 (the first key of the
+// run starts its group).  One wave per bitmap word; the words the run shares with
+// its neighbours are ORed in (the bitmap starts zeroed), the rest stored.
+template <class Get>
+__device__ __forceinline__ void cg_start_bits(Get get, uint32_t a0, uint32_t cnt, uint64_t* bmg, uint32_t tid) {
+  const uint32_t lane = tid & 63u, wave = tid >> 6;
+  const uint32_t w0 = a0 >> 6, w1 = (a0 + cnt - 1) >> 6;
+  for (uint32_t w = w0 + wave; w <= w1; w += 4) {
+    const uint32_t p = (w << 6) + lane;
+    bool sb = false;
+    if (p >= a0 && p < a0 + cnt) {
+      const uint32_t i = p - a0;
+      if (i == 0) {
+        sb = true;
+      } else {
+        const uint64_t k = get(i), kp = get(i - 1);
+        sb = cg_rank(k) != cg_rank(kp) || (uint32_t)k - (uint32_t)kp > (uint32_t)MERGE_DISTANCE;
+      }
+    }
+    const uint64_t word = __ballot(sb);
+    if (lane == 0) {
+      if (w > w0 && w < w1) bmg[w] = word;
+      else if (word) atomicOr(reinterpret_cast<unsigned long long*>(bmg + w), (unsigned long long)word);
+    }
+  }
+}
+
 // A run of whole segments (ranks q0..q1-1, cnt <= CG_BTILE keys at key[0, cnt))
 // sorted by one counting pass over buckets: segment q gets the largest power of two
 // of buckets not above its key count, each a 2^sh_q-wide slice of its offsets, so
@@ -2702,7 +2734,8 @@ __device__ __forceinline__ void wave_bitonic_n(uint64_t* key, uint32_t s, uint64
 // in LDS, and then each bucket of two to CG_BSMALL keys is put in order by one
 // lane (a sorting network in registers, bucket_net), each larger one of up to 512
 // by a wave (bitonic in registers).  One segment's tile of a larger segment is
-// sorted the same way (one_cnt: its key count, q1 = q0 + 1).  Returns false, with
+// sorted the same way (one_cnt: its key count, q1 = q0 + 1).  With bmg set, the
+// run's cluster-start bits (read positions from a0 on) are written from LDS.  Returns false, with
 // nothing written, when a bucket holds more than 512 keys (the caller then sorts
 // the run by radix).  The keys of one bucket share their rank, so within a bucket
 // they compare by their offsets (bits 0..31) alone.
@@ -2747,7 +2780,7 @@ struct CgBucketLds {
 };
 __device__ __forceinline__ bool cg_bucket_sort(uint64_t* key, uint32_t cnt, uint32_t q0, uint32_t q1, const uint32_t* st,
                                                const uint32_t* rng_bits, CgBucketLds& B, uint32_t tid,
-                                               uint32_t one_cnt = 0) {
+                                               uint32_t one_cnt = 0, uint64_t* bmg = nullptr, uint32_t a0 = 0) {
   constexpr uint32_t IPT = CG_BTILE / 256;
   // buckets per segment
   if (tid == 0) {
@@ -2842,6 +2875,7 @@ __device__ __forceinline__ bool cg_bucket_sort(uint64_t* key, uint32_t cnt, uint
     const uint32_t pos = j * 256 + tid;
     if (pos < cnt) key[pos] = B.keys[pos];
   }
+  if (bmg) cg_start_bits([&](uint32_t i) { return B.keys[i]; }, a0, cnt, bmg, tid);
   __syncthreads();
   return true;
 }
@@ -2876,6 +2910,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     __syncthreads();
     uint64_t* key = g.ck + g.ck_off[li];
     uint64_t* tmp = g.tmp + g.ck_off[li];
+    uint64_t* bmg = g.bm + g.bm_off[li];
     uint32_t q = 0;
     while (q < G) {
       const uint32_t a0 = st[q];
@@ -2883,12 +2918,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       while (q1 < G && st[q1 + 1] - a0 <= RSORT_TILE) q1++;
       const uint32_t cnt = st[q1] - a0;
       if (cnt <= RSORT_TILE) {
-        if (cnt > 1 && !cg_bucket_sort(key + a0, cnt, q, q1, st, rb, u.bs, tid)) {
+        if (cnt == 1) {
+          cg_start_bits([&](uint32_t i) { return key[a0 + i]; }, a0, cnt, bmg, tid);
+        } else if (!cg_bucket_sort(key + a0, cnt, q, q1, st, rb, u.bs, tid, 0, bmg, a0)) {
           // a bucket of over 512 keys: the smallest radix-sort tile that holds the run
           if (cnt <= 2 * 256) cg_run_sort<2>(key + a0, cnt, RUN_BITS, rs, tid);
           else if (cnt <= 4 * 256) cg_run_sort<4>(key + a0, cnt, RUN_BITS, rs, tid);
           else if (cnt <= 8 * 256) cg_run_sort<8>(key + a0, cnt, RUN_BITS, rs, tid);
           else cg_run_sort<16>(key + a0, cnt, RUN_BITS, rs, tid);
+          cg_start_bits([&](uint32_t i) { return key[a0 + i]; }, a0, cnt, bmg, tid);
         }
       } else {  // one segment over a tile: its keys share the rank bits
         for (uint32_t t0 = 0; t0 < cnt; t0 += RSORT_TILE) {
@@ -2911,6 +2949,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
           for (uint32_t i = tid; i < cnt; i += 256) key[a0 + i] = tmp[a0 + i];
           __syncthreads();
         }
+        cg_start_bits([&](uint32_t i) { return key[a0 + i]; }, a0, cnt, bmg, tid);
       }
       q = q1;
     }
@@ -2930,7 +2969,7 @@ __device__ __forceinline__ uint32_t bm_next_start(const uint64_t* bm, uint32_t q
 
 // top20_ordered's selection over composite keys, bit-parallel: a bitmap of the
 // cluster starts (bit i set when key i begins a cluster, bit n set as a
-// sentinel) built from coalesced loads, one ballot per 64 keys; a cluster of s
+// sentinel), loaded as k_cg_sort wrote it (bmg, cg_start_bits); a cluster of s
 // keys starting at p is a set bit p whose next set bit is p + s, so the clusters
 // of 1..CNT_REG keys are counted per 64-bit word with shifted masks, and only the
 // starts of larger ones are followed to their ends.  The same masks give, once T
@@ -2947,7 +2986,8 @@ struct Top20Feat {
 template <int T>
 __device__ __forceinline__ bool top20_bitmap(const uint64_t* key, uint64_t* bm, void* tabs, const CgGroup* grp,
                                              uint32_t n, uint32_t L, Cand* out, uint8_t* ncand_out, uint32_t tid,
-                                             uint16_t* fout = nullptr, Top20Feat* tf = nullptr, PfRead pr = PfRead{}) {
+                                             const uint64_t* bmg, uint16_t* fout = nullptr, Top20Feat* tf = nullptr,
+                                             PfRead pr = PfRead{}) {
   uint32_t* hist = reinterpret_cast<uint32_t*>(tabs);  // CNT_BINS counters
   uint32_t* misc = hist + CNT_BINS;  // [0] chosen, [1] T, [2] ties wanted, [3] nt, [4] T saturated, [5] ties so far, [6..] scans
   uint32_t* ocnt = misc + 16;        // chosen clusters: count, first key, offset sum
@@ -2957,17 +2997,8 @@ __device__ __forceinline__ bool top20_bitmap(const uint64_t* key, uint64_t* bm, 
   const uint32_t nw = n / 64 + 1;  // words holding bits 0..n
   for (uint32_t i = tid; i < CNT_BINS; i += T) hist[i] = 0;
   if (tid < 16) misc[tid] = 0;
-  // A. the start bitmap: wave w takes positions [base, base + 64) of each round
-  for (uint32_t base = wave * 64; base < nw * 64; base += T) {
-    const uint32_t i = base + lane;
-    const uint64_t k = i < n ? key[i] : 0ull;
-    uint64_t kp = shfl_up_u64(k, 1);
-    if (lane == 0) kp = (i > 0 && i - 1 < n) ? key[i - 1] : 0ull;
-    const bool st = i < n ? (i == 0 || cg_rank(k) != cg_rank(kp) || (uint32_t)k - (uint32_t)kp > (uint32_t)MERGE_DISTANCE)
-                          : i == n;
-    const uint64_t word = __ballot(st);
-    if (lane == 0) bm[base >> 6] = word;
-  }
+  // A. the start bitmap, and the sentinel bit n
+  for (uint32_t w = tid; w < nw; w += T) bm[w] = bmg[w] | (w == (n >> 6) ? 1ull << (n & 63u) : 0ull);
   team_sync<T>();
   const uint32_t nwd = n >> 6;
   const uint64_t sent = 1ull << (n & 63u);  // the sentinel bit, in word nwd
@@ -3220,7 +3251,7 @@ __global__ __launch_bounds__(256) void k_cg_select(ClusterArgs a, CgArgs g, cons
     }
     bool ok;
     if (n < 64 * BM_WORDS)
-      ok = top20_bitmap<256>(key, bm, lds, gs, n, L, out, a.ncand + r, tid,
+      ok = top20_bitmap<256>(key, bm, lds, gs, n, L, out, a.ncand + r, tid, g.bm + g.bm_off[li],
                              a.feat ? a.feat + (uint64_t)r * TOP_CANDIDATES * MLP_IN : nullptr, &tf, pr);
     else  // the walk over the keys (no features: such a read goes to the general tier with them)
       ok = !a.feat && top20_ordered<256, true>(key, lds, gs, n, L, out, a.ncand + r, tid);
