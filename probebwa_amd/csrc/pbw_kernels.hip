@@ -2704,24 +2704,31 @@ __device__ __forceinline__ void wave_bitonic_n(uint64_t* key, uint32_t s, uint64
 // its neighbours are ORed in (the bitmap starts zeroed), the rest stored.
 template <class Get>
 __device__ __forceinline__ void cg_start_bits(Get get, uint32_t a0, uint32_t cnt, uint64_t* bmg, uint32_t tid) {
+  constexpr uint32_t U = 4;  // words per wave and round: their loads issued together
   const uint32_t lane = tid & 63u, wave = tid >> 6;
   const uint32_t w0 = a0 >> 6, w1 = (a0 + cnt - 1) >> 6;
-  for (uint32_t w = w0 + wave; w <= w1; w += 4) {
-    const uint32_t p = (w << 6) + lane;
-    bool sb = false;
-    if (p >= a0 && p < a0 + cnt) {
+  for (uint32_t wb = w0 + wave * U; wb <= w1; wb += 4 * U) {
+    uint64_t k[U], kp[U];
+    bool in[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++) {
+      const uint32_t p = ((wb + u) << 6) + lane;
+      in[u] = wb + u <= w1 && p >= a0 && p < a0 + cnt;
       const uint32_t i = p - a0;
-      if (i == 0) {
-        sb = true;
-      } else {
-        const uint64_t k = get(i), kp = get(i - 1);
-        sb = cg_rank(k) != cg_rank(kp) || (uint32_t)k - (uint32_t)kp > (uint32_t)MERGE_DISTANCE;
-      }
+      k[u] = in[u] ? get(i) : 0ull;
+      kp[u] = in[u] && i > 0 ? get(i - 1) : 0ull;
     }
-    const uint64_t word = __ballot(sb);
-    if (lane == 0) {
-      if (w > w0 && w < w1) bmg[w] = word;
-      else if (word) atomicOr(reinterpret_cast<unsigned long long*>(bmg + w), (unsigned long long)word);
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++) {
+      const uint32_t w = wb + u;
+      const uint32_t p = (w << 6) + lane;
+      const bool sb = in[u] && (p == a0 || cg_rank(k[u]) != cg_rank(kp[u]) ||
+                                (uint32_t)k[u] - (uint32_t)kp[u] > (uint32_t)MERGE_DISTANCE);
+      const uint64_t word = __ballot(sb);
+      if (lane == 0 && w <= w1) {
+        if (w > w0 && w < w1) bmg[w] = word;
+        else if (word) atomicOr(reinterpret_cast<unsigned long long*>(bmg + w), (unsigned long long)word);
+      }
     }
   }
 }
@@ -2780,7 +2787,8 @@ struct CgBucketLds {
 };
 __device__ __forceinline__ bool cg_bucket_sort(uint64_t* key, uint32_t cnt, uint32_t q0, uint32_t q1, const uint32_t* st,
                                                const uint32_t* rng_bits, CgBucketLds& B, uint32_t tid,
-                                               uint32_t one_cnt = 0, uint64_t* bmg = nullptr, uint32_t a0 = 0) {
+                                               uint32_t one_cnt = 0, uint64_t* bmg = nullptr, uint32_t a0 = 0,
+                                               uint64_t* bits = nullptr) {
   constexpr uint32_t IPT = CG_BTILE / 256;
   // buckets per segment
   if (tid == 0) {
@@ -2870,13 +2878,48 @@ __device__ __forceinline__ bool cg_bucket_sort(uint64_t* key, uint32_t cnt, uint
     else wave_bitonic_n<8>(B.keys + s0, c, CG_KEY_MASK);
   }
   __syncthreads();
+  if (!bmg) {
+#pragma unroll
+    for (uint32_t j = 0; j < IPT; j++) {
+      const uint32_t pos = j * 256 + tid;
+      if (pos < cnt) key[pos] = B.keys[pos];
+    }
+    __syncthreads();
+    return true;
+  }
+  // write-back with the cluster-start bits: each wave's ballot over 64 run positions
+  // ORed into the run's word-aligned LDS bitmap (bits, zero on entry), which then
+  // goes to the read's bitmap (the two end words ORed in) and is left zero
+  const uint32_t lane = tid & 63u, wb0 = a0 >> 6;
 #pragma unroll
   for (uint32_t j = 0; j < IPT; j++) {
     const uint32_t pos = j * 256 + tid;
-    if (pos < cnt) key[pos] = B.keys[pos];
+    bool sb = false;
+    if (pos < cnt) {
+      const uint64_t k = B.keys[pos];
+      key[pos] = k;
+      if (pos == 0) {
+        sb = true;
+      } else {
+        const uint64_t kp = B.keys[pos - 1];
+        sb = cg_rank(k) != cg_rank(kp) || (uint32_t)k - (uint32_t)kp > (uint32_t)MERGE_DISTANCE;
+      }
+    }
+    const uint64_t m = __ballot(sb);
+    if (lane == 0 && m) {
+      const uint32_t p = a0 + pos, w = (p >> 6) - wb0, sh = p & 63u;
+      atomicOr(reinterpret_cast<unsigned long long*>(bits + w), (unsigned long long)(m << sh));
+      if (sh) atomicOr(reinterpret_cast<unsigned long long*>(bits + w + 1), (unsigned long long)(m >> (64u - sh)));
+    }
   }
-  if (bmg) cg_start_bits([&](uint32_t i) { return B.keys[i]; }, a0, cnt, bmg, tid);
   __syncthreads();
+  const uint32_t nwd = ((a0 + cnt - 1) >> 6) - wb0 + 1;
+  for (uint32_t w = tid; w < nwd; w += 256) {
+    const uint64_t v = bits[w];
+    bits[w] = 0;
+    if (w > 0 && w + 1 < nwd) bmg[wb0 + w] = v;
+    else if (v) atomicOr(reinterpret_cast<unsigned long long*>(bmg + wb0 + w), (unsigned long long)v);
+  }
   return true;
 }
 
@@ -2892,7 +2935,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     CgBucketLds bs;
   } u;
   __shared__ uint32_t st[CG_GMAX + 1], rb[CG_GMAX + 1];
+  __shared__ uint64_t bits[CG_BTILE / 64 + 2];  // a run's start bits, word-aligned (zero between runs)
   CgSortLds& rs = u.r.rs;
+  for (uint32_t w = threadIdx.x; w < CG_BTILE / 64 + 2; w += 256) bits[w] = 0;
   ClusterMem M{};
   M.rsort = &rs;
   M.stage_val = u.r.stage_val;
@@ -2920,7 +2965,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       if (cnt <= RSORT_TILE) {
         if (cnt == 1) {
           cg_start_bits([&](uint32_t i) { return key[a0 + i]; }, a0, cnt, bmg, tid);
-        } else if (!cg_bucket_sort(key + a0, cnt, q, q1, st, rb, u.bs, tid, 0, bmg, a0)) {
+        } else if (!cg_bucket_sort(key + a0, cnt, q, q1, st, rb, u.bs, tid, 0, bmg, a0, bits)) {
           // a bucket of over 512 keys: the smallest radix-sort tile that holds the run
           if (cnt <= 2 * 256) cg_run_sort<2>(key + a0, cnt, RUN_BITS, rs, tid);
           else if (cnt <= 4 * 256) cg_run_sort<4>(key + a0, cnt, RUN_BITS, rs, tid);
