@@ -12,6 +12,7 @@
 #include <string>
 #include <mutex>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/probebwa_amd.h"
@@ -178,6 +179,7 @@ struct pbw_ctx {
   DevBuf<uint64_t> d_kpos;
   DevBuf<uint16_t> d_kfp;
   uint64_t total_entries = 0;
+  bool fused = false;  // d_kpos holds fused entries (fingerprint inside, d_kfp empty)
   const HostTable* table_src = nullptr;  // set while a context is built from a loaded `.sthash`
   // LUTs
   DevBuf<double> d_lut;  // packed double tables
@@ -282,14 +284,9 @@ static void ctx_build(pbw_ctx* c) {
   c->d_clen.upload(g->lengths.data(), g->lengths.size(), s);
   {  // contig -> first contig with the same name (paired.rs:188 compares names)
     std::vector<uint32_t> nc(g->names.size());
-    for (size_t i = 0; i < g->names.size(); i++) {
-      nc[i] = (uint32_t)i;
-      for (size_t j = 0; j < i; j++)
-        if (g->names[j] == g->names[i]) {
-          nc[i] = (uint32_t)j;
-          break;
-        }
-    }
+    std::unordered_map<std::string, uint32_t> first;
+    first.reserve(g->names.size());
+    for (size_t i = 0; i < g->names.size(); i++) nc[i] = first.emplace(g->names[i], (uint32_t)i).first->second;
     c->d_name_class.upload(nc.data(), nc.size(), s);
     HIP_CHECK(hipStreamSynchronize(s));
   }
@@ -300,12 +297,15 @@ static void ctx_build(pbw_ctx* c) {
     uint16_t* kfp = nullptr;
     uint64_t nent = 0;
     std::string err;
+    // fused entries (fingerprint inside) when every contig id and position fits them
+    c->fused = g->lengths.size() <= FUSED_MAX_CONTIGS;
+    for (uint64_t len : g->lengths) c->fused = c->fused && len <= FUSED_MAX_LEN;
     if (c->table_src) {  // HashTable::load (hashtable.rs:66-72)
       if (gpu_index_from_table(c->dev_genome(), g->lengths.data(), *c->table_src, c->d_koff.p, &kpos, &kfp, &nent,
-                               s, err) != hipSuccess)
+                               c->fused, s, err) != hipSuccess)
         throw PbwError{PBW_ERR_IO, "table: " + err};
-    } else if (gpu_build_index(c->dev_genome(), g->lengths.data(), c->d_koff.p, &kpos, &kfp, &nent, s, err) !=
-               hipSuccess) {
+    } else if (gpu_build_index(c->dev_genome(), g->lengths.data(), c->d_koff.p, &kpos, &kfp, &nent, c->fused, s,
+                               err) != hipSuccess) {
       throw PbwError{err.find("2^32") != std::string::npos ? PBW_ERR_LIMIT : PBW_ERR_HIP, err};
     }
     c->d_kpos.release();
@@ -313,7 +313,7 @@ static void ctx_build(pbw_ctx* c) {
     c->d_kpos.n = std::max<uint64_t>(nent, 1);
     c->d_kfp.release();
     c->d_kfp.p = kfp;
-    c->d_kfp.n = std::max<uint64_t>(nent, 1);
+    c->d_kfp.n = kfp ? std::max<uint64_t>(nent, 1) : 0;
     c->total_entries = nent;
   }
   c->d_pres.ensure(3 * CODE_SPACE / 64);  // natural, rotated by 10, rotated by 20
@@ -1909,6 +1909,8 @@ int pbw_index_save_sthash(pbw_ctx* c, const char* path) {
     t.pos.resize(c->total_entries);
     if (c->total_entries)
       HIP_CHECK(hipMemcpy(t.pos.data(), c->d_kpos.p, c->total_entries * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    if (c->fused)
+      for (uint64_t& e : t.pos) e = fused_to_packed(e);
     const uint64_t CH = 1ull << 24;
     std::vector<uint32_t> ko(CH + 1);
     for (uint64_t c0 = 0; c0 < CODE_SPACE; c0 += CH) {
@@ -2421,7 +2423,11 @@ int64_t pbw_index_lookup(pbw_ctx* c, uint32_t code, uint64_t* out, uint64_t cap)
     uint32_t off[2];
     HIP_CHECK(hipMemcpy(off, c->d_koff.p + code, 8, hipMemcpyDeviceToHost));
     uint64_t n = off[1] - off[0];
-    if (out && n) HIP_CHECK(hipMemcpy(out, c->d_kpos.p + off[0], std::min(n, cap) * 8, hipMemcpyDeviceToHost));
+    if (out && n) {
+      HIP_CHECK(hipMemcpy(out, c->d_kpos.p + off[0], std::min(n, cap) * 8, hipMemcpyDeviceToHost));
+      if (c->fused)
+        for (uint64_t k = 0; k < std::min(n, cap); k++) out[k] = fused_to_packed(out[k]);
+    }
     result = (int64_t)n;
   });
   return rc == PBW_OK ? result : rc;

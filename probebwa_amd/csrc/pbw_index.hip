@@ -146,8 +146,9 @@ __device__ __forceinline__ uint16_t genome_flank_fp(const uint64_t* g2b, const u
 }
 
 // The first MAX_COUNT entries of every run, decoded to packed positions
-// (pack_position, hashtable.rs:157-164: bit 63 strand, 62..40 contig, 39..0 position),
-// with each entry's genome flank fingerprint beside it.
+// (pack_position, hashtable.rs:157-164: bit 63 strand, 62..40 contig, 39..0 position)
+// with each entry's genome flank fingerprint beside it, or (kfp null) to fused
+// entries that hold the fingerprint.
 __global__ void k_idx_place(const uint32_t* __restrict__ key, const uint32_t* __restrict__ val, uint64_t n,
                             const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ koff,
                             const uint64_t* __restrict__ g2b, const uint64_t* __restrict__ nmask,
@@ -166,8 +167,13 @@ __global__ void k_idx_place(const uint32_t* __restrict__ key, const uint32_t* __
     uint32_t strand;
     slot_window(j - cbase[c], clen[c], pos, strand);
     const uint64_t at = (uint64_t)koff[k] + r;
-    kpos[at] = ((uint64_t)strand << 63) | ((uint64_t)c << 40) | pos;
-    kfp[at] = genome_flank_fp(g2b, nmask, coff[c], clen[c], pos);
+    const uint16_t fp = genome_flank_fp(g2b, nmask, coff[c], clen[c], pos);
+    if (kfp) {
+      kpos[at] = ((uint64_t)strand << 63) | ((uint64_t)c << 40) | pos;
+      kfp[at] = fp;
+    } else {
+      kpos[at] = fused_entry(strand, (uint64_t)c, fp, pos);
+    }
   }
 }
 
@@ -328,11 +334,16 @@ __global__ void k_tab_place(const uint32_t* __restrict__ key, const uint32_t* __
       if (c >= (uint64_t)nctg || pos + K > clen[c]) {
         atomicAdd(bad + 1, 1u);
         kpos[at + j] = 0;
-        kfp[at + j] = GENOME_FP_OPEN;
+        if (kfp) kfp[at + j] = GENOME_FP_OPEN;
         continue;
       }
-      kpos[at + j] = p;
-      kfp[at + j] = genome_flank_fp(g2b, nmask, coff[c], clen[c], pos);
+      const uint16_t fp = genome_flank_fp(g2b, nmask, coff[c], clen[c], pos);
+      if (kfp) {
+        kpos[at + j] = p;
+        kfp[at + j] = fp;
+      } else {
+        kpos[at + j] = fused_entry(p >> 63, c, fp, pos);
+      }
     }
   }
 }
@@ -356,7 +367,7 @@ struct Tmp {
   } while (0)
 
 hipError_t gpu_build_index(const DevGenome& G, const uint64_t* h_clen, uint32_t* koff, uint64_t** kpos_out,
-                           uint16_t** kfp_out, uint64_t* n_entries, hipStream_t s, std::string& err) {
+                           uint16_t** kfp_out, uint64_t* n_entries, bool fused, hipStream_t s, std::string& err) {
   *kpos_out = nullptr;
   *kfp_out = nullptr;
   *n_entries = 0;
@@ -415,12 +426,12 @@ hipError_t gpu_build_index(const DevGenome& G, const uint64_t* h_clen, uint32_t*
     IDX_CHECK(hipMemcpyAsync(&total, koff + CODE_SPACE, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     IDX_CHECK(hipStreamSynchronize(s));
     IDX_CHECK(hipMalloc((void**)&kpos, std::max<uint64_t>(total, 1) * sizeof(uint64_t)));
-    IDX_CHECK(hipMalloc((void**)&kfp, std::max<uint64_t>(total, 1) * sizeof(uint16_t)));
+    if (!fused) IDX_CHECK(hipMalloc((void**)&kfp, std::max<uint64_t>(total, 1) * sizeof(uint16_t)));
     hipLaunchKernelGGL(k_idx_place, grid, block, 0, s, k1, v1, n, rstart, koff, G.g2b, G.nmask, G.coff, G.clen, dcb,
                        nctg, kpos, kfp);
   } else {
     IDX_CHECK(hipMalloc((void**)&kpos, sizeof(uint64_t)));
-    IDX_CHECK(hipMalloc((void**)&kfp, sizeof(uint16_t)));
+    if (!fused) IDX_CHECK(hipMalloc((void**)&kfp, sizeof(uint16_t)));
   }
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -437,8 +448,8 @@ hipError_t gpu_build_index(const DevGenome& G, const uint64_t* h_clen, uint32_t*
 }
 
 hipError_t gpu_index_from_table(const DevGenome& G, const uint64_t* h_clen, const HostTable& t, uint32_t* koff,
-                                uint64_t** kpos_out, uint16_t** kfp_out, uint64_t* n_entries, hipStream_t s,
-                                std::string& err) {
+                                uint64_t** kpos_out, uint16_t** kfp_out, uint64_t* n_entries, bool fused,
+                                hipStream_t s, std::string& err) {
   (void)h_clen;
   *kpos_out = nullptr;
   *kfp_out = nullptr;
@@ -505,7 +516,7 @@ hipError_t gpu_index_from_table(const DevGenome& G, const uint64_t* h_clen, cons
   uint64_t* kpos = nullptr;
   uint16_t* kfp = nullptr;
   IDX_CHECK(hipMalloc((void**)&kpos, std::max<uint64_t>(placed, 1) * sizeof(uint64_t)));
-  IDX_CHECK(hipMalloc((void**)&kfp, std::max<uint64_t>(placed, 1) * sizeof(uint16_t)));
+  if (!fused) IDX_CHECK(hipMalloc((void**)&kfp, std::max<uint64_t>(placed, 1) * sizeof(uint16_t)));
   if (n) {
     hipLaunchKernelGGL(k_tab_place, grid, block, 0, s, dkey, doff, dcnt, n, dsrc, koff, G.g2b, G.nmask, G.coff,
                        G.clen, G.ncontigs, kpos, kfp, bad);

@@ -41,16 +41,29 @@ struct DevGenome {
   int32_t ncontigs;
 };
 
+// Index entries with their flank fingerprint inside, for genomes of at most 2^15
+// contigs of at most 2^32 bases: bit 63 strand, 62..48 contig, 47..32 fingerprint,
+// 31..0 position (one 8-byte load per entry in the seed pass; larger genomes keep
+// the reference's packing and the fingerprints in an array beside it).
+constexpr uint64_t FUSED_MAX_CONTIGS = 1ull << 15, FUSED_MAX_LEN = 1ull << 32;
+__host__ __device__ inline uint64_t fused_entry(uint64_t strand, uint64_t contig, uint64_t fp, uint64_t pos) {
+  return (strand << 63) | (contig << 48) | (fp << 32) | pos;
+}
+// a fused entry in the reference's packing (pack_position, hashtable.rs:157-164)
+__host__ __device__ inline uint64_t fused_to_packed(uint64_t e) {
+  return (e & (1ull << 63)) | (((e >> 48) & 0x7FFFull) << 40) | (e & 0xFFFFFFFFull);
+}
 struct DevIndex {
   const uint32_t* koff;  // [2^30 + 1] bucket start by 15-mer code; bucket = [koff[c], koff[c+1])
-  const uint64_t* kpos;  // packed positions: bit 63 strand, 62..40 contig, 39..0 position
+  const uint64_t* kpos;  // entries: fused (kfp null), else packed positions (bit 63 strand, 62..40 contig, 39..0 position)
   const uint64_t* pres;  // [2^24] bit c%64 of word c/64 set when bucket c is non-empty
   // the same bitmap indexed by the code rotated right by 10 and by 20 bits, so that
   // the 1-mismatch neighbours at window positions 5-9 (resp. 0-4) share the
   // original code's 1024-bit line, as positions 10-14 do in `pres`
   const uint64_t* pres_r10;
   const uint64_t* pres_r20;
-  const uint16_t* kfp;   // per entry of kpos: genome flank fingerprint (GENOME_FP_OPEN: flanks off the contig)
+  const uint16_t* kfp;   // per entry of kpos: genome flank fingerprint (GENOME_FP_OPEN: flanks off the
+                         // contig); null when the entries are fused
 };
 
 struct DevReads {
@@ -395,9 +408,10 @@ void launch_hits_gather(const uint64_t* hits, const uint8_t* hflag, const uint64
 // out[idx[k]] = val[k] for k < n
 void launch_scatter_u64(uint64_t* out, const uint32_t* idx, const uint64_t* val, uint32_t n, hipStream_t s);
 // Bucket table on the device (pbw_index.hip): fills koff[0 .. 2^30] and allocates
-// *kpos (hipMalloc, owned by the caller) with *n_entries packed positions.
+// *kpos (hipMalloc, owned by the caller) with *n_entries entries: fused ones when
+// `fused` (*kfp null), else packed positions with their fingerprints in *kfp.
 hipError_t gpu_build_index(const DevGenome& G, const uint64_t* h_clen, uint32_t* koff, uint64_t** kpos,
-                           uint16_t** kfp, uint64_t* n_entries, hipStream_t s, std::string& err);
+                           uint16_t** kfp, uint64_t* n_entries, bool fused, hipStream_t s, std::string& err);
 
 // A persisted `.sthash` (HashTable, src/index/hashtable.rs:53-72) read into host
 // arrays: per bucket its key (mix(code)), offset and count into `pos`.
@@ -414,9 +428,10 @@ bool write_sthash(const char* path, const HostTable& t, std::string& err);
 void set_global_error(const std::string& e);
 // The same bucket table built from a loaded HostTable instead of the genome:
 // koff by code (code = the inverse of mix of each key), kpos in bucket order,
-// kfp from the genome; every position is checked against the contigs.
+// fingerprints from the genome (fused or in kfp); every position is checked
+// against the contigs.
 hipError_t gpu_index_from_table(const DevGenome& G, const uint64_t* h_clen, const HostTable& t, uint32_t* koff,
-                                uint64_t** kpos, uint16_t** kfp, uint64_t* n_entries, hipStream_t s,
+                                uint64_t** kpos, uint16_t** kfp, uint64_t* n_entries, bool fused, hipStream_t s,
                                 std::string& err);
 // hashtable.rs:117-124 mix and its inverse (each step is a bijection of u32)
 __host__ __device__ inline uint32_t kmer_mix(uint32_t h) {

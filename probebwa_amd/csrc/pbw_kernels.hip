@@ -322,10 +322,19 @@ __device__ __forceinline__ void seed_positions(const SeedArgs& a, const SeedLds&
     const uint32_t excl = lo ? S.q_incl[lo - 1] : 0u;
     const uint64_t entry = (uint64_t)S.q_off[lo] + (e - excl);
     const uint64_t packed = a.X.kpos[entry];
-    const uint16_t gfp = a.X.kfp[entry];
-    const uint32_t contig = (uint32_t)((packed >> 40) & ((1ull << 23) - 1));
-    const uint64_t pos = packed & ((1ull << 40) - 1);
     const bool rev = (packed >> 63) & 1ull;
+    uint16_t gfp;
+    uint32_t contig;
+    uint64_t pos;
+    if (a.X.kfp) {
+      gfp = a.X.kfp[entry];
+      contig = (uint32_t)((packed >> 40) & ((1ull << 23) - 1));
+      pos = packed & ((1ull << 40) - 1);
+    } else {  // fused entry
+      gfp = (uint16_t)(packed >> 32);
+      contig = (uint32_t)(packed >> 48) & 0x7FFFu;
+      pos = packed & 0xFFFFFFFFull;
+    }
     const uint32_t fl = S.q_fl[lo], fp = S.q_fp[lo];
     pass = seed_filter(gfp, rev, (fl & 1u) != 0, (uint8_t)fl, (uint16_t)fp, (uint16_t)(fp >> 16));
     const int64_t rp = (int64_t)S.q_pos[lo];

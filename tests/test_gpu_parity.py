@@ -359,6 +359,32 @@ def test_map_se_many_contig_groups(n_contigs, lmax):
     compare_records(ro, rp, label="%d contigs" % n_contigs)
 
 
+def test_map_se_over_32768_contigs():
+    """A genome of 33,000 short contigs: more contigs than an index entry with the flank
+    fingerprint inside can name (2^15), so the index keeps the reference's packed positions
+    (hashtable.rs:157-164) with the fingerprints in an array beside them.  Records equal the
+    oracle's, and bucket lookups return the reference's packing."""
+    rng = np.random.default_rng(71)
+    contigs = [("u%d" % c, synth.random_bases(rng, int(rng.integers(40, 161)))) for c in range(33000)]
+    go, gp = pair_of_genomes(contigs)
+    ht = O.HashTable(go)
+    ctx = P.Context(gp)
+    assert ctx.total_entries == ht.total_entries
+    checked = 0
+    for _ in range(2000):
+        s = bytes(contigs[int(rng.integers(0, len(contigs)))][1])
+        p = int(rng.integers(0, len(s) - 15))
+        kmer = s[p:p + 15]
+        if b"N" in kmer:
+            continue
+        assert list(ctx.lookup_code(P.code_of(kmer))) == ht.lookup(kmer)
+        checked += 1
+    assert checked > 1500
+    seq, so, qual, qo = synth.sample_reads(contigs, 3000, 171, lmin=30, lmax=120)
+    ro_b, rp_b = both_reads(seq, so, qual, qo)
+    compare_records(O.Mapper(go, ht).map_se(ro_b), ctx.map_se(rp_b), label="33000 contigs")
+
+
 def test_map_se_edge_reads():
     """Empty reads, reads shorter than k, all-N reads, quality tracks of other lengths."""
     contigs = synth.small_genome(51, n_contigs=1, length=20000)
