@@ -170,6 +170,8 @@ struct pbw_ctx {
   int device;
   hipStream_t stream = nullptr;
   hipEvent_t ev[8] = {};  // [0,1] seed, [2,3] DP, [4,5] wave-tier and [6,7] block-tier clustering
+  hipStream_t stream2 = nullptr;  // the DP walks of one launch beside the next launch's rows
+  hipEvent_t dev[4] = {};         // [0,1] rows of a launch done (by buffer), [2,3] walks done (by buffer)
   std::string error;
   pbw_stats stats{};
   // genome + index
@@ -199,6 +201,7 @@ struct pbw_ctx {
   DevBuf<Cand> d_cands;
   DevBuf<uint8_t> d_ncand, d_njobs, d_rec, d_refbytes;
   DevBuf<uint32_t> d_flag_a, d_flag_b, d_flag_c, d_flag_n, d_sorted, d_sort_hist, d_tbq, d_flag_w, d_flag_wn;
+  DevBuf<int64_t> d_tbend;  // k_dp_quad's endpoint keys for k_dp_walk
   pbw_tune tune{0, 0, 0, 0};  // batch_reads mirrors opt.batch_reads; zeros elsewhere: defaults
   DevBuf<DpJob> d_jobs;
   DevBuf<DpOut> d_out;
@@ -271,6 +274,8 @@ static void ctx_build(pbw_ctx* c) {
   noalloc_setup(c->device);
   HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   for (auto& e : c->ev) HIP_CHECK(hipEventCreate(&e));
+  HIP_CHECK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+  for (auto& e : c->dev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   hipStream_t s = c->stream;
   if (g->lengths.size() >= (1u << 22)) throw PbwError{PBW_ERR_LIMIT, "more than 2^22 contigs"};
   for (uint64_t L : g->lengths)
@@ -984,15 +989,73 @@ static void batch_dp(pbw_ctx* c, Batch& b, const uint32_t* jlist, uint32_t nlist
       else quads.push_back(quad_args(0, n_long, b.max_m));
       quads.push_back(quad_args(n_long, nsorted - n_long, tn.split_len));
     }
-    size_t tbw = 0;
-    for (const DpArgs& q : quads)
-      if (q.max_m <= QUAD_MAX_M) tbw = std::max(tbw, quad_scratch_words(q.max_m, q.max_n, q.njobs, q.quad_grid));
-    c->d_tbq.ensure(std::max<size_t>(tbw, 1));
+    // every job of a four-jobs-per-wave launch keeps its traceback planes until
+    // k_dp_walk reads them: launches of at most QUAD_CHUNK_BYTES of planes, each
+    // cut where its reads fall below 3/4 of its longest, whose length sizes the
+    // launch's LDS tile (shorter reads, more waves per CU)
+    std::vector<DpArgs> chunks;
+    if (!quads.empty()) {
+      const uint32_t nkeys = (uint32_t)MAX_READ_LEN + 1;
+      std::vector<uint32_t> hend(nkeys);  // end of each length key in the sorted list (key = MAX_READ_LEN - length)
+      HIP_CHECK(hipMemcpy(hend.data(), c->d_sort_hist.p, nkeys * 4, hipMemcpyDeviceToHost));
+      for (const DpArgs& q : quads) {
+        const uint32_t base = (uint32_t)(q.jlist - c->d_sorted.p);
+        uint32_t lo = 0;
+        while (lo < q.njobs) {
+          const uint32_t k0 = (uint32_t)(std::upper_bound(hend.begin(), hend.end(), base + lo) - hend.begin());
+          const uint32_t len = (uint32_t)MAX_READ_LEN - std::min<uint32_t>(k0, MAX_READ_LEN);
+          DpArgs k = q;
+          k.max_m = std::max<uint32_t>(1, std::min(q.max_m, len));
+          k.max_n = std::min<uint32_t>(q.max_n, k.max_m + (uint32_t)right_pad + LEFT_PAD);
+          const uint64_t per = quad_scratch_words(k.max_m, k.max_n, 4, k.quad_grid);  // bytes per job
+          uint32_t cj = (uint32_t)std::max<uint64_t>(4, std::min<uint64_t>(q.njobs - lo, QUAD_CHUNK_BYTES / per));
+          // keys up to length 3/4 of the longest stay in this launch
+          const uint32_t kcut = std::min<uint32_t>(nkeys - 1, (uint32_t)MAX_READ_LEN - (len * 3 + 3) / 4);
+          if (len > 32 && hend[kcut] > base + lo) cj = std::min<uint32_t>(cj, hend[kcut] - (base + lo));
+          if (cj < q.njobs - lo) cj = std::max<uint32_t>(4, cj & ~3u);
+          k.jlist = q.jlist + lo;
+          k.njobs = std::min(cj, q.njobs - lo);
+          chunks.push_back(k);
+          lo += k.njobs;
+        }
+      }
+    }
+    // two scratch buffers: launch i's walks (k_dp_walk, k_dp_lk) run on the second
+    // stream while launch i + 1's rows run; launch i + 2 reuses the buffer once
+    // launch i's walks are done
+    size_t tbw = 0, tbe = 0;
+    for (const DpArgs& q : chunks)
+      if (q.max_m <= QUAD_MAX_M) {
+        tbw = std::max(tbw, quad_scratch_words(q.max_m, q.max_n, q.njobs, q.quad_grid));
+        tbe = std::max<size_t>(tbe, q.njobs);
+      }
+    const size_t nbuf = chunks.size() > 1 ? 2 : 1;
+    c->d_tbq.ensure(std::max<size_t>(tbw, 1) * nbuf);
+    c->d_tbend.ensure(std::max<size_t>(tbe, 1) * nbuf);
     HIP_CHECK(hipEventRecord(c->ev[2], s));
-    for (DpArgs& q : quads) {
-      q.tbq = c->d_tbq.p;
+    bool walks = false;
+    for (size_t ci = 0; ci < chunks.size(); ci++) {
+      DpArgs& q = chunks[ci];
+      const size_t bi = ci % nbuf;
+      q.tbq = c->d_tbq.p + bi * std::max<size_t>(tbw, 1);
+      q.tbend = c->d_tbend.p + bi * std::max<size_t>(tbe, 1);
+      if (ci >= nbuf) HIP_CHECK(hipStreamWaitEvent(s, c->dev[2 + bi], 0));  // the buffer's last walks
       // a launch whose tile does not fit LDS goes to the one-job-per-wave kernel
-      if (!launch_dp_quad(q, s)) bands.push_back(band_args((uint32_t)(q.jlist - c->d_sorted.p), q.njobs));
+      const int kind = launch_dp_quad(q, s);
+      if (kind == 0) bands.push_back(band_args((uint32_t)(q.jlist - c->d_sorted.p), q.njobs));
+      if (kind == 1) {
+        HIP_CHECK(hipEventRecord(c->dev[bi], s));
+        HIP_CHECK(hipStreamWaitEvent(c->stream2, c->dev[bi], 0));
+        launch_dp_walk(q, c->stream2);
+        HIP_CHECK(hipEventRecord(c->dev[2 + bi], c->stream2));
+        walks = true;
+      } else if (ci + nbuf < chunks.size()) {
+        HIP_CHECK(hipEventRecord(c->dev[2 + bi], s));  // the buffer is free once this launch is done
+      }
+    }
+    if (walks) {  // the main stream continues after every walk
+      HIP_CHECK(hipEventRecord(c->dev[0], c->stream2));
+      HIP_CHECK(hipStreamWaitEvent(s, c->dev[0], 0));
     }
     for (const DpArgs& q : bands) run_dp_band(c, q);
     HIP_CHECK(hipGetLastError());
@@ -1870,6 +1933,7 @@ static int ctx_create(const pbw_genome* g, const pbw_options* o, int device, con
   if (rc != PBW_OK) {
     g_error = c->error;
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     delete c;
     *out = nullptr;
     return rc;
@@ -1931,9 +1995,13 @@ void pbw_ctx_destroy(pbw_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : c->dev)
+    if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->h_pe) (void)hipHostFree(c->h_pe);
   for (uint8_t* p : c->h_up)
     if (p) (void)hipHostFree(p);

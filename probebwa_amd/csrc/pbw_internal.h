@@ -264,6 +264,7 @@ struct DpArgs {
   uint32_t gstage_waves;
   uint32_t wpb;            // waves per block (banded kernel; set by the launcher)
   uint32_t* tbq;           // k_dp_quad traceback scratch: quad_scratch_words() words
+  int64_t* tbend;          // k_dp_quad -> k_dp_walk: per job of the launch, its endpoint key (INT64_MIN: none)
   uint32_t quad_grid;      // k_dp_quad grid cap (0: QUAD_MAX_WAVES)
   uint32_t pack16;         // 1: eight jobs per wave in 16-bit halves (k_dp_pair16) where allowed
 };
@@ -282,9 +283,13 @@ uint32_t dp_wide_tiles(uint32_t max_m, uint32_t max_n);  // 256-column tiles per
 void launch_dp_wide(const DpArgs& a, uint32_t nwaves, hipStream_t s);
 // true: the tiled kernel keeps its previous row in LDS (pass rows = nullptr)
 bool dp_tiled_rows_in_lds(uint32_t max_m, uint32_t max_n);
-bool launch_dp_quad(const DpArgs& a, hipStream_t s);  // false: layout too large, nothing launched
+// k_dp_quad (returns 1: its jobs then need launch_dp_walk with the same arguments)
+// or k_dp_pair16 (2: done); 0: layout too large, nothing launched
+int launch_dp_quad(const DpArgs& a, hipStream_t s);
+void launch_dp_walk(const DpArgs& a, hipStream_t s);  // k_dp_walk, then k_dp_lk
 constexpr uint32_t QUAD_MAX_WAVES = 32768;  // grid cap of k_dp_quad (more blocks than resident waves: dynamic balance)
 size_t quad_scratch_words(uint32_t max_m, uint32_t max_n, uint32_t njobs, uint32_t grid_cap);
+constexpr uint64_t QUAD_CHUNK_BYTES = 8ull << 30;  // traceback planes of one k_dp_quad launch, at most (two buffers)
 // Jobs of `a` (jlist / njobs / job_valid) sorted longest read first into `out`, count
 // to total[0]; `hist` holds MAX_READ_LEN + 1 words of scratch.
 void launch_job_sort(const DpArgs& a, uint32_t* hist, uint32_t* out, uint32_t* total, hipStream_t s);

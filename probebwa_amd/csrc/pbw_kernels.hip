@@ -3733,11 +3733,15 @@ void launch_dp_band(const DpArgs& a0, hipStream_t s) {
 //    path.  Unreachable values stay within [-2^31, -2^29).
 //  - Traceback bits go into four words per lane with v_alignbit (planes:
 //    M came from I or D; M came from D; tb_i; tb_d) and to the job's area in
-//    global scratch every 8 rows.
-//    One lane per job walks them, taking runs of diagonal M steps from one
-//    word, and pushes CIGAR runs into the consumed top of its traceback area.
+//    global scratch every 8 rows, one 16-byte store per lane ([block][lane]
+//    [plane]).  The words of the five lanes around the endpoint's are then
+//    staged into LDS for every block with loads issued together, and one lane
+//    per job walks them, taking runs of diagonal M steps from one word (global
+//    loads only where the path leaves that strip), and pushes CIGAR runs into
+//    the consumed top of its traceback area.
 // ---------------------------------------------------------------------------
 constexpr int32_t QUAD_GUARD = -(1 << 28);
+constexpr int64_t QUAD_NO_JOB = INT64_MIN;  // endpoint-key slot of a job k_dp_quad did not run (no key is this low)
 constexpr int32_t QUAD_EXCL = -(1 << 29);
 
 struct QuadLayout {  // byte offsets inside one job's LDS area
@@ -3801,9 +3805,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   const int g = lane >> 4, L = lane & 15;
   const QuadLayout Y = quad_layout(a.max_m, a.max_n);
   uint8_t* jb = dyn + (size_t)g * Y.stride;
-  // traceback planes, then CIGAR runs and log-likelihood terms, in this group's
-  // global scratch: LDS keeps only the per-row inputs, so more waves fit per CU
-  uint32_t* TB = a.tbq + ((size_t)blockIdx.x * 4 + (size_t)g) * Y.tb_words;
   // per window column c at [c - (1 + o)]: gap open, then 6 * (A, C, G, T, N -> 0..4)
   int2* GC = (int2*)(jb + Y.gc);
   int32_t* SCW = (int32_t*)(jb + Y.scw);
@@ -3813,12 +3814,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   const int32_t ge = a.gap_extend;
   int32_t gev;  // the same value held per lane
   asm volatile("v_mov_b32 %0, %1" : "=v"(gev) : "s"(ge));
-  unsigned long long c_cells = 0;
   const uint32_t nwaves = gridDim.x;
   const uint32_t nquads = (a.njobs + 3) / 4;
   for (uint32_t wq = blockIdx.x; wq < nquads; wq += nwaves) {
     // ---- this group's job and its shape ----
     const uint32_t t = wq * 4 + (uint32_t)g;
+    // the job's traceback planes in global scratch (read by k_dp_walk): LDS keeps
+    // only the per-row inputs, so more waves fit per CU
+    uint32_t* TB = a.tbq + (uint64_t)t * Y.tb_words;
     bool valid = t < a.njobs;
     uint32_t jid = 0;
     if (valid) {
@@ -4083,11 +4086,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
       Po[2] = P2;
       Po[3] = P3;
       if ((i & 7u) == 0) {
-        uint32_t* tw = TB + (((i - 1) >> 3) * 4) * 16 + L;
-        tw[0] = w0;
-        tw[16] = w0 & w1;
-        tw[32] = w2;
-        tw[48] = w3;
+        *reinterpret_cast<uint4*>(TB + ((i - 1) >> 3) * 64 + L * 4) = make_uint4(w0, w0 & w1, w2, w3);
         w0 = w1 = w2 = w3 = 0;
       }
     };
@@ -4121,11 +4120,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     }
     if (m & 7u) {
       const uint32_t sh = 4 * (8 - (m & 7u));
-      uint32_t* tw = TB + (((m - 1) >> 3) * 4) * 16 + L;
-      tw[0] = w0 << sh;
-      tw[16] = (w0 & w1) << sh;
-      tw[32] = w2 << sh;
-      tw[48] = w3 << sh;
+      *reinterpret_cast<uint4*>(TB + ((m - 1) >> 3) * 64 + L * 4) =
+          make_uint4(w0 << sh, (w0 & w1) << sh, w2 << sh, w3 << sh);
     }
     // ---- endpoint: last maximum over (j asc; M then I) of row m ----
     int64_t key = end_key(NEG_INF, 2 * n + 1);
@@ -4143,97 +4139,169 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     keys[L] = key;
     wave_sync();
     mem_sync();  // the traceback words stored by every lane
-    // ---- traceback (smith_waterman.rs:299-345) by lane 0 of the group ----
-    // Each step takes a whole run from at most two traceback words: diagonal M
-    // steps (rows r, r-1, .. of a column, into the block below), horizontal D
-    // steps (columns k, k-1, .., into the lane to the left) or vertical I steps
-    // (cells (r, k), (r-1, k+1), .., into the lane to the right).  The
-    // selection is branch-free; the pending CIGAR run is rewritten in place
-    // every step and its slot advances when the operation changes.
-    uint32_t nruns = 0, off = 0;
-    int64_t best = 0;
-    uint32_t* runs_top = TB + Y.tb_words;
-    if (L == 0 && m) {
-      best = keys[0];
+    if (L == 0 && t < a.njobs) {  // the endpoint key for k_dp_walk (QUAD_NO_JOB: not this kernel's job)
+      int64_t best = keys[0];
       for (int u = 1; u < 16; u++) best = best > keys[u] ? best : keys[u];
-      uint32_t i = m, j = ((uint32_t)best) >> 1, st = ((uint32_t)best) & 1u;
-      uint32_t cur_op = 0xFFu, cur_len = 0;
-      int32_t cb = -1, cl = -1;
-      uint32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0, p0 = 0, p1 = 0, n2 = 0, l3 = 0;
-      while (i > 0) {
-        int32_t d = (int32_t)j - (int32_t)i - o;
-        d = d < 0 ? 0 : (d > 63 ? 63 : d);  // cells on the path always lie in the band
-        const int32_t lq = d >> 2, rb = (int32_t)(i - 1) >> 3;
-        if (rb != cb || lq != cl) {
-          const uint32_t* tw = TB + (rb * 4) * 16 + lq;
-          const uint32_t* pw = TB + ((rb > 0 ? rb - 1 : 0) * 4) * 16 + lq;
-          v0 = tw[0];
-          v1 = tw[16];
-          v2 = tw[32];
-          v3 = tw[48];
-          p0 = pw[0];
-          p1 = pw[16];
-          n2 = tw[32 + (lq < 15 ? 1 : 0)];
-          l3 = tw[48 - (lq > 0 ? 1 : 0)];
-          cb = rb;
-          cl = lq;
-        }
-        const uint32_t k = (uint32_t)d & 3u, r = (i - 1) & 7u;
-        const uint32_t pos = 31u - (4u * r + k);
-        // M: stops after the first row whose M came from I or D
-        const uint32_t mav = r + 1u + (rb > 0 ? 8u : 0u);
-        const uint64_t xm = ((((uint64_t)p0 << 32) | v0) >> pos) & 0x1111111111111111ull &
-                            (mav >= 16u ? ~0ull : ((1ull << (4u * mav)) - 1ull));
-        const uint32_t f = xm ? ((uint32_t)__builtin_ctzll(xm) >> 2) : 64u;
-        const uint32_t mst = min(min(f + 1u, mav), j);
-        const uint64_t x1 = (((uint64_t)p1 << 32) | v1) >> pos;
-        const uint32_t mpar = (mst == f + 1u) ? 1u + (uint32_t)((x1 >> ((4u * f) & 63u)) & 1ull) : 0u;
-        // D: stops after the first column whose D was opened
-        const uint32_t dav = k + 1u + (lq > 0 ? 4u : 0u);
-        const uint32_t yd = ((v3 >> pos) & ((2u << k) - 1u)) | (((l3 >> (28u - 4u * r)) & 0xFu) << (k + 1u));
-        const uint32_t y = ~yd & ((1u << dav) - 1u);
-        const uint32_t fz = y ? (uint32_t)__builtin_ctz(y) : 32u;
-        const uint32_t dst = min(min(fz + 1u, dav), j);
-        const uint32_t dpar = (dst == fz + 1u) ? 0u : 2u;
-        // I: stops after the first cell whose I was opened (stride-3 bits)
-        const uint32_t tav = min((lq < 15 ? 7u : 3u) - k, r);
-        const uint32_t ma = 0x249249u & ((1u << (3u * (3u - k) + 1u)) - 1u);
-        const uint32_t zv = ((v2 >> pos) & ma) | ((r > 0 ? (n2 >> ((pos + 4u) & 31u)) : 0u) & 0x249249u & ~ma);
-        const uint32_t zi = ~zv & 0x249249u & ((1u << (3u * tav + 1u)) - 1u);
-        const uint32_t fi = zi ? (uint32_t)__builtin_ctz(zi) / 3u : 32u;
-        const uint32_t ist = min(fi + 1u, tav + 1u);
-        const uint32_t ipar = (ist == fi + 1u) ? 0u : 1u;
-        // select: reference column 0 forces a single I step with the state kept
-        const uint32_t sel = (j == 0) ? 1u : st;
-        const uint32_t isM = 0u - (uint32_t)(sel == 0u), isD = 0u - (uint32_t)(sel == 2u);
-        const uint32_t isI = ~(isM | isD);
-        const uint32_t jz = 0u - (uint32_t)(j == 0);
-        const uint32_t len = (mst & isM) | (dst & isD) | (((ist & ~jz) | (1u & jz)) & isI);
-        const uint32_t nst = (mpar & isM) | (dpar & isD) | (((ipar & ~jz) | (st & jz)) & isI);
-        i -= len & ~isD;
-        j -= len & ~isI;
-        st = nst;
-        const uint32_t fresh = 0u - (uint32_t)(sel != cur_op);
-        nruns += (fresh & (uint32_t)(cur_len != 0)) ? 1u : 0u;
-        cur_len = (len & fresh) | ((cur_len + len) & ~fresh);
-        cur_op = sel;
-        runs_top[-1 - (int32_t)nruns] = (cur_len << 4) | cur_op;
+      a.tbend[t] = m ? best : QUAD_NO_JOB;
+    }
+    wave_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Traceback and outputs of k_dp_quad's jobs, one lane per job
+// (smith_waterman.rs:299-345): every lane of the wave walks a job, where inside
+// k_dp_quad one lane of each 16-lane row would.
+// The walk takes a whole run per step from at most two traceback words of the
+// job's area ([8-row block][lane][plane], written by k_dp_quad) -- diagonal M
+// steps (rows r, r-1, .. of a column, into the block below), horizontal D steps
+// (columns k, k-1, .., into the lane to the left) or vertical I steps (cells
+// (r, k), (r-1, k+1), .., into the lane to the right) -- with the selection
+// branch-free; the pending CIGAR run is rewritten in place every step and its
+// slot advances when the operation changes, the runs going into the consumed
+// top of the area, where k_dp_lk reads them.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_dp_walk(DpArgs a) {
+  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+  const QuadLayout Y = quad_layout(a.max_m, a.max_n);
+  unsigned long long c_cells = 0;
+  const int64_t best = t < a.njobs ? a.tbend[t] : QUAD_NO_JOB;
+  if (best != QUAD_NO_JOB) {
+    const uint32_t jid = a.jlist ? a.jlist[t] : t;
+    const DpJob job = a.jobs[jid];
+    const uint32_t m = (uint32_t)(a.R.seq_off[job.read + 1] - a.R.seq_off[job.read]);
+    const uint32_t n = job.n;
+    const int32_t h = job.hint, b = job.band, o = h + b - 62;
+    const uint32_t* TB = a.tbq + (uint64_t)t * Y.tb_words;
+    uint32_t* runs_top = a.tbq + (uint64_t)t * Y.tb_words + Y.tb_words;
+    uint32_t nruns = 0, off = 0;
+    {
+    uint32_t i = m, j = ((uint32_t)best) >> 1, st = ((uint32_t)best) & 1u;
+    uint32_t cur_op = 0xFFu, cur_len = 0;
+    int32_t cb = -1, cl = -1;
+    uint32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0, p0 = 0, p1 = 0, n2 = 0, l3 = 0;
+    while (i > 0) {
+      int32_t d = (int32_t)j - (int32_t)i - o;
+      d = d < 0 ? 0 : (d > 63 ? 63 : d);  // cells on the path always lie in the band
+      const int32_t lq = d >> 2, rb = (int32_t)(i - 1) >> 3;
+      if (rb != cb || lq != cl) {
+        // this lane's four planes, the two M planes of the block above, plane 2
+        // of the lane to the right and plane 3 of the lane to the left (own
+        // lane at the edges)
+        const uint4 tv = *reinterpret_cast<const uint4*>(TB + rb * 64 + lq * 4);
+        const uint2 pv = *reinterpret_cast<const uint2*>(TB + (rb > 0 ? rb - 1 : 0) * 64 + lq * 4);
+        v0 = tv.x;
+        v1 = tv.y;
+        v2 = tv.z;
+        v3 = tv.w;
+        p0 = pv.x;
+        p1 = pv.y;
+        n2 = lq < 15 ? TB[rb * 64 + lq * 4 + 6] : tv.z;
+        l3 = lq > 0 ? TB[rb * 64 + lq * 4 - 1] : tv.w;
+        cb = rb;
+        cl = lq;
       }
+      const uint32_t k = (uint32_t)d & 3u, r = (i - 1) & 7u;
+      const uint32_t pos = 31u - (4u * r + k);
+      // M: stops after the first row whose M came from I or D
+      const uint32_t mav = r + 1u + (rb > 0 ? 8u : 0u);
+      const uint64_t xm = ((((uint64_t)p0 << 32) | v0) >> pos) & 0x1111111111111111ull &
+                          (mav >= 16u ? ~0ull : ((1ull << (4u * mav)) - 1ull));
+      const uint32_t f = xm ? ((uint32_t)__builtin_ctzll(xm) >> 2) : 64u;
+      const uint32_t mst = min(min(f + 1u, mav), j);
+      const uint64_t x1 = (((uint64_t)p1 << 32) | v1) >> pos;
+      const uint32_t mpar = (mst == f + 1u) ? 1u + (uint32_t)((x1 >> ((4u * f) & 63u)) & 1ull) : 0u;
+      // D: stops after the first column whose D was opened
+      const uint32_t dav = k + 1u + (lq > 0 ? 4u : 0u);
+      const uint32_t yd = ((v3 >> pos) & ((2u << k) - 1u)) | (((l3 >> (28u - 4u * r)) & 0xFu) << (k + 1u));
+      const uint32_t y = ~yd & ((1u << dav) - 1u);
+      const uint32_t fz = y ? (uint32_t)__builtin_ctz(y) : 32u;
+      const uint32_t dst = min(min(fz + 1u, dav), j);
+      const uint32_t dpar = (dst == fz + 1u) ? 0u : 2u;
+      // I: stops after the first cell whose I was opened (stride-3 bits)
+      const uint32_t tav = min((lq < 15 ? 7u : 3u) - k, r);
+      const uint32_t ma = 0x249249u & ((1u << (3u * (3u - k) + 1u)) - 1u);
+      const uint32_t zv = ((v2 >> pos) & ma) | ((r > 0 ? (n2 >> ((pos + 4u) & 31u)) : 0u) & 0x249249u & ~ma);
+      const uint32_t zi = ~zv & 0x249249u & ((1u << (3u * tav + 1u)) - 1u);
+      const uint32_t fi = zi ? (uint32_t)__builtin_ctz(zi) / 3u : 32u;
+      const uint32_t ist = min(fi + 1u, tav + 1u);
+      const uint32_t ipar = (ist == fi + 1u) ? 0u : 1u;
+      // select: reference column 0 forces a single I step with the state kept
+      const uint32_t sel = (j == 0) ? 1u : st;
+      const uint32_t isM = 0u - (uint32_t)(sel == 0u), isD = 0u - (uint32_t)(sel == 2u);
+      const uint32_t isI = ~(isM | isD);
+      const uint32_t jz = 0u - (uint32_t)(j == 0);
+      const uint32_t len = (mst & isM) | (dst & isD) | (((ist & ~jz) | (1u & jz)) & isI);
+      const uint32_t nst = (mpar & isM) | (dpar & isD) | (((ipar & ~jz) | (st & jz)) & isI);
+      i -= len & ~isD;
+      j -= len & ~isI;
+      st = nst;
+      const uint32_t fresh = 0u - (uint32_t)(sel != cur_op);
+      nruns += (fresh & (uint32_t)(cur_len != 0)) ? 1u : 0u;
+      cur_len = (len & fresh) | ((cur_len + len) & ~fresh);
+      cur_op = sel;
+      runs_top[-1 - (int32_t)nruns] = (cur_len << 4) | cur_op;
+    }
       nruns += cur_len != 0 ? 1u : 0u;
       off = j;
     }
-    mem_sync();  // the CIGAR runs the walker stored
-    nruns = (uint32_t)__shfl((int)nruns, lane & ~15, 64);
-    off = (uint32_t)__shfl((int)off, lane & ~15, 64);
     const uint32_t* runs = runs_top - nruns;
-    // ---- alignment log-likelihood terms (single.rs:228-262), one per read
-    //      position, in parallel into the LDS of the per-row inputs (dead after
-    //      the rows; it holds 8 max_m bytes): M columns by base match, I columns
-    //      ln(1/4).  The table reads of a lane are issued together. ----
-    double* terms = (double*)(jb + Y.gc);
-    {
-      uint32_t u = 0, qs = 0, rs = 0;
-      for (uint32_t qi = L; qi < m; qi += 16) {
+    uint32_t span = 0;
+    for (uint32_t u = 0; u < nruns; u++) {
+      const uint32_t w = runs[u], op = w & 15u;
+      if (op == 0u || op == 2u) span += w >> 4;
+    }
+    const uint64_t cells = quad_cells(m, h, b, n);
+    c_cells = cells;
+    DpOut oo;
+    oo.log_lk = 0.0;  // k_dp_lk's
+    oo.ref_off = off;
+    oo.score = (int32_t)(best >> 32);
+    oo.cigar_n = nruns;
+    oo.span = span;
+    oo.cells = cells;
+    a.out[jid] = oo;
+    // ---- CIGAR runs to the job's slot, forward order ----
+    const uint32_t ncopy = nruns < a.cigar_cap ? nruns : a.cigar_cap;
+    uint32_t* dst = a.cigar + ((uint64_t)jid - a.cigar_base) * a.cigar_cap;
+    for (uint32_t u = 0; u < ncopy; u++) dst[u] = runs[u];
+  }
+  for (int sft = 32; sft > 0; sft >>= 1) c_cells += (unsigned long long)__shfl_xor((long long)c_cells, sft, 64);
+  if (lane_id() == 0 && a.ctr && c_cells) atomicAdd(&a.ctr->cells, c_cells);
+}
+
+// The alignment log-likelihood of k_dp_quad's jobs (single.rs:228-262,
+// bayesian.rs:142-179), sixteen lanes per job: each round the lanes take 16
+// consecutive read positions (coalesced read, quality and genome loads) and
+// find their CIGAR run with a cursor that only moves forward; M positions add
+// ln(1-e) or ln(e/3) by base match against the window, I positions ln(1/4),
+// and the group's first lane adds the 16 terms in read order, as the
+// reference sums them.
+__global__ __launch_bounds__(256) void k_dp_lk(DpArgs a) {
+  __shared__ double lut_m[256], lut_x[256];  // per-base log-likelihoods of a match / mismatch by quality
+  __shared__ double terms[16][16];
+  lut_m[threadIdx.x] = a.T.lk_match[threadIdx.x];
+  lut_x[threadIdx.x] = a.T.lk_mis[threadIdx.x];
+  __syncthreads();
+  const uint32_t grp = threadIdx.x >> 4, L = threadIdx.x & 15u;
+  const uint32_t t = blockIdx.x * 16 + grp;
+  const QuadLayout Y = quad_layout(a.max_m, a.max_n);
+  const int64_t best = t < a.njobs ? a.tbend[t] : QUAD_NO_JOB;
+  if (best != QUAD_NO_JOB) {
+    const uint32_t jid = a.jlist ? a.jlist[t] : t;
+    const DpJob job = a.jobs[jid];
+    const uint64_t so = a.R.seq_off[job.read], qo = a.R.qual_off[job.read];
+    const uint32_t m = (uint32_t)(a.R.seq_off[job.read + 1] - so);
+    const uint32_t qlen = (uint32_t)(a.R.qual_off[job.read + 1] - qo);
+    const DpOut& oo = a.out[jid];
+    const uint32_t nruns = oo.cigar_n, off = oo.ref_off;
+    const uint32_t* runs = a.tbq + (uint64_t)t * Y.tb_words + Y.tb_words - nruns;
+    uint32_t u = 0, qs = 0, rs = 0;
+    double lk = 0.0;
+    for (uint32_t base = 0; base < m; base += 16) {
+      const uint32_t qi = base + L;
+      double tv = 0.0;
+      if (qi < m) {
         uint32_t op = 1u;
         while (u < nruns) {  // the run that holds read position qi
           const uint32_t w = runs[u], len = w >> 4;
@@ -4244,52 +4312,31 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
           rs += (op == 0u || op == 2u) ? len : 0u;
           u++;
         }
-        double tv = a.T.ln_quarter;
+        tv = a.T.ln_quarter;
         if (op == 0u) {
-          const uint32_t qv = qi < qlen ? QQ[qi] : 30u;
-          tv = (Q[qi] == REF[off + rs + (qi - qs)]) ? a.T.lk_match[qv] : a.T.lk_mis[qv];
+          const uint32_t qb = job.rc ? comp_base(a.R.seq[so + m - 1 - qi]) : a.R.seq[so + qi];
+          const uint32_t qv = qi < qlen ? (job.rc ? a.R.qual[qo + qlen - 1 - qi] : a.R.qual[qo + qi]) : 30u;
+          const uint64_t p = job.ref_gb + off + rs + (qi - qs);
+          const uint32_t c2 = (uint32_t)(a.G.g2b[p >> 5] >> ((p & 31) * 2)) & 3u;
+          const uint32_t gb = ((a.G.nmask[p >> 6] >> (p & 63)) & 1ull) ? (uint32_t)'N' : (uint32_t)(uint8_t)("ACGT"[c2]);
+          tv = qb == gb ? lut_m[qv] : lut_x[qv];
         }
-        terms[qi] = tv;
       }
-    }
-    wave_sync();  // the terms of all 16 lanes
-    if (L == 0 && m) {
-      double lk = 0.0;
-      uint32_t qi = 0;
-      for (; qi + 8 <= m; qi += 8) {  // read order, as the reference sums
-        double t8[8];
+      terms[grp][L] = tv;
+      wave_sync();
+      if (L == 0) {
+        const uint32_t c = min(16u, m - base);
+        double t16[16];
 #pragma unroll
-        for (int u = 0; u < 8; u++) t8[u] = terms[qi + u];
+        for (uint32_t v = 0; v < 16; v++) t16[v] = terms[grp][v];
 #pragma unroll
-        for (int u = 0; u < 8; u++) lk += t8[u];
+        for (uint32_t v = 0; v < 16; v++)
+          if (v < c) lk += t16[v];
       }
-      for (; qi < m; qi++) lk += terms[qi];
-      uint32_t span = 0;
-      for (uint32_t u = 0; u < nruns; u++) {
-        const uint32_t w = runs[u], op = w & 15u;
-        if (op == 0u || op == 2u) span += w >> 4;
-      }
-      const uint64_t cells = quad_cells(m, h, b, n);
-      c_cells += cells;
-      DpOut oo;
-      oo.log_lk = lk;
-      oo.ref_off = off;
-      oo.score = (int32_t)(best >> 32);
-      oo.cigar_n = nruns;
-      oo.span = span;
-      oo.cells = cells;
-      a.out[jid] = oo;
+      wave_sync();
     }
-    // ---- CIGAR runs to the job's slot, forward order ----
-    {
-      const uint32_t ncopy = nruns < a.cigar_cap ? nruns : a.cigar_cap;
-      for (uint32_t u = L; u < ncopy; u += 16) a.cigar[((uint64_t)jid - a.cigar_base) * a.cigar_cap + u] = runs[u];
-    }
-    wave_sync();
+    if (L == 0) a.out[jid].log_lk = lk;
   }
-  c_cells = __shfl((long long)c_cells, 0, 64) + __shfl((long long)c_cells, 16, 64) +
-            __shfl((long long)c_cells, 32, 64) + __shfl((long long)c_cells, 48, 64);
-  if (lane == 0 && a.ctr) atomicAdd(&a.ctr->cells, c_cells);
 }
 
 // ---------------------------------------------------------------------------
@@ -5010,12 +5057,12 @@ static uint32_t pair16_blocks(uint32_t njobs, uint32_t grid_cap) {
   return q < cap ? q : cap;
 }
 size_t quad_scratch_words(uint32_t max_m, uint32_t max_n, uint32_t njobs, uint32_t grid_cap) {
-  const size_t quad = (size_t)quad_blocks(njobs, grid_cap) * 4 * quad_layout(max_m, max_n).tb_words;
+  const size_t quad = (size_t)((njobs + 3) & ~3u) * quad_layout(max_m, max_n).tb_words;
   const size_t pair = (size_t)pair16_blocks(njobs, grid_cap) * 8 * pair16_layout(max_m, max_n).tb_words;
   return quad > pair ? quad : pair;
 }
-bool launch_dp_quad(const DpArgs& a, hipStream_t s) {
-  if (a.njobs == 0) return true;
+int launch_dp_quad(const DpArgs& a, hipStream_t s) {
+  if (a.njobs == 0) return 2;
   // eight jobs per wave in 16-bit halves while the gap penalties keep the D
   // scan's column shifts small (see k_dp_pair16)
   if (a.pack16 && a.max_m <= (uint32_t)QUAD_MAX_M && a.gap_extend >= -4 && a.gap_open >= -400 && a.tbq && a.jlist) {
@@ -5024,17 +5071,22 @@ bool launch_dp_quad(const DpArgs& a, hipStream_t s) {
     const size_t need = 8 * ((size_t)((a.max_n + 62) >> 5) + ((a.max_n + 126) >> 6));
     if (lds <= 64 * 1024 && need <= P.scw_len) {
       hipLaunchKernelGGL(k_dp_pair16, dim3(pair16_blocks(a.njobs, a.quad_grid)), dim3(64), lds, s, a);
-      return true;
+      return 2;
     }
   }
   const QuadLayout Y = quad_layout(a.max_m, a.max_n);
   const size_t lds = 4 * (size_t)Y.stride;
-  if (a.max_m > (uint32_t)QUAD_MAX_M || lds > 64 * 1024 || !a.tbq) return false;
+  if (a.max_m > (uint32_t)QUAD_MAX_M || lds > 64 * 1024 || !a.tbq || !a.tbend) return 0;
   // genome-word staging reuses the score-word area
   const size_t need = 8 * ((size_t)((a.max_n + 62) >> 5) + ((a.max_n + 126) >> 6));
-  if (need > Y.q - Y.scw) return false;
+  if (need > Y.q - Y.scw) return 0;
   hipLaunchKernelGGL(k_dp_quad, dim3(quad_blocks(a.njobs, a.quad_grid)), dim3(64), lds, s, a);
-  return true;
+  return 1;
+}
+void launch_dp_walk(const DpArgs& a, hipStream_t s) {
+  if (a.njobs == 0) return;
+  hipLaunchKernelGGL(k_dp_walk, dim3((a.njobs + 255) / 256), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_dp_lk, dim3((a.njobs + 15) / 16), dim3(256), 0, s, a);
 }
 
 // Any band width: the band of row i, [js, je], is processed in 64-column
