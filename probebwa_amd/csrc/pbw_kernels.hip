@@ -2792,6 +2792,7 @@ struct CgBucketLds {
   uint32_t end[CG_NBMAX];   // bucket counts, then (scan) each bucket's end
   uint32_t base[CG_GMAX + 1], sh[CG_GMAX + 1];
   uint32_t big[CG_BTILE / CG_BSMALL];  // buckets of more than CG_BSMALL keys
+  uint32_t wq[4][128];                 // per wave: buckets of 5..CG_BSMALL keys waiting for a network
   uint32_t nbig, fail, scan[8];
 };
 __device__ __forceinline__ bool cg_bucket_sort(uint64_t* key, uint32_t cnt, uint32_t q0, uint32_t q1, const uint32_t* st,
@@ -2819,7 +2820,7 @@ __device__ __forceinline__ bool cg_bucket_sort(uint64_t* key, uint32_t cnt, uint
   for (uint32_t b = tid; b < NB; b += 256) B.end[b] = 0;
   __syncthreads();
   uint64_t kk[IPT];
-  uint32_t bk[IPT];
+  uint32_t bk[IPT];  // bucket << 12 | the key's rank among its bucket's keys (both < 4096)
 #pragma unroll
   for (uint32_t j = 0; j < IPT; j++) {
     const uint32_t pos = j * 256 + tid;
@@ -2830,12 +2831,12 @@ __device__ __forceinline__ bool cg_bucket_sort(uint64_t* key, uint32_t cnt, uint
     const uint32_t pos = j * 256 + tid;
     if (pos < cnt) {
       const uint32_t ql = cg_rank(kk[j]) - q0;
-      bk[j] = B.base[ql] + (uint32_t)((uint64_t)(uint32_t)kk[j] >> B.sh[ql]);  // sh <= 32
-      atomicAdd(&B.end[bk[j]], 1u);
+      const uint32_t b = B.base[ql] + (uint32_t)((uint64_t)(uint32_t)kk[j] >> B.sh[ql]);  // sh <= 32
+      bk[j] = (b << 12) | atomicAdd(&B.end[b], 1u);
     }
   }
   __syncthreads();
-  {  // inclusive scan of the counts: bucket ends
+  {  // exclusive scan of the counts: bucket starts
     const uint32_t per = (NB + 255) / 256, b0 = min(NB, tid * per), b1 = min(NB, b0 + per);
     uint32_t local = 0;
     for (uint32_t b = b0; b < b1; b++) local += B.end[b];
@@ -2847,8 +2848,8 @@ __device__ __forceinline__ bool cg_bucket_sort(uint64_t* key, uint32_t cnt, uint
         if (c > 512) B.fail = 1;
         else B.big[atomicAdd(&B.nbig, 1u)] = b;
       }
-      run += c;
       B.end[b] = run;
+      run += c;
     }
   }
   __syncthreads();
@@ -2856,27 +2857,55 @@ __device__ __forceinline__ bool cg_bucket_sort(uint64_t* key, uint32_t cnt, uint
     __syncthreads();
     return false;
   }
-  // place: a bucket's keys fill it from its end backwards
+  // place: bucket start + the key's rank
 #pragma unroll
   for (uint32_t j = 0; j < IPT; j++) {
     const uint32_t pos = j * 256 + tid;
-    if (pos < cnt) B.keys[atomicSub(&B.end[bk[j]], 1u) - 1u] = kk[j];
+    if (pos < cnt) B.keys[B.end[bk[j] >> 12] + (bk[j] & 4095u)] = kk[j];
   }
   __syncthreads();
-  // B.end[b] is now bucket b's start; its end is the next bucket's start (cnt for
-  // the last).  Each wave sizes its network for the largest small bucket it holds.
-  for (uint32_t b0 = 0; b0 < NB; b0 += 256) {
-    const uint32_t b = b0 + tid;
-    uint32_t s0 = 0, c = 0;
-    if (b < NB) {
-      s0 = B.end[b];
-      c = (b + 1 < NB ? B.end[b + 1] : cnt) - s0;
-      if (c < 2 || c > CG_BSMALL) c = 0;
+  // B.end[b] is bucket b's start; its end is the next bucket's start (cnt for the
+  // last).  Buckets of 2..4 keys: a 2- or 4-key network per lane, sized per wave;
+  // buckets of 5..CG_BSMALL keys: queued per wave and put in order 64 at a time
+  // with every lane busy (an 8- or 16-key network, sized per wave).
+  {
+    const uint32_t lane = tid & 63u;
+    uint32_t* wq = B.wq[tid >> 6];
+    uint32_t qn = 0;  // wave-uniform
+    auto drain = [&](uint32_t k) {  // the first k <= 64 queued buckets, one per lane
+      wave_sync();
+      uint32_t s0 = 0, c = 0;
+      if (lane < k) {
+        const uint32_t b = wq[lane];
+        s0 = B.end[b];
+        c = (b + 1 < NB ? B.end[b + 1] : cnt) - s0;
+      }
+      if (__ballot(c > 8)) bucket_net<16>(B.keys + s0, c);
+      else bucket_net<8>(B.keys + s0, c);
+    };
+    for (uint32_t b0 = 0; b0 < NB; b0 += 256) {
+      const uint32_t b = b0 + tid;
+      uint32_t s0 = 0, c = 0;
+      if (b < NB) {
+        s0 = B.end[b];
+        c = (b + 1 < NB ? B.end[b + 1] : cnt) - s0;
+      }
+      const bool mid = c >= 5 && c <= CG_BSMALL;
+      const uint64_t mm = __ballot(mid);
+      if (mid) wq[qn + (uint32_t)__popcll(mm & lanemask_lt())] = b;
+      qn += (uint32_t)__popcll(mm);
+      const uint32_t cs = c >= 2 && c <= 4 ? c : 0u;
+      if (__ballot(cs > 2)) bucket_net<4>(B.keys + s0, cs);
+      else if (__ballot(cs != 0)) bucket_net<2>(B.keys + s0, cs);
+      if (qn >= 64) {
+        drain(64);
+        const uint32_t rest = qn - 64, v = lane < rest ? wq[64 + lane] : 0u;
+        wave_sync();
+        if (lane < rest) wq[lane] = v;
+        qn = rest;
+      }
     }
-    if (__ballot(c > 8)) bucket_net<16>(B.keys + s0, c);
-    else if (__ballot(c > 4)) bucket_net<8>(B.keys + s0, c);
-    else if (__ballot(c > 2)) bucket_net<4>(B.keys + s0, c);
-    else if (__ballot(c != 0)) bucket_net<2>(B.keys + s0, c);
+    if (qn) drain(qn);
   }
   for (uint32_t i = tid >> 6; i < B.nbig; i += 4) {
     const uint32_t b = B.big[i];
