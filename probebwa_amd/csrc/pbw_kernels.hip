@@ -2338,15 +2338,26 @@ __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a, const uint32_t* l
   const ClusterMem M{S.key, S.idx, S.cl_sum, S.cl_start, S.cl_gid, S.g_key, S.g_min, S.g_rank, S.g_sort, S.table,
                      S.tmp, CL_GCAP, nullptr, nullptr, nullptr, nullptr, nullptr};
   const uint32_t total = list ? list_n[0] : a.R.n;
-  for (uint32_t li = blockIdx.x; li < total; li += gridDim.x) {
-    const uint32_t r = list ? list[li] : li;
-    const uint32_t n = a.nhits[r];
-    if (n == 0) {
-      if (lane_id() == 0) a.ncand[r] = 0;
-      continue;
+  const uint32_t lane = (uint32_t)lane_id();
+  // 64 list entries a round: their hit counts read at once, the reads over NC
+  // hits deferred with one atomic for the wave, the rest clustered one by one
+  for (uint32_t l0 = blockIdx.x * 64; l0 < total; l0 += gridDim.x * 64) {
+    const uint32_t li = l0 + lane;
+    const uint32_t rl = li < total ? (list ? list[li] : li) : 0u;
+    const uint32_t nl = li < total ? a.nhits[rl] : 0u;
+    if (li < total && nl == 0) a.ncand[rl] = 0;
+    const uint64_t over = __ballot(nl > NC);
+    if (over) {
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(defer_n, (uint32_t)__popcll(over));
+      base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+      if (nl > NC) defer[base + (uint32_t)__popcll(over & lanemask_lt())] = rl;
     }
-    bool done = false;
-    if (n <= NC) {
+    uint64_t work = __ballot(nl > 0 && nl <= NC);
+    while (work) {
+      const uint32_t src = (uint32_t)__builtin_ctzll(work);
+      work &= work - 1;
+      const uint32_t r = (uint32_t)__shfl((int)rl, (int)src, 64), n = (uint32_t)__shfl((int)nl, (int)src, 64);
       const uint32_t L = (uint32_t)(a.R.seq_off[r + 1] - a.R.seq_off[r]);
       ClusterFeat F{};
       if (FEAT) {
@@ -2363,12 +2374,9 @@ __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a, const uint32_t* l
         F.cnt = FX + 4 * NC;
         F.hf = (uint8_t*)(FX + 4 * NC + 8);
       }
-      done = cluster_read<64, FEAT>(a.hits + a.hoff[r], n, L, M, a.cands + (uint64_t)r * TOP_CANDIDATES,
-                                    a.ncand + r, nullptr, F);
-    }
-    if (!done && lane_id() == 0) {
-      const uint32_t slot = atomicAdd(defer_n, 1u);
-      defer[slot] = r;
+      const bool done = cluster_read<64, FEAT>(a.hits + a.hoff[r], n, L, M, a.cands + (uint64_t)r * TOP_CANDIDATES,
+                                               a.ncand + r, nullptr, F);
+      if (!done && lane == 0) defer[atomicAdd(defer_n, 1u)] = r;
     }
   }
 }
@@ -2438,8 +2446,8 @@ __global__ __launch_bounds__(256) void k_cluster_tier2(ClusterArgs a, const uint
 
 void launch_cluster(const ClusterArgs& a, hipStream_t s) {
   if (a.R.n == 0) return;
-  const uint32_t blocks = a.R.n < 65536 ? a.R.n : 65536;
-  const uint32_t mid_blocks = a.R.n < 8192 ? a.R.n : 8192;  // the count is on the device
+  const uint32_t blocks = (a.R.n + 63) / 64 < 65536 ? (a.R.n + 63) / 64 : 65536;  // 64 reads a block and round
+  const uint32_t mid_blocks = blocks < 8192 ? blocks : 8192;  // the count is on the device
   if (a.feat) {
     hipLaunchKernelGGL((k_cluster<true, CL_SMALL>), dim3(blocks), dim3(64), 0, s, a, nullptr, nullptr, a.mid, a.nmid);
     hipLaunchKernelGGL((k_cluster<true, CL_NCAP>), dim3(mid_blocks), dim3(64), 0, s, a, a.mid, a.nmid, a.tier2,
