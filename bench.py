@@ -71,10 +71,10 @@ def _profile_rows(kind, workload, batch, ncols):
 
 
 def measured_traffic(kernel, batch, workload):
-    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, copy-calibrated) from the
-    committed rocprofv3 counter summary of this workload at the default batch (the last
-    dispatch of that name: the one of the timed step); else None.  `kernel` may be a tuple:
-    the sum over those kernels."""
+    """HBM bytes per step of `kernel` (FETCH_SIZE + WRITE_SIZE, copy-calibrated) from the
+    committed rocprofv3 counter summary of this workload at the default batch (summed over
+    the kernel's dispatches in the timed step); else None.  `kernel` may be a tuple: the
+    sum over those kernels."""
     rows = _profile_rows("traffic", workload, batch, 7)
     names = kernel if isinstance(kernel, tuple) else (kernel,)
     if not all(k in rows for k in names):
@@ -83,8 +83,8 @@ def measured_traffic(kernel, batch, workload):
 
 
 def measured_valu(kernel, batch, workload):
-    """VALU issue counters of `kernel` from the committed rocprofv3 summary: fraction of the VALU
-    issue capacity used, int32 share of it, active lanes."""
+    """VALU issue counters of `kernel` from the committed rocprofv3 summary (its dispatches in
+    the timed step): fraction of the VALU issue capacity used, int32 share of it, active lanes."""
     f = _profile_rows("valu", workload, batch, 10).get(kernel)
     if not f:
         return None
@@ -196,8 +196,9 @@ def stage_rooflines(agg, B, workload):
                "per_launch": {"bytes": cl_bytes, "ms": ms_cl,
                               "unit": "8 B per hit key read + 321 B of candidates per read written"}}
     ach = cells / (ms_dp * 1e-3) * DP_OPS_PER_CELL / 1e12 if ms_dp > 0 else 0.0
-    dp = {"kernel": "k_dp_quad", "bound": "valu-int32", "unit": "Tops/s", "peak": round(PEAK_INT32_TOPS, 2),
-          "achieved": round(ach, 3), "traffic": measured_traffic("pbw::k_dp_quad", B, workload),
+    dp = {"kernel": "DP (k_dp_quad rows, k_dp_walk + k_dp_lk beside them)", "bound": "valu-int32", "unit": "Tops/s",
+          "peak": round(PEAK_INT32_TOPS, 2), "achieved": round(ach, 3),
+          "traffic": measured_traffic(("pbw::k_dp_quad", "pbw::k_dp_walk", "pbw::k_dp_lk"), B, workload),
           "per_launch": {"cells": cells, "ms": ms_dp, "ops_per_cell": DP_OPS_PER_CELL},
           "counters": measured_valu("pbw::k_dp_quad", B, workload)}
     for r in (seed, cluster, dp):
