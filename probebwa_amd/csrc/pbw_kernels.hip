@@ -4216,27 +4216,37 @@ __global__ __launch_bounds__(256) void k_dp_walk(DpArgs a) {
     uint32_t i = m, j = ((uint32_t)best) >> 1, st = ((uint32_t)best) & 1u;
     uint32_t cur_op = 0xFFu, cur_len = 0;
     int32_t cb = -1, cl = -1;
-    uint32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0, p0 = 0, p1 = 0, n2 = 0, l3 = 0;
+    uint32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0, p0 = 0, p1 = 0, n2 = 0, l3 = 0, have = 0;
     while (i > 0) {
       int32_t d = (int32_t)j - (int32_t)i - o;
       d = d < 0 ? 0 : (d > 63 ? 63 : d);  // cells on the path always lie in the band
       const int32_t lq = d >> 2, rb = (int32_t)(i - 1) >> 3;
-      if (rb != cb || lq != cl) {
-        // this lane's four planes, the two M planes of the block above, plane 2
-        // of the lane to the right and plane 3 of the lane to the left (own
-        // lane at the edges)
+      // select: reference column 0 forces a single I step with the state kept
+      const uint32_t sel = (j == 0) ? 1u : st;
+      if (rb != cb || lq != cl) {  // this lane's four planes
         const uint4 tv = *reinterpret_cast<const uint4*>(TB + rb * 64 + lq * 4);
-        const uint2 pv = *reinterpret_cast<const uint2*>(TB + (rb > 0 ? rb - 1 : 0) * 64 + lq * 4);
         v0 = tv.x;
         v1 = tv.y;
         v2 = tv.z;
         v3 = tv.w;
-        p0 = pv.x;
-        p1 = pv.y;
-        n2 = lq < 15 ? TB[rb * 64 + lq * 4 + 6] : tv.z;
-        l3 = lq > 0 ? TB[rb * 64 + lq * 4 - 1] : tv.w;
         cb = rb;
         cl = lq;
+        have = 0;
+      }
+      // the words only one state reads, loaded once per (block, lane) when that
+      // state walks: the two M planes of the block above (M), plane 2 of the lane
+      // to the right (I), plane 3 of the lane to the left (D); own lane at the edges
+      if (sel == 0u && !(have & 1u)) {
+        const uint2 pv = *reinterpret_cast<const uint2*>(TB + (rb > 0 ? rb - 1 : 0) * 64 + lq * 4);
+        p0 = pv.x;
+        p1 = pv.y;
+        have |= 1u;
+      } else if (sel == 1u && !(have & 2u)) {
+        n2 = lq < 15 ? TB[rb * 64 + lq * 4 + 6] : v2;
+        have |= 2u;
+      } else if (sel == 2u && !(have & 4u)) {
+        l3 = lq > 0 ? TB[rb * 64 + lq * 4 - 1] : v3;
+        have |= 4u;
       }
       const uint32_t k = (uint32_t)d & 3u, r = (i - 1) & 7u;
       const uint32_t pos = 31u - (4u * r + k);
@@ -4263,8 +4273,6 @@ __global__ __launch_bounds__(256) void k_dp_walk(DpArgs a) {
       const uint32_t fi = zi ? (uint32_t)__builtin_ctz(zi) / 3u : 32u;
       const uint32_t ist = min(fi + 1u, tav + 1u);
       const uint32_t ipar = (ist == fi + 1u) ? 0u : 1u;
-      // select: reference column 0 forces a single I step with the state kept
-      const uint32_t sel = (j == 0) ? 1u : st;
       const uint32_t isM = 0u - (uint32_t)(sel == 0u), isD = 0u - (uint32_t)(sel == 2u);
       const uint32_t isI = ~(isM | isD);
       const uint32_t jz = 0u - (uint32_t)(j == 0);
