@@ -3834,8 +3834,7 @@ __device__ uint64_t quad_cells(int64_t m, int64_t h, int64_t b, int64_t n) {
   return (uint64_t)(hi - lo + m);
 }
 
-// At most 168 VGPRs: three waves per SIMD (the latency-bound phases of one wave
-// overlap the row work of the others).
+// Three waves per SIMD (held to 128 VGPRs for four it spills and runs slower).
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_dp_quad(DpArgs a) {
   extern __shared__ __align__(16) uint8_t dyn[];
   const int lane = lane_id();
