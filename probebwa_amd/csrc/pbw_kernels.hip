@@ -3496,6 +3496,67 @@ __device__ void dp_traceback(uint32_t m, uint32_t best_j, uint32_t state, TbFn t
   ref_off = j;
 }
 
+// dp_traceback's walk taken a run at a time by the whole wave: in state M lane l
+// looks at the cell l steps up the diagonal, in I l rows up, in D l columns left
+// (tb_at(i, j): the cell's bits, as dp_traceback's tb), and a ballot finds where
+// the state's run ends -- the first cell whose bits leave the state, or the edge
+// of the matrix -- so a run of up to 64 steps costs one round.  Every lane ends
+// with the same runs and offset as dp_traceback.
+template <typename TbAt>
+__device__ void dp_traceback_wave(uint32_t m, uint32_t best_j, uint32_t state, TbAt tb_at, DpStage& st,
+                                  uint32_t& nruns, uint32_t& ref_off) {
+  const uint32_t lane = (uint32_t)lane_id();
+  uint32_t i = m, j = best_j, nr = 0, cur_op = 0xFF, cur_len = 0;
+  auto push = [&](uint32_t op, uint32_t len) {
+    if (op == cur_op) {
+      cur_len += len;
+    } else {
+      if (cur_len) {
+        if (lane == 0) st.runs[nr] = (cur_len << 4) | cur_op;
+        nr++;
+      }
+      cur_op = op;
+      cur_len = len;
+    }
+  };
+  while (i > 0) {
+    if (j == 0) {  // column 0: single I steps to the first row
+      push(1, i);
+      i = 0;
+      break;
+    }
+    const uint32_t room = state == 0 ? min(i, j) : state == 1 ? i : j;  // steps before an edge
+    bool stop = true;
+    uint32_t bits = 0;
+    if (lane < room) {
+      const uint32_t ii = state == 2 ? i : i - lane, jj = state == 1 ? j : j - lane;
+      bits = tb_at(ii, jj);
+      stop = state == 0 ? (bits & 3u) != 0 : state == 1 ? (bits & 4u) == 0 : (bits & 8u) == 0;
+    }
+    const uint64_t sm = __ballot(stop);  // lanes at or past the edge stop
+    const uint32_t f = sm ? (uint32_t)__builtin_ctzll(sm) : 64u;
+    const uint32_t fb = (uint32_t)__shfl((int)bits, (int)(f & 63u), 64);
+    uint32_t len, nstate;
+    if (f < room && f < 64u) {  // the run ends at cell f, whose bits give the next state
+      len = f + 1;
+      nstate = state == 0 ? (fb & 3u) : 0u;
+    } else {  // 64 steps, or the run reaches the edge, in its own state
+      len = f;
+      nstate = state;
+    }
+    push(state == 0 ? 0u : state == 1 ? 1u : 2u, len);
+    if (state != 2) i -= len;
+    if (state != 1) j -= len;
+    state = nstate;
+  }
+  if (cur_len) {
+    if (lane == 0) st.runs[nr] = (cur_len << 4) | cur_op;
+    nr++;
+  }
+  nruns = nr;
+  ref_off = j;
+}
+
 // Lane 0: alignment_log_likelihood_bases (bayesian.rs:142-179) over the
 // reverse-ordered runs; ref_window = ref_seg[off .. off+span).
 __device__ double dp_log_lk(const DevLuts& T, const DpStage& st, uint32_t m, uint32_t qlen, uint32_t nruns,
@@ -5560,25 +5621,16 @@ __global__ __launch_bounds__(64 * T) __attribute__((amdgpu_waves_per_eu(4))) voi
       for (int tt = 0; tt < T; tt++) key = key > ekey[tt] ? key : ekey[tt];
       const int32_t best_score = (int32_t)(key >> 32);
       const uint32_t best_j = ((uint32_t)key) >> 1, best_state = ((uint32_t)key) & 1u;
-      // Every lane walks the same path (the same words, read from LDS); rows
-      // come in blocks of WIDE_TB_ROWS, loaded by the whole wave when the walk
-      // leaves the block it has.
+      // The wave walks a run per round (dp_traceback_wave), each lane reading
+      // one cell's bits from the planes in global scratch.
       uint32_t nruns = 0, off = 0, span = 0;
-      uint32_t lo = m + 1;  // first row of the staged block (none yet)
-      const uint32_t* TB32 = (const uint32_t*)TB;
       auto tb = [&](uint32_t ii, uint32_t jj) -> uint32_t {
-        if (ii < lo) {
-          lo = ii + 1 > WIDE_TB_ROWS ? ii + 1 - WIDE_TB_ROWS : 1u;
-          const uint32_t nw = (ii - lo + 1) * T * 32, w0 = lo * T * 32;
-          wave_sync();
-          for (uint32_t x = (uint32_t)lane; x < nw; x += 64) tbs[x] = TB32[w0 + x];
-          wave_sync();
-        }
         const uint32_t rel = jj - 1, tt = rel >> 8, ln = (rel >> 2) & 63u, k = rel & 3u;
-        const uint16_t hw = ((const uint16_t*)tbs)[((ii - lo) * T + tt) * 64 + ln];
+        const uint16_t hw = TB[((uint64_t)ii * T + tt) * 64 + ln];
         return ((uint32_t)hw >> (4 * k)) & 15u;
       };
-      dp_traceback(m, best_j, best_state, tb, st, nruns, off);
+      dp_traceback_wave(m, best_j, best_state, tb, st, nruns, off);
+      wave_sync();  // the runs lane 0 stored
       for (uint32_t t2 = 0; t2 < nruns; t2++) {
         const uint32_t op = st.runs[t2] & 15u;
         if (op == 0 || op == 2) span += st.runs[t2] >> 4;
