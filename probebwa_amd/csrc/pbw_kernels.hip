@@ -3586,6 +3586,54 @@ __device__ double dp_log_lk(const DevLuts& T, const DpStage& st, uint32_t m, uin
   return lk;
 }
 
+// dp_log_lk by a whole wave: each round its lanes take 64 consecutive read
+// positions, find their run with a cursor that only moves forward and write the
+// position's term to terms[lane]; lane 0 adds them in read order (the same
+// doubles in the same order as dp_log_lk; every lane returns the sum).
+__device__ double dp_log_lk_wave(const DevLuts& T, const DpStage& st, uint32_t m, uint32_t qlen, uint32_t nruns,
+                                 uint32_t off, uint32_t span, double* terms) {
+  const uint32_t lane = (uint32_t)lane_id();
+  int u = (int)nruns - 1;   // runs in read order: st.runs[nruns - 1] first
+  uint32_t qs = 0, rs = 0;  // read and reference bases before run u
+  double lk = 0.0;
+  for (uint32_t base = 0; base < m; base += 64) {
+    const uint32_t qi = base + lane;
+    double tv = 0.0;
+    if (qi < m) {
+      uint32_t op = 3u;
+      while (u >= 0) {
+        const uint32_t w = st.runs[u], len = w >> 4;
+        op = w & 15u;
+        const uint32_t qc = op <= 1u ? len : 0u;
+        if (qi < qs + qc) break;
+        qs += qc;
+        rs += (op == 0u || op == 2u) ? len : 0u;
+        u--;
+        op = 3u;
+      }
+      if (op == 0u) {
+        const uint32_t ri = rs + (qi - qs);
+        if (ri < span) {
+          const uint32_t q = qi < qlen ? st.qq[qi] : 30u;
+          tv = (st.q[qi] == st.ref[off + ri]) ? T.lk_match[q] : T.lk_mis[q];
+        }
+      } else if (op == 1u) {
+        tv = T.ln_quarter;
+      }
+    }
+    terms[lane] = tv;
+    wave_sync();
+    if (lane == 0) {
+      const uint32_t c = min(64u, m - base);
+      for (uint32_t v = 0; v < c; v++) lk += terms[v];
+    }
+    wave_sync();
+  }
+  const long long b = __double_as_longlong(lk);
+  const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)b, 0, 64), hi = (uint32_t)__shfl((int)(uint32_t)(b >> 32), 0, 64);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 // Copies the CIGAR (forward order) and fills DpOut; lane-parallel copy.
 // Hands job `jid` to the next kernel of the chain (one lane calls it).
 __device__ __forceinline__ void dp_flag(const DpArgs& a, uint32_t jid) {
@@ -5635,7 +5683,7 @@ __global__ __launch_bounds__(64 * T) __attribute__((amdgpu_waves_per_eu(4))) voi
         const uint32_t op = st.runs[t2] & 15u;
         if (op == 0 || op == 2) span += st.runs[t2] >> 4;
       }
-      const double lk = dp_log_lk(a.T, st, m, qlen, nruns, off, span);
+      const double lk = dp_log_lk_wave(a.T, st, m, qlen, nruns, off, span, reinterpret_cast<double*>(tbs));
       wave_sync();
       const uint64_t cells = (uint64_t)m * n;
       dp_emit(a, jid, st, nruns, off, span, best_score, lk, cells);
