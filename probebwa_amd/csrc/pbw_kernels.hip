@@ -1305,10 +1305,21 @@ __device__ __forceinline__ void group_table_count(HitAt hits, uint32_t n, const 
           if (i < __atomic_load_n(&M.gt_emin[sl0], __ATOMIC_RELAXED)) atomicMin(&M.gt_emin[sl0], i);
         }
       } else {
-        atomicAdd(&M.gt_cnt[sl], 1u);
+        // runs of one group over consecutive lanes (a bucket's entries come in
+        // group order; the active lanes are a prefix): a run's first lane adds
+        // the run's count and, holding its lowest emission index, the index
+        const uint32_t gp = (uint32_t)__shfl_up((int)g, 1, 64);
+        const uint32_t ln = (uint32_t)lane_id();
+        const bool head = ln == 0 || gp != g;
+        const uint64_t ends = __ballot(head) | ~act;  // run heads and the inactive tail
+        if (head) {
+          const uint64_t after = ends & ~((2ull << ln) - 1ull);  // lanes above this one (none for lane 63)
+          const uint32_t nxt = after ? (uint32_t)__builtin_ctzll(after) : 64u;
+          atomicAdd(&M.gt_cnt[sl], nxt - ln);
+          if (i < __atomic_load_n(&M.gt_emin[sl], __ATOMIC_RELAXED)) atomicMin(&M.gt_emin[sl], i);
+        }
         if (d < __atomic_load_n(&M.gt_lo[sl], __ATOMIC_RELAXED)) atomicMin(&M.gt_lo[sl], d);
         if (d > __atomic_load_n(&M.gt_hi[sl], __ATOMIC_RELAXED)) atomicMax(&M.gt_hi[sl], d);
-        if (i < __atomic_load_n(&M.gt_emin[sl], __ATOMIC_RELAXED)) atomicMin(&M.gt_emin[sl], i);
       }
     }
   }
