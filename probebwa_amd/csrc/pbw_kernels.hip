@@ -2769,7 +2769,8 @@ __device__ __forceinline__ void cg_start_bits(Get get, uint32_t a0, uint32_t cnt
 // in LDS, and then each bucket of two to CG_BSMALL keys is put in order by one
 // lane (a sorting network in registers, bucket_net), each larger one of up to 512
 // by a wave (bitonic in registers).  One segment's tile of a larger segment is
-// sorted the same way (one_cnt: its key count, q1 = q0 + 1).  With bmg set, the
+// sorted the same way (one_cnt: its key count, q1 = q0 + 1), its output to `out`
+// when that is set (without bmg).  With bmg set, the
 // run's cluster-start bits (read positions from a0 on) are written from LDS.  Returns false, with
 // nothing written, when a bucket holds more than 512 keys (the caller then sorts
 // the run by radix).  The keys of one bucket share their rank, so within a bucket
@@ -2817,7 +2818,7 @@ struct CgBucketLds {
 __device__ __forceinline__ bool cg_bucket_sort(uint64_t* key, uint32_t cnt, uint32_t q0, uint32_t q1, const uint32_t* st,
                                                const uint32_t* rng_bits, CgBucketLds& B, uint32_t tid,
                                                uint32_t one_cnt = 0, uint64_t* bmg = nullptr, uint32_t a0 = 0,
-                                               uint64_t* bits = nullptr) {
+                                               uint64_t* bits = nullptr, uint64_t* out = nullptr) {
   constexpr uint32_t IPT = CG_BTILE / 256;
   // buckets per segment
   if (tid == 0) {
@@ -2936,10 +2937,11 @@ __device__ __forceinline__ bool cg_bucket_sort(uint64_t* key, uint32_t cnt, uint
   }
   __syncthreads();
   if (!bmg) {
+    uint64_t* dst = out ? out : key;
 #pragma unroll
     for (uint32_t j = 0; j < IPT; j++) {
       const uint32_t pos = j * 256 + tid;
-      if (pos < cnt) key[pos] = B.keys[pos];
+      if (pos < cnt) dst[pos] = B.keys[pos];
     }
     __syncthreads();
     return true;
@@ -3031,12 +3033,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
           cg_start_bits([&](uint32_t i) { return key[a0 + i]; }, a0, cnt, bmg, tid);
         }
       } else {  // one segment over a tile: its keys share the rank bits
+        // the tiles go to the buffer that leaves the last merge round's output in key
+        uint32_t rounds = 0;
+        for (uint32_t w = RSORT_TILE; w < cnt; w <<= 1) rounds++;
+        const bool to_tmp = rounds & 1u;
         for (uint32_t t0 = 0; t0 < cnt; t0 += RSORT_TILE) {
           const uint32_t tc = min(RSORT_TILE, cnt - t0);
-          if (!cg_bucket_sort(key + a0 + t0, tc, q, q + 1, st, rb, u.bs, tid, tc))
+          if (!cg_bucket_sort(key + a0 + t0, tc, q, q + 1, st, rb, u.bs, tid, tc, nullptr, 0, nullptr,
+                              to_tmp ? tmp + a0 + t0 : nullptr)) {
             cg_run_sort<16>(key + a0 + t0, tc, 32u, rs, tid);
+            if (to_tmp) {
+              for (uint32_t i = tid; i < tc; i += 256) tmp[a0 + t0 + i] = key[a0 + t0 + i];
+              __syncthreads();
+            }
+          }
         }
-        bool in_key = true;
+        bool in_key = !to_tmp;
         for (uint32_t w = RSORT_TILE; w < cnt; w <<= 1) {
           const uint64_t* sk = in_key ? key : tmp;
           uint64_t* dk = in_key ? tmp : key;
