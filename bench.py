@@ -248,8 +248,9 @@ def pe_leg(ctx, contigs, pairs):
 def e2e_leg(ctx, reads, n, batch):
     """End to end, FASTQ in -> SAM out (the map subcommand, src/main.rs:130-195): the first n
     reads written as a FASTQ file, then parsed, mapped and written as SAM by pbw_map_files in
-    batches of `batch` reads (parsing the next batch and formatting the last overlap the
-    mapping of the current one); one untimed call, then one timed."""
+    batches of `batch` reads (a quarter of the timed steps' batch: parsing the next batch and
+    formatting the last overlap the mapping of the current one, so more batches keep the three
+    stages busier); one untimed call, then one timed."""
     import shutil
     import tempfile
     import probebwa_amd as P
@@ -456,7 +457,7 @@ def main():
 
     e2e = None
     if not args.no_e2e:
-        e2e = e2e_leg(ctx, (seq, so, qual, qo), min(3 * B, n), B)
+        e2e = e2e_leg(ctx, (seq, so, qual, qo), min(3 * B, n), max(B // 4, 16384))
 
     cpu = cpu1 = concord = order = None
     if not args.no_cpu:
