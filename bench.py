@@ -92,6 +92,19 @@ def measured_valu(kernel, batch, workload):
             "valu_insts": int(f[1]), "int32_insts": int(f[2])}
 
 
+def wide_roofline(st):
+    """The paired-end mate-window DP (k_dp_wide: rescue windows around the mate's placement,
+    score rows, wave traceback and likelihood) against the int32 VALU peak: its cells x the
+    reference recurrence's ops per cell over the summed duration of its launches, timed with
+    HIP events on the stream it runs on."""
+    ms, cells = float(st["ms_k_wide"]), int(st["wide_cells"])
+    ach = cells / (ms * 1e-3) * DP_OPS_PER_CELL / 1e12 if ms > 0 else 0.0
+    return {"kernel": "k_dp_wide", "bound": "valu-int32", "unit": "Tops/s", "peak": round(PEAK_INT32_TOPS, 2),
+            "achieved": round(ach, 3), "frac": round(ach / PEAK_INT32_TOPS, 4), "traffic": None,
+            "per_launch": {"cells": cells, "launches": int(st["n_k_wide"]), "ms": round(ms, 3),
+                           "ops_per_cell": DP_OPS_PER_CELL}}
+
+
 def bench_pe(args):
     """Paired-end joint posterior on the probe genome: 1M FR pairs of 2x150 bp (10% with one
     mate inside a repeat copy), one pbw_map_pe call over all of them (500 serial warm-up
@@ -144,7 +157,7 @@ def bench_pe(args):
         "reads_per_s": round(2 * n / dt, 1),
         "dp_cells": int(st["dp_cells"]), "wide_jobs": int(st["wide_jobs"]), "wide_cells": int(st["wide_cells"]),
         "stage_ms": {k: round(st[k], 1) for k in ("ms_seed", "ms_cluster", "ms_dp", "ms_pair", "ms_total")},
-        "cpu_baseline": cpu, "sample_sam_concordance": concord,
+        "roofline": wide_roofline(st), "cpu_baseline": cpu, "sample_sam_concordance": concord,
     }
     print(json.dumps(out), flush=True)
 
@@ -240,7 +253,8 @@ def pe_leg(ctx, contigs, pairs):
     log("paired-end: %d pairs in %.2f s" % (pairs, dt))
     return {"pairs": pairs, "pairs_per_s": round(pairs / dt, 1), "reads_per_s": round(2 * pairs / dt, 1),
             "seconds": round(dt, 3), "proper_pair_frac": round(proper, 4), "wide_jobs": int(st["wide_jobs"]),
-            "wide_cells": int(st["wide_cells"]), "insert_model": [round(x, 3) for x in ctx.insert_model[:2]],
+            "wide_cells": int(st["wide_cells"]), "wide_roofline": wide_roofline(st),
+            "insert_model": [round(x, 3) for x in ctx.insert_model[:2]],
             "stage_ms": {k: round(st[k], 1) for k in ("ms_seed", "ms_cluster", "ms_dp", "ms_pair", "ms_total")},
             "data": "FR pairs of 2x150 (fragment N(350,50) clipped to [200,600]), seed 58"}
 

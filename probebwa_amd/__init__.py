@@ -197,7 +197,7 @@ class Stats(C.Structure):
                [(n, C.c_uint64) for n in ("n_k_seed", "n_k_dp", "pf_jobs", "pf_jobs_aligned", "pf_fallback_reads",
                                            "pf_audit_reads", "pf_audit_mismatches")] + \
                [("ms_pf", C.c_double), ("ms_k_cluster", C.c_double), ("n_k_cluster", C.c_uint64),
-                ("mapq_near_boundary", C.c_uint64)]
+                ("mapq_near_boundary", C.c_uint64), ("ms_k_wide", C.c_double), ("n_k_wide", C.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -172,6 +172,7 @@ struct pbw_ctx {
   hipEvent_t ev[8] = {};  // [0,1] seed, [2,3] DP, [4,5] wave-tier and [6,7] block-tier clustering
   hipStream_t stream2 = nullptr;  // the DP walks of one launch beside the next launch's rows
   hipEvent_t dev[4] = {};         // [0,1] rows of a launch done (by buffer), [2,3] walks done (by buffer)
+  hipEvent_t evw[2] = {};         // the paired-end mate-window DP
   std::string error;
   pbw_stats stats{};
   // genome + index
@@ -276,6 +277,7 @@ static void ctx_build(pbw_ctx* c) {
   for (auto& e : c->ev) HIP_CHECK(hipEventCreate(&e));
   HIP_CHECK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
   for (auto& e : c->dev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  for (auto& e : c->evw) HIP_CHECK(hipEventCreate(&e));
   hipStream_t s = c->stream;
   if (g->lengths.size() >= (1u << 22)) throw PbwError{PBW_ERR_LIMIT, "more than 2^22 contigs"};
   for (uint64_t L : g->lengths)
@@ -1475,7 +1477,9 @@ static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, const
   da.max_m = b.max_m;
   da.max_n = max_n;
   da.ctr = c->d_ctr_w.p;
+  HIP_CHECK(hipEventRecord(c->evw[0], s));
   run_dp_wide_dev(c, da, c->d_wlist.p, d_wn, nws, d_wn + 1, d_wn + 2);
+  HIP_CHECK(hipEventRecord(c->evw[1], s));
   // joint posterior + records
   pa.wout = c->d_wout.p;
   pa.wcigar = c->d_wcigar.p;
@@ -1519,6 +1523,12 @@ static void pe_pairs(pbw_ctx* c, const Batch& b, uint32_t q0, uint32_t q1, const
   uint8_t* h_has = c->h_pe + 16 + (size_t)P * 8;
   HIP_CHECK(hipMemcpyAsync(c->h_pe, c->d_pe_io.p, io_bytes, hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
+  {
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, c->evw[0], c->evw[1]));
+    c->stats.ms_k_wide += ms;
+    c->stats.n_k_wide += 1;
+  }
   if (h_cnt[2]) throw PbwError{PBW_ERR_LIMIT, "alignment window beyond the tiled kernel's shape"};
   c->stats.wide_jobs += h_cnt[0];
   obs_out.assign(h_obs, h_obs + P);
@@ -1999,6 +2009,8 @@ void pbw_ctx_destroy(pbw_ctx* c) {
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : c->dev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : c->evw)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   if (c->stream2) (void)hipStreamDestroy(c->stream2);

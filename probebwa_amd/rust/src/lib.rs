@@ -158,6 +158,8 @@ pub struct pbw_stats {
     pub ms_k_cluster: f64,
     pub n_k_cluster: u64,
     pub mapq_near_boundary: u64,
+    pub ms_k_wide: f64,
+    pub n_k_wide: u64,
 }
 
 #[repr(C)]
