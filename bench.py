@@ -15,7 +15,9 @@ Workloads (probebwa_amd/synth.py):
   unit at 1% divergence); the prefilter trains on 500,000 reads and is evaluated on
   2,000,000 held-out reads (L ~ U{80..150}, 1% substitutions, 0.05% indels,
   Q ~ U{20..40}).
-- probe_pe_2mbp: 1,000,000 FR pairs of 2 x 150 on the probe genome (pairs/s).
+- probe_pe_2mbp: 1,000,000 FR pairs of 2 x 150 on the probe genome (pairs/s), with
+  the k_dp_wide roofline.  The default genome run also maps a bounded 200,000 pairs of
+  this workload (the oracle on the first 8,000) into its `probe_pe_2mbp` field.
 
 A step maps one batch of `--batch` reads that are already resident in HBM
 (`pbw_reads_upload` once, then `pbw_map_se_device` per step; calls return after the
@@ -105,13 +107,14 @@ def wide_roofline(st):
                            "ops_per_cell": DP_OPS_PER_CELL}}
 
 
-def bench_pe(args):
-    """Paired-end joint posterior on the probe genome: 1M FR pairs of 2x150 bp (10% with one
+def probe_pe_run(args, n, cpu_pairs):
+    """Paired-end joint posterior on the probe genome: n FR pairs of 2x150 bp (10% with one
     mate inside a repeat copy), one pbw_map_pe call over all of them (500 serial warm-up
-    pairs, then 2000-pair batches on a frozen insert-size snapshot, as the reference)."""
+    pairs, then 2000-pair batches on a frozen insert-size snapshot, as the reference); the
+    oracle on the first `cpu_pairs` pairs (0: none) for the CPU baseline and the SAM
+    concordance.  Returns the result line's fields."""
     import probebwa_amd as P
     from probebwa_amd import synth
-    n = args.reads or 1_000_000
     t0 = time.time()
     contigs = synth.genome_probe_se_2mbp()
     spans = synth.probe_repeat_spans()
@@ -131,9 +134,9 @@ def bench_pe(args):
     log("mapped %d pairs in %.2f s" % (n, dt))
     cpu = None
     concord = None
-    if not args.no_cpu:
+    if cpu_pairs > 0:
         from oracle import pbw_oracle as O
-        S = min(n, 20000)
+        S = min(n, cpu_pairs)
         go = O.Genome()
         for name, s in contigs:
             go.add_contig(name, s.tobytes())
@@ -159,7 +162,13 @@ def bench_pe(args):
         "stage_ms": {k: round(st[k], 1) for k in ("ms_seed", "ms_cluster", "ms_dp", "ms_pair", "ms_total")},
         "roofline": wide_roofline(st), "cpu_baseline": cpu, "sample_sam_concordance": concord,
     }
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def bench_pe(args):
+    """The probe_pe_2mbp workload on its own: 1M pairs (or --reads), the oracle on 20,000."""
+    n = args.reads or 1_000_000
+    print(json.dumps(probe_pe_run(args, n, 0 if args.no_cpu else 20000)), flush=True)
 
 
 WORKLOADS = {
@@ -481,6 +490,11 @@ def main():
         del ctx, dreads
         cpu, cpu1, concord, order = cpu_leg(args, contigs, (seq, so, qual, qo), plain_recs[0], S, S1)
 
+    # the paired-end workload on the probe genome, bounded: 200,000 pairs, the oracle on 8,000
+    probe_pe = None
+    if args.workload == "genome_se_3gbp" and not args.no_pe:
+        probe_pe = probe_pe_run(args, 200_000, 0 if args.no_cpu else 8000)
+
     out = {
         "metric": "reads/s on 1 GPU, learned vs plain @≥99.9% SAM concordance; DP GCUPS",
         "value": round(value, 1),
@@ -506,6 +520,7 @@ def main():
         "learned": learned,
         "learned_vs_plain": learned["speedup_vs_plain"] if learned else None,
         "paired_end": pe,
+        "probe_pe_2mbp": probe_pe,
         "end_to_end_fastq_to_sam": e2e,
         "cpu_baseline": cpu,
         "cpu_baseline_1_thread": cpu1,
