@@ -94,15 +94,36 @@ def measured_valu(kernel, batch, workload):
             "valu_insts": int(f[1]), "int32_insts": int(f[2])}
 
 
-def wide_roofline(st):
+PE_PROFILE_PAIRS = 200_000  # the pair count profiles/pmc_traffic_probe_pe_2mbp.txt was taken at
+
+
+def wide_traffic(pairs):
+    """HBM bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of every k_dp_wide dispatch of the
+    timed map_pe call, from the committed counter summary of probe_pe_2mbp at its profiled
+    pair count; else None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_probe_pe_2mbp.txt")
+    if pairs != PE_PROFILE_PAIRS or not os.path.exists(path):
+        return None
+    tot = 0
+    for line in open(path):
+        f = line.strip().split(",")
+        if len(f) == 8 and f[0].startswith("pbw::k_dp_wide"):
+            tot += int(f[6])
+    return tot or None
+
+
+def wide_roofline(st, pairs=None):
     """The paired-end mate-window DP (k_dp_wide: rescue windows around the mate's placement,
     score rows, wave traceback and likelihood) against the int32 VALU peak: its cells x the
     reference recurrence's ops per cell over the summed duration of its launches, timed with
-    HIP events on the stream it runs on."""
+    HIP events on the stream it runs on.  `traffic`: the measured HBM bytes of the call's
+    k_dp_wide dispatches on the probe genome at the profiled pair count (mostly its traceback
+    bits, 4 per cell)."""
     ms, cells = float(st["ms_k_wide"]), int(st["wide_cells"])
     ach = cells / (ms * 1e-3) * DP_OPS_PER_CELL / 1e12 if ms > 0 else 0.0
     return {"kernel": "k_dp_wide", "bound": "valu-int32", "unit": "Tops/s", "peak": round(PEAK_INT32_TOPS, 2),
-            "achieved": round(ach, 3), "frac": round(ach / PEAK_INT32_TOPS, 4), "traffic": None,
+            "achieved": round(ach, 3), "frac": round(ach / PEAK_INT32_TOPS, 4),
+            "traffic": wide_traffic(pairs) if pairs is not None else None,
             "per_launch": {"cells": cells, "launches": int(st["n_k_wide"]), "ms": round(ms, 3),
                            "ops_per_cell": DP_OPS_PER_CELL}}
 
@@ -160,7 +181,7 @@ def probe_pe_run(args, n, cpu_pairs):
         "reads_per_s": round(2 * n / dt, 1),
         "dp_cells": int(st["dp_cells"]), "wide_jobs": int(st["wide_jobs"]), "wide_cells": int(st["wide_cells"]),
         "stage_ms": {k: round(st[k], 1) for k in ("ms_seed", "ms_cluster", "ms_dp", "ms_pair", "ms_total")},
-        "roofline": wide_roofline(st), "cpu_baseline": cpu, "sample_sam_concordance": concord,
+        "roofline": wide_roofline(st, n), "cpu_baseline": cpu, "sample_sam_concordance": concord,
     }
     return out
 
