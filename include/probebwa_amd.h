@@ -135,8 +135,9 @@ typedef struct {
   uint64_t n_k_cluster;         /* clustering stages timed */
   uint64_t mapq_near_boundary;  /* MAPQ finalizations whose integer MAPQ rests on a device exp/log/log10
                                    value within 1e-9 of a rounding point (one ulp could move it) */
-  double ms_k_wide;             /* HIP-event time of the paired-end mate-window DP (align_at_window) */
-  uint64_t n_k_wide;            /* mate-window DP launches timed */
+  double ms_k_wide;             /* HIP-event time of the paired-end mate-window DP (align_at_window):
+                                   the wide kernel and the tiled kernel for windows it hands on */
+  uint64_t n_k_wide;            /* pair batches whose mate-window DP was timed */
 } pbw_stats;
 
 /* ---- genome: GenomeIndexBuilder (src/index/genome.rs:101-147) ---- */
